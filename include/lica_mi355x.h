@@ -1,0 +1,141 @@
+/*
+ * lica_mi355x.h -- C ABI of the MI355X (gfx950) implementation of the
+ * per-step hot path of the 1-D film-boiling solver lica_boiling1d.
+ *
+ * Plain C types only: int64_t/double scalars and pointers.  Arrays passed
+ * across the boundary are the reference's column arrays for the single
+ * (I=1,J=1) column, index 0..N3 (N3 = n+1), i.e. Fortran X(1,1,0:N3).
+ *
+ * Reference interfaces replaced (file:line in jaehee016/lica_boiling1d):
+ *   lica_parse_deck      INPUT_SIMULATION 1d.f:323-492 + grid.in/grid.f:10-322
+ *   lica_init            MAIN init 1d.f:70-190 (ALLOCINIT, GEOM, MAKEFLD,
+ *                        LVSINIT two_phase.f:116, GRID_COUPLING(1) 1d.f:143)
+ *   lica_step            MAIN time loop 1d.f:196-295
+ *   lica_determine_dt    DETERMINE_DT(CFLM,U,V,W,PSI_CN) 1d.f:683 (caller 1d.f:201)
+ *   lica_rhsnlhs         RHSNLHS(ITRACKING,U,V,W,P,PSI_XN..PSI_CN,RK3XO..ANPSO,
+ *                        QVOL_ORI,VOL1) 1d.f:1444 (caller 1d.f:227)
+ *   lica_convrge1        CONVRGE1(DVMAX,U,V,W) 1d.f:3306 (caller 1d.f:234)
+ *   lica_transport       TRANSPORT(U,V,W,PSI_CN,VOL_TOT_ORI) two_phase.f:701
+ *                        (caller 1d.f:242)
+ *   lica_grid_coupling   GRID_COUPLING(INI_PSI,PSI_XN,PSI_YN,PSI_ZN,PSI_CN)
+ *                        two_phase.f:4863 (callers 1d.f:143,244)
+ *   lica_poisson         POISSON(P,DIVGSUM,DENF_XI,DENF_YI,DENF_ZI,DENF_Y,DENF_C)
+ *                        1d.f:3492 (caller 1d.f:3428), batched
+ *   lica_batch_*         batched driver of the same loop for many cases
+ * Errors are returned as status codes (the reference STOPs).
+ */
+#ifndef LICA_MI355X_H
+#define LICA_MI355X_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* one boiling case: the deck values of lica.in/grid.in it depends on */
+typedef struct {
+    int64_t n;        /* NS cells in z (N3M)                         */
+    int64_t r;        /* level-set refinement factor (N3FM/N3M)      */
+    double zl;        /* domain height                              */
+    double re, visr, denr;
+    double surf_j, fr;
+    double prm, scr, tcr;
+    double tsat, ja;
+    double dt0, cflmax;
+    int64_t idtopt;
+    double z0;        /* initial interface height (ASSIGN_LVS)      */
+    double tslope;    /* initial vapour temperature slope (MAKEFLD) */
+} lica_case_desc;
+
+typedef struct lica_batch lica_batch;
+
+/* status codes */
+enum {
+    LICA_OK = 0,
+    LICA_BAND_OVERFLOW = 1,        /* two_phase.f:361,1989  */
+    LICA_PSIF_BUG = 2,             /* two_phase.f:2905      */
+    LICA_GRID_TOO_LARGE = 3,       /* 1d.f:4761,4775        */
+    LICA_EXTRAP_NOT_CONVERGED = 4, /* chphase.f:676-678     */
+    LICA_BAND_CAPACITY = 5,
+    LICA_NONFINITE = 6,
+    LICA_WINDOW = 7,               /* band outside the device window */
+    LICA_BAD_ARG = 20,
+    LICA_UNSUPPORTED_DECK = 21,
+    LICA_IO = 22,
+    LICA_HIP_ERROR = 30,
+    LICA_NO_DEVICE = 31,
+    LICA_NOT_CONVERGED = 32        /* Poisson iteration hit its cycle cap */
+};
+
+/* Poisson solver modes (lica_batch_set_poisson) */
+enum {
+    LICA_POISSON_THOMAS = 0,    /* serial Thomas per case, bit-identical order */
+    LICA_POISSON_DIRECT = 1,    /* parallel partitioned tridiagonal (FP64)      */
+    LICA_POISSON_MG64 = 2,      /* FP64 V-cycles from zero to tolerance          */
+    LICA_POISSON_MIXED = 3,     /* FP32 V-cycles inside FP64 defect correction   */
+    LICA_POISSON_LEARNED = 4    /* MIXED with the conv-net initial guess         */
+};
+
+/* field ids for lica_batch_get_field: NS fields have n+2 entries (0..N3),
+ * refined fields r*n entries (1..N3FM)                                  */
+enum {
+    LICA_F_W = 0, LICA_F_P = 1, LICA_F_T = 2, LICA_F_PSI_CN = 3,
+    LICA_F_PSI_ZN = 4, LICA_F_PSIF = 5, LICA_F_PHI = 6, LICA_F_SFGAS = 7,
+    LICA_F_WO = 8, LICA_F_BAND = 9, LICA_F_ALPHI = 10, LICA_F_MFLVS = 11
+};
+
+/* scalar ids for lica_batch_get_scalars (LICA_NSCAL doubles) */
+enum {
+    LICA_S_TIME = 0, LICA_S_DT = 1, LICA_S_STEP = 2, LICA_S_PLACE = 3,
+    LICA_S_DVMAX = 4, LICA_S_CFLM = 5, LICA_S_VOL_ORI = 6, LICA_S_VOL_IPHS = 7,
+    LICA_S_QWALL = 8, LICA_S_STATUS = 9, LICA_S_SUBCYCLE_WARN = 10,
+    LICA_S_MASS_WARN = 11, LICA_S_EXTRAP_HALVINGS = 12, LICA_S_EXTRAP_ITERS = 13,
+    LICA_S_REINIT_CALLS = 14, LICA_S_BANDGEN_CALLS = 15, LICA_S_MASSCOR_ITERS = 16,
+    LICA_S_POISSON_CYCLES = 17, LICA_S_POISSON_SOLVES = 18, LICA_S_POISSON_RESID = 19,
+    LICA_NSCAL = 24
+};
+
+/* deck: reads lica.in (list-directed, 1d.f:347-386) and grid.in
+ * (grid.f:29-53); only the 1-D film-boiling configuration is accepted  */
+int lica_parse_deck(const char *lica_in_path, const char *grid_in_path, lica_case_desc *out);
+
+/* ---- batched API ---------------------------------------------------- */
+int lica_batch_create(int64_t ninst, const lica_case_desc *cases, lica_batch **out);
+int lica_batch_init(lica_batch *b);                       /* 1d.f:70-190 */
+int lica_batch_run(lica_batch *b, int64_t nsteps);         /* 1d.f:196-295 x nsteps */
+int lica_batch_set_poisson(lica_batch *b, int64_t mode, double rtol, int64_t max_cycles);
+int lica_batch_set_block(lica_batch *b, int64_t threads);  /* 64..1024, power of 2 */
+int lica_batch_use_graph(lica_batch *b, int64_t on);
+/* per-kernel GPU time (events between launches) for the next runs; no graph */
+int lica_batch_set_timing(lica_batch *b, int64_t on);
+int lica_batch_sync(lica_batch *b);
+int lica_batch_get_field(lica_batch *b, int64_t field, int64_t inst, double *out, int64_t cap, int64_t *count);
+int lica_batch_get_scalars(lica_batch *b, int64_t inst, double *out);
+int lica_batch_status(lica_batch *b, int64_t *first_bad_inst);
+int64_t lica_batch_size(lica_batch *b);
+/* GPU time of the last lica_batch_run, per kernel class (ms); returns count */
+int lica_batch_last_timings(lica_batch *b, double *ms_out, int64_t cap);
+void lica_batch_destroy(lica_batch *b);
+
+/* ---- single-case API mirroring the reference call sequence ----------- */
+int lica_init(lica_batch **ctx, const char *lica_in_path, const char *grid_in_path);
+int lica_step(lica_batch *ctx, int64_t nsteps);
+int lica_determine_dt(lica_batch *ctx, double *cflm, double *dt);
+int lica_rhsnlhs(lica_batch *ctx, int64_t msub, double *w, double *p, double *psi_zn, double *psi_cn);
+int lica_convrge1(lica_batch *ctx, double *dvmax);
+int lica_transport(lica_batch *ctx);
+int lica_grid_coupling(lica_batch *ctx, int64_t ini_psi, double *psi_zn, double *psi_cn);
+int lica_end_step(lica_batch *ctx);   /* TIME=TIME+DT (1d.f:260) */
+
+/* batched POISSON system of 1d.f:3535-3564: rows k=1..n of each of nsys
+ * systems (a(1) is ignored as in the reference), contiguous per system.  */
+int lica_poisson(int64_t nsys, int64_t n, const double *a, const double *b, const double *c,
+                 const double *g, double *p, int64_t mode, double rtol, int64_t max_cycles,
+                 int64_t *cycles_out);
+
+const char *lica_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,950 @@
+// Host side of the C ABI (include/lica_mi355x.h): deck parsing, per-case
+// geometry, the device layout of a batch, the step loop and the getters.
+//
+// Geometry follows READGRID/GEOMETRIC/GEOMETRIC_LVS/GEOM_COUPLING/LHSINIT
+// (1d.f:4746-4805, 4838-4942, 5134-5369, 1963-1978) for the configuration
+// lica.in/grid.in describe: one NS cell in x and y (periodic), N cells in z,
+// a refined level-set grid r x r x rN, all grids uniform.  grid.dat is the
+// ES23.15 print of grid.f's UNIFORM nodes (grid.f:240-242, 326-346); the
+// round trip through that text format is part of the reference's geometry
+// and is reproduced here.
+#include "../../include/lica_mi355x.h"
+#include "lica_types.h"
+#include "lica_launch.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#ifdef LICA_EMUL
+int lica_emul_nthreads = 256;
+typedef void *hipStream_t;
+#else
+#include <hip/hip_runtime.h>
+#endif
+
+namespace {
+
+// ---------------------------------------------------------------------
+// device memory (the CPU logic-check build keeps "device" buffers in host
+// memory; the product build is HIP only)
+#ifdef LICA_EMUL
+int dmalloc(void **p, size_t n) { *p = calloc(1, n ? n : 1); return *p ? 0 : LICA_HIP_ERROR; }
+void dfree(void *p) { free(p); }
+int h2d(void *d, const void *h, size_t n, hipStream_t) { memcpy(d, h, n); return 0; }
+int d2h(void *h, const void *d, size_t n, hipStream_t) { memcpy(h, d, n); return 0; }
+int dsync(hipStream_t) { return 0; }
+int have_device() { return 1; }
+#else
+int dmalloc(void **p, size_t n)
+{
+    if (hipMalloc(p, n ? n : 1) != hipSuccess) { *p = nullptr; return LICA_HIP_ERROR; }
+    if (hipMemset(*p, 0, n ? n : 1) != hipSuccess) return LICA_HIP_ERROR;
+    return 0;
+}
+void dfree(void *p) { if (p) (void)hipFree(p); }
+int h2d(void *d, const void *h, size_t n, hipStream_t s)
+{
+    if (hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, s) != hipSuccess) return LICA_HIP_ERROR;
+    return hipStreamSynchronize(s) == hipSuccess ? 0 : LICA_HIP_ERROR;
+}
+int d2h(void *h, const void *d, size_t n, hipStream_t s)
+{
+    if (hipMemcpyAsync(h, d, n, hipMemcpyDeviceToHost, s) != hipSuccess) return LICA_HIP_ERROR;
+    return hipStreamSynchronize(s) == hipSuccess ? 0 : LICA_HIP_ERROR;
+}
+int dsync(hipStream_t s) { return hipStreamSynchronize(s) == hipSuccess ? 0 : LICA_HIP_ERROR; }
+int have_device()
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return 0;
+    return 1;
+}
+#endif
+
+// grid.f UNIFORM in REAL*8 printed ES23.15 and read back (grid.f:242)
+double es23_15(double v)
+{
+    char buf[64];
+    snprintf(buf, sizeof buf, "%.15E", v);
+    return strtod(buf, nullptr);
+}
+
+void uniform_nodes(int64_t ncell, double len, std::vector<double> &z)
+{
+    z.resize((size_t)ncell + 1);
+    double dc = (len - 0.0) / (double)((ncell + 1) - 1);
+    for (int64_t k = 1; k <= ncell + 1; k++) z[(size_t)k - 1] = es23_15(0.0 + (double)(k - 1) * dc);
+}
+
+int64_t round_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
+
+// per-case host image of the device blocks
+struct CaseImage {
+    InstParam p;
+    std::vector<double> ns, rf;
+    std::vector<int> nsi, rfi;
+    double &A(int f, int k) { return ns[(size_t)f * p.ln + k + 2]; }
+    double &R(int f, int k) { return rf[(size_t)f * p.lf + k - p.m3l]; }
+    int &AI(int f, int k) { return nsi[(size_t)f * p.lni + k + 2]; }
+    int &RI(int f, int k) { return rfi[(size_t)f * p.lfi + k]; }
+};
+
+// GEOM_COUPLING's search of the NS centre at or below zz (1d.f:5156-5177)
+int coump_vel_search(CaseImage &c, double zz)
+{
+    const int n = c.p.n;
+    for (int kk = 1; kk <= n; kk++) {
+        if (fabs(zz - c.A(NS_ZP, kk)) <= 1.0e-10) return kk;
+        else if (c.A(NS_ZP, 1) > zz) return 1;
+        else if (c.A(NS_ZP, kk) > zz) return kk - 1;
+        else if (c.A(NS_ZP, n) < zz) return n;
+    }
+    return -1;
+}
+
+// refined node search of GEOM_COUPLING (1d.f:5190-5240): (kf-1,kf) on a node
+int node_search(CaseImage &c, double zc, int *k1, int *k2)
+{
+    for (int kf = 1; kf <= c.p.n3f; kf++) {
+        double zf = c.R(RF_ZF, kf);
+        if (fabs(zf - zc) <= 1.0e-10) { *k1 = kf - 1; *k2 = kf; return 0; }
+        else if (zf > zc) { *k1 = kf - 1; *k2 = kf - 1; return 0; }
+    }
+    return -1;
+}
+
+int build_case(const lica_case_desc &d, CaseImage &c)
+{
+    if (d.n < 8 || d.r < 1 || d.r > 8 || d.n > (1 << 22) || !(d.zl > 0.0) || !(d.re > 0.0) || !(d.prm > 0.0) ||
+        !(d.dt0 > 0.0) || !(d.denr > 0.0) || d.denr == 1.0)
+        return LICA_BAD_ARG;
+    InstParam &P = c.p;
+    memset(&P, 0, sizeof P);
+    const int n = (int)d.n, r = (int)d.r;
+    P.n = n; P.r = r; P.n3 = n + 1; P.n3fm = r * n; P.n3f = P.n3fm + 1; P.m3l = -2 * r;
+    P.ln = (int)round_up(n + 8, 16);
+    P.lni = P.ln;
+    P.lf = (int)round_up((int64_t)P.n3f + 4 * r + 4, 16);
+    P.lfi = (int)round_up((int64_t)P.n3f + 2, 16);
+    P.mf_band = (int)(int64_t)(0.1 * (double)((int64_t)r * r * P.n3fm));   // MF_BAND=0.1*MF
+    P.idtopt = (int)d.idtopt;
+    // INPUT_SIMULATION (1d.f:434-480)
+    const double pi = acos(-1.0);
+    P.re = d.re; P.prm = d.prm; P.denr = d.denr; P.ja = d.ja; P.tsat = d.tsat;
+    P.cflmax = d.cflmax; P.dt0 = d.dt0; P.z0 = d.z0; P.tslope = d.tslope;
+    P.rei = 1.0 / d.re;
+    P.denm = 1.0; P.denp = d.denr; P.den_diff = P.denp - P.denm;
+    P.vism = 1.0 * P.rei; P.visp = d.visr * P.rei;
+    P.denscm = P.denm * 1.0;
+    double denscp = P.denscm * (d.denr * d.scr);
+    P.densc_diff = denscp - P.denscm;
+    P.tcm = 1.0; P.tcp = d.tcr; P.tc_diff = P.tcp - P.tcm;
+    P.prai = 1.0 / d.prm;
+    double surf = (d.surf_j == 0.0) ? 0.0 : 1.0 / d.surf_j;
+    double gravity = (d.fr == 0.0) ? 0.0 : 1.0 / (d.fr * d.fr);
+    P.time_st = surf * pi / (P.denm + P.denp);
+    P.time_gv = P.den_diff / (P.denm + P.denp) * gravity / pi;
+    // RK3COEF (1d.f:1314-1319)
+    P.gamma[0] = 8.0 / 15.0; P.gamma[1] = 5.0 / 12.0; P.gamma[2] = 3.0 / 4.0;
+    P.ro[0] = 0.0; P.ro[1] = -17.0 / 60.0; P.ro[2] = -5.0 / 12.0;
+
+    c.ns.assign((size_t)NS_NF * P.ln, 0.0);
+    c.rf.assign((size_t)RF_NF * P.lf, 0.0);
+    c.nsi.assign((size_t)NSI_NF * P.lni, 0);
+    c.rfi.assign((size_t)RFI_NF * P.lfi, 0);
+
+    std::vector<double> zn, zfn, xfn;
+    uniform_nodes(n, d.zl, zn);
+    uniform_nodes(P.n3fm, d.zl, zfn);
+    P.xl = es23_15(1.0); P.yl = es23_15(1.0);
+    uniform_nodes(r, P.xl, xfn);
+    P.sdx1 = P.xl - 0.0; P.sdy1 = P.yl - 0.0;
+    P.xp1 = 0.0 + 0.5 * P.sdx1; P.yp1 = 0.0 + 0.5 * P.sdy1;
+    P.sdxf = xfn[1] - xfn[0]; P.sdyf = P.sdxf;
+    std::vector<double> xpf((size_t)r + 1);
+    for (int i = 1; i <= r; i++) xpf[(size_t)i] = xfn[(size_t)i - 1] + 0.5 * P.sdxf;
+
+    const int n3 = P.n3;
+    for (int k = 1; k <= n3; k++) c.A(NS_Z, k) = zn[(size_t)k - 1];
+    c.A(NS_Z, 0) = c.A(NS_Z, 1);
+    // GEOMETRIC (1d.f:4838-4880)
+    for (int k = 1; k <= n; k++) c.A(NS_SDZ, k) = c.A(NS_Z, k + 1) - c.A(NS_Z, k);
+    for (int k = 2; k <= n; k++) c.A(NS_VDZ, k) = 0.5 * (c.A(NS_SDZ, k) + c.A(NS_SDZ, k - 1));
+    c.A(NS_VDZ, 1) = 0.5 * c.A(NS_SDZ, 1);
+    c.A(NS_VDZ, n3) = 0.5 * c.A(NS_SDZ, n);
+    for (int k = 1; k <= n; k++) c.A(NS_SSDZ, k) = 1.0 / c.A(NS_SDZ, k);
+    for (int k = 1; k <= n3; k++) c.A(NS_VVDZ, k) = 1.0 / c.A(NS_VDZ, k);
+    c.A(NS_SDZ, 0) = 0.0; c.A(NS_SDZ, n3) = 0.0;
+    for (int k = 1; k <= n; k++) c.A(NS_ZP, k) = c.A(NS_Z, k) + 0.5 * c.A(NS_SDZ, k);
+    c.A(NS_ZP, n3) = c.A(NS_Z, n3);
+    c.A(NS_ZP, 0) = c.A(NS_Z, 1);
+    // LHSINIT (1d.f:1963-1978)
+    for (int k = 2; k <= n; k++) {
+        c.A(NS_AKW, k) = -c.A(NS_VVDZ, k) * c.A(NS_SSDZ, k - 1);
+        c.A(NS_CKW, k) = -c.A(NS_VVDZ, k) * c.A(NS_SSDZ, k);
+    }
+    for (int k = 1; k <= n; k++) {
+        c.A(NS_AKUV, k) = -c.A(NS_VVDZ, k) * c.A(NS_SSDZ, k);
+        c.A(NS_CKUV, k) = -c.A(NS_VVDZ, k + 1) * c.A(NS_SSDZ, k);
+    }
+    // GEOMETRIC_LVS (1d.f:4885-4942)
+    const int n3f = P.n3f, n3fm = P.n3fm;
+    for (int k = 1; k <= n3f; k++) c.R(RF_ZF, k) = zfn[(size_t)k - 1];
+    P.sdzf = c.R(RF_ZF, 2) - c.R(RF_ZF, 1);
+    P.ssdzf = 1.0 / P.sdzf;
+    for (int k = 1; k <= n3fm; k++) c.R(RF_ZPF, k) = c.R(RF_ZF, k) + 0.5 * P.sdzf;
+    for (int k = 0; k <= 2 * r; k++) {
+        c.R(RF_ZF, -k) = c.R(RF_ZF, 1) - P.sdzf * (double)(k + 1);
+        c.R(RF_ZF, n3f + 1 + k) = c.R(RF_ZF, n3f) + P.sdzf * (double)(k + 1);
+    }
+    for (int k = 0; k <= 2 * r; k++) {
+        c.R(RF_ZPF, -k) = c.R(RF_ZF, -k) + 0.5 * P.sdzf;
+        c.R(RF_ZPF, n3f + k) = c.R(RF_ZF, n3f + k) + 0.5 * P.sdzf;
+    }
+    // running volume of an all-vapour column prefix (serial sum of DVOL)
+    {
+        double v = 0.0;
+        c.R(RF_VPREFIX, 0) = 0.0;
+        for (int j = 1; j <= n3fm; j++) { v = v + (1.0 - 0.0) * P.sdzf; c.R(RF_VPREFIX, j) = v; }
+    }
+    // GEOM_COUPLING (1d.f:5134-5369)
+    for (int k = 1; k <= n3f; k++) {
+        int a = coump_vel_search(c, c.R(RF_ZF, k));
+        int b = coump_vel_search(c, c.R(RF_ZPF, k));
+        if (a < 0 || b < 0) return LICA_GRID_TOO_LARGE;
+        c.RI(RFI_KCOU_VEL, k) = a;
+        c.RI(RFI_KCOUMP_VEL, k) = b;
+    }
+    for (int k = 1; k <= n3; k++) {
+        int k1, k2;
+        if (node_search(c, c.A(NS_Z, k), &k1, &k2)) return LICA_GRID_TOO_LARGE;
+        c.AI(NSI_KCOU1, k) = k1; c.AI(NSI_KCOU2, k) = k2;
+    }
+    for (int f : {NSI_KCOU1, NSI_KCOU2}) {
+        c.AI(f, 0) = c.AI(f, 1) - (c.AI(f, n) - c.AI(f, n - 1));
+        c.AI(f, -1) = c.AI(f, 0) - (c.AI(f, n - 1) - c.AI(f, n - 2));
+        c.AI(f, n3 + 1) = c.AI(f, n3) + (c.AI(f, 2) - c.AI(f, 1));
+        c.AI(f, n3 + 2) = c.AI(f, n3 + 1) + (c.AI(f, 3) - c.AI(f, 2));
+    }
+    for (int k = 1; k <= n; k++) {
+        int k1, k2;
+        if (node_search(c, c.A(NS_ZP, k), &k1, &k2)) return LICA_GRID_TOO_LARGE;
+        c.AI(NSI_KCOUMP1, k) = k1; c.AI(NSI_KCOUMP2, k) = k2;
+    }
+    for (int f : {NSI_KCOUMP1, NSI_KCOUMP2}) {
+        c.AI(f, 0) = c.AI(f, 1) - (c.AI(f, n) - c.AI(f, n - 1));
+        c.AI(f, -1) = c.AI(f, 0) - (c.AI(f, n - 1) - c.AI(f, n - 2));
+        c.AI(f, n3) = c.AI(f, n) + (c.AI(f, n) - c.AI(f, n - 1));
+        c.AI(f, n3 + 1) = c.AI(f, n3) + (c.AI(f, 2) - c.AI(f, 1));
+    }
+    // PHINSINTP x/y weights (chphase.f:1274-1289): I1=1, I2 = refined node of XP(1)
+    {
+        int i1 = 1, i2 = -1;
+        for (int kf = 1; kf <= r + 1; kf++) {
+            if (fabs(xfn[(size_t)kf - 1] - P.xp1) <= 1.0e-10) { i2 = kf; break; }
+            else if (xfn[(size_t)kf - 1] > P.xp1) { i2 = kf - 1; break; }
+        }
+        if (i2 < 1) return LICA_GRID_TOO_LARGE;
+        P.phins_xd = (i1 == i2) ? 0.0 : (P.xp1 - xpf[(size_t)i1]) / (xpf[(size_t)i2] - xpf[(size_t)i1]);
+        P.phins_yd = P.phins_xd;
+    }
+    // outer-layer column distance of TEMP_GRAD (chphase.f:198-211): min DX^2+DY^2
+    {
+        double best = -1.0;
+        for (int j = 1; j <= r; j++)
+            for (int i = 1; i <= r; i++) {
+                double dx = xpf[(size_t)i] - P.xp1, dy = xpf[(size_t)j] - P.yp1;
+                double d2 = dx * dx + dy * dy;
+                if (best < 0.0 || d2 < best) best = d2;
+            }
+        P.dxy2_min = best;
+    }
+    return LICA_OK;
+}
+
+// ---------------------------------------------------------------------
+// deck reading: list-directed records, one per line (1d.f:347-386, grid.f:29-53)
+struct LineReader {
+    std::vector<std::string> lines;
+    size_t pos = 0;
+    bool open(const char *path)
+    {
+        FILE *f = fopen(path, "r");
+        if (!f) return false;
+        char buf[4096];
+        while (fgets(buf, sizeof buf, f)) lines.emplace_back(buf);
+        fclose(f);
+        return true;
+    }
+    bool skip() { if (pos >= lines.size()) return false; pos++; return true; }
+    // numeric fields of the next record (Fortran D exponents accepted)
+    bool nums(int want, double *out)
+    {
+        if (pos >= lines.size()) return false;
+        std::string s = lines[pos++];
+        for (char &ch : s) if (ch == 'd' || ch == 'D') ch = 'e';
+        for (char &ch : s) if (ch == ',' || ch == '\t') ch = ' ';
+        const char *p = s.c_str();
+        for (int i = 0; i < want; i++) {
+            char *end;
+            while (*p == ' ') p++;
+            double v = strtod(p, &end);
+            if (end == p) return false;
+            out[i] = v;
+            p = end;
+        }
+        return true;
+    }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------
+struct lica_batch {
+    int64_t ninst = 0;
+    std::vector<lica_case_desc> cases;
+    std::vector<InstParam> par;
+    size_t ns_total = 0, rf_total = 0, nsi_total = 0, rfi_total = 0;
+    InstParam *d_par = nullptr;
+    InstState *d_st = nullptr;
+    double *d_ns = nullptr, *d_rf = nullptr;
+    int *d_nsi = nullptr, *d_rfi = nullptr, *d_order = nullptr;
+    std::vector<std::vector<double>> img_ns, img_rf;     // initial images (geometry)
+    std::vector<std::vector<int>> img_nsi, img_rfi;
+    int threads = 256;
+    int poisson_mode = LICA_POISSON_THOMAS;
+    double rtol = 1.0e-10;
+    int max_cycles = 100;
+    bool use_graph = false;
+    bool timing = false;
+    bool initialized = false;
+    int last_msub = 1;
+    hipStream_t stream = nullptr;
+#ifndef LICA_EMUL
+    hipGraphExec_t graph = nullptr;
+    int graph_threads = -1, graph_mode = -1;
+    std::vector<hipEvent_t> ev;
+#endif
+    double timings[KL_COUNT] = {0};
+    double wall_ms = 0.0;
+};
+
+namespace {
+
+BatchDev make_bd(const lica_batch *b, int msub)
+{
+    BatchDev bd;
+    bd.par = b->d_par; bd.st = b->d_st;
+    bd.ns = b->d_ns; bd.rf = b->d_rf; bd.nsi = b->d_nsi; bd.rfi = b->d_rfi;
+    bd.order = b->d_order;
+    bd.ninst = (int)b->ninst;
+    bd.tri_mode = (b->poisson_mode == LICA_POISSON_THOMAS) ? 0 : 1;
+    bd.msub = msub;
+    return bd;
+}
+
+int launch(lica_batch *b, int which, int msub)
+{
+    BatchDev bd = make_bd(b, msub);
+    int rc = lica_launch(which, bd, b->threads, (void *)b->stream);
+    return rc == 0 ? 0 : LICA_HIP_ERROR;
+}
+
+// the launches of one time step (1d.f:196-295)
+int enqueue_step(lica_batch *b)
+{
+    int rc = launch(b, KL_DT, 1);
+    for (int msub = 1; msub <= 3 && !rc; msub++) {
+        if (!rc) rc = launch(b, KL_CONT_ENERGY, msub);
+        if (!rc) rc = launch(b, KL_TEMP_GRAD, msub);
+        if (!rc) rc = launch(b, KL_MOMENTUM, msub);
+        if (!rc) rc = launch(b, KL_POISSON_DIRECT, msub);
+        if (!rc) rc = launch(b, KL_UCALC, msub);
+        if (!rc) rc = launch(b, KL_TRANSPORT, msub);
+    }
+    return rc;
+}
+
+#ifndef LICA_EMUL
+// same launches with an event after each one (per-kernel GPU time)
+int enqueue_step_timed(lica_batch *b, std::vector<int> &kinds)
+{
+    auto rec = [&](int kind) -> int {
+        kinds.push_back(kind);
+        size_t i = kinds.size();
+        while (b->ev.size() <= i) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return LICA_HIP_ERROR;
+            b->ev.push_back(e);
+        }
+        return hipEventRecord(b->ev[i], b->stream) == hipSuccess ? 0 : LICA_HIP_ERROR;
+    };
+    int rc = launch(b, KL_DT, 1);
+    if (!rc) rc = rec(KL_DT);
+    for (int msub = 1; msub <= 3 && !rc; msub++) {
+        const int seq[6] = {KL_CONT_ENERGY, KL_TEMP_GRAD, KL_MOMENTUM, KL_POISSON_DIRECT, KL_UCALC, KL_TRANSPORT};
+        for (int j = 0; j < 6 && !rc; j++) {
+            rc = launch(b, seq[j], msub);
+            if (!rc) rc = rec(seq[j]);
+        }
+    }
+    return rc;
+}
+
+int build_graph(lica_batch *b)
+{
+    if (b->graph) { (void)hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+    hipGraph_t g;
+    if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return LICA_HIP_ERROR;
+    int rc = enqueue_step(b);
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc || e != hipSuccess) return LICA_HIP_ERROR;
+    e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) { b->graph = nullptr; return LICA_HIP_ERROR; }
+    b->graph_threads = b->threads;
+    b->graph_mode = b->poisson_mode;
+    return 0;
+}
+#endif
+
+int check_batch(lica_batch *b) { return (b && b->ninst > 0) ? 0 : LICA_BAD_ARG; }
+
+int first_status(lica_batch *b, int64_t *bad)
+{
+    std::vector<InstState> st((size_t)b->ninst);
+    int rc = d2h(st.data(), b->d_st, sizeof(InstState) * st.size(), b->stream);
+    if (rc) return rc;
+    for (int64_t i = 0; i < b->ninst; i++)
+        if (st[(size_t)i].status) { if (bad) *bad = i; return st[(size_t)i].status; }
+    if (bad) *bad = -1;
+    return 0;
+}
+
+int get_state(lica_batch *b, int64_t inst, InstState *s)
+{
+    return d2h(s, b->d_st + inst, sizeof(InstState), b->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *lica_status_string(int s)
+{
+    switch (s) {
+    case LICA_OK: return "ok";
+    case LICA_BAND_OVERFLOW: return "narrow band exceeds MF_BAND";
+    case LICA_PSIF_BUG: return "CAL_PSIF_MTP reached its error branch";
+    case LICA_GRID_TOO_LARGE: return "grid coupling search failed";
+    case LICA_EXTRAP_NOT_CONVERGED: return "temperature extrapolation did not converge";
+    case LICA_BAND_CAPACITY: return "band layer holds more planes than the device record";
+    case LICA_NONFINITE: return "non-finite W, T or P";
+    case LICA_WINDOW: return "band left the device window";
+    case LICA_BAD_ARG: return "bad argument";
+    case LICA_UNSUPPORTED_DECK: return "deck describes a configuration other than the 1-D film-boiling column";
+    case LICA_IO: return "cannot read input file";
+    case LICA_HIP_ERROR: return "HIP runtime error";
+    case LICA_NO_DEVICE: return "no GPU device";
+    case LICA_NOT_CONVERGED: return "Poisson iteration hit its cycle cap";
+    }
+    return "unknown status";
+}
+
+int lica_parse_deck(const char *lica_in_path, const char *grid_in_path, lica_case_desc *out)
+{
+    if (!lica_in_path || !grid_in_path || !out) return LICA_BAD_ARG;
+    LineReader L;
+    if (!L.open(lica_in_path)) return LICA_IO;
+    double v[16];
+    double impl, idtopt, dt, cfl, ips, iphs, ibmon, ipx, ipy, ipz, itracking, re, visr, denr, surfj, fr, prm, scr, tcr,
+        tsat, ja;
+    bool ok = L.skip() && L.nums(5, v) && L.skip() && L.nums(4, v) && L.skip() && L.nums(4, v);
+    if (ok) { impl = v[0]; idtopt = v[1]; dt = v[2]; cfl = v[3]; }
+    ok = ok && L.skip() && L.nums(9, v) && L.skip() && L.nums(2, v);
+    if (ok) { ips = v[0]; iphs = v[1]; }
+    ok = ok && L.skip() && L.nums(2, v);
+    if (ok) ibmon = v[0];
+    ok = ok && L.skip() && L.nums(1, v) && L.skip() && L.nums(4, v);
+    if (ok) { ipx = v[1]; ipy = v[2]; ipz = v[3]; }
+    ok = ok && L.skip() && L.skip() && L.nums(3, v);
+    if (ok) itracking = v[0];
+    ok = ok && L.skip() && L.nums(3, v);
+    if (ok) { re = v[0]; visr = v[1]; denr = v[2]; }
+    ok = ok && L.skip() && L.nums(3, v);
+    if (ok) { surfj = v[0]; fr = v[1]; }
+    ok = ok && L.skip() && L.nums(3, v);
+    if (ok) { prm = v[0]; scr = v[1]; tcr = v[2]; }
+    ok = ok && L.skip() && L.nums(2, v);
+    if (ok) { tsat = v[0]; ja = v[1]; }
+    if (!ok) return LICA_IO;
+    (void)ips;
+    if (impl != 0.0 || iphs != 1.0 || ibmon != 0.0 || ipx != 1.0 || ipy != 1.0 || ipz != 3.0 || itracking != 2.0 ||
+        (idtopt != 0.0 && idtopt != 1.0))
+        return LICA_UNSUPPORTED_DECK;
+
+    LineReader G;
+    if (!G.open(grid_in_path)) return LICA_IO;
+    double g[8], sp[4];
+    if (!(G.skip() && G.nums(4, sp) && G.skip() && G.skip() && G.nums(7, g) && G.skip())) return LICA_IO;
+    if (sp[0] != 0.0 || sp[1] != 0.0 || sp[2] != 0.0 || sp[3] != 0.0) return LICA_UNSUPPORTED_DECK;
+    int nsect1 = (int)g[0];
+    double n1 = g[1], n2 = g[2], n3 = g[3], xl = g[4], yl = g[5], zl = g[6];
+    auto sections = [&](int ns, int dirbase, double len[3], double np[3]) -> int {
+        for (int l = 0; l < ns; l++) {
+            double s[8];
+            if (!G.nums(8, s)) return LICA_IO;
+            int dir = (int)s[0] - dirbase;
+            if (dir < 1 || dir > 3) return LICA_UNSUPPORTED_DECK;
+            // one uniform section spanning each direction (IOPT=1)
+            if (s[1] != 0.0 || s[5] != 1.0 || s[3] != 1.0 || s[2] != len[dir - 1] || s[4] != np[dir - 1])
+                return LICA_UNSUPPORTED_DECK;
+        }
+        return 0;
+    };
+    if (nsect1 != 3) return LICA_UNSUPPORTED_DECK;
+    double len1[3] = {xl, yl, zl}, np1[3] = {n1, n2, n3};
+    int rc = sections(nsect1, 0, len1, np1);
+    if (rc) return rc;
+    double h[8];
+    if (!(G.skip() && G.skip() && G.nums(7, h) && G.skip())) return LICA_IO;
+    int nsect2 = (int)h[0];
+    double len2[3] = {h[4], h[5], h[6]}, np2[3] = {h[1], h[2], h[3]};
+    if (nsect2 != 3) return LICA_UNSUPPORTED_DECK;
+    rc = sections(nsect2, 3, len2, np2);
+    if (rc) return rc;
+    // the 1-D column: one NS cell in x/y of unit width, refined r x r x rN
+    if (n1 != 2.0 || n2 != 2.0 || xl != 1.0 || yl != 1.0 || h[4] != xl || h[5] != yl || h[6] != zl)
+        return LICA_UNSUPPORTED_DECK;
+    int64_t ncell = (int64_t)n3 - 1, nf = (int64_t)h[3] - 1, rr = (int64_t)h[1] - 1;
+    if (ncell < 8 || rr < 1 || (int64_t)h[2] - 1 != rr || nf != rr * ncell) return LICA_UNSUPPORTED_DECK;
+
+    memset(out, 0, sizeof *out);
+    out->n = ncell; out->r = rr; out->zl = zl;
+    out->re = re; out->visr = visr; out->denr = denr;
+    out->surf_j = surfj; out->fr = fr;
+    out->prm = prm; out->scr = scr; out->tcr = tcr;
+    out->tsat = tsat; out->ja = ja;
+    out->dt0 = dt; out->cflmax = cfl; out->idtopt = (int64_t)idtopt;
+    out->z0 = 0.0607;     // ASSIGN_LVS (two_phase.f:108)
+    out->tslope = 16.47;  // MAKEFLD vapour temperature T=16.47*ZP (1d.f:909)
+    return LICA_OK;
+}
+
+int lica_batch_create(int64_t ninst, const lica_case_desc *cases, lica_batch **out)
+{
+    if (!out || ninst <= 0 || !cases) return LICA_BAD_ARG;
+    *out = nullptr;
+    if (!have_device()) return LICA_NO_DEVICE;
+    lica_batch *b = new lica_batch();
+    b->ninst = ninst;
+    b->cases.assign(cases, cases + ninst);
+    b->par.resize((size_t)ninst);
+    b->img_ns.resize((size_t)ninst); b->img_rf.resize((size_t)ninst);
+    b->img_nsi.resize((size_t)ninst); b->img_rfi.resize((size_t)ninst);
+    for (int64_t i = 0; i < ninst; i++) {
+        CaseImage c;
+        int rc = build_case(cases[i], c);
+        if (rc) { delete b; return rc; }
+        c.p.ns_off = (long long)b->ns_total; b->ns_total += c.ns.size();
+        c.p.rf_off = (long long)b->rf_total; b->rf_total += c.rf.size();
+        c.p.nsi_off = (long long)b->nsi_total; b->nsi_total += c.nsi.size();
+        c.p.rfi_off = (long long)b->rfi_total; b->rfi_total += c.rfi.size();
+        b->par[(size_t)i] = c.p;
+        b->img_ns[(size_t)i].swap(c.ns); b->img_rf[(size_t)i].swap(c.rf);
+        b->img_nsi[(size_t)i].swap(c.nsi); b->img_rfi[(size_t)i].swap(c.rfi);
+    }
+#ifndef LICA_EMUL
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) { delete b; return LICA_HIP_ERROR; }
+#endif
+    int rc = 0;
+    rc = rc ? rc : dmalloc((void **)&b->d_par, sizeof(InstParam) * (size_t)ninst);
+    rc = rc ? rc : dmalloc((void **)&b->d_st, sizeof(InstState) * (size_t)ninst);
+    rc = rc ? rc : dmalloc((void **)&b->d_ns, sizeof(double) * b->ns_total);
+    rc = rc ? rc : dmalloc((void **)&b->d_rf, sizeof(double) * b->rf_total);
+    rc = rc ? rc : dmalloc((void **)&b->d_nsi, sizeof(int) * b->nsi_total);
+    rc = rc ? rc : dmalloc((void **)&b->d_rfi, sizeof(int) * b->rfi_total);
+    rc = rc ? rc : dmalloc((void **)&b->d_order, sizeof(int) * (size_t)ninst);
+    if (rc) { lica_batch_destroy(b); return rc; }
+    // workgroup order: largest cases first (longest-processing-time first)
+    std::vector<int> order((size_t)ninst);
+    for (int64_t i = 0; i < ninst; i++) order[(size_t)i] = (int)i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return b->par[(size_t)a].n > b->par[(size_t)c].n; });
+    rc = h2d(b->d_order, order.data(), sizeof(int) * order.size(), b->stream);
+    rc = rc ? rc : h2d(b->d_par, b->par.data(), sizeof(InstParam) * b->par.size(), b->stream);
+    if (rc) { lica_batch_destroy(b); return rc; }
+    *out = b;
+    return LICA_OK;
+}
+
+int lica_batch_init(lica_batch *b)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    // fresh geometry and zeroed fields, then MAKEFLD/LVSINIT/GRID_COUPLING(1) on the device
+    for (int64_t i = 0; i < b->ninst && !rc; i++) {
+        const InstParam &p = b->par[(size_t)i];
+        rc = h2d(b->d_ns + p.ns_off, b->img_ns[(size_t)i].data(), sizeof(double) * b->img_ns[(size_t)i].size(), b->stream);
+        rc = rc ? rc : h2d(b->d_rf + p.rf_off, b->img_rf[(size_t)i].data(), sizeof(double) * b->img_rf[(size_t)i].size(), b->stream);
+        rc = rc ? rc : h2d(b->d_nsi + p.nsi_off, b->img_nsi[(size_t)i].data(), sizeof(int) * b->img_nsi[(size_t)i].size(), b->stream);
+        rc = rc ? rc : h2d(b->d_rfi + p.rfi_off, b->img_rfi[(size_t)i].data(), sizeof(int) * b->img_rfi[(size_t)i].size(), b->stream);
+    }
+    if (rc) return rc;
+    std::vector<InstState> st((size_t)b->ninst);
+    memset((void *)st.data(), 0, sizeof(InstState) * st.size());
+    rc = h2d(b->d_st, st.data(), sizeof(InstState) * st.size(), b->stream);
+    if (rc) return rc;
+    rc = launch(b, KL_INIT, 1);
+    if (!rc) rc = dsync(b->stream);
+    if (rc) return rc;
+    b->initialized = true;
+    int64_t bad;
+    return first_status(b, &bad);
+}
+
+int lica_batch_run(lica_batch *b, int64_t nsteps)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (!b->initialized || nsteps < 0) return LICA_BAD_ARG;
+    if (b->poisson_mode != LICA_POISSON_THOMAS && b->poisson_mode != LICA_POISSON_DIRECT) return LICA_UNSUPPORTED_DECK;
+    for (int i = 0; i < KL_COUNT; i++) b->timings[i] = 0.0;
+    auto t0 = std::chrono::steady_clock::now();
+#ifdef LICA_EMUL
+    for (int64_t s = 0; s < nsteps && !rc; s++) rc = enqueue_step(b);
+#else
+    if (b->timing) {
+        for (int64_t s = 0; s < nsteps && !rc; s++) {
+            std::vector<int> kinds;
+            if (b->ev.empty()) {
+                hipEvent_t e;
+                if (hipEventCreate(&e) != hipSuccess) return LICA_HIP_ERROR;
+                b->ev.push_back(e);
+            }
+            if (hipEventRecord(b->ev[0], b->stream) != hipSuccess) return LICA_HIP_ERROR;
+            rc = enqueue_step_timed(b, kinds);
+            if (rc) break;
+            if (hipEventSynchronize(b->ev[kinds.size()]) != hipSuccess) return LICA_HIP_ERROR;
+            for (size_t j = 0; j < kinds.size(); j++) {
+                float ms = 0.0f;
+                if (hipEventElapsedTime(&ms, b->ev[j], b->ev[j + 1]) != hipSuccess) return LICA_HIP_ERROR;
+                b->timings[kinds[j]] += (double)ms;
+            }
+        }
+    } else if (b->use_graph) {
+        if (!b->graph || b->graph_threads != b->threads || b->graph_mode != b->poisson_mode) rc = build_graph(b);
+        for (int64_t s = 0; s < nsteps && !rc; s++)
+            if (hipGraphLaunch(b->graph, b->stream) != hipSuccess) rc = LICA_HIP_ERROR;
+    } else {
+        for (int64_t s = 0; s < nsteps && !rc; s++) rc = enqueue_step(b);
+    }
+#endif
+    if (!rc) rc = dsync(b->stream);
+    b->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (rc) return rc;
+    int64_t bad;
+    return first_status(b, &bad);
+}
+
+int lica_batch_set_poisson(lica_batch *b, int64_t mode, double rtol, int64_t max_cycles)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (mode < LICA_POISSON_THOMAS || mode > LICA_POISSON_LEARNED || !(rtol > 0.0) || max_cycles < 1) return LICA_BAD_ARG;
+    if (mode != LICA_POISSON_THOMAS && mode != LICA_POISSON_DIRECT) return LICA_UNSUPPORTED_DECK;
+    b->poisson_mode = (int)mode;
+    b->rtol = rtol;
+    b->max_cycles = (int)max_cycles;
+    return LICA_OK;
+}
+
+int lica_batch_set_block(lica_batch *b, int64_t threads)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (threads < 64 || threads > 1024 || (threads & (threads - 1))) return LICA_BAD_ARG;
+    b->threads = (int)threads;
+    return LICA_OK;
+}
+
+int lica_batch_use_graph(lica_batch *b, int64_t on)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    b->use_graph = on != 0;
+    return LICA_OK;
+}
+
+int lica_batch_set_timing(lica_batch *b, int64_t on)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    b->timing = on != 0;
+    return LICA_OK;
+}
+
+int lica_batch_sync(lica_batch *b)
+{
+    int rc = check_batch(b);
+    return rc ? rc : dsync(b->stream);
+}
+
+int64_t lica_batch_size(lica_batch *b) { return b ? b->ninst : 0; }
+
+int lica_batch_status(lica_batch *b, int64_t *first_bad_inst)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    rc = dsync(b->stream);
+    return rc ? rc : first_status(b, first_bad_inst);
+}
+
+int lica_batch_get_scalars(lica_batch *b, int64_t inst, double *out)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (inst < 0 || inst >= b->ninst || !out) return LICA_BAD_ARG;
+    InstState s;
+    rc = get_state(b, inst, &s);
+    if (rc) return rc;
+    const InstParam &p = b->par[(size_t)inst];
+    double t01[2], vv1;
+    rc = d2h(t01, b->d_ns + p.ns_off + (size_t)NS_T * p.ln + 0 + 2, 2 * sizeof(double), b->stream);
+    rc = rc ? rc : d2h(&vv1, b->d_ns + p.ns_off + (size_t)NS_VVDZ * p.ln + 1 + 2, sizeof(double), b->stream);
+    if (rc) return rc;
+    for (int i = 0; i < LICA_NSCAL; i++) out[i] = 0.0;
+    out[LICA_S_TIME] = s.time; out[LICA_S_DT] = s.dt; out[LICA_S_STEP] = (double)s.m;
+    out[LICA_S_PLACE] = s.place; out[LICA_S_DVMAX] = s.dvmax; out[LICA_S_CFLM] = s.cflm;
+    out[LICA_S_VOL_ORI] = s.vol_tot_ori; out[LICA_S_VOL_IPHS] = s.vol_tot_iphs;
+    // wall heat flux k_v dT/dz at the wall face
+    out[LICA_S_QWALL] = p.tcm * (t01[1] - t01[0]) * vv1;
+    out[LICA_S_STATUS] = (double)s.status;
+    out[LICA_S_SUBCYCLE_WARN] = s.n_subcycle_warn; out[LICA_S_MASS_WARN] = s.n_mass_warn;
+    out[LICA_S_EXTRAP_HALVINGS] = s.n_halvings; out[LICA_S_EXTRAP_ITERS] = s.n_extrap_iters;
+    out[LICA_S_REINIT_CALLS] = s.n_reinit; out[LICA_S_BANDGEN_CALLS] = s.n_bandgen;
+    out[LICA_S_MASSCOR_ITERS] = s.n_masscor;
+    out[LICA_S_POISSON_CYCLES] = s.poisson_cycles; out[LICA_S_POISSON_SOLVES] = s.poisson_solves;
+    out[LICA_S_POISSON_RESID] = s.poisson_resid;
+    return LICA_OK;
+}
+
+int lica_batch_get_field(lica_batch *b, int64_t field, int64_t inst, double *out, int64_t cap, int64_t *count)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (inst < 0 || inst >= b->ninst || !out || cap < 0) return LICA_BAD_ARG;
+    const InstParam &p = b->par[(size_t)inst];
+    int64_t m = 0;
+    std::vector<double> buf;
+    auto ns_field = [&](int f) {
+        buf.resize((size_t)p.n3 + 1);
+        int r2 = d2h(buf.data(), b->d_ns + p.ns_off + (size_t)f * p.ln + 2, sizeof(double) * buf.size(), b->stream);
+        for (size_t k = 0; k < buf.size() && m < cap; k++) out[m++] = buf[k];
+        return r2;
+    };
+    auto rf_field = [&](int f) {
+        buf.resize((size_t)p.n3fm);
+        int r2 = d2h(buf.data(), b->d_rf + p.rf_off + (size_t)f * p.lf + (1 - p.m3l), sizeof(double) * buf.size(), b->stream);
+        for (size_t k = 0; k < buf.size() && m < cap; k++) out[m++] = buf[k];
+        return r2;
+    };
+    switch (field) {
+    case LICA_F_W: rc = ns_field(NS_W); break;
+    case LICA_F_P: rc = ns_field(NS_P); break;
+    case LICA_F_T: rc = ns_field(NS_T); break;
+    case LICA_F_PSI_CN: rc = ns_field(NS_PSI_CN); break;
+    case LICA_F_PSI_ZN: rc = ns_field(NS_PSI_ZN); break;
+    case LICA_F_WO: rc = ns_field(NS_WO); break;
+    case LICA_F_PSIF: rc = rf_field(RF_PSIF); break;
+    case LICA_F_SFGAS: rc = rf_field(RF_SFGAS); break;
+    case LICA_F_PHI: case LICA_F_BAND: case LICA_F_ALPHI: case LICA_F_MFLVS: {
+        InstState s;
+        rc = get_state(b, inst, &s);
+        if (rc) break;
+        if (field == LICA_F_PHI) {
+            // ALPHI_EXT image: zero off the band, band planes from ALPHI
+            std::vector<double> phi((size_t)p.n3fm + 1, 0.0);
+            for (int nl = 1; nl <= LICA_MLAYER; nl++)
+                for (int q = 0; q < s.bn[nl]; q++) {
+                    int k = s.bk[nl][q];
+                    if (k >= 1 && k <= p.n3fm) phi[(size_t)k] = s.alphi[nl][q];
+                }
+            for (int k = 1; k <= p.n3fm && m < cap; k++) out[m++] = phi[(size_t)k];
+        } else if (field == LICA_F_BAND) {
+            for (int nl = 1; nl <= LICA_MLAYER; nl++) {
+                if (m < cap) out[m++] = (double)s.bn[nl];
+                for (int q = 0; q < s.bn[nl] && m < cap; q++) out[m++] = (double)s.bk[nl][q];
+            }
+        } else {
+            for (int nl = 1; nl <= LICA_MLAYER; nl++)
+                for (int q = 0; q < s.bn[nl] && m < cap; q++)
+                    out[m++] = (field == LICA_F_ALPHI) ? s.alphi[nl][q] : s.mflvs[nl][q];
+        }
+        break;
+    }
+    default: return LICA_BAD_ARG;
+    }
+    if (count) *count = m;
+    return rc;
+}
+
+int lica_batch_last_timings(lica_batch *b, double *ms_out, int64_t cap)
+{
+    if (!b || !ms_out) return 0;
+    int64_t m = 0;
+    for (int i = 0; i < KL_COUNT && m < cap; i++) ms_out[m++] = b->timings[i];
+    if (m < cap) ms_out[m++] = b->wall_ms;
+    return (int)m;
+}
+
+void lica_batch_destroy(lica_batch *b)
+{
+    if (!b) return;
+#ifndef LICA_EMUL
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    for (hipEvent_t e : b->ev) (void)hipEventDestroy(e);
+#endif
+    dfree(b->d_par); dfree(b->d_st); dfree(b->d_ns); dfree(b->d_rf);
+    dfree(b->d_nsi); dfree(b->d_rfi); dfree(b->d_order);
+#ifndef LICA_EMUL
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+#endif
+    delete b;
+}
+
+// ---- single-case API -------------------------------------------------
+int lica_init(lica_batch **ctx, const char *lica_in_path, const char *grid_in_path)
+{
+    if (!ctx) return LICA_BAD_ARG;
+    *ctx = nullptr;
+    lica_case_desc d;
+    int rc = lica_parse_deck(lica_in_path, grid_in_path, &d);
+    if (rc) return rc;
+    rc = lica_batch_create(1, &d, ctx);
+    if (rc) return rc;
+    return lica_batch_init(*ctx);
+}
+
+int lica_step(lica_batch *ctx, int64_t nsteps) { return lica_batch_run(ctx, nsteps); }
+
+int lica_determine_dt(lica_batch *ctx, double *cflm, double *dt)
+{
+    int rc = check_batch(ctx);
+    if (rc) return rc;
+    rc = launch(ctx, KL_DT, 1);
+    rc = rc ? rc : dsync(ctx->stream);
+    InstState s;
+    rc = rc ? rc : get_state(ctx, 0, &s);
+    if (rc) return rc;
+    if (cflm) *cflm = s.cflm;
+    if (dt) *dt = s.dt;
+    return s.status;
+}
+
+static int copy_ns_out(lica_batch *b, int f, double *out)
+{
+    if (!out) return 0;
+    const InstParam &p = b->par[0];
+    return d2h(out, b->d_ns + p.ns_off + (size_t)f * p.ln + 2, sizeof(double) * ((size_t)p.n3 + 1), b->stream);
+}
+
+int lica_rhsnlhs(lica_batch *ctx, int64_t msub, double *w, double *p, double *psi_zn, double *psi_cn)
+{
+    int rc = check_batch(ctx);
+    if (rc) return rc;
+    if (msub < 1 || msub > 3) return LICA_BAD_ARG;
+    ctx->last_msub = (int)msub;
+    const int seq[5] = {KL_CONT_ENERGY, KL_TEMP_GRAD, KL_MOMENTUM, KL_POISSON_DIRECT, KL_UCALC};
+    for (int j = 0; j < 5 && !rc; j++) rc = launch(ctx, seq[j], (int)msub);
+    rc = rc ? rc : dsync(ctx->stream);
+    rc = rc ? rc : copy_ns_out(ctx, NS_W, w);
+    rc = rc ? rc : copy_ns_out(ctx, NS_P, p);
+    rc = rc ? rc : copy_ns_out(ctx, NS_PSI_ZN, psi_zn);
+    rc = rc ? rc : copy_ns_out(ctx, NS_PSI_CN, psi_cn);
+    if (rc) return rc;
+    int64_t bad;
+    return first_status(ctx, &bad);
+}
+
+int lica_convrge1(lica_batch *ctx, double *dvmax)
+{
+    int rc = check_batch(ctx);
+    if (rc) return rc;
+    InstState s;
+    rc = get_state(ctx, 0, &s);
+    if (rc) return rc;
+    if (dvmax) *dvmax = s.dvmax;
+    return LICA_OK;
+}
+
+int lica_transport(lica_batch *ctx)
+{
+    int rc = check_batch(ctx);
+    if (rc) return rc;
+    rc = launch(ctx, KL_TRANSPORT, ctx->last_msub);
+    rc = rc ? rc : dsync(ctx->stream);
+    if (rc) return rc;
+    int64_t bad;
+    return first_status(ctx, &bad);
+}
+
+int lica_grid_coupling(lica_batch *ctx, int64_t ini_psi, double *psi_zn, double *psi_cn)
+{
+    int rc = check_batch(ctx);
+    if (rc) return rc;
+    if (ini_psi != 0) return LICA_BAD_ARG;   // GRID_COUPLING(1) belongs to lica_init
+    rc = copy_ns_out(ctx, NS_PSI_ZN, psi_zn);
+    return rc ? rc : copy_ns_out(ctx, NS_PSI_CN, psi_cn);
+}
+
+int lica_end_step(lica_batch *ctx)
+{
+    int rc = check_batch(ctx);
+    if (rc) return rc;
+    int64_t bad;
+    rc = dsync(ctx->stream);
+    return rc ? rc : first_status(ctx, &bad);
+}
+
+}  // extern "C"
+
+extern "C" int lica_poisson(int64_t nsys, int64_t n, const double *a, const double *b, const double *c,
+                            const double *g, double *p, int64_t mode, double rtol, int64_t max_cycles,
+                            int64_t *cycles_out)
+{
+    if (nsys <= 0 || n < 1 || !a || !b || !c || !g || !p) return LICA_BAD_ARG;
+    if (mode != LICA_POISSON_THOMAS && mode != LICA_POISSON_DIRECT) return LICA_UNSUPPORTED_DECK;
+    (void)rtol; (void)max_cycles;
+    if (!have_device()) return LICA_NO_DEVICE;
+    hipStream_t st = nullptr;
+#ifndef LICA_EMUL
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return LICA_HIP_ERROR;
+#endif
+    const size_t tot = (size_t)nsys * (size_t)n, bytes = tot * sizeof(double);
+    double *da = nullptr, *db = nullptr, *dc = nullptr, *dg = nullptr;
+    int rc = dmalloc((void **)&da, bytes);
+    rc = rc ? rc : dmalloc((void **)&db, bytes);
+    rc = rc ? rc : dmalloc((void **)&dc, bytes);
+    rc = rc ? rc : dmalloc((void **)&dg, bytes);
+    rc = rc ? rc : h2d(da, a, bytes, st);
+    rc = rc ? rc : h2d(db, b, bytes, st);
+    rc = rc ? rc : h2d(dc, c, bytes, st);
+    rc = rc ? rc : h2d(dg, g, bytes, st);
+    if (!rc && lica_launch_tri((int)nsys, (int)n, da, db, dc, dg, mode == LICA_POISSON_THOMAS ? 0 : 1, 256, (void *)st))
+        rc = LICA_HIP_ERROR;
+    rc = rc ? rc : dsync(st);
+    rc = rc ? rc : d2h(p, dg, bytes, st);
+    dfree(da); dfree(db); dfree(dc); dfree(dg);
+#ifndef LICA_EMUL
+    (void)hipStreamDestroy(st);
+#endif
+    if (cycles_out) *cycles_out = 0;
+    return rc;
+}

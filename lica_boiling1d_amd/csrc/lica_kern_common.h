@@ -1,0 +1,534 @@
+// HIP kernels of the per-step path (gfx950).  One workgroup per case.
+//
+// Kernel sequence of a time step: k_dt (DETERMINE_DT, 1d.f:201), then per
+// RK3 substep (1d.f:220-246):
+//   k_cont_energy    CAL_CONTINUITY, CAL_CONT_BC,
+//                    energy RHS + LHSPS/LHSPS2 solve           (block)
+//   k_temp_grad      TEMP_GRAD incl. TEMP_EXTP, TF_GRAD, VOL_TOT_MFDOT
+//                    (steps after the first)                   (one wave)
+//   k_momentum       CAL_CONTINUITY2, RHSZ, LHSW, BC, FSM_CHOI, DIVGS and
+//                    the POISSON matrix                        (block)
+//   k_poisson        POISSON solve (1d.f:3492-3578) or multigrid (block)
+//   k_ucalc          UCALC, p-=p(1), CONVRGE1, VEL_INTP's UT/UO  (block)
+//   k_transport      TRANSPORT + GRID_COUPLING(0)              (one wave)
+// The narrow band, the refined level-set window and the temperature
+// extrapolation window live in LDS; NS fields stream from HBM.
+#include "lica_par.h"
+#include "lica_types.h"
+#pragma once
+
+#ifdef LICA_EMUL
+#include <cstdio>
+#endif
+
+#define WMAX 192       // refined window (z-planes) held in LDS
+#define NSWMAX 128     // NS window of the temperature extrapolation
+#define MAXC 128       // tridiagonal partition chunks
+
+struct CaseP {
+    const InstParam *P;
+    double *ns, *rf;
+    int *nsi, *rfi;
+    DEV double *A(int f) const { return ns + (size_t)f * P->ln + 2; }
+    DEV double *R(int f) const { return rf + (size_t)f * P->lf - P->m3l; }
+    DEV int *AI(int f) const { return nsi + (size_t)f * P->lni + 2; }
+    DEV int *RI(int f) const { return rfi + (size_t)f * P->lfi; }
+};
+
+DEV CaseP case_ptrs(const BatchDev &bd, int inst)
+{
+    CaseP c;
+    c.P = &bd.par[inst];
+    c.ns = bd.ns + c.P->ns_off;
+    c.rf = bd.rf + c.P->rf_off;
+    c.nsi = bd.nsi + c.P->nsi_off;
+    c.rfi = bd.rfi + c.P->rfi_off;
+    return c;
+}
+
+// ---------------------------------------------------------------------
+// scalar building blocks (two_phase.f:2448-2500, 2869-2920)
+
+DEV double d_hjweno(double a, double b, double c, double d)
+{
+    double eps = 1.0e-6;
+    double s1 = 13.0 * ((a - b) * (a - b)) + 3.0 * ((a - 3.0 * b) * (a - 3.0 * b));
+    double s2 = 13.0 * ((b - c) * (b - c)) + 3.0 * ((b + c) * (b + c));
+    double s3 = 13.0 * ((c - d) * (c - d)) + 3.0 * ((3.0 * c - d) * (3.0 * c - d));
+    double a1 = 1.0 / ((eps + s1) * (eps + s1));
+    double a2 = 6.0 / ((eps + s2) * (eps + s2));
+    double a3 = 3.0 / ((eps + s3) * (eps + s3));
+    double w1 = a1 / (a1 + a2 + a3);
+    double w3 = a3 / (a1 + a2 + a3);
+    return 1.0 / 3.0 * w1 * (a - 2.0 * b + c) + 1.0 / 6.0 * (w3 - 0.5) * (b - 2.0 * c + d);
+}
+
+DEV double d_godunov(double asign, double adlphip, double adlphim)
+{
+    double aplus = asign * adlphip, aminus = asign * adlphim, r = 0.0;
+    if (aplus >= 0.0 && aminus >= 0.0) r = adlphim;
+    else if (aplus <= 0.0 && aminus <= 0.0) r = adlphip;
+    else if (aplus > 0.0 && aminus < 0.0) r = 0.0;
+    else if (aplus < 0.0 && aminus > 0.0) r = (aplus + aminus > 0.0) ? adlphim : adlphip;
+    return r;
+}
+
+// liquid fraction of one refined cell of an x/y-flat column; -1 = bug branch
+DEV double d_psif_cell(double phi, double d3)
+{
+    const double eps = 1.0e-8;
+    double d1 = 0.0, d2 = 0.0;
+    double e1 = fmax(fmax(d1, d2), d3);
+    double e3 = fmin(fmin(d1, d2), d3);
+    double e2 = d1 + d2 + d3 - e1 - e3;
+    double t;
+    if (e1 > eps && e2 > eps && e3 > eps) {
+        double a = fabs(phi);
+        double f1 = fmax(0.5 * (e1 + e2 + e3) - a, 0.0), f2 = fmax(0.5 * (e1 + e2 - e3) - a, 0.0);
+        double f3 = fmax(0.5 * (e1 - e2 + e3) - a, 0.0), f4 = fmax(0.5 * (-e1 + e2 + e3) - a, 0.0);
+        double f5 = fmax(0.5 * (e1 - e2 - e3) - a, 0.0);
+        t = (f1 * f1 * f1 - f2 * f2 * f2 - f3 * f3 * f3 - f4 * f4 * f4 + f5 * f5 * f5) / (6.0 * e1 * e2 * e3);
+    } else if (e1 > eps && e2 > eps && e3 <= eps) {
+        double a = fabs(phi);
+        double f1 = fmax(0.5 * (e1 + e2 + e3) - a, 0.0), f3 = fmax(0.5 * (e1 - e2 + e3) - a, 0.0);
+        t = (f1 * f1 - f3 * f3) / (2.0 * e1 * e2);
+    } else if (e1 > eps && e2 <= eps && e3 <= eps) {
+        double f1 = fmax(0.5 * (e1 + e2 + e3) - fabs(phi), 0.0);
+        t = f1 / e1;
+    } else if (e1 <= eps && e2 <= eps && e3 <= eps && phi != 0.0) {
+        t = 0.0;
+    } else if (e1 <= eps && e2 <= eps && e3 <= eps && phi == 0.0) {
+        t = 0.5;
+    } else {
+        return -1.0;
+    }
+    return (phi <= 0.0) ? t : 1.0 - t;
+}
+
+DEV double snap5(double v) { return (v < 1.0e-5) ? 0.0 : ((v > 0.99999) ? 1.0 : v); }
+DEV double snap3(double v) { return (v < 1.0e-3) ? 0.0 : ((v > 0.999) ? 1.0 : v); }
+
+// box average of a refined field over [z1,z2] (INTER_PSI* family with the
+// x/y loops of the single NS column collapsed)
+template <class F>
+DEV double box_avg(const double *zf, double ssdzf, F f, double z1, double z2, int k1, int k2)
+{
+    double sum = 0.0, volf = 0.0;
+    for (int kk = k1; kk <= k2; kk++) {
+        double zl = zf[kk], zh = zf[kk + 1];
+        if (zl <= z1 && zh >= z2) { sum = sum + f(kk); volf = volf + 1.0; break; }
+        else if (zl < z1 && zh > z1) { sum = sum + f(kk) * (zh - z1) * ssdzf; volf = volf + (zh - z1) * ssdzf; }
+        else if (zl >= z1 && zh <= z2) { sum = sum + f(kk); volf = volf + 1.0; }
+        else if (zl < z2 && zh > z2) { sum = sum + f(kk) * (z2 - zl) * ssdzf; volf = volf + (z2 - zl) * ssdzf; }
+        else if (zl >= z2) break;
+    }
+    return sum / volf;
+}
+
+// ---------------------------------------------------------------------
+// Block-level tridiagonal solve of rows l1..l2: a(j)x(j-1)+b(j)x(j)+c(j)x(j+1)=g(j),
+// a(l1) and c(l2) ignored; the solution overwrites g.  Mode 0 is TRDIAG3
+// (1d.f:3783-3812) on one thread; mode 1 is a partitioned elimination
+// (chunks reduced to their end rows) with parallel cyclic reduction of the
+// end-row system in LDS.
+
+struct TriSh {
+    double ra[2][2 * MAXC], rb[2][2 * MAXC], rc[2][2 * MAXC], rg[2][2 * MAXC];
+};
+
+DEV void tri_solve(double *a, double *b, double *c, double *g, int l1, int l2, int mode, TriSh &ts)
+{
+    const int nrow = l2 - l1 + 1;
+    if (mode == 0 || nrow < 4) {
+        SERIAL {
+            double bet = 1.0 / b[l1];
+            g[l1] = g[l1] * bet;
+            for (int j = l1 + 1; j <= l2; j++) {
+                double gam = c[j - 1] * bet;
+                bet = 1.0 / (b[j] - a[j] * gam);
+                g[j] = (g[j] - a[j] * g[j - 1]) * bet;
+                a[j] = gam;
+            }
+            for (int j = l2 - 1; j >= l1; j--) g[j] = g[j] - a[j + 1] * g[j + 1];
+        }
+        SERIAL_END
+        return;
+    }
+    int nc = NTHREADS < MAXC ? NTHREADS : MAXC;
+    if (nc > nrow / 2) nc = nrow / 2;
+    const int base = nrow / nc, rem = nrow % nc;
+    const int m2 = 2 * nc;
+    // local elimination of every chunk
+    PFOR(ch, 0, nc) {
+        int s = l1 + ch * base + (ch < rem ? ch : rem);
+        int L = base + (ch < rem ? 1 : 0);
+        int e = s + L - 1;
+        for (int j = 2; j <= L - 1; j++) {
+            int i = s + j;
+            double r = a[i] / b[i - 1];
+            a[i] = -r * a[i - 1];
+            b[i] = b[i] - r * c[i - 1];
+            g[i] = g[i] - r * g[i - 1];
+        }
+        double sa = a[s], sb, sc, sg;
+        if (L >= 3) {
+            int i = s + L - 2;
+            double inv = 1.0 / b[i];
+            a[i] = a[i] * inv;
+            c[i] = c[i] * inv;
+            g[i] = g[i] * inv;
+            for (int j = L - 3; j >= 1; j--) {
+                i = s + j;
+                double cc = c[i];
+                inv = 1.0 / b[i];
+                double ah = (a[i] - cc * a[i + 1]) * inv;
+                double ch2 = -(cc * c[i + 1]) * inv;
+                double gh = (g[i] - cc * g[i + 1]) * inv;
+                a[i] = ah; c[i] = ch2; g[i] = gh;
+            }
+            double cs = c[s];
+            sb = b[s] - cs * a[s + 1];
+            sc = -(cs * c[s + 1]);
+            sg = g[s] - cs * g[s + 1];
+        } else {
+            sb = b[s]; sc = c[s]; sg = g[s];
+        }
+        ts.ra[0][2 * ch] = (ch == 0) ? 0.0 : sa;
+        ts.rb[0][2 * ch] = sb;
+        ts.rc[0][2 * ch] = sc;
+        ts.rg[0][2 * ch] = sg;
+        ts.ra[0][2 * ch + 1] = a[e];
+        ts.rb[0][2 * ch + 1] = b[e];
+        ts.rc[0][2 * ch + 1] = (ch == nc - 1) ? 0.0 : c[e];
+        ts.rg[0][2 * ch + 1] = g[e];
+    }
+    PFOR_END
+    // parallel cyclic reduction of the end-row system
+    int cur = 0;
+    for (int stride = 1; stride < m2; stride *= 2) {
+        PFOR(i, 0, m2) {
+            double ai = ts.ra[cur][i], bi = ts.rb[cur][i], ci = ts.rc[cur][i], gi = ts.rg[cur][i];
+            double am = 0.0, bm = 1.0, cm = 0.0, gm = 0.0, ap = 0.0, bp = 1.0, cp = 0.0, gp = 0.0;
+            if (i - stride >= 0) { am = ts.ra[cur][i - stride]; bm = ts.rb[cur][i - stride]; cm = ts.rc[cur][i - stride]; gm = ts.rg[cur][i - stride]; }
+            if (i + stride < m2) { ap = ts.ra[cur][i + stride]; bp = ts.rb[cur][i + stride]; cp = ts.rc[cur][i + stride]; gp = ts.rg[cur][i + stride]; }
+            double k1 = ai / bm, k2 = ci / bp;
+            ts.ra[1 - cur][i] = -am * k1;
+            ts.rc[1 - cur][i] = -cp * k2;
+            ts.rb[1 - cur][i] = bi - cm * k1 - ap * k2;
+            ts.rg[1 - cur][i] = gi - gm * k1 - gp * k2;
+        }
+        PFOR_END
+        cur = 1 - cur;
+    }
+    PFOR(ch, 0, nc) {
+        int s = l1 + ch * base + (ch < rem ? ch : rem);
+        int L = base + (ch < rem ? 1 : 0);
+        int e = s + L - 1;
+        double xs = ts.rg[cur][2 * ch] / ts.rb[cur][2 * ch];
+        double xe = ts.rg[cur][2 * ch + 1] / ts.rb[cur][2 * ch + 1];
+        for (int j = 1; j <= L - 2; j++) {
+            int i = s + j;
+            g[i] = g[i] - a[i] * xs - c[i] * xe;
+        }
+        g[s] = xs;
+        g[e] = xe;
+    }
+    PFOR_END
+}
+
+// ---------------------------------------------------------------------
+// Region classes of CAL_CONTINUITY / CAL_CONTINUITY2 (1d.f:2037-2056):
+// 2 within two cells of a jump of PSI_CN, 1 pure liquid, 0 otherwise.
+DEV int region_of(const double *pc, int n, int k)
+{
+    int j0 = k - 2 < 1 ? 1 : k - 2, j1 = k + 2 > n ? n : k + 2;
+    for (int j = j0; j <= j1; j++)
+        if ((pc[j + 1] - pc[j] != 0.0) || (pc[j] - pc[j - 1] != 0.0)) return 2;
+    return (pc[k] == 1.0) ? 1 : 0;
+}
+
+struct BlkSh {
+    double red[16];
+    double v[16];
+    int iv[16];
+    TriSh tri;
+};
+
+// DETERMINE_DT (1d.f:683-817) at the start of a step, with the per-step
+// reset of RK3ZO (1d.f:206-218); the CFL maximum |W(k)+W(k+1)|*SSDZ(k) was
+// recorded after the last projection of the previous step
+DEV void body_dt(const BatchDev &bd, int inst, BlkSh &sh)
+{
+    InstState &S = bd.st[inst];
+    if (S.status) return;
+    CaseP cp = case_ptrs(bd, inst);
+    const InstParam &P = *cp.P;
+    const int n = P.n, n3 = P.n3;
+    const double *PCN = cp.A(NS_PSI_CN), *SDZ = cp.A(NS_SDZ);
+    // DETERMINE_DT: capillary limit over interface cells, CFL from the
+    // max |W(k)+W(k+1)|*SSDZ(k) recorded after the last projection
+    double dtsrc;
+    if (P.time_st == 0.0) dtsrc = 1000000.0;
+    else {
+        dtsrc = red_min(1, n + 1, 1000.0, [&](int k) {
+            if (fabs(PCN[k] - 0.5) < 0.5) {
+                double dx1 = fmin(P.sdy1, SDZ[k]);
+                double t1 = 0.5 * dx1 / sqrt(P.time_st / dx1 + P.time_gv * dx1);
+                double dx2 = fmax(P.sdy1, SDZ[k]);
+                double t2 = 0.5 * dx2 / sqrt(P.time_st / dx2 + P.time_gv * dx2);
+                return fmin(t1, t2);
+            }
+            return 1000.0;
+        }, sh.red);
+    }
+    SERIAL {
+        S.m += 1;
+        double dt_old = S.dt;
+        double cflm = 0.5 * S.c3max * S.dt;
+        S.cflm = cflm;
+        double cfl_surf = S.dt / dtsrc;
+        if (P.idtopt == 1) {
+            double cfl_sum = 0.5 * (cflm + sqrt(cflm * cflm + 4.0 * cfl_surf));
+            S.dt = S.dt * P.cflmax / cfl_sum;
+        }
+        if (S.dt / dt_old > 2.0) S.dt = dt_old * 1.2;
+    }
+    SERIAL_END
+    double *RK3ZO = cp.A(NS_RK3ZO);
+    PFOR(k, 0, n3 + 1) { RK3ZO[k] = 0.0; }
+    PFOR_END
+}
+
+// ---------------------------------------------------------------------
+// k_cont_energy: CAL_CONTINUITY (1d.f:1994-2345), CAL_CONT_BC (2961-3003),
+// energy stage of PASSIVE_SCALAR (3630-3707) with CAL_RHSPS/CAL_RHSPS2 and
+// LHSPS (4038-4497).
+DEV void body_cont_energy(const BatchDev &bd, int inst, BlkSh &sh)
+{
+    InstState &S = bd.st[inst];
+    if (S.status) return;
+    CaseP cp = case_ptrs(bd, inst);
+    const InstParam &P = *cp.P;
+    const int n = P.n, n3 = P.n3, msub = bd.msub;
+    double *W = cp.A(NS_W), *T = cp.A(NS_T), *PZN = cp.A(NS_PSI_ZN), *PCN = cp.A(NS_PSI_CN);
+    double *DFZ = cp.A(NS_DF_Z), *VFZ = cp.A(NS_VF_Z), *DFC = cp.A(NS_DF_C);
+    double *PZ = cp.A(NS_PSI_Z), *PC = cp.A(NS_PSI_C), *RHSPS = cp.A(NS_RHSPS);
+    double *TA = cp.A(NS_TA), *TB = cp.A(NS_TB), *TC = cp.A(NS_TC), *TG = cp.A(NS_TG);
+    const double *ZP = cp.A(NS_ZP), *Z = cp.A(NS_Z), *SSDZ = cp.A(NS_SSDZ), *VVDZ = cp.A(NS_VVDZ);
+    const double *AKUV = cp.A(NS_AKUV), *CKUV = cp.A(NS_CKUV);
+    const double *ZF = cp.R(RF_ZF), *PSIF = cp.R(RF_PSIF);
+    const int *KCOU1 = cp.AI(NSI_KCOU1), *KCOU2 = cp.AI(NSI_KCOU2);
+    const int *KCOUMP1 = cp.AI(NSI_KCOUMP1), *KCOUMP2 = cp.AI(NSI_KCOUMP2);
+
+    SERIAL {
+        S.msub = msub;
+        S.alpha = 0.5 * (P.gamma[msub - 1] + P.ro[msub - 1]);
+        S.acoef = S.alpha * S.dt;
+        S.dtconst = 2.0 * S.alpha * S.dt;
+        S.dtconsti = 1.0 / S.dtconst;
+    }
+    SERIAL_END
+    const double dt = S.dt, alpha = S.alpha, dtconst = S.dtconst;
+    const long long m = S.m;
+
+    // PSIFACE_Z / PSIFACE_C (1d.f:2349-2549) and the face densities
+    auto psiface_z = [&](int k, double uu) {
+        double z1, z2; int k1, k2;
+        if (uu >= 0.0) { z1 = ZP[k] - uu * dtconst; z2 = ZP[k]; k1 = KCOUMP2[k - 1]; k2 = KCOUMP1[k]; }
+        else { z1 = ZP[k]; z2 = ZP[k] - uu * dtconst; k1 = KCOUMP2[k]; k2 = KCOUMP1[k + 1]; }
+        return box_avg(ZF, P.ssdzf, [&](int kk) { return PSIF[kk]; }, z1, z2, k1, k2);
+    };
+    auto psiface_c = [&](int k, double uu) {
+        double z1, z2; int k1, k2;
+        if (uu >= 0.0) { z1 = Z[k + 1] - uu * dtconst; z2 = Z[k + 1]; k1 = KCOU2[k]; k2 = KCOU1[k + 1]; }
+        else { z1 = Z[k + 1]; z2 = Z[k + 1] - uu * dtconst; k1 = KCOU2[k + 1]; k2 = KCOU1[k + 2]; }
+        return box_avg(ZF, P.ssdzf, [&](int kk) { return PSIF[kk]; }, z1, z2, k1, k2);
+    };
+    // face value computed by a region-2 iteration for the z face at ZP(f)
+    auto zface2 = [&](int f, double *vf) {
+        double wt = 0.5 * (W[f + 1] + W[f]);
+        double ps;
+        if (fabs(wt) <= 1.0e-8) ps = 0.5 * (PZN[f] + PZN[f + 1]);
+        else ps = psiface_z(f, wt);
+        *vf = P.vism + (P.visp - P.vism) * ps;
+        return P.denm + P.den_diff * ps;
+    };
+    auto zown = [&](int f, int rg, double *vf) {
+        if (rg == 0) { *vf = P.vism; return P.denm; }
+        if (rg == 1) { *vf = P.visp; return P.denp; }
+        return zface2(f, vf);
+    };
+    auto cface2 = [&](int f) {
+        double wt = W[f + 1];
+        double ps;
+        if (fabs(wt) <= 1.0e-8) ps = PCN[f + 1];
+        else ps = psiface_c(f, wt);
+        return P.denm + P.den_diff * ps;
+    };
+    auto cown = [&](int f, int rg) {
+        if (rg == 0) return P.denm;
+        if (rg == 1) return P.denp;
+        return cface2(f);
+    };
+
+    // CONTI_Z / CONTI_C in closed form: the serial loops leave on each face
+    // the value written last, i.e. the next cell's constant when that cell
+    // is region 0/1 and otherwise the value the face's own cell computed.
+    PFOR(k, 1, n + 1) {
+        const int rk = region_of(PCN, n, k);
+        const int rn = (k < n) ? region_of(PCN, n, k + 1) : -1;
+        double vf, df;
+        if (k == n) df = zown(k, rk, &vf);
+        else if (rn == 0) { df = P.denm; vf = P.vism; }
+        else if (rn == 1) { df = P.denp; vf = P.visp; }
+        else if (k >= 2) df = zown(k, rk, &vf);
+        else df = zface2(1, &vf);
+        DFZ[k] = df;
+        VFZ[k] = vf;
+        if (k >= 2) {
+            double pz;
+            if (rk != 2) pz = PZN[k];
+            else {
+                double vv, own = zface2(k, &vv);
+                double prev = (k - 1 >= 2) ? zown(k - 1, region_of(PCN, n, k - 1), &vv) : zface2(1, &vv);
+                double wt = 0.5 * (W[k + 1] + W[k]);
+                double wb = 0.5 * (W[k] + W[k - 1]);
+                double flux = (own * wt - prev * wb) * VVDZ[k];
+                double den_zn = P.denm + P.den_diff * PZN[k];
+                double den_z = den_zn - dtconst * flux;
+                pz = (den_z - P.denm) / P.den_diff;
+            }
+            if (pz > 1.0) pz = 1.0;
+            else if (pz < 0.0) pz = 0.0;
+            PZ[k] = pz;
+        }
+        // CONTI_C
+        double dc;
+        if (k == n) dc = cown(k, rk);
+        else if (rn == 0) dc = P.denm;
+        else if (rn == 1) dc = P.denp;
+        else dc = cown(k, rk);
+        DFC[k] = dc;
+        if (k == 1) {
+            double d0;
+            if (rk == 0) d0 = P.denm;
+            else if (rk == 1) d0 = P.denp;
+            else d0 = cface2(0);
+            DFC[0] = d0;
+        }
+        double pc;
+        if (rk != 2) pc = PCN[k];
+        else {
+            double own = cface2(k);
+            double prev = (k >= 2) ? cown(k - 1, region_of(PCN, n, k - 1)) : cface2(0);
+            double flux = (own * W[k + 1] - prev * W[k]) * SSDZ[k];
+            double den_cn = P.denm + P.den_diff * PCN[k];
+            double den_c = den_cn - dtconst * (flux);
+            pc = (den_c - P.denm) / P.den_diff;
+        }
+        if (pc > 1.0) pc = 1.0;
+        else if (pc < 0.0) pc = 0.0;
+        PC[k] = pc;
+    }
+    PFOR_END
+    SERIAL {
+        PZ[1] = PZ[2]; PZ[n3] = PZ[n];
+        PC[1] = PC[2]; PC[n3] = PC[n];
+        DFZ[0] = 0.0; DFZ[n3] = 0.0; DFC[n3] = 0.0;
+    }
+    SERIAL_END
+
+    // energy RHS (CAL_RHSPS/CAL_RHSPS2 + PASSIVE_SCALAR scaling) and LHSPS
+    const double g = P.gamma[msub - 1], ro = P.ro[msub - 1];
+    const double den_diffi = (P.den_diff == 0.0) ? 1.0 : 1.0 / P.den_diff;
+    const double acops = alpha * dt / (P.re * P.prm);
+    PFOR(k, 1, n + 1) {
+        double rhs;
+        if (m != 1) {
+            rhs = (g * T[k] * (W[k + 1] - W[k]) * VVDZ[k]) * dt;
+            double densc, denscf;
+            if (PCN[k] > 0.5) { densc = P.denscm + P.densc_diff; denscf = densc; }
+            else { densc = P.denscm; denscf = densc; }
+            rhs = ((densc - denscf) * T[k] + rhs) / denscf;
+        } else {
+            rhs = ((g * 0.0 + ro * 0.0) + 2.0 * alpha * 0.0 + 2.0 * alpha * (0.0 - 0.0)) * dt;
+            double densc = P.denscm + P.densc_diff * PCN[k];
+            double denscf = P.denscm + P.densc_diff * PC[k];
+            rhs = ((densc - denscf) * T[k] + rhs) / denscf;
+        }
+        double denscf = P.denscm + P.densc_diff * PC[k];
+        double coef = acops / denscf;
+        double pt = (DFC[k - 1] - P.denm) * den_diffi;
+        double tcb = P.tcm + P.tc_diff * pt;
+        pt = (DFC[k] - P.denm) * den_diffi;
+        double tct = P.tcm + P.tc_diff * pt;
+        double ak = coef * tcb * AKUV[k];
+        double ck = coef * tct * CKUV[k];
+        double bk = 1.0 - (ak + ck);
+        double pcn = PCN[k];
+        if (fabs(pcn - 0.5) != 0.5) {
+            if (pcn <= 0.5) ck = 0.0;
+            else if (pcn > 0.5) ak = 0.0;
+        }
+        TA[k] = ak; TB[k] = bk; TC[k] = ck; TG[k] = rhs;
+    }
+    PFOR_END
+    tri_solve(TA, TB, TC, TG, 1, n, bd.tri_mode, sh.tri);
+    PFOR(k, 1, n + 1) {
+        RHSPS[k] = TG[k];
+        T[k] = TG[k] + T[k];
+    }
+    PFOR_END
+    SERIAL {
+        T[0] = 0.0;
+        T[n] = 1.0;
+        T[n3] = T[n];
+    }
+    SERIAL_END
+}
+
+// ---------------------------------------------------------------------
+// Band context of the wave kernels (narrow band + refined window in LDS)
+struct BandSh {
+    int bn[LICA_MLAYER + 2];
+    int bk[LICA_MLAYER + 2][LICA_MAXP];
+    int off[LICA_MLAYER + 2];           // entries of layer n: [off[n], off[n+1])
+    int ent_k[LICA_MAXENT], ent_n[LICA_MAXENT], ent_q[LICA_MAXENT];
+    int wlo, whi, dlo;                  // window [wlo,whi] of z-planes, work base dlo=wlo-4
+    int status;
+    double phw[WMAX], gl[3], gu[4];
+    double phio[WMAX], phrk[WMAX], phs[WMAX];
+    double dphi[WMAX + 8], wf[WMAX + 8];
+    double newv[LICA_MAXENT], pm1[LICA_MAXENT], pm2[LICA_MAXENT], mfd[LICA_MAXENT];
+    double tf0[WMAX], tf1[WMAX];
+    int mask_s[WMAX];
+    double red[16];
+    double v[24];
+    int iv[24];
+    // temperature extrapolation window (NS cells [nlo,nhi])
+    int nlo, nhi;
+    double t0[NSWMAX], t1[NSWMAX], nphi[NSWMAX], psih[NSWMAX];
+    double dtn[NSWMAX], dtn_rk[NSWMAX], tmpo[NSWMAX], tmp_rk[NSWMAX], tmp_new[NSWMAX];
+    double psitmp[NSWMAX];
+    int mcell[2 * LICA_MAXENT];
+};
+
+struct Fill {           // CAL_PSIF_MTP column fill outside the band (two_phase.f:3011-3075)
+    int knum, kc_low;
+    double tmp1, tmp_outer;
+};
+
+DEV double phi_get(const BandSh &B, int k, int n3f)
+{
+    if (k <= 0) return B.gl[k + 2];
+    if (k >= n3f) return B.gu[k - n3f];
+    if (k < B.wlo || k > B.whi) return 0.0;
+    return B.phw[k - B.wlo];
+}
+DEV void phi_set(BandSh &B, int k, int n3f, double v)
+{
+    if (k <= 0) { B.gl[k + 2] = v; return; }
+    if (k >= n3f) { B.gu[k - n3f] = v; return; }
+    if (k < B.wlo || k > B.whi) { B.status = ST_WINDOW; return; }
+    B.phw[k - B.wlo] = v;
+}

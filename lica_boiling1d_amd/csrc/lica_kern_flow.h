@@ -1,0 +1,378 @@
+// NS-column kernels: momentum + projection (1d.f:1603-1627, 1749-1923,
+// 3006-3169, 3180-3627), CONVRGE1 (3306-3362) and the initial state
+// (MAKEFLD 1d.f:821-946, LVSINIT two_phase.f:116-458, GRID_COUPLING(1)).
+#pragma once
+#include "lica_kern_tgrad.h"
+
+// 1/DENF_Z from PSI_Z (CAL_CONT_BC 1d.f:2986-2994 incl. the end copies)
+DEV double denf_zi(const InstParam &P, const double *PZ, int k)
+{
+    int kk = k < 1 ? 1 : (k > P.n ? P.n : k);
+    double d = P.denm + P.den_diff * PZ[kk];
+    return 1.0 / d;
+}
+
+DEV void body_momentum(const BatchDev &bd, int inst, BlkSh &sh)
+{
+    InstState &S = bd.st[inst];
+    if (S.status) return;
+    CaseP cp = case_ptrs(bd, inst);
+    const InstParam &P = *cp.P;
+    const int n = P.n, n3 = P.n3, msub = bd.msub;
+    double *W = cp.A(NS_W), *PP = cp.A(NS_P), *PZN = cp.A(NS_PSI_ZN), *PCN = cp.A(NS_PSI_CN);
+    double *DFZ = cp.A(NS_DF_Z), *VFZ = cp.A(NS_VF_Z);
+    double *PZ = cp.A(NS_PSI_Z), *PC = cp.A(NS_PSI_C), *RK3ZO = cp.A(NS_RK3ZO);
+    double *TA = cp.A(NS_TA), *TB = cp.A(NS_TB), *TC = cp.A(NS_TC), *TG = cp.A(NS_TG);
+    double *PA = cp.A(NS_PA), *PB = cp.A(NS_PB), *PCc = cp.A(NS_PC), *PG = cp.A(NS_PG);
+    const double *ZP = cp.A(NS_ZP), *Z = cp.A(NS_Z), *SSDZ = cp.A(NS_SSDZ), *VVDZ = cp.A(NS_VVDZ);
+    const double *AKW = cp.A(NS_AKW), *CKW = cp.A(NS_CKW), *ZF = cp.R(RF_ZF);
+    const double *SFGAS = cp.R(RF_SFGAS), *SFGAS2 = cp.R(RF_SFGAS2);
+    const int *KCOU1 = cp.AI(NSI_KCOU1), *KCOU2 = cp.AI(NSI_KCOU2), *IMASK = cp.AI(NSI_IMASK);
+    const long long m = S.m;
+    const double dt = S.dt, alpha = S.alpha, acoef = S.acoef, dtconst = S.dtconst, dtconsti = S.dtconsti;
+    const int sfl = S.sf_lo, sfh = S.sf_hi;
+    auto fsf = [&](int kk) { return SFGAS[kk]; };
+    auto fsf2 = [&](int kk) { return SFGAS2[kk]; };
+    // a refined range [k1,k2] that misses the nonzero SFGAS window averages to 0
+    auto touches = [&](int k1, int k2) { return !(k2 < sfl || k1 > sfh); };
+
+    if (m != 1) {
+        // CAL_CONTINUITY2 (1d.f:3006-3169)
+        PFOR(k, 1, n + 1) {
+            int rg = region_of(PCN, n, k);
+            if (k >= 2) {
+                double pz = PZ[k];
+                if (rg == 2 && touches(KCOU2[k], KCOU1[k + 1]))
+                    pz = pz - box_avg(ZF, P.ssdzf, fsf2, ZP[k - 1], ZP[k], KCOU2[k], KCOU1[k + 1]);
+                else if (rg == 2)
+                    pz = pz - 0.0;
+                if (pz > 1.0) pz = 1.0;
+                else if (pz < 0.0) pz = 0.0;
+                PZ[k] = pz;
+            }
+            double pc = PC[k];
+            if (rg == 2 && touches(KCOU2[k], KCOU1[k + 1]))
+                pc = pc - box_avg(ZF, P.ssdzf, fsf2, Z[k], Z[k + 1], KCOU2[k], KCOU1[k + 1]);
+            if (pc > 1.0) pc = 1.0;
+            else if (pc < 0.0) pc = 0.0;
+            PC[k] = pc;
+        }
+        PFOR_END
+        SERIAL {
+            PZ[1] = PZ[2]; PZ[n3] = PZ[n];
+            PC[1] = PC[2]; PC[n3] = PC[n];
+        }
+        SERIAL_END
+    }
+    // RHSZ (1d.f:1603-1745), RHS1/DENF_Z - W (1557-1560), LHSW (1749-1809)
+    const double g = P.gamma[msub - 1], ro = P.ro[msub - 1], a2 = 2.0 * alpha;
+    PFOR(k, 2, n + 1) {
+        int kp1 = k + 1, km1 = k - 1;
+        double wt = 0.5 * (W[kp1] + W[k]);
+        double wb = 0.5 * (W[k] + W[km1]);
+        double anzz = (DFZ[k] * (wt * wt) - DFZ[km1] * (wb * wb)) * VVDZ[k];
+        double anz = -0.0 - 0.0 - anzz;
+        double alz5 = (W[kp1] - W[k]) * SSDZ[k];
+        double alz6 = (W[k] - W[km1]) * SSDZ[km1];
+        double alzz = (VFZ[k] * alz5 - VFZ[km1] * alz6) * VVDZ[k];
+        double alz_sum1 = 0.0 + 0.0 + alzz + alzz;
+        double alz_sum2 = 0.0 + 0.0;
+        double pgr_z = (PP[k] - PP[km1]) * VVDZ[k];
+        double den_zn = P.denm + P.den_diff * PZN[k];
+        double rk3z = anz + alz_sum2;
+        double cn2z = alz_sum1;
+        double rhs1 = den_zn * W[k] +
+                      (g * rk3z + ro * RK3ZO[k] + a2 * cn2z + a2 * (0.0 - pgr_z) - a2 * 0.0 - a2 * (0.0 + 0.0)) * dt;
+        if (IMASK[k] == 1 && fabs(PZN[k] - 0.5) != 0.5)
+            rhs1 = rhs1 + g * dt * den_zn * W[k] * (0.0 * 1.0 + 0.0 * 1.0 + (wt - wb) * VVDZ[k]);
+        RK3ZO[k] = rk3z;
+        double dzi = denf_zi(P, PZ, k);
+        rhs1 = rhs1 * dzi - W[k];
+        double dk1 = 2.0 * acoef * dzi * VFZ[km1] * AKW[k];
+        double dk3 = 2.0 * acoef * dzi * VFZ[k] * CKW[k];
+        double dk2 = -dk1 - dk3;
+        TA[k] = dk1; TC[k] = dk3; TB[k] = 1.0 + dk2; TG[k] = rhs1;
+    }
+    PFOR_END
+    tri_solve(TA, TB, TC, TG, 2, n, bd.tri_mode, sh.tri);
+    PFOR(k, 2, n + 1) { W[k] = TG[k] + W[k]; }
+    PFOR_END
+    // BC + BC_FLUX (1d.f:1812-1923)
+    SERIAL {
+        W[1] = 0.0;
+        W[n3] = W[n];
+        double qq3 = 0.0 + (W[n3] - W[1]) * P.sdx1 * P.sdy1;
+        double aa = 0.0 + 2.0 * P.xl * P.yl;
+        double ubc = -qq3 / aa;
+        W[1] = 0.0;
+        W[n3] = W[n3] + ubc;
+    }
+    SERIAL_END
+    // FSM_CHOI (1d.f:3365-3400)
+    PFOR(k, 2, n + 1) { W[k] = W[k] + dtconst * ((PP[k] - PP[k - 1]) * VVDZ[k]) * denf_zi(P, PZ, k); }
+    PFOR_END
+    // DIVGS (1d.f:3180-3303) and the POISSON matrix (3529-3543)
+    PFOR(k, 1, n + 1) {
+        double d = (k <= n - 1) ? (W[k + 1] - W[k]) * SSDZ[k] : 0.0;
+        double ss = touches(KCOU2[k], KCOU1[k + 1]) ? box_avg(ZF, P.ssdzf, fsf, Z[k], Z[k + 1], KCOU2[k], KCOU1[k + 1]) : 0.0;
+        d = d - ss * (P.den_diff / (P.denp * P.denm));
+        if (k >= 2) d = d * dtconsti;
+        double fixkl = (k == 1) ? 1.0 : 0.0;
+        double ak = VVDZ[k] * SSDZ[k] * denf_zi(P, PZ, k) * (1.0 - fixkl);
+        double ck = VVDZ[k + 1] * SSDZ[k] * denf_zi(P, PZ, k + 1);
+        PA[k] = ak; PCc[k] = ck; PB[k] = -(ak + ck); PG[k] = d;
+    }
+    PFOR_END
+}
+
+// POISSON solve (1d.f:3544-3576) of the system written by body_momentum
+DEV void body_poisson_direct(const BatchDev &bd, int inst, BlkSh &sh)
+{
+    InstState &S = bd.st[inst];
+    if (S.status) return;
+    CaseP cp = case_ptrs(bd, inst);
+    const InstParam &P = *cp.P;
+    const int n = P.n, n3 = P.n3;
+    double *PP = cp.A(NS_P), *PA = cp.A(NS_PA), *PB = cp.A(NS_PB), *PCc = cp.A(NS_PC), *PG = cp.A(NS_PG);
+    tri_solve(PA, PB, PCc, PG, 1, n, bd.tri_mode, sh.tri);
+    PFOR(k, 1, n + 1) { PP[k] = PG[k]; }
+    PFOR_END
+    SERIAL {
+        PP[0] = PP[1];
+        PP[n3] = PP[n];
+        S.poisson_solves += 1.0;
+    }
+    SERIAL_END
+}
+
+// UCALC (1d.f:3582-3627), p -= p(1) (3433-3470), CONVRGE1 (3306-3362), the
+// CFL input of the next DETERMINE_DT, and VEL_INTP's UT/UO (two_phase.f:5570-5616)
+DEV void body_ucalc(const BatchDev &bd, int inst, BlkSh &sh)
+{
+    InstState &S = bd.st[inst];
+    if (S.status) return;
+    CaseP cp = case_ptrs(bd, inst);
+    const InstParam &P = *cp.P;
+    const int n = P.n, n3 = P.n3, msub = bd.msub;
+    double *W = cp.A(NS_W), *PP = cp.A(NS_P), *PZ = cp.A(NS_PSI_Z), *WO = cp.A(NS_WO), *WT = cp.A(NS_WT);
+    const double *T = cp.A(NS_T);
+    const double *Z = cp.A(NS_Z), *SSDZ = cp.A(NS_SSDZ), *VVDZ = cp.A(NS_VVDZ), *ZF = cp.R(RF_ZF);
+    const double *SFGAS = cp.R(RF_SFGAS);
+    const int *KCOU1 = cp.AI(NSI_KCOU1), *KCOU2 = cp.AI(NSI_KCOU2);
+    const double dtconst = S.dtconst;
+    const int sfl = S.sf_lo, sfh = S.sf_hi;
+    const long long m = S.m;
+    PFOR(k, 2, n + 1) { W[k] = W[k] - dtconst * (PP[k] - PP[k - 1]) * VVDZ[k] * denf_zi(P, PZ, k); }
+    PFOR_END
+    SERIAL {
+        W[1] = 0.0;
+        W[n3] = W[n];
+        sh.v[0] = PP[1];
+    }
+    SERIAL_END
+    const double phiref = sh.v[0];
+    PFOR(k, 1, n + 1) { PP[k] = PP[k] - phiref; }
+    PFOR_END
+    SERIAL { PP[0] = PP[1]; }
+    SERIAL_END
+    auto fsf = [&](int kk) { return SFGAS[kk]; };
+    double dvmax = red_max(1, n + 1, 0.0, [&](int k) {
+        double d = (W[k + 1] - W[k]) * SSDZ[k];
+        bool t = !(KCOU1[k + 1] < sfl || KCOU2[k] > sfh);
+        double ss = t ? box_avg(ZF, P.ssdzf, fsf, Z[k], Z[k + 1], KCOU2[k], KCOU1[k + 1]) : 0.0;
+        d = d - ss * (P.den_diff / P.denp);
+        return fabs(d);
+    }, sh.red);
+    double c3max = red_max(1, n + 1, 0.0, [&](int k) { return fabs(W[k] + W[k + 1]) * SSDZ[k]; }, sh.red);
+    int bad = 0;
+    if (msub == 3)
+        bad = red_or(0, n3 + 1, [&](int k) { return !(isfinite(W[k]) && isfinite(T[k]) && isfinite(PP[k])); }, sh.red);
+    SERIAL {
+        S.dvmax = dvmax;
+        S.c3max = c3max;
+        if (bad) S.status = ST_NONFINITE;
+    }
+    SERIAL_END
+    const int first = (m == 1 && msub == 1);
+    PFOR(k, 0, n3 + 1) {
+        double wo = first ? W[k] : WO[k];
+        WT[k] = 0.5 * (W[k] + wo);
+        WO[k] = W[k];
+    }
+    PFOR_END
+}
+
+// ---------------------------------------------------------------------
+// initial state: MAKEFLD, LVSINIT (ITRACKING=2), GRID_COUPLING(1)
+struct InitSh {
+    BlkSh blk;
+    int mask_s[WMAX];
+    int bn[LICA_MLAYER + 1];
+    int bk[LICA_MLAYER + 1][LICA_MAXP];
+    int iv[16];
+    double v[16];
+    int inband[WMAX];
+    double pm[LICA_MAXENT];
+    int ek[LICA_MAXENT];
+};
+
+DEV double phi0(const InstParam &P, const double *ZPF, int k) { return (ZPF[k] - P.z0) + 1.0e-12; }
+
+DEV void body_init(const BatchDev &bd, int inst, InitSh &is)
+{
+    InstState &S = bd.st[inst];
+    CaseP cp = case_ptrs(bd, inst);
+    const InstParam &P = *cp.P;
+    const int n = P.n, n3 = P.n3, n3f = P.n3f, n3fm = P.n3fm;
+    double *W = cp.A(NS_W), *T = cp.A(NS_T), *PZN = cp.A(NS_PSI_ZN), *PCN = cp.A(NS_PSI_CN);
+    double *PSIF = cp.R(RF_PSIF);
+    const double *ZP = cp.A(NS_ZP), *Z = cp.A(NS_Z), *SSDZ = cp.A(NS_SSDZ), *ZF = cp.R(RF_ZF), *ZPF = cp.R(RF_ZPF);
+    const int *KCOU1 = cp.AI(NSI_KCOU1), *KCOU2 = cp.AI(NSI_KCOU2);
+    const int *KCOUMP1 = cp.AI(NSI_KCOUMP1), *KCOUMP2 = cp.AI(NSI_KCOUMP2);
+    // MAKEFLD (1d.f:879-917)
+    PFOR(k, 1, n + 1) {
+        double pbody = (ZP[k] - P.z0) + 1.0e-12;
+        if (pbody > 0.0) { T[k] = 1.0; W[k] = 1.0; }
+        else { T[k] = P.tslope * ZP[k]; W[k] = 0.0; }
+    }
+    PFOR_END
+    int any_liq = red_or(1, n + 1, [&](int k) { return ((ZP[k] - P.z0) + 1.0e-12) > 0.0; }, is.blk.red);
+    int any_vap = red_or(1, n + 1, [&](int k) { return !(((ZP[k] - P.z0) + 1.0e-12) > 0.0); }, is.blk.red);
+    SERIAL {
+        if (any_liq) W[n3] = 1.0;
+        if (any_vap) { T[0] = 0.0; W[1] = 0.0; }
+        S.dt = P.dt0; S.time = 0.0; S.m = 0; S.status = 0;
+        S.n_bandgen = 1.0;   // the seed BAND_GENERATION of LVSINIT
+        S.sf_lo = 1; S.sf_hi = 0; S.im_lo = 1; S.im_hi = 0;
+        // LVSINIT seed (two_phase.f:209-252): a contiguous range of planes
+        double cri = 1.2 * fmax(fmax(P.sdxf, P.sdyf), P.sdzf);
+        int slo = -1, shi = -2;
+        for (int k = 1; k <= n3fm; k++) {
+            if (fabs(phi0(P, ZPF, k)) <= cri) { if (slo < 0) slo = k; shi = k; }
+        }
+        long long nseed = (slo < 0) ? 0 : (shi - slo + 1);
+        long long rr = (long long)P.r * P.r, total = nseed * rr, crin = total / (LICA_MLAYER - 1);
+        long long keep = (crin * (LICA_MLAYER - 1) + rr - 1) / rr;
+        if (keep > nseed) keep = nseed;
+        int scan_hi = slo + (int)keep - 1;
+        auto seedphi = [&](int k) { return (slo >= 0 && k >= slo && k <= shi) ? phi0(P, ZPF, k) : 0.0; };
+        // S-tube over the scanned seed planes (two_phase.f:1667-1694)
+        for (int nl = 0; nl <= LICA_MLAYER; nl++) is.bn[nl] = 0;
+        int st = 0;
+        for (int k = slo; slo >= 0 && k <= scan_hi; k++) {
+            double a = seedphi(k);
+            if (a * seedphi(k + 1) < 0.0 || a * seedphi(k - 1) < 0.0) {
+                if (is.bn[1] >= LICA_MAXP) { st = ST_BAND_CAPACITY; break; }
+                is.bk[1][is.bn[1]++] = k;
+            }
+        }
+        int klo = 1 << 30, khi = -1;
+        for (int q = 0; q < is.bn[1]; q++) { if (is.bk[1][q] < klo) klo = is.bk[1][q]; if (is.bk[1][q] > khi) khi = is.bk[1][q]; }
+        if (khi < 0) { klo = 1; khi = 1; }
+        int wlo = klo - 14 < -4 ? -4 : klo - 14;
+        is.iv[0] = wlo;
+        for (int i = 0; i < WMAX; i++) { is.mask_s[i] = 0; is.inband[i] = 0; }
+        auto msk = [&](int k) -> int & { return is.mask_s[k - wlo]; };
+        for (int k = (slo > wlo ? slo : wlo); slo >= 0 && k <= scan_hi && k < wlo + WMAX; k++) msk(k) = 2;
+        for (int q = 0; q < is.bn[1]; q++) msk(is.bk[1][q]) = 1;
+        // layers 2..N_MAX (two_phase.f:1700-1960), planes only
+        for (int nl = 2; nl <= LICA_MLAYER && !st; nl++)
+            for (int q = 0; q < is.bn[nl - 1] && !st; q++) {
+                int k = is.bk[nl - 1][q];
+                for (int k2 = k - 1; k2 <= k + 1; k2++) {
+                    if (k2 == 1 || k2 == n3f) continue;
+                    if (k2 < wlo || k2 >= wlo + WMAX) { st = ST_WINDOW; break; }
+                    if (msk(k2) != 1) {
+                        if (is.bn[nl] >= LICA_MAXP) { st = ST_BAND_CAPACITY; break; }
+                        is.bk[nl][is.bn[nl]++] = k2;
+                        msk(k2) = 1;
+                    }
+                }
+            }
+        for (int nl = 1; nl <= LICA_MLAYER; nl++)
+            if ((long long)is.bn[nl] * P.r * P.r > P.mf_band) st = ST_BAND_OVERFLOW;
+        int e = 0;
+        for (int nl = 1; nl <= LICA_MLAYER; nl++) {
+            S.bn[nl] = is.bn[nl];
+            for (int q = 0; q < is.bn[nl]; q++) {
+                int k = is.bk[nl][q];
+                S.bk[nl][q] = k;
+                S.alphi[nl][q] = phi0(P, ZPF, k);
+                is.inband[k - wlo] = 1;
+                is.ek[e++] = k;
+            }
+        }
+        is.iv[1] = e;
+        is.iv[2] = slo; is.iv[3] = shi;
+        S.status = st;
+    }
+    SERIAL_END
+    if (S.status) return;
+    const int wlo = is.iv[0], ne = is.iv[1], slo = is.iv[2], shi = is.iv[3];
+    auto phinit = [&](int k) {
+        bool seed = (slo >= 0 && k >= slo && k <= shi);
+        bool band = (k >= wlo && k < wlo + WMAX) ? (is.inband[k - wlo] != 0) : false;
+        return (seed || band) ? phi0(P, ZPF, k) : 0.0;
+    };
+    // CAL_PSIF(1,N_MAX): band formula on all layers, column fill elsewhere
+    PFOR(e, 0, ne) {
+        int k = is.ek[e];
+        double d3 = 0.5 * fabs(phinit(k + 1) - phinit(k - 1));
+        double v = d_psif_cell(phinit(k), d3);
+        is.pm[e] = snap5(v);
+        if (v == -1.0) S.status = ST_PSIF_BUG;
+    }
+    PFOR_END
+    SERIAL {
+        int knum = 0, khigh = 0, klow = 0;
+        int kt = shi < n3fm ? shi : n3fm;
+        int kl = slo < 1 ? 1 : slo;
+        for (int k = kt; slo >= 0 && k >= kl; k--)
+            if (phinit(k) * phinit(k + 1) < 0.0) { if (knum == 0) khigh = k; knum++; klow = k; }
+        // band planes outside the seed range
+        for (int e = 0; e < ne; e++) {
+            int k = is.ek[e];
+            if (k > kt || k < kl)
+                if (phinit(k) * phinit(k + 1) < 0.0) { knum++; if (k > khigh) khigh = k; if (k < klow || klow == 0) klow = k; }
+        }
+        is.iv[4] = knum; is.iv[5] = klow;
+        is.v[0] = (knum > 0) ? ((phinit(khigh) < 0.0) ? 0.0 : 1.0) : 1.0;
+        is.v[1] = (knum > 0) ? ((phinit(klow + 1) < 0.0) ? 0.0 : 1.0) : 1.0;
+    }
+    SERIAL_END
+    const int knum = is.iv[4], klow = is.iv[5];
+    const double tmp1 = is.v[0], tmpo = is.v[1];
+    PFOR(k, 1, n3fm + 1) {
+        double v = (knum == 0) ? 1.0 : ((k <= klow) ? tmp1 : tmpo);
+        PSIF[k] = v;
+    }
+    PFOR_END
+    PFOR(e, 0, ne) { PSIF[is.ek[e]] = is.pm[e]; }
+    PFOR_END
+    SERIAL {
+        for (int k = 1; k <= n3fm; k++) PSIF[k] = snap5(PSIF[k]);
+        for (int k = 0; k <= P.r; k++) { PSIF[-k] = PSIF[1]; PSIF[n3f + k] = PSIF[n3fm]; }
+        // VOL_TOT_ORI (two_phase.f:423-433)
+        double vol = 0.0, dvol = P.sdzf;
+        for (int k = 1; k <= n3fm; k++) vol = vol + (1.0 - PSIF[k]) * dvol;
+        S.vol_tot_ori = vol;
+        S.vol_tot_iphs = vol;
+    }
+    SERIAL_END
+    // GRID_COUPLING(1)
+    auto fpsif = [&](int kk) { return PSIF[kk]; };
+    PFOR(k, 1, n + 1) {
+        if (k >= 2) PZN[k] = snap3(box_avg(ZF, P.ssdzf, fpsif, ZP[k - 1], ZP[k], KCOUMP2[k - 1], KCOUMP1[k]));
+        else PZN[k] = snap3(PZN[k]);
+        PCN[k] = snap3(box_avg(ZF, P.ssdzf, fpsif, Z[k], Z[k + 1], KCOU2[k], KCOU1[k + 1]));
+    }
+    PFOR_END
+    SERIAL {
+        PCN[0] = PCN[1];
+        PCN[n3] = PCN[n];
+    }
+    SERIAL_END
+    double c3max = red_max(1, n + 1, 0.0, [&](int k) { return fabs(W[k] + W[k + 1]) * SSDZ[k]; }, is.blk.red);
+    SERIAL { S.c3max = c3max; }
+    SERIAL_END
+}

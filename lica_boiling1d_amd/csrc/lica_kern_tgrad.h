@@ -1,0 +1,405 @@
+// TEMP_GRAD (chphase.f:55-368) on one wave per case: phase marking of the
+// NS cells under the band, the pseudo-time temperature extrapolation
+// TEMP_EXTP (467-803) on the NS window under the band, TF_GRAD (959-1159)
+// and VOL_TOT_MFDOT (1328-1493).
+#pragma once
+#include "lica_kern_band.h"
+
+struct TgradSh {
+    BandSh B;
+    int nbk[LICA_MLAYER + 1][LICA_MAXP];
+    int nbn[LICA_MLAYER + 1];
+    double pmw1[WMAX];
+    int tmk1[WMAX];
+    double pmw2[WMAX];
+    int tmk2[WMAX];
+};
+
+// TEMP_EXTP_RK3 (chphase.f:806-956) on the NS window, in place on arr
+DEV void temp_extp_rk3(BandSh &B, const InstParam &P, const double *VVDZ, double *arr, int im, double redtl,
+                       int nd, int ia, int ib)
+{
+    const int nlo = B.nlo;
+    const double sm_grid = fmin(fmin(2.0 * P.sdxf, 2.0 * P.sdyf), 2.0 * P.sdzf);
+    PFOR(k, ia, ib + 1) {
+        int i = k - nlo;
+        double np = B.nphi[i];
+        double tn = arr[i];
+        if (np != 0.0) {
+            double dpx = 0.5 * (np - np) * (1.0 / P.sdx1);
+            double dpz;
+            if (B.nphi[i - 1] == 0.0) dpz = (B.nphi[i + 1] - B.nphi[i]) * VVDZ[k + 1];
+            else if (B.nphi[i + 1] == 0.0) dpz = (B.nphi[i] - B.nphi[i - 1]) * VVDZ[k];
+            else dpz = 0.5 * (B.nphi[i + 1] - B.nphi[i - 1]) * VVDZ[k];
+            double dalphi = dpx * dpx + dpx * dpx + dpz * dpz;
+            if (dalphi != 0.0) {
+                double nx = 0.0, ny = 0.0, nz = 1.0;
+                double apxm = (arr[i] - arr[i]) * (1.0 / P.sdx1);
+                double apxp = apxm;
+                double apzm = (arr[i] - arr[i - 1]) * VVDZ[k];
+                double apzp = (arr[i + 1] - arr[i]) * VVDZ[k];
+                if (im == 1) { nx = (-1.0) * nx; ny = (-1.0) * ny; nz = (-1.0) * nz; }
+                double ax = fmin(nx, 0.0) * apxp + fmax(nx, 0.0) * apxm;
+                double ay = fmin(ny, 0.0) * apxp + fmax(ny, 0.0) * apxm;
+                double az = fmin(nz, 0.0) * apzp + fmax(nz, 0.0) * apzm;
+                double drhs;
+                if (im == 1) {
+                    drhs = (ax + ay + az) + (double)nd * B.dtn[i];
+                    tn = arr[i] - redtl * drhs * (1.0 - B.psih[i]);
+                } else {
+                    drhs = (ax + ay + az) - (double)nd * B.dtn[i];
+                    tn = arr[i] - redtl * drhs * B.psih[i];
+                }
+                if (nd == 0 && fabs(np) <= sm_grid) {
+                    drhs = (ax + ay + az);
+                    tn = arr[i] - redtl * drhs;
+                }
+            }
+        }
+        B.tmp_new[i] = tn;
+    }
+    PFOR_END
+    PFOR(k, ia, ib + 1) { arr[k - nlo] = B.tmp_new[k - nlo]; }
+    PFOR_END
+}
+
+// PHINSINTP (chphase.f:1258-1325) on the flat column; call inside SERIAL
+DEV double phinsintp(const BandSh &B, const CaseP &cp, int ko, int *ic)
+{
+    const InstParam &P = *cp.P;
+    const int *KCOU2 = cp.AI(NSI_KCOU2), *KCOUMP2 = cp.AI(NSI_KCOUMP2);
+    const double *ZP = cp.A(NS_ZP), *ZPF = cp.R(RF_ZPF);
+    int k1 = KCOU2[ko], k2 = KCOUMP2[ko];
+    double xd = P.phins_xd, yd = P.phins_yd, zd;
+    zd = (k1 == k2) ? 0.0 : (ZP[ko] - ZPF[k1]) / (ZPF[k2] - ZPF[k1]);
+    double a = phi_get(B, k1, P.n3f), b = phi_get(B, k2, P.n3f);
+    double c00 = a * (1.0 - xd) + a * xd, c10 = a * (1.0 - xd) + a * xd;
+    double c01 = b * (1.0 - xd) + b * xd, c11 = b * (1.0 - xd) + b * xd;
+    double c0 = c00 * (1.0 - yd) + c10 * yd, c1 = c01 * (1.0 - yd) + c11 * yd;
+    double v = c0 * (1.0 - zd) + c1 * zd;
+    if (v > 0.0) *ic = 1;
+    if (v < 0.0) *ic = 0;
+    if (a == 0.0 || b == 0.0) v = 0.0;
+    return v;
+}
+
+// TEMPASSIGN (chphase.f:372-463): x/y weights are zero on the flat column
+DEV double tempassign(const BandSh &B, const CaseP &cp, int k, const double *tw)
+{
+    const double *ZP = cp.A(NS_ZP), *ZPF = cp.R(RF_ZPF), *VDZ = cp.A(NS_VDZ);
+    const int *KCMV = cp.RI(RFI_KCOUMP_VEL);
+    int kk = KCMV[k], kk1, kk2;
+    double zd;
+    if (ZP[kk] > ZPF[k]) {
+        kk1 = kk - 1; kk2 = kk;
+        zd = (kk1 == kk2) ? 0.0 : (ZPF[k] - (ZP[kk] - VDZ[kk])) / (ZP[kk2] - ZP[kk1]);
+    } else {
+        kk1 = kk; kk2 = kk + 1;
+        zd = (kk1 == kk2) ? 0.0 : (ZPF[k] - ZP[kk1]) / (ZP[kk2] - ZP[kk1]);
+    }
+    double xd = 0.0, yd = 0.0;
+    double t_kk = tw[kk - B.nlo], t_k2 = tw[kk2 - B.nlo];
+    double c00 = t_kk * (1.0 - xd) + t_kk * xd;
+    double c10 = t_kk * (1.0 - xd) + t_kk * xd;
+    double c01 = t_k2 * (1.0 - xd) + t_k2 * xd;
+    double c11 = t_k2 * (1.0 - xd) + t_k2 * xd;
+    double c0 = c00 * (1.0 - yd) + c10 * yd;
+    double c1 = c01 * (1.0 - yd) + c11 * yd;
+    return c0 * (1.0 - zd) + c1 * zd;
+}
+
+DEV double tf_get(const BandSh &B, const double *tfw, int k)
+{
+    if (k < B.wlo || k > B.whi) return 0.0;
+    return tfw[k - B.wlo];
+}
+
+DEV void body_temp_grad(const BatchDev &bd, int inst, TgradSh &TS)
+{
+    InstState &S = bd.st[inst];
+    if (S.status || S.m == 1) return;   // TEMP_GRAD runs from the second step (1d.f:3654-3667)
+    BandSh &B = TS.B;
+    CaseP cp = case_ptrs(bd, inst);
+    const InstParam &P = *cp.P;
+    const int n = P.n, n3 = P.n3, n3f = P.n3f;
+    const double *T = cp.A(NS_T), *PCN = cp.A(NS_PSI_CN), *VVDZ = cp.A(NS_VVDZ);
+    const double *ZP = cp.A(NS_ZP), *Z = cp.A(NS_Z), *ZPF = cp.R(RF_ZPF), *ZF = cp.R(RF_ZF);
+    const int *KCMV = cp.RI(RFI_KCOUMP_VEL);
+    const int *KCOU1 = cp.AI(NSI_KCOU1), *KCOU2 = cp.AI(NSI_KCOU2);
+    int *IMASK = cp.AI(NSI_IMASK);
+    double *SFGAS = cp.R(RF_SFGAS), *SFGAS2 = cp.R(RF_SFGAS2);
+    const double *VPREF = cp.R(RF_VPREFIX);
+
+    SERIAL {
+        B.status = 0;
+        for (int nl = 0; nl <= LICA_MLAYER + 1; nl++) B.bn[nl] = 0;
+        for (int nl = 1; nl <= LICA_MLAYER; nl++) {
+            B.bn[nl] = S.bn[nl];
+            for (int q = 0; q < S.bn[nl]; q++) B.bk[nl][q] = S.bk[nl][q];
+        }
+        band_index(B, P, 1);
+        for (int i = 0; i < 3; i++) B.gl[i] = 0.0;
+        for (int i = 0; i < 4; i++) B.gu[i] = 0.0;
+        // NS window under the band
+        int kmin = 1 << 30, kmax = -1;
+        for (int e = 0; e < B.off[LICA_MLAYER + 1]; e++) {
+            int kk = KCMV[B.ent_k[e]];
+            if (kk < kmin) kmin = kk;
+            if (kk > kmax) kmax = kk;
+        }
+        if (kmax < 0) { kmin = 1; kmax = 1; }
+        B.nlo = kmin - 3 < 0 ? 0 : kmin - 3;
+        B.nhi = kmax + 3 > n3 ? n3 : kmax + 3;
+        if (B.nhi - B.nlo + 1 > NSWMAX) B.status = ST_WINDOW;
+        for (int nl = 0; nl <= LICA_MLAYER; nl++)
+            for (int q = 0; q < LICA_MAXP; q++) S.mflvs[nl][q] = 0.0;
+    }
+    SERIAL_END
+    if (B.status) { SERIAL { S.status = B.status; } SERIAL_END return; }
+    const int nlo = B.nlo, nhi = B.nhi;
+    PFOR(i, 0, WMAX) { B.phw[i] = 0.0; B.tf0[i] = 0.0; B.tf1[i] = 0.0; }
+    PFOR_END
+    PFOR(e, 0, B.off[LICA_MLAYER + 1]) { B.phw[B.ent_k[e] - B.wlo] = S.alphi[B.ent_n[e]][B.ent_q[e]]; }
+    PFOR_END
+    PFOR(k, nlo, nhi + 1) {
+        int i = k - nlo;
+        B.t0[i] = T[k]; B.t1[i] = T[k]; B.nphi[i] = 0.0; B.psih[i] = 0.0;
+    }
+    PFOR_END
+    // (1) phase of the NS cells under the band (chphase.f:174-211)
+    SERIAL {
+        int ic = 0;
+        for (int e = B.off[1]; e < B.off[LICA_NN2 + 1]; e++) {
+            int kk = KCMV[B.ent_k[e]];
+            int i = kk - nlo;
+            if (B.nphi[i] == 0.0) {
+                double v = phinsintp(B, cp, kk, &ic);
+                B.nphi[i] = v;
+                if (fabs(PCN[kk] - 0.5) != 0.5) B.psih[i] = (double)ic;
+                else B.psih[i] = PCN[kk];
+            }
+        }
+        for (int e = B.off[LICA_MLAYER]; e < B.off[LICA_MLAYER + 1]; e++) {
+            int k = B.ent_k[e];
+            int kk = KCMV[k];
+            int i = kk - nlo;
+            double dz = ZPF[k] - ZP[kk];
+            double d2 = P.dxy2_min + dz * dz;
+            double pre = phi_get(B, k, n3f) + sqrt(d2);
+            B.nphi[i] = fmin(B.nphi[i], pre);
+            B.psih[i] = PCN[kk];
+        }
+    }
+    SERIAL_END
+    // (2) TEMP_EXTP
+    const double sm_grid = fmin(fmin(2.0 * P.sdxf, 2.0 * P.sdyf), 2.0 * P.sdzf);
+    const double eps = 1.0e-4;
+    const int ia = nlo < 1 ? 1 : nlo, ib = nhi > n ? n : nhi;
+    for (int im = 0; im <= 1; im++) {
+        double *tw = (im == 0) ? B.t0 : B.t1;
+        PFOR(k, nlo, nhi + 1) { B.dtn[k - nlo] = 0.0; B.dtn_rk[k - nlo] = 0.0; }
+        PFOR_END
+        PFOR(k, ia, ib + 1) {
+            int i = k - nlo;
+            double np = B.nphi[i], pc = PCN[k];
+            double v = 0.0;
+            if (((fabs(np) > 1.5 * sm_grid) && (np != 0.0)) || ((fabs(pc - 0.5) == 0.5) && (np != 0.0))) {
+                double dpx = 0.5 * (np - np) * (1.0 / P.sdx1);
+                double dtdx = 0.5 * (tw[i] - tw[i]) * (1.0 / P.sdx1);
+                double dpz, dtdz;
+                if (B.nphi[i - 1] == 0.0) { dpz = (B.nphi[i + 1] - B.nphi[i]) * VVDZ[k + 1]; dtdz = (tw[i + 1] - tw[i]) * VVDZ[k + 1]; }
+                else if (B.nphi[i + 1] == 0.0) { dpz = (B.nphi[i] - B.nphi[i - 1]) * VVDZ[k]; dtdz = (tw[i] - tw[i - 1]) * VVDZ[k]; }
+                else { dpz = 0.5 * (B.nphi[i + 1] - B.nphi[i - 1]) * VVDZ[k]; dtdz = 0.5 * (tw[i + 1] - tw[i - 1]) * VVDZ[k]; }
+                double dalphi = dpx * dpx + dpx * dpx + dpz * dpz;
+                if (dalphi != 0.0) {
+                    double nx = 0.0, ny = 0.0, nz = 1.0;
+                    if (im == 1) v = nx * dtdx + ny * dtdx + nz * dtdz * B.psih[i];
+                    else v = nx * dtdx + ny * dtdx + nz * dtdz * (1.0 - B.psih[i]);
+                }
+            } else if (((fabs(np) <= 1.5 * sm_grid) || (fabs(pc - 0.5) != 0.5)) && ((np != 1.0e-7) && (np != 0.0))) {
+                if (im == 1) v = (tw[i] - P.tsat) / np * B.psih[i];
+                else v = (tw[i] - P.tsat) / np * (1.0 - B.psih[i]);
+            }
+            B.dtn[i] = v;
+        }
+        PFOR_END
+        // extrapolation of dT/dn (ND=0), halving the pseudo step on failure
+        double redtl = 0.2 * sm_grid;
+        int halvings = 0;
+        for (;;) {
+            int conv = 0;
+            for (int it = 1; it <= 100; it++) {
+                PFOR(k, ia, ib + 1) { B.dtn_rk[k - nlo] = B.dtn[k - nlo]; }
+                PFOR_END
+                temp_extp_rk3(B, P, VVDZ, B.dtn, im, redtl, 0, ia, ib);
+                temp_extp_rk3(B, P, VVDZ, B.dtn, im, redtl, 0, ia, ib);
+                PFOR(k, ia, ib + 1) { int i = k - nlo; B.dtn[i] = 0.75 * B.dtn_rk[i] + 0.25 * B.dtn[i]; }
+                PFOR_END
+                temp_extp_rk3(B, P, VVDZ, B.dtn, im, redtl, 0, ia, ib);
+                PFOR(k, ia, ib + 1) { int i = k - nlo; B.dtn[i] = 1.0 / 3.0 * B.dtn_rk[i] + 2.0 / 3.0 * B.dtn[i]; }
+                PFOR_END
+                double rat = red_max(ia, ib + 1, 0.0, [&](int k) {
+                    int i = k - nlo;
+                    if (B.dtn_rk[i] != 0.0) return fabs(B.dtn[i] / B.dtn_rk[i] - 1.0);
+                    else if (B.dtn[i] != 0.0) return fabs(B.dtn_rk[i] / B.dtn[i] - 1.0);
+                    return 0.0;
+                }, B.red);
+                SERIAL { S.n_extrap_iters += 1.0; }
+                SERIAL_END
+                if (fabs(rat) < eps) { conv = 1; break; }
+            }
+            if (conv) break;
+            if (++halvings > 20) { SERIAL { S.status = ST_EXTRAP; } SERIAL_END return; }
+            SERIAL { S.n_halvings += 1.0; }
+            SERIAL_END
+            redtl = 0.5 * redtl;
+        }
+        // extrapolation of T (ND=1)
+        redtl = 0.2 * sm_grid;
+        halvings = 0;
+        for (;;) {
+            int conv = 0;
+            for (int it = 1; it <= 100; it++) {
+                PFOR(k, nlo, nhi + 1) { int i = k - nlo; B.tmp_rk[i] = tw[i]; B.tmpo[i] = tw[i]; }
+                PFOR_END
+                temp_extp_rk3(B, P, VVDZ, B.tmpo, im, redtl, 1, ia, ib);
+                temp_extp_rk3(B, P, VVDZ, B.tmpo, im, redtl, 1, ia, ib);
+                PFOR(k, ia, ib + 1) { int i = k - nlo; B.tmpo[i] = 0.75 * B.tmp_rk[i] + 0.25 * B.tmpo[i]; }
+                PFOR_END
+                temp_extp_rk3(B, P, VVDZ, B.tmpo, im, redtl, 1, ia, ib);
+                PFOR(k, ia, ib + 1) {
+                    int i = k - nlo;
+                    B.tmpo[i] = 1.0 / 3.0 * B.tmp_rk[i] + 2.0 / 3.0 * B.tmpo[i];
+                    tw[i] = B.tmpo[i];
+                }
+                PFOR_END
+                double rat = red_max(ia, ib + 1, 0.0, [&](int k) {
+                    int i = k - nlo;
+                    if (B.tmp_rk[i] != 0.0) return fabs(tw[i] / B.tmp_rk[i] - 1.0);
+                    return 0.0;
+                }, B.red);
+                SERIAL { S.n_extrap_iters += 1.0; }
+                SERIAL_END
+                if (fabs(rat) < eps) { conv = 1; break; }
+            }
+            if (conv) break;
+            if (++halvings > 20) { SERIAL { S.status = ST_EXTRAP; } SERIAL_END return; }
+            SERIAL { S.n_halvings += 1.0; }
+            SERIAL_END
+            redtl = 0.5 * redtl;
+        }
+    }
+    // (3) TF on the T-band, (4)-(6) TF_GRAD
+    PFOR(e, B.off[1], B.off[LICA_NN1 + 1]) {
+        int k = B.ent_k[e];
+        B.tf0[k - B.wlo] = tempassign(B, cp, k, B.t0);
+        B.tf1[k - B.wlo] = tempassign(B, cp, k, B.t1);
+    }
+    PFOR_END
+    psif_mtp_band(B, P, LICA_NN1, B.pm1);
+    SERIAL {
+        Fill f = fill_params(B, P);
+        B.v[2] = f.tmp1; B.v[3] = f.tmp_outer; B.iv[2] = f.knum; B.iv[3] = f.kc_low;
+    }
+    SERIAL_END
+    Fill fill1;
+    fill1.tmp1 = B.v[2]; fill1.tmp_outer = B.v[3]; fill1.knum = B.iv[2]; fill1.kc_low = B.iv[3];
+    PFOR(e, B.off[1], B.off[LICA_NN1 - 2 + 1]) {
+        int k = B.ent_k[e];
+        double gx = 0.5 * (0.0 - 0.0) * (1.0 / P.sdxf);
+        double gz = 0.5 * (phi_get(B, k + 1, n3f) - phi_get(B, k - 1, n3f)) * P.ssdzf;
+        double agi = 1.0 / sqrt(gx * gx + gx * gx + gz * gz);
+        double dtx = 0.5 * (0.0 - 0.0) * (1.0 / P.sdxf);
+        double dtz0 = 0.5 * (tf_get(B, B.tf0, k + 1) - tf_get(B, B.tf0, k - 1)) * P.ssdzf;
+        double dtdn0 = (dtx * gx + dtx * gx + dtz0 * gz) * agi;
+        double dtz1 = 0.5 * (tf_get(B, B.tf1, k + 1) - tf_get(B, B.tf1, k - 1)) * P.ssdzf;
+        double dtdn1 = (dtx * gx + dtx * gx + dtz1 * gz) * agi;
+        S.mflvs[B.ent_n[e]][B.ent_q[e]] = P.ja * P.prai * P.rei * (-P.tcp * dtdn1 + P.tcm * dtdn0);
+    }
+    PFOR_END
+    // VOL_TOT_MFDOT: provisional advance, reinitialisation, new volume
+    PFOR(i, 0, WMAX) { B.phs[i] = B.phw[i]; }
+    PFOR_END
+    PFOR(e, B.off[1], B.off[LICA_NN1 - 2 + 1]) {
+        int k = B.ent_k[e];
+        double mf = S.mflvs[B.ent_n[e]][B.ent_q[e]];
+        B.mfd[e] = (phi_get(B, k, n3f) > 0.0) ? mf / P.denr : mf;
+    }
+    PFOR_END
+    const double dtconst = S.dtconst;
+    PFOR(e, B.off[1], B.off[LICA_NN1 - 2 + 1]) {
+        int k = B.ent_k[e];
+        double a = phi_get(B, k, n3f);
+        double v = a - 2.0 * dtconst * B.mfd[e];
+        v = 0.75 * a + 0.25 * v;
+        v = v - dtconst * B.mfd[e];
+        v = 1.0 / 3.0 * a + 2.0 / 3.0 * v;
+        phi_set(B, k, n3f, v);
+    }
+    PFOR_END
+    reinitialization(B, P, S, LICA_NN1, LICA_NN2, LICA_NN2);
+    psif_mtp_band(B, P, LICA_NN1, B.pm2);
+    psif_view(B, LICA_NN1, B.pm2, TS.pmw2, TS.tmk2);
+    SERIAL {
+        double place = 0.0;
+        int kt = B.whi < P.n3fm ? B.whi : P.n3fm;
+        for (int k = (B.wlo < 1 ? 1 : B.wlo); k <= kt; k++)
+            if (phi_get(B, k, n3f) * phi_get(B, k + 1, n3f) < 0.0) place = ZPF[k] + fabs(phi_get(B, k, n3f));
+        S.place = place;
+        Fill f2 = fill_params(B, P);
+        S.vol_tot_iphs = vol_implicit(B, P, VPREF, f2, TS.pmw2, TS.tmk2);
+    }
+    SERIAL_END
+    // SFGAS/SFGAS2: clear the previous window, then write the new one
+    const int sfl = S.sf_lo, sfh = S.sf_hi;
+    PFOR(k, sfl, sfh + 1) { SFGAS[k] = 0.0; SFGAS2[k] = 0.0; }
+    PFOR_END
+    PFOR(e, B.off[1], B.off[LICA_NN1 - 2 + 1]) {
+        int nl = B.ent_n[e], q = B.ent_q[e], k = B.ent_k[e];
+        if (S.mflvs[nl][q] != 0.0) {
+            if (fabs(B.pm1[e] - 0.5) != 0.5) {
+                double d = B.pm2[e] - B.pm1[e];
+                if (S.alphi[nl][q] > 0.0) { SFGAS[k] = -d * P.denr; SFGAS2[k] = -d; }
+                else { SFGAS[k] = -d; SFGAS2[k] = -d; }
+            }
+        }
+    }
+    PFOR_END
+    SERIAL {
+        int lo = 1 << 30, hi = -1;
+        for (int e = B.off[1]; e < B.off[LICA_NN1 - 2 + 1]; e++) {
+            int k = B.ent_k[e];
+            if (k < lo) lo = k;
+            if (k > hi) hi = k;
+        }
+        if (hi < 0) { lo = 1; hi = 0; }
+        S.sf_lo = lo; S.sf_hi = hi;
+    }
+    SERIAL_END
+    PFOR(i, 0, WMAX) { B.phw[i] = B.phs[i]; }
+    PFOR_END
+    // (7) CAL_PSI_TMP(1) and IMASK_PH (chphase.f:300-358)
+    psif_view(B, LICA_NN1, B.pm1, TS.pmw1, TS.tmk1);
+    const int ka = (nlo - 1) < 1 ? 1 : nlo - 1, kb = (nhi + 1) > n ? n : nhi + 1;
+    PFOR(k, ka, kb + 1) {
+        auto fpm = [&](int kk) { return psif_implicit(B, fill1, TS.pmw1, TS.tmk1, kk); };
+        double ps = box_avg(ZF, P.ssdzf, fpm, Z[k], Z[k + 1], KCOU2[k], KCOU1[k + 1]);
+        if (ps <= 1.0e-3) ps = 0.0;
+        else if (ps >= 0.999) ps = 1.0;
+        B.psitmp[k - ka] = ps;
+    }
+    PFOR_END
+    const int iml = S.im_lo, imh = S.im_hi;
+    PFOR(k, iml, imh + 1) { IMASK[k] = 0; }
+    PFOR_END
+    const int ja = (ka + 1) < 2 ? 2 : ka + 1;
+    PFOR(k, ja, kb + 1) {
+        if (fabs(B.psitmp[k - ka] - B.psitmp[k - 1 - ka]) != 0.0) IMASK[k] = 1;
+    }
+    PFOR_END
+    SERIAL {
+        S.im_lo = ja; S.im_hi = kb;
+        if (B.status) S.status = B.status;
+    }
+    SERIAL_END
+}

@@ -1,0 +1,127 @@
+// Kernel entry points and launchers of the per-step path (gfx950).
+#include "lica_kern_flow.h"
+#include "lica_launch.h"
+
+#ifndef LICA_EMUL
+__global__ void __launch_bounds__(1024) k_init(BatchDev bd)
+{
+    __shared__ InitSh sh;
+    body_init(bd, bd.order[blockIdx.x], sh);
+}
+__global__ void __launch_bounds__(1024) k_dt(BatchDev bd)
+{
+    __shared__ BlkSh sh;
+    body_dt(bd, bd.order[blockIdx.x], sh);
+}
+__global__ void __launch_bounds__(1024) k_cont_energy(BatchDev bd)
+{
+    __shared__ BlkSh sh;
+    body_cont_energy(bd, bd.order[blockIdx.x], sh);
+}
+__global__ void __launch_bounds__(64) k_temp_grad(BatchDev bd)
+{
+    __shared__ TgradSh sh;
+    body_temp_grad(bd, bd.order[blockIdx.x], sh);
+}
+__global__ void __launch_bounds__(1024) k_momentum(BatchDev bd)
+{
+    __shared__ BlkSh sh;
+    body_momentum(bd, bd.order[blockIdx.x], sh);
+}
+__global__ void __launch_bounds__(1024) k_poisson_direct(BatchDev bd)
+{
+    __shared__ BlkSh sh;
+    body_poisson_direct(bd, bd.order[blockIdx.x], sh);
+}
+__global__ void __launch_bounds__(1024) k_ucalc(BatchDev bd)
+{
+    __shared__ BlkSh sh;
+    body_ucalc(bd, bd.order[blockIdx.x], sh);
+}
+__global__ void __launch_bounds__(64) k_transport(BatchDev bd)
+{
+    __shared__ TransportSh sh;
+    body_transport(bd, bd.order[blockIdx.x], sh);
+}
+
+// nsys independent tridiagonal systems of n rows, contiguous per system
+__global__ void __launch_bounds__(1024) k_tri_systems(double *a, double *b, double *c, double *g, int n, int mode)
+{
+    __shared__ BlkSh sh;
+    const size_t off = (size_t)blockIdx.x * (size_t)n - 1;   // row k at [off + k], k = 1..n
+    tri_solve(a + off, b + off, c + off, g + off, 1, n, mode, sh.tri);
+}
+
+int lica_launch_tri(int nsys, int n, double *a, double *b, double *c, double *g, int mode, int threads, void *stream)
+{
+    if (nsys <= 0) return 0;
+    hipLaunchKernelGGL(k_tri_systems, dim3(nsys), dim3(threads), 0, (hipStream_t)stream, a, b, c, g, n, mode);
+    return (int)hipGetLastError();
+}
+
+#define LAUNCH(K, threads) \
+    hipLaunchKernelGGL(K, dim3(bd.ninst), dim3(threads), 0, (hipStream_t)stream, bd)
+
+int lica_launch(int which, const BatchDev &bd, int threads, void *stream)
+{
+    if (bd.ninst <= 0) return 0;
+    switch (which) {
+    case KL_INIT: LAUNCH(k_init, threads); break;
+    case KL_DT: LAUNCH(k_dt, threads); break;
+    case KL_CONT_ENERGY: LAUNCH(k_cont_energy, threads); break;
+    case KL_TEMP_GRAD: LAUNCH(k_temp_grad, 64); break;
+    case KL_MOMENTUM: LAUNCH(k_momentum, threads); break;
+    case KL_POISSON_DIRECT: LAUNCH(k_poisson_direct, threads); break;
+    case KL_UCALC: LAUNCH(k_ucalc, threads); break;
+    case KL_TRANSPORT: LAUNCH(k_transport, 64); break;
+    default: return -1;
+    }
+    return (int)hipGetLastError();
+}
+#else
+int lica_launch_tri(int nsys, int n, double *a, double *b, double *c, double *g, int mode, int threads, void *stream)
+{
+    (void)stream;
+    static BlkSh *sh = nullptr;
+    if (!sh) sh = new BlkSh();
+    int saved = lica_emul_nthreads;
+    lica_emul_nthreads = threads;
+    for (int s = 0; s < nsys; s++) {
+        const size_t off = (size_t)s * (size_t)n - 1;
+        tri_solve(a + off, b + off, c + off, g + off, 1, n, mode, sh->tri);
+    }
+    lica_emul_nthreads = saved;
+    return 0;
+}
+
+template <class Sh, class F>
+static void emul_run(const BatchDev &bd, int threads, F body)
+{
+    static Sh *sh = nullptr;
+    if (!sh) sh = new Sh();
+    int saved = lica_emul_nthreads;
+    lica_emul_nthreads = threads;
+    for (int b = 0; b < bd.ninst; b++) {
+        memset((void *)sh, 0, sizeof(Sh));
+        body(bd, bd.order[b], *sh);
+    }
+    lica_emul_nthreads = saved;
+}
+
+int lica_launch(int which, const BatchDev &bd, int threads, void *stream)
+{
+    (void)stream;
+    switch (which) {
+    case KL_INIT: emul_run<InitSh>(bd, threads, body_init); break;
+    case KL_DT: emul_run<BlkSh>(bd, threads, body_dt); break;
+    case KL_CONT_ENERGY: emul_run<BlkSh>(bd, threads, body_cont_energy); break;
+    case KL_TEMP_GRAD: emul_run<TgradSh>(bd, 64, body_temp_grad); break;
+    case KL_MOMENTUM: emul_run<BlkSh>(bd, threads, body_momentum); break;
+    case KL_POISSON_DIRECT: emul_run<BlkSh>(bd, threads, body_poisson_direct); break;
+    case KL_UCALC: emul_run<BlkSh>(bd, threads, body_ucalc); break;
+    case KL_TRANSPORT: emul_run<TransportSh>(bd, 64, body_transport); break;
+    default: return -1;
+    }
+    return 0;
+}
+#endif

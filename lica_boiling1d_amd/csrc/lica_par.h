@@ -1,0 +1,94 @@
+// Workgroup-level parallel primitives used by the kernels.
+//
+// The kernels are written as a uniform orchestration (executed by every
+// thread of the workgroup) over three primitives:
+//   PFOR(i, lo, hi) { body }  -- work items i in [lo,hi) spread over the
+//                                workgroup, followed by a barrier
+//   SERIAL { body }           -- executed by thread 0, followed by a barrier
+//   red_max / red_min         -- exact order-free reductions
+// Uniform decisions read shared (LDS) or global state after a barrier.
+//
+// With LICA_EMUL defined the same source compiles as plain C++ where a
+// workgroup runs its work items in order; that build exists only for the
+// CPU logic tests of the kernels and is never part of the product library.
+#pragma once
+
+#ifdef LICA_EMUL
+#include <cmath>
+#include <cstring>
+#include <algorithm>
+using std::isfinite;
+#define DEV inline
+#define HOSTDEV inline
+#define SHARED static
+#define BARRIER() ((void)0)
+#define PFOR(i, lo, hi) for (int i = (lo); i < (hi); ++i)
+#define PFOR_END
+#define SERIAL if (1)
+#define SERIAL_END
+extern int lica_emul_nthreads;
+#define NTHREADS (lica_emul_nthreads)
+#else
+#include <hip/hip_runtime.h>
+#define DEV __device__ __forceinline__
+#define HOSTDEV __host__ __device__ __forceinline__
+#define SHARED __shared__
+#define BARRIER() __syncthreads()
+#define PFOR(i, lo, hi) for (int i = (lo) + (int)threadIdx.x; i < (hi); i += (int)blockDim.x)
+#define PFOR_END __syncthreads();
+#define SERIAL if (threadIdx.x == 0)
+#define SERIAL_END __syncthreads();
+#define NTHREADS ((int)blockDim.x)
+#endif
+
+// exact max/min over f(i), i in [lo,hi), combined with init
+template <class F>
+DEV double red_max(int lo, int hi, double init, F f, double *scr)
+{
+#ifdef LICA_EMUL
+    (void)scr;
+    double m = init;
+    for (int i = lo; i < hi; ++i) m = fmax(m, f(i));
+    return m;
+#else
+    double m = init;
+    for (int i = lo + (int)threadIdx.x; i < hi; i += (int)blockDim.x) m = fmax(m, f(i));
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    if ((threadIdx.x & 63) == 0) scr[w] = m;
+    __syncthreads();
+    double r = init;
+    for (int i = 0; i < nw; ++i) r = fmax(r, scr[i]);
+    __syncthreads();
+    return r;
+#endif
+}
+
+template <class F>
+DEV double red_min(int lo, int hi, double init, F f, double *scr)
+{
+#ifdef LICA_EMUL
+    (void)scr;
+    double m = init;
+    for (int i = lo; i < hi; ++i) m = fmin(m, f(i));
+    return m;
+#else
+    double m = init;
+    for (int i = lo + (int)threadIdx.x; i < hi; i += (int)blockDim.x) m = fmin(m, f(i));
+    for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off, 64));
+    const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    if ((threadIdx.x & 63) == 0) scr[w] = m;
+    __syncthreads();
+    double r = init;
+    for (int i = 0; i < nw; ++i) r = fmin(r, scr[i]);
+    __syncthreads();
+    return r;
+#endif
+}
+
+// logical OR over work items (as an int reduction)
+template <class F>
+DEV int red_or(int lo, int hi, F f, double *scr)
+{
+    return red_max(lo, hi, 0.0, [&](int i) { return f(i) ? 1.0 : 0.0; }, scr) > 0.5;
+}

@@ -1,0 +1,1798 @@
+/*
+ * lica_oracle.c -- TEST INFRASTRUCTURE ONLY (parity checker + CPU baseline).
+ *
+ * Serial FP64 restatement of the reference's per-step path, following
+ *   1d.f        MAIN 196-295, DETERMINE_DT 683-817, CFL 1325-1373,
+ *               RHSNLHS 1444-1600 and everything it calls,
+ *               CONVRGE1 3306-3362, geometry 4616-5376
+ *   two_phase.f LVSINIT 116-458, ALPHI_BC 462-642, REIN_TRIGGER 646-697,
+ *               TRANSPORT 701-1130, TRANSPORT_RK3 1133-1495,
+ *               BAND_GENERATION 1622-2030, REINITIALIZATION 2033-2444,
+ *               GODUNOV/HJWENO 2448-2500, CAL_PSIF 2503-2831,
+ *               CAL_PSIF_MTP 2835-3164, CAL_PSI_TMP 3167-3315,
+ *               GRID_COUPLING 4863-5118, box averages 4433-4860/5123-5536,
+ *               VEL_INTP/VELASSIGN/VEL_R3 5539-6191,
+ *               GLOBAL_MASS_CORRECTION 6196-6318
+ *   chphase.f   TEMP_GRAD 55-368, TEMPASSIGN 372-463, TEMP_EXTP 467-803,
+ *               TEMP_EXTP_RK3 806-956, TF_GRAD 959-1159,
+ *               PHINSINTP 1258-1325, VOL_TOT_MFDOT 1328-1493
+ *   grid.f      UNIFORM 326-346 and the grid.dat format 240-242
+ * with every literal and default-kind variable taken as REAL*8/INTEGER*8
+ * (the reference only makes sense built with -fdefault-real-8
+ * -fdefault-double-8 -fdefault-integer-8), OpenMP loops in serial ascending
+ * order, and evaluation order left-to-right as written.
+ *
+ * One z column is kept: the reference's refined grid has r x r identical
+ * x/y columns (periodic x/y, U=V=0), so every x/y difference is exactly zero.
+ * Rounding-level departures from the 3-D code (each documented where it
+ * happens):
+ *   R1  sums over the r*r identical x/y copies are taken once per z index;
+ *   R2  x/y Lagrange/trilinear weights applied to identical columns are
+ *       replaced by their exact sum (one);
+ *   R3  ALPHI_BC's z copy writes outside ALPHI_EXT(-2:N3F+3) (r=4) are
+ *       dropped instead of corrupting memory.
+ * Behavioural decisions for code that cannot run as shipped:
+ *   D1  LHSPS2 (called at 1d.f:3665, absent) is LHSPS;
+ *   D2  PASSIVE_SCALAR (called with no arguments at 1d.f:1528) works on the
+ *       arrays of RHSNLHS;
+ *   D7  arrays read before their first assignment (ANPSO local of
+ *       PASSIVE_SCALAR, MFLVS/SGAS/IMASK_PH on step 1, SFGAS before the first
+ *       TF_GRAD, TRANSPORT's WT ghosts, TEMP_EXTP's DTEMPN ghosts, local
+ *       DPHI arrays) read as zero;
+ *   D8  TEMP_EXTP's unbounded halve-and-continue loops stop after
+ *       ORC_MAX_HALVINGS halvings with status ORC_EXTRAP_NOT_CONVERGED;
+ *   D9  the GLOBAL_MASS_CORRECTION call after the subcycles
+ *       (two_phase.f:979-984) shifts ALPHI_EXT after ALPHI was stored and
+ *       ALPHI_EXT is then deallocated, so it has no effect and is skipped.
+ */
+#include "lica_oracle.h"
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define M_MAX 11
+#define NN1 8       /* N_MAX-3, two_phase.f:744 */
+#define NN2 10      /* N_MAX-1, two_phase.f:745 */
+#define MAXP 16     /* z-planes stored per band layer */
+#define ORC_MAX_HALVINGS 20
+
+#ifdef ORC_CHECK
+#define CHK(c) do { if (!(c)) { fprintf(stderr, "oracle bounds: %s at %s:%d\n", #c, __FILE__, __LINE__); abort(); } } while (0)
+#else
+#define CHK(c) ((void)0)
+#endif
+
+typedef struct { double *d; int64_t lo, hi; } darr;
+typedef struct { int64_t *d; int64_t lo, hi; } iarr;
+
+static inline double *dat(const darr *a, int64_t i) { CHK(i >= a->lo && i <= a->hi); return &a->d[i - a->lo]; }
+static inline int64_t *iat(const iarr *a, int64_t i) { CHK(i >= a->lo && i <= a->hi); return &a->d[i - a->lo]; }
+#define D(a, i) (*dat(&(a), (i)))
+#define I(a, i) (*iat(&(a), (i)))
+
+static darr dnew(int64_t lo, int64_t hi) { darr a; a.lo = lo; a.hi = hi; a.d = (double *)calloc((size_t)(hi - lo + 1), sizeof(double)); return a; }
+static iarr inew(int64_t lo, int64_t hi) { iarr a; a.lo = lo; a.hi = hi; a.d = (int64_t *)calloc((size_t)(hi - lo + 1), sizeof(int64_t)); return a; }
+static void dzero(darr *a) { memset(a->d, 0, sizeof(double) * (size_t)(a->hi - a->lo + 1)); }
+
+struct orc_inst {
+    orc_case c;
+    int status;
+    /* sizes */
+    int64_t n3m, n3, r, n3f, n3fm, m3l, m3u;
+    /* derived properties (INPUT_SIMULATION 1d.f:434-480) */
+    double pi, rei, denm, denp, den_diff, vism, visp, denscm, denscp, densc_diff;
+    double tcm, tcp, tc_diff, prai, surf, gravity, time_st, time_gv;
+    double gamma[3], ro[3];
+    /* NS geometry */
+    darr z, zp, sdz, vdz, ssdz, vvdz, fixkl, akw, ckw, akuv, ckuv;
+    iarr kcou1, kcou2, kcoump1, kcoump2;
+    double sdx1, sdy1, xp1, yp1, xl, yl;
+    /* refined geometry */
+    darr zf, zpf;
+    iarr kcou_vel, kcoump_vel;
+    double sdxf, sdyf, sdzf, ssdzf;
+    double *xpf, *ypf;          /* 1..r */
+    double phins_xd, phins_yd;  /* PHINSINTP x/y weights */
+    /* fields */
+    darr w, p, t, psi_zn, psi_cn, wo, temp0, temp1, rk3zo;
+    darr psif, sfgas, sfgas2;
+    iarr mask_global, imask_ph;
+    /* band: ordered z-planes per layer, per-slot ALPHI and MFLVS */
+    int64_t bn[M_MAX + 1];
+    int64_t bk[M_MAX + 1][MAXP];
+    double alphi[M_MAX + 1][MAXP];
+    double mflvs[M_MAX + 1][MAXP];
+    /* scalars */
+    double time, dt, alpha, acoef, dtconst, dtconsti;
+    int64_t m, msub;
+    double vol_tot_ori, vol_tot_iphs, place, dvmax, cflm;
+    double n_subcycle_warn, n_mass_warn, n_halvings, n_extrap_iters, n_reinit, n_bandgen, n_masscor;
+    /* work arrays */
+    darr df_z, vf_z, df_c, psi_z, psi_c, denf_z, denf_zi, denf_c, rhsps, rhs1, divgsum;
+    darr ga, gb, gc, gg;    /* tridiagonal scratch */
+    iarr region;
+    darr phi, phi_rk3, phio, dphi, wf, wt, psif_mtp, psif_mtp2, tfe0, tfe1, mfd, phi_s;
+    darr nphi, psih, dtempn, dtempn_rk3, tempo, temp_rk3, temp_new, psi_tmp;
+    iarr mask_s, maskc;
+    /* last Poisson system (diagnostic) */
+    darr pa, pb, pc, pg;
+};
+
+/* ------------------------------------------------------------------ */
+/* grid.f UNIFORM (326-346) in REAL*8, written ES23.15 (grid.f:242) and
+ * read back by READGRID (1d.f:4746-4805).                               */
+static double es23_15(double v)
+{
+    char buf[64];
+    snprintf(buf, sizeof buf, "%.15E", v);
+    return strtod(buf, NULL);
+}
+
+void orc_grid_nodes(int64_t n, double zl, double *z)
+{
+    /* NGI=1, NGF=n+1, XYZI=0, XYZF=zl */
+    double dc = (zl - 0.0) / (double)((n + 1) - 1);
+    for (int64_t k = 1; k <= n + 1; k++) z[k - 1] = es23_15(0.0 + (double)(k - 1) * dc);
+}
+
+/* GEOM_COUPLING's cell-centre search (1d.f:5156-5177): the NS centre at or
+ * below zz, clamped to [1, n3m].                                          */
+static int64_t coump_vel_search(const orc_inst *s, double zz)
+{
+    for (int64_t kk = 1; kk <= s->n3m; kk++) {
+        if (fabs(zz - D(s->zp, kk)) <= 1.0e-10) return kk;
+        else if (D(s->zp, 1) > zz) return 1;
+        else if (D(s->zp, kk) > zz) return kk - 1;
+        else if (D(s->zp, s->n3m) < zz) return s->n3m;
+    }
+    return -1000000;
+}
+
+static int build_geometry(orc_inst *s)
+{
+    int64_t n = s->n3m, r = s->r;
+    int64_t n3 = s->n3, n3f = s->n3f, n3fm = s->n3fm;
+    double *zn = (double *)malloc(sizeof(double) * (size_t)(n3 + 1));
+    double *zfn = (double *)malloc(sizeof(double) * (size_t)(n3f + 1));
+    orc_grid_nodes(n, s->c.zl, zn);
+    orc_grid_nodes(n3fm, s->c.zl, zfn);
+    /* x/y: one NS cell of width XL=YL=1 split into r refined cells */
+    double xl = es23_15(1.0), yl = es23_15(1.0);
+    s->xl = xl; s->yl = yl;
+    double *xfn = (double *)malloc(sizeof(double) * (size_t)(r + 1));
+    orc_grid_nodes(r, xl, xfn);
+    s->sdx1 = xl - 0.0;                 /* SDX(1)=X(2)-X(1) */
+    s->sdy1 = yl - 0.0;
+    s->xp1 = 0.0 + 0.5 * s->sdx1;       /* XP(1)=X(1)+0.5*SDX(1) */
+    s->yp1 = 0.0 + 0.5 * s->sdy1;
+    s->sdxf = xfn[1] - xfn[0];
+    s->sdyf = s->sdxf;
+    s->xpf = (double *)malloc(sizeof(double) * (size_t)(r + 1));
+    s->ypf = (double *)malloc(sizeof(double) * (size_t)(r + 1));
+    for (int64_t i = 1; i <= r; i++) { s->xpf[i] = xfn[i - 1] + 0.5 * s->sdxf; s->ypf[i] = s->xpf[i]; }
+
+    int64_t lo = -2, hi = n3 + 3;
+    s->z = dnew(lo, hi); s->zp = dnew(lo, hi); s->sdz = dnew(lo, hi); s->vdz = dnew(lo, hi);
+    s->ssdz = dnew(lo, hi); s->vvdz = dnew(lo, hi); s->fixkl = dnew(lo, hi);
+    s->akw = dnew(lo, hi); s->ckw = dnew(lo, hi); s->akuv = dnew(lo, hi); s->ckuv = dnew(lo, hi);
+    for (int64_t k = 1; k <= n3; k++) D(s->z, k) = zn[k - 1];
+    D(s->z, 0) = D(s->z, 1);                               /* 1d.f:4791 */
+    /* GEOMETRIC 1d.f:4838-4880 */
+    for (int64_t k = 1; k <= n; k++) D(s->sdz, k) = D(s->z, k + 1) - D(s->z, k);
+    for (int64_t k = 2; k <= n; k++) D(s->vdz, k) = 0.5 * (D(s->sdz, k) + D(s->sdz, k - 1));
+    D(s->vdz, 1) = 0.5 * D(s->sdz, 1);
+    D(s->vdz, n3) = 0.5 * D(s->sdz, n);
+    for (int64_t k = 1; k <= n; k++) D(s->ssdz, k) = 1.0 / D(s->sdz, k);
+    for (int64_t k = 1; k <= n3; k++) D(s->vvdz, k) = 1.0 / D(s->vdz, k);
+    D(s->sdz, 0) = 0.0; D(s->sdz, n3) = 0.0;
+    for (int64_t k = 1; k <= n; k++) D(s->zp, k) = D(s->z, k) + 0.5 * D(s->sdz, k);
+    D(s->zp, n3) = D(s->z, n3);
+    D(s->zp, 0) = D(s->z, 1);
+    /* INDICES 1d.f:4986-4997: FIXKL(1)=1 */
+    D(s->fixkl, 1) = 1.0;
+    /* LHSINIT 1d.f:1963-1978 (KBG=2 since IPZ=3) */
+    for (int64_t k = 2; k <= n; k++) { D(s->akw, k) = -D(s->vvdz, k) * D(s->ssdz, k - 1); D(s->ckw, k) = -D(s->vvdz, k) * D(s->ssdz, k); }
+    for (int64_t k = 1; k <= n; k++) { D(s->akuv, k) = -D(s->vvdz, k) * D(s->ssdz, k); D(s->ckuv, k) = -D(s->vvdz, k + 1) * D(s->ssdz, k); }
+
+    /* refined z: READGRID + GEOMETRIC_LVS 1d.f:4885-4942 */
+    s->zf = dnew(s->m3l, s->m3u); s->zpf = dnew(s->m3l, s->m3u);
+    for (int64_t k = 1; k <= n3f; k++) D(s->zf, k) = zfn[k - 1];
+    s->sdzf = D(s->zf, 2) - D(s->zf, 1);
+    s->ssdzf = 1.0 / s->sdzf;
+    for (int64_t k = 1; k <= n3fm; k++) D(s->zpf, k) = D(s->zf, k) + 0.5 * s->sdzf;
+    for (int64_t k = 0; k <= 2 * r; k++) {
+        D(s->zf, -k) = D(s->zf, 1) - s->sdzf * (double)(k + 1);
+        D(s->zf, n3f + 1 + k) = D(s->zf, n3f) + s->sdzf * (double)(k + 1);
+    }
+    for (int64_t k = 0; k <= 2 * r; k++) {
+        D(s->zpf, -k) = D(s->zf, -k) + 0.5 * s->sdzf;
+        D(s->zpf, n3f + k) = D(s->zf, n3f + k) + 0.5 * s->sdzf;
+    }
+    /* GEOM_COUPLING z (1d.f:5134-5369) */
+    s->kcou_vel = inew(1, n3f); s->kcoump_vel = inew(1, n3f);
+    for (int64_t k = 1; k <= n3f; k++) {
+        I(s->kcou_vel, k) = coump_vel_search(s, D(s->zf, k));
+        I(s->kcoump_vel, k) = coump_vel_search(s, D(s->zpf, k));
+        if (I(s->kcou_vel, k) < 0 || I(s->kcoump_vel, k) < 0) return ORC_GRID;
+    }
+    s->kcou1 = inew(-1, n3 + 2); s->kcou2 = inew(-1, n3 + 2);
+    s->kcoump1 = inew(-1, n3 + 2); s->kcoump2 = inew(-1, n3 + 2);
+    for (int64_t k = 1; k <= n3; k++) {
+        double zc = D(s->z, k);
+        int found = 0;
+        for (int64_t kf = 1; kf <= n3f; kf++) {
+            if (fabs(D(s->zf, kf) - zc) <= 1.0e-10) { I(s->kcou1, k) = kf - 1; I(s->kcou2, k) = kf; found = 1; break; }
+            else if (D(s->zf, kf) > zc) { I(s->kcou1, k) = kf - 1; I(s->kcou2, k) = kf - 1; found = 1; break; }
+        }
+        if (!found) return ORC_GRID;
+    }
+    I(s->kcou1, 0) = I(s->kcou1, 1) - (I(s->kcou1, n) - I(s->kcou1, n - 1));
+    I(s->kcou1, -1) = I(s->kcou1, 0) - (I(s->kcou1, n - 1) - I(s->kcou1, n - 2));
+    I(s->kcou1, n3 + 1) = I(s->kcou1, n3) + (I(s->kcou1, 2) - I(s->kcou1, 1));
+    I(s->kcou1, n3 + 2) = I(s->kcou1, n3 + 1) + (I(s->kcou1, 3) - I(s->kcou1, 2));
+    I(s->kcou2, 0) = I(s->kcou2, 1) - (I(s->kcou2, n) - I(s->kcou2, n - 1));
+    I(s->kcou2, -1) = I(s->kcou2, 0) - (I(s->kcou2, n - 1) - I(s->kcou2, n - 2));
+    I(s->kcou2, n3 + 1) = I(s->kcou2, n3) + (I(s->kcou2, 2) - I(s->kcou2, 1));
+    I(s->kcou2, n3 + 2) = I(s->kcou2, n3 + 1) + (I(s->kcou2, 3) - I(s->kcou2, 2));
+    for (int64_t k = 1; k <= n; k++) {
+        double zc = D(s->zp, k);
+        int found = 0;
+        for (int64_t kf = 1; kf <= n3f; kf++) {
+            if (fabs(D(s->zf, kf) - zc) <= 1.0e-10) { I(s->kcoump1, k) = kf - 1; I(s->kcoump2, k) = kf; found = 1; break; }
+            else if (D(s->zf, kf) > zc) { I(s->kcoump1, k) = kf - 1; I(s->kcoump2, k) = kf - 1; found = 1; break; }
+        }
+        if (!found) return ORC_GRID;
+    }
+    I(s->kcoump1, 0) = I(s->kcoump1, 1) - (I(s->kcoump1, n) - I(s->kcoump1, n - 1));
+    I(s->kcoump1, -1) = I(s->kcoump1, 0) - (I(s->kcoump1, n - 1) - I(s->kcoump1, n - 2));
+    I(s->kcoump1, n3) = I(s->kcoump1, n) + (I(s->kcoump1, n) - I(s->kcoump1, n - 1));
+    I(s->kcoump1, n3 + 1) = I(s->kcoump1, n3) + (I(s->kcoump1, 2) - I(s->kcoump1, 1));
+    I(s->kcoump2, 0) = I(s->kcoump2, 1) - (I(s->kcoump2, n) - I(s->kcoump2, n - 1));
+    I(s->kcoump2, -1) = I(s->kcoump2, 0) - (I(s->kcoump2, n - 1) - I(s->kcoump2, n - 2));
+    I(s->kcoump2, n3) = I(s->kcoump2, n) + (I(s->kcoump2, n) - I(s->kcoump2, n - 1));
+    I(s->kcoump2, n3 + 1) = I(s->kcoump2, n3) + (I(s->kcoump2, 2) - I(s->kcoump2, 1));
+
+    /* PHINSINTP x/y weights (chphase.f:1274-1289): I1=ICOU2(1)=1,
+     * I2=ICOUMP2(1) = refined node index of XP(1) (a node for even r)   */
+    {
+        int64_t i1 = 1, i2 = -1;
+        for (int64_t kf = 1; kf <= r + 1; kf++) {
+            if (fabs(xfn[kf - 1] - s->xp1) <= 1.0e-10) { i2 = kf; break; }
+            else if (xfn[kf - 1] > s->xp1) { i2 = kf - 1; break; }
+        }
+        if (i2 < 1) return ORC_GRID;
+        s->phins_xd = (i1 == i2) ? 0.0 : (s->xp1 - s->xpf[i1]) / (s->xpf[i2] - s->xpf[i1]);
+        s->phins_yd = s->phins_xd;
+    }
+    free(zn); free(zfn); free(xfn);
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------ */
+orc_inst *orc_new(const orc_case *c)
+{
+    orc_inst *s = (orc_inst *)calloc(1, sizeof(orc_inst));
+    s->c = *c;
+    s->n3m = c->n; s->n3 = c->n + 1; s->r = c->r;
+    s->n3fm = c->r * c->n; s->n3f = s->n3fm + 1;
+    s->m3l = -2 * c->r; s->m3u = s->n3f + 2 * c->r + 1;
+    /* INPUT_SIMULATION 1d.f:434-480 */
+    s->pi = acos(-1.0);
+    double re = c->re;
+    s->rei = 1.0 / re;
+    s->denm = 1.0; s->denp = c->denr; s->den_diff = s->denp - s->denm;
+    s->vism = 1.0 * s->rei; s->visp = c->visr * s->rei;
+    s->denscm = s->denm * 1.0; s->denscp = s->denscm * (c->denr * c->scr); s->densc_diff = s->denscp - s->denscm;
+    s->tcm = 1.0; s->tcp = c->tcr; s->tc_diff = s->tcp - s->tcm;
+    s->prai = 1.0 / c->prm;
+    s->surf = (c->surf_j == 0.0) ? 0.0 : 1.0 / c->surf_j;
+    s->gravity = (c->fr == 0.0) ? 0.0 : 1.0 / (c->fr * c->fr);
+    s->time_st = s->surf * s->pi / (s->denm + s->denp);
+    s->time_gv = s->den_diff / (s->denm + s->denp) * s->gravity / s->pi;
+    /* RK3COEF 1d.f:1314-1319 */
+    s->gamma[0] = 8.0 / 15.0; s->gamma[1] = 5.0 / 12.0; s->gamma[2] = 3.0 / 4.0;
+    s->ro[0] = 0.0; s->ro[1] = -17.0 / 60.0; s->ro[2] = -5.0 / 12.0;
+
+    s->status = build_geometry(s);
+    int64_t lo = -2, hi = s->n3 + 3;
+    s->w = dnew(lo, hi); s->p = dnew(lo, hi); s->t = dnew(lo, hi); s->psi_zn = dnew(lo, hi);
+    s->psi_cn = dnew(lo, hi); s->wo = dnew(lo, hi); s->temp0 = dnew(lo, hi); s->temp1 = dnew(lo, hi);
+    s->rk3zo = dnew(lo, hi);
+    s->df_z = dnew(lo, hi); s->vf_z = dnew(lo, hi); s->df_c = dnew(lo, hi); s->psi_z = dnew(lo, hi);
+    s->psi_c = dnew(lo, hi); s->denf_z = dnew(lo, hi); s->denf_zi = dnew(lo, hi); s->denf_c = dnew(lo, hi);
+    s->rhsps = dnew(lo, hi); s->rhs1 = dnew(lo, hi); s->divgsum = dnew(lo, hi);
+    s->ga = dnew(lo, hi); s->gb = dnew(lo, hi); s->gc = dnew(lo, hi); s->gg = dnew(lo, hi);
+    s->pa = dnew(lo, hi); s->pb = dnew(lo, hi); s->pc = dnew(lo, hi); s->pg = dnew(lo, hi);
+    s->region = inew(-1, s->n3 + 1);
+    s->imask_ph = inew(0, s->n3);
+    s->nphi = dnew(lo, hi); s->psih = dnew(lo, hi); s->dtempn = dnew(lo, hi); s->dtempn_rk3 = dnew(lo, hi);
+    s->tempo = dnew(lo, hi); s->temp_rk3 = dnew(lo, hi); s->temp_new = dnew(lo, hi); s->psi_tmp = dnew(lo, hi);
+    s->wt = dnew(-1, s->n3 + 2);
+    s->psif = dnew(s->m3l, s->m3u); s->sfgas = dnew(s->m3l, s->m3u); s->sfgas2 = dnew(s->m3l, s->m3u);
+    s->psif_mtp = dnew(s->m3l, s->m3u); s->psif_mtp2 = dnew(s->m3l, s->m3u);
+    s->mask_global = inew(0, s->n3f);
+    int64_t plo = -2, phi_hi = s->n3f + 3;
+    s->phi = dnew(plo, phi_hi); s->phi_rk3 = dnew(plo, phi_hi); s->phio = dnew(plo, phi_hi);
+    s->dphi = dnew(plo, phi_hi); s->tfe0 = dnew(plo, phi_hi); s->tfe1 = dnew(plo, phi_hi);
+    s->mfd = dnew(plo, phi_hi); s->phi_s = dnew(plo, phi_hi);
+    s->wf = dnew(1, s->n3f);
+    s->mask_s = inew(plo, phi_hi);
+    s->maskc = inew(0, s->n3f);
+    return s;
+}
+
+void orc_free(orc_inst *s)
+{
+    if (!s) return;
+    darr *ds[] = {&s->z, &s->zp, &s->sdz, &s->vdz, &s->ssdz, &s->vvdz, &s->fixkl, &s->akw, &s->ckw, &s->akuv, &s->ckuv,
+                  &s->zf, &s->zpf, &s->w, &s->p, &s->t, &s->psi_zn, &s->psi_cn, &s->wo, &s->temp0, &s->temp1, &s->rk3zo,
+                  &s->psif, &s->sfgas, &s->sfgas2, &s->df_z, &s->vf_z, &s->df_c, &s->psi_z, &s->psi_c, &s->denf_z,
+                  &s->denf_zi, &s->denf_c, &s->rhsps, &s->rhs1, &s->divgsum, &s->ga, &s->gb, &s->gc, &s->gg,
+                  &s->phi, &s->phi_rk3, &s->phio, &s->dphi, &s->wf, &s->wt, &s->psif_mtp, &s->psif_mtp2, &s->tfe0,
+                  &s->tfe1, &s->mfd, &s->phi_s, &s->nphi, &s->psih, &s->dtempn, &s->dtempn_rk3, &s->tempo,
+                  &s->temp_rk3, &s->temp_new, &s->psi_tmp, &s->pa, &s->pb, &s->pc, &s->pg};
+    for (size_t i = 0; i < sizeof ds / sizeof ds[0]; i++) free(ds[i]->d);
+    iarr *is[] = {&s->kcou1, &s->kcou2, &s->kcoump1, &s->kcoump2, &s->kcou_vel, &s->kcoump_vel, &s->mask_global,
+                  &s->imask_ph, &s->region, &s->mask_s, &s->maskc};
+    for (size_t i = 0; i < sizeof is / sizeof is[0]; i++) free(is[i]->d);
+    free(s->xpf); free(s->ypf);
+    free(s);
+}
+
+int64_t orc_n(orc_inst *s) { return s->n3m; }
+
+/* ------------------------------------------------------------------ */
+/* TRDIAG3 1d.f:3783-3812 (rows l1..l2)                                  */
+static void trdiag3(darr *a, darr *b, darr *c, darr *r, int64_t l1, int64_t l2, darr *gam)
+{
+    double bet = 1.0 / D(*b, l1);
+    D(*r, l1) = D(*r, l1) * bet;
+    for (int64_t j = l1 + 1; j <= l2; j++) {
+        D(*gam, j) = D(*c, j - 1) * bet;
+        bet = 1.0 / (D(*b, j) - D(*a, j) * D(*gam, j));
+        D(*r, j) = (D(*r, j) - D(*a, j) * D(*r, j - 1)) * bet;
+    }
+    for (int64_t j = l2 - 1; j >= l1; j--) D(*r, j) = D(*r, j) - D(*gam, j + 1) * D(*r, j + 1);
+}
+
+void orc_trdiag3(int64_t n, const double *a, const double *b, const double *c, double *g)
+{
+    darr A = dnew(1, n), B = dnew(1, n), C = dnew(1, n), R = dnew(1, n), G = dnew(1, n);
+    for (int64_t i = 1; i <= n; i++) { D(A, i) = a[i - 1]; D(B, i) = b[i - 1]; D(C, i) = c[i - 1]; D(R, i) = g[i - 1]; }
+    trdiag3(&A, &B, &C, &R, 1, n, &G);
+    for (int64_t i = 1; i <= n; i++) g[i - 1] = D(R, i);
+    free(A.d); free(B.d); free(C.d); free(R.d); free(G.d);
+}
+
+/* HJWENO two_phase.f:2477-2500 */
+double orc_hjweno(double a, double b, double c, double d)
+{
+    double eps = 1.0e-6;
+    double s1 = 13.0 * ((a - b) * (a - b)) + 3.0 * ((a - 3.0 * b) * (a - 3.0 * b));
+    double s2 = 13.0 * ((b - c) * (b - c)) + 3.0 * ((b + c) * (b + c));
+    double s3 = 13.0 * ((c - d) * (c - d)) + 3.0 * ((3.0 * c - d) * (3.0 * c - d));
+    double a1 = 1.0 / ((eps + s1) * (eps + s1));
+    double a2 = 6.0 / ((eps + s2) * (eps + s2));
+    double a3 = 3.0 / ((eps + s3) * (eps + s3));
+    double w1 = a1 / (a1 + a2 + a3);
+    double w3 = a3 / (a1 + a2 + a3);
+    return 1.0 / 3.0 * w1 * (a - 2.0 * b + c) + 1.0 / 6.0 * (w3 - 0.5) * (b - 2.0 * c + d);
+}
+
+/* GODUNOV two_phase.f:2448-2474 */
+double orc_godunov(double asign, double adlphip, double adlphim)
+{
+    double aplus = asign * adlphip, aminus = asign * adlphim, r = 0.0;
+    if (aplus >= 0.0 && aminus >= 0.0) r = adlphim;
+    else if (aplus <= 0.0 && aminus <= 0.0) r = adlphip;
+    else if (aplus > 0.0 && aminus < 0.0) r = 0.0;
+    else if (aplus < 0.0 && aminus > 0.0) r = (aplus + aminus > 0.0) ? adlphim : adlphip;
+    return r;
+}
+
+/* CAL_PSIF_MTP cell formula two_phase.f:2869-2920 with D1=D2=0:
+ * E1=D3, E2=E3=0.  Returns -1 for the 'CAL_PSI HAS BUG' branch.        */
+double orc_psif_cell(double phi, double d3)
+{
+    const double eps = 1.0e-8;
+    double d1 = 0.0, d2 = 0.0;
+    double e1 = fmax(fmax(d1, d2), d3);
+    double e3 = fmin(fmin(d1, d2), d3);
+    double e2 = d1 + d2 + d3 - e1 - e3;
+    double t;
+    if (e1 > eps && e2 > eps && e3 > eps) {
+        double a = fabs(phi);
+        double f1 = fmax(0.5 * (e1 + e2 + e3) - a, 0.0), f2 = fmax(0.5 * (e1 + e2 - e3) - a, 0.0);
+        double f3 = fmax(0.5 * (e1 - e2 + e3) - a, 0.0), f4 = fmax(0.5 * (-e1 + e2 + e3) - a, 0.0);
+        double f5 = fmax(0.5 * (e1 - e2 - e3) - a, 0.0);
+        t = (f1 * f1 * f1 - f2 * f2 * f2 - f3 * f3 * f3 - f4 * f4 * f4 + f5 * f5 * f5) / (6.0 * e1 * e2 * e3);
+    } else if (e1 > eps && e2 > eps && e3 <= eps) {
+        double a = fabs(phi);
+        double f1 = fmax(0.5 * (e1 + e2 + e3) - a, 0.0), f3 = fmax(0.5 * (e1 - e2 + e3) - a, 0.0);
+        t = (f1 * f1 - f3 * f3) / (2.0 * e1 * e2);
+    } else if (e1 > eps && e2 <= eps && e3 <= eps) {
+        double f1 = fmax(0.5 * (e1 + e2 + e3) - fabs(phi), 0.0);
+        t = f1 / e1;
+    } else if (e1 <= eps && e2 <= eps && e3 <= eps && phi != 0.0) {
+        t = 0.0;
+    } else if (e1 <= eps && e2 <= eps && e3 <= eps && phi == 0.0) {
+        t = 0.5;
+    } else {
+        return -1.0;
+    }
+    return (phi <= 0.0) ? t : 1.0 - t;
+}
+
+/* Box average INTER_PSI_FACE / INTER_PSI / INTER_PSI_MTP (1d.f:2552-2959,
+ * two_phase.f:4433-4860, 5123-5536): every refined x/y cell of the single
+ * NS column is fully covered (branch 'IIIIIIII2'/'JJJJJJJJJ2'), so only
+ * the z loop remains (R1).                                             */
+static double box_avg(const orc_inst *s, const darr *f, double z1, double z2, int64_t k1, int64_t k2)
+{
+    double sum = 0.0, volf = 0.0;
+    for (int64_t kk = k1; kk <= k2; kk++) {
+        double zl = D(s->zf, kk), zh = D(s->zf, kk + 1);
+        if (zl <= z1 && zh >= z2) { sum = sum + D(*f, kk); volf = volf + 1.0; break; }
+        else if (zl < z1 && zh > z1) { sum = sum + D(*f, kk) * (zh - z1) * s->ssdzf; volf = volf + (zh - z1) * s->ssdzf; }
+        else if (zl >= z1 && zh <= z2) { sum = sum + D(*f, kk); volf = volf + 1.0; }
+        else if (zl < z2 && zh > z2) { sum = sum + D(*f, kk) * (z2 - zl) * s->ssdzf; volf = volf + (z2 - zl) * s->ssdzf; }
+        else if (zl >= z2) break;
+    }
+    return sum / volf;
+}
+
+/* box over NS cell k: [Z(k),Z(k+1)], refined KCOU2(k)..KCOU1(k+1) */
+static double box_cell(const orc_inst *s, const darr *f, int64_t k)
+{
+    return box_avg(s, f, D(s->z, k), D(s->z, k + 1), I(s->kcou2, k), I(s->kcou1, k + 1));
+}
+
+/* ------------------------------------------------------------------ */
+/* Level-set helpers on the 1-D refined column                          */
+
+/* rebuild ALPHI_EXT: zero everywhere, band planes from ALPHI
+ * (two_phase.f:750-772, chphase.f:146-167)                              */
+static void phi_from_band(orc_inst *s)
+{
+    dzero(&s->phi);
+    for (int n = 1; n <= M_MAX; n++)
+        for (int64_t q = 0; q < s->bn[n]; q++) D(s->phi, s->bk[n][q]) = s->alphi[n][q];
+}
+
+/* ALPHI_BC z part (two_phase.f:565-600) then the z copy 623-634 (R3) */
+static void alphi_bc(orc_inst *s)
+{
+    int64_t n3f = s->n3f, n3fm = s->n3fm;
+    double sd = s->sdzf;
+    double a1 = D(s->phi, 1);
+    if (fabs(a1) < 1.0e-8) { D(s->phi, 0) = 0.0; D(s->phi, -1) = 0.0; D(s->phi, -2) = 0.0; }
+    else if (a1 > 0.0) { D(s->phi, 0) = a1 + sd; D(s->phi, -1) = D(s->phi, 0) + sd; D(s->phi, -2) = D(s->phi, -1) + sd; }
+    else if (a1 < 0.0) { D(s->phi, 0) = a1 - sd; D(s->phi, -1) = D(s->phi, 0) - sd; D(s->phi, -2) = D(s->phi, -1) - sd; }
+    double an = D(s->phi, n3fm);
+    if (fabs(an) < 1.0e-8) { for (int k = 0; k <= 3; k++) D(s->phi, n3f + k) = 0.0; }
+    else if (an > 0.0) { D(s->phi, n3f) = an + sd; for (int k = 1; k <= 3; k++) D(s->phi, n3f + k) = D(s->phi, n3f + k - 1) + sd; }
+    else if (an < 0.0) { D(s->phi, n3f) = an - sd; for (int k = 1; k <= 3; k++) D(s->phi, n3f + k) = D(s->phi, n3f + k - 1) - sd; }
+    for (int64_t k = 0; k <= s->r; k++) {
+        if (-k >= s->phi.lo) D(s->phi, -k) = D(s->phi, n3fm - k);
+        if (n3f + k <= s->phi.hi) D(s->phi, n3f + k) = D(s->phi, 1 + k);
+    }
+}
+
+/* BAND_GENERATION two_phase.f:1622-2030 on z-planes.  mask_s must hold
+ * 0 for every plane on entry; 'scan' lists the planes of the old layers
+ * 1..NN2 in list order.                                                 */
+static int band_generation(orc_inst *s, const int64_t *scan, int64_t nscan)
+{
+    iarr *ms = &s->mask_s;
+    memset(ms->d, 0, sizeof(int64_t) * (size_t)(ms->hi - ms->lo + 1));
+    int64_t nb[M_MAX + 1] = {0};
+    int64_t bk[M_MAX + 1][MAXP];
+    /* S-tube (two_phase.f:1667-1694) */
+    for (int64_t q = 0; q < nscan; q++) {
+        int64_t k = scan[q];
+        double a = D(s->phi, k);
+        if (a * D(s->phi, k + 1) < 0.0 || a * D(s->phi, k - 1) < 0.0) {
+            if (nb[1] >= MAXP) return ORC_BAND_CAPACITY;
+            bk[1][nb[1]++] = k;
+            I(*ms, k) = 1;
+        } else {
+            I(*ms, k) = 2;
+        }
+    }
+    /* layers 2..N_MAX (two_phase.f:1700-1960) */
+    for (int n = 2; n <= M_MAX; n++) {
+        for (int64_t q = 0; q < nb[n - 1]; q++) {
+            int64_t k = bk[n - 1][q];
+            for (int64_t k2 = k - 1; k2 <= k + 1; k2++) {
+                if (k2 == 1 || k2 == s->n3f) continue;   /* boundary touch */
+                if (I(*ms, k2) != 1) {
+                    if (I(*ms, k2) == 0 || n >= NN2) {
+                        /* x/y-flat column: DX=DY=0, DZ=SDZF (two_phase.f:1928-1933) */
+                        double dz = s->sdzf;
+                        double dist = sqrt(0.0 * 0.0 + 0.0 * 0.0 + dz * dz);
+                        if (D(s->phi, k) >= 0.0) D(s->phi, k2) = D(s->phi, k) + dist;
+                        else D(s->phi, k2) = D(s->phi, k) - dist;
+                    }
+                    if (nb[n] >= MAXP) return ORC_BAND_CAPACITY;
+                    bk[n][nb[n]++] = k2;
+                    I(*ms, k2) = 1;
+                }
+            }
+        }
+    }
+    /* MF_BAND check (two_phase.f:1987-1994): MF_BAND=0.1*MF */
+    int64_t mf_band = (int64_t)(0.1 * (double)(s->r * s->r * s->n3fm));
+    for (int n = 1; n <= M_MAX; n++) {
+        if (nb[n] * s->r * s->r > mf_band) return ORC_BAND_OVERFLOW;
+        s->bn[n] = nb[n];
+        for (int64_t q = 0; q < nb[n]; q++) s->bk[n][q] = bk[n][q];
+    }
+    s->n_bandgen += 1;
+    return ORC_OK;
+}
+
+/* planes of layers 1..nl in list order */
+static int64_t band_scan(const orc_inst *s, int nl, int64_t *out)
+{
+    int64_t m = 0;
+    for (int n = 1; n <= nl; n++)
+        for (int64_t q = 0; q < s->bn[n]; q++) out[m++] = s->bk[n][q];
+    return m;
+}
+
+/* CAL_PSIF_MTP two_phase.f:2835-3164 -> out */
+static int cal_psif_mtp(orc_inst *s, int iouter, int nn1, darr *out)
+{
+    int64_t n3f = s->n3f, n3fm = s->n3fm;
+    for (int64_t k = 0; k <= n3f; k++) { D(*out, k) = 0.0; I(s->maskc, k) = 0; }
+    for (int n = 1; n <= nn1; n++)
+        for (int64_t q = 0; q < s->bn[n]; q++) {
+            int64_t k = s->bk[n][q];
+            double d3 = 0.5 * fabs(D(s->phi, k + 1) - D(s->phi, k - 1));
+            double v = orc_psif_cell(D(s->phi, k), d3);
+            if (v == -1.0) return ORC_PSIF_BUG;
+            D(*out, k) = v;
+            I(s->maskc, k) = 1;
+        }
+    for (int n = 1; n <= nn1; n++)
+        for (int64_t q = 0; q < s->bn[n]; q++) {
+            int64_t k = s->bk[n][q];
+            if (D(*out, k) < 1.0e-5) D(*out, k) = 0.0;
+            else if (D(*out, k) > 0.99999) D(*out, k) = 1.0;
+        }
+    if (iouter == 1) {
+        /* column fill, z scanned downward (two_phase.f:3011-3075) */
+        int64_t *kc = (int64_t *)calloc((size_t)n3f + 2, sizeof(int64_t));
+        int64_t knum = 0;
+        for (int64_t k = n3fm; k >= 1; k--)
+            if (D(s->phi, k) * D(s->phi, k + 1) < 0.0) kc[++knum] = k;
+        if (knum == 0) {
+            for (int64_t k = 1; k <= n3fm; k++) if (I(s->maskc, k) == 0) D(*out, k) = 1.0;
+        } else {
+            int64_t l;
+            for (l = 1; l <= knum; l++) {
+                double tmp = (D(s->phi, kc[l]) < 0.0) ? 0.0 : 1.0;
+                if (l == 1) {
+                    if (fabs(D(*out, 1) - 0.5) < 0.5) {
+                        for (int64_t k = 2; k <= kc[l]; k++) if (I(s->maskc, k) == 0) D(*out, k) = tmp;
+                    } else {
+                        for (int64_t k = 1; k <= kc[l]; k++) if (I(s->maskc, k) == 0) D(*out, k) = tmp;
+                    }
+                } else {
+                    for (int64_t k = kc[l - 1]; k <= kc[l]; k++) if (I(s->maskc, k) == 0) D(*out, k) = tmp;
+                }
+            }
+            /* after the loop L=K_NUM+1 and KC(L)=0 (never N3FM) */
+            if (kc[l] != n3fm) {
+                double tmp = (D(s->phi, kc[knum] + 1) < 0.0) ? 0.0 : 1.0;
+                for (int64_t k = kc[knum] + 1; k <= n3fm; k++) if (I(s->maskc, k) == 0) D(*out, k) = tmp;
+            }
+        }
+        free(kc);
+    }
+    for (int64_t k = 0; k <= s->r; k++) { D(*out, -k) = D(*out, 1); D(*out, n3f + k) = D(*out, n3fm); }
+    return ORC_OK;
+}
+
+/* volume sum over the refined column, DVOL per unit x/y area (R1) */
+static double vol_sum(const orc_inst *s, const darr *pm)
+{
+    double dvol = s->sdzf;
+    double v = 0.0;
+    for (int64_t k = 1; k <= s->n3fm; k++) v = v + (1.0 - D(*pm, k)) * dvol;
+    return v;
+}
+
+/* DPHI_Z at refined faces for layers 1..nn1 and around layer nn1
+ * (two_phase.f:1165-1232, 2206-2271)                                     */
+static void dphi_z(orc_inst *s, int nn1)
+{
+    dzero(&s->dphi);
+    for (int n = 1; n <= nn1; n++)
+        for (int64_t q = 0; q < s->bn[n]; q++) {
+            int64_t k = s->bk[n][q];
+            D(s->dphi, k) = (D(s->phi, k + 1) - D(s->phi, k)) * s->ssdzf;
+        }
+    for (int64_t q = 0; q < s->bn[nn1]; q++) {
+        int64_t k = s->bk[nn1][q];
+        int64_t km3 = k - 3, km2 = k - 2, km1 = k - 1, kp1 = k + 1, kp2 = k + 2;
+        D(s->dphi, km3) = (D(s->phi, km2) - D(s->phi, km3)) * s->ssdzf;
+        D(s->dphi, km2) = (D(s->phi, km1) - D(s->phi, km2)) * s->ssdzf;
+        D(s->dphi, km1) = (D(s->phi, k) - D(s->phi, km1)) * s->ssdzf;
+        D(s->dphi, kp1) = (D(s->phi, kp2) - D(s->phi, kp1)) * s->ssdzf;
+        D(s->dphi, kp2) = (D(s->phi, kp2 + 1) - D(s->phi, kp2)) * s->ssdzf;
+    }
+}
+
+/* the x/y WENO derivatives of a flat column: HJWENO(0,0,0,0) */
+static double weno_flat_m(void) { return 1.0 / 12.0 * (-0.0 + 7.0 * 0.0 + 7.0 * 0.0 - 0.0) - orc_hjweno(0.0, 0.0, 0.0, 0.0); }
+static double weno_flat_p(void) { return 1.0 / 12.0 * (-0.0 + 7.0 * 0.0 + 7.0 * 0.0 - 0.0) + orc_hjweno(0.0, 0.0, 0.0, 0.0); }
+
+static double weno_m(const orc_inst *s, int64_t k)
+{
+    double v1 = D(s->dphi, k - 3), v2 = D(s->dphi, k - 2), v3 = D(s->dphi, k - 1), v4 = D(s->dphi, k), v5 = D(s->dphi, k + 1);
+    double wz = orc_hjweno(v2 - v1, v3 - v2, v4 - v3, v5 - v4);
+    return 1.0 / 12.0 * (-v2 + 7.0 * v3 + 7.0 * v4 - v5) - wz;
+}
+static double weno_p(const orc_inst *s, int64_t k)
+{
+    double v1 = D(s->dphi, k + 2), v2 = D(s->dphi, k + 1), v3 = D(s->dphi, k), v4 = D(s->dphi, k - 1), v5 = D(s->dphi, k - 2);
+    double wz = orc_hjweno(v1 - v2, v2 - v3, v3 - v4, v4 - v5);
+    return 1.0 / 12.0 * (-v5 + 7.0 * v4 + 7.0 * v3 - v2) + wz;
+}
+
+/* REINITIALIZATION_RK3 two_phase.f:2174-2444 (one forward-Euler stage) */
+static void reinit_rk3(orc_inst *s, double redtl, int nn1, int nn2)
+{
+    double eps = s->sdzf * s->sdzf;  /* MIN(SDXF,SDYF,SDZF)**2 */
+    static double newv[M_MAX + 1][MAXP];
+    if (nn1 != 0) dphi_z(s, nn1);
+    double wxm = weno_flat_m(), wxp = weno_flat_p();
+    for (int n = 1; n <= nn1; n++)
+        for (int64_t q = 0; q < s->bn[n]; q++) {
+            int64_t k = s->bk[n][q];
+            double dax = 0.5 * (0.0 - 0.0) * (1.0 / s->sdxf);
+            double daz = 0.5 * (D(s->phi, k + 1) - D(s->phi, k - 1)) * s->ssdzf;
+            double dalphi = dax * dax + dax * dax + daz * daz;
+            double po = D(s->phio, k);
+            double asign = po / sqrt(po * po + dalphi * eps);
+            double azm = weno_m(s, k), azp = weno_p(s, k);
+            double ax = orc_godunov(asign, wxp, wxm);
+            double ay = orc_godunov(asign, wxp, wxm);
+            double az = orc_godunov(asign, azp, azm);
+            double drhs = asign * (1.0 - sqrt(ax * ax + ay * ay + az * az));
+            newv[n][q] = D(s->phi, k) + redtl * drhs;
+        }
+    for (int n = nn1 + 1; n <= nn2; n++)
+        for (int64_t q = 0; q < s->bn[n]; q++) {
+            int64_t k = s->bk[n][q];
+            double dax = 0.5 * (0.0 - 0.0) * (1.0 / s->sdxf);
+            double daz = 0.5 * (D(s->phi, k + 1) - D(s->phi, k - 1)) * s->ssdzf;
+            double dalphi = dax * dax + dax * dax + daz * daz;
+            double po = D(s->phio, k);
+            double asign = po / sqrt(po * po + dalphi * eps);
+            double axm = 0.0, axp = 0.0;
+            double azm = (D(s->phi, k) - D(s->phi, k - 1)) * s->ssdzf;
+            double azp = (D(s->phi, k + 1) - D(s->phi, k)) * s->ssdzf;
+            double ax = orc_godunov(asign, axp, axm);
+            double ay = orc_godunov(asign, axp, axm);
+            double az = orc_godunov(asign, azp, azm);
+            double drhs = asign * (1.0 - sqrt(ax * ax + ay * ay + az * az));
+            newv[n][q] = D(s->phi, k) + redtl * drhs;
+        }
+    for (int n = 1; n <= nn2; n++)
+        for (int64_t q = 0; q < s->bn[n]; q++) D(s->phi, s->bk[n][q]) = newv[n][q];
+}
+
+/* REINITIALIZATION two_phase.f:2033-2170 */
+static void reinitialization(orc_inst *s, int nn1, int nn2, int ilimit)
+{
+    for (int n = 1; n <= nn2; n++)
+        for (int64_t q = 0; q < s->bn[n]; q++) { int64_t k = s->bk[n][q]; D(s->phio, k) = D(s->phi, k); }
+    double sm_grid = fmin(fmin(s->sdxf, s->sdyf), s->sdzf);
+    double redtl = 0.5 * sm_grid;
+    for (int it = 1; it <= ilimit; it++) {
+        for (int n = 1; n <= nn2; n++)
+            for (int64_t q = 0; q < s->bn[n]; q++) { int64_t k = s->bk[n][q]; D(s->phi_rk3, k) = D(s->phi, k); }
+        reinit_rk3(s, redtl, nn1, nn2);
+        reinit_rk3(s, redtl, nn1, nn2);
+        for (int n = 1; n <= nn2; n++)
+            for (int64_t q = 0; q < s->bn[n]; q++) { int64_t k = s->bk[n][q]; D(s->phi, k) = 0.75 * D(s->phi_rk3, k) + 0.25 * D(s->phi, k); }
+        reinit_rk3(s, redtl, nn1, nn2);
+        for (int n = 1; n <= nn2; n++)
+            for (int64_t q = 0; q < s->bn[n]; q++) { int64_t k = s->bk[n][q]; D(s->phi, k) = 1.0 / 3.0 * D(s->phi_rk3, k) + 2.0 / 3.0 * D(s->phi, k); }
+    }
+    s->n_reinit += 1;
+}
+
+/* ------------------------------------------------------------------ */
+/* CAL_PSIF two_phase.f:2503-2831 (LLVS=1; bubble-contact bookkeeping and
+ * CAL_PSI_TMP/CAL_CUR only feed the surface-tension term, which is exactly
+ * zero for a flat column: GX=GY=GXX=GYY=GXY=0 gives CURF=0, 3383-3400)    */
+static int cal_psif(orc_inst *s, int ini_psi, int nn1)
+{
+    int64_t n3f = s->n3f, n3fm = s->n3fm;
+    int st;
+    if (ini_psi == 1) {
+        st = cal_psif_mtp(s, 1, nn1, &s->psif_mtp);
+        if (st) return st;
+        for (int64_t k = 1; k <= n3fm; k++) D(s->psif, k) = D(s->psif_mtp, k);
+    } else {
+        st = cal_psif_mtp(s, 0, nn1, &s->psif_mtp);
+        if (st) return st;
+        for (int64_t k = 1; k <= n3fm; k++) I(s->mask_global, k) = 0;
+        for (int n = 1; n <= nn1; n++)
+            for (int64_t q = 0; q < s->bn[n]; q++) {
+                int64_t k = s->bk[n][q];
+                if (I(s->mask_global, k) == 0) { D(s->psif, k) = D(s->psif_mtp, k); I(s->mask_global, k) = 1; }
+                else D(s->psif, k) = fmin(D(s->psif, k), D(s->psif_mtp, k));
+            }
+    }
+    for (int64_t k = 1; k <= n3fm; k++) {
+        if (D(s->psif, k) < 1.0e-5) D(s->psif, k) = 0.0;
+        else if (D(s->psif, k) > 0.99999) D(s->psif, k) = 1.0;
+    }
+    for (int64_t k = 0; k <= s->r; k++) { D(s->psif, -k) = D(s->psif, 1); D(s->psif, n3f + k) = D(s->psif, n3fm); }
+    return ORC_OK;
+}
+
+/* GRID_COUPLING two_phase.f:4863-5118 */
+static void grid_coupling(orc_inst *s, int ini_psi)
+{
+    int64_t n = s->n3m;
+    int64_t *mask = (int64_t *)calloc((size_t)n + 2, sizeof(int64_t));
+    if (ini_psi == 1) {
+        for (int64_t k = 1; k <= n; k++) mask[k] = 1;
+    } else {
+        for (int64_t k = 1; k <= s->n3fm; k++)
+            if (I(s->mask_global, k) != 0) mask[I(s->kcoump_vel, k)] = 1;
+    }
+    for (int64_t k = 2; k <= n; k++)
+        if (mask[k] == 1)
+            D(s->psi_zn, k) = box_avg(s, &s->psif, D(s->zp, k - 1), D(s->zp, k), I(s->kcoump2, k - 1), I(s->kcoump1, k));
+    for (int64_t k = 1; k <= n; k++) {
+        if (D(s->psi_zn, k) < 1.0e-3) D(s->psi_zn, k) = 0.0;
+        else if (D(s->psi_zn, k) > 0.999) D(s->psi_zn, k) = 1.0;
+    }
+    for (int64_t k = 1; k <= n; k++)
+        if (mask[k] == 1) D(s->psi_cn, k) = box_cell(s, &s->psif, k);
+    for (int64_t k = 1; k <= n; k++) {
+        if (D(s->psi_cn, k) < 1.0e-3) D(s->psi_cn, k) = 0.0;
+        else if (D(s->psi_cn, k) > 0.999) D(s->psi_cn, k) = 1.0;
+    }
+    D(s->psi_cn, 0) = D(s->psi_cn, 1);
+    D(s->psi_cn, s->n3) = D(s->psi_cn, n);
+    free(mask);
+}
+
+/* GLOBAL_MASS_CORRECTION two_phase.f:6196-6318 */
+static int global_mass_correction(orc_inst *s, int nn1)
+{
+    double recor = fmin(fmin(s->sdxf, s->sdyf), s->sdzf);
+    double dvol = s->sdzf;          /* per unit x/y area (R1) */
+    recor = recor * 1.5;
+    s->vol_tot_ori = s->vol_tot_iphs;
+    double vol_tot = 0.0, vol_in = 0.0;
+    int st;
+    for (int l = 1; l <= 15; l++) {
+        double vol_band = 0.0;
+        if (l == 1) {
+            st = cal_psif_mtp(s, 1, nn1, &s->psif_mtp2);
+            if (st) return st;
+            vol_tot = vol_sum(s, &s->psif_mtp2);
+            for (int n = 1; n <= nn1; n++)
+                for (int64_t q = 0; q < s->bn[n]; q++) vol_band = vol_band + (1.0 - D(s->psif_mtp2, s->bk[n][q])) * dvol;
+            vol_in = vol_tot - vol_band;
+        } else {
+            st = cal_psif_mtp(s, 0, nn1, &s->psif_mtp2);
+            if (st) return st;
+            for (int n = 1; n <= nn1; n++)
+                for (int64_t q = 0; q < s->bn[n]; q++) vol_band = vol_band + (1.0 - D(s->psif_mtp2, s->bk[n][q])) * dvol;
+            vol_tot = vol_in + vol_band;
+        }
+        s->n_masscor += 1;
+        if (fabs(vol_tot - s->vol_tot_ori) >= 1.0e-5) {
+            double cor = recor * (-(s->vol_tot_ori - vol_tot) / s->vol_tot_ori);
+            for (int n = 1; n <= nn1; n++)
+                for (int64_t q = 0; q < s->bn[n]; q++) { int64_t k = s->bk[n][q]; D(s->phi, k) = D(s->phi, k) + cor; }
+        } else {
+            break;
+        }
+    }
+    if (fabs(vol_tot - s->vol_tot_ori) >= 1.0e-5) s->n_mass_warn += 1;
+    return ORC_OK;
+}
+
+/* VEL_INTP two_phase.f:5539-5682 with VELASSIGN/VEL_R3 5685-6191 (R2) */
+static void vel_intp(orc_inst *s, int ivel_update)
+{
+    int64_t n3 = s->n3, n = s->n3m;
+    if (s->m == 1 && s->msub == 1)
+        for (int64_t k = 0; k <= n3; k++) D(s->wo, k) = D(s->w, k);
+    if (ivel_update == 1 || ivel_update == 2) {
+        for (int64_t k = 0; k <= n3; k++) D(s->wt, k) = 0.5 * (D(s->w, k) + D(s->wo, k));
+        if (ivel_update == 2)
+            for (int64_t k = 0; k <= n3; k++) D(s->wo, k) = D(s->w, k);
+    }
+    for (int nl = 1; nl <= NN2; nl++)
+        for (int64_t q = 0; q < s->bn[nl]; q++) {
+            int64_t k = s->bk[nl][q];
+            int64_t kk = I(s->kcou_vel, k);
+            int64_t km1 = kk - 1, kp1 = kk + 1, kp2 = kp1 + 1;
+            double zz = D(s->zf, k), z2 = D(s->zp, kk), z1, z3, z4;
+            z1 = z2 - D(s->vdz, kk);
+            if (kk == n - 1) { z3 = z2 + D(s->vdz, kp1); z4 = z3 + 2.0 * D(s->vdz, n3); }
+            else if (kk == n) { z3 = z2 + 2.0 * D(s->vdz, n3); z4 = z3 + D(s->vdz, n); }
+            else { z3 = z2 + D(s->vdz, kp1); z4 = z3 + D(s->vdz, kp2); }
+            double cz1 = ((zz - z2) * (zz - z3) * (zz - z4)) / ((z1 - z2) * (z1 - z3) * (z1 - z4));
+            double cz2 = ((zz - z1) * (zz - z3) * (zz - z4)) / ((z2 - z1) * (z2 - z3) * (z2 - z4));
+            double cz3 = ((zz - z1) * (zz - z2) * (zz - z4)) / ((z3 - z1) * (z3 - z2) * (z3 - z4));
+            double cz4 = ((zz - z1) * (zz - z2) * (zz - z3)) / ((z4 - z1) * (z4 - z2) * (z4 - z3));
+            double az1 = 0.5 * (D(s->wt, km1) + D(s->wt, km1 + 1));
+            double az2 = 0.5 * (D(s->wt, kk) + D(s->wt, kk + 1));
+            double az3 = 0.5 * (D(s->wt, kp1) + D(s->wt, kp1 + 1));
+            double az4 = 0.5 * (D(s->wt, kp2) + D(s->wt, kp2 + 1));
+            double wi = cz1 * az1 + cz2 * az2 + cz3 * az3 + cz4 * az4;
+            D(s->wf, k) = (nl < NN2 - 1) ? wi : wi * 0.5;
+        }
+    for (int64_t q = 0; q < s->bn[M_MAX]; q++) D(s->wf, s->bk[M_MAX][q]) = 0.0;
+}
+
+/* REIN_TRIGGER two_phase.f:646-697 */
+static void rein_trigger(orc_inst *s, double *dmax, double *dmin)
+{
+    double mx = 0.0, mn = 1.0;
+    for (int n = 1; n <= NN1; n++) {
+        double mxo = 0.0, mno = 1.0;
+        for (int64_t q = 0; q < s->bn[n]; q++) {
+            int64_t k = s->bk[n][q];
+            double dx = 0.5 * (0.0 - 0.0) * (1.0 / s->sdxf);
+            double dz = 0.5 * (D(s->phi, k + 1) - D(s->phi, k - 1)) * s->ssdzf;
+            double sum = dx * dx + dx * dx + dz * dz;
+            mxo = fmax(sum, mxo);
+            mno = fmin(sum, mno);
+        }
+        mx = fmax(mx, mxo);
+        mn = fmin(mn, mno);
+    }
+    *dmax = sqrt(mx);
+    *dmin = sqrt(mn);
+}
+
+/* TRANSPORT_RK3 two_phase.f:1133-1495 (WENO branch: FUNCBODY=100>CRI) */
+static void transport_rk3(orc_inst *s, double dtlvs, const double (*alphi_old)[MAXP], const int64_t *bn_old)
+{
+    static double newv[M_MAX + 1][MAXP];
+    dphi_z(s, NN1);
+    double wxm = weno_flat_m();
+    for (int n = 1; n <= NN1; n++)
+        for (int64_t q = 0; q < s->bn[n]; q++) {
+            int64_t k = s->bk[n][q];
+            /* x, y: UF=VF=0 -> first branch, ALPHI_X=0*ADLPHIXM */
+            double ax = 0.0 * wxm, ay = 0.0 * wxm, az;
+            double uu1 = D(s->wf, k), uu2 = D(s->wf, k + 1), uu3 = 0.5 * (uu1 + uu2);
+            if (uu1 >= 0.0 && uu2 >= 0.0 && uu3 >= 0.0) {
+                az = uu3 * weno_m(s, k);
+            } else if (uu1 <= 0.0 && uu2 <= 0.0 && uu3 <= 0.0) {
+                az = uu3 * weno_p(s, k);
+            } else {
+                double m = weno_m(s, k), p = weno_p(s, k);
+                double umax = fmax(fmax(fabs(uu1), fabs(uu2)), fabs(uu3));
+                az = 0.5 * (uu3 * (p + m) - umax * (p - m));
+            }
+            newv[n][q] = D(s->phi, k) - dtlvs * (ax + ay + az);
+        }
+    /* phase-change source with the ALPHI/MFLVS of the band held when
+     * TRANSPORT started (two_phase.f:1468-1480): slot q of layer n      */
+    for (int n = 1; n <= NN1; n++)
+        for (int64_t q = 0; q < s->bn[n]; q++) {
+            double mf = (q < bn_old[n]) ? s->mflvs[n][q] : 0.0;
+            if (alphi_old[n][q] > 0.0) newv[n][q] = newv[n][q] - dtlvs * mf / s->c.denr;
+            else newv[n][q] = newv[n][q] - dtlvs * mf;
+        }
+    for (int n = 1; n <= NN1; n++)
+        for (int64_t q = 0; q < s->bn[n]; q++) D(s->phi, s->bk[n][q]) = newv[n][q];
+}
+
+/* TRANSPORT two_phase.f:701-1130 */
+static int transport(orc_inst *s)
+{
+    int st;
+    /* ALPHI and MFLVS as held at entry (indexed by the entry band) */
+    double alphi_old[M_MAX + 1][MAXP];
+    int64_t bn_old[M_MAX + 1];
+    memcpy(alphi_old, s->alphi, sizeof alphi_old);
+    memcpy(bn_old, s->bn, sizeof bn_old);
+    phi_from_band(s);
+    alphi_bc(s);
+    dzero(&s->wf);
+    vel_intp(s, 2);
+    double time_tran = 0.0, dmax = 0.0, dmin = 1.0;
+    int imass = 0, done = 0;
+    for (int isub = 1; isub <= 10; isub++) {
+        double cflm = 0.0, dtlvs;
+        for (int n = 1; n <= NN1; n++) {
+            double cflo = 0.0;
+            for (int64_t q = 0; q < s->bn[n]; q++) {
+                int64_t k = s->bk[n][q];
+                double cflx = fabs(0.5 * (0.0 + 0.0)) * (1.0 / s->sdxf);
+                double cflz = fabs(0.5 * (D(s->wf, k - 1) + D(s->wf, k))) * s->ssdzf;
+                double cfli = cflx + cflx + cflz;
+                cflo = fmax(cflo, cfli);
+            }
+            cflm = fmax(cflm, cflo);
+        }
+        if (cflm == 0.0) {
+            double uvw = 0.0;
+            for (int n = 1; n <= NN1; n++)
+                for (int64_t q = 0; q < s->bn[n]; q++) uvw = uvw + (0.0 + 0.0 + D(s->wf, s->bk[n][q]));
+            if (uvw == 0.0) dtlvs = s->dtconst / 10.0;  /* IPHS=1 */
+            else dtlvs = 1.0 / cflm;
+        } else {
+            dtlvs = 1.0 / cflm;
+        }
+        if (time_tran + dtlvs >= s->dtconst) dtlvs = s->dtconst - time_tran;
+        time_tran = time_tran + dtlvs;
+        for (int n = 1; n <= NN1; n++)
+            for (int64_t q = 0; q < s->bn[n]; q++) { int64_t k = s->bk[n][q]; D(s->phi_rk3, k) = D(s->phi, k); }
+        transport_rk3(s, dtlvs, (const double (*)[MAXP])alphi_old, bn_old);
+        transport_rk3(s, dtlvs, (const double (*)[MAXP])alphi_old, bn_old);
+        for (int n = 1; n <= NN1; n++)
+            for (int64_t q = 0; q < s->bn[n]; q++) { int64_t k = s->bk[n][q]; D(s->phi, k) = 0.75 * D(s->phi_rk3, k) + 0.25 * D(s->phi, k); }
+        transport_rk3(s, dtlvs, (const double (*)[MAXP])alphi_old, bn_old);
+        for (int n = 1; n <= NN1; n++)
+            for (int64_t q = 0; q < s->bn[n]; q++) { int64_t k = s->bk[n][q]; D(s->phi, k) = 1.0 / 3.0 * D(s->phi_rk3, k) + 2.0 / 3.0 * D(s->phi, k); }
+        /* 1-D projection ALPHI_EXT(I,J,K)=ALPHI_EXT(2,2,K): identity here */
+        rein_trigger(s, &dmax, &dmin);
+        imass = (dmax > 2.0 || dmin < 1.0e-4) ? 1 : 0;
+        reinitialization(s, NN1, NN2, NN2);               /* IPHS=1 */
+        if (s->m % 50 == 0) imass = 1;
+        if (dmax > 2.0 || dmin < 1.0e-4 || imass == 1) {
+            if (imass == 1) { st = global_mass_correction(s, NN1); if (st) return st; }
+            alphi_bc(s);
+            int64_t scan[M_MAX * MAXP];
+            int64_t ns = band_scan(s, NN2, scan);
+            st = band_generation(s, scan, ns);
+            if (st) return st;
+            alphi_bc(s);
+            reinitialization(s, NN1, NN2, NN2);
+        }
+        alphi_bc(s);
+        if (time_tran == s->dtconst) { done = 1; break; }
+        if (dmax > 2.0 || dmin < 1.0e-4) vel_intp(s, 0);
+    }
+    if (!done) s->n_subcycle_warn += 1;
+    st = cal_psif(s, 0, NN1);
+    if (st) return st;
+    for (int n = 1; n <= M_MAX; n++)
+        for (int64_t q = 0; q < s->bn[n]; q++) s->alphi[n][q] = D(s->phi, s->bk[n][q]);
+    (void)imass;   /* D9: the post-subcycle mass correction has no effect */
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* CAL_CONTINUITY 1d.f:1994-2068 / CAL_CONTINUITY2 3030-3062 regions    */
+static void mark_regions(orc_inst *s)
+{
+    int64_t n = s->n3m;
+    for (int64_t k = -1; k <= s->n3 + 1; k++) I(s->region, k) = 0;
+    for (int64_t k = 1; k <= n; k++) {
+        if ((D(s->psi_cn, k + 1) - D(s->psi_cn, k) != 0.0) || (D(s->psi_cn, k) - D(s->psi_cn, k - 1) != 0.0)) {
+            for (int64_t k2 = k - 2; k2 <= k + 2; k2++) I(s->region, k2) = 2;
+        } else if (D(s->psi_cn, k) == 1.0) {
+            if (I(s->region, k) != 2) I(s->region, k) = 1;
+        }
+    }
+}
+
+/* PSIFACE_Z (1d.f:2349-2441, IDIR=3) */
+static double psiface_z(const orc_inst *s, int64_t k, double uu)
+{
+    double z1, z2; int64_t k1, k2;
+    if (uu >= 0.0) { z1 = D(s->zp, k) - uu * s->dtconst; z2 = D(s->zp, k); k1 = I(s->kcoump2, k - 1); k2 = I(s->kcoump1, k); }
+    else { z1 = D(s->zp, k); z2 = D(s->zp, k) - uu * s->dtconst; k1 = I(s->kcoump2, k); k2 = I(s->kcoump1, k + 1); }
+    return box_avg(s, &s->psif, z1, z2, k1, k2);
+}
+
+/* PSIFACE_C (1d.f:2444-2549, IDIR=3) */
+static double psiface_c(const orc_inst *s, int64_t k, double uu)
+{
+    double z1, z2; int64_t k1, k2;
+    if (uu >= 0.0) { z1 = D(s->z, k + 1) - uu * s->dtconst; z2 = D(s->z, k + 1); k1 = I(s->kcou2, k); k2 = I(s->kcou1, k + 1); }
+    else { z1 = D(s->z, k + 1); z2 = D(s->z, k + 1) - uu * s->dtconst; k1 = I(s->kcou2, k + 1); k2 = I(s->kcou1, k + 2); }
+    return box_avg(s, &s->psif, z1, z2, k1, k2);
+}
+
+static void cal_continuity(orc_inst *s)
+{
+    int64_t n = s->n3m, n3 = s->n3;
+    mark_regions(s);
+    /* CONTI_Z 1d.f:2071-2214 */
+    for (int64_t k = 0; k <= n3; k++) D(s->df_z, k) = 0.0;
+    for (int64_t k = 2; k <= n; k++) {
+        int64_t kp1 = k + 1, km1 = k - 1;
+        int64_t rg = I(s->region, k);
+        if (rg == 0) {
+            D(s->vf_z, k) = s->vism; D(s->vf_z, km1) = s->vism;
+            D(s->df_z, k) = s->denm; D(s->df_z, km1) = s->denm;
+            D(s->psi_z, k) = D(s->psi_zn, k);
+        } else if (rg == 1) {
+            D(s->vf_z, k) = s->visp; D(s->vf_z, km1) = s->visp;
+            D(s->df_z, k) = s->denp; D(s->df_z, km1) = s->denp;
+            D(s->psi_z, k) = D(s->psi_zn, k);
+        } else {
+            double wt = 0.5 * (D(s->w, kp1) + D(s->w, k));
+            double wb = 0.5 * (D(s->w, k) + D(s->w, km1));
+            double ps;
+            if (D(s->df_z, k) == 0.0) {
+                if (fabs(wt) <= 1.0e-8) ps = 0.5 * (D(s->psi_zn, k) + D(s->psi_zn, kp1));
+                else ps = psiface_z(s, k, wt);
+                D(s->vf_z, k) = s->vism + (s->visp - s->vism) * ps;
+                D(s->df_z, k) = s->denm + s->den_diff * ps;
+            }
+            if (D(s->df_z, km1) == 0.0) {
+                if (fabs(wb) <= 1.0e-8) ps = 0.5 * (D(s->psi_zn, k) + D(s->psi_zn, km1));
+                else ps = psiface_z(s, km1, wb);
+                D(s->vf_z, km1) = s->vism + (s->visp - s->vism) * ps;
+                D(s->df_z, km1) = s->denm + s->den_diff * ps;
+            }
+            double flux = (D(s->df_z, k) * wt - D(s->df_z, km1) * wb) * D(s->vvdz, k);
+            double den_zn = s->denm + s->den_diff * D(s->psi_zn, k);
+            double den_z = den_zn - s->dtconst * flux;
+            D(s->psi_z, k) = (den_z - s->denm) / s->den_diff;
+        }
+    }
+    for (int64_t k = 2; k <= n; k++) {
+        if (D(s->psi_z, k) > 1.0) D(s->psi_z, k) = 1.0;
+        else if (D(s->psi_z, k) < 0.0) D(s->psi_z, k) = 0.0;
+    }
+    D(s->psi_z, 1) = D(s->psi_z, 2);
+    D(s->psi_z, n3) = D(s->psi_z, n);
+    /* CONTI_C 1d.f:2217-2345 */
+    for (int64_t k = 0; k <= n3; k++) D(s->df_c, k) = 0.0;
+    for (int64_t k = 1; k <= n; k++) {
+        int64_t kp1 = k + 1, km1 = k - 1;
+        int64_t rg = I(s->region, k);
+        if (rg == 0) {
+            D(s->psi_c, k) = D(s->psi_cn, k);
+            D(s->df_c, k) = s->denm; D(s->df_c, km1) = s->denm;
+        } else if (rg == 1) {
+            D(s->psi_c, k) = D(s->psi_cn, k);
+            D(s->df_c, k) = s->denp; D(s->df_c, km1) = s->denp;
+        } else {
+            double wt = D(s->w, kp1), wb = D(s->w, k), ps;
+            if (D(s->df_c, k) == 0.0) {
+                if (fabs(wt) <= 1.0e-8) ps = D(s->psi_cn, kp1);
+                else ps = psiface_c(s, k, wt);
+                D(s->df_c, k) = s->denm + s->den_diff * ps;
+            }
+            if (D(s->df_c, km1) == 0.0) {
+                if (fabs(wb) <= 1.0e-8) ps = D(s->psi_cn, k);
+                else ps = psiface_c(s, km1, wb);
+                D(s->df_c, km1) = s->denm + s->den_diff * ps;
+            }
+            double flux = (D(s->df_c, k) * wt - D(s->df_c, km1) * wb) * D(s->ssdz, k);
+            double den_cn = s->denm + s->den_diff * D(s->psi_cn, k);
+            double den_c = den_cn - s->dtconst * (flux);
+            D(s->psi_c, k) = (den_c - s->denm) / s->den_diff;
+        }
+    }
+    for (int64_t k = 1; k <= n; k++) {
+        if (D(s->psi_c, k) > 1.0) D(s->psi_c, k) = 1.0;
+        else if (D(s->psi_c, k) < 0.0) D(s->psi_c, k) = 0.0;
+    }
+    D(s->psi_c, 1) = D(s->psi_c, 2);
+    D(s->psi_c, n3) = D(s->psi_c, n);
+}
+
+/* CAL_CONT_BC 1d.f:2961-3003 */
+static void cal_cont_bc(orc_inst *s)
+{
+    int64_t n = s->n3m, n3 = s->n3;
+    for (int64_t k = 1; k <= n; k++) {
+        D(s->denf_z, k) = s->denm + s->den_diff * D(s->psi_z, k);
+        D(s->denf_zi, k) = 1.0 / D(s->denf_z, k);
+    }
+    D(s->denf_z, n3) = D(s->denf_z, n); D(s->denf_zi, n3) = D(s->denf_zi, n);
+    D(s->denf_z, 0) = D(s->denf_z, 1); D(s->denf_zi, 0) = D(s->denf_zi, 1);
+    for (int64_t k = 1; k <= n; k++) D(s->denf_c, k) = s->denm + s->den_diff * D(s->psi_c, k);
+}
+
+/* CAL_CONTINUITY2 1d.f:3006-3169 */
+static void cal_continuity2(orc_inst *s)
+{
+    int64_t n = s->n3m, n3 = s->n3;
+    mark_regions(s);
+    for (int64_t k = 2; k <= n; k++)
+        if (I(s->region, k) == 2) {
+            double ss = box_avg(s, &s->sfgas2, D(s->zp, k - 1), D(s->zp, k), I(s->kcou2, k), I(s->kcou1, k + 1));
+            D(s->psi_z, k) = D(s->psi_z, k) - ss;
+        }
+    for (int64_t k = 2; k <= n; k++) {
+        if (D(s->psi_z, k) > 1.0) D(s->psi_z, k) = 1.0;
+        else if (D(s->psi_z, k) < 0.0) D(s->psi_z, k) = 0.0;
+    }
+    for (int64_t k = 1; k <= n; k++)
+        if (I(s->region, k) == 2) {
+            double ss = box_cell(s, &s->sfgas2, k);
+            D(s->psi_c, k) = D(s->psi_c, k) - ss;
+        }
+    for (int64_t k = 1; k <= n; k++) {
+        if (D(s->psi_c, k) > 1.0) D(s->psi_c, k) = 1.0;
+        else if (D(s->psi_c, k) < 0.0) D(s->psi_c, k) = 0.0;
+    }
+    D(s->psi_z, 1) = D(s->psi_z, 2); D(s->psi_z, n3) = D(s->psi_z, n);
+    D(s->psi_c, 1) = D(s->psi_c, 2); D(s->psi_c, n3) = D(s->psi_c, n);
+}
+
+/* LHSPS 1d.f:4380-4497 (also LHSPS2, D1) */
+static void lhsps(orc_inst *s)
+{
+    int64_t n = s->n3m;
+    double den_diffi = (s->den_diff == 0.0) ? 1.0 : 1.0 / s->den_diff;
+    double acops = s->alpha * s->dt / (s->c.re * s->c.prm);
+    for (int64_t k = 1; k <= n; k++) {
+        double denscf = s->denscm + s->densc_diff * D(s->psi_c, k);
+        double coef = acops / denscf;
+        double pt = (D(s->df_c, k - 1) - s->denm) * den_diffi;
+        double tcb = s->tcm + s->tc_diff * pt;
+        pt = (D(s->df_c, k) - s->denm) * den_diffi;
+        double tct = s->tcm + s->tc_diff * pt;
+        D(s->ga, k) = coef * tcb * D(s->akuv, k);
+        D(s->gc, k) = coef * tct * D(s->ckuv, k);
+        D(s->gb, k) = 1.0 - (D(s->ga, k) + D(s->gc, k));
+        D(s->gg, k) = D(s->rhsps, k);
+        double pc = D(s->psi_cn, k);
+        if (fabs(pc - 0.5) != 0.5) {
+            if (pc <= 0.5) D(s->gc, k) = 0.0;
+            else if (pc > 0.5) D(s->ga, k) = 0.0;
+        }
+    }
+    darr gam = dnew(0, n + 1);
+    trdiag3(&s->ga, &s->gb, &s->gc, &s->gg, 1, n, &gam);
+    free(gam.d);
+    for (int64_t k = 1; k <= n; k++) D(s->rhsps, k) = D(s->gg, k);
+    for (int64_t k = 1; k <= n; k++) D(s->t, k) = D(s->gg, k) + D(s->t, k);
+    D(s->t, 0) = 0.0;
+    D(s->t, n) = 1.0;
+    D(s->t, s->n3) = D(s->t, n);
+}
+
+/* PHINSINTP chphase.f:1258-1325 on the flat column */
+static double phinsintp(orc_inst *s, int64_t ko, int64_t *ic)
+{
+    int64_t k1 = I(s->kcou2, ko), k2 = I(s->kcoump2, ko);
+    double xd = s->phins_xd, yd = s->phins_yd, zd;
+    zd = (k1 == k2) ? 0.0 : (D(s->zp, ko) - D(s->zpf, k1)) / (D(s->zpf, k2) - D(s->zpf, k1));
+    double a = D(s->phi, k1), b = D(s->phi, k2);
+    double c00 = a * (1.0 - xd) + a * xd, c10 = a * (1.0 - xd) + a * xd;
+    double c01 = b * (1.0 - xd) + b * xd, c11 = b * (1.0 - xd) + b * xd;
+    double c0 = c00 * (1.0 - yd) + c10 * yd, c1 = c01 * (1.0 - yd) + c11 * yd;
+    double v = c0 * (1.0 - zd) + c1 * zd;
+    if (v > 0.0) *ic = 1;
+    if (v < 0.0) *ic = 0;
+    if (a == 0.0 || b == 0.0) v = 0.0;
+    return v;
+}
+
+/* TEMPASSIGN chphase.f:372-463: x/y weights are exactly zero (XD=YD=0) */
+static double tempassign(orc_inst *s, int64_t k, const darr *temp)
+{
+    int64_t kk = I(s->kcoump_vel, k), kk1, kk2;
+    double zd;
+    if (D(s->zp, kk) > D(s->zpf, k)) {
+        kk1 = kk - 1; kk2 = kk;
+        zd = (kk1 == kk2) ? 0.0 : (D(s->zpf, k) - (D(s->zp, kk) - D(s->vdz, kk))) / (D(s->zp, kk2) - D(s->zp, kk1));
+    } else {
+        kk1 = kk; kk2 = kk + 1;
+        zd = (kk1 == kk2) ? 0.0 : (D(s->zpf, k) - D(s->zp, kk1)) / (D(s->zp, kk2) - D(s->zp, kk1));
+    }
+    double xd = 0.0, yd = 0.0;
+    double c00 = D(*temp, kk) * (1.0 - xd) + D(*temp, kk) * xd;
+    double c10 = D(*temp, kk) * (1.0 - xd) + D(*temp, kk) * xd;
+    double c01 = D(*temp, kk2) * (1.0 - xd) + D(*temp, kk2) * xd;
+    double c11 = D(*temp, kk2) * (1.0 - xd) + D(*temp, kk2) * xd;
+    double c0 = c00 * (1.0 - yd) + c10 * yd;
+    double c1 = c01 * (1.0 - yd) + c11 * yd;
+    return c0 * (1.0 - zd) + c1 * zd;
+}
+
+/* z derivative of a 3-point stencil with the NPHI==0 one-sided rule */
+static double dz_rule(const orc_inst *s, const darr *nphi, const darr *f, int64_t k)
+{
+    if (D(*nphi, k - 1) == 0.0) return (D(*f, k + 1) - D(*f, k)) * D(s->vvdz, k + 1);
+    else if (D(*nphi, k + 1) == 0.0) return (D(*f, k) - D(*f, k - 1)) * D(s->vvdz, k);
+    return 0.5 * (D(*f, k + 1) - D(*f, k - 1)) * D(s->vvdz, k);
+}
+
+/* TEMP_EXTP_RK3 chphase.f:806-956 on the array 'tmpo' (in place) */
+static void temp_extp_rk3(orc_inst *s, darr *tmpo, int im, double redtl, int nd, const darr *dtn)
+{
+    int64_t n = s->n3m;
+    double sm_grid = fmin(fmin(2.0 * s->sdxf, 2.0 * s->sdyf), 2.0 * s->sdzf);
+    for (int64_t k = 1; k <= n; k++) D(s->temp_new, k) = D(*tmpo, k);
+    for (int64_t k = 1; k <= n; k++) {
+        double np = D(s->nphi, k);
+        if (np == 0.0) continue;
+        double dpx = 0.5 * (np - np) * (1.0 / s->sdx1);
+        double dpz = dz_rule(s, &s->nphi, &s->nphi, k);
+        double dalphi = dpx * dpx + dpx * dpx + dpz * dpz;
+        if (dalphi == 0.0) continue;
+        double nx = 0.0, ny = 0.0, nz = 1.0;
+        double apxm = (D(*tmpo, k) - D(*tmpo, k)) * (1.0 / s->sdx1);
+        double apxp = apxm;
+        double apzm = (D(*tmpo, k) - D(*tmpo, k - 1)) * D(s->vvdz, k);
+        double apzp = (D(*tmpo, k + 1) - D(*tmpo, k)) * D(s->vvdz, k);
+        if (im == 1) { nx = (-1.0) * nx; ny = (-1.0) * ny; nz = (-1.0) * nz; }
+        double ax = fmin(nx, 0.0) * apxp + fmax(nx, 0.0) * apxm;
+        double ay = fmin(ny, 0.0) * apxp + fmax(ny, 0.0) * apxm;
+        double az = fmin(nz, 0.0) * apzp + fmax(nz, 0.0) * apzm;
+        double drhs;
+        if (im == 1) {
+            drhs = (ax + ay + az) + (double)nd * D(*dtn, k);
+            D(s->temp_new, k) = D(*tmpo, k) - redtl * drhs * (1.0 - D(s->psih, k));
+        } else {
+            drhs = (ax + ay + az) - (double)nd * D(*dtn, k);
+            D(s->temp_new, k) = D(*tmpo, k) - redtl * drhs * D(s->psih, k);
+        }
+        if (nd == 0 && fabs(np) <= sm_grid) {
+            drhs = (ax + ay + az);
+            D(s->temp_new, k) = D(*tmpo, k) - redtl * drhs;
+        }
+    }
+    for (int64_t k = 1; k <= n; k++) D(*tmpo, k) = D(s->temp_new, k);
+}
+
+/* TEMP_EXTP chphase.f:467-803 */
+static int temp_extp(orc_inst *s)
+{
+    int64_t n = s->n3m;
+    double sm_grid = fmin(fmin(2.0 * s->sdxf, 2.0 * s->sdyf), 2.0 * s->sdzf);
+    const double eps = 1.0e-4;
+    for (int im = 0; im <= 1; im++) {
+        darr *temp = (im == 0) ? &s->temp0 : &s->temp1;
+        double redtl = 0.2 * sm_grid;
+        dzero(&s->dtempn); dzero(&s->dtempn_rk3);
+        for (int64_t k = 1; k <= n; k++) {
+            double np = D(s->nphi, k), pc = D(s->psi_cn, k);
+            D(s->dtempn, k) = 0.0;
+            if (((fabs(np) > 1.5 * sm_grid) && (np != 0.0)) || ((fabs(pc - 0.5) == 0.5) && (np != 0.0))) {
+                double dpx = 0.5 * (np - np) * (1.0 / s->sdx1);
+                double dtdx = 0.5 * (D(*temp, k) - D(*temp, k)) * (1.0 / s->sdx1);
+                double dpz = dz_rule(s, &s->nphi, &s->nphi, k);
+                double dtdz = dz_rule(s, &s->nphi, temp, k);
+                double dalphi = dpx * dpx + dpx * dpx + dpz * dpz;
+                if (dalphi != 0.0) {
+                    double nx = 0.0, ny = 0.0, nz = 1.0;
+                    if (im == 1) D(s->dtempn, k) = nx * dtdx + ny * dtdx + nz * dtdz * D(s->psih, k);
+                    else D(s->dtempn, k) = nx * dtdx + ny * dtdx + nz * dtdz * (1.0 - D(s->psih, k));
+                }
+            } else if (((fabs(np) <= 1.5 * sm_grid) || (fabs(pc - 0.5) != 0.5)) && ((np != 1.0e-7) && (np != 0.0))) {
+                if (im == 1) D(s->dtempn, k) = (D(*temp, k) - s->c.tsat) / np * D(s->psih, k);
+                else D(s->dtempn, k) = (D(*temp, k) - s->c.tsat) / np * (1.0 - D(s->psih, k));
+            }
+        }
+        /* pseudo-time extrapolation of dT/dn (ND=0) */
+        int halvings = 0;
+        for (;;) {
+            int conv = 0;
+            for (int it = 1; it <= 100; it++) {
+                for (int64_t k = 1; k <= n; k++) D(s->dtempn_rk3, k) = D(s->dtempn, k);
+                temp_extp_rk3(s, &s->dtempn, im, redtl, 0, &s->dtempn);
+                temp_extp_rk3(s, &s->dtempn, im, redtl, 0, &s->dtempn);
+                for (int64_t k = 1; k <= n; k++) D(s->dtempn, k) = 0.75 * D(s->dtempn_rk3, k) + 0.25 * D(s->dtempn, k);
+                temp_extp_rk3(s, &s->dtempn, im, redtl, 0, &s->dtempn);
+                for (int64_t k = 1; k <= n; k++) D(s->dtempn, k) = 1.0 / 3.0 * D(s->dtempn_rk3, k) + 2.0 / 3.0 * D(s->dtempn, k);
+                double rat = 0.0;
+                for (int64_t k = 1; k <= n; k++) {
+                    if (D(s->dtempn_rk3, k) != 0.0) rat = fmax(fabs(D(s->dtempn, k) / D(s->dtempn_rk3, k) - 1.0), rat);
+                    else if (D(s->dtempn, k) != 0.0) rat = fmax(fabs(D(s->dtempn_rk3, k) / D(s->dtempn, k) - 1.0), rat);
+                }
+                s->n_extrap_iters += 1;
+                if (fabs(rat) < eps) { conv = 1; break; }
+            }
+            if (conv) break;
+            if (++halvings > ORC_MAX_HALVINGS) return ORC_EXTRAP_NOT_CONVERGED;
+            s->n_halvings += 1;
+            redtl = 0.5 * redtl;
+        }
+        /* pseudo-time extrapolation of T (ND=1) */
+        redtl = 0.2 * sm_grid;
+        halvings = 0;
+        for (;;) {
+            int conv = 0;
+            for (int it = 1; it <= 100; it++) {
+                for (int64_t k = 0; k <= s->n3; k++) { D(s->temp_rk3, k) = D(*temp, k); D(s->tempo, k) = D(*temp, k); }
+                temp_extp_rk3(s, &s->tempo, im, redtl, 1, &s->dtempn);
+                temp_extp_rk3(s, &s->tempo, im, redtl, 1, &s->dtempn);
+                for (int64_t k = 1; k <= n; k++) D(s->tempo, k) = 0.75 * D(s->temp_rk3, k) + 0.25 * D(s->tempo, k);
+                temp_extp_rk3(s, &s->tempo, im, redtl, 1, &s->dtempn);
+                for (int64_t k = 1; k <= n; k++) {
+                    D(s->tempo, k) = 1.0 / 3.0 * D(s->temp_rk3, k) + 2.0 / 3.0 * D(s->tempo, k);
+                    D(*temp, k) = D(s->tempo, k);
+                }
+                double rat = 0.0;
+                for (int64_t k = 1; k <= n; k++)
+                    if (D(s->temp_rk3, k) != 0.0) rat = fmax(fabs(D(*temp, k) / D(s->temp_rk3, k) - 1.0), rat);
+                s->n_extrap_iters += 1;
+                if (fabs(rat) < eps) { conv = 1; break; }
+            }
+            if (conv) break;
+            if (++halvings > ORC_MAX_HALVINGS) return ORC_EXTRAP_NOT_CONVERGED;
+            s->n_halvings += 1;
+            redtl = 0.5 * redtl;
+        }
+    }
+    return ORC_OK;
+}
+
+/* TEMP_GRAD chphase.f:55-368 with TF_GRAD 959-1159 and VOL_TOT_MFDOT
+ * 1328-1493 (LLVS=1)                                                    */
+static int temp_grad(orc_inst *s)
+{
+    int64_t n = s->n3m, n3 = s->n3;
+    int st;
+    memset(s->mflvs, 0, sizeof s->mflvs);
+    for (int64_t k = 0; k <= n3; k++) I(s->imask_ph, k) = 0;
+    for (int64_t k = 0; k <= n3; k++) {
+        D(s->temp0, k) = D(s->t, k); D(s->temp1, k) = D(s->t, k);
+        D(s->nphi, k) = 0.0; D(s->psih, k) = 0.0;
+    }
+    phi_from_band(s);     /* no ALPHI_BC in TEMP_GRAD */
+    /* (1) phase of the NS cells under the band (chphase.f:174-194) */
+    int64_t ic = 0;
+    for (int nl = 1; nl <= NN2; nl++)
+        for (int64_t q = 0; q < s->bn[nl]; q++) {
+            int64_t kk = I(s->kcoump_vel, s->bk[nl][q]);
+            if (D(s->nphi, kk) == 0.0) {
+                double v = phinsintp(s, kk, &ic);
+                D(s->nphi, kk) = v;
+                if (fabs(D(s->psi_cn, kk) - 0.5) != 0.5) D(s->psih, kk) = (double)ic;
+                else D(s->psih, kk) = D(s->psi_cn, kk);
+            }
+        }
+    /* outermost layer (chphase.f:198-211): min over the r*r columns */
+    for (int64_t q = 0; q < s->bn[M_MAX]; q++) {
+        int64_t k = s->bk[M_MAX][q];
+        int64_t kk = I(s->kcoump_vel, k);
+        double dz = D(s->zpf, k) - D(s->zp, kk);
+        double best = -1.0;
+        for (int64_t j = 1; j <= s->r; j++)
+            for (int64_t i = 1; i <= s->r; i++) {
+                double dx = s->xpf[i] - s->xp1, dy = s->ypf[j] - s->yp1;
+                double d2 = dx * dx + dy * dy + dz * dz;
+                if (best < 0.0 || d2 < best) best = d2;
+            }
+        double pre = D(s->phi, k) + sqrt(best);
+        D(s->nphi, kk) = fmin(D(s->nphi, kk), pre);
+        D(s->psih, kk) = D(s->psi_cn, kk);
+    }
+    /* (2) extrapolation */
+    st = temp_extp(s);
+    if (st) return st;
+    /* (3)-(6) TF on the T-band, TF_GRAD */
+    dzero(&s->tfe0); dzero(&s->tfe1);
+    for (int64_t k = 1; k <= s->n3fm; k++) { D(s->sfgas, k) = 0.0; D(s->sfgas2, k) = 0.0; }
+    for (int nl = 1; nl <= NN1; nl++)
+        for (int64_t q = 0; q < s->bn[nl]; q++) {
+            int64_t k = s->bk[nl][q];
+            D(s->tfe0, k) = tempassign(s, k, &s->temp0);
+            D(s->tfe1, k) = tempassign(s, k, &s->temp1);
+        }
+    st = cal_psif_mtp(s, 1, NN1, &s->psif_mtp);
+    if (st) return st;
+    for (int nl = 1; nl <= NN1 - 2; nl++)
+        for (int64_t q = 0; q < s->bn[nl]; q++) {
+            int64_t k = s->bk[nl][q];
+            double gx = 0.5 * (0.0 - 0.0) * (1.0 / s->sdxf);
+            double gz = 0.5 * (D(s->phi, k + 1) - D(s->phi, k - 1)) * s->ssdzf;
+            double agi = 1.0 / sqrt(gx * gx + gx * gx + gz * gz);
+            double dtx = 0.5 * (0.0 - 0.0) * (1.0 / s->sdxf);
+            double dtz0 = 0.5 * (D(s->tfe0, k + 1) - D(s->tfe0, k - 1)) * s->ssdzf;
+            double dtdn0 = (dtx * gx + dtx * gx + dtz0 * gz) * agi;
+            double dtz1 = 0.5 * (D(s->tfe1, k + 1) - D(s->tfe1, k - 1)) * s->ssdzf;
+            double dtdn1 = (dtx * gx + dtx * gx + dtz1 * gz) * agi;
+            double mfdot = s->c.ja * s->prai * s->rei * (-s->tcp * dtdn1 + s->tcm * dtdn0);
+            s->mflvs[nl][q] = mfdot;
+        }
+    /* VOL_TOT_MFDOT */
+    for (int64_t k = 0; k <= s->n3f; k++) D(s->phi_s, k) = D(s->phi, k);
+    dzero(&s->mfd);
+    for (int nl = 1; nl <= NN1 - 2; nl++)
+        for (int64_t q = 0; q < s->bn[nl]; q++) {
+            int64_t k = s->bk[nl][q];
+            if (D(s->phi, k) > 0.0) D(s->mfd, k) = s->mflvs[nl][q] / s->c.denr;
+            else D(s->mfd, k) = s->mflvs[nl][q];
+        }
+    for (int nl = 1; nl <= NN1 - 2; nl++)
+        for (int64_t q = 0; q < s->bn[nl]; q++) {
+            int64_t k = s->bk[nl][q];
+            double a = D(s->phi, k);
+            double v = a - 2.0 * s->dtconst * D(s->mfd, k);
+            v = 0.75 * a + 0.25 * v;
+            v = v - s->dtconst * D(s->mfd, k);
+            v = 1.0 / 3.0 * a + 2.0 / 3.0 * v;
+            D(s->phi, k) = v;
+        }
+    reinitialization(s, NN1, NN2, NN2);
+    {
+        double place = 0.0;
+        for (int64_t k = 1; k <= s->n3fm; k++)
+            if (D(s->phi, k) * D(s->phi, k + 1) < 0.0) place = D(s->zpf, k) + fabs(D(s->phi, k));
+        s->place = place;
+    }
+    st = cal_psif_mtp(s, 1, NN1, &s->psif_mtp2);
+    if (st) return st;
+    s->vol_tot_iphs = vol_sum(s, &s->psif_mtp2);
+    for (int nl = 1; nl <= NN1 - 2; nl++)
+        for (int64_t q = 0; q < s->bn[nl]; q++) {
+            if (s->mflvs[nl][q] == 0.0) continue;
+            int64_t k = s->bk[nl][q];
+            if (fabs(D(s->psif_mtp, k) - 0.5) != 0.5) {
+                double d = D(s->psif_mtp2, k) - D(s->psif_mtp, k);
+                if (s->alphi[nl][q] > 0.0) { D(s->sfgas, k) = -d * s->c.denr; D(s->sfgas2, k) = -d; }
+                else { D(s->sfgas, k) = -d; D(s->sfgas2, k) = -d; }
+            }
+        }
+    for (int64_t k = 0; k <= s->n3f; k++) D(s->phi, k) = D(s->phi_s, k);
+    /* (7) CAL_PSI_TMP(1) and the interface mask (chphase.f:300-358) */
+    for (int64_t k = 1; k <= n; k++) {
+        double ps = box_cell(s, &s->psif_mtp, k);
+        if (ps <= 1.0e-3) ps = 0.0;
+        else if (ps >= 0.999) ps = 1.0;
+        D(s->psi_tmp, k) = ps;
+    }
+    for (int64_t k = 2; k <= n; k++)
+        if (fabs(D(s->psi_tmp, k) - D(s->psi_tmp, k - 1)) != 0.0) I(s->imask_ph, k) = 1;
+    return ORC_OK;
+}
+
+/* PASSIVE_SCALAR 1d.f:3630-3707 (D1, D2, D7) */
+static int passive_scalar(orc_inst *s)
+{
+    int64_t n = s->n3m;
+    int st;
+    double g = s->gamma[s->msub - 1];
+    if (s->m != 1) {
+        /* CAL_RHSPS2 1d.f:4175-4376: RHSPS overwritten when IPHS=1 */
+        for (int64_t k = 1; k <= n; k++)
+            D(s->rhsps, k) = (g * D(s->t, k) * (D(s->w, k + 1) - D(s->w, k)) * D(s->vvdz, k)) * s->dt;
+        for (int64_t k = 1; k <= n; k++) {
+            double densc, denscf;
+            if (D(s->psi_cn, k) > 0.5) { densc = s->denscm + s->densc_diff; denscf = densc; }
+            else { densc = s->denscm; denscf = densc; }
+            D(s->rhsps, k) = ((densc - denscf) * D(s->t, k) + D(s->rhsps, k)) / denscf;
+        }
+        lhsps(s);
+        st = temp_grad(s);
+        if (st) return st;
+        cal_continuity2(s);
+        cal_cont_bc(s);
+    } else {
+        /* CAL_RHSPS 1d.f:4038-4171: ANPS=ALPS=TGR=0 and ANPSO=0 */
+        double ro = s->ro[s->msub - 1];
+        for (int64_t k = 1; k <= n; k++)
+            D(s->rhsps, k) = ((g * 0.0 + ro * 0.0) + 2.0 * s->alpha * 0.0 + 2.0 * s->alpha * (0.0 - 0.0)) * s->dt;
+        for (int64_t k = 1; k <= n; k++) {
+            double densc = s->denscm + s->densc_diff * D(s->psi_cn, k);
+            double denscf = s->denscm + s->densc_diff * D(s->psi_c, k);
+            D(s->rhsps, k) = ((densc - denscf) * D(s->t, k) + D(s->rhsps, k)) / denscf;
+        }
+        lhsps(s);
+    }
+    return ORC_OK;
+}
+
+/* RHSZ 1d.f:1603-1745 */
+static void rhsz(orc_inst *s)
+{
+    int64_t n = s->n3m;
+    double g = s->gamma[s->msub - 1], ro = s->ro[s->msub - 1], a2 = 2.0 * s->alpha;
+    for (int64_t k = 2; k <= n; k++) {
+        int64_t kp1 = k + 1, km1 = k - 1;
+        double wt = 0.5 * (D(s->w, kp1) + D(s->w, k));
+        double wb = 0.5 * (D(s->w, k) + D(s->w, km1));
+        double anzz = (D(s->df_z, k) * (wt * wt) - D(s->df_z, km1) * (wb * wb)) * D(s->vvdz, k);
+        double anz = -0.0 - 0.0 - anzz;
+        double alz5 = (D(s->w, kp1) - D(s->w, k)) * D(s->ssdz, k);
+        double alz6 = (D(s->w, k) - D(s->w, km1)) * D(s->ssdz, km1);
+        double alzz = (D(s->vf_z, k) * alz5 - D(s->vf_z, km1) * alz6) * D(s->vvdz, k);
+        double alz_sum1 = 0.0 + 0.0 + alzz + alzz;
+        double alz_sum2 = 0.0 + 0.0;
+        double pgr_z = (D(s->p, k) - D(s->p, km1)) * D(s->vvdz, k);
+        double den_zn = s->denm + s->den_diff * D(s->psi_zn, k);
+        double rk3z = anz + alz_sum2;
+        double cn2z = alz_sum1;
+        D(s->rhs1, k) = den_zn * D(s->w, k) +
+                        (g * rk3z + ro * D(s->rk3zo, k) + a2 * cn2z + a2 * (0.0 - pgr_z) - a2 * 0.0 - a2 * (0.0 + 0.0)) * s->dt;
+        if (I(s->imask_ph, k) == 1 && fabs(D(s->psi_zn, k) - 0.5) != 0.5)
+            D(s->rhs1, k) = D(s->rhs1, k) + g * s->dt * den_zn * D(s->w, k) * (0.0 * 1.0 + 0.0 * 1.0 + (wt - wb) * D(s->vvdz, k));
+        D(s->rk3zo, k) = rk3z;
+    }
+}
+
+/* LHSW 1d.f:1749-1809 */
+static void lhsw(orc_inst *s)
+{
+    int64_t n = s->n3m;
+    for (int64_t k = 2; k <= n; k++) {
+        double dk1 = 2.0 * s->acoef * D(s->denf_zi, k) * D(s->vf_z, k - 1) * D(s->akw, k);
+        double dk3 = 2.0 * s->acoef * D(s->denf_zi, k) * D(s->vf_z, k) * D(s->ckw, k);
+        double dk2 = -dk1 - dk3;
+        D(s->ga, k) = dk1; D(s->gc, k) = dk3; D(s->gb, k) = 1.0 + dk2; D(s->gg, k) = D(s->rhs1, k);
+    }
+    darr gam = dnew(0, n + 1);
+    trdiag3(&s->ga, &s->gb, &s->gc, &s->gg, 2, n, &gam);
+    free(gam.d);
+    for (int64_t k = 2; k <= n; k++) D(s->w, k) = D(s->gg, k) + D(s->w, k);
+}
+
+/* BC + BC_FLUX 1d.f:1812-1923 */
+static void bc(orc_inst *s)
+{
+    int64_t n = s->n3m, n3 = s->n3;
+    D(s->w, 1) = 0.0;
+    D(s->w, n3) = D(s->w, n);
+    double qq3 = 0.0 + (D(s->w, n3) - D(s->w, 1)) * s->sdx1 * s->sdy1;
+    double aa = 0.0 + 2.0 * s->xl * s->yl;
+    double ubc = -qq3 / aa;
+    D(s->w, 1) = 0.0;
+    D(s->w, n3) = D(s->w, n3) + ubc;
+}
+
+/* FSM_CHOI 1d.f:3365-3400 */
+static void fsm_choi(orc_inst *s)
+{
+    for (int64_t k = 2; k <= s->n3m; k++)
+        D(s->w, k) = D(s->w, k) + s->dtconst * ((D(s->p, k) - D(s->p, k - 1)) * D(s->vvdz, k)) * D(s->denf_zi, k);
+}
+
+/* PRESSURE_CORRECTION 1d.f:3403-3488 with DIVGS, POISSON, UCALC */
+static void pressure_correction(orc_inst *s)
+{
+    int64_t n = s->n3m, n3 = s->n3;
+    /* DIVGS 1d.f:3180-3303 */
+    for (int64_t k = 1; k <= n; k++) D(s->divgsum, k) = 0.0;
+    for (int64_t k = 1; k <= n - 1; k++) D(s->divgsum, k) = (D(s->w, k + 1) - D(s->w, k)) * D(s->ssdz, k);
+    D(s->divgsum, n) = 0.0;
+    for (int64_t k = 1; k <= n; k++) {
+        double ss = box_cell(s, &s->sfgas, k);
+        D(s->divgsum, k) = D(s->divgsum, k) - ss * (s->den_diff / (s->denp * s->denm));
+    }
+    for (int64_t k = 2; k <= n; k++) D(s->divgsum, k) = D(s->divgsum, k) * s->dtconsti;
+    /* POISSON 1d.f:3492-3578 */
+    for (int64_t k = 1; k <= n; k++) {
+        int64_t kp1 = k + 1;
+        D(s->ga, k) = D(s->vvdz, k) * D(s->ssdz, k) * D(s->denf_zi, k) * (1.0 - D(s->fixkl, k));
+        D(s->gc, k) = D(s->vvdz, kp1) * D(s->ssdz, k) * D(s->denf_zi, kp1);
+        D(s->gb, k) = -(D(s->ga, k) + D(s->gc, k));
+        D(s->gg, k) = D(s->divgsum, k);
+        D(s->pa, k) = D(s->ga, k); D(s->pb, k) = D(s->gb, k); D(s->pc, k) = D(s->gc, k); D(s->pg, k) = D(s->gg, k);
+    }
+    darr gam = dnew(0, n + 1);
+    trdiag3(&s->ga, &s->gb, &s->gc, &s->gg, 1, n, &gam);
+    free(gam.d);
+    for (int64_t k = 1; k <= n; k++) D(s->p, k) = D(s->gg, k);
+    D(s->p, 0) = D(s->p, 1);
+    D(s->p, n3) = D(s->p, n);
+    /* UCALC 1d.f:3582-3627 */
+    for (int64_t k = 2; k <= n; k++)
+        D(s->w, k) = D(s->w, k) - s->dtconst * (D(s->p, k) - D(s->p, k - 1)) * D(s->vvdz, k) * D(s->denf_zi, k);
+    D(s->w, 1) = 0.0;
+    D(s->w, n3) = D(s->w, n);
+    /* p -= p(1) 1d.f:3433-3451, ghost P(0) 3466-3470 */
+    double phiref = D(s->p, 1);
+    for (int64_t k = 1; k <= n; k++) D(s->p, k) = D(s->p, k) - phiref;
+    D(s->p, 0) = D(s->p, 1);
+}
+
+/* CONVRGE1 1d.f:3306-3362 */
+static void convrge1(orc_inst *s)
+{
+    double dvmax = 0.0;
+    for (int64_t k = 1; k <= s->n3m; k++) {
+        double d = (D(s->w, k + 1) - D(s->w, k)) * D(s->ssdz, k);
+        double ss = box_cell(s, &s->sfgas, k);
+        d = d - ss * (s->den_diff / s->denp);
+        dvmax = fmax(fabs(d), dvmax);
+    }
+    s->dvmax = dvmax;
+}
+
+/* RHSNLHS 1d.f:1444-1600 */
+static int rhsnlhs(orc_inst *s)
+{
+    int64_t n = s->n3m;
+    int st;
+    cal_continuity(s);
+    cal_cont_bc(s);
+    st = passive_scalar(s);
+    if (st) return st;
+    rhsz(s);
+    for (int64_t k = 2; k <= n; k++) D(s->rhs1, k) = D(s->rhs1, k) * D(s->denf_zi, k) - D(s->w, k);
+    lhsw(s);
+    bc(s);
+    fsm_choi(s);
+    pressure_correction(s);
+    return ORC_OK;
+}
+
+/* DETERMINE_DT 1d.f:683-817 with CFL 1325-1373 */
+static void determine_dt(orc_inst *s)
+{
+    double dt_old = s->dt, dt_source;
+    if (s->time_st == 0.0) dt_source = 1000000.0;
+    else {
+        dt_source = 1000.0;
+        for (int64_t k = 1; k <= s->n3m; k++) {
+            if (fabs(D(s->psi_cn, k) - 0.5) < 0.5) {
+                double dx1 = fmin(s->sdy1, D(s->sdz, k));
+                double t1 = 0.5 * dx1 / sqrt(s->time_st / dx1 + s->time_gv * dx1);
+                double dx2 = fmax(s->sdy1, D(s->sdz, k));
+                double t2 = 0.5 * dx2 / sqrt(s->time_st / dx2 + s->time_gv * dx2);
+                dt_source = fmin(fmin(dt_source, t1), t2);
+            }
+        }
+    }
+    double cflm = 0.0;
+    for (int64_t k = 1; k <= s->n3m; k++) {
+        double c1 = fabs(0.0 + 0.0) * (1.0 / s->sdx1);
+        double c3 = fabs(D(s->w, k) + D(s->w, k + 1)) * D(s->ssdz, k);
+        double cfli = 0.5 * (c1 + c1 + c3) * s->dt;
+        cflm = fmax(cfli, cflm);
+    }
+    s->cflm = cflm;
+    double cfl_surf = s->dt / dt_source;
+    if (s->c.idtopt == 1) {
+        double cfl_sum = 0.5 * (cflm + sqrt(cflm * cflm + 4.0 * cfl_surf));
+        s->dt = s->dt * s->c.cflmax / cfl_sum;
+    }
+    if (s->dt / dt_old > 2.0) s->dt = dt_old * 1.2;
+}
+
+/* ------------------------------------------------------------------ */
+/* initial state: MAKEFLD 1d.f:821-946, LVSINIT two_phase.f:116-458,
+ * GRID_COUPLING(1) 1d.f:143, VOL_TOT_IPHS 1d.f:147-152               */
+int orc_init(orc_inst *s)
+{
+    if (s->status) return s->status;
+    int64_t n = s->n3m, n3 = s->n3;
+    s->dt = s->c.dt0;
+    s->time = 0.0;
+    s->m = 0;
+    for (int64_t k = 1; k <= n; k++) {
+        double pbody = (D(s->zp, k) - s->c.z0) + 1.0e-12;
+        if (pbody > 0.0) { D(s->t, k) = 1.0; D(s->w, k) = 1.0; D(s->w, n3) = 1.0; }
+        else { D(s->t, k) = s->c.tslope * D(s->zp, k); D(s->t, 0) = 0.0; D(s->w, k) = 0.0; D(s->w, 1) = 0.0; }
+    }
+    /* LVSINIT seed (two_phase.f:209-252) */
+    double cri = 1.2 * fmax(fmax(s->sdxf, s->sdyf), s->sdzf);
+    dzero(&s->phi);
+    int64_t *seed = (int64_t *)malloc(sizeof(int64_t) * (size_t)(s->n3fm + 1));
+    int64_t nseed = 0;
+    for (int64_t k = 1; k <= s->n3fm; k++) {
+        double a = (D(s->zpf, k) - s->c.z0) + 1.0e-12;
+        if (fabs(a) <= cri) { seed[nseed++] = k; D(s->phi, k) = a; }
+    }
+    /* entries beyond 10*NUMS_CRI sit in layer 11 and are not scanned by
+     * the S-tube; drop planes lying wholly in that tail                  */
+    {
+        int64_t rr = s->r * s->r;
+        int64_t total = nseed * rr, cri_n = total / (M_MAX - 1);
+        int64_t scanned = cri_n * (M_MAX - 1);
+        int64_t keep = (scanned + rr - 1) / rr;
+        if (keep < nseed) nseed = keep;
+    }
+    int st = band_generation(s, seed, nseed);
+    free(seed);
+    if (st) return s->status = st;
+    for (int nl = 1; nl <= M_MAX; nl++)
+        for (int64_t q = 0; q < s->bn[nl]; q++) {
+            int64_t k = s->bk[nl][q];
+            D(s->phi, k) = (D(s->zpf, k) - s->c.z0) + 1.0e-12;
+        }
+    st = cal_psif(s, 1, M_MAX);
+    if (st) return s->status = st;
+    st = cal_psif_mtp(s, 1, M_MAX, &s->psif_mtp);
+    if (st) return s->status = st;
+    s->vol_tot_ori = vol_sum(s, &s->psif_mtp);
+    for (int nl = 1; nl <= M_MAX; nl++)
+        for (int64_t q = 0; q < s->bn[nl]; q++) s->alphi[nl][q] = D(s->phi, s->bk[nl][q]);
+    grid_coupling(s, 1);
+    s->vol_tot_iphs = s->vol_tot_ori;
+    return ORC_OK;
+}
+
+static int check_finite(orc_inst *s)
+{
+    for (int64_t k = 0; k <= s->n3; k++)
+        if (!isfinite(D(s->w, k)) || !isfinite(D(s->t, k)) || !isfinite(D(s->p, k))) return ORC_NONFINITE;
+    return ORC_OK;
+}
+
+/* MAIN time loop 1d.f:196-295 */
+int orc_step(orc_inst *s, int64_t nsteps)
+{
+    if (s->status) return s->status;
+    for (int64_t it = 0; it < nsteps; it++) {
+        s->m += 1;
+        determine_dt(s);
+        for (int64_t k = 0; k <= s->n3; k++) D(s->rk3zo, k) = 0.0;
+        for (int msub = 1; msub <= 3; msub++) {
+            s->msub = msub;
+            s->alpha = 0.5 * (s->gamma[msub - 1] + s->ro[msub - 1]);
+            s->acoef = s->alpha * s->dt;
+            s->dtconst = 2.0 * s->alpha * s->dt;
+            s->dtconsti = 1.0 / s->dtconst;
+            int st = rhsnlhs(s);
+            if (st) return s->status = st;
+            convrge1(s);
+            st = transport(s);
+            if (st) return s->status = st;
+            grid_coupling(s, 0);
+        }
+        s->time = s->time + s->dt;
+        int st = check_finite(s);
+        if (st) return s->status = st;
+    }
+    return ORC_OK;
+}
+
+int64_t orc_get(orc_inst *s, int field, double *out, int64_t cap)
+{
+    int64_t m = 0;
+    switch (field) {
+    case ORC_F_W: case ORC_F_P: case ORC_F_T: case ORC_F_PSI_CN: case ORC_F_PSI_ZN: case ORC_F_WO: {
+        darr *a = field == ORC_F_W ? &s->w : field == ORC_F_P ? &s->p : field == ORC_F_T ? &s->t :
+                  field == ORC_F_PSI_CN ? &s->psi_cn : field == ORC_F_PSI_ZN ? &s->psi_zn : &s->wo;
+        for (int64_t k = 0; k <= s->n3 && m < cap; k++) out[m++] = D(*a, k);
+        return m;
+    }
+    case ORC_F_PSIF: case ORC_F_SFGAS:
+    {
+        darr *a = (field == ORC_F_PSIF) ? &s->psif : &s->sfgas;
+        for (int64_t k = 1; k <= s->n3fm && m < cap; k++) out[m++] = D(*a, k);
+        return m;
+    }
+    case ORC_F_PHI:
+        phi_from_band(s);
+        for (int64_t k = 1; k <= s->n3fm && m < cap; k++) out[m++] = D(s->phi, k);
+        return m;
+    case ORC_F_BAND:
+        for (int nl = 1; nl <= M_MAX; nl++) {
+            if (m < cap) out[m++] = (double)s->bn[nl];
+            for (int64_t q = 0; q < s->bn[nl] && m < cap; q++) out[m++] = (double)s->bk[nl][q];
+        }
+        return m;
+    case ORC_F_ALPHI: case ORC_F_MFLVS:
+        for (int nl = 1; nl <= M_MAX; nl++)
+            for (int64_t q = 0; q < s->bn[nl] && m < cap; q++) out[m++] = field == ORC_F_ALPHI ? s->alphi[nl][q] : s->mflvs[nl][q];
+        return m;
+    }
+    return 0;
+}
+
+void orc_scalars(orc_inst *s, double *out)
+{
+    for (int i = 0; i < ORC_NSCAL; i++) out[i] = 0.0;
+    out[ORC_S_TIME] = s->time; out[ORC_S_DT] = s->dt; out[ORC_S_STEP] = (double)s->m;
+    out[ORC_S_PLACE] = s->place; out[ORC_S_DVMAX] = s->dvmax; out[ORC_S_CFLM] = s->cflm;
+    out[ORC_S_VOL_ORI] = s->vol_tot_ori; out[ORC_S_VOL_IPHS] = s->vol_tot_iphs;
+    /* wall heat flux (build-defined observable): k_v dT/dz at the wall face */
+    out[ORC_S_QWALL] = s->tcm * (D(s->t, 1) - D(s->t, 0)) * D(s->vvdz, 1);
+    out[ORC_S_STATUS] = (double)s->status;
+    out[ORC_S_SUBCYCLE_WARN] = s->n_subcycle_warn; out[ORC_S_MASS_WARN] = s->n_mass_warn;
+    out[ORC_S_EXTRAP_HALVINGS] = s->n_halvings; out[ORC_S_EXTRAP_ITERS] = s->n_extrap_iters;
+    out[ORC_S_REINIT_CALLS] = s->n_reinit; out[ORC_S_BANDGEN_CALLS] = s->n_bandgen;
+    out[ORC_S_MASSCOR_ITERS] = s->n_masscor;
+}
+
+int64_t orc_last_poisson(orc_inst *s, double *a, double *b, double *c, double *g, double *p)
+{
+    for (int64_t k = 1; k <= s->n3m; k++) {
+        a[k - 1] = D(s->pa, k); b[k - 1] = D(s->pb, k); c[k - 1] = D(s->pc, k); g[k - 1] = D(s->pg, k);
+        if (p) p[k - 1] = D(s->p, k);
+    }
+    return s->n3m;
+}

@@ -1,0 +1,88 @@
+/*
+ * lica_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Serial FP64 CPU restatement of the per-step path of the 1-D film-boiling
+ * solver (reference files 1d.f, two_phase.f, chphase.f, grid.f).  It is the
+ * parity checker for the HIP path and the CPU baseline of bench.py; nothing
+ * in lica_boiling1d_amd/ may link, load or call it.
+ *
+ * The reference stores the refined level-set grid as (r x r x rN) with x/y
+ * columns that stay identical; this restatement keeps one z column and keeps
+ * the narrow-band lists as ordered lists of z-planes (each plane standing for
+ * the r*r list entries the reference holds for it).
+ */
+#ifndef LICA_ORACLE_H
+#define LICA_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int64_t n;          /* NS cells in z (N3M)                       */
+    int64_t r;          /* level-set refinement factor (N3F_BD)      */
+    double zl;          /* domain height (grid.in ZL)                */
+    double re, visr, denr;            /* lica.in line 21           */
+    double surf_j, fr;                /* lica.in line 23           */
+    double prm, scr, tcr;             /* lica.in line 25           */
+    double tsat, ja;                  /* lica.in line 27           */
+    double dt0, cflmax;               /* lica.in line 6            */
+    int64_t idtopt;
+    double z0;          /* initial interface height (ASSIGN_LVS)     */
+    double tslope;      /* initial vapour temperature slope (MAKEFLD) */
+} orc_case;
+
+typedef struct orc_inst orc_inst;
+
+/* status codes */
+enum {
+    ORC_OK = 0,
+    ORC_BAND_OVERFLOW = 1,       /* two_phase.f:361,1989 */
+    ORC_PSIF_BUG = 2,            /* two_phase.f:2905 */
+    ORC_GRID = 3,                /* 1d.f:4761 / coupling search failure */
+    ORC_EXTRAP_NOT_CONVERGED = 4,/* chphase.f:676-678 (bounded here) */
+    ORC_BAND_CAPACITY = 5,       /* more z-planes per layer than stored */
+    ORC_NONFINITE = 6
+};
+
+/* field ids for orc_get */
+enum {
+    ORC_F_W = 0, ORC_F_P = 1, ORC_F_T = 2, ORC_F_PSI_CN = 3, ORC_F_PSI_ZN = 4,
+    ORC_F_PSIF = 5, ORC_F_PHI = 6, ORC_F_SFGAS = 7, ORC_F_WO = 8,
+    ORC_F_BAND = 9,   /* per layer: count then planes (int as double) */
+    ORC_F_ALPHI = 10, ORC_F_MFLVS = 11
+};
+
+/* scalar ids for orc_scalars (array of ORC_NSCAL doubles) */
+enum {
+    ORC_S_TIME = 0, ORC_S_DT = 1, ORC_S_STEP = 2, ORC_S_PLACE = 3,
+    ORC_S_DVMAX = 4, ORC_S_CFLM = 5, ORC_S_VOL_ORI = 6, ORC_S_VOL_IPHS = 7,
+    ORC_S_QWALL = 8, ORC_S_STATUS = 9, ORC_S_SUBCYCLE_WARN = 10,
+    ORC_S_MASS_WARN = 11, ORC_S_EXTRAP_HALVINGS = 12, ORC_S_EXTRAP_ITERS = 13,
+    ORC_S_REINIT_CALLS = 14, ORC_S_BANDGEN_CALLS = 15, ORC_S_MASSCOR_ITERS = 16,
+    ORC_NSCAL = 24
+};
+
+orc_inst *orc_new(const orc_case *c);
+void orc_free(orc_inst *s);
+int orc_init(orc_inst *s);
+int orc_step(orc_inst *s, int64_t nsteps);
+int64_t orc_get(orc_inst *s, int field, double *out, int64_t cap);
+void orc_scalars(orc_inst *s, double *out);
+int64_t orc_n(orc_inst *s);
+/* the Poisson inputs (a, b, c, g of POISSON) of the last substep */
+int64_t orc_last_poisson(orc_inst *s, double *a, double *b, double *c, double *g, double *p);
+
+/* unit-level restatements used by the tests */
+void orc_trdiag3(int64_t n, const double *a, const double *b, const double *c,
+                 double *g);
+double orc_hjweno(double a, double b, double c, double d);
+double orc_godunov(double asign, double p, double m);
+double orc_psif_cell(double phi, double d3);
+void orc_grid_nodes(int64_t n, double zl, double *z);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
