@@ -1,0 +1,16 @@
+"""MI355X (gfx950) implementation of lica_boiling1d's per-step hot path.
+
+The compute runs in ``liblica_mi355x.so`` (hand-written HIP kernels behind
+the C ABI declared in ``include/lica_mi355x.h``); this package is the host
+mirror of the reference's call sequence on top of that ABI.  There is no
+CPU implementation: without the library or a GPU every entry point raises.
+"""
+from ._abi import (CaseDesc, LicaError, FIELDS, SCALARS, POISSON_THOMAS, POISSON_DIRECT,
+                   POISSON_MG64, POISSON_MIXED, POISSON_LEARNED, load, status_string)
+from .solver import Batch, Context, parse_deck, poisson
+from .workloads import deck_case, single_large_case, batched_cases
+
+__all__ = ["CaseDesc", "LicaError", "FIELDS", "SCALARS", "POISSON_THOMAS", "POISSON_DIRECT",
+           "POISSON_MG64", "POISSON_MIXED", "POISSON_LEARNED", "load", "status_string",
+           "Batch", "Context", "parse_deck", "poisson", "deck_case", "single_large_case",
+           "batched_cases"]

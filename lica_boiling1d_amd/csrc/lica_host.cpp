@@ -23,6 +23,7 @@
 
 #ifdef LICA_EMUL
 int lica_emul_nthreads = 256;
+int lica_emul_depth = 0;
 typedef void *hipStream_t;
 #else
 #include <hip/hip_runtime.h>

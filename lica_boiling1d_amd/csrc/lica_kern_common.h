@@ -128,12 +128,12 @@ DEV double box_avg(const double *zf, double ssdzf, F f, double z1, double z2, in
 // ---------------------------------------------------------------------
 // Block-level tridiagonal solve of rows l1..l2: a(j)x(j-1)+b(j)x(j)+c(j)x(j+1)=g(j),
 // a(l1) and c(l2) ignored; the solution overwrites g.  Mode 0 is TRDIAG3
-// (1d.f:3783-3812) on one thread; mode 1 is a partitioned elimination
-// (chunks reduced to their end rows) with parallel cyclic reduction of the
-// end-row system in LDS.
+// (1d.f:3783-3812) on one thread; mode 1 is a partitioned elimination: every
+// thread reduces a chunk of rows to its two end rows, the end-row system is
+// solved in LDS, and the chunks' interior rows are recovered in parallel.
 
 struct TriSh {
-    double ra[2][2 * MAXC], rb[2][2 * MAXC], rc[2][2 * MAXC], rg[2][2 * MAXC];
+    double ra[1][2 * MAXC], rb[1][2 * MAXC], rc[1][2 * MAXC], rg[1][2 * MAXC];
 };
 
 DEV void tri_solve(double *a, double *b, double *c, double *g, int l1, int l2, int mode, TriSh &ts)
@@ -203,29 +203,29 @@ DEV void tri_solve(double *a, double *b, double *c, double *g, int l1, int l2, i
         ts.rg[0][2 * ch + 1] = g[e];
     }
     PFOR_END
-    // parallel cyclic reduction of the end-row system
-    int cur = 0;
-    for (int stride = 1; stride < m2; stride *= 2) {
-        PFOR(i, 0, m2) {
-            double ai = ts.ra[cur][i], bi = ts.rb[cur][i], ci = ts.rc[cur][i], gi = ts.rg[cur][i];
-            double am = 0.0, bm = 1.0, cm = 0.0, gm = 0.0, ap = 0.0, bp = 1.0, cp = 0.0, gp = 0.0;
-            if (i - stride >= 0) { am = ts.ra[cur][i - stride]; bm = ts.rb[cur][i - stride]; cm = ts.rc[cur][i - stride]; gm = ts.rg[cur][i - stride]; }
-            if (i + stride < m2) { ap = ts.ra[cur][i + stride]; bp = ts.rb[cur][i + stride]; cp = ts.rc[cur][i + stride]; gp = ts.rg[cur][i + stride]; }
-            double k1 = ai / bm, k2 = ci / bp;
-            ts.ra[1 - cur][i] = -am * k1;
-            ts.rc[1 - cur][i] = -cp * k2;
-            ts.rb[1 - cur][i] = bi - cm * k1 - ap * k2;
-            ts.rg[1 - cur][i] = gi - gm * k1 - gp * k2;
+    // the end-row system (2*nc rows, tridiagonal in the order s0,e0,s1,e1,...)
+    // by Gaussian elimination in LDS: the Schur complement of a diagonally
+    // dominant matrix stays so, and plain elimination keeps the accuracy of
+    // TRDIAG3 where cyclic reduction loses digits on Laplacian-like rows
+    SERIAL {
+        double *ra = ts.ra[0], *rb = ts.rb[0], *rc = ts.rc[0], *rg = ts.rg[0];
+        double bet = 1.0 / rb[0];
+        rg[0] = rg[0] * bet;
+        for (int j = 1; j < m2; j++) {
+            double gam = rc[j - 1] * bet;
+            bet = 1.0 / (rb[j] - ra[j] * gam);
+            rg[j] = (rg[j] - ra[j] * rg[j - 1]) * bet;
+            ra[j] = gam;
         }
-        PFOR_END
-        cur = 1 - cur;
+        for (int j = m2 - 2; j >= 0; j--) rg[j] = rg[j] - ra[j + 1] * rg[j + 1];
     }
+    SERIAL_END
     PFOR(ch, 0, nc) {
         int s = l1 + ch * base + (ch < rem ? ch : rem);
         int L = base + (ch < rem ? 1 : 0);
         int e = s + L - 1;
-        double xs = ts.rg[cur][2 * ch] / ts.rb[cur][2 * ch];
-        double xe = ts.rg[cur][2 * ch + 1] / ts.rb[cur][2 * ch + 1];
+        double xs = ts.rg[0][2 * ch];
+        double xe = ts.rg[0][2 * ch + 1];
         for (int j = 1; j <= L - 2; j++) {
             int i = s + j;
             g[i] = g[i] - a[i] * xs - c[i] * xe;

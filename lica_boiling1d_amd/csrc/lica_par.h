@@ -18,13 +18,30 @@
 #include <cstring>
 #include <algorithm>
 using std::isfinite;
+#include <cstdio>
+#include <cstdlib>
 #define DEV inline
 #define HOSTDEV inline
 #define SHARED static
-#define BARRIER() ((void)0)
-#define PFOR(i, lo, hi) for (int i = (lo); i < (hi); ++i)
+// Every barrier-bearing primitive must be reached by the whole workgroup:
+// the logic-check build aborts when one is entered from inside a PFOR body
+// or a SERIAL section (on the GPU that would be a barrier in divergent code).
+extern int lica_emul_depth;
+inline void lica_emul_check()
+{
+    if (lica_emul_depth) {
+        fprintf(stderr, "lica: workgroup barrier inside divergent code\n");
+        abort();
+    }
+}
+struct EmulScope {
+    EmulScope() { lica_emul_check(); ++lica_emul_depth; }
+    ~EmulScope() { --lica_emul_depth; }
+};
+#define BARRIER() lica_emul_check()
+#define PFOR(i, lo, hi) if (EmulScope lica_scope_{}; true) for (int i = (lo); i < (hi); ++i)
 #define PFOR_END
-#define SERIAL if (1)
+#define SERIAL if (EmulScope lica_scope_{}; true)
 #define SERIAL_END
 extern int lica_emul_nthreads;
 #define NTHREADS (lica_emul_nthreads)
@@ -47,6 +64,7 @@ DEV double red_max(int lo, int hi, double init, F f, double *scr)
 {
 #ifdef LICA_EMUL
     (void)scr;
+    lica_emul_check();
     double m = init;
     for (int i = lo; i < hi; ++i) m = fmax(m, f(i));
     return m;
@@ -69,6 +87,7 @@ DEV double red_min(int lo, int hi, double init, F f, double *scr)
 {
 #ifdef LICA_EMUL
     (void)scr;
+    lica_emul_check();
     double m = init;
     for (int i = lo; i < hi; ++i) m = fmin(m, f(i));
     return m;

@@ -1,0 +1,178 @@
+"""Host mirror of the reference's driver calls over the C ABI.
+
+``Context`` follows MAIN's call sequence for one case (1d.f:70-295):
+``determine_dt`` (DETERMINE_DT 1d.f:683), ``rhsnlhs`` (RHSNLHS 1d.f:1444,
+which contains the POISSON projection 1d.f:3403-3488), ``convrge1``
+(CONVRGE1 1d.f:3306), ``transport`` (TRANSPORT two_phase.f:701 followed by
+GRID_COUPLING(0) two_phase.f:4863, as at 1d.f:242-244).  ``Batch`` runs the
+same loop for many independent cases in one launch per kernel.
+Non-zero statuses raise ``LicaError`` (the reference STOPs).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from ._abi import CaseDesc, LicaError, FIELDS, SCALARS, NSCAL
+
+
+def _check(rc, where):
+    if rc != 0:
+        raise LicaError(rc, where)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_abi.P_dbl)
+
+
+def parse_deck(lica_in, grid_in, lib=None):
+    """lica.in + grid.in -> CaseDesc (INPUT_SIMULATION 1d.f:323-492, grid.f:29-53)."""
+    lib = lib if lib is not None else _abi.load()
+    d = CaseDesc()
+    _check(lib.lica_parse_deck(str(lica_in).encode(), str(grid_in).encode(), ctypes.byref(d)), "lica_parse_deck")
+    return d
+
+
+class _Handle:
+    def __init__(self, lib=None):
+        self._lib = lib if lib is not None else _abi.load()
+        self._h = ctypes.c_void_p()
+
+    def close(self):
+        if self._h:
+            self._lib.lica_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # getters shared by Batch and Context
+    def field(self, name, inst=0):
+        f = FIELDS[name] if isinstance(name, str) else int(name)
+        n = int(self.cases[inst].n)
+        r = int(self.cases[inst].r)
+        cap = max(n + 2, r * n, 12 * 11 * 17) + 16
+        out = np.zeros(cap)
+        cnt = ctypes.c_int64()
+        _check(self._lib.lica_batch_get_field(self._h, f, inst, _dptr(out), cap, ctypes.byref(cnt)), "get_field")
+        return out[:cnt.value].copy()
+
+    def scalars(self, inst=0):
+        out = np.zeros(NSCAL)
+        _check(self._lib.lica_batch_get_scalars(self._h, inst, _dptr(out)), "get_scalars")
+        return {k: float(out[i]) for i, k in enumerate(SCALARS)}
+
+    def scalar_array(self, inst=0):
+        out = np.zeros(NSCAL)
+        _check(self._lib.lica_batch_get_scalars(self._h, inst, _dptr(out)), "get_scalars")
+        return out
+
+
+class Batch(_Handle):
+    """Many independent boiling cases advanced together on the GPU."""
+
+    def __init__(self, cases, init=True, lib=None):
+        super().__init__(lib)
+        self.cases = list(cases)
+        arr = (CaseDesc * len(self.cases))(*self.cases)
+        _check(self._lib.lica_batch_create(len(self.cases), arr, ctypes.byref(self._h)), "lica_batch_create")
+        if init:
+            self.init()
+
+    def __len__(self):
+        return len(self.cases)
+
+    def init(self):
+        _check(self._lib.lica_batch_init(self._h), "lica_batch_init")
+
+    def run(self, nsteps):
+        _check(self._lib.lica_batch_run(self._h, int(nsteps)), "lica_batch_run")
+
+    def set_poisson(self, mode, rtol=1.0e-10, max_cycles=100):
+        _check(self._lib.lica_batch_set_poisson(self._h, int(mode), float(rtol), int(max_cycles)), "set_poisson")
+
+    def set_block(self, threads):
+        _check(self._lib.lica_batch_set_block(self._h, int(threads)), "set_block")
+
+    def use_graph(self, on=True):
+        _check(self._lib.lica_batch_use_graph(self._h, 1 if on else 0), "use_graph")
+
+    def set_timing(self, on=True):
+        _check(self._lib.lica_batch_set_timing(self._h, 1 if on else 0), "set_timing")
+
+    def sync(self):
+        _check(self._lib.lica_batch_sync(self._h), "sync")
+
+    def status(self):
+        bad = ctypes.c_int64(-1)
+        rc = self._lib.lica_batch_status(self._h, ctypes.byref(bad))
+        return rc, bad.value
+
+    def timings(self):
+        """GPU ms per kernel class of the last run (timing mode) and the wall ms."""
+        out = np.zeros(16)
+        m = self._lib.lica_batch_last_timings(self._h, _dptr(out), 16)
+        names = ["init", "dt", "cont_energy", "temp_grad", "momentum", "poisson", "ucalc", "transport", "poisson_mg"]
+        res = {names[i]: float(out[i]) for i in range(min(m - 1, len(names)))}
+        res["wall"] = float(out[m - 1])
+        return res
+
+
+class Context(_Handle):
+    """One case driven through the reference's subroutine sequence."""
+
+    def __init__(self, lica_in, grid_in, lib=None):
+        super().__init__(lib)
+        self.cases = [parse_deck(lica_in, grid_in, lib=self._lib)]
+        _check(self._lib.lica_init(ctypes.byref(self._h), str(lica_in).encode(), str(grid_in).encode()), "lica_init")
+        self.n = int(self.cases[0].n)
+
+    def step(self, nsteps=1):
+        _check(self._lib.lica_step(self._h, int(nsteps)), "lica_step")
+
+    def determine_dt(self):
+        cflm, dt = ctypes.c_double(), ctypes.c_double()
+        _check(self._lib.lica_determine_dt(self._h, ctypes.byref(cflm), ctypes.byref(dt)), "lica_determine_dt")
+        return cflm.value, dt.value
+
+    def rhsnlhs(self, msub):
+        bufs = [np.zeros(self.n + 2) for _ in range(4)]
+        _check(self._lib.lica_rhsnlhs(self._h, int(msub), *[_dptr(b) for b in bufs]), "lica_rhsnlhs")
+        return tuple(bufs)
+
+    def convrge1(self):
+        d = ctypes.c_double()
+        _check(self._lib.lica_convrge1(self._h, ctypes.byref(d)), "lica_convrge1")
+        return d.value
+
+    def transport(self):
+        _check(self._lib.lica_transport(self._h), "lica_transport")
+
+    def grid_coupling(self, ini_psi=0):
+        zn, cn = np.zeros(self.n + 2), np.zeros(self.n + 2)
+        _check(self._lib.lica_grid_coupling(self._h, int(ini_psi), _dptr(zn), _dptr(cn)), "lica_grid_coupling")
+        return zn, cn
+
+    def end_step(self):
+        _check(self._lib.lica_end_step(self._h), "lica_end_step")
+
+
+def poisson(a, b, c, g, mode=_abi.POISSON_THOMAS, rtol=1.0e-10, max_cycles=100, lib=None):
+    """Batched POISSON tridiagonal systems (1d.f:3535-3564); arrays (nsys, n)."""
+    lib = lib if lib is not None else _abi.load()
+    a, b, c, g = (np.ascontiguousarray(np.atleast_2d(x), dtype=np.float64) for x in (a, b, c, g))
+    nsys, n = g.shape
+    p = np.zeros_like(g)
+    cyc = ctypes.c_int64()
+    _check(lib.lica_poisson(nsys, n, _dptr(a), _dptr(b), _dptr(c), _dptr(g), _dptr(p), int(mode),
+                            float(rtol), int(max_cycles), ctypes.byref(cyc)), "lica_poisson")
+    return p

@@ -1,0 +1,143 @@
+"""ctypes binding of oracle/lica_oracle.h (TEST INFRASTRUCTURE ONLY)."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(BUILD, "liblica_oracle.so")
+
+ORC_FIELDS = {"W": 0, "P": 1, "T": 2, "PSI_CN": 3, "PSI_ZN": 4, "PSIF": 5, "PHI": 6,
+              "SFGAS": 7, "WO": 8, "BAND": 9, "ALPHI": 10, "MFLVS": 11}
+NSCAL = 24
+c_i64, c_dbl = ctypes.c_int64, ctypes.c_double
+P_dbl = ctypes.POINTER(ctypes.c_double)
+
+
+class OrcCase(ctypes.Structure):
+    _fields_ = [("n", c_i64), ("r", c_i64), ("zl", c_dbl),
+                ("re", c_dbl), ("visr", c_dbl), ("denr", c_dbl),
+                ("surf_j", c_dbl), ("fr", c_dbl),
+                ("prm", c_dbl), ("scr", c_dbl), ("tcr", c_dbl),
+                ("tsat", c_dbl), ("ja", c_dbl),
+                ("dt0", c_dbl), ("cflmax", c_dbl),
+                ("idtopt", c_i64),
+                ("z0", c_dbl), ("tslope", c_dbl)]
+
+
+def build(force=False, extra=()):
+    """gcc -O2 -ffp-contract=off: every operation rounds on its own, in the
+    serial order the reference's loops run."""
+    os.makedirs(BUILD, exist_ok=True)
+    src = os.path.join(HERE, "lica_oracle.c")
+    if not force and os.path.exists(LIB) and os.path.getmtime(LIB) >= max(
+            os.path.getmtime(src), os.path.getmtime(os.path.join(HERE, "lica_oracle.h"))):
+        return LIB
+    tmp = LIB + ".tmp"
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-std=c11", "-Wall", *extra, src,
+           "-o", tmp, "-lm"]
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        lib = ctypes.CDLL(LIB)
+        V = ctypes.c_void_p
+        lib.orc_new.restype = V
+        lib.orc_new.argtypes = [ctypes.POINTER(OrcCase)]
+        lib.orc_free.argtypes = [V]
+        lib.orc_init.argtypes = [V]
+        lib.orc_init.restype = ctypes.c_int
+        lib.orc_step.argtypes = [V, c_i64]
+        lib.orc_step.restype = ctypes.c_int
+        lib.orc_get.argtypes = [V, ctypes.c_int, P_dbl, c_i64]
+        lib.orc_get.restype = c_i64
+        lib.orc_scalars.argtypes = [V, P_dbl]
+        lib.orc_n.argtypes = [V]
+        lib.orc_n.restype = c_i64
+        lib.orc_last_poisson.argtypes = [V, P_dbl, P_dbl, P_dbl, P_dbl, P_dbl]
+        lib.orc_last_poisson.restype = c_i64
+        lib.orc_trdiag3.argtypes = [c_i64, P_dbl, P_dbl, P_dbl, P_dbl]
+        lib.orc_hjweno.argtypes = [c_dbl] * 4
+        lib.orc_hjweno.restype = c_dbl
+        lib.orc_godunov.argtypes = [c_dbl] * 3
+        lib.orc_godunov.restype = c_dbl
+        lib.orc_psif_cell.argtypes = [c_dbl, c_dbl]
+        lib.orc_psif_cell.restype = c_dbl
+        lib.orc_grid_nodes.argtypes = [c_i64, c_dbl, P_dbl]
+        _lib = lib
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(P_dbl)
+
+
+class Oracle:
+    """One case of the serial FP64 restatement."""
+
+    def __init__(self, case, init=True):
+        self.lib = load()
+        oc = OrcCase()
+        for f, _ in OrcCase._fields_:
+            setattr(oc, f, getattr(case, f))
+        self.case = oc
+        self.h = self.lib.orc_new(ctypes.byref(oc))
+        self.n = int(case.n)
+        self.r = int(case.r)
+        self.init_status = self.lib.orc_init(self.h) if init else None
+
+    def close(self):
+        if self.h:
+            self.lib.orc_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, nsteps=1):
+        return self.lib.orc_step(self.h, int(nsteps))
+
+    def field(self, name):
+        f = ORC_FIELDS[name] if isinstance(name, str) else int(name)
+        cap = max(self.n + 2, self.r * self.n, 12 * 11 * 17) + 16
+        out = np.zeros(cap)
+        m = self.lib.orc_get(self.h, f, _p(out), cap)
+        return out[:m].copy()
+
+    def scalar_array(self):
+        out = np.zeros(NSCAL)
+        self.lib.orc_scalars(self.h, _p(out))
+        return out
+
+    def last_poisson(self):
+        n = self.n
+        a, b, c, g, p = (np.zeros(n) for _ in range(5))
+        self.lib.orc_last_poisson(self.h, _p(a), _p(b), _p(c), _p(g), _p(p))
+        return a, b, c, g, p
+
+
+def trdiag3(a, b, c, g):
+    a, b, c = (np.ascontiguousarray(x, dtype=np.float64).copy() for x in (a, b, c))
+    g = np.ascontiguousarray(g, dtype=np.float64).copy()
+    load().orc_trdiag3(len(g), _p(a), _p(b), _p(c), _p(g))
+    return g
+
+
+def grid_nodes(n, zl):
+    z = np.zeros(n + 1)
+    load().orc_grid_nodes(n, zl, _p(z))
+    return z

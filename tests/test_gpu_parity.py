@@ -1,0 +1,150 @@
+"""GPU parity: the gfx950 library against the FP64 oracle.
+
+In Thomas mode every kernel evaluates the reference's operations in the
+reference's order (FP contraction off, IEEE division and square root), so
+the bar is bit-for-bit equality of every field and scalar.  The partitioned
+(DIRECT) mode changes the rounding of the tridiagonal solves only; its bar is
+a relative difference of at most 1e-10 per field.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import lica_boiling1d_amd as L
+from oracle import Oracle
+from parity_util import exact_mismatches, max_rel
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def test_deck_case_200_steps_bit_exact():
+    """The shipped deck (N=256, r=2), 200 steps, checked every 50 steps."""
+    case = L.deck_case()
+    _say("deck case N=256 r=2, 200 steps")
+    b, o = L.Batch([case]), Oracle(case)
+    assert o.init_status == 0
+    assert exact_mismatches(b, 0, o) == []
+    for _ in range(4):
+        b.run(50)
+        assert o.step(50) == 0
+        assert exact_mismatches(b, 0, o) == []
+    s = b.scalars()
+    assert s["STEP"] == 200 and s["STATUS"] == 0
+    assert s["PLACE"] > case.z0   # the vapour film grows
+
+
+@pytest.mark.parametrize("n,r,steps", [(64, 1, 40), (100, 2, 40), (512, 3, 20), (1024, 4, 20)])
+def test_sizes_bit_exact(n, r, steps):
+    _say(f"size case n={n} r={r} steps={steps}")
+    case = L.deck_case(n, r)
+    b, o = L.Batch([case]), Oracle(case)
+    b.run(steps)
+    assert o.step(steps) == 0
+    assert exact_mismatches(b, 0, o) == []
+
+
+def test_single_large_case_bit_exact():
+    """N=8192, r=4 with the seeded film perturbation, 10 steps."""
+    case = L.single_large_case(seed=1)
+    _say("single large case N=8192 r=4, 10 steps")
+    b, o = L.Batch([case]), Oracle(case)
+    b.run(10)
+    assert o.step(10) == 0
+    assert exact_mismatches(b, 0, o) == []
+
+
+def test_heterogeneous_batch_bit_exact():
+    """12 cases of different N, superheat and film thickness in one batch."""
+    cases = L.batched_cases(12, seed=7, nmin=128, nmax=1024)
+    _say("heterogeneous batch of 12, 15 steps")
+    b = L.Batch(cases)
+    b.run(15)
+    for i, c in enumerate(cases):
+        o = Oracle(c)
+        assert o.step(15) == 0
+        assert exact_mismatches(b, i, o) == [], f"case {i} n={c.n}"
+
+
+def test_graph_replay_equals_stream_launches():
+    cases = L.batched_cases(6, seed=3, nmin=128, nmax=512)
+    _say("graph vs stream, 6 cases, 3 x 4 steps")
+    a, g = L.Batch(cases), L.Batch(cases)
+    g.use_graph(True)
+    for _ in range(3):
+        a.run(4)
+        g.run(4)
+        for i in range(len(cases)):
+            for f in ("W", "P", "T", "PSIF", "PHI"):
+                assert np.array_equal(a.field(f, i), g.field(f, i))
+            assert np.array_equal(a.scalar_array(i), g.scalar_array(i))
+
+
+def test_block_size_does_not_change_results():
+    case = L.deck_case(300, 2)
+    _say("block sizes 64/256/1024, 10 steps")
+    ref = None
+    for t in (64, 256, 1024):
+        b = L.Batch([case])
+        b.set_block(t)
+        b.run(10)
+        cur = [b.field(f) for f in ("W", "P", "T", "PSIF", "PHI")]
+        if ref is None:
+            ref = cur
+        else:
+            assert all(np.array_equal(x, y) for x, y in zip(ref, cur))
+
+
+def test_direct_mode_within_1e10():
+    case = L.deck_case()
+    _say("direct (partitioned) tridiagonal mode, 50 steps")
+    b, o = L.Batch([case]), Oracle(case)
+    b.set_poisson(L.POISSON_DIRECT)
+    b.run(50)
+    assert o.step(50) == 0
+    rel = max_rel(b, 0, o)
+    assert all(v <= 1e-10 for v in rel.values()), rel
+    assert abs(b.scalars()["PLACE"] - o.scalar_array()[3]) <= 1e-10 * o.scalar_array()[3]
+
+
+def test_reference_call_sequence_matches_step():
+    """determine_dt / rhsnlhs / convrge1 / transport as MAIN calls them."""
+    _say("single-case call sequence from the deck files")
+    lin, gin = os.path.join(GOLDEN, "lica.in"), os.path.join(GOLDEN, "grid.in")
+    ctx = L.Context(lin, gin)
+    o = Oracle(L.deck_case())
+    for _ in range(3):
+        ctx.determine_dt()
+        for msub in (1, 2, 3):
+            w, p, zn, cn = ctx.rhsnlhs(msub)
+            ctx.convrge1()
+            ctx.transport()
+            zn2, cn2 = ctx.grid_coupling(0)
+        ctx.end_step()
+    assert o.step(3) == 0
+    assert np.array_equal(w, o.field("W")) and np.array_equal(p, o.field("P"))
+    assert np.array_equal(zn2, o.field("PSI_ZN")) and np.array_equal(cn2, o.field("PSI_CN"))
+    assert exact_mismatches(ctx, 0, o) == []
+
+
+@pytest.mark.parametrize("mode", [L.POISSON_THOMAS, L.POISSON_DIRECT])
+def test_batched_poisson_systems(mode):
+    """lica_poisson against a dense solve of the same POISSON matrices."""
+    _say(f"batched poisson mode {mode}")
+    rng = np.random.default_rng(11)
+    nsys, n = 37, 700
+    a = -rng.uniform(0.5, 1.5, (nsys, n))
+    c = -rng.uniform(0.5, 1.5, (nsys, n))
+    a[:, 0] = 0.0
+    b = -(a + c) - rng.uniform(0.01, 0.1, (nsys, n))
+    g = rng.standard_normal((nsys, n))
+    p = L.poisson(a, b, c, g, mode=mode)
+    for s in range(nsys):
+        A = np.diag(b[s]) + np.diag(a[s, 1:], -1) + np.diag(c[s, :-1], 1)
+        x = np.linalg.solve(A, g[s])
+        assert np.max(np.abs(p[s] - x)) <= 1e-10 * np.max(np.abs(x))
