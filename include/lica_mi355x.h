@@ -112,6 +112,8 @@ int lica_batch_sync(lica_batch *b);
 int lica_batch_get_field(lica_batch *b, int64_t field, int64_t inst, double *out, int64_t cap, int64_t *count);
 int lica_batch_get_scalars(lica_batch *b, int64_t inst, double *out);
 int lica_batch_status(lica_batch *b, int64_t *first_bad_inst);
+/* per case: status word and completed time steps (arrays of lica_batch_size) */
+int lica_batch_get_progress(lica_batch *b, int64_t *status_out, int64_t *steps_out);
 int64_t lica_batch_size(lica_batch *b);
 /* GPU time of the last lica_batch_run, per kernel class (ms); returns count */
 int lica_batch_last_timings(lica_batch *b, double *ms_out, int64_t cap);

@@ -85,6 +85,7 @@ def _declare(lib):
         "lica_batch_get_scalars": (ctypes.c_int, [V, c_i64, P_dbl]),
         "lica_batch_status": (ctypes.c_int, [V, P_i64]),
         "lica_batch_size": (c_i64, [V]),
+        "lica_batch_get_progress": (ctypes.c_int, [V, P_i64, P_i64]),
         "lica_batch_last_timings": (ctypes.c_int, [V, P_dbl, c_i64]),
         "lica_batch_destroy": (None, [V]),
         "lica_init": (ctypes.c_int, [PV, ctypes.c_char_p, ctypes.c_char_p]),

@@ -350,10 +350,24 @@ BatchDev make_bd(const lica_batch *b, int msub)
     return bd;
 }
 
+// threads per workgroup: the requested count, raised in the partitioned mode
+// until every case's rows fit the per-thread register chunks
+int block_threads(const lica_batch *b)
+{
+    int t = b->threads;
+    if (b->poisson_mode != LICA_POISSON_THOMAS) {
+        int nmax = 0;
+        for (const InstParam &p : b->par) nmax = std::max(nmax, p.n);
+        const int need = (nmax + 7) / 8;   // rows per thread of the register level
+        while (t < need && t < 1024) t *= 2;
+    }
+    return t;
+}
+
 int launch(lica_batch *b, int which, int msub)
 {
     BatchDev bd = make_bd(b, msub);
-    int rc = lica_launch(which, bd, b->threads, (void *)b->stream);
+    int rc = lica_launch(which, bd, block_threads(b), (void *)b->stream);
     return rc == 0 ? 0 : LICA_HIP_ERROR;
 }
 
@@ -704,6 +718,20 @@ int lica_batch_status(lica_batch *b, int64_t *first_bad_inst)
     if (rc) return rc;
     rc = dsync(b->stream);
     return rc ? rc : first_status(b, first_bad_inst);
+}
+
+int lica_batch_get_progress(lica_batch *b, int64_t *status_out, int64_t *steps_out)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    std::vector<InstState> st((size_t)b->ninst);
+    rc = d2h(st.data(), b->d_st, sizeof(InstState) * st.size(), b->stream);
+    if (rc) return rc;
+    for (int64_t i = 0; i < b->ninst; i++) {
+        if (status_out) status_out[i] = st[(size_t)i].status;
+        if (steps_out) steps_out[i] = st[(size_t)i].m;
+    }
+    return LICA_OK;
 }
 
 int lica_batch_get_scalars(lica_batch *b, int64_t inst, double *out)

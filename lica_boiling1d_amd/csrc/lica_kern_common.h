@@ -15,6 +15,7 @@
 // extrapolation window live in LDS; NS fields stream from HBM.
 #include "lica_par.h"
 #include "lica_types.h"
+#include <type_traits>
 #pragma once
 
 #ifdef LICA_EMUL
@@ -23,7 +24,6 @@
 
 #define WMAX 192       // refined window (z-planes) held in LDS
 #define NSWMAX 128     // NS window of the temperature extrapolation
-#define MAXC 128       // tridiagonal partition chunks
 
 struct CaseP {
     const InstParam *P;
@@ -127,113 +127,424 @@ DEV double box_avg(const double *zf, double ssdzf, F f, double z1, double z2, in
 
 // ---------------------------------------------------------------------
 // Block-level tridiagonal solve of rows l1..l2: a(j)x(j-1)+b(j)x(j)+c(j)x(j+1)=g(j),
-// a(l1) and c(l2) ignored; the solution overwrites g.  Mode 0 is TRDIAG3
-// (1d.f:3783-3812) on one thread; mode 1 is a partitioned elimination: every
-// thread reduces a chunk of rows to its two end rows, the end-row system is
-// solved in LDS, and the chunks' interior rows are recovered in parallel.
+// a(l1) and c(l2) ignored; the solution overwrites g.
+//   mode 0: TRDIAG3 (1d.f:3783-3812) on one thread, the reference's order;
+//   mode 1: partitioned elimination.  Every thread reduces a chunk of rows
+//           to its two end rows (Gaussian elimination inside the chunk, the
+//           chunk's rows held in registers when they fit), the end-row
+//           system -- tridiagonal in the order s0,e0,s1,e1,... -- is reduced
+//           the same way level by level in LDS down to TRI_SERIAL rows,
+//           which one thread eliminates, and the levels are substituted
+//           back.  The Schur complements of a diagonally dominant matrix
+//           stay so, and elimination (unlike cyclic reduction) keeps
+//           TRDIAG3's accuracy on the Laplacian-like rows of POISSON.
+
+#define TRI_RPT 8        // rows per thread held in registers at the first level
+#define TRI_LCHUNK 8     // rows per chunk of the LDS levels
+#define TRI_SERIAL 64    // end-row systems this small are eliminated by one thread
+
+// rows of the LDS workspace (4 arrays) the end-row levels of nthreads chunks need
+HOSTDEV int tri_ws_rows(int nthreads)
+{
+    int m = 2 * nthreads, tot = 0;
+    while (m > TRI_SERIAL) {
+        tot += m;
+        m = 2 * (m / TRI_LCHUNK);
+    }
+    return tot + m;
+}
 
 struct TriSh {
-    double ra[1][2 * MAXC], rb[1][2 * MAXC], rc[1][2 * MAXC], rg[1][2 * MAXC];
+    double *ws;    // LDS: ra | rb | rc | rg, ws_rows each
+    int ws_rows;
 };
 
-DEV void tri_solve(double *a, double *b, double *c, double *g, int l1, int l2, int mode, TriSh &ts)
+DEV void chunk_bounds(int nrow, int nc, int ch, int *s, int *L)
 {
-    const int nrow = l2 - l1 + 1;
-    if (mode == 0 || nrow < 4) {
-        SERIAL {
-            double bet = 1.0 / b[l1];
-            g[l1] = g[l1] * bet;
-            for (int j = l1 + 1; j <= l2; j++) {
-                double gam = c[j - 1] * bet;
-                bet = 1.0 / (b[j] - a[j] * gam);
-                g[j] = (g[j] - a[j] * g[j - 1]) * bet;
-                a[j] = gam;
-            }
-            for (int j = l2 - 1; j >= l1; j--) g[j] = g[j] - a[j + 1] * g[j + 1];
-        }
-        SERIAL_END
-        return;
-    }
-    int nc = NTHREADS < MAXC ? NTHREADS : MAXC;
-    if (nc > nrow / 2) nc = nrow / 2;
     const int base = nrow / nc, rem = nrow % nc;
-    const int m2 = 2 * nc;
-    // local elimination of every chunk
-    PFOR(ch, 0, nc) {
-        int s = l1 + ch * base + (ch < rem ? ch : rem);
-        int L = base + (ch < rem ? 1 : 0);
-        int e = s + L - 1;
-        for (int j = 2; j <= L - 1; j++) {
-            int i = s + j;
-            double r = a[i] / b[i - 1];
-            a[i] = -r * a[i - 1];
-            b[i] = b[i] - r * c[i - 1];
-            g[i] = g[i] - r * g[i - 1];
-        }
-        double sa = a[s], sb, sc, sg;
-        if (L >= 3) {
-            int i = s + L - 2;
-            double inv = 1.0 / b[i];
-            a[i] = a[i] * inv;
-            c[i] = c[i] * inv;
-            g[i] = g[i] * inv;
-            for (int j = L - 3; j >= 1; j--) {
-                i = s + j;
-                double cc = c[i];
-                inv = 1.0 / b[i];
-                double ah = (a[i] - cc * a[i + 1]) * inv;
-                double ch2 = -(cc * c[i + 1]) * inv;
-                double gh = (g[i] - cc * g[i + 1]) * inv;
-                a[i] = ah; c[i] = ch2; g[i] = gh;
-            }
-            double cs = c[s];
-            sb = b[s] - cs * a[s + 1];
-            sc = -(cs * c[s + 1]);
-            sg = g[s] - cs * g[s + 1];
-        } else {
-            sb = b[s]; sc = c[s]; sg = g[s];
-        }
-        ts.ra[0][2 * ch] = (ch == 0) ? 0.0 : sa;
-        ts.rb[0][2 * ch] = sb;
-        ts.rc[0][2 * ch] = sc;
-        ts.rg[0][2 * ch] = sg;
-        ts.ra[0][2 * ch + 1] = a[e];
-        ts.rb[0][2 * ch + 1] = b[e];
-        ts.rc[0][2 * ch + 1] = (ch == nc - 1) ? 0.0 : c[e];
-        ts.rg[0][2 * ch + 1] = g[e];
+    *s = ch * base + (ch < rem ? ch : rem);
+    *L = base + (ch < rem ? 1 : 0);
+}
+
+// reduce rows s..s+L-1 (L >= 2) in place; r = start row (a,b,c,g), end row (a,b,c,g)
+DEV void chunk_reduce(double *a, double *b, double *c, double *g, int s, int L, double *r)
+{
+    for (int j = 2; j <= L - 1; j++) {
+        int i = s + j;
+        double q = a[i] / b[i - 1];
+        a[i] = -q * a[i - 1];
+        b[i] = b[i] - q * c[i - 1];
+        g[i] = g[i] - q * g[i - 1];
     }
-    PFOR_END
-    // the end-row system (2*nc rows, tridiagonal in the order s0,e0,s1,e1,...)
-    // by Gaussian elimination in LDS: the Schur complement of a diagonally
-    // dominant matrix stays so, and plain elimination keeps the accuracy of
-    // TRDIAG3 where cyclic reduction loses digits on Laplacian-like rows
+    if (L >= 3) {
+        int i = s + L - 2;
+        double inv = 1.0 / b[i];
+        a[i] = a[i] * inv;
+        c[i] = c[i] * inv;
+        g[i] = g[i] * inv;
+        for (int j = L - 3; j >= 1; j--) {
+            i = s + j;
+            double cc = c[i];
+            inv = 1.0 / b[i];
+            double ah = (a[i] - cc * a[i + 1]) * inv;
+            double ch2 = -(cc * c[i + 1]) * inv;
+            double gh = (g[i] - cc * g[i + 1]) * inv;
+            a[i] = ah; c[i] = ch2; g[i] = gh;
+        }
+        double cs = c[s];
+        r[1] = b[s] - cs * a[s + 1];
+        r[2] = -(cs * c[s + 1]);
+        r[3] = g[s] - cs * g[s + 1];
+    } else {
+        r[1] = b[s]; r[2] = c[s]; r[3] = g[s];
+    }
+    r[0] = a[s];
+    const int e = s + L - 1;
+    r[4] = a[e]; r[5] = b[e]; r[6] = c[e]; r[7] = g[e];
+}
+
+DEV void chunk_recover(const double *a, const double *c, double *g, int s, int L, double xs, double xe)
+{
+    for (int j = 1; j <= L - 2; j++) {
+        int i = s + j;
+        g[i] = g[i] - a[i] * xs - c[i] * xe;
+    }
+    g[s] = xs;
+    g[s + L - 1] = xe;
+}
+
+// compile-time loop: f(std::integral_constant<int, J>) for J = 0..N-1, so that
+// register arrays are only ever indexed by constants
+template <int J, int N, class F>
+DEV void static_for_impl(F &f)
+{
+    if constexpr (J < N) {
+        f(std::integral_constant<int, J>{});
+        static_for_impl<J + 1, N>(f);
+    }
+}
+template <int N, class F>
+DEV void static_for(F f) { static_for_impl<0, N>(f); }
+
+// chunk_reduce / chunk_recover on a chunk held in registers (L <= TRI_RPT):
+// the same operations in the same order, with every index a constant
+DEV void chunk_reduce_reg(double (&a)[TRI_RPT], double (&b)[TRI_RPT], double (&c)[TRI_RPT], double (&g)[TRI_RPT],
+                          int L, double (&r)[8])
+{
+    static_for<TRI_RPT>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        if constexpr (j >= 2) {
+            if (j <= L - 1) {
+                double q = a[j] / b[j - 1];
+                a[j] = -q * a[j - 1];
+                b[j] = b[j] - q * c[j - 1];
+                g[j] = g[j] - q * g[j - 1];
+            }
+        }
+    });
+    if (L >= 3) {
+        static_for<TRI_RPT>([&](auto jc) {
+            constexpr int j = TRI_RPT - 1 - decltype(jc)::value;   // TRI_RPT-1 .. 0
+            if constexpr (j >= 1 && j <= TRI_RPT - 2) {
+                if (j == L - 2) {
+                    double inv = 1.0 / b[j];
+                    a[j] = a[j] * inv;
+                    c[j] = c[j] * inv;
+                    g[j] = g[j] * inv;
+                } else if (j <= L - 3) {
+                    double cc = c[j];
+                    double inv = 1.0 / b[j];
+                    double ah = (a[j] - cc * a[j + 1]) * inv;
+                    double ch2 = -(cc * c[j + 1]) * inv;
+                    double gh = (g[j] - cc * g[j + 1]) * inv;
+                    a[j] = ah; c[j] = ch2; g[j] = gh;
+                }
+            }
+        });
+        double cs = c[0];
+        r[1] = b[0] - cs * a[1];
+        r[2] = -(cs * c[1]);
+        r[3] = g[0] - cs * g[1];
+    } else {
+        r[1] = b[0]; r[2] = c[0]; r[3] = g[0];
+    }
+    r[0] = a[0];
+    static_for<TRI_RPT>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        if (j == L - 1) { r[4] = a[j]; r[5] = b[j]; r[6] = c[j]; r[7] = g[j]; }
+    });
+}
+
+DEV void chunk_recover_reg(const double (&a)[TRI_RPT], const double (&c)[TRI_RPT], double (&g)[TRI_RPT], int L,
+                           double xs, double xe)
+{
+    static_for<TRI_RPT>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        double v = g[j];
+        if constexpr (j >= 1 && j <= TRI_RPT - 2) {
+            double t = v - a[j] * xs - c[j] * xe;
+            v = (j <= L - 2) ? t : v;
+        }
+        if constexpr (j >= 1) v = (j == L - 1) ? xe : v;
+        if constexpr (j == 0) v = xs;
+        g[j] = v;
+    });
+}
+
+// end rows of chunk ch of nc into rows 2ch, 2ch+1 of the next level
+DEV void put_end_rows(double *ra, double *rb, double *rc, double *rg, int ch, int nc, const double *r)
+{
+    ra[2 * ch] = (ch == 0) ? 0.0 : r[0];
+    rb[2 * ch] = r[1];
+    rc[2 * ch] = r[2];
+    rg[2 * ch] = r[3];
+    ra[2 * ch + 1] = r[4];
+    rb[2 * ch + 1] = r[5];
+    rc[2 * ch + 1] = (ch == nc - 1) ? 0.0 : r[6];
+    rg[2 * ch + 1] = r[7];
+}
+
+// offset and size of level l of the end-row systems that start with m rows
+DEV void lds_level(int m, int l, int *off, int *rows)
+{
+    int o = 0, cur = m;
+    for (int i = 0; i < l; i++) { o += cur; cur = 2 * (cur / TRI_LCHUNK); }
+    *off = o;
+    *rows = cur;
+}
+
+// the end-row system of m rows at row 0 of the LDS workspace; solution in rg
+DEV void lds_tri_solve(TriSh &ts, int m)
+{
+    double *RA = ts.ws, *RB = ts.ws + ts.ws_rows, *RC = ts.ws + 2 * ts.ws_rows, *RG = ts.ws + 3 * ts.ws_rows;
+    int nlev = 0;
+    for (int cur = m; cur > TRI_SERIAL; cur = 2 * (cur / TRI_LCHUNK)) nlev++;
+    for (int l = 0; l < nlev; l++) {
+        int bo, rows;
+        lds_level(m, l, &bo, &rows);
+        const int nc = rows / TRI_LCHUNK, no = bo + rows;
+        PFOR(ch, 0, nc) {
+            int s, L;
+            chunk_bounds(rows, nc, ch, &s, &L);
+            double r[8];
+            chunk_reduce(RA + bo, RB + bo, RC + bo, RG + bo, s, L, r);
+            put_end_rows(RA + no, RB + no, RC + no, RG + no, ch, nc, r);
+        }
+        PFOR_END
+    }
+    int o, cur;
+    lds_level(m, nlev, &o, &cur);
     SERIAL {
-        double *ra = ts.ra[0], *rb = ts.rb[0], *rc = ts.rc[0], *rg = ts.rg[0];
+        double *ra = RA + o, *rb = RB + o, *rc = RC + o, *rg = RG + o;
         double bet = 1.0 / rb[0];
         rg[0] = rg[0] * bet;
-        for (int j = 1; j < m2; j++) {
+        for (int j = 1; j < cur; j++) {
             double gam = rc[j - 1] * bet;
             bet = 1.0 / (rb[j] - ra[j] * gam);
             rg[j] = (rg[j] - ra[j] * rg[j - 1]) * bet;
             ra[j] = gam;
         }
-        for (int j = m2 - 2; j >= 0; j--) rg[j] = rg[j] - ra[j + 1] * rg[j + 1];
+        for (int j = cur - 2; j >= 0; j--) rg[j] = rg[j] - ra[j + 1] * rg[j + 1];
     }
     SERIAL_END
-    PFOR(ch, 0, nc) {
-        int s = l1 + ch * base + (ch < rem ? ch : rem);
-        int L = base + (ch < rem ? 1 : 0);
-        int e = s + L - 1;
-        double xs = ts.rg[0][2 * ch];
-        double xe = ts.rg[0][2 * ch + 1];
-        for (int j = 1; j <= L - 2; j++) {
-            int i = s + j;
-            g[i] = g[i] - a[i] * xs - c[i] * xe;
+    for (int l = nlev - 1; l >= 0; l--) {
+        int bo, rows;
+        lds_level(m, l, &bo, &rows);
+        const int nc = rows / TRI_LCHUNK, no = bo + rows;
+        PFOR(ch, 0, nc) {
+            int s, L;
+            chunk_bounds(rows, nc, ch, &s, &L);
+            chunk_recover(RA + bo, RC + bo, RG + bo, s, L, RG[no + 2 * ch], RG[no + 2 * ch + 1]);
         }
-        g[s] = xs;
-        g[e] = xe;
+        PFOR_END
     }
-    PFOR_END
+}
+
+// g - (a*xm + b*x + c*xp) with error-free products (fma) and sums, rounded once
+DEV void two_sum(double a, double b, double *s, double *e)
+{
+    double t = a + b;
+    double bb = t - a;
+    *e = (a - (t - bb)) + (b - bb);
+    *s = t;
+}
+DEV double dd_residual(double g, double a, double xm, double b, double x, double c, double xp)
+{
+    double p1 = a * xm, e1 = fma(a, xm, -p1);
+    double p2 = b * x, e2 = fma(b, x, -p2);
+    double p3 = c * xp, e3 = fma(c, xp, -p3);
+    double hi, lo, t;
+    two_sum(g, -p1, &hi, &lo);
+    two_sum(hi, -p2, &hi, &t);
+    lo = lo + t;
+    two_sum(hi, -p3, &hi, &t);
+    lo = lo + t;
+    lo = lo - e1 - e2 - e3;
+    return hi + lo;
+}
+
+#define TRI_NREF 2    // refinement steps of the partitioned solve
+
+// TRDIAG3 (1d.f:3783-3812) on one thread, pivots stored in a
+DEV void tri_thomas(double *a, const double *b, const double *c, double *g, int l1, int l2)
+{
+    SERIAL {
+        double bet = 1.0 / b[l1];
+        g[l1] = g[l1] * bet;
+        for (int j = l1 + 1; j <= l2; j++) {
+            double gam = c[j - 1] * bet;
+            bet = 1.0 / (b[j] - a[j] * gam);
+            g[j] = (g[j] - a[j] * g[j - 1]) * bet;
+            a[j] = gam;
+        }
+        for (int j = l2 - 1; j >= l1; j--) g[j] = g[j] - a[j + 1] * g[j + 1];
+    }
+    SERIAL_END
+}
+
+#ifndef LICA_EMUL
+// one partitioned pass of the register path: rhs (registers) -> solution
+DEV void part_pass_reg(const double *A, const double *B, const double *C, double (&rhs)[TRI_RPT], bool act, int ch,
+                       int nc, int L, int s, TriSh &ts)
+{
+    double *RA = ts.ws, *RB = ts.ws + ts.ws_rows, *RC = ts.ws + 2 * ts.ws_rows, *RG = ts.ws + 3 * ts.ws_rows;
+    double xa[TRI_RPT], xb[TRI_RPT], xc[TRI_RPT];
+    if (act) {
+        static_for<TRI_RPT>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            xa[j] = (j < L) ? A[s + j] : 0.0;
+            xb[j] = (j < L) ? B[s + j] : 1.0;
+            xc[j] = (j < L) ? C[s + j] : 0.0;
+        });
+        double r[8];
+        chunk_reduce_reg(xa, xb, xc, rhs, L, r);
+        put_end_rows(RA, RB, RC, RG, ch, nc, r);
+    }
+    BARRIER();
+    lds_tri_solve(ts, 2 * nc);
+    if (act) chunk_recover_reg(xa, xc, rhs, L, RG[2 * ch], RG[2 * ch + 1]);
+    BARRIER();
+}
+#endif
+
+DEV void tri_solve(double *a, double *b, double *c, double *g, int l1, int l2, int mode, TriSh &ts)
+{
+    const int nrow = l2 - l1 + 1;
+    if (mode == 0 || nrow < 4) {
+        tri_thomas(a, b, c, g, l1, l2);
+        return;
+    }
+    double *RA = ts.ws, *RB = ts.ws + ts.ws_rows, *RC = ts.ws + 2 * ts.ws_rows, *RG = ts.ws + 3 * ts.ws_rows;
+    int nc = NTHREADS;
+    if (nc > nrow / 2) nc = nrow / 2;
+    const double *A = a + l1, *B = b + l1, *C = c + l1;
+    double *G = g + l1;
+    // The partitioned elimination is backward stable but, on the nearly
+    // singular POISSON rows, its forward error grows with the chunk count;
+    // TRI_NREF refinement steps with residuals in double-double bring the
+    // solution back to (or below) TRDIAG3's error.  a, b and c are left
+    // untouched; g receives the solution.
+#ifndef LICA_EMUL
+    const int lmax = (nrow + nc - 1) / nc;
+    if (lmax > TRI_RPT) {
+        // rows beyond registers: the serial elimination (exact reference order)
+        tri_thomas(a, b, c, g, l1, l2);
+        return;
+    }
+    const int ch = (int)threadIdx.x;
+    const bool act = ch < nc;
+    int s = 0, L = 0;
+    double x[TRI_RPT];
+    if (act) {
+        chunk_bounds(nrow, nc, ch, &s, &L);
+        static_for<TRI_RPT>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            x[j] = (j < L) ? G[s + j] : 0.0;
+        });
+    }
+    part_pass_reg(A, B, C, x, act, ch, nc, L, s, ts);
+    for (int it = 0; it < TRI_NREF; it++) {
+        if (act) {
+            RA[2 * ch] = x[0];
+            static_for<TRI_RPT>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                if (j == L - 1) RA[2 * ch + 1] = x[j];
+            });
+        }
+        BARRIER();
+        double d[TRI_RPT];
+        if (act) {
+            const double xl = (ch > 0) ? RA[2 * ch - 1] : 0.0, xr = (ch < nc - 1) ? RA[2 * ch + 2] : 0.0;
+            static_for<TRI_RPT>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                if (j < L) {
+                    const int i = s + j;
+                    double xm = xl, xp = xr;
+                    if constexpr (j > 0) xm = x[j - 1];
+                    if constexpr (j + 1 < TRI_RPT) { if (j != L - 1) xp = x[j + 1]; }
+                    const double am = (i == 0) ? 0.0 : A[i];
+                    const double cp = (i == nrow - 1) ? 0.0 : C[i];
+                    d[j] = dd_residual(G[i], am, xm, B[i], x[j], cp, xp);
+                } else {
+                    d[j] = 0.0;
+                }
+            });
+        }
+        BARRIER();
+        part_pass_reg(A, B, C, d, act, ch, nc, L, s, ts);
+        if (act) {
+            static_for<TRI_RPT>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                x[j] = x[j] + d[j];
+            });
+        }
+    }
+    if (act) {
+        static_for<TRI_RPT>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            if (j < L) G[s + j] = x[j];
+        });
+    }
+    BARRIER();
+#else
+    // the same passes over copies of the rows (CPU logic-check build)
+    static double *wa = nullptr, *wb, *wc, *wx, *wd;
+    static int cap = 0;
+    if (nrow > cap) {
+        delete[] wa;
+        cap = nrow;
+        wa = new double[5 * (size_t)cap];
+        wb = wa + cap; wc = wb + cap; wx = wc + cap; wd = wx + cap;
+    }
+    auto pass = [&](double *rhs) {
+        for (int i = 0; i < nrow; i++) { wa[i] = A[i]; wb[i] = B[i]; wc[i] = C[i]; }
+        for (int ch = 0; ch < nc; ch++) {
+            int s, L;
+            chunk_bounds(nrow, nc, ch, &s, &L);
+            double r[8];
+            chunk_reduce(wa, wb, wc, rhs, s, L, r);
+            put_end_rows(RA, RB, RC, RG, ch, nc, r);
+        }
+        lds_tri_solve(ts, 2 * nc);
+        for (int ch = 0; ch < nc; ch++) {
+            int s, L;
+            chunk_bounds(nrow, nc, ch, &s, &L);
+            chunk_recover(wa, wc, rhs, s, L, RG[2 * ch], RG[2 * ch + 1]);
+        }
+    };
+    for (int i = 0; i < nrow; i++) wx[i] = G[i];
+    pass(wx);
+    for (int it = 0; it < TRI_NREF; it++) {
+        for (int i = 0; i < nrow; i++) {
+            const double xm = (i == 0) ? 0.0 : wx[i - 1], xp = (i == nrow - 1) ? 0.0 : wx[i + 1];
+            const double am = (i == 0) ? 0.0 : A[i], cp = (i == nrow - 1) ? 0.0 : C[i];
+            wd[i] = dd_residual(G[i], am, xm, B[i], wx[i], cp, xp);
+        }
+        pass(wd);
+        for (int i = 0; i < nrow; i++) wx[i] = wx[i] + wd[i];
+    }
+    for (int i = 0; i < nrow; i++) G[i] = wx[i];
+#endif
 }
 
 // ---------------------------------------------------------------------
@@ -251,7 +562,7 @@ struct BlkSh {
     double red[16];
     double v[16];
     int iv[16];
-    TriSh tri;
+    TriSh tri;     // points at the kernel's dynamic LDS (tri_ws_rows(blockDim) x 4 doubles)
 };
 
 // DETERMINE_DT (1d.f:683-817) at the start of a step, with the per-step
@@ -508,7 +819,7 @@ struct BandSh {
     // temperature extrapolation window (NS cells [nlo,nhi])
     int nlo, nhi;
     double t0[NSWMAX], t1[NSWMAX], nphi[NSWMAX], psih[NSWMAX];
-    double dtn[NSWMAX], dtn_rk[NSWMAX], tmpo[NSWMAX], tmp_rk[NSWMAX], tmp_new[NSWMAX];
+    double dtn[NSWMAX];
     double psitmp[NSWMAX];
     int mcell[2 * LICA_MAXENT];
 };

@@ -15,52 +15,184 @@ struct TgradSh {
     int tmk2[WMAX];
 };
 
-// TEMP_EXTP_RK3 (chphase.f:806-956) on the NS window, in place on arr
-DEV void temp_extp_rk3(BandSh &B, const InstParam &P, const double *VVDZ, double *arr, int im, double redtl,
-                       int nd, int ia, int ib)
+// One point of TEMP_EXTP_RK3 (chphase.f:806-956): the upwind pseudo-time
+// update of cell value a with neighbours al/ar; the normal is fixed to +z
+// (chphase.f:897-899) and x/y differences vanish on the flat column.
+DEV double extp_point(double a, double al, double ar, double np, double npl, double npr, double vv, double vvp,
+                      double psih, double dtn, int im, int nd, double redtl, double sm_grid, double sdx1)
 {
-    const int nlo = B.nlo;
-    const double sm_grid = fmin(fmin(2.0 * P.sdxf, 2.0 * P.sdyf), 2.0 * P.sdzf);
-    PFOR(k, ia, ib + 1) {
-        int i = k - nlo;
-        double np = B.nphi[i];
-        double tn = arr[i];
-        if (np != 0.0) {
-            double dpx = 0.5 * (np - np) * (1.0 / P.sdx1);
-            double dpz;
-            if (B.nphi[i - 1] == 0.0) dpz = (B.nphi[i + 1] - B.nphi[i]) * VVDZ[k + 1];
-            else if (B.nphi[i + 1] == 0.0) dpz = (B.nphi[i] - B.nphi[i - 1]) * VVDZ[k];
-            else dpz = 0.5 * (B.nphi[i + 1] - B.nphi[i - 1]) * VVDZ[k];
-            double dalphi = dpx * dpx + dpx * dpx + dpz * dpz;
-            if (dalphi != 0.0) {
-                double nx = 0.0, ny = 0.0, nz = 1.0;
-                double apxm = (arr[i] - arr[i]) * (1.0 / P.sdx1);
-                double apxp = apxm;
-                double apzm = (arr[i] - arr[i - 1]) * VVDZ[k];
-                double apzp = (arr[i + 1] - arr[i]) * VVDZ[k];
-                if (im == 1) { nx = (-1.0) * nx; ny = (-1.0) * ny; nz = (-1.0) * nz; }
-                double ax = fmin(nx, 0.0) * apxp + fmax(nx, 0.0) * apxm;
-                double ay = fmin(ny, 0.0) * apxp + fmax(ny, 0.0) * apxm;
-                double az = fmin(nz, 0.0) * apzp + fmax(nz, 0.0) * apzm;
-                double drhs;
-                if (im == 1) {
-                    drhs = (ax + ay + az) + (double)nd * B.dtn[i];
-                    tn = arr[i] - redtl * drhs * (1.0 - B.psih[i]);
-                } else {
-                    drhs = (ax + ay + az) - (double)nd * B.dtn[i];
-                    tn = arr[i] - redtl * drhs * B.psih[i];
-                }
-                if (nd == 0 && fabs(np) <= sm_grid) {
-                    drhs = (ax + ay + az);
-                    tn = arr[i] - redtl * drhs;
-                }
+    double tn = a;
+    if (np != 0.0) {
+        double dpx = 0.5 * (np - np) * (1.0 / sdx1);
+        double dpz;
+        if (npl == 0.0) dpz = (npr - np) * vvp;
+        else if (npr == 0.0) dpz = (np - npl) * vv;
+        else dpz = 0.5 * (npr - npl) * vv;
+        double dalphi = dpx * dpx + dpx * dpx + dpz * dpz;
+        if (dalphi != 0.0) {
+            double nx = 0.0, ny = 0.0, nz = 1.0;
+            double apxm = (a - a) * (1.0 / sdx1);
+            double apxp = apxm;
+            double apzm = (a - al) * vv;
+            double apzp = (ar - a) * vv;
+            if (im == 1) { nx = (-1.0) * nx; ny = (-1.0) * ny; nz = (-1.0) * nz; }
+            double ax = fmin(nx, 0.0) * apxp + fmax(nx, 0.0) * apxm;
+            double ay = fmin(ny, 0.0) * apxp + fmax(ny, 0.0) * apxm;
+            double az = fmin(nz, 0.0) * apzp + fmax(nz, 0.0) * apzm;
+            double drhs;
+            if (im == 1) {
+                drhs = (ax + ay + az) + (double)nd * dtn;
+                tn = a - redtl * drhs * (1.0 - psih);
+            } else {
+                drhs = (ax + ay + az) - (double)nd * dtn;
+                tn = a - redtl * drhs * psih;
+            }
+            if (nd == 0 && fabs(np) <= sm_grid) {
+                drhs = (ax + ay + az);
+                tn = a - redtl * drhs;
             }
         }
-        B.tmp_new[i] = tn;
     }
-    PFOR_END
-    PFOR(k, ia, ib + 1) { arr[k - nlo] = B.tmp_new[k - nlo]; }
-    PFOR_END
+    return tn;
+}
+
+// The pseudo-time loops of TEMP_EXTP (chphase.f:619-679 for dT/dn, 704-769
+// for T) held in registers: window cell i = XCELL(s) of slot s of the lane
+// (two consecutive cells per lane of the one wave; the CPU logic-check
+// build keeps the whole window in one lane), neighbours by lane shuffles,
+// RAT by a wave reduction.  Every cell update is extp_point in the
+// reference's Jacobi order (all cells from the previous stage).
+#ifdef LICA_EMUL
+#define XCPL NSWMAX
+#define XCELL(s) (s)
+DEV void xneighbors(const double *a, double *l, double *r)
+{
+    for (int s = 0; s < XCPL; s++) {
+        l[s] = s > 0 ? a[s - 1] : a[s];
+        r[s] = s + 1 < XCPL ? a[s + 1] : a[s];
+    }
+}
+DEV double xwave_max(double m) { return m; }
+#else
+#define XCPL 2
+#define XCELL(s) (2 * (int)threadIdx.x + (s))
+DEV void xneighbors(const double *a, double *l, double *r)
+{
+    double up = __shfl_up(a[1], 1, 64);
+    double dn = __shfl_down(a[0], 1, 64);
+    l[0] = up; r[0] = a[1];
+    l[1] = a[0]; r[1] = dn;
+}
+DEV double xwave_max(double m)
+{
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    return m;
+}
+#endif
+
+struct XCells {
+    double np[XCPL], npl[XCPL], npr[XCPL], vv[XCPL], vvp[XCPL], ps[XCPL], dtn[XCPL];
+    int upd[XCPL];
+};
+
+// one forward-Euler stage on a (cells outside [ia,ib] stay as they are);
+// the dT/dn pass (nd=0) uses the array being advanced as its DTEMPN
+DEV void extp_stage(double *a, const XCells &c, int im, int nd, double redtl, double sm_grid, double sdx1)
+{
+    double l[XCPL], r[XCPL], tn[XCPL];
+    xneighbors(a, l, r);
+    for (int s = 0; s < XCPL; s++) {
+        double dt = (nd == 0) ? a[s] : c.dtn[s];
+        tn[s] = c.upd[s] ? extp_point(a[s], l[s], r[s], c.np[s], c.npl[s], c.npr[s], c.vv[s], c.vvp[s], c.ps[s], dt,
+                                      im, nd, redtl, sm_grid, sdx1)
+                         : a[s];
+    }
+    for (int s = 0; s < XCPL; s++) a[s] = tn[s];
+}
+
+// TEMP_EXTP's two pseudo-time loops for phase im on the LDS window tw (the
+// dT/dn field is in B.dtn on entry).  Returns ST_EXTRAP after 20 halvings.
+DEV int temp_extp_loops(BandSh &B, const InstParam &P, const double *VVDZ, double *tw, int im, int ia, int ib,
+                        double sm_grid, double eps, double *iters, double *halv)
+{
+    const int nlo = B.nlo, nw = B.nhi - B.nlo + 1;
+    XCells c;
+    double a[XCPL], rk[XCPL];
+    for (int s = 0; s < XCPL; s++) {
+        int i = XCELL(s), k = nlo + i;
+        bool ok = i < nw;
+        c.np[s] = ok ? B.nphi[i] : 0.0;
+        c.npl[s] = (ok && i > 0) ? B.nphi[i - 1] : 0.0;
+        c.npr[s] = (ok && i + 1 < nw) ? B.nphi[i + 1] : 0.0;
+        c.vv[s] = ok ? VVDZ[k] : 0.0;
+        c.vvp[s] = ok ? VVDZ[k + 1] : 0.0;
+        c.ps[s] = ok ? B.psih[i] : 0.0;
+        c.dtn[s] = ok ? B.dtn[i] : 0.0;
+        c.upd[s] = ok && k >= ia && k <= ib;
+        a[s] = c.dtn[s];
+    }
+    // (a) dT/dn (ND=0), continuing from the current field after a halving
+    double redtl = 0.2 * sm_grid;
+    int halvings = 0;
+    for (;;) {
+        int conv = 0;
+        for (int it = 1; it <= 100; it++) {
+            for (int s = 0; s < XCPL; s++) rk[s] = a[s];
+            extp_stage(a, c, im, 0, redtl, sm_grid, P.sdx1);
+            extp_stage(a, c, im, 0, redtl, sm_grid, P.sdx1);
+            for (int s = 0; s < XCPL; s++) if (c.upd[s]) a[s] = 0.75 * rk[s] + 0.25 * a[s];
+            extp_stage(a, c, im, 0, redtl, sm_grid, P.sdx1);
+            double m = 0.0;
+            for (int s = 0; s < XCPL; s++) {
+                if (c.upd[s]) a[s] = 1.0 / 3.0 * rk[s] + 2.0 / 3.0 * a[s];
+                double v = 0.0;
+                if (rk[s] != 0.0) v = fabs(a[s] / rk[s] - 1.0);
+                else if (a[s] != 0.0) v = fabs(rk[s] / a[s] - 1.0);
+                if (c.upd[s]) m = fmax(m, v);
+            }
+            double rat = xwave_max(m);
+            *iters += 1.0;
+            if (fabs(rat) < eps) { conv = 1; break; }
+        }
+        if (conv) break;
+        if (++halvings > 20) return ST_EXTRAP;
+        *halv += 1.0;
+        redtl = 0.5 * redtl;
+    }
+    for (int s = 0; s < XCPL; s++) c.dtn[s] = a[s];
+    // (b) T (ND=1) with the extrapolated dT/dn as source
+    double t[XCPL];
+    for (int s = 0; s < XCPL; s++) { int i = XCELL(s); t[s] = (i < nw) ? tw[i] : 0.0; }
+    redtl = 0.2 * sm_grid;
+    halvings = 0;
+    for (;;) {
+        int conv = 0;
+        for (int it = 1; it <= 100; it++) {
+            for (int s = 0; s < XCPL; s++) { rk[s] = t[s]; a[s] = t[s]; }
+            extp_stage(a, c, im, 1, redtl, sm_grid, P.sdx1);
+            extp_stage(a, c, im, 1, redtl, sm_grid, P.sdx1);
+            for (int s = 0; s < XCPL; s++) if (c.upd[s]) a[s] = 0.75 * rk[s] + 0.25 * a[s];
+            extp_stage(a, c, im, 1, redtl, sm_grid, P.sdx1);
+            double m = 0.0;
+            for (int s = 0; s < XCPL; s++) {
+                if (c.upd[s]) { a[s] = 1.0 / 3.0 * rk[s] + 2.0 / 3.0 * a[s]; t[s] = a[s]; }
+                double v = (rk[s] != 0.0) ? fabs(t[s] / rk[s] - 1.0) : 0.0;
+                if (c.upd[s]) m = fmax(m, v);
+            }
+            double rat = xwave_max(m);
+            *iters += 1.0;
+            if (fabs(rat) < eps) { conv = 1; break; }
+        }
+        if (conv) break;
+        if (++halvings > 20) return ST_EXTRAP;
+        *halv += 1.0;
+        redtl = 0.5 * redtl;
+    }
+    for (int s = 0; s < XCPL; s++) {
+        int i = XCELL(s);
+        if (i < nw) { tw[i] = t[s]; B.dtn[i] = c.dtn[s]; }
+    }
+    return ST_OK;
 }
 
 // PHINSINTP (chphase.f:1258-1325) on the flat column; call inside SERIAL
@@ -197,7 +329,7 @@ DEV void body_temp_grad(const BatchDev &bd, int inst, TgradSh &TS)
     const int ia = nlo < 1 ? 1 : nlo, ib = nhi > n ? n : nhi;
     for (int im = 0; im <= 1; im++) {
         double *tw = (im == 0) ? B.t0 : B.t1;
-        PFOR(k, nlo, nhi + 1) { B.dtn[k - nlo] = 0.0; B.dtn_rk[k - nlo] = 0.0; }
+        PFOR(k, nlo, nhi + 1) { B.dtn[k - nlo] = 0.0; }
         PFOR_END
         PFOR(k, ia, ib + 1) {
             int i = k - nlo;
@@ -223,71 +355,18 @@ DEV void body_temp_grad(const BatchDev &bd, int inst, TgradSh &TS)
             B.dtn[i] = v;
         }
         PFOR_END
-        // extrapolation of dT/dn (ND=0), halving the pseudo step on failure
-        double redtl = 0.2 * sm_grid;
-        int halvings = 0;
-        for (;;) {
-            int conv = 0;
-            for (int it = 1; it <= 100; it++) {
-                PFOR(k, ia, ib + 1) { B.dtn_rk[k - nlo] = B.dtn[k - nlo]; }
-                PFOR_END
-                temp_extp_rk3(B, P, VVDZ, B.dtn, im, redtl, 0, ia, ib);
-                temp_extp_rk3(B, P, VVDZ, B.dtn, im, redtl, 0, ia, ib);
-                PFOR(k, ia, ib + 1) { int i = k - nlo; B.dtn[i] = 0.75 * B.dtn_rk[i] + 0.25 * B.dtn[i]; }
-                PFOR_END
-                temp_extp_rk3(B, P, VVDZ, B.dtn, im, redtl, 0, ia, ib);
-                PFOR(k, ia, ib + 1) { int i = k - nlo; B.dtn[i] = 1.0 / 3.0 * B.dtn_rk[i] + 2.0 / 3.0 * B.dtn[i]; }
-                PFOR_END
-                double rat = red_max(ia, ib + 1, 0.0, [&](int k) {
-                    int i = k - nlo;
-                    if (B.dtn_rk[i] != 0.0) return fabs(B.dtn[i] / B.dtn_rk[i] - 1.0);
-                    else if (B.dtn[i] != 0.0) return fabs(B.dtn_rk[i] / B.dtn[i] - 1.0);
-                    return 0.0;
-                }, B.red);
-                SERIAL { S.n_extrap_iters += 1.0; }
-                SERIAL_END
-                if (fabs(rat) < eps) { conv = 1; break; }
-            }
-            if (conv) break;
-            if (++halvings > 20) { SERIAL { S.status = ST_EXTRAP; } SERIAL_END return; }
-            SERIAL { S.n_halvings += 1.0; }
-            SERIAL_END
-            redtl = 0.5 * redtl;
+        // the two pseudo-time loops, in registers of the one wave
+        BARRIER();
+        double iters = 0.0, halv = 0.0;
+        int xst = temp_extp_loops(B, P, VVDZ, tw, im, ia, ib, sm_grid, eps, &iters, &halv);
+        BARRIER();
+        SERIAL {
+            S.n_extrap_iters += iters;
+            S.n_halvings += halv;
+            if (xst) S.status = xst;
         }
-        // extrapolation of T (ND=1)
-        redtl = 0.2 * sm_grid;
-        halvings = 0;
-        for (;;) {
-            int conv = 0;
-            for (int it = 1; it <= 100; it++) {
-                PFOR(k, nlo, nhi + 1) { int i = k - nlo; B.tmp_rk[i] = tw[i]; B.tmpo[i] = tw[i]; }
-                PFOR_END
-                temp_extp_rk3(B, P, VVDZ, B.tmpo, im, redtl, 1, ia, ib);
-                temp_extp_rk3(B, P, VVDZ, B.tmpo, im, redtl, 1, ia, ib);
-                PFOR(k, ia, ib + 1) { int i = k - nlo; B.tmpo[i] = 0.75 * B.tmp_rk[i] + 0.25 * B.tmpo[i]; }
-                PFOR_END
-                temp_extp_rk3(B, P, VVDZ, B.tmpo, im, redtl, 1, ia, ib);
-                PFOR(k, ia, ib + 1) {
-                    int i = k - nlo;
-                    B.tmpo[i] = 1.0 / 3.0 * B.tmp_rk[i] + 2.0 / 3.0 * B.tmpo[i];
-                    tw[i] = B.tmpo[i];
-                }
-                PFOR_END
-                double rat = red_max(ia, ib + 1, 0.0, [&](int k) {
-                    int i = k - nlo;
-                    if (B.tmp_rk[i] != 0.0) return fabs(tw[i] / B.tmp_rk[i] - 1.0);
-                    return 0.0;
-                }, B.red);
-                SERIAL { S.n_extrap_iters += 1.0; }
-                SERIAL_END
-                if (fabs(rat) < eps) { conv = 1; break; }
-            }
-            if (conv) break;
-            if (++halvings > 20) { SERIAL { S.status = ST_EXTRAP; } SERIAL_END return; }
-            SERIAL { S.n_halvings += 1.0; }
-            SERIAL_END
-            redtl = 0.5 * redtl;
-        }
+        SERIAL_END
+        if (xst) return;
     }
     // (3) TF on the T-band, (4)-(6) TF_GRAD
     PFOR(e, B.off[1], B.off[LICA_NN1 + 1]) {

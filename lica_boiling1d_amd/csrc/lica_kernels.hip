@@ -3,6 +3,15 @@
 #include "lica_launch.h"
 
 #ifndef LICA_EMUL
+// LDS workspace of the partitioned tridiagonal solves, sized by the launch
+extern __shared__ double lica_tri_lds[];
+
+DEV void tri_attach(BlkSh &sh)
+{
+    if (threadIdx.x == 0) { sh.tri.ws = lica_tri_lds; sh.tri.ws_rows = tri_ws_rows((int)blockDim.x); }
+    __syncthreads();
+}
+
 __global__ void __launch_bounds__(1024) k_init(BatchDev bd)
 {
     __shared__ InitSh sh;
@@ -16,6 +25,7 @@ __global__ void __launch_bounds__(1024) k_dt(BatchDev bd)
 __global__ void __launch_bounds__(1024) k_cont_energy(BatchDev bd)
 {
     __shared__ BlkSh sh;
+    tri_attach(sh);
     body_cont_energy(bd, bd.order[blockIdx.x], sh);
 }
 __global__ void __launch_bounds__(64) k_temp_grad(BatchDev bd)
@@ -26,11 +36,13 @@ __global__ void __launch_bounds__(64) k_temp_grad(BatchDev bd)
 __global__ void __launch_bounds__(1024) k_momentum(BatchDev bd)
 {
     __shared__ BlkSh sh;
+    tri_attach(sh);
     body_momentum(bd, bd.order[blockIdx.x], sh);
 }
 __global__ void __launch_bounds__(1024) k_poisson_direct(BatchDev bd)
 {
     __shared__ BlkSh sh;
+    tri_attach(sh);
     body_poisson_direct(bd, bd.order[blockIdx.x], sh);
 }
 __global__ void __launch_bounds__(1024) k_ucalc(BatchDev bd)
@@ -44,10 +56,13 @@ __global__ void __launch_bounds__(64) k_transport(BatchDev bd)
     body_transport(bd, bd.order[blockIdx.x], sh);
 }
 
+static int tri_lds_setup();
+
 // nsys independent tridiagonal systems of n rows, contiguous per system
 __global__ void __launch_bounds__(1024) k_tri_systems(double *a, double *b, double *c, double *g, int n, int mode)
 {
     __shared__ BlkSh sh;
+    tri_attach(sh);
     const size_t off = (size_t)blockIdx.x * (size_t)n - 1;   // row k at [off + k], k = 1..n
     tri_solve(a + off, b + off, c + off, g + off, 1, n, mode, sh.tri);
 }
@@ -55,23 +70,44 @@ __global__ void __launch_bounds__(1024) k_tri_systems(double *a, double *b, doub
 int lica_launch_tri(int nsys, int n, double *a, double *b, double *c, double *g, int mode, int threads, void *stream)
 {
     if (nsys <= 0) return 0;
-    hipLaunchKernelGGL(k_tri_systems, dim3(nsys), dim3(threads), 0, (hipStream_t)stream, a, b, c, g, n, mode);
+    if (int rc = tri_lds_setup()) return rc;
+    const size_t lds = sizeof(double) * 4 * (size_t)tri_ws_rows(threads);
+    hipLaunchKernelGGL(k_tri_systems, dim3(nsys), dim3(threads), lds, (hipStream_t)stream, a, b, c, g, n, mode);
     return (int)hipGetLastError();
 }
 
 #define LAUNCH(K, threads) \
     hipLaunchKernelGGL(K, dim3(bd.ninst), dim3(threads), 0, (hipStream_t)stream, bd)
+#define LAUNCH_TRI(K, threads) \
+    hipLaunchKernelGGL(K, dim3(bd.ninst), dim3(threads), sizeof(double) * 4 * (size_t)tri_ws_rows(threads), \
+                       (hipStream_t)stream, bd)
+
+// allow the dynamic LDS of the partitioned solves beyond 64 KB (1024 threads)
+static int tri_lds_setup()
+{
+    static int done = 0, rc = 0;
+    if (!done) {
+        const int bytes = (int)(sizeof(double) * 4 * (size_t)tri_ws_rows(1024));
+        const void *ks[4] = {(const void *)k_cont_energy, (const void *)k_momentum, (const void *)k_poisson_direct,
+                             (const void *)k_tri_systems};
+        for (int i = 0; i < 4 && !rc; i++)
+            rc = (int)hipFuncSetAttribute(ks[i], hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        done = 1;
+    }
+    return rc;
+}
 
 int lica_launch(int which, const BatchDev &bd, int threads, void *stream)
 {
     if (bd.ninst <= 0) return 0;
+    if (int rc = tri_lds_setup()) return rc;
     switch (which) {
     case KL_INIT: LAUNCH(k_init, threads); break;
     case KL_DT: LAUNCH(k_dt, threads); break;
-    case KL_CONT_ENERGY: LAUNCH(k_cont_energy, threads); break;
+    case KL_CONT_ENERGY: LAUNCH_TRI(k_cont_energy, threads); break;
     case KL_TEMP_GRAD: LAUNCH(k_temp_grad, 64); break;
-    case KL_MOMENTUM: LAUNCH(k_momentum, threads); break;
-    case KL_POISSON_DIRECT: LAUNCH(k_poisson_direct, threads); break;
+    case KL_MOMENTUM: LAUNCH_TRI(k_momentum, threads); break;
+    case KL_POISSON_DIRECT: LAUNCH_TRI(k_poisson_direct, threads); break;
     case KL_UCALC: LAUNCH(k_ucalc, threads); break;
     case KL_TRANSPORT: LAUNCH(k_transport, 64); break;
     default: return -1;
@@ -79,11 +115,21 @@ int lica_launch(int which, const BatchDev &bd, int threads, void *stream)
     return (int)hipGetLastError();
 }
 #else
+static double *emul_tri_ws(int threads)
+{
+    static double *ws = nullptr;
+    if (!ws) ws = new double[4 * (size_t)tri_ws_rows(1024)];
+    (void)threads;
+    return ws;
+}
+
 int lica_launch_tri(int nsys, int n, double *a, double *b, double *c, double *g, int mode, int threads, void *stream)
 {
     (void)stream;
     static BlkSh *sh = nullptr;
     if (!sh) sh = new BlkSh();
+    sh->tri.ws = emul_tri_ws(threads);
+    sh->tri.ws_rows = tri_ws_rows(threads);
     int saved = lica_emul_nthreads;
     lica_emul_nthreads = threads;
     for (int s = 0; s < nsys; s++) {
@@ -94,6 +140,10 @@ int lica_launch_tri(int nsys, int n, double *a, double *b, double *c, double *g,
     return 0;
 }
 
+static void attach(BlkSh &sh, int threads) { sh.tri.ws = emul_tri_ws(threads); sh.tri.ws_rows = tri_ws_rows(threads); }
+template <class Sh>
+static void attach(Sh &, int) {}
+
 template <class Sh, class F>
 static void emul_run(const BatchDev &bd, int threads, F body)
 {
@@ -103,6 +153,7 @@ static void emul_run(const BatchDev &bd, int threads, F body)
     lica_emul_nthreads = threads;
     for (int b = 0; b < bd.ninst; b++) {
         memset((void *)sh, 0, sizeof(Sh));
+        attach(*sh, threads);
         body(bd, bd.order[b], *sh);
     }
     lica_emul_nthreads = saved;

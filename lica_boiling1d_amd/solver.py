@@ -117,6 +117,25 @@ class Batch(_Handle):
         rc = self._lib.lica_batch_status(self._h, ctypes.byref(bad))
         return rc, bad.value
 
+    def progress(self):
+        """(status, completed steps) of every case."""
+        st = np.zeros(len(self.cases), dtype=np.int64)
+        steps = np.zeros(len(self.cases), dtype=np.int64)
+        _check(self._lib.lica_batch_get_progress(self._h, st.ctypes.data_as(_abi.P_i64),
+                                                 steps.ctypes.data_as(_abi.P_i64)), "get_progress")
+        return st, steps
+
+    def run_all(self, nsteps):
+        """Advance every healthy case by nsteps; a case that stops (the
+        reference STOPs there) keeps its status and is left behind.
+        Returns the number of cases with a non-zero status."""
+        rc = self._lib.lica_batch_run(self._h, int(nsteps))
+        if rc not in (0,) and rc < 20:
+            st, _ = self.progress()
+            return int(np.count_nonzero(st))
+        _check(rc, "lica_batch_run")
+        return 0
+
     def timings(self):
         """GPU ms per kernel class of the last run (timing mode) and the wall ms."""
         out = np.zeros(16)
