@@ -57,31 +57,45 @@ DEV double extp_point(double a, double al, double ar, double np, double npl, dou
 }
 
 // The pseudo-time loops of TEMP_EXTP (chphase.f:619-679 for dT/dn, 704-769
-// for T) held in registers: window cell i = XCELL(s) of slot s of the lane
-// (two consecutive cells per lane of the one wave; the CPU logic-check
-// build keeps the whole window in one lane), neighbours by lane shuffles,
-// RAT by a wave reduction.  Every cell update is extp_point in the
-// reference's Jacobi order (all cells from the previous stage).
+// for T) held in registers: window cell i = CPL*lane + s of slot s of the
+// lane (one or two cells per lane of the one wave; the CPU logic-check build
+// keeps the whole window in one lane), neighbours by lane shuffles, RAT by a
+// wave reduction.  Every cell update equals extp_point in the reference's
+// Jacobi order (all cells from the previous stage); the parts of it that
+// depend only on NPHI/PSIH are evaluated once per loop.
+template <int CPL>
+struct XCells {
+    double vv[CPL], coef[CPL], dtn[CPL];
+    int upd[CPL], live[CPL], small[CPL];
+};
+
 #ifdef LICA_EMUL
-#define XCPL NSWMAX
-#define XCELL(s) (s)
+template <int CPL>
+DEV int xcell(int s) { return s; }
+template <int CPL>
 DEV void xneighbors(const double *a, double *l, double *r)
 {
-    for (int s = 0; s < XCPL; s++) {
+    for (int s = 0; s < CPL; s++) {
         l[s] = s > 0 ? a[s - 1] : a[s];
-        r[s] = s + 1 < XCPL ? a[s + 1] : a[s];
+        r[s] = s + 1 < CPL ? a[s + 1] : a[s];
     }
 }
 DEV double xwave_max(double m) { return m; }
 #else
-#define XCPL 2
-#define XCELL(s) (2 * (int)threadIdx.x + (s))
+template <int CPL>
+DEV int xcell(int s) { return CPL * (int)threadIdx.x + s; }
+template <int CPL>
 DEV void xneighbors(const double *a, double *l, double *r)
 {
-    double up = __shfl_up(a[1], 1, 64);
-    double dn = __shfl_down(a[0], 1, 64);
-    l[0] = up; r[0] = a[1];
-    l[1] = a[0]; r[1] = dn;
+    if constexpr (CPL == 1) {
+        l[0] = __shfl_up(a[0], 1, 64);
+        r[0] = __shfl_down(a[0], 1, 64);
+    } else {
+        double up = __shfl_up(a[1], 1, 64);
+        double dn = __shfl_down(a[0], 1, 64);
+        l[0] = up; r[0] = a[1];
+        l[1] = a[0]; r[1] = dn;
+    }
 }
 DEV double xwave_max(double m)
 {
@@ -90,43 +104,74 @@ DEV double xwave_max(double m)
 }
 #endif
 
-struct XCells {
-    double np[XCPL], npl[XCPL], npr[XCPL], vv[XCPL], vvp[XCPL], ps[XCPL], dtn[XCPL];
-    int upd[XCPL];
-};
-
 // one forward-Euler stage on a (cells outside [ia,ib] stay as they are);
 // the dT/dn pass (nd=0) uses the array being advanced as its DTEMPN
-DEV void extp_stage(double *a, const XCells &c, int im, int nd, double redtl, double sm_grid, double sdx1)
+template <int CPL>
+DEV void extp_stage(double *a, const XCells<CPL> &c, int im, int nd, double redtl, double inv_sdx1, double fmn,
+                    double fmx)
 {
-    double l[XCPL], r[XCPL], tn[XCPL];
-    xneighbors(a, l, r);
-    for (int s = 0; s < XCPL; s++) {
-        double dt = (nd == 0) ? a[s] : c.dtn[s];
-        tn[s] = c.upd[s] ? extp_point(a[s], l[s], r[s], c.np[s], c.npl[s], c.npr[s], c.vv[s], c.vvp[s], c.ps[s], dt,
-                                      im, nd, redtl, sm_grid, sdx1)
-                         : a[s];
+    double l[CPL], r[CPL], tn[CPL];
+    xneighbors<CPL>(a, l, r);
+    const double ndd = (double)nd;
+    for (int s = 0; s < CPL; s++) {
+        double v = a[s];
+        if (c.upd[s] && c.live[s]) {
+            // extp_point with nx=ny=0 (or -0), nz=+-1: fmn/fmx are min/max(nx,0),
+            // the z factors are min/max(nz,0)
+            double apxm = (v - v) * inv_sdx1;
+            double apxp = apxm;
+            double apzm = (v - l[s]) * c.vv[s];
+            double apzp = (r[s] - v) * c.vv[s];
+            double ax = fmn * apxp + fmx * apxm;
+            double az = (im == 1) ? ((-1.0) * apzp + 0.0 * apzm) : (0.0 * apzp + 1.0 * apzm);
+            double dt = (nd == 0) ? v : c.dtn[s];
+            double sum = ax + ax + az;
+            double drhs = (im == 1) ? sum + ndd * dt : sum - ndd * dt;
+            double t = v - redtl * drhs * c.coef[s];
+            if (c.small[s]) t = v - redtl * sum;
+            v = t;
+        }
+        tn[s] = v;
     }
-    for (int s = 0; s < XCPL; s++) a[s] = tn[s];
+    for (int s = 0; s < CPL; s++) a[s] = tn[s];
 }
 
 // TEMP_EXTP's two pseudo-time loops for phase im on the LDS window tw (the
 // dT/dn field is in B.dtn on entry).  Returns ST_EXTRAP after 20 halvings.
+template <int CPL>
 DEV int temp_extp_loops(BandSh &B, const InstParam &P, const double *VVDZ, double *tw, int im, int ia, int ib,
                         double sm_grid, double eps, double *iters, double *halv)
 {
     const int nlo = B.nlo, nw = B.nhi - B.nlo + 1;
-    XCells c;
-    double a[XCPL], rk[XCPL];
-    for (int s = 0; s < XCPL; s++) {
-        int i = XCELL(s), k = nlo + i;
-        bool ok = i < nw;
-        c.np[s] = ok ? B.nphi[i] : 0.0;
-        c.npl[s] = (ok && i > 0) ? B.nphi[i - 1] : 0.0;
-        c.npr[s] = (ok && i + 1 < nw) ? B.nphi[i + 1] : 0.0;
-        c.vv[s] = ok ? VVDZ[k] : 0.0;
-        c.vvp[s] = ok ? VVDZ[k + 1] : 0.0;
-        c.ps[s] = ok ? B.psih[i] : 0.0;
+    const double inv_sdx1 = 1.0 / P.sdx1;
+    double nx = 0.0;
+    if (im == 1) nx = (-1.0) * nx;
+    const double fmn = fmin(nx, 0.0), fmx = fmax(nx, 0.0);
+    XCells<CPL> c;
+    double a[CPL], rk[CPL];
+    for (int s = 0; s < CPL; s++) {
+        const int i = xcell<CPL>(s), k = nlo + i;
+        const bool ok = i < nw;
+        const double np = ok ? B.nphi[i] : 0.0;
+        const double npl = (ok && i > 0) ? B.nphi[i - 1] : 0.0;
+        const double npr = (ok && i + 1 < nw) ? B.nphi[i + 1] : 0.0;
+        const double vv = ok ? VVDZ[k] : 0.0, vvp = ok ? VVDZ[k + 1] : 0.0;
+        const double ps = ok ? B.psih[i] : 0.0;
+        // the static part of extp_point
+        int live = 0;
+        if (np != 0.0) {
+            double dpx = 0.5 * (np - np) * (1.0 / P.sdx1);
+            double dpz;
+            if (npl == 0.0) dpz = (npr - np) * vvp;
+            else if (npr == 0.0) dpz = (np - npl) * vv;
+            else dpz = 0.5 * (npr - npl) * vv;
+            double dalphi = dpx * dpx + dpx * dpx + dpz * dpz;
+            live = dalphi != 0.0;
+        }
+        c.vv[s] = vv;
+        c.coef[s] = (im == 1) ? (1.0 - ps) : ps;
+        c.small[s] = fabs(np) <= sm_grid;
+        c.live[s] = live;
         c.dtn[s] = ok ? B.dtn[i] : 0.0;
         c.upd[s] = ok && k >= ia && k <= ib;
         a[s] = c.dtn[s];
@@ -137,13 +182,13 @@ DEV int temp_extp_loops(BandSh &B, const InstParam &P, const double *VVDZ, doubl
     for (;;) {
         int conv = 0;
         for (int it = 1; it <= 100; it++) {
-            for (int s = 0; s < XCPL; s++) rk[s] = a[s];
-            extp_stage(a, c, im, 0, redtl, sm_grid, P.sdx1);
-            extp_stage(a, c, im, 0, redtl, sm_grid, P.sdx1);
-            for (int s = 0; s < XCPL; s++) if (c.upd[s]) a[s] = 0.75 * rk[s] + 0.25 * a[s];
-            extp_stage(a, c, im, 0, redtl, sm_grid, P.sdx1);
+            for (int s = 0; s < CPL; s++) rk[s] = a[s];
+            extp_stage<CPL>(a, c, im, 0, redtl, inv_sdx1, fmn, fmx);
+            extp_stage<CPL>(a, c, im, 0, redtl, inv_sdx1, fmn, fmx);
+            for (int s = 0; s < CPL; s++) if (c.upd[s]) a[s] = 0.75 * rk[s] + 0.25 * a[s];
+            extp_stage<CPL>(a, c, im, 0, redtl, inv_sdx1, fmn, fmx);
             double m = 0.0;
-            for (int s = 0; s < XCPL; s++) {
+            for (int s = 0; s < CPL; s++) {
                 if (c.upd[s]) a[s] = 1.0 / 3.0 * rk[s] + 2.0 / 3.0 * a[s];
                 double v = 0.0;
                 if (rk[s] != 0.0) v = fabs(a[s] / rk[s] - 1.0);
@@ -159,22 +204,22 @@ DEV int temp_extp_loops(BandSh &B, const InstParam &P, const double *VVDZ, doubl
         *halv += 1.0;
         redtl = 0.5 * redtl;
     }
-    for (int s = 0; s < XCPL; s++) c.dtn[s] = a[s];
+    for (int s = 0; s < CPL; s++) { c.dtn[s] = a[s]; c.small[s] = 0; }
     // (b) T (ND=1) with the extrapolated dT/dn as source
-    double t[XCPL];
-    for (int s = 0; s < XCPL; s++) { int i = XCELL(s); t[s] = (i < nw) ? tw[i] : 0.0; }
+    double t[CPL];
+    for (int s = 0; s < CPL; s++) { int i = xcell<CPL>(s); t[s] = (i < nw) ? tw[i] : 0.0; }
     redtl = 0.2 * sm_grid;
     halvings = 0;
     for (;;) {
         int conv = 0;
         for (int it = 1; it <= 100; it++) {
-            for (int s = 0; s < XCPL; s++) { rk[s] = t[s]; a[s] = t[s]; }
-            extp_stage(a, c, im, 1, redtl, sm_grid, P.sdx1);
-            extp_stage(a, c, im, 1, redtl, sm_grid, P.sdx1);
-            for (int s = 0; s < XCPL; s++) if (c.upd[s]) a[s] = 0.75 * rk[s] + 0.25 * a[s];
-            extp_stage(a, c, im, 1, redtl, sm_grid, P.sdx1);
+            for (int s = 0; s < CPL; s++) { rk[s] = t[s]; a[s] = t[s]; }
+            extp_stage<CPL>(a, c, im, 1, redtl, inv_sdx1, fmn, fmx);
+            extp_stage<CPL>(a, c, im, 1, redtl, inv_sdx1, fmn, fmx);
+            for (int s = 0; s < CPL; s++) if (c.upd[s]) a[s] = 0.75 * rk[s] + 0.25 * a[s];
+            extp_stage<CPL>(a, c, im, 1, redtl, inv_sdx1, fmn, fmx);
             double m = 0.0;
-            for (int s = 0; s < XCPL; s++) {
+            for (int s = 0; s < CPL; s++) {
                 if (c.upd[s]) { a[s] = 1.0 / 3.0 * rk[s] + 2.0 / 3.0 * a[s]; t[s] = a[s]; }
                 double v = (rk[s] != 0.0) ? fabs(t[s] / rk[s] - 1.0) : 0.0;
                 if (c.upd[s]) m = fmax(m, v);
@@ -188,8 +233,8 @@ DEV int temp_extp_loops(BandSh &B, const InstParam &P, const double *VVDZ, doubl
         *halv += 1.0;
         redtl = 0.5 * redtl;
     }
-    for (int s = 0; s < XCPL; s++) {
-        int i = XCELL(s);
+    for (int s = 0; s < CPL; s++) {
+        int i = xcell<CPL>(s);
         if (i < nw) { tw[i] = t[s]; B.dtn[i] = c.dtn[s]; }
     }
     return ST_OK;
@@ -358,7 +403,12 @@ DEV void body_temp_grad(const BatchDev &bd, int inst, TgradSh &TS)
         // the two pseudo-time loops, in registers of the one wave
         BARRIER();
         double iters = 0.0, halv = 0.0;
-        int xst = temp_extp_loops(B, P, VVDZ, tw, im, ia, ib, sm_grid, eps, &iters, &halv);
+#ifdef LICA_EMUL
+        int xst = temp_extp_loops<NSWMAX>(B, P, VVDZ, tw, im, ia, ib, sm_grid, eps, &iters, &halv);
+#else
+        int xst = (nhi - nlo + 1 <= 64) ? temp_extp_loops<1>(B, P, VVDZ, tw, im, ia, ib, sm_grid, eps, &iters, &halv)
+                                        : temp_extp_loops<2>(B, P, VVDZ, tw, im, ia, ib, sm_grid, eps, &iters, &halv);
+#endif
         BARRIER();
         SERIAL {
             S.n_extrap_iters += iters;
