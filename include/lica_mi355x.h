@@ -22,6 +22,10 @@
  *   lica_poisson         POISSON(P,DIVGSUM,DENF_XI,DENF_YI,DENF_ZI,DENF_Y,DENF_C)
  *                        1d.f:3492 (caller 1d.f:3428), batched
  *   lica_batch_*         batched driver of the same loop for many cases
+ *   lica_batch_write_fld WRITEFIELD 1d.f:1121-1232 (the fld###### restart/post file)
+ *   lica_batch_save/load no counterpart: the reference's fld restart keeps 12
+ *                        digits and omits PSIF/TEMP/WO/MFLVS; these keep every
+ *                        byte of the device state
  * Errors are returned as status codes (the reference STOPs).
  */
 #ifndef LICA_MI355X_H
@@ -118,6 +122,13 @@ int64_t lica_batch_size(lica_batch *b);
 /* GPU time of the last lica_batch_run, per kernel class (ms); returns count */
 int lica_batch_last_timings(lica_batch *b, double *ms_out, int64_t cap);
 void lica_batch_destroy(lica_batch *b);
+
+/* ---- checkpoint / restart -------------------------------------------- */
+/* binary checkpoint of the whole device state: a restart continues bit for bit */
+int lica_batch_save(lica_batch *b, const char *path);
+int lica_batch_load(const char *path, lica_batch **out);
+/* WRITEFIELD text file (1d.f:1121-1232, read by post.f READFLD 238-401) of one case */
+int lica_batch_write_fld(lica_batch *b, int64_t inst, const char *path, int64_t ihist);
 
 /* ---- single-case API mirroring the reference call sequence ----------- */
 int lica_init(lica_batch **ctx, const char *lica_in_path, const char *grid_in_path);

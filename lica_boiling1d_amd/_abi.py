@@ -88,6 +88,9 @@ def _declare(lib):
         "lica_batch_get_progress": (ctypes.c_int, [V, P_i64, P_i64]),
         "lica_batch_last_timings": (ctypes.c_int, [V, P_dbl, c_i64]),
         "lica_batch_destroy": (None, [V]),
+        "lica_batch_save": (ctypes.c_int, [V, ctypes.c_char_p]),
+        "lica_batch_load": (ctypes.c_int, [ctypes.c_char_p, PV]),
+        "lica_batch_write_fld": (ctypes.c_int, [V, c_i64, ctypes.c_char_p, c_i64]),
         "lica_init": (ctypes.c_int, [PV, ctypes.c_char_p, ctypes.c_char_p]),
         "lica_step": (ctypes.c_int, [V, c_i64]),
         "lica_determine_dt": (ctypes.c_int, [V, P_dbl, P_dbl]),

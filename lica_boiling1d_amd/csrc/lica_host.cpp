@@ -977,3 +977,175 @@ extern "C" int lica_poisson(int64_t nsys, int64_t n, const double *a, const doub
     if (cycles_out) *cycles_out = 0;
     return rc;
 }
+
+// ---------------------------------------------------------------------
+// Checkpoint / restart.  The binary checkpoint holds every byte of the
+// batch's device state (fields, band, scalars, counters) so that a restart
+// continues bit for bit; the field file is the text format of WRITEFIELD
+// (1d.f:1121-1232), which post.f's READFLD (post.f:238-401) reads.
+
+namespace {
+const char CKPT_MAGIC[8] = {'L', 'I', 'C', 'A', 'C', 'K', 'P', '1'};
+
+template <class T>
+bool put(FILE *f, const T *p, size_t n) { return fwrite(p, sizeof(T), n, f) == n; }
+template <class T>
+bool get(FILE *f, T *p, size_t n) { return fread(p, sizeof(T), n, f) == n; }
+}  // namespace
+
+extern "C" int lica_batch_save(lica_batch *b, const char *path)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (!path || !b->initialized) return LICA_BAD_ARG;
+    rc = dsync(b->stream);
+    if (rc) return rc;
+    std::vector<InstState> st((size_t)b->ninst);
+    std::vector<double> ns(b->ns_total), rf(b->rf_total);
+    std::vector<int> nsi(b->nsi_total), rfi(b->rfi_total);
+    rc = d2h(st.data(), b->d_st, sizeof(InstState) * st.size(), b->stream);
+    rc = rc ? rc : d2h(ns.data(), b->d_ns, sizeof(double) * ns.size(), b->stream);
+    rc = rc ? rc : d2h(rf.data(), b->d_rf, sizeof(double) * rf.size(), b->stream);
+    rc = rc ? rc : d2h(nsi.data(), b->d_nsi, sizeof(int) * nsi.size(), b->stream);
+    rc = rc ? rc : d2h(rfi.data(), b->d_rfi, sizeof(int) * rfi.size(), b->stream);
+    if (rc) return rc;
+    FILE *f = fopen(path, "wb");
+    if (!f) return LICA_IO;
+    const int64_t hdr[6] = {b->ninst, (int64_t)sizeof(InstState), (int64_t)b->ns_total, (int64_t)b->rf_total,
+                            (int64_t)b->nsi_total, (int64_t)b->rfi_total};
+    bool ok = put(f, CKPT_MAGIC, 8) && put(f, hdr, 6) && put(f, b->cases.data(), b->cases.size()) &&
+              put(f, st.data(), st.size()) && put(f, ns.data(), ns.size()) && put(f, rf.data(), rf.size()) &&
+              put(f, nsi.data(), nsi.size()) && put(f, rfi.data(), rfi.size());
+    ok = (fclose(f) == 0) && ok;
+    return ok ? LICA_OK : LICA_IO;
+}
+
+extern "C" int lica_batch_load(const char *path, lica_batch **out)
+{
+    if (!path || !out) return LICA_BAD_ARG;
+    *out = nullptr;
+    FILE *f = fopen(path, "rb");
+    if (!f) return LICA_IO;
+    char magic[8];
+    int64_t hdr[6];
+    if (!get(f, magic, 8) || memcmp(magic, CKPT_MAGIC, 8) != 0 || !get(f, hdr, 6) || hdr[0] <= 0 ||
+        hdr[1] != (int64_t)sizeof(InstState)) {
+        fclose(f);
+        return LICA_IO;
+    }
+    std::vector<lica_case_desc> cases((size_t)hdr[0]);
+    if (!get(f, cases.data(), cases.size())) { fclose(f); return LICA_IO; }
+    lica_batch *b = nullptr;
+    int rc = lica_batch_create(hdr[0], cases.data(), &b);
+    if (rc) { fclose(f); return rc; }
+    if ((int64_t)b->ns_total != hdr[2] || (int64_t)b->rf_total != hdr[3] || (int64_t)b->nsi_total != hdr[4] ||
+        (int64_t)b->rfi_total != hdr[5]) {
+        fclose(f);
+        lica_batch_destroy(b);
+        return LICA_IO;
+    }
+    std::vector<InstState> st((size_t)hdr[0]);
+    std::vector<double> ns(b->ns_total), rf(b->rf_total);
+    std::vector<int> nsi(b->nsi_total), rfi(b->rfi_total);
+    bool ok = get(f, st.data(), st.size()) && get(f, ns.data(), ns.size()) && get(f, rf.data(), rf.size()) &&
+              get(f, nsi.data(), nsi.size()) && get(f, rfi.data(), rfi.size());
+    fclose(f);
+    if (!ok) { lica_batch_destroy(b); return LICA_IO; }
+    rc = h2d(b->d_st, st.data(), sizeof(InstState) * st.size(), b->stream);
+    rc = rc ? rc : h2d(b->d_ns, ns.data(), sizeof(double) * ns.size(), b->stream);
+    rc = rc ? rc : h2d(b->d_rf, rf.data(), sizeof(double) * rf.size(), b->stream);
+    rc = rc ? rc : h2d(b->d_nsi, nsi.data(), sizeof(int) * nsi.size(), b->stream);
+    rc = rc ? rc : h2d(b->d_rfi, rfi.data(), sizeof(int) * rfi.size(), b->stream);
+    if (rc) { lica_batch_destroy(b); return rc; }
+    b->initialized = true;
+    *out = b;
+    return LICA_OK;
+}
+
+namespace {
+// Fortran list of REAL*8 in rows of `per` ESw.12 fields (FORMAT(nES20.12))
+void fld_reals(FILE *f, const std::vector<double> &v, int per)
+{
+    for (size_t i = 0; i < v.size(); i++) {
+        fprintf(f, "%20.12E", v[i]);
+        if ((i + 1) % (size_t)per == 0 || i + 1 == v.size()) fputc('\n', f);
+    }
+    if (v.empty()) fputc('\n', f);
+}
+void fld_ints(FILE *f, const std::vector<long long> &v, int per)
+{
+    for (size_t i = 0; i < v.size(); i++) {
+        fprintf(f, "%8lld", v[i]);
+        if ((i + 1) % (size_t)per == 0 || i + 1 == v.size()) fputc('\n', f);
+    }
+    if (v.empty()) fputc('\n', f);
+}
+}  // namespace
+
+// WRITEFIELD (1d.f:1121-1232) for case inst: the NS grid has N1=N2=2 nodes
+// (one cell) in x and y, every x/y copy of the column holds the same values,
+// U=V=0; the band list is written plane by plane with its r x r columns.
+extern "C" int lica_batch_write_fld(lica_batch *b, int64_t inst, const char *path, int64_t ihist)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (inst < 0 || inst >= b->ninst || !path) return LICA_BAD_ARG;
+    const InstParam &p = b->par[(size_t)inst];
+    const lica_case_desc &cd = b->cases[(size_t)inst];
+    InstState s;
+    rc = get_state(b, inst, &s);
+    if (rc) return rc;
+    const int n = p.n, n3 = p.n3, r = p.r;
+    std::vector<double> W((size_t)n3 + 1), P((size_t)n3 + 1), T((size_t)n3 + 1);
+    auto nsget = [&](int fld, std::vector<double> &v) {
+        return d2h(v.data(), b->d_ns + p.ns_off + (size_t)fld * p.ln + 2, sizeof(double) * v.size(), b->stream);
+    };
+    rc = nsget(NS_W, W);
+    rc = rc ? rc : nsget(NS_P, P);
+    rc = rc ? rc : nsget(NS_T, T);
+    if (rc) return rc;
+    FILE *f = fopen(path, "w");
+    if (!f) return LICA_IO;
+    const int N1 = 2, N2 = 2, N3 = n3;
+    fprintf(f, "%8d%8d%8d\n", N1, N2, N3);
+    fprintf(f, "%8lld%8lld%20.12E%20.12E\n", (long long)ihist, (long long)s.m, s.time, s.dt);
+    fprintf(f, "%8d%8d%8d%8d%20.12E\n", 1, 1, 1, 3, cd.prm);
+    fld_reals(f, {0.0}, 1);                                   // QVOL_ORI
+    std::vector<double> v;
+    v.assign((size_t)N1 * (N2 + 1) * (N3 + 1), 0.0);          // U(1:N1,0:N2,0:N3)
+    fld_reals(f, v, 5);
+    v.assign((size_t)(N1 + 1) * N2 * (N3 + 1), 0.0);          // V(0:N1,1:N2,0:N3)
+    fld_reals(f, v, 5);
+    v.clear();                                                // W(0:N1,0:N2,1:N3)
+    for (int k = 1; k <= N3; k++)
+        for (int j = 0; j <= N2; j++)
+            for (int i = 0; i <= N1; i++) v.push_back(W[(size_t)k]);
+    fld_reals(f, v, 5);
+    v.clear();                                                // P(1:N1M,1:N2M,1:N3M)
+    for (int k = 1; k <= n; k++) v.push_back(P[(size_t)k]);
+    fld_reals(f, v, 5);
+    fprintf(f, "%8d%8d%8d\n", r + 1, r + 1, p.n3f);
+    fld_reals(f, {cd.denr, cd.visr, cd.fr, cd.surf_j, 0.0, cd.re}, 6);
+    fprintf(f, "%8d%8d\n", 1, LICA_MLAYER);
+    fld_reals(f, {s.vol_tot_ori}, 8);
+    std::vector<long long> numa;
+    for (int nl = 1; nl <= LICA_MLAYER; nl++) numa.push_back((long long)s.bn[nl] * r * r);
+    fld_ints(f, numa, 8);
+    for (int nl = 1; nl <= LICA_MLAYER; nl++)
+        for (int q = 0; q < s.bn[nl]; q++)
+            for (int j = 1; j <= r; j++)
+                for (int i = 1; i <= r; i++) fprintf(f, "%8d%8d%8d%20.12E\n", i, j, s.bk[nl][q], s.alphi[nl][q]);
+    fld_ints(f, {0}, 4);                                      // CONTACT
+    fld_ints(f, {0}, 4);                                      // CONTACT_OLD
+    fld_reals(f, {0.0}, 5);                                   // CONTACT_START
+    fld_reals(f, {0.0}, 5);                                   // CONTACT_TIME
+    fld_ints(f, {1}, 4);                                      // LVSON
+    fld_reals(f, {0.0}, 5);                                   // T_CON
+    fld_reals(f, {cd.prm, cd.scr, cd.tcr}, 3);
+    v.clear();                                                // T(0:N1,0:N2,0:N3)
+    for (int k = 0; k <= N3; k++)
+        for (int j = 0; j <= N2; j++)
+            for (int i = 0; i <= N1; i++) v.push_back(T[(size_t)k]);
+    fld_reals(f, v, 5);
+    return (fclose(f) == 0) ? LICA_OK : LICA_IO;
+}

@@ -109,7 +109,51 @@ DEV double snap5(double v) { return (v < 1.0e-5) ? 0.0 : ((v > 0.99999) ? 1.0 : 
 DEV double snap3(double v) { return (v < 1.0e-3) ? 0.0 : ((v > 0.999) ? 1.0 : v); }
 
 // box average of a refined field over [z1,z2] (INTER_PSI* family with the
-// x/y loops of the single NS column collapsed)
+// x/y loops of the single NS column collapsed); zf(k) and f(k) are accessors
+template <class Z, class F>
+DEV double box_avg_acc(Z zf, double ssdzf, F f, double z1, double z2, int k1, int k2)
+{
+    double sum = 0.0, volf = 0.0;
+    for (int kk = k1; kk <= k2; kk++) {
+        double zl = zf(kk), zh = zf(kk + 1);
+        if (zl <= z1 && zh >= z2) { sum = sum + f(kk); volf = volf + 1.0; break; }
+        else if (zl < z1 && zh > z1) { sum = sum + f(kk) * (zh - z1) * ssdzf; volf = volf + (zh - z1) * ssdzf; }
+        else if (zl >= z1 && zh <= z2) { sum = sum + f(kk); volf = volf + 1.0; }
+        else if (zl < z2 && zh > z2) { sum = sum + f(kk) * (z2 - zl) * ssdzf; volf = volf + (z2 - zl) * ssdzf; }
+        else if (zl >= z2) break;
+    }
+    return sum / volf;
+}
+
+// A window [lo, hi] of refined z nodes and of up to two refined fields staged
+// in LDS, so that the dependent loads of the box averages near the interface
+// are LDS loads; indices outside the window read global memory.
+#define RWMAX 384
+struct RefWin {
+    double zf[RWMAX + 1];
+    double f0[RWMAX], f1[RWMAX];
+    int lo, hi;           // staged refined cells; hi < lo: nothing staged
+};
+
+// stage [lo, hi] (clamped to [klo, khi]) of zf and of f0/f1 (either may be null)
+DEV void stage_refwin(RefWin &w, int lo, int hi, int klo, int khi, const double *zf, const double *f0,
+                      const double *f1)
+{
+    if (lo < klo) lo = klo;
+    if (hi > khi) hi = khi;
+    if (hi - lo + 1 > RWMAX || hi < lo) { lo = 1; hi = 0; }
+    PFOR(i, 0, hi - lo + 2) {
+        w.zf[i] = zf[lo + i];
+        if (i <= hi - lo) {
+            if (f0) w.f0[i] = f0[lo + i];
+            if (f1) w.f1[i] = f1[lo + i];
+        }
+    }
+    PFOR_END
+    SERIAL { w.lo = lo; w.hi = hi; }
+    SERIAL_END
+}
+
 template <class F>
 DEV double box_avg(const double *zf, double ssdzf, F f, double z1, double z2, int k1, int k2)
 {
@@ -563,6 +607,7 @@ struct BlkSh {
     double v[16];
     int iv[16];
     TriSh tri;     // points at the kernel's dynamic LDS (tri_ws_rows(blockDim) x 4 doubles)
+    RefWin win;    // refined window of the box averages
 };
 
 // DETERMINE_DT (1d.f:683-817) at the start of a step, with the per-step
@@ -631,29 +676,48 @@ DEV void body_cont_energy(const BatchDev &bd, int inst, BlkSh &sh)
     const int *KCOU1 = cp.AI(NSI_KCOU1), *KCOU2 = cp.AI(NSI_KCOU2);
     const int *KCOUMP1 = cp.AI(NSI_KCOUMP1), *KCOUMP2 = cp.AI(NSI_KCOUMP2);
 
+    // substep constants (1d.f:221-225), evaluated by every thread
+    const double dt = S.dt;
+    const long long m = S.m;
+    const double alpha = 0.5 * (P.gamma[msub - 1] + P.ro[msub - 1]);
+    const double dtconst = 2.0 * alpha * dt;
     SERIAL {
         S.msub = msub;
-        S.alpha = 0.5 * (P.gamma[msub - 1] + P.ro[msub - 1]);
-        S.acoef = S.alpha * S.dt;
-        S.dtconst = 2.0 * S.alpha * S.dt;
-        S.dtconsti = 1.0 / S.dtconst;
+        S.alpha = alpha;
+        S.acoef = alpha * dt;
+        S.dtconst = dtconst;
+        S.dtconsti = 1.0 / dtconst;
     }
     SERIAL_END
-    const double dt = S.dt, alpha = S.alpha, dtconst = S.dtconst;
-    const long long m = S.m;
 
+    // the donor boxes of region-2 cells lie within a few NS cells of the
+    // jumps of PSI_CN: stage that stretch of ZF and PSIF in LDS
+    {
+        const double jlo = red_min(0, n + 1, 1.0e9, [&](int k) { return (PCN[k + 1] != PCN[k]) ? (double)k : 1.0e9; }, sh.red);
+        const double jhi = red_max(0, n + 1, -1.0, [&](int k) { return (PCN[k + 1] != PCN[k]) ? (double)k : -1.0; }, sh.red);
+        int lo = 1, hi = 0;
+        if (jhi >= 0.0) {
+            const int ka = (int)jlo - 5 < -1 ? -1 : (int)jlo - 5, kb = (int)jhi + 6 > n3 + 2 ? n3 + 2 : (int)jhi + 6;
+            lo = KCOU1[ka] - 2;
+            hi = KCOU2[kb] + 2;
+        }
+        stage_refwin(sh.win, lo, hi, P.m3l, P.n3f + P.r, ZF, PSIF, nullptr);
+    }
+    const RefWin &win = sh.win;
+    auto zfa = [&](int kk) { return (kk >= win.lo && kk <= win.hi + 1) ? win.zf[kk - win.lo] : ZF[kk]; };
+    auto psa = [&](int kk) { return (kk >= win.lo && kk <= win.hi) ? win.f0[kk - win.lo] : PSIF[kk]; };
     // PSIFACE_Z / PSIFACE_C (1d.f:2349-2549) and the face densities
     auto psiface_z = [&](int k, double uu) {
         double z1, z2; int k1, k2;
         if (uu >= 0.0) { z1 = ZP[k] - uu * dtconst; z2 = ZP[k]; k1 = KCOUMP2[k - 1]; k2 = KCOUMP1[k]; }
         else { z1 = ZP[k]; z2 = ZP[k] - uu * dtconst; k1 = KCOUMP2[k]; k2 = KCOUMP1[k + 1]; }
-        return box_avg(ZF, P.ssdzf, [&](int kk) { return PSIF[kk]; }, z1, z2, k1, k2);
+        return box_avg_acc(zfa, P.ssdzf, psa, z1, z2, k1, k2);
     };
     auto psiface_c = [&](int k, double uu) {
         double z1, z2; int k1, k2;
         if (uu >= 0.0) { z1 = Z[k + 1] - uu * dtconst; z2 = Z[k + 1]; k1 = KCOU2[k]; k2 = KCOU1[k + 1]; }
         else { z1 = Z[k + 1]; z2 = Z[k + 1] - uu * dtconst; k1 = KCOU2[k + 1]; k2 = KCOU1[k + 2]; }
-        return box_avg(ZF, P.ssdzf, [&](int kk) { return PSIF[kk]; }, z1, z2, k1, k2);
+        return box_avg_acc(zfa, P.ssdzf, psa, z1, z2, k1, k2);
     };
     // face value computed by a region-2 iteration for the z face at ZP(f)
     auto zface2 = [&](int f, double *vf) {

@@ -31,10 +31,14 @@ DEV void body_momentum(const BatchDev &bd, int inst, BlkSh &sh)
     const long long m = S.m;
     const double dt = S.dt, alpha = S.alpha, acoef = S.acoef, dtconst = S.dtconst, dtconsti = S.dtconsti;
     const int sfl = S.sf_lo, sfh = S.sf_hi;
-    auto fsf = [&](int kk) { return SFGAS[kk]; };
-    auto fsf2 = [&](int kk) { return SFGAS2[kk]; };
-    // a refined range [k1,k2] that misses the nonzero SFGAS window averages to 0
+    // a refined range [k1,k2] that misses the nonzero SFGAS window averages to 0;
+    // the boxes that touch it read ZF/SFGAS/SFGAS2 staged in LDS
     auto touches = [&](int k1, int k2) { return !(k2 < sfl || k1 > sfh); };
+    stage_refwin(sh.win, sfl - 2 * P.r - 4, sfh + 2 * P.r + 4, P.m3l, P.n3f + P.r, ZF, SFGAS, SFGAS2);
+    const RefWin &win = sh.win;
+    auto zfa = [&](int kk) { return (kk >= win.lo && kk <= win.hi + 1) ? win.zf[kk - win.lo] : ZF[kk]; };
+    auto fsf = [&](int kk) { return (kk >= win.lo && kk <= win.hi) ? win.f0[kk - win.lo] : SFGAS[kk]; };
+    auto fsf2 = [&](int kk) { return (kk >= win.lo && kk <= win.hi) ? win.f1[kk - win.lo] : SFGAS2[kk]; };
 
     if (m != 1) {
         // CAL_CONTINUITY2 (1d.f:3006-3169)
@@ -43,7 +47,7 @@ DEV void body_momentum(const BatchDev &bd, int inst, BlkSh &sh)
             if (k >= 2) {
                 double pz = PZ[k];
                 if (rg == 2 && touches(KCOU2[k], KCOU1[k + 1]))
-                    pz = pz - box_avg(ZF, P.ssdzf, fsf2, ZP[k - 1], ZP[k], KCOU2[k], KCOU1[k + 1]);
+                    pz = pz - box_avg_acc(zfa, P.ssdzf, fsf2, ZP[k - 1], ZP[k], KCOU2[k], KCOU1[k + 1]);
                 else if (rg == 2)
                     pz = pz - 0.0;
                 if (pz > 1.0) pz = 1.0;
@@ -52,7 +56,7 @@ DEV void body_momentum(const BatchDev &bd, int inst, BlkSh &sh)
             }
             double pc = PC[k];
             if (rg == 2 && touches(KCOU2[k], KCOU1[k + 1]))
-                pc = pc - box_avg(ZF, P.ssdzf, fsf2, Z[k], Z[k + 1], KCOU2[k], KCOU1[k + 1]);
+                pc = pc - box_avg_acc(zfa, P.ssdzf, fsf2, Z[k], Z[k + 1], KCOU2[k], KCOU1[k + 1]);
             if (pc > 1.0) pc = 1.0;
             else if (pc < 0.0) pc = 0.0;
             PC[k] = pc;
@@ -114,7 +118,7 @@ DEV void body_momentum(const BatchDev &bd, int inst, BlkSh &sh)
     // DIVGS (1d.f:3180-3303) and the POISSON matrix (3529-3543)
     PFOR(k, 1, n + 1) {
         double d = (k <= n - 1) ? (W[k + 1] - W[k]) * SSDZ[k] : 0.0;
-        double ss = touches(KCOU2[k], KCOU1[k + 1]) ? box_avg(ZF, P.ssdzf, fsf, Z[k], Z[k + 1], KCOU2[k], KCOU1[k + 1]) : 0.0;
+        double ss = touches(KCOU2[k], KCOU1[k + 1]) ? box_avg_acc(zfa, P.ssdzf, fsf, Z[k], Z[k + 1], KCOU2[k], KCOU1[k + 1]) : 0.0;
         d = d - ss * (P.den_diff / (P.denp * P.denm));
         if (k >= 2) d = d * dtconsti;
         double fixkl = (k == 1) ? 1.0 : 0.0;
@@ -175,11 +179,14 @@ DEV void body_ucalc(const BatchDev &bd, int inst, BlkSh &sh)
     PFOR_END
     SERIAL { PP[0] = PP[1]; }
     SERIAL_END
-    auto fsf = [&](int kk) { return SFGAS[kk]; };
+    stage_refwin(sh.win, sfl - 2 * P.r - 4, sfh + 2 * P.r + 4, P.m3l, P.n3f + P.r, ZF, SFGAS, nullptr);
+    const RefWin &win = sh.win;
+    auto zfa = [&](int kk) { return (kk >= win.lo && kk <= win.hi + 1) ? win.zf[kk - win.lo] : ZF[kk]; };
+    auto fsf = [&](int kk) { return (kk >= win.lo && kk <= win.hi) ? win.f0[kk - win.lo] : SFGAS[kk]; };
     double dvmax = red_max(1, n + 1, 0.0, [&](int k) {
         double d = (W[k + 1] - W[k]) * SSDZ[k];
         bool t = !(KCOU1[k + 1] < sfl || KCOU2[k] > sfh);
-        double ss = t ? box_avg(ZF, P.ssdzf, fsf, Z[k], Z[k + 1], KCOU2[k], KCOU1[k + 1]) : 0.0;
+        double ss = t ? box_avg_acc(zfa, P.ssdzf, fsf, Z[k], Z[k + 1], KCOU2[k], KCOU1[k + 1]) : 0.0;
         d = d - ss * (P.den_diff / P.denp);
         return fabs(d);
     }, sh.red);

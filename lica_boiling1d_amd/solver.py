@@ -80,9 +80,12 @@ class _Handle:
 class Batch(_Handle):
     """Many independent boiling cases advanced together on the GPU."""
 
-    def __init__(self, cases, init=True, lib=None):
+    def __init__(self, cases, init=True, lib=None, _handle=None):
         super().__init__(lib)
         self.cases = list(cases)
+        if _handle is not None:
+            self._h = _handle
+            return
         arr = (CaseDesc * len(self.cases))(*self.cases)
         _check(self._lib.lica_batch_create(len(self.cases), arr, ctypes.byref(self._h)), "lica_batch_create")
         if init:
@@ -90,6 +93,30 @@ class Batch(_Handle):
 
     def __len__(self):
         return len(self.cases)
+
+    def save(self, path):
+        """Binary checkpoint of the whole device state (bit-exact restart)."""
+        _check(self._lib.lica_batch_save(self._h, str(path).encode()), "lica_batch_save")
+
+    @classmethod
+    def load(cls, path, lib=None):
+        lib = lib if lib is not None else _abi.load()
+        h = ctypes.c_void_p()
+        _check(lib.lica_batch_load(str(path).encode(), ctypes.byref(h)), "lica_batch_load")
+        n = int(lib.lica_batch_size(h))
+        # the case list is read back from the file header by the library; keep
+        # a Python copy for the getters' sizes
+        import struct
+        with open(path, "rb") as f:
+            f.read(8)
+            hdr = struct.unpack("<6q", f.read(48))
+            raw = f.read(ctypes.sizeof(CaseDesc) * n)
+        cases = [CaseDesc.from_buffer_copy(raw, i * ctypes.sizeof(CaseDesc)) for i in range(n)]
+        return cls(cases, lib=lib, _handle=h)
+
+    def write_fld(self, path, inst=0, ihist=0):
+        """WRITEFIELD text file of one case (1d.f:1121-1232)."""
+        _check(self._lib.lica_batch_write_fld(self._h, int(inst), str(path).encode(), int(ihist)), "write_fld")
 
     def init(self):
         _check(self._lib.lica_batch_init(self._h), "lica_batch_init")

@@ -176,3 +176,19 @@ def test_batched_poisson_systems(mode, n):
         A = np.diag(b[s]) + np.diag(a[s, 1:], -1) + np.diag(c[s, :-1], 1)
         x = np.linalg.solve(A, g[s])
         assert np.max(np.abs(p[s] - x)) <= 1e-10 * np.max(np.abs(x))
+
+
+def test_checkpoint_restart_bit_exact(tmp_path):
+    """Binary checkpoint of the device state: save, load, continue == uninterrupted."""
+    cases = L.batched_cases(4, seed=9, nmin=128, nmax=600)
+    _say("checkpoint/restart, 4 cases")
+    a = L.Batch(cases)
+    a.run(6)
+    a.save(tmp_path / "ck.bin")
+    a.run(6)
+    b = L.Batch.load(tmp_path / "ck.bin")
+    b.run(6)
+    for i in range(len(cases)):
+        for f in ("W", "P", "T", "PSIF", "PHI", "ALPHI", "MFLVS"):
+            assert np.array_equal(a.field(f, i), b.field(f, i)), (i, f)
+        assert np.array_equal(a.scalar_array(i), b.scalar_array(i))

@@ -74,3 +74,50 @@ def test_stopped_case_keeps_status_others_continue(emul):
     nbad = b.run_all(3)
     st, steps = b.progress()
     assert nbad == 1 and st[1] == st_o != 0 and st[0] == 0 and steps[0] == 3
+
+
+def test_checkpoint_restart_bit_exact(emul, tmp_path):
+    """save after 7 steps, load, 8 more == 15 uninterrupted steps"""
+    cases = L.batched_cases(3, seed=5, nmin=128, nmax=300)
+    a = L.Batch(cases, lib=emul)
+    a.run(7)
+    a.save(tmp_path / "ck.bin")
+    a.run(8)
+    b = L.Batch.load(tmp_path / "ck.bin", lib=emul)
+    b.run(8)
+    for i in range(len(cases)):
+        for f in ("W", "P", "T", "PSIF", "PHI", "ALPHI", "MFLVS", "SFGAS"):
+            assert np.array_equal(a.field(f, i), b.field(f, i)), (i, f)
+        assert np.array_equal(a.scalar_array(i), b.scalar_array(i))
+
+
+def test_writefield_file(emul, tmp_path):
+    """The WRITEFIELD layout (1d.f:1121-1232): headers, U/V/W/P blocks, band
+    list with r x r columns per plane, T block; values to 12 digits."""
+    case = L.deck_case()
+    b = L.Batch([case], lib=emul)
+    b.run(5)
+    path = tmp_path / "fld000005"
+    b.write_fld(path, 0, 5)
+    toks = path.read_text().split()
+    n, r = int(case.n), int(case.r)
+    N1, N2, N3 = 2, 2, n + 1
+    assert [int(t) for t in toks[:3]] == [N1, N2, N3]
+    assert int(toks[3]) == 5 and int(toks[4]) == 5
+    assert float(toks[5]) == pytest.approx(b.scalars()["TIME"], rel=1e-11)
+    pos = 5 + 2 + 4 + 1 + 1                      # TIME DT, IPS..PRA, QVOL_ORI
+    pos += N1 * (N2 + 1) * (N3 + 1) + (N1 + 1) * N2 * (N3 + 1)
+    w = np.array([float(t) for t in toks[pos:pos + (N1 + 1) * (N2 + 1) * N3]]).reshape(N3, N2 + 1, N1 + 1)
+    assert np.allclose(w[:, 1, 1], b.field("W")[1:], rtol=1e-11, atol=1e-300)
+    pos += (N1 + 1) * (N2 + 1) * N3
+    p = np.array([float(t) for t in toks[pos:pos + n]])
+    assert np.allclose(p, b.field("P")[1:n + 1], rtol=1e-11, atol=1e-300)
+    pos += n
+    assert [int(t) for t in toks[pos:pos + 3]] == [r + 1, r + 1, r * n + 1]
+    pos += 3 + 6 + 2 + 1
+    numa = [int(t) for t in toks[pos:pos + 11]]
+    band = b.field("BAND")
+    assert sum(numa) == r * r * (len(band) - 11)
+    pos += 11 + 4 * sum(numa) + 1 + 1 + 1 + 1 + 1 + 1 + 3
+    t = np.array([float(x) for x in toks[pos:]]).reshape(N3 + 1, N2 + 1, N1 + 1)
+    assert np.allclose(t[:, 0, 0], b.field("T"), rtol=1e-11, atol=1e-300)
