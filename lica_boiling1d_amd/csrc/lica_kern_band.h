@@ -32,7 +32,6 @@ DEV void band_index(BandSh &B, const InstParam &P, int set_window)
     }
 }
 
-DEV double dphi_get(const BandSh &B, int k) { return B.dphi[k - B.dlo]; }
 DEV double wf_get(const BandSh &B, int k) { return B.wf[k - B.dlo]; }
 
 // ALPHI_BC z part (two_phase.f:565-600) and the z copy (623-634); copies
@@ -55,109 +54,8 @@ DEV void alphi_bc(BandSh &B, const InstParam &P)
     }
 }
 
-// DPHI_Z (two_phase.f:1165-1232, 2206-2271)
-DEV void dphi_z(BandSh &B, const InstParam &P, int nn1)
-{
-    const int n3f = P.n3f;
-    PFOR(i, 0, WMAX + 8) { B.dphi[i] = 0.0; }
-    PFOR_END
-    PFOR(e, B.off[1], B.off[nn1 + 1]) {
-        int k = B.ent_k[e];
-        B.dphi[k - B.dlo] = (phi_get(B, k + 1, n3f) - phi_get(B, k, n3f)) * P.ssdzf;
-    }
-    PFOR_END
-    PFOR(e, B.off[nn1], B.off[nn1 + 1]) {
-        int k = B.ent_k[e];
-        const int ks[5] = {k - 3, k - 2, k - 1, k + 1, k + 2};
-        for (int t = 0; t < 5; t++) {
-            int kk = ks[t];
-            B.dphi[kk - B.dlo] = (phi_get(B, kk + 1, n3f) - phi_get(B, kk, n3f)) * P.ssdzf;
-        }
-    }
-    PFOR_END
-}
-
-DEV double weno_m(const BandSh &B, int k)
-{
-    double v1 = dphi_get(B, k - 3), v2 = dphi_get(B, k - 2), v3 = dphi_get(B, k - 1), v4 = dphi_get(B, k), v5 = dphi_get(B, k + 1);
-    double wz = d_hjweno(v2 - v1, v3 - v2, v4 - v3, v5 - v4);
-    return 1.0 / 12.0 * (-v2 + 7.0 * v3 + 7.0 * v4 - v5) - wz;
-}
-DEV double weno_p(const BandSh &B, int k)
-{
-    double v1 = dphi_get(B, k + 2), v2 = dphi_get(B, k + 1), v3 = dphi_get(B, k), v4 = dphi_get(B, k - 1), v5 = dphi_get(B, k - 2);
-    double wz = d_hjweno(v1 - v2, v2 - v3, v3 - v4, v4 - v5);
-    return 1.0 / 12.0 * (-v5 + 7.0 * v4 + 7.0 * v3 - v2) + wz;
-}
 DEV double weno_flat_m() { return 1.0 / 12.0 * (-0.0 + 7.0 * 0.0 + 7.0 * 0.0 - 0.0) - d_hjweno(0.0, 0.0, 0.0, 0.0); }
 DEV double weno_flat_p() { return 1.0 / 12.0 * (-0.0 + 7.0 * 0.0 + 7.0 * 0.0 - 0.0) + d_hjweno(0.0, 0.0, 0.0, 0.0); }
-
-// REINITIALIZATION_RK3 (two_phase.f:2174-2444)
-DEV void reinit_rk3(BandSh &B, const InstParam &P, double redtl, int nn1, int nn2)
-{
-    const int n3f = P.n3f;
-    const double mg = fmin(fmin(P.sdxf, P.sdyf), P.sdzf);
-    const double eps = mg * mg;
-    if (nn1 != 0) dphi_z(B, P, nn1);
-    const double wxm = weno_flat_m(), wxp = weno_flat_p();
-    PFOR(e, B.off[1], B.off[nn2 + 1]) {
-        int k = B.ent_k[e], nl = B.ent_n[e];
-        double ph = phi_get(B, k, n3f);
-        double dax = 0.5 * (0.0 - 0.0) * (1.0 / P.sdxf);
-        double daz = 0.5 * (phi_get(B, k + 1, n3f) - phi_get(B, k - 1, n3f)) * P.ssdzf;
-        double dalphi = dax * dax + dax * dax + daz * daz;
-        double po = B.phio[k - B.wlo];
-        double asign = po / sqrt(po * po + dalphi * eps);
-        double ax, ay, az;
-        if (nl <= nn1) {
-            double azm = weno_m(B, k), azp = weno_p(B, k);
-            ax = d_godunov(asign, wxp, wxm);
-            ay = d_godunov(asign, wxp, wxm);
-            az = d_godunov(asign, azp, azm);
-        } else {
-            double axm = 0.0, axp = 0.0;
-            double azm = (ph - phi_get(B, k - 1, n3f)) * P.ssdzf;
-            double azp = (phi_get(B, k + 1, n3f) - ph) * P.ssdzf;
-            ax = d_godunov(asign, axp, axm);
-            ay = d_godunov(asign, axp, axm);
-            az = d_godunov(asign, azp, azm);
-        }
-        double drhs = asign * (1.0 - sqrt(ax * ax + ay * ay + az * az));
-        B.newv[e] = ph + redtl * drhs;
-    }
-    PFOR_END
-    PFOR(e, B.off[1], B.off[nn2 + 1]) { phi_set(B, B.ent_k[e], n3f, B.newv[e]); }
-    PFOR_END
-}
-
-// REINITIALIZATION (two_phase.f:2033-2170)
-DEV void reinitialization(BandSh &B, const InstParam &P, InstState &S, int nn1, int nn2, int ilimit)
-{
-    const int n3f = P.n3f;
-    PFOR(e, B.off[1], B.off[nn2 + 1]) { int k = B.ent_k[e]; B.phio[k - B.wlo] = phi_get(B, k, n3f); }
-    PFOR_END
-    const double sm_grid = fmin(fmin(P.sdxf, P.sdyf), P.sdzf);
-    const double redtl = 0.5 * sm_grid;
-    for (int it = 1; it <= ilimit; it++) {
-        PFOR(e, B.off[1], B.off[nn2 + 1]) { int k = B.ent_k[e]; B.phrk[k - B.wlo] = phi_get(B, k, n3f); }
-        PFOR_END
-        reinit_rk3(B, P, redtl, nn1, nn2);
-        reinit_rk3(B, P, redtl, nn1, nn2);
-        PFOR(e, B.off[1], B.off[nn2 + 1]) {
-            int k = B.ent_k[e];
-            phi_set(B, k, n3f, 0.75 * B.phrk[k - B.wlo] + 0.25 * phi_get(B, k, n3f));
-        }
-        PFOR_END
-        reinit_rk3(B, P, redtl, nn1, nn2);
-        PFOR(e, B.off[1], B.off[nn2 + 1]) {
-            int k = B.ent_k[e];
-            phi_set(B, k, n3f, 1.0 / 3.0 * B.phrk[k - B.wlo] + 2.0 / 3.0 * phi_get(B, k, n3f));
-        }
-        PFOR_END
-    }
-    SERIAL { S.n_reinit += 1.0; }
-    SERIAL_END
-}
 
 // CAL_PSIF_MTP band formula (two_phase.f:2867-2936) for layers 1..nn1 into
 // out[entry], plus the window view pmw/tmk used by the implicit column.
@@ -325,37 +223,7 @@ DEV void vel_intp(BandSh &B, const CaseP &cp)
     PFOR_END
 }
 
-// TRANSPORT_RK3 (two_phase.f:1133-1495); ALPHI/MFLVS are the records held
-// at TRANSPORT entry, indexed by list slot (the reference keeps using the
-// entry-time order after a band rebuild)
-DEV void transport_rk3(BandSh &B, const InstParam &P, const InstState &S, double dtlvs)
-{
-    const int n3f = P.n3f;
-    dphi_z(B, P, LICA_NN1);
-    const double wxm = weno_flat_m();
-    PFOR(e, B.off[1], B.off[LICA_NN1 + 1]) {
-        int k = B.ent_k[e], nl = B.ent_n[e], q = B.ent_q[e];
-        double ax = 0.0 * wxm, ay = 0.0 * wxm, az;
-        double uu1 = wf_get(B, k), uu2 = wf_get(B, k + 1), uu3 = 0.5 * (uu1 + uu2);
-        if (uu1 >= 0.0 && uu2 >= 0.0 && uu3 >= 0.0) {
-            az = uu3 * weno_m(B, k);
-        } else if (uu1 <= 0.0 && uu2 <= 0.0 && uu3 <= 0.0) {
-            az = uu3 * weno_p(B, k);
-        } else {
-            double mm = weno_m(B, k), pp = weno_p(B, k);
-            double umax = fmax(fmax(fabs(uu1), fabs(uu2)), fabs(uu3));
-            az = 0.5 * (uu3 * (pp + mm) - umax * (pp - mm));
-        }
-        double v = phi_get(B, k, n3f) - dtlvs * (ax + ay + az);
-        double mf = (q < S.bn[nl]) ? S.mflvs[nl][q] : 0.0;
-        if (S.alphi[nl][q] > 0.0) v = v - dtlvs * mf / P.denr;
-        else v = v - dtlvs * mf;
-        B.newv[e] = v;
-    }
-    PFOR_END
-    PFOR(e, B.off[1], B.off[LICA_NN1 + 1]) { phi_set(B, B.ent_k[e], n3f, B.newv[e]); }
-    PFOR_END
-}
+#include "lica_kern_bandreg.h"
 
 // GLOBAL_MASS_CORRECTION (two_phase.f:6196-6318)
 DEV void global_mass_correction(BandSh &B, const CaseP &cp, InstState &S, int nn1)
@@ -476,21 +344,7 @@ DEV void body_transport(const BatchDev &bd, int inst, TransportSh &T)
         SERIAL_END
         const double dtlvs = B.v[0];
         time_tran = B.v[1];
-        PFOR(e, B.off[1], B.off[LICA_NN1 + 1]) { int k = B.ent_k[e]; B.phrk[k - B.wlo] = phi_get(B, k, n3f); }
-        PFOR_END
-        transport_rk3(B, P, S, dtlvs);
-        transport_rk3(B, P, S, dtlvs);
-        PFOR(e, B.off[1], B.off[LICA_NN1 + 1]) {
-            int k = B.ent_k[e];
-            phi_set(B, k, n3f, 0.75 * B.phrk[k - B.wlo] + 0.25 * phi_get(B, k, n3f));
-        }
-        PFOR_END
-        transport_rk3(B, P, S, dtlvs);
-        PFOR(e, B.off[1], B.off[LICA_NN1 + 1]) {
-            int k = B.ent_k[e];
-            phi_set(B, k, n3f, 1.0 / 3.0 * B.phrk[k - B.wlo] + 2.0 / 3.0 * phi_get(B, k, n3f));
-        }
-        PFOR_END
+        transport_stages(B, P, S, dtlvs);
         // REIN_TRIGGER (two_phase.f:646-697)
         auto dsum = [&](int e) {
             int k = B.ent_k[e];
@@ -503,7 +357,7 @@ DEV void body_transport(const BatchDev &bd, int inst, TransportSh &T)
         double dmax = sqrt(mx), dmin = sqrt(mn);
         int trig = (dmax > 2.0 || dmin < 1.0e-4) ? 1 : 0;
         int imass = trig;
-        reinitialization(B, P, S, LICA_NN1, LICA_NN2, LICA_NN2);
+        reinit_band(B, P, S, LICA_NN1, LICA_NN2, LICA_NN2);
         if (S.m % 50 == 0) imass = 1;
         if (trig || imass) {
             if (imass) global_mass_correction(B, cp, S, LICA_NN1);
@@ -512,7 +366,7 @@ DEV void body_transport(const BatchDev &bd, int inst, TransportSh &T)
             if (B.status) { SERIAL { S.status = B.status; } SERIAL_END return; }
             SERIAL { alphi_bc(B, P); }
             SERIAL_END
-            reinitialization(B, P, S, LICA_NN1, LICA_NN2, LICA_NN2);
+            reinit_band(B, P, S, LICA_NN1, LICA_NN2, LICA_NN2);
         }
         SERIAL { alphi_bc(B, P); }
         SERIAL_END

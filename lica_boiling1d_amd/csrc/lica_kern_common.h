@@ -872,9 +872,9 @@ struct BandSh {
     int wlo, whi, dlo;                  // window [wlo,whi] of z-planes, work base dlo=wlo-4
     int status;
     double phw[WMAX], gl[3], gu[4];
-    double phio[WMAX], phrk[WMAX], phs[WMAX];
-    double dphi[WMAX + 8], wf[WMAX + 8];
-    double newv[LICA_MAXENT], pm1[LICA_MAXENT], pm2[LICA_MAXENT], mfd[LICA_MAXENT];
+    double phs[WMAX];
+    double wf[WMAX + 8];
+    double pm1[LICA_MAXENT], pm2[LICA_MAXENT], mfd[LICA_MAXENT];
     double tf0[WMAX], tf1[WMAX];
     int mask_s[WMAX];
     double red[16];
@@ -886,6 +886,8 @@ struct BandSh {
     double dtn[NSWMAX];
     double psitmp[NSWMAX];
     int mcell[2 * LICA_MAXENT];
+    // per-plane tables of the window for the register-resident stages
+    int lay[WMAX + 8], slot[WMAX + 8], dmk[WMAX + 8];
 };
 
 struct Fill {           // CAL_PSIF_MTP column fill outside the band (two_phase.f:3011-3075)

@@ -466,7 +466,7 @@ DEV void body_temp_grad(const BatchDev &bd, int inst, TgradSh &TS)
         phi_set(B, k, n3f, v);
     }
     PFOR_END
-    reinitialization(B, P, S, LICA_NN1, LICA_NN2, LICA_NN2);
+    reinit_band(B, P, S, LICA_NN1, LICA_NN2, LICA_NN2);
     psif_mtp_band(B, P, LICA_NN1, B.pm2);
     psif_view(B, LICA_NN1, B.pm2, TS.pmw2, TS.tmk2);
     SERIAL {
