@@ -602,12 +602,30 @@ DEV int region_of(const double *pc, int n, int k)
     return (pc[k] == 1.0) ? 1 : 0;
 }
 
+// The same classes from jump flags jf[j] = (PSI_CN(j+1) != PSI_CN(j)),
+// j = 0..n, held in LDS (PSI_CN lies in [0,1], so a nonzero difference and
+// inequality coincide).
+#define JFMAX 8200
+DEV void jump_flags(unsigned char *jf, const double *pc, int n)
+{
+    PFOR(j, 0, n + 1) { jf[j] = (pc[j + 1] - pc[j] != 0.0) ? 1 : 0; }
+    PFOR_END
+}
+DEV int region_jf(const unsigned char *jf, const double *pc, int n, int k)
+{
+    int j0 = k - 2 < 1 ? 1 : k - 2, j1 = k + 2 > n ? n : k + 2;
+    for (int j = j0 - 1; j <= j1; j++)
+        if (jf[j]) return 2;
+    return (pc[k] == 1.0) ? 1 : 0;
+}
+
 struct BlkSh {
     double red[16];
     double v[16];
     int iv[16];
     TriSh tri;     // points at the kernel's dynamic LDS (tri_ws_rows(blockDim) x 4 doubles)
     RefWin win;    // refined window of the box averages
+    unsigned char jf[JFMAX];   // jump flags of PSI_CN (n <= JFMAX - 1)
 };
 
 // DETERMINE_DT (1d.f:683-817) at the start of a step, with the per-step
@@ -666,15 +684,15 @@ DEV void body_cont_energy(const BatchDev &bd, int inst, BlkSh &sh)
     CaseP cp = case_ptrs(bd, inst);
     const InstParam &P = *cp.P;
     const int n = P.n, n3 = P.n3, msub = bd.msub;
-    double *W = cp.A(NS_W), *T = cp.A(NS_T), *PZN = cp.A(NS_PSI_ZN), *PCN = cp.A(NS_PSI_CN);
-    double *DFZ = cp.A(NS_DF_Z), *VFZ = cp.A(NS_VF_Z), *DFC = cp.A(NS_DF_C);
-    double *PZ = cp.A(NS_PSI_Z), *PC = cp.A(NS_PSI_C), *RHSPS = cp.A(NS_RHSPS);
-    double *TA = cp.A(NS_TA), *TB = cp.A(NS_TB), *TC = cp.A(NS_TC), *TG = cp.A(NS_TG);
-    const double *ZP = cp.A(NS_ZP), *Z = cp.A(NS_Z), *SSDZ = cp.A(NS_SSDZ), *VVDZ = cp.A(NS_VVDZ);
-    const double *AKUV = cp.A(NS_AKUV), *CKUV = cp.A(NS_CKUV);
-    const double *ZF = cp.R(RF_ZF), *PSIF = cp.R(RF_PSIF);
-    const int *KCOU1 = cp.AI(NSI_KCOU1), *KCOU2 = cp.AI(NSI_KCOU2);
-    const int *KCOUMP1 = cp.AI(NSI_KCOUMP1), *KCOUMP2 = cp.AI(NSI_KCOUMP2);
+    double *__restrict__ W = cp.A(NS_W), *__restrict__ T = cp.A(NS_T), *__restrict__ PZN = cp.A(NS_PSI_ZN), *__restrict__ PCN = cp.A(NS_PSI_CN);
+    double *__restrict__ DFZ = cp.A(NS_DF_Z), *__restrict__ VFZ = cp.A(NS_VF_Z), *__restrict__ DFC = cp.A(NS_DF_C);
+    double *__restrict__ PZ = cp.A(NS_PSI_Z), *__restrict__ PC = cp.A(NS_PSI_C), *__restrict__ RHSPS = cp.A(NS_RHSPS);
+    double *__restrict__ TA = cp.A(NS_TA), *__restrict__ TB = cp.A(NS_TB), *__restrict__ TC = cp.A(NS_TC), *__restrict__ TG = cp.A(NS_TG);
+    const double *__restrict__ ZP = cp.A(NS_ZP), *__restrict__ Z = cp.A(NS_Z), *__restrict__ SSDZ = cp.A(NS_SSDZ), *__restrict__ VVDZ = cp.A(NS_VVDZ);
+    const double *__restrict__ AKUV = cp.A(NS_AKUV), *__restrict__ CKUV = cp.A(NS_CKUV);
+    const double *__restrict__ ZF = cp.R(RF_ZF), *__restrict__ PSIF = cp.R(RF_PSIF);
+    const int *__restrict__ KCOU1 = cp.AI(NSI_KCOU1), *__restrict__ KCOU2 = cp.AI(NSI_KCOU2);
+    const int *__restrict__ KCOUMP1 = cp.AI(NSI_KCOUMP1), *__restrict__ KCOUMP2 = cp.AI(NSI_KCOUMP2);
 
     // substep constants (1d.f:221-225), evaluated by every thread
     const double dt = S.dt;
@@ -746,12 +764,15 @@ DEV void body_cont_energy(const BatchDev &bd, int inst, BlkSh &sh)
         return cface2(f);
     };
 
+    const bool use_jf = n + 1 <= JFMAX;
+    if (use_jf) jump_flags(sh.jf, PCN, n);
+    auto region = [&](int k) { return use_jf ? region_jf(sh.jf, PCN, n, k) : region_of(PCN, n, k); };
     // CONTI_Z / CONTI_C in closed form: the serial loops leave on each face
     // the value written last, i.e. the next cell's constant when that cell
     // is region 0/1 and otherwise the value the face's own cell computed.
     PFOR(k, 1, n + 1) {
-        const int rk = region_of(PCN, n, k);
-        const int rn = (k < n) ? region_of(PCN, n, k + 1) : -1;
+        const int rk = region(k);
+        const int rn = (k < n) ? region(k + 1) : -1;
         double vf, df;
         if (k == n) df = zown(k, rk, &vf);
         else if (rn == 0) { df = P.denm; vf = P.vism; }
@@ -765,7 +786,7 @@ DEV void body_cont_energy(const BatchDev &bd, int inst, BlkSh &sh)
             if (rk != 2) pz = PZN[k];
             else {
                 double vv, own = zface2(k, &vv);
-                double prev = (k - 1 >= 2) ? zown(k - 1, region_of(PCN, n, k - 1), &vv) : zface2(1, &vv);
+                double prev = (k - 1 >= 2) ? zown(k - 1, region(k - 1), &vv) : zface2(1, &vv);
                 double wt = 0.5 * (W[k + 1] + W[k]);
                 double wb = 0.5 * (W[k] + W[k - 1]);
                 double flux = (own * wt - prev * wb) * VVDZ[k];
@@ -795,7 +816,7 @@ DEV void body_cont_energy(const BatchDev &bd, int inst, BlkSh &sh)
         if (rk != 2) pc = PCN[k];
         else {
             double own = cface2(k);
-            double prev = (k >= 2) ? cown(k - 1, region_of(PCN, n, k - 1)) : cface2(0);
+            double prev = (k >= 2) ? cown(k - 1, region(k - 1)) : cface2(0);
             double flux = (own * W[k + 1] - prev * W[k]) * SSDZ[k];
             double den_cn = P.denm + P.den_diff * PCN[k];
             double den_c = den_cn - dtconst * (flux);

@@ -19,15 +19,15 @@ DEV void body_momentum(const BatchDev &bd, int inst, BlkSh &sh)
     CaseP cp = case_ptrs(bd, inst);
     const InstParam &P = *cp.P;
     const int n = P.n, n3 = P.n3, msub = bd.msub;
-    double *W = cp.A(NS_W), *PP = cp.A(NS_P), *PZN = cp.A(NS_PSI_ZN), *PCN = cp.A(NS_PSI_CN);
-    double *DFZ = cp.A(NS_DF_Z), *VFZ = cp.A(NS_VF_Z);
-    double *PZ = cp.A(NS_PSI_Z), *PC = cp.A(NS_PSI_C), *RK3ZO = cp.A(NS_RK3ZO);
-    double *TA = cp.A(NS_TA), *TB = cp.A(NS_TB), *TC = cp.A(NS_TC), *TG = cp.A(NS_TG);
-    double *PA = cp.A(NS_PA), *PB = cp.A(NS_PB), *PCc = cp.A(NS_PC), *PG = cp.A(NS_PG);
-    const double *ZP = cp.A(NS_ZP), *Z = cp.A(NS_Z), *SSDZ = cp.A(NS_SSDZ), *VVDZ = cp.A(NS_VVDZ);
-    const double *AKW = cp.A(NS_AKW), *CKW = cp.A(NS_CKW), *ZF = cp.R(RF_ZF);
-    const double *SFGAS = cp.R(RF_SFGAS), *SFGAS2 = cp.R(RF_SFGAS2);
-    const int *KCOU1 = cp.AI(NSI_KCOU1), *KCOU2 = cp.AI(NSI_KCOU2), *IMASK = cp.AI(NSI_IMASK);
+    double *__restrict__ W = cp.A(NS_W), *__restrict__ PP = cp.A(NS_P), *__restrict__ PZN = cp.A(NS_PSI_ZN), *__restrict__ PCN = cp.A(NS_PSI_CN);
+    double *__restrict__ DFZ = cp.A(NS_DF_Z), *__restrict__ VFZ = cp.A(NS_VF_Z);
+    double *__restrict__ PZ = cp.A(NS_PSI_Z), *__restrict__ PC = cp.A(NS_PSI_C), *__restrict__ RK3ZO = cp.A(NS_RK3ZO);
+    double *__restrict__ TA = cp.A(NS_TA), *__restrict__ TB = cp.A(NS_TB), *__restrict__ TC = cp.A(NS_TC), *__restrict__ TG = cp.A(NS_TG);
+    double *__restrict__ PA = cp.A(NS_PA), *__restrict__ PB = cp.A(NS_PB), *__restrict__ PCc = cp.A(NS_PC), *__restrict__ PG = cp.A(NS_PG);
+    const double *__restrict__ ZP = cp.A(NS_ZP), *__restrict__ Z = cp.A(NS_Z), *__restrict__ SSDZ = cp.A(NS_SSDZ), *__restrict__ VVDZ = cp.A(NS_VVDZ);
+    const double *__restrict__ AKW = cp.A(NS_AKW), *__restrict__ CKW = cp.A(NS_CKW), *__restrict__ ZF = cp.R(RF_ZF);
+    const double *__restrict__ SFGAS = cp.R(RF_SFGAS), *__restrict__ SFGAS2 = cp.R(RF_SFGAS2);
+    const int *__restrict__ KCOU1 = cp.AI(NSI_KCOU1), *__restrict__ KCOU2 = cp.AI(NSI_KCOU2), *__restrict__ IMASK = cp.AI(NSI_IMASK);
     const long long m = S.m;
     const double dt = S.dt, alpha = S.alpha, acoef = S.acoef, dtconst = S.dtconst, dtconsti = S.dtconsti;
     const int sfl = S.sf_lo, sfh = S.sf_hi;
@@ -42,8 +42,10 @@ DEV void body_momentum(const BatchDev &bd, int inst, BlkSh &sh)
 
     if (m != 1) {
         // CAL_CONTINUITY2 (1d.f:3006-3169)
+        const bool use_jf = n + 1 <= JFMAX;
+        if (use_jf) jump_flags(sh.jf, PCN, n);
         PFOR(k, 1, n + 1) {
-            int rg = region_of(PCN, n, k);
+            int rg = use_jf ? region_jf(sh.jf, PCN, n, k) : region_of(PCN, n, k);
             if (k >= 2) {
                 double pz = PZ[k];
                 if (rg == 2 && touches(KCOU2[k], KCOU1[k + 1]))
@@ -158,11 +160,11 @@ DEV void body_ucalc(const BatchDev &bd, int inst, BlkSh &sh)
     CaseP cp = case_ptrs(bd, inst);
     const InstParam &P = *cp.P;
     const int n = P.n, n3 = P.n3, msub = bd.msub;
-    double *W = cp.A(NS_W), *PP = cp.A(NS_P), *PZ = cp.A(NS_PSI_Z), *WO = cp.A(NS_WO), *WT = cp.A(NS_WT);
-    const double *T = cp.A(NS_T);
-    const double *Z = cp.A(NS_Z), *SSDZ = cp.A(NS_SSDZ), *VVDZ = cp.A(NS_VVDZ), *ZF = cp.R(RF_ZF);
-    const double *SFGAS = cp.R(RF_SFGAS);
-    const int *KCOU1 = cp.AI(NSI_KCOU1), *KCOU2 = cp.AI(NSI_KCOU2);
+    double *__restrict__ W = cp.A(NS_W), *__restrict__ PP = cp.A(NS_P), *__restrict__ PZ = cp.A(NS_PSI_Z), *__restrict__ WO = cp.A(NS_WO), *__restrict__ WT = cp.A(NS_WT);
+    const double *__restrict__ T = cp.A(NS_T);
+    const double *__restrict__ Z = cp.A(NS_Z), *__restrict__ SSDZ = cp.A(NS_SSDZ), *__restrict__ VVDZ = cp.A(NS_VVDZ), *__restrict__ ZF = cp.R(RF_ZF);
+    const double *__restrict__ SFGAS = cp.R(RF_SFGAS);
+    const int *__restrict__ KCOU1 = cp.AI(NSI_KCOU1), *__restrict__ KCOU2 = cp.AI(NSI_KCOU2);
     const double dtconst = S.dtconst;
     const int sfl = S.sf_lo, sfh = S.sf_hi;
     const long long m = S.m;
