@@ -29,7 +29,7 @@ HBM_PEAK_GBS = 8000.0
 KERNELS = ["init", "dt", "cont_energy", "temp_grad", "momentum", "poisson", "ucalc", "transport"]
 
 
-def kernel_bytes(case, kernel, mode):
+def kernel_bytes(case, kernel, mode, vcycles=0.0):
     """Algorithmic HBM bytes of one launch for one case: every distinct
     global array the kernel touches, elements touched x 8 B (4 B for int
     arrays), twice when read and written.  Band-sized work (a few dozen
@@ -44,8 +44,14 @@ def kernel_bytes(case, kernel, mode):
         # W,PSI_Z,PSI_C,RK3ZO r/w; PSI_CN,DF_Z,VF_Z,SSDZ,VVDZ,P,PSI_ZN,AKW,CKW read;
         # IMASK (int); KCOU1/KCOU2 (int); tridiagonal work r/w; Poisson matrix written
         "momentum": 4 * 16 + 9 * 8 + 4 + 8 + 4 * 16 + 4 * 8,
-        # Thomas: a r/w (pivots), b,c read, g r/w, P written; partitioned: a,b,c,g r/w
-        "poisson": (16 + 8 + 8 + 16 + 8) if mode == 0 else (4 * 16 + 8),
+        # Thomas: a r/w (pivots), b,c read, g r/w, P written; partitioned: a,b,c,g r/w;
+        # multigrid: read a,b,c,g and write P, build the level hierarchy (about
+        # 2n rows of a,b,c,x,f,r), then per V-cycle 384 B per fine cell in FP64
+        # (2+2 smoothing sweeps, residual, transfers over all levels), half of
+        # that for the FP32 cycles plus the FP64 defect correction (32 B)
+        "poisson": ((16 + 8 + 8 + 16 + 8) if mode == 0 else
+                    (4 * 16 + 8) if mode == 1 else
+                    int(40 + 2 * 48 + vcycles * (384 if mode == 2 else 192 + 32))),
         # W,P,WO r/w; PSI_Z,VVDZ,SSDZ read; KCOU1/KCOU2; WT written
         "ucalc": 3 * 16 + 3 * 8 + 8 + 8,
     }
@@ -89,7 +95,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", choices=["large", "batch", "deck"], default="large")
     ap.add_argument("--batch", type=int, default=4096)
-    ap.add_argument("--mode", choices=["direct", "thomas"], default="direct")
+    ap.add_argument("--mode", choices=["direct", "thomas", "mg64", "mixed"], default="direct")
+    ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--threads", type=int, default=256)
     ap.add_argument("--graph", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -112,12 +119,13 @@ def main():
         cases = L.batched_cases(args.batch)
         wl = {"workload": "batched cases", "instances": args.batch, "n": "U{128..4096}", "r": 2,
               "superheat_K": "U(2,20)", "film_mm": "U(0.05,0.5)", "seed": 20261202}
-    mode = L.POISSON_DIRECT if args.mode == "direct" else L.POISSON_THOMAS
+    mode = {"direct": L.POISSON_DIRECT, "thomas": L.POISSON_THOMAS, "mg64": L.POISSON_MG64,
+            "mixed": L.POISSON_MIXED}[args.mode]
     wl["poisson"] = args.mode
     print(f"bench: {wl['workload']} x{len(cases)}, mode {args.mode}, warmup {args.warmup}, steps {args.steps}",
           file=sys.stderr, flush=True)
     b = L.Batch(cases)
-    b.set_poisson(mode)
+    b.set_poisson(mode, args.rtol, 200)
     b.set_block(args.threads)
     if args.graph:
         b.use_graph(True)
@@ -149,17 +157,25 @@ def main():
            "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f64", "data": "synthetic", "config": wl}
     out["cell_updates_per_s"] = float(sum(int(c.n) * int(d) for c, d in zip(cases, steps1 - steps0))) / el
+    if mode >= L.POISSON_MG64:
+        cyc = sum(b.scalars(i)["POISSON_CYCLES"] for i in range(min(len(cases), 64)))
+        sol = sum(b.scalars(i)["POISSON_SOLVES"] for i in range(min(len(cases), 64)))
+        out["vcycles_per_solve"] = cyc / max(sol, 1.0)
+        out["poisson_backward_error_max"] = max(b.scalars(i)["POISSON_RESID"] for i in range(min(len(cases), 64)))
     if nbad:
         import numpy as np
         codes, counts = np.unique(st[st != 0], return_counts=True)
         out["stopped_cases"] = {L.status_string(int(k)): int(v) for k, v in zip(codes, counts)}
     if tim is not None:
         launches = {"dt": 1, "cont_energy": 3, "temp_grad": 3, "momentum": 3, "poisson": 3, "ucalc": 3, "transport": 3}
+        if mode >= L.POISSON_MG64:
+            tim["poisson"] = tim.get("poisson_mg", 0.0)
         per = {}
         for k, nl in launches.items():
             per[k] = tim.get(k, 0.0) / (args.steps * nl)   # ms per launch
         dom = max(per, key=lambda k: per[k])
-        bytes_launch = sum(kernel_bytes(c, dom, 0 if mode == L.POISSON_THOMAS else 1) for c in cases)
+        vc = out.get("vcycles_per_solve", 0.0)
+        bytes_launch = sum(kernel_bytes(c, dom, mode, vc) for c in cases)
         ach = bytes_launch / (per[dom] * 1e-3) / 1e9
         out["roofline"] = {"bound": "hbm", "kernel": "k_" + dom, "achieved": ach, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,

@@ -316,6 +316,7 @@ struct lica_batch {
     InstState *d_st = nullptr;
     double *d_ns = nullptr, *d_rf = nullptr;
     int *d_nsi = nullptr, *d_rfi = nullptr, *d_order = nullptr;
+    double *d_mg = nullptr;      // multigrid scratch (allocated on first use of a multigrid mode)
     std::vector<std::vector<double>> img_ns, img_rf;     // initial images (geometry)
     std::vector<std::vector<int>> img_nsi, img_rfi;
     int threads = 256;
@@ -347,6 +348,11 @@ BatchDev make_bd(const lica_batch *b, int msub)
     bd.ninst = (int)b->ninst;
     bd.tri_mode = (b->poisson_mode == LICA_POISSON_THOMAS) ? 0 : 1;
     bd.msub = msub;
+    bd.pmode = b->poisson_mode;
+    bd.mg = b->d_mg;
+    bd.mg_rtol = b->rtol;
+    bd.mg_maxc = b->max_cycles;
+    bd.guess = nullptr;
     return bd;
 }
 
@@ -375,11 +381,12 @@ int launch(lica_batch *b, int which, int msub)
 int enqueue_step(lica_batch *b)
 {
     int rc = launch(b, KL_DT, 1);
+    const int kp = (b->poisson_mode >= LICA_POISSON_MG64) ? KL_POISSON_MG : KL_POISSON_DIRECT;
     for (int msub = 1; msub <= 3 && !rc; msub++) {
         if (!rc) rc = launch(b, KL_CONT_ENERGY, msub);
         if (!rc) rc = launch(b, KL_TEMP_GRAD, msub);
         if (!rc) rc = launch(b, KL_MOMENTUM, msub);
-        if (!rc) rc = launch(b, KL_POISSON_DIRECT, msub);
+        if (!rc) rc = launch(b, kp, msub);
         if (!rc) rc = launch(b, KL_UCALC, msub);
         if (!rc) rc = launch(b, KL_TRANSPORT, msub);
     }
@@ -403,7 +410,8 @@ int enqueue_step_timed(lica_batch *b, std::vector<int> &kinds)
     int rc = launch(b, KL_DT, 1);
     if (!rc) rc = rec(KL_DT);
     for (int msub = 1; msub <= 3 && !rc; msub++) {
-        const int seq[6] = {KL_CONT_ENERGY, KL_TEMP_GRAD, KL_MOMENTUM, KL_POISSON_DIRECT, KL_UCALC, KL_TRANSPORT};
+        const int kp = (b->poisson_mode >= LICA_POISSON_MG64) ? KL_POISSON_MG : KL_POISSON_DIRECT;
+        const int seq[6] = {KL_CONT_ENERGY, KL_TEMP_GRAD, KL_MOMENTUM, kp, KL_UCALC, KL_TRANSPORT};
         for (int j = 0; j < 6 && !rc; j++) {
             rc = launch(b, seq[j], msub);
             if (!rc) rc = rec(seq[j]);
@@ -628,7 +636,6 @@ int lica_batch_run(lica_batch *b, int64_t nsteps)
     int rc = check_batch(b);
     if (rc) return rc;
     if (!b->initialized || nsteps < 0) return LICA_BAD_ARG;
-    if (b->poisson_mode != LICA_POISSON_THOMAS && b->poisson_mode != LICA_POISSON_DIRECT) return LICA_UNSUPPORTED_DECK;
     for (int i = 0; i < KL_COUNT; i++) b->timings[i] = 0.0;
     auto t0 = std::chrono::steady_clock::now();
 #ifdef LICA_EMUL
@@ -672,7 +679,14 @@ int lica_batch_set_poisson(lica_batch *b, int64_t mode, double rtol, int64_t max
     int rc = check_batch(b);
     if (rc) return rc;
     if (mode < LICA_POISSON_THOMAS || mode > LICA_POISSON_LEARNED || !(rtol > 0.0) || max_cycles < 1) return LICA_BAD_ARG;
-    if (mode != LICA_POISSON_THOMAS && mode != LICA_POISSON_DIRECT) return LICA_UNSUPPORTED_DECK;
+    if (mode == LICA_POISSON_LEARNED) return LICA_UNSUPPORTED_DECK;
+    if (mode >= LICA_POISSON_MG64 && !b->d_mg) {
+        long long tot = 0;
+        for (InstParam &p : b->par) { p.mg_off = tot; tot += lica_mg_scratch_doubles(p.n); }
+        rc = dmalloc((void **)&b->d_mg, sizeof(double) * (size_t)tot);
+        rc = rc ? rc : h2d(b->d_par, b->par.data(), sizeof(InstParam) * b->par.size(), b->stream);
+        if (rc) return rc;
+    }
     b->poisson_mode = (int)mode;
     b->rtol = rtol;
     b->max_cycles = (int)max_cycles;
@@ -841,7 +855,7 @@ void lica_batch_destroy(lica_batch *b)
     for (hipEvent_t e : b->ev) (void)hipEventDestroy(e);
 #endif
     dfree(b->d_par); dfree(b->d_st); dfree(b->d_ns); dfree(b->d_rf);
-    dfree(b->d_nsi); dfree(b->d_rfi); dfree(b->d_order);
+    dfree(b->d_nsi); dfree(b->d_rfi); dfree(b->d_order); dfree(b->d_mg);
 #ifndef LICA_EMUL
     if (b->stream) (void)hipStreamDestroy(b->stream);
 #endif
@@ -890,7 +904,8 @@ int lica_rhsnlhs(lica_batch *ctx, int64_t msub, double *w, double *p, double *ps
     if (rc) return rc;
     if (msub < 1 || msub > 3) return LICA_BAD_ARG;
     ctx->last_msub = (int)msub;
-    const int seq[5] = {KL_CONT_ENERGY, KL_TEMP_GRAD, KL_MOMENTUM, KL_POISSON_DIRECT, KL_UCALC};
+    const int kp = (ctx->poisson_mode >= LICA_POISSON_MG64) ? KL_POISSON_MG : KL_POISSON_DIRECT;
+    const int seq[5] = {KL_CONT_ENERGY, KL_TEMP_GRAD, KL_MOMENTUM, kp, KL_UCALC};
     for (int j = 0; j < 5 && !rc; j++) rc = launch(ctx, seq[j], (int)msub);
     rc = rc ? rc : dsync(ctx->stream);
     rc = rc ? rc : copy_ns_out(ctx, NS_W, w);
@@ -949,8 +964,8 @@ extern "C" int lica_poisson(int64_t nsys, int64_t n, const double *a, const doub
                             int64_t *cycles_out)
 {
     if (nsys <= 0 || n < 1 || !a || !b || !c || !g || !p) return LICA_BAD_ARG;
-    if (mode != LICA_POISSON_THOMAS && mode != LICA_POISSON_DIRECT) return LICA_UNSUPPORTED_DECK;
-    (void)rtol; (void)max_cycles;
+    if (mode < LICA_POISSON_THOMAS || mode > LICA_POISSON_MIXED) return LICA_UNSUPPORTED_DECK;
+    if (mode >= LICA_POISSON_MG64 && (!(rtol > 0.0) || max_cycles < 1)) return LICA_BAD_ARG;
     if (!have_device()) return LICA_NO_DEVICE;
     hipStream_t st = nullptr;
 #ifndef LICA_EMUL
@@ -966,15 +981,37 @@ extern "C" int lica_poisson(int64_t nsys, int64_t n, const double *a, const doub
     rc = rc ? rc : h2d(db, b, bytes, st);
     rc = rc ? rc : h2d(dc, c, bytes, st);
     rc = rc ? rc : h2d(dg, g, bytes, st);
-    if (!rc && lica_launch_tri((int)nsys, (int)n, da, db, dc, dg, mode == LICA_POISSON_THOMAS ? 0 : 1, 256, (void *)st))
-        rc = LICA_HIP_ERROR;
-    rc = rc ? rc : dsync(st);
-    rc = rc ? rc : d2h(p, dg, bytes, st);
-    dfree(da); dfree(db); dfree(dc); dfree(dg);
+    double *dp = nullptr, *dmg = nullptr, *dres = nullptr;
+    int *dcyc = nullptr;
+    int64_t maxcyc = 0;
+    if (mode >= LICA_POISSON_MG64) {
+        const size_t sc = (size_t)lica_mg_scratch_doubles((int)n) * (size_t)nsys;
+        rc = rc ? rc : dmalloc((void **)&dp, bytes);
+        rc = rc ? rc : dmalloc((void **)&dmg, sizeof(double) * sc);
+        rc = rc ? rc : dmalloc((void **)&dres, sizeof(double) * (size_t)nsys);
+        rc = rc ? rc : dmalloc((void **)&dcyc, sizeof(int) * (size_t)nsys);
+        if (!rc && lica_launch_mg((int)nsys, (int)n, da, db, dc, dg, dp, dmg, (int)mode, rtol, (int)max_cycles, dcyc,
+                                  dres, 256, (void *)st))
+            rc = LICA_HIP_ERROR;
+        rc = rc ? rc : dsync(st);
+        rc = rc ? rc : d2h(p, dp, bytes, st);
+        std::vector<int> cyc((size_t)nsys);
+        rc = rc ? rc : d2h(cyc.data(), dcyc, sizeof(int) * cyc.size(), st);
+        for (int cv : cyc) {
+            if (cv < 0) { maxcyc = max_cycles; if (!rc) rc = LICA_NOT_CONVERGED; }
+            else maxcyc = std::max<int64_t>(maxcyc, cv);
+        }
+    } else {
+        if (!rc && lica_launch_tri((int)nsys, (int)n, da, db, dc, dg, mode == LICA_POISSON_THOMAS ? 0 : 1, 256, (void *)st))
+            rc = LICA_HIP_ERROR;
+        rc = rc ? rc : dsync(st);
+        rc = rc ? rc : d2h(p, dg, bytes, st);
+    }
+    dfree(da); dfree(db); dfree(dc); dfree(dg); dfree(dp); dfree(dmg); dfree(dres); dfree(dcyc);
 #ifndef LICA_EMUL
     (void)hipStreamDestroy(st);
 #endif
-    if (cycles_out) *cycles_out = 0;
+    if (cycles_out) *cycles_out = maxcyc;
     return rc;
 }
 

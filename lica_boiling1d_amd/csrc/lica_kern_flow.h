@@ -3,6 +3,7 @@
 // (MAKEFLD 1d.f:821-946, LVSINIT two_phase.f:116-458, GRID_COUPLING(1)).
 #pragma once
 #include "lica_kern_tgrad.h"
+#include "lica_kern_mg.h"
 
 // 1/DENF_Z from PSI_Z (CAL_CONT_BC 1d.f:2986-2994 incl. the end copies)
 DEV double denf_zi(const InstParam &P, const double *PZ, int k)
@@ -147,6 +148,31 @@ DEV void body_poisson_direct(const BatchDev &bd, int inst, BlkSh &sh)
         PP[0] = PP[1];
         PP[n3] = PP[n];
         S.poisson_solves += 1.0;
+    }
+    SERIAL_END
+}
+
+// POISSON by multigrid (the same system, solved to max|g-Ap| <= rtol max|g|)
+DEV void body_poisson_mg(const BatchDev &bd, int inst, BlkSh &sh)
+{
+    InstState &S = bd.st[inst];
+    if (S.status) return;
+    CaseP cp = case_ptrs(bd, inst);
+    const InstParam &P = *cp.P;
+    const int n = P.n, n3 = P.n3;
+    double *PP = cp.A(NS_P);
+    const double *PA = cp.A(NS_PA), *PB = cp.A(NS_PB), *PCc = cp.A(NS_PC), *PG = cp.A(NS_PG);
+    const double *x0 = (bd.pmode == LICA_MG_LEARNED_MODE && bd.guess) ? bd.guess + P.ns_off + (size_t)NS_P * P.ln + 2 : nullptr;
+    double res = 0.0;
+    int cyc = mg_solve(PA, PB, PCc, PG, PP, n, bd.mg + P.mg_off, bd.pmode, bd.mg_rtol, bd.mg_maxc, x0, sh.mgws,
+                       sh.red, &res);
+    SERIAL {
+        PP[0] = PP[1];
+        PP[n3] = PP[n];
+        S.poisson_solves += 1.0;
+        S.poisson_cycles += (cyc < 0) ? (double)bd.mg_maxc : (double)cyc;
+        S.poisson_resid = fmax(S.poisson_resid, res);
+        if (cyc < 0) S.status = ST_NOT_CONVERGED;
     }
     SERIAL_END
 }

@@ -2,6 +2,8 @@
 #include "lica_kern_flow.h"
 #include "lica_launch.h"
 
+long long lica_mg_scratch_doubles(int n) { return mg_scratch_doubles(n); }
+
 #ifndef LICA_EMUL
 // LDS workspace of the partitioned tridiagonal solves, sized by the launch
 extern __shared__ double lica_tri_lds[];
@@ -45,6 +47,36 @@ __global__ void __launch_bounds__(1024) k_poisson_direct(BatchDev bd)
     tri_attach(sh);
     body_poisson_direct(bd, bd.order[blockIdx.x], sh);
 }
+__global__ void __launch_bounds__(1024) k_poisson_mg(BatchDev bd)
+{
+    __shared__ BlkSh sh;
+    body_poisson_mg(bd, bd.order[blockIdx.x], sh);
+}
+
+// nsys independent POISSON systems by multigrid (scratch: mg_scratch_doubles(n) each)
+__global__ void __launch_bounds__(1024) k_mg_systems(const double *a, const double *b, const double *c, const double *g,
+                                                     double *p, int n, double *scratch, int mode, double rtol,
+                                                     int maxc, int *cycles, double *resid)
+{
+    __shared__ BlkSh sh;
+    const size_t off = (size_t)blockIdx.x * (size_t)n - 1;
+    double res = 0.0;
+    int cyc = mg_solve(a + off, b + off, c + off, g + off, p + off, n,
+                       scratch + (size_t)blockIdx.x * (size_t)mg_scratch_doubles(n), mode, rtol, maxc, nullptr,
+                       sh.mgws, sh.red, &res);
+    if (threadIdx.x == 0) { cycles[blockIdx.x] = cyc; resid[blockIdx.x] = res; }
+}
+
+int lica_launch_mg(int nsys, int n, const double *a, const double *b, const double *c, const double *g, double *p,
+                   double *scratch, int mode, double rtol, int maxc, int *cycles, double *resid, int threads,
+                   void *stream)
+{
+    if (nsys <= 0) return 0;
+    hipLaunchKernelGGL(k_mg_systems, dim3(nsys), dim3(threads), 0, (hipStream_t)stream, a, b, c, g, p, n, scratch,
+                       mode, rtol, maxc, cycles, resid);
+    return (int)hipGetLastError();
+}
+
 __global__ void __launch_bounds__(1024) k_ucalc(BatchDev bd)
 {
     __shared__ BlkSh sh;
@@ -110,6 +142,7 @@ int lica_launch(int which, const BatchDev &bd, int threads, void *stream)
     case KL_POISSON_DIRECT: LAUNCH_TRI(k_poisson_direct, threads); break;
     case KL_UCALC: LAUNCH(k_ucalc, threads); break;
     case KL_TRANSPORT: LAUNCH(k_transport, 64); break;
+    case KL_POISSON_MG: LAUNCH(k_poisson_mg, threads); break;
     default: return -1;
     }
     return (int)hipGetLastError();
@@ -135,6 +168,27 @@ int lica_launch_tri(int nsys, int n, double *a, double *b, double *c, double *g,
     for (int s = 0; s < nsys; s++) {
         const size_t off = (size_t)s * (size_t)n - 1;
         tri_solve(a + off, b + off, c + off, g + off, 1, n, mode, sh->tri);
+    }
+    lica_emul_nthreads = saved;
+    return 0;
+}
+
+int lica_launch_mg(int nsys, int n, const double *a, const double *b, const double *c, const double *g, double *p,
+                   double *scratch, int mode, double rtol, int maxc, int *cycles, double *resid, int threads,
+                   void *stream)
+{
+    (void)stream;
+    static BlkSh *sh = nullptr;
+    if (!sh) sh = new BlkSh();
+    int saved = lica_emul_nthreads;
+    lica_emul_nthreads = threads;
+    for (int s = 0; s < nsys; s++) {
+        const size_t off = (size_t)s * (size_t)n - 1;
+        double res = 0.0;
+        cycles[s] = mg_solve(a + off, b + off, c + off, g + off, p + off, n,
+                             scratch + (size_t)s * (size_t)mg_scratch_doubles(n), mode, rtol, maxc, nullptr,
+                             sh->mgws, sh->red, &res);
+        resid[s] = res;
     }
     lica_emul_nthreads = saved;
     return 0;
@@ -171,6 +225,7 @@ int lica_launch(int which, const BatchDev &bd, int threads, void *stream)
     case KL_POISSON_DIRECT: emul_run<BlkSh>(bd, threads, body_poisson_direct); break;
     case KL_UCALC: emul_run<BlkSh>(bd, threads, body_ucalc); break;
     case KL_TRANSPORT: emul_run<TransportSh>(bd, 64, body_transport); break;
+    case KL_POISSON_MG: emul_run<BlkSh>(bd, threads, body_poisson_mg); break;
     default: return -1;
     }
     return 0;

@@ -36,6 +36,7 @@ struct InstParam {
     int ln, lf, lni, lfi;
     int mf_band, idtopt;
     long long ns_off, rf_off, nsi_off, rfi_off;
+    long long mg_off;            // multigrid scratch (doubles), multigrid modes only
     // properties (INPUT_SIMULATION 1d.f:434-480)
     double re, prm, denr, ja, tsat, cflmax, dt0, z0, tslope;
     double rei, denm, denp, den_diff, vism, visp, denscm, densc_diff;
@@ -73,7 +74,16 @@ struct BatchDev {
     int ninst;
     int tri_mode;        // 0: serial Thomas, 1: partitioned
     int msub;            // RK3 substep 1..3
+    int pmode;           // Poisson mode (0/1 direct, 2 MG64, 3 mixed, 4 learned)
+    int mg_maxc;         // V-cycle cap per solve
+    double mg_rtol;      // relative residual target
+    double *mg;          // multigrid scratch of the batch
+    const double *guess; // initial guesses of the learned mode (NS layout, field P)
 };
+
+#define LICA_MG_FP64_MODE 2
+#define LICA_MG_MIXED_MODE 3
+#define LICA_MG_LEARNED_MODE 4
 
 // status codes shared with include/lica_mi355x.h
 #define ST_OK 0

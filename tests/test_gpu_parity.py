@@ -192,3 +192,44 @@ def test_checkpoint_restart_bit_exact(tmp_path):
         for f in ("W", "P", "T", "PSIF", "PHI", "ALPHI", "MFLVS"):
             assert np.array_equal(a.field(f, i), b.field(f, i)), (i, f)
         assert np.array_equal(a.scalar_array(i), b.scalar_array(i))
+
+
+@pytest.mark.parametrize("mode", [L.POISSON_MG64, L.POISSON_MIXED])
+@pytest.mark.parametrize("which,steps", [("deck", 60), ("large", 6)])
+def test_multigrid_modes(mode, which, steps):
+    """V-cycles to a backward error of 1e-10 (FP64, or FP32 corrections in
+    FP64 defect correction): every Poisson solve meets the tolerance and the
+    interface stays within 1e-6 relative of the FP64 oracle."""
+    case = L.deck_case() if which == "deck" else L.single_large_case(1)
+    _say(f"multigrid mode {mode}, {which} case, {steps} steps")
+    b, o = L.Batch([case]), Oracle(case)
+    b.set_poisson(mode, 1e-10, 100)
+    b.run(steps)
+    assert o.step(steps) == 0
+    s = b.scalars()
+    assert s["POISSON_SOLVES"] == 3 * steps
+    assert 0 < s["POISSON_RESID"] <= 1e-10
+    assert s["POISSON_CYCLES"] / s["POISSON_SOLVES"] < 30
+    zo = o.scalar_array()[3]
+    assert abs(s["PLACE"] - zo) <= 1e-6 * zo
+
+
+@pytest.mark.parametrize("mode", [L.POISSON_MG64, L.POISSON_MIXED])
+def test_batched_poisson_multigrid(mode):
+    """lica_poisson by V-cycles on the POISSON systems of the deck case (the
+    last substep of steps 1..8): backward error <= 1e-10 and the solution
+    close to TRDIAG3's."""
+    _say(f"batched poisson multigrid mode {mode}")
+    o = Oracle(L.deck_case())
+    rows = []
+    for _ in range(8):
+        assert o.step(1) == 0
+        rows.append(o.last_poisson()[:4])
+    a, b, c, g = (np.array([r[j] for r in rows]) for j in range(4))
+    p = L.poisson(a, b, c, g, mode=mode, rtol=1e-10, max_cycles=100)
+    x = L.poisson(a, b, c, g, mode=L.POISSON_THOMAS)
+    for s in range(len(rows)):
+        r = g[s] - (b[s] * p[s] + np.r_[0.0, a[s, 1:] * p[s, :-1]] + np.r_[c[s, :-1] * p[s, 1:], 0.0])
+        anorm = np.max(np.abs(a[s]) + np.abs(b[s]) + np.abs(c[s]))
+        assert np.max(np.abs(r)) <= 1e-10 * (anorm * np.max(np.abs(p[s])) + np.max(np.abs(g[s])))
+        assert np.max(np.abs(p[s] - x[s])) <= 1e-4 * np.max(np.abs(x[s]))
