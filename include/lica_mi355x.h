@@ -85,7 +85,11 @@ enum {
 enum {
     LICA_F_W = 0, LICA_F_P = 1, LICA_F_T = 2, LICA_F_PSI_CN = 3,
     LICA_F_PSI_ZN = 4, LICA_F_PSIF = 5, LICA_F_PHI = 6, LICA_F_SFGAS = 7,
-    LICA_F_WO = 8, LICA_F_BAND = 9, LICA_F_ALPHI = 10, LICA_F_MFLVS = 11
+    LICA_F_WO = 8, LICA_F_BAND = 9, LICA_F_ALPHI = 10, LICA_F_MFLVS = 11,
+    /* rows 1..n of the last POISSON system (n entries each); PG holds the
+     * solution after a direct solve, the captured PRHS its right-hand side */
+    LICA_F_PA = 12, LICA_F_PB = 13, LICA_F_PC = 14, LICA_F_PG = 15,
+    LICA_F_PPREV = 16, LICA_F_PSOL = 17, LICA_F_PRHS = 18
 };
 
 /* scalar ids for lica_batch_get_scalars (LICA_NSCAL doubles) */
@@ -129,6 +133,33 @@ int lica_batch_save(lica_batch *b, const char *path);
 int lica_batch_load(const char *path, lica_batch **out);
 /* WRITEFIELD text file (1d.f:1121-1232, read by post.f READFLD 238-401) of one case */
 int lica_batch_write_fld(lica_batch *b, int64_t inst, const char *path, int64_t ihist);
+
+/* ---- learned warm start of the Poisson solve -------------------------- */
+/* No counterpart in the reference: a 1-D convolutional network (4 layers, 21
+ * taps, 32 channels, BF16 weights, FP32 accumulation on MFMA) maps per-cell
+ * features of a POISSON system (g/max|g|, a/max|b|, c/max|b|, P/max|P| with
+ * P the pressure before the solve) to (p - P)/max|P|; LICA_POISSON_LEARNED
+ * starts the mixed-precision V-cycles from P + max|P| * network.           */
+typedef struct lica_net lica_net;
+int64_t lica_net_param_count(void);
+int lica_net_create(uint64_t seed, lica_net **out);
+void lica_net_destroy(lica_net *net);
+/* parameters layer by layer: W[tap][cout][32], then bias[cout]; cout 32,32,32,16 */
+int lica_net_get_params(lica_net *net, float *out);
+int lica_net_set_params(lica_net *net, const float *in);
+/* features [cells][4] of nseq sequences of lens[s] cells -> y[cells] */
+int lica_net_forward(lica_net *net, const float *feat, const int64_t *lens, int64_t nseq, float *y);
+/* loss = mean (y - target)^2 and its gradient in the parameter layout */
+int lica_net_grad(lica_net *net, const float *feat, const float *target, const int64_t *lens, int64_t nseq,
+                  float *grad, double *loss);
+/* training windows kept on the GPU; Adam steps on random minibatches */
+int lica_net_set_data(lica_net *net, const float *feat, const float *target, int64_t n, int64_t len);
+int lica_net_train(lica_net *net, int64_t iters, int64_t batch, double lr, uint64_t seed, double *loss);
+/* the network a batch uses in LICA_POISSON_LEARNED (not owned by the batch) */
+int lica_batch_set_net(lica_batch *b, lica_net *net);
+/* keep, for every case, P before and after the last substep's POISSON solve
+ * and its right-hand side (fields LICA_F_PPREV, LICA_F_PSOL, LICA_F_PRHS)  */
+int lica_batch_set_capture(lica_batch *b, int64_t on);
 
 /* ---- single-case API mirroring the reference call sequence ----------- */
 int lica_init(lica_batch **ctx, const char *lica_in_path, const char *grid_in_path);

@@ -9,8 +9,9 @@ from ._abi import (CaseDesc, LicaError, FIELDS, SCALARS, POISSON_THOMAS, POISSON
                    POISSON_MG64, POISSON_MIXED, POISSON_LEARNED, load, status_string)
 from .solver import Batch, Context, parse_deck, poisson
 from .workloads import deck_case, single_large_case, batched_cases
+from .learned import Net, collect_pairs
 
 __all__ = ["CaseDesc", "LicaError", "FIELDS", "SCALARS", "POISSON_THOMAS", "POISSON_DIRECT",
            "POISSON_MG64", "POISSON_MIXED", "POISSON_LEARNED", "load", "status_string",
            "Batch", "Context", "parse_deck", "poisson", "deck_case", "single_large_case",
-           "batched_cases"]
+           "batched_cases", "Net", "collect_pairs"]

@@ -9,6 +9,7 @@ c_i64 = ctypes.c_int64
 c_dbl = ctypes.c_double
 P_dbl = ctypes.POINTER(ctypes.c_double)
 P_i64 = ctypes.POINTER(ctypes.c_int64)
+P_flt = ctypes.POINTER(ctypes.c_float)
 
 # status codes (lica_mi355x.h)
 OK = 0
@@ -29,7 +30,8 @@ NOT_CONVERGED = 32
 POISSON_THOMAS, POISSON_DIRECT, POISSON_MG64, POISSON_MIXED, POISSON_LEARNED = range(5)
 
 FIELDS = {"W": 0, "P": 1, "T": 2, "PSI_CN": 3, "PSI_ZN": 4, "PSIF": 5, "PHI": 6,
-          "SFGAS": 7, "WO": 8, "BAND": 9, "ALPHI": 10, "MFLVS": 11}
+          "SFGAS": 7, "WO": 8, "BAND": 9, "ALPHI": 10, "MFLVS": 11,
+          "PA": 12, "PB": 13, "PC": 14, "PG": 15, "PPREV": 16, "PSOL": 17, "PRHS": 18}
 SCALARS = ["TIME", "DT", "STEP", "PLACE", "DVMAX", "CFLM", "VOL_ORI", "VOL_IPHS",
            "QWALL", "STATUS", "SUBCYCLE_WARN", "MASS_WARN", "EXTRAP_HALVINGS",
            "EXTRAP_ITERS", "REINIT_CALLS", "BANDGEN_CALLS", "MASSCOR_ITERS",
@@ -101,6 +103,17 @@ def _declare(lib):
         "lica_end_step": (ctypes.c_int, [V]),
         "lica_poisson": (ctypes.c_int, [c_i64, c_i64, P_dbl, P_dbl, P_dbl, P_dbl, P_dbl, c_i64, c_dbl, c_i64, P_i64]),
         "lica_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+        "lica_batch_set_net": (ctypes.c_int, [V, V]),
+        "lica_batch_set_capture": (ctypes.c_int, [V, c_i64]),
+        "lica_net_param_count": (c_i64, []),
+        "lica_net_create": (ctypes.c_int, [ctypes.c_uint64, PV]),
+        "lica_net_destroy": (None, [V]),
+        "lica_net_get_params": (ctypes.c_int, [V, P_flt]),
+        "lica_net_set_params": (ctypes.c_int, [V, P_flt]),
+        "lica_net_forward": (ctypes.c_int, [V, P_flt, P_i64, c_i64, P_flt]),
+        "lica_net_grad": (ctypes.c_int, [V, P_flt, P_flt, P_i64, c_i64, P_flt, P_dbl]),
+        "lica_net_set_data": (ctypes.c_int, [V, P_flt, P_flt, c_i64, c_i64]),
+        "lica_net_train": (ctypes.c_int, [V, c_i64, c_i64, c_dbl, ctypes.c_uint64, P_dbl]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -12,9 +12,9 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblica_mi355x.so")
-SOURCES = ["lica_kernels.hip", "lica_host.cpp"]
+SOURCES = ["lica_kernels.hip", "lica_host.cpp", "lica_net.hip"]
 HEADERS = ["lica_types.h", "lica_par.h", "lica_launch.h", "lica_kern_common.h",
-           "lica_kern_band.h", "lica_kern_tgrad.h", "lica_kern_flow.h"]
+           "lica_kern_band.h", "lica_kern_bandreg.h", "lica_kern_tgrad.h", "lica_kern_flow.h", "lica_kern_mg.h"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -25,6 +25,23 @@
 int lica_emul_nthreads = 256;
 int lica_emul_depth = 0;
 typedef void *hipStream_t;
+// the warm-start network runs on MFMA only; the CPU logic-check build has none
+int lica_net_guess(lica_net *, const NetCase *, int, const int64_t *, double *, void *) { return LICA_UNSUPPORTED_DECK; }
+int lica_net_prepare(lica_net *, const int64_t *, int, const NetCase *) { return LICA_UNSUPPORTED_DECK; }
+extern "C" {
+int64_t lica_net_param_count(void) { return 0; }
+int lica_net_create(uint64_t, lica_net **out) { if (out) *out = nullptr; return LICA_UNSUPPORTED_DECK; }
+void lica_net_destroy(lica_net *) {}
+int lica_net_get_params(lica_net *, float *) { return LICA_UNSUPPORTED_DECK; }
+int lica_net_set_params(lica_net *, const float *) { return LICA_UNSUPPORTED_DECK; }
+int lica_net_forward(lica_net *, const float *, const int64_t *, int64_t, float *) { return LICA_UNSUPPORTED_DECK; }
+int lica_net_grad(lica_net *, const float *, const float *, const int64_t *, int64_t, float *, double *)
+{
+    return LICA_UNSUPPORTED_DECK;
+}
+int lica_net_set_data(lica_net *, const float *, const float *, int64_t, int64_t) { return LICA_UNSUPPORTED_DECK; }
+int lica_net_train(lica_net *, int64_t, int64_t, double, uint64_t, double *) { return LICA_UNSUPPORTED_DECK; }
+}
 #else
 #include <hip/hip_runtime.h>
 #endif
@@ -317,6 +334,11 @@ struct lica_batch {
     double *d_ns = nullptr, *d_rf = nullptr;
     int *d_nsi = nullptr, *d_rfi = nullptr, *d_order = nullptr;
     double *d_mg = nullptr;      // multigrid scratch (allocated on first use of a multigrid mode)
+    lica_net *net = nullptr;     // warm-start network of the learned mode (not owned)
+    double *d_guess = nullptr, *d_scale = nullptr, *d_cap = nullptr;
+    NetCase *d_netcases = nullptr;
+    std::vector<int64_t> lens;
+    bool capture = false;
     std::vector<std::vector<double>> img_ns, img_rf;     // initial images (geometry)
     std::vector<std::vector<int>> img_nsi, img_rfi;
     int threads = 256;
@@ -352,7 +374,8 @@ BatchDev make_bd(const lica_batch *b, int msub)
     bd.mg = b->d_mg;
     bd.mg_rtol = b->rtol;
     bd.mg_maxc = b->max_cycles;
-    bd.guess = nullptr;
+    bd.guess = b->d_guess;
+    bd.cap = b->capture ? b->d_cap : nullptr;
     return bd;
 }
 
@@ -377,6 +400,20 @@ int launch(lica_batch *b, int which, int msub)
     return rc == 0 ? 0 : LICA_HIP_ERROR;
 }
 
+// work between the POISSON matrix and its solve: the learned initial guess
+// and the capture of P before the last substep's solve
+int pre_poisson(lica_batch *b, int msub)
+{
+    int rc = 0;
+    if (b->capture && msub == 3) rc = launch(b, KL_CAPTURE_PRE, msub);
+    if (!rc && b->poisson_mode == LICA_POISSON_LEARNED) {
+        if (!b->net) return LICA_BAD_ARG;
+        if (lica_net_guess(b->net, b->d_netcases, (int)b->ninst, b->lens.data(), b->d_scale, (void *)b->stream))
+            rc = LICA_HIP_ERROR;
+    }
+    return rc;
+}
+
 // the launches of one time step (1d.f:196-295)
 int enqueue_step(lica_batch *b)
 {
@@ -386,7 +423,9 @@ int enqueue_step(lica_batch *b)
         if (!rc) rc = launch(b, KL_CONT_ENERGY, msub);
         if (!rc) rc = launch(b, KL_TEMP_GRAD, msub);
         if (!rc) rc = launch(b, KL_MOMENTUM, msub);
+        if (!rc) rc = pre_poisson(b, msub);
         if (!rc) rc = launch(b, kp, msub);
+        if (!rc && b->capture && msub == 3) rc = launch(b, KL_CAPTURE_POST, msub);
         if (!rc) rc = launch(b, KL_UCALC, msub);
         if (!rc) rc = launch(b, KL_TRANSPORT, msub);
     }
@@ -413,8 +452,14 @@ int enqueue_step_timed(lica_batch *b, std::vector<int> &kinds)
         const int kp = (b->poisson_mode >= LICA_POISSON_MG64) ? KL_POISSON_MG : KL_POISSON_DIRECT;
         const int seq[6] = {KL_CONT_ENERGY, KL_TEMP_GRAD, KL_MOMENTUM, kp, KL_UCALC, KL_TRANSPORT};
         for (int j = 0; j < 6 && !rc; j++) {
+            if (seq[j] == kp) {
+                rc = pre_poisson(b, msub);
+                if (!rc) rc = rec(KL_CAPTURE_PRE);    // time of the guess/capture work
+                if (rc) break;
+            }
             rc = launch(b, seq[j], msub);
             if (!rc) rc = rec(seq[j]);
+            if (!rc && seq[j] == kp && b->capture && msub == 3) rc = launch(b, KL_CAPTURE_POST, msub);
         }
     }
     return rc;
@@ -571,6 +616,7 @@ int lica_batch_create(int64_t ninst, const lica_case_desc *cases, lica_batch **o
     b->par.resize((size_t)ninst);
     b->img_ns.resize((size_t)ninst); b->img_rf.resize((size_t)ninst);
     b->img_nsi.resize((size_t)ninst); b->img_rfi.resize((size_t)ninst);
+    long long gs_total = 0;
     for (int64_t i = 0; i < ninst; i++) {
         CaseImage c;
         int rc = build_case(cases[i], c);
@@ -579,6 +625,8 @@ int lica_batch_create(int64_t ninst, const lica_case_desc *cases, lica_batch **o
         c.p.rf_off = (long long)b->rf_total; b->rf_total += c.rf.size();
         c.p.nsi_off = (long long)b->nsi_total; b->nsi_total += c.nsi.size();
         c.p.rfi_off = (long long)b->rfi_total; b->rfi_total += c.rfi.size();
+        c.p.gs_off = gs_total; gs_total += c.p.n + 2;
+        b->lens.push_back(c.p.n);
         b->par[(size_t)i] = c.p;
         b->img_ns[(size_t)i].swap(c.ns); b->img_rf[(size_t)i].swap(c.rf);
         b->img_nsi[(size_t)i].swap(c.nsi); b->img_rfi[(size_t)i].swap(c.rfi);
@@ -594,6 +642,9 @@ int lica_batch_create(int64_t ninst, const lica_case_desc *cases, lica_batch **o
     rc = rc ? rc : dmalloc((void **)&b->d_nsi, sizeof(int) * b->nsi_total);
     rc = rc ? rc : dmalloc((void **)&b->d_rfi, sizeof(int) * b->rfi_total);
     rc = rc ? rc : dmalloc((void **)&b->d_order, sizeof(int) * (size_t)ninst);
+    rc = rc ? rc : dmalloc((void **)&b->d_guess, sizeof(double) * (size_t)gs_total);
+    rc = rc ? rc : dmalloc((void **)&b->d_cap, sizeof(double) * 3 * (size_t)gs_total);
+    rc = rc ? rc : dmalloc((void **)&b->d_scale, sizeof(double) * (size_t)ninst);
     if (rc) { lica_batch_destroy(b); return rc; }
     // workgroup order: largest cases first (longest-processing-time first)
     std::vector<int> order((size_t)ninst);
@@ -679,7 +730,7 @@ int lica_batch_set_poisson(lica_batch *b, int64_t mode, double rtol, int64_t max
     int rc = check_batch(b);
     if (rc) return rc;
     if (mode < LICA_POISSON_THOMAS || mode > LICA_POISSON_LEARNED || !(rtol > 0.0) || max_cycles < 1) return LICA_BAD_ARG;
-    if (mode == LICA_POISSON_LEARNED) return LICA_UNSUPPORTED_DECK;
+    if (mode == LICA_POISSON_LEARNED && !b->net) return LICA_BAD_ARG;
     if (mode >= LICA_POISSON_MG64 && !b->d_mg) {
         long long tot = 0;
         for (InstParam &p : b->par) { p.mg_off = tot; tot += lica_mg_scratch_doubles(p.n); }
@@ -804,6 +855,21 @@ int lica_batch_get_field(lica_batch *b, int64_t field, int64_t inst, double *out
     case LICA_F_PSI_CN: rc = ns_field(NS_PSI_CN); break;
     case LICA_F_PSI_ZN: rc = ns_field(NS_PSI_ZN); break;
     case LICA_F_WO: rc = ns_field(NS_WO); break;
+    case LICA_F_PA: case LICA_F_PB: case LICA_F_PC: case LICA_F_PG: {
+        const int f = (field == LICA_F_PA) ? NS_PA : (field == LICA_F_PB) ? NS_PB : (field == LICA_F_PC) ? NS_PC : NS_PG;
+        buf.resize((size_t)p.n);
+        rc = d2h(buf.data(), b->d_ns + p.ns_off + (size_t)f * p.ln + 1 + 2, sizeof(double) * buf.size(), b->stream);
+        for (size_t k = 0; k < buf.size() && m < cap; k++) out[m++] = buf[k];
+        break;
+    }
+    case LICA_F_PPREV: case LICA_F_PSOL: case LICA_F_PRHS: {
+        buf.resize((size_t)p.n);
+        const int which = (field == LICA_F_PPREV) ? 0 : (field == LICA_F_PSOL) ? 1 : 2;
+        const long long o = 3 * p.gs_off + (long long)which * (p.n + 2) + 1;
+        rc = d2h(buf.data(), b->d_cap + o, sizeof(double) * buf.size(), b->stream);
+        for (size_t k = 0; k < buf.size() && m < cap; k++) out[m++] = buf[k];
+        break;
+    }
     case LICA_F_PSIF: rc = rf_field(RF_PSIF); break;
     case LICA_F_SFGAS: rc = rf_field(RF_SFGAS); break;
     case LICA_F_PHI: case LICA_F_BAND: case LICA_F_ALPHI: case LICA_F_MFLVS: {
@@ -856,10 +922,45 @@ void lica_batch_destroy(lica_batch *b)
 #endif
     dfree(b->d_par); dfree(b->d_st); dfree(b->d_ns); dfree(b->d_rf);
     dfree(b->d_nsi); dfree(b->d_rfi); dfree(b->d_order); dfree(b->d_mg);
+    dfree(b->d_guess); dfree(b->d_scale); dfree(b->d_cap); dfree(b->d_netcases);
 #ifndef LICA_EMUL
     if (b->stream) (void)hipStreamDestroy(b->stream);
 #endif
     delete b;
+}
+
+int lica_batch_set_net(lica_batch *b, lica_net *net)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (!net) return LICA_BAD_ARG;
+    std::vector<NetCase> nc((size_t)b->ninst);
+    long long off = 0;
+    for (int64_t i = 0; i < b->ninst; i++) {
+        const InstParam &p = b->par[(size_t)i];
+        double *base = b->d_ns + p.ns_off + 2;
+        NetCase &c = nc[(size_t)i];
+        c.a = base + (size_t)NS_PA * p.ln; c.b = base + (size_t)NS_PB * p.ln;
+        c.c = base + (size_t)NS_PC * p.ln; c.g = base + (size_t)NS_PG * p.ln;
+        c.p = base + (size_t)NS_P * p.ln;
+        c.guess = b->d_guess + p.gs_off;
+        c.off = off; c.n = p.n;
+        off += p.n;
+    }
+    if (!b->d_netcases) rc = dmalloc((void **)&b->d_netcases, sizeof(NetCase) * nc.size());
+    rc = rc ? rc : h2d(b->d_netcases, nc.data(), sizeof(NetCase) * nc.size(), b->stream);
+    if (rc) return rc;
+    if (lica_net_prepare(net, b->lens.data(), (int)b->ninst, b->d_netcases)) return LICA_HIP_ERROR;
+    b->net = net;
+    return LICA_OK;
+}
+
+int lica_batch_set_capture(lica_batch *b, int64_t on)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    b->capture = on != 0;
+    return LICA_OK;
 }
 
 // ---- single-case API -------------------------------------------------
@@ -906,7 +1007,10 @@ int lica_rhsnlhs(lica_batch *ctx, int64_t msub, double *w, double *p, double *ps
     ctx->last_msub = (int)msub;
     const int kp = (ctx->poisson_mode >= LICA_POISSON_MG64) ? KL_POISSON_MG : KL_POISSON_DIRECT;
     const int seq[5] = {KL_CONT_ENERGY, KL_TEMP_GRAD, KL_MOMENTUM, kp, KL_UCALC};
-    for (int j = 0; j < 5 && !rc; j++) rc = launch(ctx, seq[j], (int)msub);
+    for (int j = 0; j < 5 && !rc; j++) {
+        if (seq[j] == kp) rc = pre_poisson(ctx, (int)msub);
+        if (!rc) rc = launch(ctx, seq[j], (int)msub);
+    }
     rc = rc ? rc : dsync(ctx->stream);
     rc = rc ? rc : copy_ns_out(ctx, NS_W, w);
     rc = rc ? rc : copy_ns_out(ctx, NS_P, p);

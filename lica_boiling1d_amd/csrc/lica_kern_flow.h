@@ -162,7 +162,7 @@ DEV void body_poisson_mg(const BatchDev &bd, int inst, BlkSh &sh)
     const int n = P.n, n3 = P.n3;
     double *PP = cp.A(NS_P);
     const double *PA = cp.A(NS_PA), *PB = cp.A(NS_PB), *PCc = cp.A(NS_PC), *PG = cp.A(NS_PG);
-    const double *x0 = (bd.pmode == LICA_MG_LEARNED_MODE && bd.guess) ? bd.guess + P.ns_off + (size_t)NS_P * P.ln + 2 : nullptr;
+    const double *x0 = (bd.pmode == LICA_MG_LEARNED_MODE && bd.guess) ? bd.guess + P.gs_off : nullptr;
     double res = 0.0;
     int cyc = mg_solve(PA, PB, PCc, PG, PP, n, bd.mg + P.mg_off, bd.pmode, bd.mg_rtol, bd.mg_maxc, x0, sh.mgws,
                        sh.red, &res);
@@ -175,6 +175,25 @@ DEV void body_poisson_mg(const BatchDev &bd, int inst, BlkSh &sh)
         if (cyc < 0) S.status = ST_NOT_CONVERGED;
     }
     SERIAL_END
+}
+
+// capture rows of every case (3 x (n+2) at 3*gs_off): before the POISSON
+// solve (which 0) P and the right-hand side, after it (which 1) P
+DEV void body_capture(const BatchDev &bd, int inst, int which)
+{
+    InstState &S = bd.st[inst];
+    if (S.status || !bd.cap) return;
+    CaseP cp = case_ptrs(bd, inst);
+    const InstParam &P = *cp.P;
+    const double *PP = cp.A(NS_P), *PG = cp.A(NS_PG);
+    double *base = bd.cap + 3 * P.gs_off;
+    double *dst = base + (long long)which * (P.n + 2);
+    double *rhs = base + 2LL * (P.n + 2);
+    PFOR(k, 1, P.n + 1) {
+        dst[k] = PP[k];
+        if (which == 0) rhs[k] = PG[k];
+    }
+    PFOR_END
 }
 
 // UCALC (1d.f:3582-3627), p -= p(1) (3433-3470), CONVRGE1 (3306-3362), the

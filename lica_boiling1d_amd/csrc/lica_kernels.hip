@@ -77,6 +77,11 @@ int lica_launch_mg(int nsys, int n, const double *a, const double *b, const doub
     return (int)hipGetLastError();
 }
 
+__global__ void __launch_bounds__(1024) k_capture(BatchDev bd, int which)
+{
+    body_capture(bd, bd.order[blockIdx.x], which);
+}
+
 __global__ void __launch_bounds__(1024) k_ucalc(BatchDev bd)
 {
     __shared__ BlkSh sh;
@@ -143,6 +148,12 @@ int lica_launch(int which, const BatchDev &bd, int threads, void *stream)
     case KL_UCALC: LAUNCH(k_ucalc, threads); break;
     case KL_TRANSPORT: LAUNCH(k_transport, 64); break;
     case KL_POISSON_MG: LAUNCH(k_poisson_mg, threads); break;
+    case KL_CAPTURE_PRE:
+        hipLaunchKernelGGL(k_capture, dim3(bd.ninst), dim3(256), 0, (hipStream_t)stream, bd, 0);
+        break;
+    case KL_CAPTURE_POST:
+        hipLaunchKernelGGL(k_capture, dim3(bd.ninst), dim3(256), 0, (hipStream_t)stream, bd, 1);
+        break;
     default: return -1;
     }
     return (int)hipGetLastError();
@@ -226,6 +237,12 @@ int lica_launch(int which, const BatchDev &bd, int threads, void *stream)
     case KL_UCALC: emul_run<BlkSh>(bd, threads, body_ucalc); break;
     case KL_TRANSPORT: emul_run<TransportSh>(bd, 64, body_transport); break;
     case KL_POISSON_MG: emul_run<BlkSh>(bd, threads, body_poisson_mg); break;
+    case KL_CAPTURE_PRE:
+        for (int b = 0; b < bd.ninst; b++) body_capture(bd, bd.order[b], 0);
+        break;
+    case KL_CAPTURE_POST:
+        for (int b = 0; b < bd.ninst; b++) body_capture(bd, bd.order[b], 1);
+        break;
     default: return -1;
     }
     return 0;

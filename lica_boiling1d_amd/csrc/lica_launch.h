@@ -5,7 +5,7 @@
 
 enum KernelId {
     KL_INIT = 0, KL_DT, KL_CONT_ENERGY, KL_TEMP_GRAD, KL_MOMENTUM,
-    KL_POISSON_DIRECT, KL_UCALC, KL_TRANSPORT, KL_POISSON_MG, KL_COUNT
+    KL_POISSON_DIRECT, KL_UCALC, KL_TRANSPORT, KL_POISSON_MG, KL_CAPTURE_PRE, KL_CAPTURE_POST, KL_COUNT
 };
 
 int lica_launch(int which, const BatchDev &bd, int threads, void *stream);
@@ -14,3 +14,14 @@ int lica_launch_mg(int nsys, int n, const double *a, const double *b, const doub
                    double *scratch, int mode, double rtol, int maxc, int *cycles, double *resid, int threads,
                    void *stream);
 long long lica_mg_scratch_doubles(int n);
+
+// warm-start network (lica_net.hip): one POISSON system per case
+struct NetCase {
+    const double *a, *b, *c, *g, *p;   // rows 1..n of the case (index k)
+    double *guess;                     // rows 1..n: receives P + s_p * network
+    long long off;                     // first cell of the case in the activation arena
+    int n;
+};
+struct lica_net;
+int lica_net_guess(lica_net *t, const NetCase *dcases, int ncase, const int64_t *lens, double *dscale, void *stream);
+int lica_net_prepare(lica_net *t, const int64_t *lens, int ncase, const NetCase *dcases);

@@ -1,0 +1,676 @@
+// Convolutional warm start of the POISSON solve (gfx950, BF16 MFMA).
+//
+// The network maps per-cell features of a POISSON system (1d.f:3529-3543) --
+// the right-hand side, the two off-diagonal coefficients and the previous
+// substep's pressure, each scaled per system -- to a correction of the
+// previous pressure; the multigrid solve then starts from that guess.
+// Layers: NET_LAYERS 1-D convolutions of NET_TAPS taps over the cell index,
+// NET_C channels (features padded to NET_C), ReLU between layers, one output
+// channel.  Weights and activations are stored in BF16, every product is
+// accumulated in FP32 by v_mfma_f32_16x16x32_bf16, master weights and the
+// optimiser are FP32.
+//
+// Activation layout: channels-last, [cell][NET_C] BF16 per sequence, with
+// sequences (one POISSON system, or one training window) zero-padded at both
+// ends.  Weight layout for the MFMA B operand: [tap][co][ci] BF16.
+//
+// Operand layouts of v_mfma_f32_16x16x32_bf16 (wave of 64 lanes, l = lane):
+//   A (16 x 32, rows x k): lane l holds row l%16, k = 8*(l/16) .. +7
+//   B (32 x 16, k x cols): lane l holds col l%16, k = 8*(l/16) .. +7
+//   D (16 x 16 FP32):      lane l holds col l%16, rows 4*(l/16) .. +3
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <stdint.h>
+#include "lica_launch.h"
+
+#define NET_C 32
+#define NET_TAPS 21
+#define NET_HALF (NET_TAPS / 2)
+#define NET_LAYERS 4
+#define NET_TILE 64          // cells per workgroup in the convolution kernels
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ __bf16 to_bf16(float v) { return (__bf16)v; }
+__device__ __forceinline__ float from_bf16(__bf16 v) { return (float)v; }
+
+struct NetSeq {
+    long long off;   // first cell of the sequence in the activation arena
+    int len;         // cells
+    int tile0;       // first tile of the sequence in the tile list
+};
+
+// ---------------------------------------------------------------------
+// Y[cell][co] = act(bias[co] + sum_t sum_ci W[t][co][ci] X[cell + t - HALF][ci])
+// grid: one workgroup per tile (tile -> sequence via seq_of_tile), 256 threads.
+// cout is 32 or 16 (the last layer, one meaningful channel); relu 0/1;
+// yf (optional) receives the FP32 pre-activation of channel 0.
+__global__ void __launch_bounds__(256) k_net_conv(const __bf16 *__restrict__ X, const __bf16 *__restrict__ W,
+                                                  const float *__restrict__ bias, __bf16 *__restrict__ Y,
+                                                  float *__restrict__ yf, const NetSeq *__restrict__ seqs,
+                                                  const int *__restrict__ seq_of_tile, int cout, int relu)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 xs[(NET_TILE + NET_TAPS) * NET_C];
+    __shared__ __attribute__((aligned(16))) __bf16 ws[NET_TAPS * NET_C * NET_C];
+    const int tile = blockIdx.x;
+    const NetSeq sq = seqs[seq_of_tile[tile]];
+    const int c0 = (tile - sq.tile0) * NET_TILE;      // first cell of the tile in the sequence
+    const int tid = threadIdx.x;
+    // stage the input rows c0-HALF .. c0+TILE+HALF-1 (zero outside the sequence)
+    for (int i = tid; i < (NET_TILE + NET_TAPS - 1) * NET_C / 8; i += blockDim.x) {
+        const int row = i / (NET_C / 8), part = i % (NET_C / 8);
+        const int cell = c0 - NET_HALF + row;
+        bf16x8 v;
+        if (cell >= 0 && cell < sq.len) v = *(const bf16x8 *)(X + (sq.off + cell) * NET_C + part * 8);
+        else for (int j = 0; j < 8; j++) v[j] = to_bf16(0.0f);
+        *(bf16x8 *)(xs + row * NET_C + part * 8) = v;
+    }
+    for (int i = tid; i < NET_TAPS * cout * NET_C / 8; i += blockDim.x)
+        *(bf16x8 *)(ws + i * 8) = *(const bf16x8 *)(W + i * 8);
+    __syncthreads();
+    const int wave = tid >> 6, lane = tid & 63;
+    const int m = lane & 15, kq = lane >> 4;                  // operand row / k group
+    const int ntiles = cout / 16;
+    f32x4 acc[2];
+    for (int nt = 0; nt < 2; nt++) acc[nt] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+    const int r0 = wave * 16;                                 // first tile row of this wave
+    for (int t = 0; t < NET_TAPS; t++) {
+        const bf16x8 a = *(const bf16x8 *)(xs + (r0 + m + t) * NET_C + kq * 8);
+        for (int nt = 0; nt < ntiles; nt++) {
+            const bf16x8 b = *(const bf16x8 *)(ws + (t * cout + nt * 16 + m) * NET_C + kq * 8);
+            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[nt], 0, 0, 0);
+        }
+    }
+    // epilogue: lane holds col (l%16), rows 4*(l/16)+i
+    for (int nt = 0; nt < ntiles; nt++) {
+        const int co = nt * 16 + m;
+        for (int i = 0; i < 4; i++) {
+            const int cell = c0 + r0 + kq * 4 + i;
+            if (cell >= sq.len) continue;
+            float v = acc[nt][i] + bias[co];
+            if (yf && co == 0) yf[sq.off + cell] = v;
+            if (relu) v = fmaxf(v, 0.0f);
+            Y[(sq.off + cell) * NET_C + co] = to_bf16(v);
+        }
+    }
+    if (cout == 16) {   // zero the padding channels 16..31 of the output rows
+        for (int i = tid; i < NET_TILE * 16; i += blockDim.x) {
+            const int row = i / 16, ch = 16 + i % 16;
+            const int cell = c0 + row;
+            if (cell < sq.len) Y[(sq.off + cell) * NET_C + ch] = to_bf16(0.0f);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------
+// dW[t][co][ci] += sum_cells dZ[cell][co] X[cell + t - HALF][ci] and
+// db[co] += sum_cells dZ[cell][co] over the tile's cells (FP32 atomics).
+// A = dZ^T (co x cells), B = X shifted (cells x ci): K runs over 32 cells.
+__global__ void __launch_bounds__(256) k_net_wgrad(const __bf16 *__restrict__ X, const __bf16 *__restrict__ dZ,
+                                                   float *__restrict__ dW, float *__restrict__ db,
+                                                   const NetSeq *__restrict__ seqs,
+                                                   const int *__restrict__ seq_of_tile, int cout)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 xs[(NET_TILE + NET_TAPS) * NET_C];
+    __shared__ __attribute__((aligned(16))) __bf16 zt[NET_C * NET_TILE];   // dZ transposed: [co][cell]
+    const int tile = blockIdx.x;
+    const NetSeq sq = seqs[seq_of_tile[tile]];
+    const int c0 = (tile - sq.tile0) * NET_TILE;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < (NET_TILE + NET_TAPS - 1) * NET_C; i += blockDim.x) {
+        const int row = i / NET_C, ch = i % NET_C;
+        const int cell = c0 - NET_HALF + row;
+        xs[row * NET_C + ch] = (cell >= 0 && cell < sq.len) ? X[(sq.off + cell) * NET_C + ch] : to_bf16(0.0f);
+    }
+    for (int i = tid; i < NET_C * NET_TILE; i += blockDim.x) {
+        const int co = i / NET_TILE, r = i % NET_TILE;
+        const int cell = c0 + r;
+        zt[co * NET_TILE + r] = (cell < sq.len && co < cout) ? dZ[(sq.off + cell) * NET_C + co] : to_bf16(0.0f);
+    }
+    __syncthreads();
+    const int wave = tid >> 6, lane = tid & 63;
+    const int m = lane & 15, kq = lane >> 4;
+    const int mt_n = cout / 16;
+    // taps split over the waves; 2 x 2 (or 1 x 2) output tiles per tap
+    for (int t = wave; t < NET_TAPS; t += 4) {
+        for (int mt = 0; mt < mt_n; mt++) {
+            for (int nt = 0; nt < 2; nt++) {
+                f32x4 acc = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+                for (int kb = 0; kb < NET_TILE; kb += 32) {
+                    bf16x8 a, b;
+                    for (int j = 0; j < 8; j++) {
+                        const int r = kb + kq * 8 + j;                       // cell of the tile
+                        a[j] = zt[(mt * 16 + m) * NET_TILE + r];
+                        b[j] = xs[(r + t) * NET_C + nt * 16 + m];
+                    }
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+                }
+                for (int i = 0; i < 4; i++) {
+                    const int co = mt * 16 + kq * 4 + i, ci = nt * 16 + m;
+                    atomicAdd(&dW[(t * cout + co) * NET_C + ci], acc[i]);
+                }
+            }
+        }
+    }
+    if (tid < cout) {
+        float s = 0.0f;
+        for (int r = 0; r < NET_TILE; r++) s += from_bf16(zt[tid * NET_TILE + r]);
+        atomicAdd(&db[tid], s);
+    }
+}
+
+// dX = conv(dZ, flipped transposed W) is k_net_conv with these weights:
+// Wt[t'][ci][co] = W[HALF*2 - t'][co][ci] (cin of the transposed layer = cout).
+__global__ void k_net_flip(const float *__restrict__ W, __bf16 *__restrict__ Wt, int cout, int cin)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NET_TAPS * NET_C * NET_C) return;
+    const int t = i / (NET_C * NET_C), ci = (i / NET_C) % NET_C, co = i % NET_C;   // output [t][ci][co]
+    float v = 0.0f;
+    if (co < cout && ci < cin) v = W[((NET_TAPS - 1 - t) * cout + co) * NET_C + ci];
+    Wt[i] = to_bf16(v);
+}
+
+// BF16 copy of FP32 weights ([t][co][ci], n elements)
+__global__ void k_net_cast(const float *__restrict__ w, __bf16 *__restrict__ wb, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) wb[i] = to_bf16(w[i]);
+}
+
+// dZ_prev = dX * (Yprev > 0)   (ReLU of the previous layer), channels-last
+__global__ void k_net_relu_mask(__bf16 *__restrict__ dX, const __bf16 *__restrict__ Yprev, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && !(from_bf16(Yprev[i]) > 0.0f)) dX[i] = to_bf16(0.0f);
+}
+
+// loss = mean (y - target)^2 over cells; dZ_last[cell][0] = 2 (y - target) / count
+__global__ void k_net_loss(const float *__restrict__ y, const float *__restrict__ target, __bf16 *__restrict__ dZ,
+                           long long ncell, float inv_count, float *__restrict__ loss)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0.0f;
+    if (i < ncell) {
+        const float d = y[i] - target[i];
+        l = d * d * inv_count;
+        dZ[i * NET_C] = to_bf16(2.0f * d * inv_count);
+        for (int c = 1; c < NET_C; c++) dZ[i * NET_C + c] = to_bf16(0.0f);
+    }
+    for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(loss, l);
+}
+
+// minibatch gather: window idx[s] of the data set into row block s
+__global__ void k_net_gather(const __bf16 *__restrict__ dx, const float *__restrict__ dt, const int *__restrict__ idx,
+                             __bf16 *__restrict__ x, float *__restrict__ t, int len, int batch)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)batch * len) return;
+    const int s = (int)(i / len), c = (int)(i % len);
+    const long long src = (long long)idx[s] * len + c;
+    for (int ch = 0; ch < NET_C; ch++) x[i * NET_C + ch] = dx[src * NET_C + ch];
+    t[i] = dt[src];
+}
+
+// Adam on FP32 master weights
+__global__ void k_net_adam(float *__restrict__ w, float *__restrict__ m, float *__restrict__ v,
+                           const float *__restrict__ g, int n, float lr, float b1, float b2, float eps, float c1,
+                           float c2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// ---------------------------------------------------------------------
+// Features of the POISSON system of every case of a batch (one workgroup per
+// case): scales s_g = max|g|, s_b = max|b|, s_p = max|P before the solve|,
+// features g/s_g, a/s_b, c/s_b, P/s_p in channels 0..3 (the rest zero).
+
+__global__ void __launch_bounds__(256) k_net_features(const NetCase *__restrict__ cases, __bf16 *__restrict__ X,
+                                                      double *__restrict__ scale)
+{
+    __shared__ double red[3][8];
+    const NetCase cs = cases[blockIdx.x];
+    const int tid = threadIdx.x;
+    double sg = 0.0, sb = 0.0, sp = 0.0;
+    for (int k = 1 + tid; k <= cs.n; k += blockDim.x) {
+        sg = fmax(sg, fabs(cs.g[k]));
+        sb = fmax(sb, fabs(cs.b[k]));
+        sp = fmax(sp, fabs(cs.p[k]));
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sg = fmax(sg, __shfl_xor(sg, off, 64));
+        sb = fmax(sb, __shfl_xor(sb, off, 64));
+        sp = fmax(sp, __shfl_xor(sp, off, 64));
+    }
+    if ((tid & 63) == 0) { red[0][tid >> 6] = sg; red[1][tid >> 6] = sb; red[2][tid >> 6] = sp; }
+    __syncthreads();
+    sg = 0.0; sb = 0.0; sp = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) { sg = fmax(sg, red[0][w]); sb = fmax(sb, red[1][w]); sp = fmax(sp, red[2][w]); }
+    if (sg == 0.0) sg = 1.0;
+    if (sb == 0.0) sb = 1.0;
+    if (sp == 0.0) sp = 1.0;
+    for (int k = 1 + tid; k <= cs.n; k += blockDim.x) {
+        __bf16 *row = X + (cs.off + k - 1) * NET_C;
+        row[0] = to_bf16((float)(cs.g[k] / sg));
+        row[1] = to_bf16((float)(cs.a[k] / sb));
+        row[2] = to_bf16((float)(cs.c[k] / sb));
+        row[3] = to_bf16((float)(cs.p[k] / sp));
+        for (int ch = 4; ch < NET_C; ch++) row[ch] = to_bf16(0.0f);
+    }
+    if (tid == 0) scale[blockIdx.x] = sp;
+}
+
+// guess(k) = P(k) + s_p * y(k)
+__global__ void __launch_bounds__(256) k_net_guess(const NetCase *__restrict__ cases, const float *__restrict__ y,
+                                                   const double *__restrict__ scale)
+{
+    const NetCase cs = cases[blockIdx.x];
+    const double sp = scale[blockIdx.x];
+    for (int k = 1 + threadIdx.x; k <= cs.n; k += blockDim.x) cs.guess[k] = cs.p[k] + sp * (double)y[cs.off + k - 1];
+}
+
+// ---------------------------------------------------------------------
+// host side of the network
+#include "../../include/lica_mi355x.h"
+#include <vector>
+#include <random>
+#include <cstring>
+#include <cmath>
+
+namespace {
+const int NET_COUT[NET_LAYERS] = {NET_C, NET_C, NET_C, 16};
+const int NET_CIN_REAL[NET_LAYERS] = {4, NET_C, NET_C, NET_C};
+
+size_t wsize(int l) { return (size_t)NET_TAPS * NET_COUT[l] * NET_C; }
+
+int hip_ok(hipError_t e) { return e == hipSuccess ? 0 : LICA_HIP_ERROR; }
+}  // namespace
+
+struct lica_net {
+    float *w[NET_LAYERS] = {}, *b[NET_LAYERS] = {};                 // FP32 master
+    float *gw[NET_LAYERS] = {}, *gb[NET_LAYERS] = {};               // gradients
+    float *mw[NET_LAYERS] = {}, *vw[NET_LAYERS] = {}, *mb[NET_LAYERS] = {}, *vb[NET_LAYERS] = {};
+    __bf16 *wb[NET_LAYERS] = {}, *wt[NET_LAYERS] = {};              // BF16 forward / flipped
+    float *zero_bias = nullptr;
+    long long adam_t = 0;
+    hipStream_t stream = nullptr;
+    // activation arena
+    long long cap_cells = 0;
+    int cap_tiles = 0, cap_seqs = 0;
+    __bf16 *act[NET_LAYERS + 1] = {};   // act[0] features, act[l+1] output of layer l
+    __bf16 *dz[2] = {};
+    float *yf = nullptr, *target = nullptr, *loss = nullptr;
+    NetSeq *seqs = nullptr;
+    int *seq_of_tile = nullptr;
+    int nseq = 0, ntiles = 0;
+    long long ncell = 0;
+    // training set (device resident, fixed-length windows)
+    __bf16 *data_x = nullptr;
+    float *data_t = nullptr;
+    int *idx = nullptr;
+    long long data_n = 0;
+    int data_len = 0;
+    const void *layout_owner = nullptr;   // batch whose cases the current layout holds
+};
+
+namespace {
+int net_alloc(void **p, size_t bytes)
+{
+    if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess) { *p = nullptr; return LICA_HIP_ERROR; }
+    return hip_ok(hipMemset(*p, 0, bytes ? bytes : 1));
+}
+
+// room for ncell cells in nseq sequences
+int net_reserve(lica_net *t, long long ncell, int nseq, int ntiles)
+{
+    if (ncell <= t->cap_cells && nseq <= t->cap_seqs && ntiles <= t->cap_tiles) return 0;
+    long long cells = std::max(ncell, t->cap_cells);
+    int seqs = std::max(nseq, t->cap_seqs), tiles = std::max(ntiles, t->cap_tiles);
+    for (int l = 0; l <= NET_LAYERS; l++) { (void)hipFree(t->act[l]); t->act[l] = nullptr; }
+    for (int i = 0; i < 2; i++) { (void)hipFree(t->dz[i]); t->dz[i] = nullptr; }
+    (void)hipFree(t->yf); (void)hipFree(t->target); (void)hipFree(t->seqs); (void)hipFree(t->seq_of_tile);
+    int rc = 0;
+    const size_t ab = sizeof(__bf16) * (size_t)cells * NET_C;
+    for (int l = 0; l <= NET_LAYERS && !rc; l++) rc = net_alloc((void **)&t->act[l], ab);
+    for (int i = 0; i < 2 && !rc; i++) rc = net_alloc((void **)&t->dz[i], ab);
+    rc = rc ? rc : net_alloc((void **)&t->yf, sizeof(float) * (size_t)cells);
+    rc = rc ? rc : net_alloc((void **)&t->target, sizeof(float) * (size_t)cells);
+    rc = rc ? rc : net_alloc((void **)&t->seqs, sizeof(NetSeq) * (size_t)seqs);
+    rc = rc ? rc : net_alloc((void **)&t->seq_of_tile, sizeof(int) * (size_t)tiles);
+    if (rc) return rc;
+    t->cap_cells = cells; t->cap_seqs = seqs; t->cap_tiles = tiles;
+    return 0;
+}
+
+// sequence and tile tables for the given lengths (cells contiguous in order)
+int net_layout(lica_net *t, const int64_t *lens, int nseq)
+{
+    std::vector<NetSeq> sq((size_t)nseq);
+    std::vector<int> sot;
+    long long off = 0;
+    for (int s = 0; s < nseq; s++) {
+        sq[(size_t)s].off = off;
+        sq[(size_t)s].len = (int)lens[s];
+        sq[(size_t)s].tile0 = (int)sot.size();
+        const int nt = (int)((lens[s] + NET_TILE - 1) / NET_TILE);
+        for (int i = 0; i < nt; i++) sot.push_back(s);
+        off += lens[s];
+    }
+    int rc = net_reserve(t, off, nseq, (int)sot.size());
+    if (rc) return rc;
+    rc = hip_ok(hipMemcpyAsync(t->seqs, sq.data(), sizeof(NetSeq) * sq.size(), hipMemcpyHostToDevice, t->stream));
+    rc = rc ? rc : hip_ok(hipMemcpyAsync(t->seq_of_tile, sot.data(), sizeof(int) * sot.size(), hipMemcpyHostToDevice, t->stream));
+    t->nseq = nseq;
+    t->ntiles = (int)sot.size();
+    t->ncell = off;
+    t->layout_owner = nullptr;
+    return rc ? rc : hip_ok(hipStreamSynchronize(t->stream));
+}
+
+int net_cast_weights(lica_net *t)
+{
+    for (int l = 0; l < NET_LAYERS; l++) {
+        const int n = (int)wsize(l);
+        hipLaunchKernelGGL(k_net_cast, dim3((n + 255) / 256), dim3(256), 0, t->stream, t->w[l], t->wb[l], n);
+    }
+    return hip_ok(hipGetLastError());
+}
+
+// forward over the current layout; act[0] must hold the features
+int net_forward(lica_net *t)
+{
+    int rc = net_cast_weights(t);
+    for (int l = 0; l < NET_LAYERS && !rc; l++) {
+        const bool last = (l == NET_LAYERS - 1);
+        hipLaunchKernelGGL(k_net_conv, dim3(t->ntiles), dim3(256), 0, t->stream, t->act[l], t->wb[l], t->b[l],
+                           t->act[l + 1], last ? t->yf : (float *)nullptr, t->seqs, t->seq_of_tile, NET_COUT[l],
+                           last ? 0 : 1);
+        rc = hip_ok(hipGetLastError());
+    }
+    return rc;
+}
+
+// loss and gradients of mean (y - target)^2 for the current layout
+int net_backward(lica_net *t, double *loss_out)
+{
+    int rc = hip_ok(hipMemsetAsync(t->loss, 0, sizeof(float), t->stream));
+    for (int l = 0; l < NET_LAYERS && !rc; l++) {
+        rc = hip_ok(hipMemsetAsync(t->gw[l], 0, sizeof(float) * wsize(l), t->stream));
+        rc = rc ? rc : hip_ok(hipMemsetAsync(t->gb[l], 0, sizeof(float) * NET_COUT[l], t->stream));
+    }
+    if (rc) return rc;
+    const long long nc = t->ncell;
+    hipLaunchKernelGGL(k_net_loss, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, t->stream, t->yf, t->target,
+                       t->dz[0], nc, (float)(1.0 / (double)nc), t->loss);
+    int cur = 0;
+    for (int l = NET_LAYERS - 1; l >= 0 && !rc; l--) {
+        hipLaunchKernelGGL(k_net_wgrad, dim3(t->ntiles), dim3(256), 0, t->stream, t->act[l], t->dz[cur], t->gw[l],
+                           t->gb[l], t->seqs, t->seq_of_tile, NET_COUT[l]);
+        if (l > 0) {
+            const int n = NET_TAPS * NET_C * NET_C;
+            hipLaunchKernelGGL(k_net_flip, dim3((n + 255) / 256), dim3(256), 0, t->stream, t->w[l], t->wt[l],
+                               NET_COUT[l], NET_C);
+            hipLaunchKernelGGL(k_net_conv, dim3(t->ntiles), dim3(256), 0, t->stream, t->dz[cur], t->wt[l],
+                               t->zero_bias, t->dz[1 - cur], (float *)nullptr, t->seqs, t->seq_of_tile, NET_C, 0);
+            const long long ne = nc * NET_C;
+            hipLaunchKernelGGL(k_net_relu_mask, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, t->stream,
+                               t->dz[1 - cur], t->act[l], ne);
+            cur = 1 - cur;
+        }
+        rc = hip_ok(hipGetLastError());
+    }
+    if (!rc && loss_out) {
+        float l = 0.0f;
+        rc = hip_ok(hipMemcpyAsync(&l, t->loss, sizeof(float), hipMemcpyDeviceToHost, t->stream));
+        rc = rc ? rc : hip_ok(hipStreamSynchronize(t->stream));
+        *loss_out = l;
+    }
+    return rc;
+}
+
+// host features [cells][4] (FP32) -> act[0] (BF16, 32 channels)
+int net_upload_features(lica_net *t, const float *feat, long long ncell)
+{
+    std::vector<__bf16> x((size_t)ncell * NET_C);
+    for (long long i = 0; i < ncell; i++)
+        for (int c = 0; c < NET_C; c++) x[(size_t)i * NET_C + c] = (__bf16)(c < 4 ? feat[i * 4 + c] : 0.0f);
+    int rc = hip_ok(hipMemcpyAsync(t->act[0], x.data(), sizeof(__bf16) * x.size(), hipMemcpyHostToDevice, t->stream));
+    return rc ? rc : hip_ok(hipStreamSynchronize(t->stream));
+}
+}  // namespace
+
+extern "C" {
+
+int64_t lica_net_param_count(void)
+{
+    int64_t n = 0;
+    for (int l = 0; l < NET_LAYERS; l++) n += (int64_t)wsize(l) + NET_COUT[l];
+    return n;
+}
+
+int lica_net_create(uint64_t seed, lica_net **out)
+{
+    if (!out) return LICA_BAD_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return LICA_NO_DEVICE;
+    lica_net *t = new lica_net();
+    int rc = hip_ok(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking));
+    std::mt19937_64 rng(seed);
+    for (int l = 0; l < NET_LAYERS && !rc; l++) {
+        const size_t n = wsize(l);
+        rc = net_alloc((void **)&t->w[l], sizeof(float) * n);
+        rc = rc ? rc : net_alloc((void **)&t->gw[l], sizeof(float) * n);
+        rc = rc ? rc : net_alloc((void **)&t->mw[l], sizeof(float) * n);
+        rc = rc ? rc : net_alloc((void **)&t->vw[l], sizeof(float) * n);
+        rc = rc ? rc : net_alloc((void **)&t->b[l], sizeof(float) * NET_C);
+        rc = rc ? rc : net_alloc((void **)&t->gb[l], sizeof(float) * NET_C);
+        rc = rc ? rc : net_alloc((void **)&t->mb[l], sizeof(float) * NET_C);
+        rc = rc ? rc : net_alloc((void **)&t->vb[l], sizeof(float) * NET_C);
+        rc = rc ? rc : net_alloc((void **)&t->wb[l], sizeof(__bf16) * n);
+        rc = rc ? rc : net_alloc((void **)&t->wt[l], sizeof(__bf16) * (size_t)NET_TAPS * NET_C * NET_C);
+        if (rc) break;
+        // He initialisation over the real fan-in; padded inputs and outputs stay zero
+        std::vector<float> w(n, 0.0f);
+        const double sd = std::sqrt(2.0 / (double)(NET_TAPS * NET_CIN_REAL[l]));
+        std::normal_distribution<double> nd(0.0, sd);
+        const int creal = (l == NET_LAYERS - 1) ? 1 : NET_COUT[l];
+        for (int tp = 0; tp < NET_TAPS; tp++)
+            for (int co = 0; co < creal; co++)
+                for (int ci = 0; ci < NET_CIN_REAL[l]; ci++)
+                    w[((size_t)tp * NET_COUT[l] + co) * NET_C + ci] = (float)(nd(rng) * (l == NET_LAYERS - 1 ? 0.1 : 1.0));
+        rc = hip_ok(hipMemcpy(t->w[l], w.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    }
+    rc = rc ? rc : net_alloc((void **)&t->zero_bias, sizeof(float) * NET_C);
+    rc = rc ? rc : net_alloc((void **)&t->loss, sizeof(float));
+    if (rc) { lica_net_destroy(t); return rc; }
+    *out = t;
+    return LICA_OK;
+}
+
+void lica_net_destroy(lica_net *t)
+{
+    if (!t) return;
+    if (t->stream) (void)hipStreamSynchronize(t->stream);
+    for (int l = 0; l < NET_LAYERS; l++) {
+        (void)hipFree(t->w[l]); (void)hipFree(t->b[l]); (void)hipFree(t->gw[l]); (void)hipFree(t->gb[l]);
+        (void)hipFree(t->mw[l]); (void)hipFree(t->vw[l]); (void)hipFree(t->mb[l]); (void)hipFree(t->vb[l]);
+        (void)hipFree(t->wb[l]); (void)hipFree(t->wt[l]);
+    }
+    for (int l = 0; l <= NET_LAYERS; l++) (void)hipFree(t->act[l]);
+    for (int i = 0; i < 2; i++) (void)hipFree(t->dz[i]);
+    (void)hipFree(t->yf); (void)hipFree(t->target); (void)hipFree(t->loss); (void)hipFree(t->zero_bias);
+    (void)hipFree(t->seqs); (void)hipFree(t->seq_of_tile);
+    (void)hipFree(t->data_x); (void)hipFree(t->data_t); (void)hipFree(t->idx);
+    if (t->stream) (void)hipStreamDestroy(t->stream);
+    delete t;
+}
+
+// parameters, layer by layer: W[tap][co][ci] (padded to 32 in/out, 16 out for
+// the last layer), then the bias (cout values)
+int lica_net_get_params(lica_net *t, float *out)
+{
+    if (!t || !out) return LICA_BAD_ARG;
+    size_t o = 0;
+    int rc = 0;
+    for (int l = 0; l < NET_LAYERS && !rc; l++) {
+        rc = hip_ok(hipMemcpy(out + o, t->w[l], sizeof(float) * wsize(l), hipMemcpyDeviceToHost));
+        o += wsize(l);
+        rc = rc ? rc : hip_ok(hipMemcpy(out + o, t->b[l], sizeof(float) * NET_COUT[l], hipMemcpyDeviceToHost));
+        o += NET_COUT[l];
+    }
+    return rc;
+}
+
+int lica_net_set_params(lica_net *t, const float *in)
+{
+    if (!t || !in) return LICA_BAD_ARG;
+    size_t o = 0;
+    int rc = 0;
+    for (int l = 0; l < NET_LAYERS && !rc; l++) {
+        rc = hip_ok(hipMemcpy(t->w[l], in + o, sizeof(float) * wsize(l), hipMemcpyHostToDevice));
+        o += wsize(l);
+        rc = rc ? rc : hip_ok(hipMemcpy(t->b[l], in + o, sizeof(float) * NET_COUT[l], hipMemcpyHostToDevice));
+        o += NET_COUT[l];
+    }
+    return rc;
+}
+
+// y[cells] = network(features[cells][4]) for nseq sequences of lens[s] cells
+int lica_net_forward(lica_net *t, const float *feat, const int64_t *lens, int64_t nseq, float *y)
+{
+    if (!t || !feat || !lens || nseq <= 0 || !y) return LICA_BAD_ARG;
+    int rc = net_layout(t, lens, (int)nseq);
+    rc = rc ? rc : net_upload_features(t, feat, t->ncell);
+    rc = rc ? rc : net_forward(t);
+    rc = rc ? rc : hip_ok(hipMemcpyAsync(y, t->yf, sizeof(float) * (size_t)t->ncell, hipMemcpyDeviceToHost, t->stream));
+    return rc ? rc : hip_ok(hipStreamSynchronize(t->stream));
+}
+
+// loss = mean (y - target)^2 and its gradient (parameter layout of get_params)
+int lica_net_grad(lica_net *t, const float *feat, const float *target, const int64_t *lens, int64_t nseq,
+                  float *grad, double *loss)
+{
+    if (!t || !feat || !target || !lens || nseq <= 0 || !grad) return LICA_BAD_ARG;
+    int rc = net_layout(t, lens, (int)nseq);
+    rc = rc ? rc : net_upload_features(t, feat, t->ncell);
+    rc = rc ? rc : hip_ok(hipMemcpyAsync(t->target, target, sizeof(float) * (size_t)t->ncell, hipMemcpyHostToDevice, t->stream));
+    rc = rc ? rc : net_forward(t);
+    rc = rc ? rc : net_backward(t, loss);
+    size_t o = 0;
+    for (int l = 0; l < NET_LAYERS && !rc; l++) {
+        rc = hip_ok(hipMemcpy(grad + o, t->gw[l], sizeof(float) * wsize(l), hipMemcpyDeviceToHost));
+        o += wsize(l);
+        rc = rc ? rc : hip_ok(hipMemcpy(grad + o, t->gb[l], sizeof(float) * NET_COUT[l], hipMemcpyDeviceToHost));
+        o += NET_COUT[l];
+    }
+    return rc;
+}
+
+// training windows: n windows of len cells, features [n*len][4], target [n*len]
+int lica_net_set_data(lica_net *t, const float *feat, const float *target, int64_t n, int64_t len)
+{
+    if (!t || !feat || !target || n <= 0 || len <= 0) return LICA_BAD_ARG;
+    (void)hipFree(t->data_x); (void)hipFree(t->data_t);
+    t->data_x = nullptr; t->data_t = nullptr;
+    const size_t cells = (size_t)n * (size_t)len;
+    std::vector<__bf16> x(cells * NET_C);
+    for (size_t i = 0; i < cells; i++)
+        for (int c = 0; c < NET_C; c++) x[i * NET_C + c] = (__bf16)(c < 4 ? feat[i * 4 + c] : 0.0f);
+    int rc = net_alloc((void **)&t->data_x, sizeof(__bf16) * x.size());
+    rc = rc ? rc : net_alloc((void **)&t->data_t, sizeof(float) * cells);
+    rc = rc ? rc : hip_ok(hipMemcpy(t->data_x, x.data(), sizeof(__bf16) * x.size(), hipMemcpyHostToDevice));
+    rc = rc ? rc : hip_ok(hipMemcpy(t->data_t, target, sizeof(float) * cells, hipMemcpyHostToDevice));
+    if (rc) return rc;
+    t->data_n = n;
+    t->data_len = (int)len;
+    return LICA_OK;
+}
+
+// Adam steps on random minibatches of `batch` windows; loss_out gets the mean
+// minibatch loss of the last 10 % of the steps
+int lica_net_train(lica_net *t, int64_t iters, int64_t batch, double lr, uint64_t seed, double *loss_out)
+{
+    if (!t || !t->data_x || iters < 1 || batch < 1) return LICA_BAD_ARG;
+    const int len = t->data_len;
+    std::vector<int64_t> lens((size_t)batch, len);
+    int rc = net_layout(t, lens.data(), (int)batch);
+    if (rc) return rc;
+    std::mt19937_64 rng(seed);
+    std::uniform_int_distribution<int64_t> pick(0, t->data_n - 1);
+    std::vector<int> hidx((size_t)batch);
+    (void)hipFree(t->idx);
+    rc = net_alloc((void **)&t->idx, sizeof(int) * (size_t)batch);
+    if (rc) return rc;
+    double acc = 0.0;
+    int64_t nacc = 0;
+    const double b1 = 0.9, b2 = 0.999;
+    for (int64_t it = 0; it < iters && !rc; it++) {
+        for (int64_t s = 0; s < batch; s++) hidx[(size_t)s] = (int)pick(rng);
+        rc = hip_ok(hipMemcpyAsync(t->idx, hidx.data(), sizeof(int) * hidx.size(), hipMemcpyHostToDevice, t->stream));
+        const long long nc = batch * (long long)len;
+        hipLaunchKernelGGL(k_net_gather, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, t->stream, t->data_x,
+                           t->data_t, t->idx, t->act[0], t->target, len, (int)batch);
+        // the host index buffer is reused next iteration: wait for the copy
+        rc = rc ? rc : hip_ok(hipStreamSynchronize(t->stream));
+        double l = 0.0;
+        rc = rc ? rc : net_forward(t);
+        rc = rc ? rc : net_backward(t, (it >= iters - std::max<int64_t>(1, iters / 10)) ? &l : nullptr);
+        if (it >= iters - std::max<int64_t>(1, iters / 10)) { acc += l; nacc++; }
+        t->adam_t++;
+        const float c1 = (float)(1.0 - std::pow(b1, (double)t->adam_t)), c2 = (float)(1.0 - std::pow(b2, (double)t->adam_t));
+        for (int ly = 0; ly < NET_LAYERS && !rc; ly++) {
+            const int n = (int)wsize(ly);
+            hipLaunchKernelGGL(k_net_adam, dim3((n + 255) / 256), dim3(256), 0, t->stream, t->w[ly], t->mw[ly],
+                               t->vw[ly], t->gw[ly], n, (float)lr, (float)b1, (float)b2, 1e-8f, c1, c2);
+            hipLaunchKernelGGL(k_net_adam, dim3(1), dim3(64), 0, t->stream, t->b[ly], t->mb[ly], t->vb[ly],
+                               t->gb[ly], NET_COUT[ly], (float)lr, (float)b1, (float)b2, 1e-8f, c1, c2);
+            rc = hip_ok(hipGetLastError());
+        }
+    }
+    rc = rc ? rc : hip_ok(hipStreamSynchronize(t->stream));
+    if (loss_out) *loss_out = nacc ? acc / (double)nacc : 0.0;
+    return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------
+// warm start inside a batch: features of every case's POISSON system, the
+// network, guess = P + s_p * y (called by the batch driver before the
+// multigrid kernel of a substep)
+int lica_net_guess(lica_net *t, const NetCase *dcases, int ncase, const int64_t *lens, double *dscale, void *stream)
+{
+    int rc = 0;
+    if (t->layout_owner != (const void *)dcases) {
+        rc = net_layout(t, lens, ncase);
+        if (rc) return rc;
+        t->layout_owner = (const void *)dcases;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    hipStream_t saved = t->stream;
+    t->stream = s;
+    hipLaunchKernelGGL(k_net_features, dim3(ncase), dim3(256), 0, s, dcases, t->act[0], dscale);
+    rc = net_forward(t);
+    hipLaunchKernelGGL(k_net_guess, dim3(ncase), dim3(256), 0, s, dcases, t->yf, dscale);
+    t->stream = saved;
+    return rc ? rc : hip_ok(hipGetLastError());
+}
+
+// layout of a batch's cases (called when the batch adopts the network, so
+// that the per-substep guess launches no copies and can be graph-captured)
+int lica_net_prepare(lica_net *t, const int64_t *lens, int ncase, const NetCase *dcases)
+{
+    int rc = net_layout(t, lens, ncase);
+    if (!rc) t->layout_owner = (const void *)dcases;
+    return rc;
+}

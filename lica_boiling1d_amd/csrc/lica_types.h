@@ -37,6 +37,7 @@ struct InstParam {
     int mf_band, idtopt;
     long long ns_off, rf_off, nsi_off, rfi_off;
     long long mg_off;            // multigrid scratch (doubles), multigrid modes only
+    long long gs_off;            // guess / capture rows (n+2 per case)
     // properties (INPUT_SIMULATION 1d.f:434-480)
     double re, prm, denr, ja, tsat, cflmax, dt0, z0, tslope;
     double rei, denm, denp, den_diff, vism, visp, denscm, densc_diff;
@@ -78,7 +79,8 @@ struct BatchDev {
     int mg_maxc;         // V-cycle cap per solve
     double mg_rtol;      // relative residual target
     double *mg;          // multigrid scratch of the batch
-    const double *guess; // initial guesses of the learned mode (NS layout, field P)
+    const double *guess; // initial guesses of the learned mode (rows at gs_off)
+    double *cap;         // P before / after the last substep's solve, its RHS (3 x (n+2) at 3*gs_off)
 };
 
 #define LICA_MG_FP64_MODE 2

@@ -133,6 +133,14 @@ class Batch(_Handle):
     def use_graph(self, on=True):
         _check(self._lib.lica_batch_use_graph(self._h, 1 if on else 0), "use_graph")
 
+    def set_net(self, net):
+        """Network of the learned warm start (LICA_POISSON_LEARNED)."""
+        self._net = net
+        _check(self._lib.lica_batch_set_net(self._h, net.handle), "set_net")
+
+    def set_capture(self, on=True):
+        _check(self._lib.lica_batch_set_capture(self._h, 1 if on else 0), "set_capture")
+
     def set_timing(self, on=True):
         _check(self._lib.lica_batch_set_timing(self._h, 1 if on else 0), "set_timing")
 
