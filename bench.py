@@ -1,20 +1,27 @@
 """Benchmark of the boiling path on one MI355X.
 
-Workload (default): the single large case -- base grid N=8192 cells,
-level-set refinement r=4, 10 K superheat (Ja=0.009), a 0.2 mm film
-(z0=0.0607) with a seeded 1 % sinusoidal height perturbation -- advanced by
-the plain-FP64 GPU path, Poisson solved directly (residual at FP64 rounding,
-far below 1e-10).  A "step" is one full RK3 time step of the case
-(1d.f:196-295).  The value is instance-steps/s with the state resident in
-HBM.  ``--workload batch`` runs independent cases of uneven size in one
-launch per kernel instead.
+Workload (default): 4096 independent cases of uneven size in one launch per
+kernel -- base grid N ~ U{128..4096}, superheat U(2,20) K, film thickness
+U(0.05,0.5) mm, refinement r=2, drawn from SplitMix64 seed 20261202.  A
+"step" is one full RK3 time step of every case (1d.f:196-295); the value is
+completed instance-steps/s with the state resident in HBM, every Poisson
+solve taken by V-cycles to a normwise backward error of 1e-10.
+
+Two timed runs of the same batch from the same initial state:
+  mixed+learned  FP32 V-cycle corrections in FP64 defect correction, started
+                 from the warm-start network's guess (the value);
+  plain FP64     FP64 V-cycles from a zero initial guess (IOLDV=0).
+Before them the network is trained on the GPU on pairs the solver generates
+from a disjoint batch (seed 20261203); the training is not timed.
 
 The JSON line also carries
   roofline      the dominant kernel's algorithmic bytes per launch over its
                 mean launch time (HIP events between the launches on the
                 library's stream, over the timed steps), against 8 TB/s;
   cpu_baseline  the oracle (serial FP64 restatement, one core) on a bounded
-                number of steps of the same case.
+                sample of the same cases.
+``--workload large`` / ``deck`` and ``--mode`` select the single large case,
+the shipped deck and the other Poisson modes.
 """
 import argparse
 import json
@@ -26,7 +33,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-KERNELS = ["init", "dt", "cont_energy", "temp_grad", "momentum", "poisson", "ucalc", "transport"]
 
 
 def kernel_bytes(case, kernel, mode, vcycles=0.0):
@@ -68,37 +74,124 @@ def kernel_bytes(case, kernel, mode, vcycles=0.0):
     return 0
 
 
-def cpu_baseline(case, seconds=15.0):
-    """Oracle (serial FP64 restatement, one core) on the same case."""
+def cpu_baseline(cases, seconds=15.0):
+    """Oracle (serial FP64 restatement, one core) on a bounded sample: the
+    cases in order, a few steps each (every step of a single case), until
+    `seconds` have passed."""
     from oracle import Oracle
-    o = Oracle(case)
+    per_case = 3 if len(cases) > 1 else 100000
     t0 = time.perf_counter()
-    steps = 0
-    while True:
-        st = o.step(1)
-        steps += 1
-        if st != 0:
-            break
-        el = time.perf_counter() - t0
-        if el >= seconds or steps >= 100000:
+    steps, ncase, cells = 0, 0, 0
+    for c in cases:
+        o = Oracle(c)
+        ncase += 1
+        for _ in range(per_case):
+            st = o.step(1)
+            steps += 1
+            cells += int(c.n)
+            if st != 0 or time.perf_counter() - t0 >= seconds:
+                break
+        o.close()
+        if time.perf_counter() - t0 >= seconds:
             break
     el = time.perf_counter() - t0
-    o.close()
-    return {"value": steps / el, "unit": "instance-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} steps of the same case (N={case.n}, r={case.r}) on one host core, {el:.1f} s"}
+    if len(cases) == 1:
+        sample = f"{steps} steps of the same case (N={cases[0].n}, r={cases[0].r}) on one host core, {el:.1f} s"
+    else:
+        sample = (f"first {ncase} cases of the same batch, up to {per_case} steps each ({steps} instance-steps, "
+                  f"mean N={cells / max(steps, 1):.0f}) on one host core, {el:.1f} s, case set-up included")
+    return {"value": steps / el, "unit": "instance-steps/s", "cores": 1, "kind": "port", "sample": sample}
+
+
+def timed_run(L, cases, mode, args, net=None):
+    """One timed run: warmup steps, then args.steps steps with per-kernel HIP
+    events; only the steps the cases completed are counted (a case that
+    reaches one of the reference's STOP conditions keeps its status and is
+    left behind)."""
+    import numpy as np
+    b = L.Batch(cases)
+    if net is not None:
+        b.set_net(net)
+    b.set_poisson(mode, args.rtol, 200)
+    b.set_block(args.threads)
+    if args.warmup:
+        b.run_all(args.warmup)
+    b.sync()
+    s0 = [b.scalar_array(i) for i in range(len(cases))]
+    _, steps0 = b.progress()
+    b.set_timing(True)
+    b.sync()
+    t0 = time.perf_counter()
+    nbad = b.run_all(args.steps)
+    b.sync()
+    el = time.perf_counter() - t0
+    tim = b.timings()
+    st, steps1 = b.progress()
+    s1 = [b.scalar_array(i) for i in range(len(cases))]
+    b.close()
+    done = steps1 - steps0
+    ok = (st == 0)
+    res = {"el": el, "done": int(done.sum()), "value": float(done.sum()) / el, "timings": tim,
+           "cells": float(sum(int(c.n) * int(d) for c, d in zip(cases, done))) / el,
+           "ok": ok, "place": np.array([s[3] for s in s1]), "nbad": nbad, "status": st}
+    if mode >= L.POISSON_MG64:
+        ci, si, ri = (L.SCALARS.index(k) for k in ("POISSON_CYCLES", "POISSON_SOLVES", "POISSON_RESID"))
+        cyc = sum(a[ci] - z[ci] for a, z, g in zip(s1, s0, ok) if g)
+        sol = sum(a[si] - z[si] for a, z, g in zip(s1, s0, ok) if g)
+        res["vcycles_per_solve"] = cyc / max(sol, 1.0)
+        res["resid_max"] = max((a[ri] for a, g in zip(s1, ok) if g), default=0.0)
+    return res
+
+
+def train_net(L, args):
+    """Pairs from a disjoint batch, then Adam on the GPU (not timed)."""
+    from lica_boiling1d_amd.learned import Net, collect_pairs
+    t0 = time.perf_counter()
+    net = Net(seed=args.seed)
+    tcases = L.batched_cases(args.train_cases, seed=20261203)
+    nwin = collect_pairs(net, tcases, steps=args.train_steps, windows_per_case=args.train_windows, window=256)
+    t1 = time.perf_counter()
+    loss = net.train(args.train_iters, batch=64, lr=1e-3, seed=1)
+    t2 = time.perf_counter()
+    return net, {"pairs": nwin, "window_cells": 256, "cases": args.train_cases, "seed": 20261203,
+                 "steps": args.train_steps, "iters": args.train_iters, "minibatch": 64, "final_loss": loss,
+                 "collect_s": t1 - t0, "train_s": t2 - t1}
+
+
+def roofline(L, cases, mode, args, res):
+    """Dominant kernel of a timed run: algorithmic bytes per launch (the cases
+    still running) over its mean launch time."""
+    launches = {"dt": 1, "cont_energy": 3, "temp_grad": 3, "momentum": 3, "poisson": 3, "ucalc": 3, "transport": 3}
+    tim = dict(res["timings"])
+    if mode >= L.POISSON_MG64:
+        tim["poisson"] = tim.get("poisson_mg", 0.0)
+    per = {k: tim.get(k, 0.0) / (args.steps * nl) for k, nl in launches.items()}   # ms per launch
+    dom = max(per, key=lambda k: per[k])
+    vc = res.get("vcycles_per_solve", 0.0)
+    live = [c for c, g in zip(cases, res["ok"]) if g]
+    bytes_launch = sum(kernel_bytes(c, dom, mode, vc) for c in live)
+    ach = bytes_launch / (per[dom] * 1e-3) / 1e9
+    return ({"bound": "hbm", "kernel": "k_" + ("poisson_mg" if dom == "poisson" and mode >= L.POISSON_MG64 else dom),
+             "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+             "ms_per_launch": per[dom], "bytes_per_launch": bytes_launch}, per)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", choices=["large", "batch", "deck"], default="large")
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--workload", choices=["large", "batch", "deck"], default="batch")
     ap.add_argument("--batch", type=int, default=4096)
-    ap.add_argument("--mode", choices=["direct", "thomas", "mg64", "mixed"], default="direct")
+    ap.add_argument("--mode", choices=["learned", "direct", "thomas", "mg64", "mixed"], default="learned")
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--threads", type=int, default=256)
-    ap.add_argument("--graph", action="store_true")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--train-cases", type=int, default=4096)
+    ap.add_argument("--train-steps", type=int, default=25)
+    ap.add_argument("--train-windows", type=int, default=1)
+    ap.add_argument("--train-iters", type=int, default=3000)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the plain-FP64 comparison run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -106,7 +199,12 @@ def main():
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the cases of a batch are "
               "independent, so more GPUs would shard the batch by case with no exchange.", file=sys.stderr)
         sys.exit(2)
+    if args.steps is None:
+        args.steps = 40 if args.workload == "batch" else 200
+    if args.warmup is None:
+        args.warmup = 3 if args.workload == "batch" else 10
 
+    import numpy as np
     import lica_boiling1d_amd as L
     if args.workload == "large":
         cases = [L.single_large_case(seed=1)]
@@ -119,72 +217,60 @@ def main():
         cases = L.batched_cases(args.batch)
         wl = {"workload": "batched cases", "instances": args.batch, "n": "U{128..4096}", "r": 2,
               "superheat_K": "U(2,20)", "film_mm": "U(0.05,0.5)", "seed": 20261202}
-    mode = {"direct": L.POISSON_DIRECT, "thomas": L.POISSON_THOMAS, "mg64": L.POISSON_MG64,
-            "mixed": L.POISSON_MIXED}[args.mode]
+    mode = {"learned": L.POISSON_LEARNED, "direct": L.POISSON_DIRECT, "thomas": L.POISSON_THOMAS,
+            "mg64": L.POISSON_MG64, "mixed": L.POISSON_MIXED}[args.mode]
     wl["poisson"] = args.mode
-    print(f"bench: {wl['workload']} x{len(cases)}, mode {args.mode}, warmup {args.warmup}, steps {args.steps}",
-          file=sys.stderr, flush=True)
-    b = L.Batch(cases)
-    b.set_poisson(mode, args.rtol, 200)
-    b.set_block(args.threads)
-    if args.graph:
-        b.use_graph(True)
-    # a case that reaches one of the reference's STOP conditions keeps its
-    # status and is left behind; only steps the cases completed are counted
-    if args.warmup:
-        b.run_all(args.warmup)
-    b.sync()
-    _, steps0 = b.progress()
-    # timed region: per-kernel HIP events between the launches (no graph)
-    timed_graph = args.graph
-    if not timed_graph:
-        b.set_timing(True)
-    b.sync()
-    t0 = time.perf_counter()
-    nbad = b.run_all(args.steps)
-    b.sync()
-    el = time.perf_counter() - t0
-    tim = b.timings() if not timed_graph else None
-    st, steps1 = b.progress()
-    done = int((steps1 - steps0).sum())
-    if done == 0:
+    if mode >= L.POISSON_MG64:
+        wl["poisson_backward_error"] = args.rtol
+
+    def say(m):
+        print("bench: " + m, file=sys.stderr, flush=True)
+
+    say(f"{wl['workload']} x{len(cases)}, mode {args.mode}, warmup {args.warmup}, steps {args.steps}")
+    net, train = None, None
+    if mode == L.POISSON_LEARNED:
+        args.train_cases = min(args.train_cases, max(len(cases), 64))
+        say(f"training the warm-start network: {args.train_cases} cases x {args.train_steps} steps of pairs")
+        net, train = train_net(L, args)
+        say(f"trained: {train['pairs']} windows, loss {train['final_loss']:.3e}, "
+            f"{train['collect_s']:.1f} s + {train['train_s']:.1f} s")
+    say(f"timed run, mode {args.mode}")
+    res = timed_run(L, cases, mode, args, net)
+    if res["done"] == 0:
         print("bench: no case advanced", file=sys.stderr)
         sys.exit(1)
-    ms_step = el * 1000.0 / args.steps
-    value = done / el
+    say(f"{res['value']:.1f} instance-steps/s")
     out = {"metric": "instance-steps/s at Poisson resid 1e-10: mixed+learned vs plain FP64 (1 GPU)",
-           "value": value, "unit": "instance-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-           "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f64", "data": "synthetic", "config": wl}
-    out["cell_updates_per_s"] = float(sum(int(c.n) * int(d) for c, d in zip(cases, steps1 - steps0))) / el
-    if mode >= L.POISSON_MG64:
-        cyc = sum(b.scalars(i)["POISSON_CYCLES"] for i in range(min(len(cases), 64)))
-        sol = sum(b.scalars(i)["POISSON_SOLVES"] for i in range(min(len(cases), 64)))
-        out["vcycles_per_solve"] = cyc / max(sol, 1.0)
-        out["poisson_backward_error_max"] = max(b.scalars(i)["POISSON_RESID"] for i in range(min(len(cases), 64)))
-    if nbad:
-        import numpy as np
+           "value": res["value"], "unit": "instance-steps/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": res["el"] * 1000.0 / args.steps, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64+f32+bf16" if mode == L.POISSON_LEARNED else "f64+f32" if mode == L.POISSON_MIXED else "f64",
+           "data": "synthetic", "config": wl, "cell_updates_per_s": res["cells"]}
+    if "vcycles_per_solve" in res:
+        out["vcycles_per_solve"] = res["vcycles_per_solve"]
+        out["poisson_backward_error_max"] = res["resid_max"]
+    out["roofline"], out["kernel_ms_per_launch"] = roofline(L, cases, mode, args, res)
+    if res["nbad"]:
+        st = res["status"]
         codes, counts = np.unique(st[st != 0], return_counts=True)
         out["stopped_cases"] = {L.status_string(int(k)): int(v) for k, v in zip(codes, counts)}
-    if tim is not None:
-        launches = {"dt": 1, "cont_energy": 3, "temp_grad": 3, "momentum": 3, "poisson": 3, "ucalc": 3, "transport": 3}
-        if mode >= L.POISSON_MG64:
-            tim["poisson"] = tim.get("poisson_mg", 0.0)
-        per = {}
-        for k, nl in launches.items():
-            per[k] = tim.get(k, 0.0) / (args.steps * nl)   # ms per launch
-        dom = max(per, key=lambda k: per[k])
-        vc = out.get("vcycles_per_solve", 0.0)
-        bytes_launch = sum(kernel_bytes(c, dom, mode, vc) for c in cases)
-        ach = bytes_launch / (per[dom] * 1e-3) / 1e9
-        out["roofline"] = {"bound": "hbm", "kernel": "k_" + dom, "achieved": ach, "peak": HBM_PEAK_GBS,
-                           "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
-                           "ms_per_launch": per[dom], "bytes_per_launch": bytes_launch}
-        out["kernel_ms_per_launch"] = per
+    if train is not None:
+        out["training"] = train
+    if mode >= L.POISSON_MIXED and not args.no_fp64:
+        say("timed run, plain FP64 V-cycles from zero")
+        ref = timed_run(L, cases, L.POISSON_MG64, args)
+        both = res["ok"] & ref["ok"]
+        rel = np.abs(res["place"][both] - ref["place"][both]) / np.abs(ref["place"][both])
+        out["plain_fp64"] = {"value": ref["value"], "ms_per_step": ref["el"] * 1000.0 / args.steps,
+                             "vcycles_per_solve": ref.get("vcycles_per_solve"),
+                             "poisson_backward_error_max": ref.get("resid_max"),
+                             "kernel_ms_per_launch": roofline(L, cases, L.POISSON_MG64, args, ref)[1]}
+        out["speedup_vs_plain_fp64"] = res["value"] / ref["value"]
+        out["vcycles_saved_per_solve"] = ref.get("vcycles_per_solve", 0.0) - res.get("vcycles_per_solve", 0.0)
+        out["interface_rel_diff_vs_plain_fp64_max"] = float(rel.max()) if rel.size else None
     if not args.no_cpu:
-        big = max(cases, key=lambda c: int(c.n))
-        cb = cpu_baseline(big if len(cases) == 1 else cases[0], args.cpu_seconds)
-        out["cpu_baseline"] = cb
+        say("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(cases, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 

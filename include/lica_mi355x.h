@@ -100,6 +100,7 @@ enum {
     LICA_S_MASS_WARN = 11, LICA_S_EXTRAP_HALVINGS = 12, LICA_S_EXTRAP_ITERS = 13,
     LICA_S_REINIT_CALLS = 14, LICA_S_BANDGEN_CALLS = 15, LICA_S_MASSCOR_ITERS = 16,
     LICA_S_POISSON_CYCLES = 17, LICA_S_POISSON_SOLVES = 18, LICA_S_POISSON_RESID = 19,
+    LICA_S_DTCONST = 20, LICA_S_PSHIFT = 21,
     LICA_NSCAL = 24
 };
 
@@ -137,9 +138,10 @@ int lica_batch_write_fld(lica_batch *b, int64_t inst, const char *path, int64_t 
 /* ---- learned warm start of the Poisson solve -------------------------- */
 /* No counterpart in the reference: a 1-D convolutional network (4 layers, 21
  * taps, 32 channels, BF16 weights, FP32 accumulation on MFMA) maps per-cell
- * features of a POISSON system (g/max|g|, a/max|b|, c/max|b|, P/max|P| with
- * P the pressure before the solve) to (p - P)/max|P|; LICA_POISSON_LEARNED
- * starts the mixed-precision V-cycles from P + max|P| * network.           */
+ * features of a POISSON system (g/max|g|, a/max|b|, c/max|b|, Q/max|Q| with
+ * Q the last solution: P before the solve plus the p(1) that UCALC removed,
+ * 1d.f:3433-3470) to (p - Q)/max|Q|; LICA_POISSON_LEARNED starts the
+ * mixed-precision V-cycles from Q + max|Q| * network.                      */
 typedef struct lica_net lica_net;
 int64_t lica_net_param_count(void);
 int lica_net_create(uint64_t seed, lica_net **out);
@@ -152,13 +154,26 @@ int lica_net_forward(lica_net *net, const float *feat, const int64_t *lens, int6
 /* loss = mean (y - target)^2 and its gradient in the parameter layout */
 int lica_net_grad(lica_net *net, const float *feat, const float *target, const int64_t *lens, int64_t nseq,
                   float *grad, double *loss);
-/* training windows kept on the GPU; Adam steps on random minibatches */
+/* training windows kept on the GPU ([window][cell][4] features, [window][cell]
+ * targets); set_data replaces them, add_pairs appends windows_per_case
+ * windows of len cells from every healthy case of a batch run with capture
+ * on in a multigrid mode (features of its last POISSON system, target
+ * (p - Q)/max|Q|), cases shorter than len are skipped                     */
 int lica_net_set_data(lica_net *net, const float *feat, const float *target, int64_t n, int64_t len);
+int lica_net_add_pairs(lica_net *net, lica_batch *b, int64_t windows_per_case, int64_t len, uint64_t seed,
+                       int64_t *added);
+int lica_net_data_size(lica_net *net, int64_t *n, int64_t *len);
+int lica_net_get_data(lica_net *net, float *feat, float *target);
 int lica_net_train(lica_net *net, int64_t iters, int64_t batch, double lr, uint64_t seed, double *loss);
+/* No counterpart in the reference (tuning): cap on the LDS bytes a multigrid
+ * workgroup keeps its coarse levels in (-1: 60 KB for batches of >= 512
+ * cases, else the device limit); levels beyond the cap use global memory */
+int lica_batch_set_mg_lds(lica_batch *b, int64_t bytes);
 /* the network a batch uses in LICA_POISSON_LEARNED (not owned by the batch) */
 int lica_batch_set_net(lica_batch *b, lica_net *net);
-/* keep, for every case, P before and after the last substep's POISSON solve
- * and its right-hand side (fields LICA_F_PPREV, LICA_F_PSOL, LICA_F_PRHS)  */
+/* keep, for every case, the last solution Q before substep msub's POISSON
+ * solve (1..3; 0 switches capture off), the new solution and the
+ * right-hand side (fields LICA_F_PPREV, LICA_F_PSOL, LICA_F_PRHS)          */
 int lica_batch_set_capture(lica_batch *b, int64_t on);
 
 /* ---- single-case API mirroring the reference call sequence ----------- */

@@ -35,7 +35,7 @@ FIELDS = {"W": 0, "P": 1, "T": 2, "PSI_CN": 3, "PSI_ZN": 4, "PSIF": 5, "PHI": 6,
 SCALARS = ["TIME", "DT", "STEP", "PLACE", "DVMAX", "CFLM", "VOL_ORI", "VOL_IPHS",
            "QWALL", "STATUS", "SUBCYCLE_WARN", "MASS_WARN", "EXTRAP_HALVINGS",
            "EXTRAP_ITERS", "REINIT_CALLS", "BANDGEN_CALLS", "MASSCOR_ITERS",
-           "POISSON_CYCLES", "POISSON_SOLVES", "POISSON_RESID"]
+           "POISSON_CYCLES", "POISSON_SOLVES", "POISSON_RESID", "DTCONST", "PSHIFT"]
 NSCAL = 24
 
 
@@ -105,6 +105,7 @@ def _declare(lib):
         "lica_status_string": (ctypes.c_char_p, [ctypes.c_int]),
         "lica_batch_set_net": (ctypes.c_int, [V, V]),
         "lica_batch_set_capture": (ctypes.c_int, [V, c_i64]),
+        "lica_batch_set_mg_lds": (ctypes.c_int, [V, c_i64]),
         "lica_net_param_count": (c_i64, []),
         "lica_net_create": (ctypes.c_int, [ctypes.c_uint64, PV]),
         "lica_net_destroy": (None, [V]),
@@ -114,6 +115,9 @@ def _declare(lib):
         "lica_net_grad": (ctypes.c_int, [V, P_flt, P_flt, P_i64, c_i64, P_flt, P_dbl]),
         "lica_net_set_data": (ctypes.c_int, [V, P_flt, P_flt, c_i64, c_i64]),
         "lica_net_train": (ctypes.c_int, [V, c_i64, c_i64, c_dbl, ctypes.c_uint64, P_dbl]),
+        "lica_net_add_pairs": (ctypes.c_int, [V, V, c_i64, c_i64, ctypes.c_uint64, P_i64]),
+        "lica_net_data_size": (ctypes.c_int, [V, P_i64, P_i64]),
+        "lica_net_get_data": (ctypes.c_int, [V, P_flt, P_flt]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

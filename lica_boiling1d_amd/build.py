@@ -17,7 +17,7 @@ HEADERS = ["lica_types.h", "lica_par.h", "lica_launch.h", "lica_kern_common.h",
            "lica_kern_band.h", "lica_kern_bandreg.h", "lica_kern_tgrad.h", "lica_kern_flow.h", "lica_kern_mg.h"]
 
 HIP_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-pthread",
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall",

@@ -13,7 +13,9 @@
 #include "lica_launch.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <thread>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -28,7 +30,13 @@ typedef void *hipStream_t;
 // the warm-start network runs on MFMA only; the CPU logic-check build has none
 int lica_net_guess(lica_net *, const NetCase *, int, const int64_t *, double *, void *) { return LICA_UNSUPPORTED_DECK; }
 int lica_net_prepare(lica_net *, const int64_t *, int, const NetCase *) { return LICA_UNSUPPORTED_DECK; }
+int lica_net_append(lica_net *, const NetPairSrc *, int, int64_t, int64_t, uint64_t, int64_t *)
+{
+    return LICA_UNSUPPORTED_DECK;
+}
 extern "C" {
+int lica_net_data_size(lica_net *, int64_t *, int64_t *) { return LICA_UNSUPPORTED_DECK; }
+int lica_net_get_data(lica_net *, float *, float *) { return LICA_UNSUPPORTED_DECK; }
 int64_t lica_net_param_count(void) { return 0; }
 int lica_net_create(uint64_t, lica_net **out) { if (out) *out = nullptr; return LICA_UNSUPPORTED_DECK; }
 void lica_net_destroy(lica_net *) {}
@@ -113,28 +121,47 @@ struct CaseImage {
     int &RI(int f, int k) { return rfi[(size_t)f * p.lfi + k]; }
 };
 
-// GEOM_COUPLING's search of the NS centre at or below zz (1d.f:5156-5177)
+// first k in [lo, hi] where the monotone predicate hit(k) holds (hi + 1 if none)
+template <class F>
+int first_hit(int lo, int hi, F hit)
+{
+    while (lo <= hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (hit(mid)) hi = mid - 1;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+// GEOM_COUPLING's search of the NS centre at or below zz (1d.f:5156-5177).
+// The reference scans kk = 1, 2, ... and stops at the first kk with
+// |zz - ZP(kk)| <= 1e-10 or ZP(kk) > zz (the two end tests can only fire at
+// kk = 1); with ZP increasing that predicate stays true once true, so the
+// first such kk is found by bisection and the result is the scan's.
 int coump_vel_search(CaseImage &c, double zz)
 {
     const int n = c.p.n;
-    for (int kk = 1; kk <= n; kk++) {
-        if (fabs(zz - c.A(NS_ZP, kk)) <= 1.0e-10) return kk;
-        else if (c.A(NS_ZP, 1) > zz) return 1;
-        else if (c.A(NS_ZP, kk) > zz) return kk - 1;
-        else if (c.A(NS_ZP, n) < zz) return n;
-    }
-    return -1;
+    auto zp = [&](int kk) { return c.A(NS_ZP, kk); };
+    if (fabs(zz - zp(1)) <= 1.0e-10) return 1;
+    if (zp(1) > zz) return 1;
+    if (zp(n) < zz) return n;
+    auto hit = [&](int kk) { return fabs(zz - zp(kk)) <= 1.0e-10 || zp(kk) > zz; };
+    const int kk = first_hit(2, n, hit);
+    if (kk > n) return -1;
+    return (fabs(zz - zp(kk)) <= 1.0e-10) ? kk : kk - 1;
 }
 
-// refined node search of GEOM_COUPLING (1d.f:5190-5240): (kf-1,kf) on a node
+// refined node search of GEOM_COUPLING (1d.f:5190-5240): (kf-1,kf) on a node;
+// the scan's first kf with |ZF(kf) - zc| <= 1e-10 or ZF(kf) > zc, by bisection
 int node_search(CaseImage &c, double zc, int *k1, int *k2)
 {
-    for (int kf = 1; kf <= c.p.n3f; kf++) {
-        double zf = c.R(RF_ZF, kf);
-        if (fabs(zf - zc) <= 1.0e-10) { *k1 = kf - 1; *k2 = kf; return 0; }
-        else if (zf > zc) { *k1 = kf - 1; *k2 = kf - 1; return 0; }
-    }
-    return -1;
+    const int n3f = c.p.n3f;
+    auto hit = [&](int kf) { const double zf = c.R(RF_ZF, kf); return fabs(zf - zc) <= 1.0e-10 || zf > zc; };
+    const int kf = first_hit(1, n3f, hit);
+    if (kf > n3f) return -1;
+    if (fabs(c.R(RF_ZF, kf) - zc) <= 1.0e-10) { *k1 = kf - 1; *k2 = kf; }
+    else { *k1 = kf - 1; *k2 = kf - 1; }
+    return 0;
 }
 
 int build_case(const lica_case_desc &d, CaseImage &c)
@@ -338,7 +365,9 @@ struct lica_batch {
     double *d_guess = nullptr, *d_scale = nullptr, *d_cap = nullptr;
     NetCase *d_netcases = nullptr;
     std::vector<int64_t> lens;
-    bool capture = false;
+    int capture = 0;             // substep whose POISSON solve is captured (0: none)
+    bool cap_valid = false;      // the capture rows hold the last run's last substep
+    long long mg_lds_cap = -1;   // dynamic LDS cap of the multigrid kernel (-1: by batch size)
     std::vector<std::vector<double>> img_ns, img_rf;     // initial images (geometry)
     std::vector<std::vector<int>> img_nsi, img_rfi;
     int threads = 256;
@@ -353,6 +382,9 @@ struct lica_batch {
 #ifndef LICA_EMUL
     hipGraphExec_t graph = nullptr;
     int graph_threads = -1, graph_mode = -1;
+    int graph_capture = 0;
+    const lica_net *graph_net = nullptr;
+    long long graph_mg_lds = -1;
     std::vector<hipEvent_t> ev;
 #endif
     double timings[KL_COUNT] = {0};
@@ -360,6 +392,21 @@ struct lica_batch {
 };
 
 namespace {
+
+// dynamic LDS of the multigrid kernel: x, f of every level of the largest
+// case in the cycles' precision, capped (by default at 60 KB for batches of
+// many cases, so that several workgroups share a CU, else at the device's
+// per-workgroup limit); levels that do not fit stay in the global scratch
+long long mg_lds(const lica_batch *b)
+{
+    if (b->poisson_mode < LICA_POISSON_MG64) return 0;
+    int nmax = 0;
+    for (const InstParam &p : b->par) nmax = std::max(nmax, p.n);
+    const long long need = lica_mg_lds_need(nmax, b->poisson_mode == LICA_POISSON_MG64 ? 8 : 4);
+    long long cap = (b->mg_lds_cap >= 0) ? b->mg_lds_cap : (b->ninst >= 512 ? 65536 - 4096 : lica_mg_lds_max());
+    cap = std::min(cap, lica_mg_lds_max());
+    return std::min(need, cap) / 16 * 16;
+}
 
 BatchDev make_bd(const lica_batch *b, int msub)
 {
@@ -376,6 +423,7 @@ BatchDev make_bd(const lica_batch *b, int msub)
     bd.mg_maxc = b->max_cycles;
     bd.guess = b->d_guess;
     bd.cap = b->capture ? b->d_cap : nullptr;
+    bd.mg_lds = mg_lds(b);
     return bd;
 }
 
@@ -405,7 +453,7 @@ int launch(lica_batch *b, int which, int msub)
 int pre_poisson(lica_batch *b, int msub)
 {
     int rc = 0;
-    if (b->capture && msub == 3) rc = launch(b, KL_CAPTURE_PRE, msub);
+    if (b->capture == msub) rc = launch(b, KL_CAPTURE_PRE, msub);
     if (!rc && b->poisson_mode == LICA_POISSON_LEARNED) {
         if (!b->net) return LICA_BAD_ARG;
         if (lica_net_guess(b->net, b->d_netcases, (int)b->ninst, b->lens.data(), b->d_scale, (void *)b->stream))
@@ -425,7 +473,7 @@ int enqueue_step(lica_batch *b)
         if (!rc) rc = launch(b, KL_MOMENTUM, msub);
         if (!rc) rc = pre_poisson(b, msub);
         if (!rc) rc = launch(b, kp, msub);
-        if (!rc && b->capture && msub == 3) rc = launch(b, KL_CAPTURE_POST, msub);
+        if (!rc && b->capture == msub) rc = launch(b, KL_CAPTURE_POST, msub);
         if (!rc) rc = launch(b, KL_UCALC, msub);
         if (!rc) rc = launch(b, KL_TRANSPORT, msub);
     }
@@ -459,7 +507,7 @@ int enqueue_step_timed(lica_batch *b, std::vector<int> &kinds)
             }
             rc = launch(b, seq[j], msub);
             if (!rc) rc = rec(seq[j]);
-            if (!rc && seq[j] == kp && b->capture && msub == 3) rc = launch(b, KL_CAPTURE_POST, msub);
+            if (!rc && seq[j] == kp && b->capture == msub) rc = launch(b, KL_CAPTURE_POST, msub);
         }
     }
     return rc;
@@ -478,6 +526,9 @@ int build_graph(lica_batch *b)
     if (e != hipSuccess) { b->graph = nullptr; return LICA_HIP_ERROR; }
     b->graph_threads = b->threads;
     b->graph_mode = b->poisson_mode;
+    b->graph_capture = b->capture;
+    b->graph_net = b->net;
+    b->graph_mg_lds = mg_lds(b);
     return 0;
 }
 #endif
@@ -617,9 +668,23 @@ int lica_batch_create(int64_t ninst, const lica_case_desc *cases, lica_batch **o
     b->img_ns.resize((size_t)ninst); b->img_rf.resize((size_t)ninst);
     b->img_nsi.resize((size_t)ninst); b->img_rfi.resize((size_t)ninst);
     long long gs_total = 0;
+    // host images of the cases, built on all host threads
+    std::vector<CaseImage> imgs((size_t)ninst);
+    std::vector<int> brc((size_t)ninst, 0);
+    {
+        const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(ninst, std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+        std::atomic<int64_t> next(0);
+        auto work = [&]() {
+            for (int64_t i = next++; i < ninst; i = next++) brc[(size_t)i] = build_case(cases[i], imgs[(size_t)i]);
+        };
+        std::vector<std::thread> th;
+        for (int t = 1; t < nt; t++) th.emplace_back(work);
+        work();
+        for (auto &t : th) t.join();
+    }
     for (int64_t i = 0; i < ninst; i++) {
-        CaseImage c;
-        int rc = build_case(cases[i], c);
+        CaseImage &c = imgs[(size_t)i];
+        int rc = brc[(size_t)i];
         if (rc) { delete b; return rc; }
         c.p.ns_off = (long long)b->ns_total; b->ns_total += c.ns.size();
         c.p.rf_off = (long long)b->rf_total; b->rf_total += c.rf.size();
@@ -711,7 +776,9 @@ int lica_batch_run(lica_batch *b, int64_t nsteps)
             }
         }
     } else if (b->use_graph) {
-        if (!b->graph || b->graph_threads != b->threads || b->graph_mode != b->poisson_mode) rc = build_graph(b);
+        if (!b->graph || b->graph_threads != b->threads || b->graph_mode != b->poisson_mode ||
+            b->graph_capture != b->capture || b->graph_net != b->net || b->graph_mg_lds != mg_lds(b))
+            rc = build_graph(b);
         for (int64_t s = 0; s < nsteps && !rc; s++)
             if (hipGraphLaunch(b->graph, b->stream) != hipSuccess) rc = LICA_HIP_ERROR;
     } else {
@@ -721,6 +788,7 @@ int lica_batch_run(lica_batch *b, int64_t nsteps)
     if (!rc) rc = dsync(b->stream);
     b->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (rc) return rc;
+    if (b->capture && nsteps > 0) b->cap_valid = true;
     int64_t bad;
     return first_status(b, &bad);
 }
@@ -741,6 +809,14 @@ int lica_batch_set_poisson(lica_batch *b, int64_t mode, double rtol, int64_t max
     b->poisson_mode = (int)mode;
     b->rtol = rtol;
     b->max_cycles = (int)max_cycles;
+    return LICA_OK;
+}
+
+int lica_batch_set_mg_lds(lica_batch *b, int64_t bytes)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    b->mg_lds_cap = (bytes < 0) ? -1 : (long long)bytes;
     return LICA_OK;
 }
 
@@ -825,6 +901,7 @@ int lica_batch_get_scalars(lica_batch *b, int64_t inst, double *out)
     out[LICA_S_MASSCOR_ITERS] = s.n_masscor;
     out[LICA_S_POISSON_CYCLES] = s.poisson_cycles; out[LICA_S_POISSON_SOLVES] = s.poisson_solves;
     out[LICA_S_POISSON_RESID] = s.poisson_resid;
+    out[LICA_S_DTCONST] = s.dtconst; out[LICA_S_PSHIFT] = s.pshift;
     return LICA_OK;
 }
 
@@ -943,6 +1020,7 @@ int lica_batch_set_net(lica_batch *b, lica_net *net)
         c.a = base + (size_t)NS_PA * p.ln; c.b = base + (size_t)NS_PB * p.ln;
         c.c = base + (size_t)NS_PC * p.ln; c.g = base + (size_t)NS_PG * p.ln;
         c.p = base + (size_t)NS_P * p.ln;
+        c.pshift = &b->d_st[i].pshift;
         c.guess = b->d_guess + p.gs_off;
         c.off = off; c.n = p.n;
         off += p.n;
@@ -959,8 +1037,38 @@ int lica_batch_set_capture(lica_batch *b, int64_t on)
 {
     int rc = check_batch(b);
     if (rc) return rc;
-    b->capture = on != 0;
+    if (on < 0 || on > 3) return LICA_BAD_ARG;
+    b->capture = (int)on;
+    b->cap_valid = false;
     return LICA_OK;
+}
+
+// training windows from the captured systems of every healthy case; needs a
+// multigrid mode (the solve leaves the matrix rows in place) and a run with
+// capture on since capture was switched on
+int lica_net_add_pairs(lica_net *net, lica_batch *b, int64_t windows_per_case, int64_t len, uint64_t seed,
+                       int64_t *added)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (!net || windows_per_case < 1 || len < 1) return LICA_BAD_ARG;
+    if (!b->capture || !b->cap_valid || b->poisson_mode < LICA_POISSON_MG64) return LICA_BAD_ARG;
+    std::vector<int64_t> st((size_t)b->ninst);
+    rc = lica_batch_get_progress(b, st.data(), nullptr);
+    if (rc) return rc;
+    std::vector<NetPairSrc> src;
+    for (int64_t i = 0; i < b->ninst; i++) {
+        if (st[(size_t)i] != 0) continue;
+        const InstParam &p = b->par[(size_t)i];
+        const double *base = b->d_ns + p.ns_off + 2;
+        const double *cap = b->d_cap + 3 * p.gs_off;
+        NetPairSrc s;
+        s.a = base + (size_t)NS_PA * p.ln; s.b = base + (size_t)NS_PB * p.ln; s.c = base + (size_t)NS_PC * p.ln;
+        s.p0 = cap; s.p1 = cap + (p.n + 2); s.g = cap + 2 * (p.n + 2);
+        s.n = p.n;
+        src.push_back(s);
+    }
+    return lica_net_append(net, src.data(), (int)src.size(), windows_per_case, len, seed, added);
 }
 
 // ---- single-case API -------------------------------------------------

@@ -626,7 +626,6 @@ struct BlkSh {
     TriSh tri;     // points at the kernel's dynamic LDS (tri_ws_rows(blockDim) x 4 doubles)
     RefWin win;    // refined window of the box averages
     unsigned char jf[JFMAX];   // jump flags of PSI_CN (n <= JFMAX - 1)
-    double mgws[80];           // coarsest multigrid level
 };
 
 // DETERMINE_DT (1d.f:683-817) at the start of a step, with the per-step

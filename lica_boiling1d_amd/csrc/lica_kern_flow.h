@@ -153,7 +153,7 @@ DEV void body_poisson_direct(const BatchDev &bd, int inst, BlkSh &sh)
 }
 
 // POISSON by multigrid (the same system, solved to max|g-Ap| <= rtol max|g|)
-DEV void body_poisson_mg(const BatchDev &bd, int inst, BlkSh &sh)
+DEV void body_poisson_mg(const BatchDev &bd, int inst, MgSh &sh)
 {
     InstState &S = bd.st[inst];
     if (S.status) return;
@@ -164,8 +164,7 @@ DEV void body_poisson_mg(const BatchDev &bd, int inst, BlkSh &sh)
     const double *PA = cp.A(NS_PA), *PB = cp.A(NS_PB), *PCc = cp.A(NS_PC), *PG = cp.A(NS_PG);
     const double *x0 = (bd.pmode == LICA_MG_LEARNED_MODE && bd.guess) ? bd.guess + P.gs_off : nullptr;
     double res = 0.0;
-    int cyc = mg_solve(PA, PB, PCc, PG, PP, n, bd.mg + P.mg_off, bd.pmode, bd.mg_rtol, bd.mg_maxc, x0, sh.mgws,
-                       sh.red, &res);
+    int cyc = mg_solve(PA, PB, PCc, PG, PP, n, bd.mg + P.mg_off, bd.pmode, bd.mg_rtol, bd.mg_maxc, x0, sh, &res);
     SERIAL {
         PP[0] = PP[1];
         PP[n3] = PP[n];
@@ -178,7 +177,8 @@ DEV void body_poisson_mg(const BatchDev &bd, int inst, BlkSh &sh)
 }
 
 // capture rows of every case (3 x (n+2) at 3*gs_off): before the POISSON
-// solve (which 0) P and the right-hand side, after it (which 1) P
+// solve (which 0) the last solution P + pshift and the right-hand side,
+// after it (which 1) the new solution
 DEV void body_capture(const BatchDev &bd, int inst, int which)
 {
     InstState &S = bd.st[inst];
@@ -189,8 +189,9 @@ DEV void body_capture(const BatchDev &bd, int inst, int which)
     double *base = bd.cap + 3 * P.gs_off;
     double *dst = base + (long long)which * (P.n + 2);
     double *rhs = base + 2LL * (P.n + 2);
+    const double sh0 = (which == 0) ? S.pshift : 0.0;
     PFOR(k, 1, P.n + 1) {
-        dst[k] = PP[k];
+        dst[k] = PP[k] + sh0;
         if (which == 0) rhs[k] = PG[k];
     }
     PFOR_END
@@ -219,6 +220,7 @@ DEV void body_ucalc(const BatchDev &bd, int inst, BlkSh &sh)
         W[1] = 0.0;
         W[n3] = W[n];
         sh.v[0] = PP[1];
+        S.pshift = PP[1];
     }
     SERIAL_END
     const double phiref = sh.v[0];

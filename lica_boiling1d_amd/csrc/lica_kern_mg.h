@@ -2,14 +2,21 @@
 // workgroup per case: V(2,2)-cycles with red-black Gauss-Seidel smoothing,
 // Galerkin coarse operators P^T A P (linear interpolation, coarse points at
 // the even fine rows, zero beyond the top row as in the reference's implicit
-// p = 0 above the last face) and an exact elimination on the coarsest level.
-//   MG64:    FP64 cycles from a zero initial guess (IOLDV=0, lica.in:8) until
-//            the backward error max|g - A p| / (||A|| max|p| + max|g|) <= rtol;
-//   MIXED:   FP64 defect correction: the residual and the update are FP64, the
-//            correction comes from one FP32 V-cycle (FP32 matrices, smoother,
-//            transfers; the coarsest solve in FP64);
-//   LEARNED: MIXED from the initial guess of the convolutional network.
-// The solution replaces P(1:N) as the direct solve does.
+// p = 0 above the last face) and an exact FP64 elimination on the coarsest
+// level.  Every mode is FP64 defect correction x += V(g - A x): the residual,
+// the update and the stopping test are FP64, the V-cycle runs on the residual
+// equation from a zero start in
+//   MG64:    FP64 (IOLDV=0, lica.in:8: the iteration starts from x = 0),
+//   MIXED:   FP32 (matrices, smoother, transfers; the coarsest solve in FP64),
+//   LEARNED: FP32, the iteration started from the network's guess,
+// until the normwise backward error max|g - A x| / (||A|| max|x| + max|g|)
+// reaches rtol.  The solution replaces P(1:N) as the direct solve does.
+//
+// Memory: the iterate, the fine matrix and the matrices of every level sit in
+// a per-case global scratch (read-only during the cycles); x and f of the
+// levels live in LDS from the coarsest level up as far as the launch's
+// dynamic LDS allows (the rest in the scratch), so that the passes of a cycle
+// -- each one barrier -- do not wait on global stores.
 #pragma once
 #include "lica_kern_common.h"
 
@@ -30,42 +37,79 @@ HOSTDEV int mg_levels(int n)
     while (m > MG_COARSE && l < MG_MAXLEV) { m = (m + 1) / 2; l++; }
     return l;
 }
-// doubles of the per-case scratch: FP64 a,b,c,x,f,r per level, then the
-// FP32 a,b,c,x,f,r per level (packed two to a double)
+// rows of levels 0..l-1
+HOSTDEV long long mg_rows_before(int n, int l)
+{
+    long long s = 0;
+    int m = n;
+    for (int i = 0; i < l; i++) { s += m; m = (m + 1) / 2; }
+    return s;
+}
+HOSTDEV long long mg_rows_total(int n) { return mg_rows_before(n, mg_levels(n)); }
+
+// Per-case scratch, in doubles:
+//   7n         fine level FP64: a, b, c, 1/b, g, iterate, its double buffer
+//   4(tot-n)   FP64 a, b, c, 1/b of levels 1..L-1
+//   2 tot      FP32 a, b, c, 1/b of every level
+//   2 tot      x, f of the levels kept outside LDS (room for FP64)
 HOSTDEV long long mg_scratch_doubles(int n)
 {
-    long long tot = 0;
-    const int L = mg_levels(n);
-    for (int l = 0; l < L; l++) tot += 6LL * mg_rows(n, l) + 3LL * mg_rows(n, l) + 8;
-    return tot;
+    const long long tot = mg_rows_total(n);
+    return 7LL * n + 4LL * (tot - n) + 2LL * tot + 2LL * tot + 8;
 }
+// LDS that holds x, f of every level in elements of tsize bytes
+HOSTDEV long long mg_lds_bytes(int n, int tsize) { return 2LL * tsize * mg_rows_total(n); }
+
+struct MgSh {
+    double red[32];                                        // two-value reductions
+    double cg[MG_COARSE], cib[MG_COARSE], ca[MG_COARSE];   // coarsest-level elimination factors
+    double cy[MG_COARSE];                                  // its forward sweep
+    void *lds;                                             // x, f of the coarse levels
+    long long lds_bytes;
+};
 
 template <class T>
-struct MgLevel {
-    T *a, *b, *c, *x, *f, *r;
+struct MgLev {
+    const T *a, *b, *c, *ib;
+    T *x, *f;
+    int n;
+};
+struct MgFine {
+    double *a, *b, *c, *ib, *g, *x0, *x1;
     int n;
 };
 
-// level l of the FP64 (T=double) or FP32 (T=float) hierarchy
-template <class T>
-DEV MgLevel<T> mg_level(double *base, int n, int l)
+DEV MgFine mg_fine(double *scr, int n)
 {
-    const int L = mg_levels(n);
-    long long off64 = 0, off32 = 0;
-    for (int i = 0; i < L; i++) off32 += 6LL * mg_rows(n, i);
-    long long o64 = 0, o32 = 0;
-    for (int i = 0; i < l; i++) { o64 += 6LL * mg_rows(n, i); o32 += 6LL * mg_rows(n, i); }
+    MgFine F;
+    F.n = n;
+    F.a = scr; F.b = scr + n; F.c = scr + 2 * n; F.ib = scr + 3 * n;
+    F.g = scr + 4 * n; F.x0 = scr + 5 * n; F.x1 = scr + 6 * n;
+    return F;
+}
+
+// matrices of level l (FP64 or FP32); x, f in LDS when levels l..L-1 fit
+template <class T>
+DEV MgLev<T> mg_lev(double *scr, int n, int l, const MgSh &sh)
+{
     const int m = mg_rows(n, l);
-    MgLevel<T> v;
+    const long long pre = mg_rows_before(n, l), tot = mg_rows_total(n);
+    MgLev<T> v;
     v.n = m;
     if constexpr (sizeof(T) == 8) {
-        T *p = (T *)(base + o64);
-        v.a = p; v.b = p + m; v.c = p + 2 * m; v.x = p + 3 * m; v.f = p + 4 * m; v.r = p + 5 * m;
+        const double *q = (l == 0) ? scr : scr + 7LL * n + 4 * (pre - n);
+        const long long s = (l == 0) ? n : m;
+        v.a = q; v.b = q + s; v.c = q + 2 * s; v.ib = q + 3 * s;
     } else {
-        T *p = (T *)(base + off32) + o32;
-        v.a = p; v.b = p + m; v.c = p + 2 * m; v.x = p + 3 * m; v.f = p + 4 * m; v.r = p + 5 * m;
+        const float *q = (const float *)(scr + 7LL * n + 4 * (tot - n)) + 4 * pre;
+        v.a = q; v.b = q + m; v.c = q + 2 * m; v.ib = q + 3 * m;
     }
-    (void)off64;
+    const long long suffix = tot - pre;   // rows of levels l..L-1
+    T *q;
+    if ((long long)(2 * sizeof(T)) * suffix <= sh.lds_bytes) q = (T *)sh.lds + 2 * (suffix - m);
+    else q = (T *)(scr + 7LL * n + 4 * (tot - n) + 2 * tot) + 2 * pre;
+    v.x = q;
+    v.f = q + m;
     return v;
 }
 
@@ -79,42 +123,51 @@ DEV double mg_pw(int k, int J, int nf, int nc)
     return 0.0;
 }
 
-// Galerkin coarse operator of level l+1 from level l (FP64)
-DEV void mg_coarsen(const MgLevel<double> &F, const MgLevel<double> &C)
+// Galerkin operator of level l+1 from level l (FP64), with 1/b, and the FP32
+// copy of level l+1 when the cycles run in FP32
+DEV void mg_coarsen(const MgLev<double> &F, const MgLev<double> &C, float *a32, bool f32)
 {
     const int nf = F.n, nc = C.n;
+    double *ca = (double *)C.a, *cb = (double *)C.b, *cc = (double *)C.c, *cib = (double *)C.ib;
     PFOR(I, 0, nc) {
         double acc[3] = {0.0, 0.0, 0.0};
         for (int t = 0; t < 3; t++) {
             const int i = 2 * I - 1 + t;
             const double wi = mg_pw(i, I, nf, nc);
             if (wi == 0.0) continue;
-            const double ai = (i == 0) ? 0.0 : F.a[i], ci = (i == nf - 1) ? 0.0 : F.c[i], bi = F.b[i];
+            const double ai = F.a[i], ci = F.c[i], bi = F.b[i];
             for (int d = -1; d <= 1; d++) {
                 const int J = I + d;
                 double apj = ai * mg_pw(i - 1, J, nf, nc) + bi * mg_pw(i, J, nf, nc) + ci * mg_pw(i + 1, J, nf, nc);
                 acc[d + 1] += wi * apj;
             }
         }
-        C.a[I] = (I == 0) ? 0.0 : acc[0];
-        C.b[I] = acc[1];
-        C.c[I] = (I == nc - 1) ? 0.0 : acc[2];
+        const double va = (I == 0) ? 0.0 : acc[0], vc = (I == nc - 1) ? 0.0 : acc[2], vb = acc[1];
+        ca[I] = va; cb[I] = vb; cc[I] = vc; cib[I] = 1.0 / vb;
+        if (f32) {
+            a32[I] = (float)va; a32[nc + I] = (float)vb; a32[2 * nc + I] = (float)vc; a32[3 * nc + I] = 1.0f / (float)vb;
+        }
     }
     PFOR_END
 }
 
+// red-black Gauss-Seidel; zero_start: x is taken as zero before the first
+// half-sweep (its stale values are never read)
 template <class T>
-DEV void mg_smooth(const MgLevel<T> &L, int sweeps)
+DEV void mg_smooth(const MgLev<T> &L, int sweeps, bool zero_start)
 {
     const int n = L.n;
     for (int s = 0; s < sweeps; s++) {
         for (int color = 0; color < 2; color++) {
+            const bool z = zero_start && s == 0 && color == 0;
             PFOR(h, 0, (n - color + 1) / 2) {
                 const int i = 2 * h + color;
                 T v = L.f[i];
-                if (i > 0) v = v - L.a[i] * L.x[i - 1];
-                if (i < n - 1) v = v - L.c[i] * L.x[i + 1];
-                L.x[i] = v / L.b[i];
+                if (!z) {
+                    if (i > 0) v = v - L.a[i] * L.x[i - 1];
+                    if (i < n - 1) v = v - L.c[i] * L.x[i + 1];
+                }
+                L.x[i] = v * L.ib[i];
             }
             PFOR_END
         }
@@ -122,36 +175,31 @@ DEV void mg_smooth(const MgLevel<T> &L, int sweeps)
 }
 
 template <class T>
-DEV void mg_residual(const MgLevel<T> &L)
+DEV T mg_res_at(const MgLev<T> &F, int i)
 {
-    const int n = L.n;
-    PFOR(i, 0, n) {
-        T v = L.f[i] - L.b[i] * L.x[i];
-        if (i > 0) v = v - L.a[i] * L.x[i - 1];
-        if (i < n - 1) v = v - L.c[i] * L.x[i + 1];
-        L.r[i] = v;
-    }
-    PFOR_END
+    T v = F.f[i] - F.b[i] * F.x[i];
+    if (i > 0) v = v - F.a[i] * F.x[i - 1];
+    if (i < F.n - 1) v = v - F.c[i] * F.x[i + 1];
+    return v;
 }
 
-// f_c = P^T r_f, x_c = 0
+// f_c = P^T (f - A x) with the residual evaluated on the fly
 template <class T>
-DEV void mg_restrict(const MgLevel<T> &F, const MgLevel<T> &C)
+DEV void mg_restrict(const MgLev<T> &F, const MgLev<T> &C)
 {
     PFOR(J, 0, C.n) {
         const int i = 2 * J;
-        T v = F.r[i];
-        if (i - 1 >= 0) v = v + (T)0.5 * F.r[i - 1];
-        if (i + 1 < F.n) v = v + (T)0.5 * F.r[i + 1];
+        T v = mg_res_at(F, i);
+        if (i - 1 >= 0) v = v + (T)0.5 * mg_res_at(F, i - 1);
+        if (i + 1 < F.n) v = v + (T)0.5 * mg_res_at(F, i + 1);
         C.f[J] = v;
-        C.x[J] = (T)0.0;
     }
     PFOR_END
 }
 
 // x_f += P x_c
 template <class T>
-DEV void mg_prolong(const MgLevel<T> &F, const MgLevel<T> &C)
+DEV void mg_prolong(const MgLev<T> &F, const MgLev<T> &C)
 {
     PFOR(i, 0, F.n) {
         const int J = i / 2;
@@ -163,108 +211,141 @@ DEV void mg_prolong(const MgLevel<T> &F, const MgLevel<T> &C)
     PFOR_END
 }
 
-// exact solve on the coarsest level (FP64 elimination by one thread)
-template <class T>
-DEV void mg_coarsest(const MgLevel<T> &L, double *ws)
+// elimination factors of the coarsest level (FP64, once per solve)
+DEV void mg_coarsest_factor(const MgLev<double> &L, MgSh &sh)
 {
     SERIAL {
         const int n = L.n;
-        double *gam = ws;
-        double bet = 1.0 / (double)L.b[0];
-        double prev = (double)L.f[0] * bet;
-        ws[n] = prev;
+        double ib = 1.0 / L.b[0];
+        sh.cib[0] = ib;
+        sh.ca[0] = 0.0;
+        sh.cg[0] = 0.0;
         for (int j = 1; j < n; j++) {
-            gam[j] = (double)L.c[j - 1] * bet;
-            bet = 1.0 / ((double)L.b[j] - (double)L.a[j] * gam[j]);
-            prev = ((double)L.f[j] - (double)L.a[j] * prev) * bet;
-            ws[n + j] = prev;
+            const double g = L.c[j - 1] * ib;
+            ib = 1.0 / (L.b[j] - L.a[j] * g);
+            sh.cg[j] = g;
+            sh.cib[j] = ib;
+            sh.ca[j] = L.a[j];
         }
-        for (int j = n - 2; j >= 0; j--) ws[n + j] = ws[n + j] - gam[j + 1] * ws[n + j + 1];
-        for (int j = 0; j < n; j++) L.x[j] = (T)ws[n + j];
     }
     SERIAL_END
 }
 
+// exact solve on the coarsest level in FP64 (one thread)
 template <class T>
-DEV void mg_vcycle(double *base, int n, double *ws)
+DEV void mg_coarsest(const MgLev<T> &L, MgSh &sh)
+{
+    SERIAL {
+        const int n = L.n;
+        double y = (double)L.f[0] * sh.cib[0];
+        sh.cy[0] = y;
+        for (int j = 1; j < n; j++) {
+            y = ((double)L.f[j] - sh.ca[j] * y) * sh.cib[j];
+            sh.cy[j] = y;
+        }
+        double x = sh.cy[n - 1];
+        L.x[n - 1] = (T)x;
+        for (int j = n - 2; j >= 0; j--) {
+            x = sh.cy[j] - sh.cg[j + 1] * x;
+            L.x[j] = (T)x;
+        }
+    }
+    SERIAL_END
+}
+
+// one V-cycle on the residual equation held in level 0's f; result in its x
+template <class T>
+DEV void mg_vcycle(double *scr, int n, MgSh &sh)
 {
     const int L = mg_levels(n);
     for (int l = 0; l + 1 < L; l++) {
-        MgLevel<T> F = mg_level<T>(base, n, l), C = mg_level<T>(base, n, l + 1);
-        mg_smooth<T>(F, MG_NU);
-        mg_residual<T>(F);
+        MgLev<T> F = mg_lev<T>(scr, n, l, sh), C = mg_lev<T>(scr, n, l + 1, sh);
+        mg_smooth<T>(F, MG_NU, true);
         mg_restrict<T>(F, C);
     }
-    mg_coarsest<T>(mg_level<T>(base, n, L - 1), ws);
+    mg_coarsest<T>(mg_lev<T>(scr, n, L - 1, sh), sh);
     for (int l = L - 2; l >= 0; l--) {
-        MgLevel<T> F = mg_level<T>(base, n, l), C = mg_level<T>(base, n, l + 1);
+        MgLev<T> F = mg_lev<T>(scr, n, l, sh), C = mg_lev<T>(scr, n, l + 1, sh);
         mg_prolong<T>(F, C);
-        mg_smooth<T>(F, MG_NU);
+        mg_smooth<T>(F, MG_NU, false);
     }
 }
 
-// normwise backward error max|g - A p| / (||A|| max|p| + max|g|) on the FP64
-// fine level (residual recomputed).  max|g - A p| / max|g| alone has a
-// rounding floor of about eps ||A|| max|p| / max|g|, which for the POISSON rows
-// of a substep reaches 1e-10 (the reference's direct solve lands there too),
-// so the stopping test is on the backward error.
-DEV double mg_relres(const MgLevel<double> &F, double fmax, double anorm, double *red)
+// xn = x + e (e = Z.x, absent on the first pass) into the other buffer,
+// r = g - A xn in FP64, Z.f = r in T; *rmax = max|r|, *xmax = max|xn|
+template <class T>
+DEV void mg_update(const MgFine &F, const MgLev<T> &Z, bool first, const double *xa, double *xb, MgSh &sh,
+                   double *rmax, double *xmax)
 {
-    mg_residual<double>(F);
-    double rmax = red_max(0, F.n, 0.0, [&](int i) { return fabs(F.r[i]); }, red);
-    double xmax = red_max(0, F.n, 0.0, [&](int i) { return fabs(F.x[i]); }, red);
+    const int n = F.n;
+    auto xn = [&](int k) -> double { return first ? xa[k] : xa[k] + (double)Z.x[k]; };
+    red_max2(0, n, [&](int i, double &ra, double &xa_) {
+        const double xi = xn(i);
+        double v = F.g[i] - F.b[i] * xi;
+        if (i > 0) v = v - F.a[i] * xn(i - 1);
+        if (i < n - 1) v = v - F.c[i] * xn(i + 1);
+        xb[i] = xi;
+        Z.f[i] = (T)v;
+        ra = fabs(v);
+        xa_ = fabs(xi);
+    }, sh.red, rmax, xmax);
+}
+
+template <class T>
+DEV int mg_iterate(const MgFine &F, double *scr, int n, double rtol, int maxc, double fmax, double anorm, MgSh &sh,
+                   double *p, double *res_out)
+{
+    MgLev<T> Z = mg_lev<T>(scr, n, 0, sh);
+    double *cur = F.x0, *oth = F.x1;
+    double rmax, xmax;
+    mg_update<T>(F, Z, true, cur, oth, sh, &rmax, &xmax);
+    { double *t = cur; cur = oth; oth = t; }
     double den = anorm * xmax + fmax;
-    return (den > 0.0) ? rmax / den : rmax;
-}
-
-// Solve rows 1..n of a, b, c, g (a(1), c(n) ignored) into p(1..n).
-// mode 2: FP64 cycles, 3/4: FP32 corrections in FP64 defect correction
-// (x0 holds the initial guess for mode 4).  Returns cycles; *res the final
-// relative residual; cycles = -1 when maxc cycles did not reach rtol.
-DEV int mg_solve(const double *a, const double *b, const double *c, const double *g, double *p, int n,
-                 double *base, int mode, double rtol, int maxc, const double *x0, double *ws, double *red,
-                 double *res_out)
-{
-    const int L = mg_levels(n);
-    MgLevel<double> F = mg_level<double>(base, n, 0);
-    PFOR(i, 0, n) {
-        F.a[i] = (i == 0) ? 0.0 : a[i + 1];
-        F.b[i] = b[i + 1];
-        F.c[i] = (i == n - 1) ? 0.0 : c[i + 1];
-        F.f[i] = g[i + 1];
-        F.x[i] = (mode == LICA_MG_LEARNED_MODE && x0) ? x0[i + 1] : 0.0;
-    }
-    PFOR_END
-    for (int l = 0; l + 1 < L; l++) mg_coarsen(mg_level<double>(base, n, l), mg_level<double>(base, n, l + 1));
-    if (mode != LICA_MG_FP64_MODE) {
-        for (int l = 0; l < L; l++) {
-            MgLevel<double> D = mg_level<double>(base, n, l);
-            MgLevel<float> S = mg_level<float>(base, n, l);
-            PFOR(i, 0, D.n) { S.a[i] = (float)D.a[i]; S.b[i] = (float)D.b[i]; S.c[i] = (float)D.c[i]; }
-            PFOR_END
-        }
-    }
-    const double fmax = red_max(0, n, 0.0, [&](int i) { return fabs(F.f[i]); }, red);
-    const double anorm = red_max(0, n, 0.0, [&](int i) { return fabs(F.a[i]) + fabs(F.b[i]) + fabs(F.c[i]); }, red);
-    double res = mg_relres(F, fmax, anorm, red);
+    double res = (den > 0.0) ? rmax / den : rmax;
     int cyc = 0;
     while (res > rtol && cyc < maxc) {
-        if (mode == LICA_MG_FP64_MODE) {
-            mg_vcycle<double>(base, n, ws);
-        } else {
-            // one FP32 correction of the FP64 residual held in F.r
-            MgLevel<float> S = mg_level<float>(base, n, 0);
-            PFOR(i, 0, n) { S.f[i] = (float)F.r[i]; S.x[i] = 0.0f; }
-            PFOR_END
-            mg_vcycle<float>(base, n, ws);
-            PFOR(i, 0, n) { F.x[i] = F.x[i] + (double)S.x[i]; }
-            PFOR_END
-        }
+        mg_vcycle<T>(scr, n, sh);
+        mg_update<T>(F, Z, false, cur, oth, sh, &rmax, &xmax);
+        { double *t = cur; cur = oth; oth = t; }
         cyc++;
-        res = mg_relres(F, fmax, anorm, red);
+        den = anorm * xmax + fmax;
+        res = (den > 0.0) ? rmax / den : rmax;
     }
-    PFOR(i, 0, n) { p[i + 1] = F.x[i]; }
+    PFOR(i, 0, n) { p[i + 1] = cur[i]; }
     PFOR_END
     *res_out = res;
     return (res > rtol) ? -1 : cyc;
+}
+
+// Solve rows 1..n of a, b, c, g (a(1), c(n) ignored) into p(1..n).
+// mode 2: FP64 cycles; 3/4: FP32 cycles (x0 holds the initial guess for mode
+// 4).  Returns the cycles, or -1 when maxc cycles did not reach rtol; *res
+// the final backward error.
+DEV int mg_solve(const double *a, const double *b, const double *c, const double *g, double *p, int n,
+                 double *scr, int mode, double rtol, int maxc, const double *x0, MgSh &sh, double *res_out)
+{
+    const int L = mg_levels(n);
+    const bool f32 = (mode != LICA_MG_FP64_MODE);
+    const bool guess = (mode == LICA_MG_LEARNED_MODE && x0);
+    MgFine F = mg_fine(scr, n);
+    MgLev<float> Z32 = mg_lev<float>(scr, n, 0, sh);
+    float *a32 = (float *)Z32.a;
+    double fmax, anorm;
+    red_max2(0, n, [&](int i, double &fm, double &an) {
+        const double ai = (i == 0) ? 0.0 : a[i + 1], bi = b[i + 1], ci = (i == n - 1) ? 0.0 : c[i + 1];
+        F.a[i] = ai; F.b[i] = bi; F.c[i] = ci; F.ib[i] = 1.0 / bi;
+        F.g[i] = g[i + 1];
+        F.x0[i] = guess ? x0[i + 1] : 0.0;
+        if (f32) {
+            a32[i] = (float)ai; a32[n + i] = (float)bi; a32[2 * n + i] = (float)ci; a32[3 * n + i] = 1.0f / (float)bi;
+        }
+        fm = fabs(g[i + 1]);
+        an = fabs(ai) + fabs(bi) + fabs(ci);
+    }, sh.red, &fmax, &anorm);
+    for (int l = 0; l + 1 < L; l++)
+        mg_coarsen(mg_lev<double>(scr, n, l, sh), mg_lev<double>(scr, n, l + 1, sh),
+                   (float *)mg_lev<float>(scr, n, l + 1, sh).a, f32);
+    mg_coarsest_factor(mg_lev<double>(scr, n, L - 1, sh), sh);
+    if (f32) return mg_iterate<float>(F, scr, n, rtol, maxc, fmax, anorm, sh, p, res_out);
+    return mg_iterate<double>(F, scr, n, rtol, maxc, fmax, anorm, sh, p, res_out);
 }

@@ -3,6 +3,7 @@
 #include "lica_launch.h"
 
 long long lica_mg_scratch_doubles(int n) { return mg_scratch_doubles(n); }
+long long lica_mg_lds_need(int n, int tsize) { return mg_lds_bytes(n, tsize); }
 
 #ifndef LICA_EMUL
 // LDS workspace of the partitioned tridiagonal solves, sized by the launch
@@ -47,24 +48,51 @@ __global__ void __launch_bounds__(1024) k_poisson_direct(BatchDev bd)
     tri_attach(sh);
     body_poisson_direct(bd, bd.order[blockIdx.x], sh);
 }
+DEV void mg_attach(MgSh &sh, long long lds_bytes)
+{
+    if (threadIdx.x == 0) { sh.lds = (void *)lica_tri_lds; sh.lds_bytes = lds_bytes; }
+    __syncthreads();
+}
 __global__ void __launch_bounds__(1024) k_poisson_mg(BatchDev bd)
 {
-    __shared__ BlkSh sh;
+    __shared__ MgSh sh;
+    mg_attach(sh, bd.mg_lds);
     body_poisson_mg(bd, bd.order[blockIdx.x], sh);
 }
 
 // nsys independent POISSON systems by multigrid (scratch: mg_scratch_doubles(n) each)
 __global__ void __launch_bounds__(1024) k_mg_systems(const double *a, const double *b, const double *c, const double *g,
                                                      double *p, int n, double *scratch, int mode, double rtol,
-                                                     int maxc, int *cycles, double *resid)
+                                                     int maxc, int *cycles, double *resid, long long lds_bytes)
 {
-    __shared__ BlkSh sh;
+    __shared__ MgSh sh;
+    mg_attach(sh, lds_bytes);
     const size_t off = (size_t)blockIdx.x * (size_t)n - 1;
     double res = 0.0;
     int cyc = mg_solve(a + off, b + off, c + off, g + off, p + off, n,
-                       scratch + (size_t)blockIdx.x * (size_t)mg_scratch_doubles(n), mode, rtol, maxc, nullptr,
-                       sh.mgws, sh.red, &res);
+                       scratch + (size_t)blockIdx.x * (size_t)mg_scratch_doubles(n), mode, rtol, maxc, nullptr, sh,
+                       &res);
     if (threadIdx.x == 0) { cycles[blockIdx.x] = cyc; resid[blockIdx.x] = res; }
+}
+
+// largest dynamic LDS a multigrid workgroup may take (the static part is small)
+long long lica_mg_lds_max()
+{
+    static long long cap = -1;
+    if (cap < 0) {
+        int dev = 0, optin = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess || optin <= 0)
+            optin = 65536;
+        cap = (long long)optin - 4096;
+        const void *ks[2] = {(const void *)k_poisson_mg, (const void *)k_mg_systems};
+        for (int i = 0; i < 2; i++)
+            if (hipFuncSetAttribute(ks[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)cap) != hipSuccess) {
+                cap = 65536 - 4096;
+                break;
+            }
+    }
+    return cap;
 }
 
 int lica_launch_mg(int nsys, int n, const double *a, const double *b, const double *c, const double *g, double *p,
@@ -72,8 +100,10 @@ int lica_launch_mg(int nsys, int n, const double *a, const double *b, const doub
                    void *stream)
 {
     if (nsys <= 0) return 0;
-    hipLaunchKernelGGL(k_mg_systems, dim3(nsys), dim3(threads), 0, (hipStream_t)stream, a, b, c, g, p, n, scratch,
-                       mode, rtol, maxc, cycles, resid);
+    long long lds = mg_lds_bytes(n, mode == LICA_MG_FP64_MODE ? 8 : 4);
+    lds = std::min(lds, std::min(lica_mg_lds_max(), (long long)(nsys >= 512 ? 65536 - 4096 : 1 << 30)));
+    hipLaunchKernelGGL(k_mg_systems, dim3(nsys), dim3(threads), (size_t)lds, (hipStream_t)stream, a, b, c, g, p, n,
+                       scratch, mode, rtol, maxc, cycles, resid, lds);
     return (int)hipGetLastError();
 }
 
@@ -147,7 +177,9 @@ int lica_launch(int which, const BatchDev &bd, int threads, void *stream)
     case KL_POISSON_DIRECT: LAUNCH_TRI(k_poisson_direct, threads); break;
     case KL_UCALC: LAUNCH(k_ucalc, threads); break;
     case KL_TRANSPORT: LAUNCH(k_transport, 64); break;
-    case KL_POISSON_MG: LAUNCH(k_poisson_mg, threads); break;
+    case KL_POISSON_MG:
+        hipLaunchKernelGGL(k_poisson_mg, dim3(bd.ninst), dim3(threads), (size_t)bd.mg_lds, (hipStream_t)stream, bd);
+        break;
     case KL_CAPTURE_PRE:
         hipLaunchKernelGGL(k_capture, dim3(bd.ninst), dim3(256), 0, (hipStream_t)stream, bd, 0);
         break;
@@ -159,6 +191,9 @@ int lica_launch(int which, const BatchDev &bd, int threads, void *stream)
     return (int)hipGetLastError();
 }
 #else
+// the CPU logic-check build models a gfx950 workgroup's 160 KB of LDS
+long long lica_mg_lds_max() { return 160 * 1024 - 4096; }
+
 static double *emul_tri_ws(int threads)
 {
     static double *ws = nullptr;
@@ -189,16 +224,21 @@ int lica_launch_mg(int nsys, int n, const double *a, const double *b, const doub
                    void *stream)
 {
     (void)stream;
-    static BlkSh *sh = nullptr;
-    if (!sh) sh = new BlkSh();
+    static MgSh *sh = nullptr;
+    if (!sh) sh = new MgSh();
+    static std::vector<double> lds;
+    const long long lb = std::min(mg_lds_bytes(n, mode == LICA_MG_FP64_MODE ? 8 : 4), lica_mg_lds_max());
+    lds.assign((size_t)(lb / 8 + 1), 0.0);
+    sh->lds = lds.data();
+    sh->lds_bytes = lb;
     int saved = lica_emul_nthreads;
     lica_emul_nthreads = threads;
     for (int s = 0; s < nsys; s++) {
         const size_t off = (size_t)s * (size_t)n - 1;
         double res = 0.0;
         cycles[s] = mg_solve(a + off, b + off, c + off, g + off, p + off, n,
-                             scratch + (size_t)s * (size_t)mg_scratch_doubles(n), mode, rtol, maxc, nullptr,
-                             sh->mgws, sh->red, &res);
+                             scratch + (size_t)s * (size_t)mg_scratch_doubles(n), mode, rtol, maxc, nullptr, *sh,
+                             &res);
         resid[s] = res;
     }
     lica_emul_nthreads = saved;
@@ -206,6 +246,9 @@ int lica_launch_mg(int nsys, int n, const double *a, const double *b, const doub
 }
 
 static void attach(BlkSh &sh, int threads) { sh.tri.ws = emul_tri_ws(threads); sh.tri.ws_rows = tri_ws_rows(threads); }
+static double *emul_mg_lds = nullptr;
+static long long emul_mg_lds_bytes = 0;
+static void attach(MgSh &sh, int) { sh.lds = emul_mg_lds; sh.lds_bytes = emul_mg_lds_bytes; }
 template <class Sh>
 static void attach(Sh &, int) {}
 
@@ -236,7 +279,14 @@ int lica_launch(int which, const BatchDev &bd, int threads, void *stream)
     case KL_POISSON_DIRECT: emul_run<BlkSh>(bd, threads, body_poisson_direct); break;
     case KL_UCALC: emul_run<BlkSh>(bd, threads, body_ucalc); break;
     case KL_TRANSPORT: emul_run<TransportSh>(bd, 64, body_transport); break;
-    case KL_POISSON_MG: emul_run<BlkSh>(bd, threads, body_poisson_mg); break;
+    case KL_POISSON_MG: {
+        static std::vector<double> lds;
+        lds.assign((size_t)(bd.mg_lds / 8 + 1), 0.0);
+        emul_mg_lds = lds.data();
+        emul_mg_lds_bytes = bd.mg_lds;
+        emul_run<MgSh>(bd, threads, body_poisson_mg);
+        break;
+    }
     case KL_CAPTURE_PRE:
         for (int b = 0; b < bd.ninst; b++) body_capture(bd, bd.order[b], 0);
         break;

@@ -28,6 +28,7 @@
 #define NET_HALF (NET_TAPS / 2)
 #define NET_LAYERS 4
 #define NET_TILE 64          // cells per workgroup in the convolution kernels
+#define NET_NF 4             // input features per cell
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -202,15 +203,17 @@ __global__ void k_net_loss(const float *__restrict__ y, const float *__restrict_
     if ((threadIdx.x & 63) == 0) atomicAdd(loss, l);
 }
 
-// minibatch gather: window idx[s] of the data set into row block s
-__global__ void k_net_gather(const __bf16 *__restrict__ dx, const float *__restrict__ dt, const int *__restrict__ idx,
-                             __bf16 *__restrict__ x, float *__restrict__ t, int len, int batch)
+// minibatch gather: window idx[s] of the data set ([cell][NET_NF]) into row
+// block s of the activations ([cell][NET_C], channels beyond NET_NF zero)
+__global__ void k_net_gather(const __bf16 *__restrict__ dx, const float *__restrict__ dt,
+                             const long long *__restrict__ idx, __bf16 *__restrict__ x, float *__restrict__ t, int len,
+                             int batch)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long long)batch * len) return;
     const int s = (int)(i / len), c = (int)(i % len);
-    const long long src = (long long)idx[s] * len + c;
-    for (int ch = 0; ch < NET_C; ch++) x[i * NET_C + ch] = dx[src * NET_C + ch];
+    const long long src = idx[s] * len + c;
+    for (int ch = 0; ch < NET_C; ch++) x[i * NET_C + ch] = (ch < NET_NF) ? dx[src * NET_NF + ch] : to_bf16(0.0f);
     t[i] = dt[src];
 }
 
@@ -231,20 +234,21 @@ __global__ void k_net_adam(float *__restrict__ w, float *__restrict__ m, float *
 
 // ---------------------------------------------------------------------
 // Features of the POISSON system of every case of a batch (one workgroup per
-// case): scales s_g = max|g|, s_b = max|b|, s_p = max|P before the solve|,
-// features g/s_g, a/s_b, c/s_b, P/s_p in channels 0..3 (the rest zero).
+// case): with Q = P + pshift the last solution (UCALC's shift of P undone),
+// scales s_g = max|g|, s_b = max|b|, s_q = max|Q| (1 where the maximum is 0),
+// features g/s_g, a/s_b, c/s_b, Q/s_q.
 
-__global__ void __launch_bounds__(256) k_net_features(const NetCase *__restrict__ cases, __bf16 *__restrict__ X,
-                                                      double *__restrict__ scale)
+// the three scales over rows 1..n (256-thread workgroup); q = p + pq
+__device__ void net_scales(const double *g, const double *b, const double *p, double pq, int n, double *sg_out,
+                           double *sb_out, double *sp_out)
 {
     __shared__ double red[3][8];
-    const NetCase cs = cases[blockIdx.x];
     const int tid = threadIdx.x;
     double sg = 0.0, sb = 0.0, sp = 0.0;
-    for (int k = 1 + tid; k <= cs.n; k += blockDim.x) {
-        sg = fmax(sg, fabs(cs.g[k]));
-        sb = fmax(sb, fabs(cs.b[k]));
-        sp = fmax(sp, fabs(cs.p[k]));
+    for (int k = 1 + tid; k <= n; k += blockDim.x) {
+        sg = fmax(sg, fabs(g[k]));
+        sb = fmax(sb, fabs(b[k]));
+        sp = fmax(sp, fabs(p[k] + pq));
     }
     for (int off = 32; off > 0; off >>= 1) {
         sg = fmax(sg, __shfl_xor(sg, off, 64));
@@ -255,27 +259,66 @@ __global__ void __launch_bounds__(256) k_net_features(const NetCase *__restrict_
     __syncthreads();
     sg = 0.0; sb = 0.0; sp = 0.0;
     for (int w = 0; w < (int)(blockDim.x >> 6); w++) { sg = fmax(sg, red[0][w]); sb = fmax(sb, red[1][w]); sp = fmax(sp, red[2][w]); }
-    if (sg == 0.0) sg = 1.0;
-    if (sb == 0.0) sb = 1.0;
-    if (sp == 0.0) sp = 1.0;
+    *sg_out = (sg == 0.0) ? 1.0 : sg;
+    *sb_out = (sb == 0.0) ? 1.0 : sb;
+    *sp_out = (sp == 0.0) ? 1.0 : sp;
+}
+
+__global__ void __launch_bounds__(256) k_net_features(const NetCase *__restrict__ cases, __bf16 *__restrict__ X,
+                                                      double *__restrict__ scale)
+{
+    const NetCase cs = cases[blockIdx.x];
+    const int tid = threadIdx.x;
+    const double pq = *cs.pshift;
+    double sg, sb, sp;
+    net_scales(cs.g, cs.b, cs.p, pq, cs.n, &sg, &sb, &sp);
     for (int k = 1 + tid; k <= cs.n; k += blockDim.x) {
         __bf16 *row = X + (cs.off + k - 1) * NET_C;
         row[0] = to_bf16((float)(cs.g[k] / sg));
         row[1] = to_bf16((float)(cs.a[k] / sb));
         row[2] = to_bf16((float)(cs.c[k] / sb));
-        row[3] = to_bf16((float)(cs.p[k] / sp));
-        for (int ch = 4; ch < NET_C; ch++) row[ch] = to_bf16(0.0f);
+        row[3] = to_bf16((float)((cs.p[k] + pq) / sp));
+        for (int ch = NET_NF; ch < NET_C; ch++) row[ch] = to_bf16(0.0f);
     }
     if (tid == 0) scale[blockIdx.x] = sp;
 }
 
-// guess(k) = P(k) + s_p * y(k)
+// training windows from captured systems (one workgroup per case): window w
+// of case i starts at row 1 + woff[i*wpc + w] (-1: none) and goes to data-set
+// slot slot[i*wpc + w]; features as above ([cell][NET_NF] BF16), target
+// (p1 - p0) / s_p (FP32)
+__global__ void __launch_bounds__(256) k_net_pairs(const NetPairSrc *__restrict__ src, const int *__restrict__ woff,
+                                                   const long long *__restrict__ slot, int wpc, int len,
+                                                   __bf16 *__restrict__ dx, float *__restrict__ dt)
+{
+    const NetPairSrc s = src[blockIdx.x];
+    double sg, sb, sp;
+    net_scales(s.g, s.b, s.p0, 0.0, s.n, &sg, &sb, &sp);
+    for (int w = 0; w < wpc; w++) {
+        const int o = woff[blockIdx.x * wpc + w];
+        if (o < 0) continue;
+        const long long base = slot[blockIdx.x * wpc + w] * len;
+        for (int c = threadIdx.x; c < len; c += blockDim.x) {
+            const int k = 1 + o + c;
+            __bf16 *row = dx + (base + c) * NET_NF;
+            row[0] = to_bf16((float)(s.g[k] / sg));
+            row[1] = to_bf16((float)(s.a[k] / sb));
+            row[2] = to_bf16((float)(s.c[k] / sb));
+            row[3] = to_bf16((float)(s.p0[k] / sp));
+            dt[base + c] = (float)((s.p1[k] - s.p0[k]) / sp);
+        }
+    }
+}
+
+// guess(k) = Q(k) + s_q * y(k)
 __global__ void __launch_bounds__(256) k_net_guess(const NetCase *__restrict__ cases, const float *__restrict__ y,
                                                    const double *__restrict__ scale)
 {
     const NetCase cs = cases[blockIdx.x];
     const double sp = scale[blockIdx.x];
-    for (int k = 1 + threadIdx.x; k <= cs.n; k += blockDim.x) cs.guess[k] = cs.p[k] + sp * (double)y[cs.off + k - 1];
+    const double pq = *cs.pshift;
+    for (int k = 1 + threadIdx.x; k <= cs.n; k += blockDim.x)
+        cs.guess[k] = (cs.p[k] + pq) + sp * (double)y[cs.off + k - 1];
 }
 
 // ---------------------------------------------------------------------
@@ -314,10 +357,10 @@ struct lica_net {
     int nseq = 0, ntiles = 0;
     long long ncell = 0;
     // training set (device resident, fixed-length windows)
-    __bf16 *data_x = nullptr;
-    float *data_t = nullptr;
-    int *idx = nullptr;
-    long long data_n = 0;
+    __bf16 *data_x = nullptr;           // [window][cell][NET_NF]
+    float *data_t = nullptr;            // [window][cell]
+    long long *idx = nullptr;
+    long long data_n = 0, data_cap = 0; // windows held / room for
     int data_len = 0;
     const void *layout_owner = nullptr;   // batch whose cases the current layout holds
 };
@@ -445,6 +488,28 @@ int net_upload_features(lica_net *t, const float *feat, long long ncell)
         for (int c = 0; c < NET_C; c++) x[(size_t)i * NET_C + c] = (__bf16)(c < 4 ? feat[i * 4 + c] : 0.0f);
     int rc = hip_ok(hipMemcpyAsync(t->act[0], x.data(), sizeof(__bf16) * x.size(), hipMemcpyHostToDevice, t->stream));
     return rc ? rc : hip_ok(hipStreamSynchronize(t->stream));
+}
+// room for n windows of t->data_len cells in the data set (keeps the windows held)
+int net_data_reserve(lica_net *t, long long n)
+{
+    if (n <= t->data_cap) return 0;
+    const long long cap = std::max(n, 2 * t->data_cap);
+    const size_t cells = (size_t)cap * (size_t)t->data_len;
+    __bf16 *x = nullptr;
+    float *tg = nullptr;
+    int rc = net_alloc((void **)&x, sizeof(__bf16) * cells * NET_NF);
+    rc = rc ? rc : net_alloc((void **)&tg, sizeof(float) * cells);
+    const size_t held = (size_t)t->data_n * (size_t)t->data_len;
+    if (!rc && held) {
+        rc = hip_ok(hipMemcpy(x, t->data_x, sizeof(__bf16) * held * NET_NF, hipMemcpyDeviceToDevice));
+        rc = rc ? rc : hip_ok(hipMemcpy(tg, t->data_t, sizeof(float) * held, hipMemcpyDeviceToDevice));
+    }
+    if (rc) { (void)hipFree(x); (void)hipFree(tg); return rc; }
+    (void)hipFree(t->data_x); (void)hipFree(t->data_t);
+    t->data_x = x;
+    t->data_t = tg;
+    t->data_cap = cap;
+    return 0;
 }
 }  // namespace
 
@@ -577,22 +642,43 @@ int lica_net_grad(lica_net *t, const float *feat, const float *target, const int
 }
 
 // training windows: n windows of len cells, features [n*len][4], target [n*len]
+// (replaces the data set)
 int lica_net_set_data(lica_net *t, const float *feat, const float *target, int64_t n, int64_t len)
 {
     if (!t || !feat || !target || n <= 0 || len <= 0) return LICA_BAD_ARG;
-    (void)hipFree(t->data_x); (void)hipFree(t->data_t);
-    t->data_x = nullptr; t->data_t = nullptr;
+    t->data_n = 0;
+    t->data_len = (int)len;
+    int rc = net_data_reserve(t, n);
+    if (rc) return rc;
     const size_t cells = (size_t)n * (size_t)len;
-    std::vector<__bf16> x(cells * NET_C);
-    for (size_t i = 0; i < cells; i++)
-        for (int c = 0; c < NET_C; c++) x[i * NET_C + c] = (__bf16)(c < 4 ? feat[i * 4 + c] : 0.0f);
-    int rc = net_alloc((void **)&t->data_x, sizeof(__bf16) * x.size());
-    rc = rc ? rc : net_alloc((void **)&t->data_t, sizeof(float) * cells);
-    rc = rc ? rc : hip_ok(hipMemcpy(t->data_x, x.data(), sizeof(__bf16) * x.size(), hipMemcpyHostToDevice));
+    std::vector<__bf16> x(cells * NET_NF);
+    for (size_t i = 0; i < cells * NET_NF; i++) x[i] = (__bf16)feat[i];
+    rc = hip_ok(hipMemcpy(t->data_x, x.data(), sizeof(__bf16) * x.size(), hipMemcpyHostToDevice));
     rc = rc ? rc : hip_ok(hipMemcpy(t->data_t, target, sizeof(float) * cells, hipMemcpyHostToDevice));
     if (rc) return rc;
     t->data_n = n;
-    t->data_len = (int)len;
+    return LICA_OK;
+}
+
+int lica_net_data_size(lica_net *t, int64_t *n, int64_t *len)
+{
+    if (!t || !n || !len) return LICA_BAD_ARG;
+    *n = t->data_n;
+    *len = t->data_len;
+    return LICA_OK;
+}
+
+// the data set back on the host: features [n*len][4] (BF16 values), targets
+int lica_net_get_data(lica_net *t, float *feat, float *target)
+{
+    if (!t || !feat || !target) return LICA_BAD_ARG;
+    const size_t cells = (size_t)t->data_n * (size_t)t->data_len;
+    if (!cells) return LICA_OK;
+    std::vector<__bf16> x(cells * NET_NF);
+    int rc = hip_ok(hipMemcpy(x.data(), t->data_x, sizeof(__bf16) * x.size(), hipMemcpyDeviceToHost));
+    rc = rc ? rc : hip_ok(hipMemcpy(target, t->data_t, sizeof(float) * cells, hipMemcpyDeviceToHost));
+    if (rc) return rc;
+    for (size_t i = 0; i < x.size(); i++) feat[i] = (float)x[i];
     return LICA_OK;
 }
 
@@ -607,16 +693,16 @@ int lica_net_train(lica_net *t, int64_t iters, int64_t batch, double lr, uint64_
     if (rc) return rc;
     std::mt19937_64 rng(seed);
     std::uniform_int_distribution<int64_t> pick(0, t->data_n - 1);
-    std::vector<int> hidx((size_t)batch);
+    std::vector<long long> hidx((size_t)batch);
     (void)hipFree(t->idx);
-    rc = net_alloc((void **)&t->idx, sizeof(int) * (size_t)batch);
+    rc = net_alloc((void **)&t->idx, sizeof(long long) * (size_t)batch);
     if (rc) return rc;
     double acc = 0.0;
     int64_t nacc = 0;
     const double b1 = 0.9, b2 = 0.999;
     for (int64_t it = 0; it < iters && !rc; it++) {
-        for (int64_t s = 0; s < batch; s++) hidx[(size_t)s] = (int)pick(rng);
-        rc = hip_ok(hipMemcpyAsync(t->idx, hidx.data(), sizeof(int) * hidx.size(), hipMemcpyHostToDevice, t->stream));
+        for (int64_t s = 0; s < batch; s++) hidx[(size_t)s] = (long long)pick(rng);
+        rc = hip_ok(hipMemcpyAsync(t->idx, hidx.data(), sizeof(long long) * hidx.size(), hipMemcpyHostToDevice, t->stream));
         const long long nc = batch * (long long)len;
         hipLaunchKernelGGL(k_net_gather, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, t->stream, t->data_x,
                            t->data_t, t->idx, t->act[0], t->target, len, (int)batch);
@@ -673,4 +759,49 @@ int lica_net_prepare(lica_net *t, const int64_t *lens, int ncase, const NetCase 
     int rc = net_layout(t, lens, ncase);
     if (!rc) t->layout_owner = (const void *)dcases;
     return rc;
+}
+
+// training windows from a batch's captured systems (see lica_net_add_pairs)
+int lica_net_append(lica_net *t, const NetPairSrc *hsrc, int ncase, int64_t wpc, int64_t len, uint64_t seed,
+                    int64_t *added)
+{
+    if (!t || (!hsrc && ncase > 0) || ncase < 0 || wpc < 1 || len < 1 || len > (1 << 30)) return LICA_BAD_ARG;
+    if (t->data_n > 0 && len != t->data_len) return LICA_BAD_ARG;
+    if (added) *added = 0;
+    if (ncase == 0) return LICA_OK;
+    std::mt19937_64 rng(seed);
+    std::vector<int> woff((size_t)ncase * (size_t)wpc, -1);
+    std::vector<long long> slot((size_t)ncase * (size_t)wpc, 0);
+    long long n_new = 0;
+    for (int i = 0; i < ncase; i++) {
+        if (hsrc[i].n < len) continue;
+        std::uniform_int_distribution<int64_t> pick(0, hsrc[i].n - len);
+        for (int64_t w = 0; w < wpc; w++) {
+            woff[(size_t)i * wpc + w] = (int)pick(rng);
+            slot[(size_t)i * wpc + w] = t->data_n + n_new++;
+        }
+    }
+    if (!n_new) return LICA_OK;
+    t->data_len = (int)len;
+    int rc = net_data_reserve(t, t->data_n + n_new);
+    NetPairSrc *dsrc = nullptr;
+    int *dwoff = nullptr;
+    long long *dslot = nullptr;
+    rc = rc ? rc : net_alloc((void **)&dsrc, sizeof(NetPairSrc) * (size_t)ncase);
+    rc = rc ? rc : net_alloc((void **)&dwoff, sizeof(int) * woff.size());
+    rc = rc ? rc : net_alloc((void **)&dslot, sizeof(long long) * slot.size());
+    rc = rc ? rc : hip_ok(hipMemcpy(dsrc, hsrc, sizeof(NetPairSrc) * (size_t)ncase, hipMemcpyHostToDevice));
+    rc = rc ? rc : hip_ok(hipMemcpy(dwoff, woff.data(), sizeof(int) * woff.size(), hipMemcpyHostToDevice));
+    rc = rc ? rc : hip_ok(hipMemcpy(dslot, slot.data(), sizeof(long long) * slot.size(), hipMemcpyHostToDevice));
+    if (!rc) {
+        hipLaunchKernelGGL(k_net_pairs, dim3(ncase), dim3(256), 0, t->stream, dsrc, dwoff, dslot, (int)wpc,
+                           (int)len, t->data_x, t->data_t);
+        rc = hip_ok(hipGetLastError());
+        rc = rc ? rc : hip_ok(hipStreamSynchronize(t->stream));
+    }
+    (void)hipFree(dsrc); (void)hipFree(dwoff); (void)hipFree(dslot);
+    if (rc) return rc;
+    t->data_n += n_new;
+    if (added) *added = n_new;
+    return LICA_OK;
 }

@@ -105,6 +105,46 @@ DEV double red_min(int lo, int hi, double init, F f, double *scr)
 #endif
 }
 
+// two maxima at once over f(i, u, v) (u, v >= 0), i in [lo,hi); f may also
+// write memory, which is visible to the whole workgroup on return
+template <class F>
+DEV void red_max2(int lo, int hi, F f, double *scr, double *m0, double *m1)
+{
+#ifdef LICA_EMUL
+    (void)scr;
+    lica_emul_check();
+    double a = 0.0, b = 0.0;
+    for (int i = lo; i < hi; ++i) {
+        double u, v;
+        f(i, u, v);
+        a = fmax(a, u);
+        b = fmax(b, v);
+    }
+    *m0 = a;
+    *m1 = b;
+#else
+    double a = 0.0, b = 0.0;
+    for (int i = lo + (int)threadIdx.x; i < hi; i += (int)blockDim.x) {
+        double u, v;
+        f(i, u, v);
+        a = fmax(a, u);
+        b = fmax(b, v);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        a = fmax(a, __shfl_xor(a, off, 64));
+        b = fmax(b, __shfl_xor(b, off, 64));
+    }
+    const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    if ((threadIdx.x & 63) == 0) { scr[2 * w] = a; scr[2 * w + 1] = b; }
+    __syncthreads();
+    double ra = 0.0, rb = 0.0;
+    for (int i = 0; i < nw; ++i) { ra = fmax(ra, scr[2 * i]); rb = fmax(rb, scr[2 * i + 1]); }
+    __syncthreads();
+    *m0 = ra;
+    *m1 = rb;
+#endif
+}
+
 // logical OR over work items (as an int reduction)
 template <class F>
 DEV int red_or(int lo, int hi, F f, double *scr)

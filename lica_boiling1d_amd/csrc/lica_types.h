@@ -57,6 +57,7 @@ struct InstState {
     double n_subcycle_warn, n_mass_warn, n_halvings, n_extrap_iters;
     double n_reinit, n_bandgen, n_masscor;
     double poisson_cycles, poisson_solves, poisson_resid;
+    double pshift;        // p(1) removed by the last UCALC: P + pshift is the last solution
     int sf_lo, sf_hi;     // refined window of nonzero SFGAS/SFGAS2
     int im_lo, im_hi;     // NS window of IMASK_PH==1
     int bn[LICA_MLAYER + 1];
@@ -79,6 +80,7 @@ struct BatchDev {
     int mg_maxc;         // V-cycle cap per solve
     double mg_rtol;      // relative residual target
     double *mg;          // multigrid scratch of the batch
+    long long mg_lds;    // dynamic LDS bytes of the multigrid kernel (x, f of the coarse levels)
     const double *guess; // initial guesses of the learned mode (rows at gs_off)
     double *cap;         // P before / after the last substep's solve, its RHS (3 x (n+2) at 3*gs_off)
 };

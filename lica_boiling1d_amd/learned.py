@@ -4,16 +4,18 @@ A 1-D convolutional network (4 layers of 21 taps, 32 channels, ReLU; BF16
 weights and activations, FP32 accumulation on MFMA; about 46k parameters)
 maps, cell by cell, the features of a POISSON system (1d.f:3529-3543)
 
-    g / max|g|,  a / max|b|,  c / max|b|,  P / max|P|
+    g / max|g|,  a / max|b|,  c / max|b|,  Q / max|Q|
 
-(g the right-hand side, a/b/c the three diagonals, P the pressure held before
-the solve, i.e. the previous substep's) to the correction (p - P) / max|P| of
-the solution p.  ``LICA_POISSON_LEARNED`` starts the mixed-precision V-cycles
-from P + max|P| * network; the V-cycles always continue to the same residual
+(g the right-hand side, a/b/c the three diagonals, Q the previous substep's
+solution: the pressure held before the solve plus the p(1) that UCALC
+subtracted from it, 1d.f:3433-3470) to the correction (p - Q) / max|Q| of the
+solution p.  ``LICA_POISSON_LEARNED`` starts the mixed-precision V-cycles from
+Q + max|Q| * network; the V-cycles always continue to the same residual
 tolerance, so multigrid decides the answer and the network only saves
 cycles.  Training pairs come from the solver itself (``collect_pairs``): a
-batch runs with capture on and keeps, for every case, P before and after the
-last substep's solve.
+batch runs with capture on in a multigrid mode, and after every step windows
+of every case's last POISSON system go from its capture rows straight into
+the network's data set in device memory.
 """
 import ctypes
 
@@ -88,8 +90,26 @@ class Net:
                                        ctypes.byref(loss)), "lica_net_grad")
         return loss.value, g
 
+    def add_pairs(self, batch, windows_per_case=4, window=WINDOW, seed=0):
+        """Append windows of the batch's captured POISSON systems to the data
+        set (device to device); returns the number of windows added."""
+        added = ctypes.c_int64()
+        _check(self._lib.lica_net_add_pairs(self._h, batch._h, int(windows_per_case), int(window), int(seed),
+                                            ctypes.byref(added)), "lica_net_add_pairs")
+        return added.value
+
+    def data(self):
+        """The data set: features (n, len, 4) as stored (BF16 values), targets (n, len)."""
+        n, ln = ctypes.c_int64(), ctypes.c_int64()
+        _check(self._lib.lica_net_data_size(self._h, ctypes.byref(n), ctypes.byref(ln)), "lica_net_data_size")
+        x = np.zeros((n.value, ln.value, NFEAT), dtype=np.float32)
+        t = np.zeros((n.value, ln.value), dtype=np.float32)
+        if n.value:
+            _check(self._lib.lica_net_get_data(self._h, _f(x), _f(t)), "lica_net_get_data")
+        return x, t
+
     def set_data(self, feats, targets):
-        """feats (n, WINDOW, 4), targets (n, WINDOW)"""
+        """feats (n, WINDOW, 4), targets (n, WINDOW); replaces the data set"""
         n, ln = targets.shape
         x = np.ascontiguousarray(feats.reshape(-1, NFEAT).astype(np.float32))
         t = np.ascontiguousarray(targets.reshape(-1).astype(np.float32))
@@ -102,40 +122,30 @@ class Net:
         return loss.value
 
 
-def features(a, b, c, g, p):
+def features(a, b, c, g, q):
     """Per-cell features and the solution scale of one POISSON system (rows
-    1..n); the same arithmetic as the device feature kernel."""
+    1..n, q the previous solution); the same arithmetic as the device
+    feature kernels, before the BF16 rounding."""
     sg = np.max(np.abs(g)) or 1.0
     sb = np.max(np.abs(b)) or 1.0
-    sp = np.max(np.abs(p)) or 1.0
-    f = np.stack([g / sg, a / sb, c / sb, p / sp], axis=1)
-    return f.astype(np.float32), sp
+    sq = np.max(np.abs(q)) or 1.0
+    f = np.stack([g / sg, a / sb, c / sb, q / sq], axis=1)
+    return f.astype(np.float32), sq
 
 
-def collect_pairs(cases, steps, per_step=4, window=WINDOW, seed=0, mode=None, lib=None):
-    """Run the cases with capture on and cut (features, target) windows of
-    `window` cells from every case's last-substep POISSON system after every
-    step.  Returns (feats (m, window, 4), targets (m, window))."""
+def collect_pairs(net, cases, steps, windows_per_case=4, window=WINDOW, seed=0, mode=None, rtol=1e-10, lib=None):
+    """Run the cases `steps` steps in a multigrid mode (mixed precision by
+    default) with capture on and append, after every step, windows of every
+    healthy case's last POISSON system to the network's data set.  Returns the
+    number of windows added."""
     from .solver import Batch
     from . import _abi as A
-    rng = np.random.default_rng(seed)
-    b = Batch(cases, lib=lib)
-    b.set_poisson(A.POISSON_DIRECT if mode is None else mode, 1e-10, 200)
-    _check(b._lib.lica_batch_set_capture(b._h, 1), "set_capture")
-    fx, tx = [], []
-    for _ in range(steps):
+    b = Batch(cases, lib=lib if lib is not None else net._lib)
+    b.set_poisson(A.POISSON_MIXED if mode is None else mode, rtol, 200)
+    b.set_capture(True)
+    total = 0
+    for s in range(steps):
         b.run_all(1)
-        st, _ = b.progress()
-        for i, cs in enumerate(cases):
-            if st[i] != 0 or int(cs.n) < window:
-                continue
-            a, bb, c, g = (b.field(k, i) for k in ("PA", "PB", "PC", "PRHS"))
-            p0, p1 = b.field("PPREV", i), b.field("PSOL", i)
-            f, sp = features(a, bb, c, g, p0)
-            t = ((p1 - p0) / sp).astype(np.float32)
-            for _ in range(per_step):
-                o = int(rng.integers(0, int(cs.n) - window + 1))
-                fx.append(f[o:o + window])
-                tx.append(t[o:o + window])
+        total += net.add_pairs(b, windows_per_case, window, seed * 1000003 + s)
     b.close()
-    return np.array(fx), np.array(tx)
+    return total

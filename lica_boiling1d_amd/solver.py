@@ -127,6 +127,10 @@ class Batch(_Handle):
     def set_poisson(self, mode, rtol=1.0e-10, max_cycles=100):
         _check(self._lib.lica_batch_set_poisson(self._h, int(mode), float(rtol), int(max_cycles)), "set_poisson")
 
+    def set_mg_lds(self, nbytes):
+        """Cap on the multigrid kernel's LDS (bytes; -1 restores the default)."""
+        _check(self._lib.lica_batch_set_mg_lds(self._h, int(nbytes)), "set_mg_lds")
+
     def set_block(self, threads):
         _check(self._lib.lica_batch_set_block(self._h, int(threads)), "set_block")
 
@@ -138,8 +142,9 @@ class Batch(_Handle):
         self._net = net
         _check(self._lib.lica_batch_set_net(self._h, net.handle), "set_net")
 
-    def set_capture(self, on=True):
-        _check(self._lib.lica_batch_set_capture(self._h, 1 if on else 0), "set_capture")
+    def set_capture(self, on=True, substep=3):
+        """Keep the POISSON system of substep `substep` (1..3) of every step."""
+        _check(self._lib.lica_batch_set_capture(self._h, int(substep) if on else 0), "set_capture")
 
     def set_timing(self, on=True):
         _check(self._lib.lica_batch_set_timing(self._h, 1 if on else 0), "set_timing")

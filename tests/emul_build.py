@@ -22,7 +22,7 @@ def build(force=False):
     if not force and os.path.exists(LIB) and os.path.getmtime(LIB) >= max(os.path.getmtime(p) for p in deps):
         return LIB
     tmp = LIB + ".tmp"
-    cmd = ["g++", "-O2", "-std=c++17", "-DLICA_EMUL", "-ffp-contract=off", "-fPIC", "-shared",
+    cmd = ["g++", "-O2", "-std=c++17", "-DLICA_EMUL", "-ffp-contract=off", "-fPIC", "-shared", "-pthread",
            "-x", "c++", *srcs, "-o", tmp]
     subprocess.run(cmd, check=True, cwd=CSRC)
     os.replace(tmp, LIB)

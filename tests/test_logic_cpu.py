@@ -121,3 +121,53 @@ def test_writefield_file(emul, tmp_path):
     pos += 11 + 4 * sum(numa) + 1 + 1 + 1 + 1 + 1 + 1 + 3
     t = np.array([float(x) for x in toks[pos:]]).reshape(N3 + 1, N2 + 1, N1 + 1)
     assert np.allclose(t[:, 0, 0], b.field("T"), rtol=1e-11, atol=1e-300)
+
+
+@pytest.mark.parametrize("mode", [L.POISSON_MG64, L.POISSON_MIXED])
+def test_multigrid_modes_close_to_oracle(emul, mode):
+    """V-cycles to a backward error of 1e-10 on the deck case: every solve
+    meets it, the interface stays within 1e-6 relative of the oracle, and
+    where the levels' x, f live (all in LDS, split, all in global memory)
+    does not change a bit of the result."""
+    case = L.deck_case()
+    o = Oracle(case)
+    assert o.step(8) == 0
+    runs = []
+    for cap in (-1, 3000, 0):
+        b = L.Batch([case], lib=emul)
+        b.set_poisson(mode, 1e-10, 100)
+        b.set_mg_lds(cap)
+        b.run(8)
+        s = b.scalars()
+        assert s["POISSON_SOLVES"] == 24
+        assert 0 < s["POISSON_RESID"] <= 1e-10
+        assert s["POISSON_CYCLES"] / s["POISSON_SOLVES"] < 30
+        zo = o.scalar_array()[3]
+        assert abs(s["PLACE"] - zo) <= 1e-6 * zo
+        runs.append([b.field(f) for f in ("W", "P", "T", "PHI")])
+    for other in runs[1:]:
+        assert all(np.array_equal(x, y) for x, y in zip(runs[0], other))
+
+
+@pytest.mark.parametrize("mode", [L.POISSON_MG64, L.POISSON_MIXED])
+@pytest.mark.parametrize("n", [20, 33, 257, 1000])
+def test_batched_poisson_multigrid_cpu(emul, mode, n):
+    """lica_poisson by V-cycles against a dense solve: backward error within
+    1e-10 (one level for n <= 32, odd level sizes above)."""
+    rng = np.random.default_rng(n)
+    nsys = 3
+    a = -rng.uniform(0.5, 1.5, (nsys, n))
+    c = -rng.uniform(0.5, 1.5, (nsys, n))
+    a[:, 0] = 0.0
+    # diagonally dominant M-matrices (the POISSON rows' sign pattern, strict
+    # dominance so that Gauss-Seidel smoothing converges on random coefficients)
+    b = -(a + c) + rng.uniform(0.01, 0.1, (nsys, n))
+    g = rng.standard_normal((nsys, n))
+    p = L.poisson(a, b, c, g, mode=mode, rtol=1e-10, max_cycles=100, lib=emul)
+    for s in range(nsys):
+        A = np.diag(b[s]) + np.diag(a[s, 1:], -1) + np.diag(c[s, :-1], 1)
+        r = g[s] - A @ p[s]
+        anorm = np.max(np.abs(a[s]) + np.abs(b[s]) + np.abs(c[s]))
+        assert np.max(np.abs(r)) <= 1e-10 * (anorm * np.max(np.abs(p[s])) + np.max(np.abs(g[s])))
+        x = np.linalg.solve(A, g[s])
+        assert np.max(np.abs(p[s] - x)) <= 1e-6 * np.max(np.abs(x))

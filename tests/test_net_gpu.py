@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import lica_boiling1d_amd as L
-from lica_boiling1d_amd.learned import Net, collect_pairs
+from lica_boiling1d_amd.learned import Net, collect_pairs, features
 from oracle import Oracle
 import net_ref as R
 
@@ -95,16 +95,60 @@ def test_training_fits_a_learnable_map():
     assert l1 <= 0.05 * l0, (l0, l1)
 
 
+def test_device_pairs_equal_host_features():
+    """Windows cut on the device from the capture rows equal the features and
+    targets computed on the host from the same rows (window = whole column,
+    so its offset is 0)."""
+    _say("device training pairs, 3 cases n=128")
+    cases = [L.deck_case(128, 2).copy(z0=z) for z in (0.05, 0.0607, 0.08)]
+    net = Net(seed=0)
+    b = L.Batch(cases)
+    b.set_poisson(L.POISSON_MIXED, 1e-10, 100)
+    b.set_capture(True)
+    b.run(3)
+    assert net.add_pairs(b, 1, 128, seed=0) == 3
+    x, t = net.data()
+    assert x.shape == (3, 128, 4) and t.shape == (3, 128)
+    for i in range(3):
+        a, bb, c, g = (b.field(k, i) for k in ("PA", "PB", "PC", "PRHS"))
+        q, p = b.field("PPREV", i), b.field("PSOL", i)
+        f, sq = features(a, bb, c, g, q)
+        assert np.array_equal(x[i], R.bf16(f).astype(np.float32)), i
+        assert np.array_equal(t[i], ((p - q) / sq).astype(np.float32)), i
+    # appending keeps what is held; the window length is fixed by the data set
+    b.run(1)
+    assert net.add_pairs(b, 2, 128, seed=1) == 6
+    x2, t2 = net.data()
+    assert x2.shape == (9, 128, 4)
+    assert np.array_equal(x2[:3], x) and np.array_equal(t2[:3], t)
+    with pytest.raises(L.LicaError):
+        net.add_pairs(b, 1, 64)
+    b.close()
+
+
+def test_add_pairs_needs_capture_and_multigrid():
+    cases = [L.deck_case(128, 2)]
+    net = Net(seed=0)
+    b = L.Batch(cases)
+    b.set_poisson(L.POISSON_MIXED, 1e-10, 100)
+    b.run(1)
+    with pytest.raises(L.LicaError):
+        net.add_pairs(b, 1, 64)
+    b.set_poisson(L.POISSON_DIRECT)
+    b.set_capture(True)
+    b.run(1)
+    with pytest.raises(L.LicaError):
+        net.add_pairs(b, 1, 64)
+
+
 def test_learned_mode_meets_tolerance():
     """Train on the solver's own POISSON pairs of the large case, then run the
     learned mode: every solve meets the backward-error tolerance and the
     interface stays within 1e-6 relative of the FP64 oracle."""
     case = L.single_large_case(1)
     _say("learned mode: collect pairs, train, 6 steps of the large case")
-    fx, tx = collect_pairs([case], steps=6, per_step=16, mode=L.POISSON_MIXED)
-    assert fx.shape[0] == 6 * 16
     net = Net(seed=0)
-    net.set_data(fx, tx)
+    assert collect_pairs(net, [case], steps=6, windows_per_case=16) == 6 * 16
     net.train(100, batch=32, lr=1e-3)
     b, o = L.Batch([case]), Oracle(case)
     b.set_net(net)
