@@ -7,10 +7,14 @@ U(0.05,0.5) mm, refinement r=2, drawn from SplitMix64 seed 20261202.  A
 completed instance-steps/s with the state resident in HBM, every Poisson
 solve taken by V-cycles to a normwise backward error of 1e-10.
 
-Two timed runs of the same batch from the same initial state:
-  mixed+learned  FP32 V-cycle corrections in FP64 defect correction, started
-                 from the warm-start network's guess (the value);
-  plain FP64     FP64 V-cycles from a zero initial guess (IOLDV=0).
+Three timed runs of the same batch from the same initial state, each over
+the same window of steps after the same untimed warmup (past the start-up
+transient in which the pressure field builds up from zero):
+  mixed+learned    FP32 V-cycle corrections in FP64 defect correction,
+                   started from the history projection corrected by the
+                   warm-start network (the value);
+  mixed+projected  the same without the network (what the network adds);
+  plain FP64       FP64 V-cycles from a zero initial guess (IOLDV=0).
 Before them the network is trained on the GPU on pairs the solver generates
 from a disjoint batch (seed 20261203); the training is not timed.
 
@@ -149,7 +153,8 @@ def train_net(L, args):
     t0 = time.perf_counter()
     net = Net(seed=args.seed)
     tcases = L.batched_cases(args.train_cases, seed=20261203)
-    nwin = collect_pairs(net, tcases, steps=args.train_steps, windows_per_case=args.train_windows, window=256)
+    nwin = collect_pairs(net, tcases, steps=args.train_steps, windows_per_case=args.train_windows, window=256,
+                         warmup=args.warmup)
     t1 = time.perf_counter()
     loss = net.train(args.train_iters, batch=64, lr=1e-3, seed=1)
     t2 = time.perf_counter()
@@ -161,12 +166,14 @@ def train_net(L, args):
 def roofline(L, cases, mode, args, res):
     """Dominant kernel of a timed run: algorithmic bytes per launch (the cases
     still running) over its mean launch time."""
-    launches = {"dt": 1, "cont_energy": 3, "temp_grad": 3, "momentum": 3, "poisson": 3, "ucalc": 3, "transport": 3}
+    launches = {"dt": 1, "cont_energy": 3, "temp_grad": 3, "momentum": 3, "poisson": 3, "ucalc": 3, "transport": 3,
+                "warm_start": 3}
     tim = dict(res["timings"])
     if mode >= L.POISSON_MG64:
         tim["poisson"] = tim.get("poisson_mg", 0.0)
     per = {k: tim.get(k, 0.0) / (args.steps * nl) for k, nl in launches.items()}   # ms per launch
-    dom = max(per, key=lambda k: per[k])
+    # the warm-start slot (projection, network) is reported, not rated
+    dom = max((k for k in per if k != "warm_start"), key=lambda k: per[k])
     vc = res.get("vcycles_per_solve", 0.0)
     live = [c for c, g in zip(cases, res["ok"]) if g]
     bytes_launch = sum(kernel_bytes(c, dom, mode, vc) for c in live)
@@ -183,7 +190,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", choices=["large", "batch", "deck"], default="batch")
     ap.add_argument("--batch", type=int, default=4096)
-    ap.add_argument("--mode", choices=["learned", "direct", "thomas", "mg64", "mixed"], default="learned")
+    ap.add_argument("--mode", choices=["learned", "projected", "direct", "thomas", "mg64", "mixed"], default="learned")
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--threads", type=int, default=256)
     ap.add_argument("--seed", type=int, default=0)
@@ -200,9 +207,9 @@ def main():
               "independent, so more GPUs would shard the batch by case with no exchange.", file=sys.stderr)
         sys.exit(2)
     if args.steps is None:
-        args.steps = 40 if args.workload == "batch" else 200
+        args.steps = 60 if args.workload == "batch" else 200
     if args.warmup is None:
-        args.warmup = 3 if args.workload == "batch" else 10
+        args.warmup = 20
 
     import numpy as np
     import lica_boiling1d_amd as L
@@ -217,8 +224,8 @@ def main():
         cases = L.batched_cases(args.batch)
         wl = {"workload": "batched cases", "instances": args.batch, "n": "U{128..4096}", "r": 2,
               "superheat_K": "U(2,20)", "film_mm": "U(0.05,0.5)", "seed": 20261202}
-    mode = {"learned": L.POISSON_LEARNED, "direct": L.POISSON_DIRECT, "thomas": L.POISSON_THOMAS,
-            "mg64": L.POISSON_MG64, "mixed": L.POISSON_MIXED}[args.mode]
+    mode = {"learned": L.POISSON_LEARNED, "projected": L.POISSON_PROJECTED, "direct": L.POISSON_DIRECT,
+            "thomas": L.POISSON_THOMAS, "mg64": L.POISSON_MG64, "mixed": L.POISSON_MIXED}[args.mode]
     wl["poisson"] = args.mode
     if mode >= L.POISSON_MG64:
         wl["poisson_backward_error"] = args.rtol
@@ -244,7 +251,8 @@ def main():
            "value": res["value"], "unit": "instance-steps/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": res["el"] * 1000.0 / args.steps, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None,
-           "dtype": "f64+f32+bf16" if mode == L.POISSON_LEARNED else "f64+f32" if mode == L.POISSON_MIXED else "f64",
+           "dtype": ("f64+f32+bf16" if mode == L.POISSON_LEARNED else
+                     "f64+f32" if mode in (L.POISSON_MIXED, L.POISSON_PROJECTED) else "f64"),
            "data": "synthetic", "config": wl, "cell_updates_per_s": res["cells"]}
     if "vcycles_per_solve" in res:
         out["vcycles_per_solve"] = res["vcycles_per_solve"]
@@ -256,6 +264,15 @@ def main():
         out["stopped_cases"] = {L.status_string(int(k)): int(v) for k, v in zip(codes, counts)}
     if train is not None:
         out["training"] = train
+    if mode == L.POISSON_LEARNED and not args.no_fp64:
+        say("timed run, mixed+projected (no network)")
+        pr = timed_run(L, cases, L.POISSON_PROJECTED, args)
+        out["mixed_projected"] = {"value": pr["value"], "ms_per_step": pr["el"] * 1000.0 / args.steps,
+                                  "vcycles_per_solve": pr.get("vcycles_per_solve"),
+                                  "poisson_backward_error_max": pr.get("resid_max"),
+                                  "kernel_ms_per_launch": roofline(L, cases, L.POISSON_PROJECTED, args, pr)[1]}
+        out["network_gain_vs_projected"] = res["value"] / pr["value"]
+        out["vcycles_saved_by_network_per_solve"] = pr.get("vcycles_per_solve", 0.0) - res.get("vcycles_per_solve", 0.0)
     if mode >= L.POISSON_MIXED and not args.no_fp64:
         say("timed run, plain FP64 V-cycles from zero")
         ref = timed_run(L, cases, L.POISSON_MG64, args)

@@ -77,7 +77,8 @@ enum {
     LICA_POISSON_DIRECT = 1,    /* parallel partitioned tridiagonal (FP64)      */
     LICA_POISSON_MG64 = 2,      /* FP64 V-cycles from zero to tolerance          */
     LICA_POISSON_MIXED = 3,     /* FP32 V-cycles inside FP64 defect correction   */
-    LICA_POISSON_LEARNED = 4    /* MIXED with the conv-net initial guess         */
+    LICA_POISSON_LEARNED = 4,   /* MIXED from the projection + conv-net guess    */
+    LICA_POISSON_PROJECTED = 5  /* MIXED from the history projection alone       */
 };
 
 /* field ids for lica_batch_get_field: NS fields have n+2 entries (0..N3),
@@ -135,13 +136,17 @@ int lica_batch_load(const char *path, lica_batch **out);
 /* WRITEFIELD text file (1d.f:1121-1232, read by post.f READFLD 238-401) of one case */
 int lica_batch_write_fld(lica_batch *b, int64_t inst, const char *path, int64_t ihist);
 
-/* ---- learned warm start of the Poisson solve -------------------------- */
-/* No counterpart in the reference: a 1-D convolutional network (4 layers, 21
- * taps, 32 channels, BF16 weights, FP32 accumulation on MFMA) maps per-cell
- * features of a POISSON system (g/max|g|, a/max|b|, c/max|b|, Q/max|Q| with
- * Q the last solution: P before the solve plus the p(1) that UCALC removed,
- * 1d.f:3433-3470) to (p - Q)/max|Q|; LICA_POISSON_LEARNED starts the
- * mixed-precision V-cycles from Q + max|Q| * network.                      */
+/* ---- warm start of the Poisson solve ----------------------------------- */
+/* No counterpart in the reference.  LICA_POISSON_PROJECTED starts the
+ * mixed-precision V-cycles from B, the combination of the case's solutions at
+ * the same substep in the last two steps and its last solution that
+ * minimises |g - A B|_2.  LICA_POISSON_LEARNED adds a 1-D convolutional
+ * network's correction (4 layers, 21 taps, 32 channels, BF16 weights, FP32
+ * accumulation on MFMA): from per-cell features (r/max|r|, a/max|b|,
+ * b/max|b|, c/max|b| with r = g - A B) over 256 rows around the interface
+ * it gives a direction d, and the guess is B + alpha d with
+ * alpha = <A d, r>/<A d, A d>; cases whose B meets the tolerance skip it.
+ * The V-cycles always run to the tolerance.                                */
 typedef struct lica_net lica_net;
 int64_t lica_net_param_count(void);
 int lica_net_create(uint64_t seed, lica_net **out);
@@ -157,8 +162,9 @@ int lica_net_grad(lica_net *net, const float *feat, const float *target, const i
 /* training windows kept on the GPU ([window][cell][4] features, [window][cell]
  * targets); set_data replaces them, add_pairs appends windows_per_case
  * windows of len cells from every healthy case of a batch run with capture
- * on in a multigrid mode (features of its last POISSON system, target
- * (p - Q)/max|Q|), cases shorter than len are skipped                     */
+ * on in a multigrid mode (features of the captured system and starting
+ * point B around the interface, target p - B scaled to unit maximum over
+ * the window), cases shorter than len are skipped                         */
 int lica_net_set_data(lica_net *net, const float *feat, const float *target, int64_t n, int64_t len);
 int lica_net_add_pairs(lica_net *net, lica_batch *b, int64_t windows_per_case, int64_t len, uint64_t seed,
                        int64_t *added);
@@ -171,8 +177,9 @@ int lica_net_train(lica_net *net, int64_t iters, int64_t batch, double lr, uint6
 int lica_batch_set_mg_lds(lica_batch *b, int64_t bytes);
 /* the network a batch uses in LICA_POISSON_LEARNED (not owned by the batch) */
 int lica_batch_set_net(lica_batch *b, lica_net *net);
-/* keep, for every case, the last solution Q before substep msub's POISSON
- * solve (1..3; 0 switches capture off), the new solution and the
+/* keep, for every case, the starting point of substep msub's POISSON solve
+ * (1..3; 0 switches capture off) -- the warm-start base B in the PROJECTED
+ * and LEARNED modes, else the last solution -- the new solution and the
  * right-hand side (fields LICA_F_PPREV, LICA_F_PSOL, LICA_F_PRHS)          */
 int lica_batch_set_capture(lica_batch *b, int64_t on);
 

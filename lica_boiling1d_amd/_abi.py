@@ -27,7 +27,7 @@ HIP_ERROR = 30
 NO_DEVICE = 31
 NOT_CONVERGED = 32
 
-POISSON_THOMAS, POISSON_DIRECT, POISSON_MG64, POISSON_MIXED, POISSON_LEARNED = range(5)
+POISSON_THOMAS, POISSON_DIRECT, POISSON_MG64, POISSON_MIXED, POISSON_LEARNED, POISSON_PROJECTED = range(6)
 
 FIELDS = {"W": 0, "P": 1, "T": 2, "PSI_CN": 3, "PSI_ZN": 4, "PSIF": 5, "PHI": 6,
           "SFGAS": 7, "WO": 8, "BAND": 9, "ALPHI": 10, "MFLVS": 11,

@@ -28,12 +28,13 @@ int lica_emul_nthreads = 256;
 int lica_emul_depth = 0;
 typedef void *hipStream_t;
 // the warm-start network runs on MFMA only; the CPU logic-check build has none
-int lica_net_guess(lica_net *, const NetCase *, int, const int64_t *, double *, void *) { return LICA_UNSUPPORTED_DECK; }
+int lica_net_guess(lica_net *, const NetCase *, int, const int64_t *, double *, double, void *) { return LICA_UNSUPPORTED_DECK; }
 int lica_net_prepare(lica_net *, const int64_t *, int, const NetCase *) { return LICA_UNSUPPORTED_DECK; }
 int lica_net_append(lica_net *, const NetPairSrc *, int, int64_t, int64_t, uint64_t, int64_t *)
 {
     return LICA_UNSUPPORTED_DECK;
 }
+int64_t lica_net_window() { return 256; }
 extern "C" {
 int lica_net_data_size(lica_net *, int64_t *, int64_t *) { return LICA_UNSUPPORTED_DECK; }
 int lica_net_get_data(lica_net *, float *, float *) { return LICA_UNSUPPORTED_DECK; }
@@ -362,9 +363,10 @@ struct lica_batch {
     int *d_nsi = nullptr, *d_rfi = nullptr, *d_order = nullptr;
     double *d_mg = nullptr;      // multigrid scratch (allocated on first use of a multigrid mode)
     lica_net *net = nullptr;     // warm-start network of the learned mode (not owned)
-    double *d_guess = nullptr, *d_scale = nullptr, *d_cap = nullptr;
+    double *d_guess = nullptr, *d_scale = nullptr, *d_cap = nullptr, *d_hist = nullptr;
     NetCase *d_netcases = nullptr;
     std::vector<int64_t> lens;
+    std::vector<int64_t> net_lens;   // window of every case in the network's layout
     int capture = 0;             // substep whose POISSON solve is captured (0: none)
     bool cap_valid = false;      // the capture rows hold the last run's last substep
     long long mg_lds_cap = -1;   // dynamic LDS cap of the multigrid kernel (-1: by batch size)
@@ -422,6 +424,7 @@ BatchDev make_bd(const lica_batch *b, int msub)
     bd.mg_rtol = b->rtol;
     bd.mg_maxc = b->max_cycles;
     bd.guess = b->d_guess;
+    bd.hist = b->d_hist;
     bd.cap = b->capture ? b->d_cap : nullptr;
     bd.mg_lds = mg_lds(b);
     return bd;
@@ -448,15 +451,18 @@ int launch(lica_batch *b, int which, int msub)
     return rc == 0 ? 0 : LICA_HIP_ERROR;
 }
 
-// work between the POISSON matrix and its solve: the learned initial guess
-// and the capture of P before the last substep's solve
+// work between the POISSON matrix and its solve: the warm-start base, the
+// capture of the starting point, the network's correction
 int pre_poisson(lica_batch *b, int msub)
 {
     int rc = 0;
-    if (b->capture == msub) rc = launch(b, KL_CAPTURE_PRE, msub);
+    const bool warm = (b->poisson_mode == LICA_POISSON_LEARNED || b->poisson_mode == LICA_POISSON_PROJECTED);
+    if (warm) rc = launch(b, KL_WARM_BASE, msub);
+    if (!rc && b->capture == msub) rc = launch(b, KL_CAPTURE_PRE, msub);
     if (!rc && b->poisson_mode == LICA_POISSON_LEARNED) {
         if (!b->net) return LICA_BAD_ARG;
-        if (lica_net_guess(b->net, b->d_netcases, (int)b->ninst, b->lens.data(), b->d_scale, (void *)b->stream))
+        if (lica_net_guess(b->net, b->d_netcases, (int)b->ninst, b->net_lens.data(), b->d_scale, b->rtol,
+                           (void *)b->stream))
             rc = LICA_HIP_ERROR;
     }
     return rc;
@@ -709,7 +715,7 @@ int lica_batch_create(int64_t ninst, const lica_case_desc *cases, lica_batch **o
     rc = rc ? rc : dmalloc((void **)&b->d_order, sizeof(int) * (size_t)ninst);
     rc = rc ? rc : dmalloc((void **)&b->d_guess, sizeof(double) * (size_t)gs_total);
     rc = rc ? rc : dmalloc((void **)&b->d_cap, sizeof(double) * 3 * (size_t)gs_total);
-    rc = rc ? rc : dmalloc((void **)&b->d_scale, sizeof(double) * (size_t)ninst);
+    rc = rc ? rc : dmalloc((void **)&b->d_scale, sizeof(double) * 2 * (size_t)ninst);
     if (rc) { lica_batch_destroy(b); return rc; }
     // workgroup order: largest cases first (longest-processing-time first)
     std::vector<int> order((size_t)ninst);
@@ -797,12 +803,19 @@ int lica_batch_set_poisson(lica_batch *b, int64_t mode, double rtol, int64_t max
 {
     int rc = check_batch(b);
     if (rc) return rc;
-    if (mode < LICA_POISSON_THOMAS || mode > LICA_POISSON_LEARNED || !(rtol > 0.0) || max_cycles < 1) return LICA_BAD_ARG;
+    if (mode < LICA_POISSON_THOMAS || mode > LICA_POISSON_PROJECTED || !(rtol > 0.0) || max_cycles < 1) return LICA_BAD_ARG;
     if (mode == LICA_POISSON_LEARNED && !b->net) return LICA_BAD_ARG;
     if (mode >= LICA_POISSON_MG64 && !b->d_mg) {
         long long tot = 0;
         for (InstParam &p : b->par) { p.mg_off = tot; tot += lica_mg_scratch_doubles(p.n); }
         rc = dmalloc((void **)&b->d_mg, sizeof(double) * (size_t)tot);
+        rc = rc ? rc : h2d(b->d_par, b->par.data(), sizeof(InstParam) * b->par.size(), b->stream);
+        if (rc) return rc;
+    }
+    if ((mode == LICA_POISSON_LEARNED || mode == LICA_POISSON_PROJECTED) && !b->d_hist) {
+        long long tot = 0;
+        for (InstParam &p : b->par) { p.hist_off = tot; tot += 3LL * MG_HIST * p.n; }
+        rc = dmalloc((void **)&b->d_hist, sizeof(double) * (size_t)tot);
         rc = rc ? rc : h2d(b->d_par, b->par.data(), sizeof(InstParam) * b->par.size(), b->stream);
         if (rc) return rc;
     }
@@ -999,7 +1012,7 @@ void lica_batch_destroy(lica_batch *b)
 #endif
     dfree(b->d_par); dfree(b->d_st); dfree(b->d_ns); dfree(b->d_rf);
     dfree(b->d_nsi); dfree(b->d_rfi); dfree(b->d_order); dfree(b->d_mg);
-    dfree(b->d_guess); dfree(b->d_scale); dfree(b->d_cap); dfree(b->d_netcases);
+    dfree(b->d_guess); dfree(b->d_scale); dfree(b->d_cap); dfree(b->d_netcases); dfree(b->d_hist);
 #ifndef LICA_EMUL
     if (b->stream) (void)hipStreamDestroy(b->stream);
 #endif
@@ -1012,6 +1025,7 @@ int lica_batch_set_net(lica_batch *b, lica_net *net)
     if (rc) return rc;
     if (!net) return LICA_BAD_ARG;
     std::vector<NetCase> nc((size_t)b->ninst);
+    std::vector<int64_t> wl((size_t)b->ninst);
     long long off = 0;
     for (int64_t i = 0; i < b->ninst; i++) {
         const InstParam &p = b->par[(size_t)i];
@@ -1019,16 +1033,20 @@ int lica_batch_set_net(lica_batch *b, lica_net *net)
         NetCase &c = nc[(size_t)i];
         c.a = base + (size_t)NS_PA * p.ln; c.b = base + (size_t)NS_PB * p.ln;
         c.c = base + (size_t)NS_PC * p.ln; c.g = base + (size_t)NS_PG * p.ln;
-        c.p = base + (size_t)NS_P * p.ln;
-        c.pshift = &b->d_st[i].pshift;
         c.guess = b->d_guess + p.gs_off;
-        c.off = off; c.n = p.n;
-        off += p.n;
+        c.place = &b->d_st[i].place;
+        c.dzi = 1.0 / ((double)p.r * p.sdzf);
+        c.n = p.n;
+        c.win = (int)std::min<int64_t>(p.n, lica_net_window());
+        c.off = off;
+        wl[(size_t)i] = c.win;
+        off += c.win;
     }
     if (!b->d_netcases) rc = dmalloc((void **)&b->d_netcases, sizeof(NetCase) * nc.size());
     rc = rc ? rc : h2d(b->d_netcases, nc.data(), sizeof(NetCase) * nc.size(), b->stream);
     if (rc) return rc;
-    if (lica_net_prepare(net, b->lens.data(), (int)b->ninst, b->d_netcases)) return LICA_HIP_ERROR;
+    b->net_lens = wl;
+    if (lica_net_prepare(net, b->net_lens.data(), (int)b->ninst, b->d_netcases)) return LICA_HIP_ERROR;
     b->net = net;
     return LICA_OK;
 }
@@ -1053,12 +1071,12 @@ int lica_net_add_pairs(lica_net *net, lica_batch *b, int64_t windows_per_case, i
     if (rc) return rc;
     if (!net || windows_per_case < 1 || len < 1) return LICA_BAD_ARG;
     if (!b->capture || !b->cap_valid || b->poisson_mode < LICA_POISSON_MG64) return LICA_BAD_ARG;
-    std::vector<int64_t> st((size_t)b->ninst);
-    rc = lica_batch_get_progress(b, st.data(), nullptr);
+    std::vector<InstState> st((size_t)b->ninst);
+    rc = d2h(st.data(), b->d_st, sizeof(InstState) * st.size(), b->stream);
     if (rc) return rc;
     std::vector<NetPairSrc> src;
     for (int64_t i = 0; i < b->ninst; i++) {
-        if (st[(size_t)i] != 0) continue;
+        if (st[(size_t)i].status != 0) continue;
         const InstParam &p = b->par[(size_t)i];
         const double *base = b->d_ns + p.ns_off + 2;
         const double *cap = b->d_cap + 3 * p.gs_off;
@@ -1066,6 +1084,7 @@ int lica_net_add_pairs(lica_net *net, lica_batch *b, int64_t windows_per_case, i
         s.a = base + (size_t)NS_PA * p.ln; s.b = base + (size_t)NS_PB * p.ln; s.c = base + (size_t)NS_PC * p.ln;
         s.p0 = cap; s.p1 = cap + (p.n + 2); s.g = cap + 2 * (p.n + 2);
         s.n = p.n;
+        s.kif = (int)(st[(size_t)i].place / ((double)p.r * p.sdzf)) + 1;
         src.push_back(s);
     }
     return lica_net_append(net, src.data(), (int)src.size(), windows_per_case, len, seed, added);

@@ -162,9 +162,22 @@ DEV void body_poisson_mg(const BatchDev &bd, int inst, MgSh &sh)
     const int n = P.n, n3 = P.n3;
     double *PP = cp.A(NS_P);
     const double *PA = cp.A(NS_PA), *PB = cp.A(NS_PB), *PCc = cp.A(NS_PC), *PG = cp.A(NS_PG);
-    const double *x0 = (bd.pmode == LICA_MG_LEARNED_MODE && bd.guess) ? bd.guess + P.gs_off : nullptr;
+    const bool warm = LICA_MG_WARM(bd.pmode) && bd.guess && bd.hist;
+    const double *x0 = warm ? bd.guess + P.gs_off : nullptr;
     double res = 0.0;
     int cyc = mg_solve(PA, PB, PCc, PG, PP, n, bd.mg + P.mg_off, bd.pmode, bd.mg_rtol, bd.mg_maxc, x0, sh, &res);
+    if (warm) {
+        // the solution joins this substep's history ring
+        const int s = bd.msub - 1, head = S.hist_head[s];
+        double *Hs = bd.hist + P.hist_off + ((size_t)s * MG_HIST + (size_t)head) * (size_t)n;
+        PFOR(i, 0, n) { Hs[i] = PP[i + 1]; }
+        PFOR_END
+        SERIAL {
+            S.hist_head[s] = (head + 1) % MG_HIST;
+            S.hist_cnt[s] = (S.hist_cnt[s] < MG_HIST) ? S.hist_cnt[s] + 1 : MG_HIST;
+        }
+        SERIAL_END
+    }
     SERIAL {
         PP[0] = PP[1];
         PP[n3] = PP[n];
@@ -177,8 +190,9 @@ DEV void body_poisson_mg(const BatchDev &bd, int inst, MgSh &sh)
 }
 
 // capture rows of every case (3 x (n+2) at 3*gs_off): before the POISSON
-// solve (which 0) the last solution P + pshift and the right-hand side,
-// after it (which 1) the new solution
+// solve (which 0) the right-hand side and the starting point -- the
+// warm-start base in the warm-started modes, else the last solution
+// P + pshift -- after it (which 1) the new solution
 DEV void body_capture(const BatchDev &bd, int inst, int which)
 {
     InstState &S = bd.st[inst];
@@ -190,9 +204,99 @@ DEV void body_capture(const BatchDev &bd, int inst, int which)
     double *dst = base + (long long)which * (P.n + 2);
     double *rhs = base + 2LL * (P.n + 2);
     const double sh0 = (which == 0) ? S.pshift : 0.0;
+    const double *start = (which == 0 && LICA_MG_WARM(bd.pmode) && bd.guess) ? bd.guess + P.gs_off : nullptr;
     PFOR(k, 1, P.n + 1) {
-        dst[k] = PP[k] + sh0;
+        dst[k] = start ? start[k] : PP[k] + sh0;
         if (which == 0) rhs[k] = PG[k];
+    }
+    PFOR_END
+}
+
+// Warm start of the multigrid solve (LICA_POISSON_PROJECTED, _LEARNED): the
+// POISSON solution of a substep differs from the one of the same substep a
+// step earlier by a slowly drifting factor and a small local change, so the
+// base is the combination B = sum_j c_j v_j of the case's solutions at this
+// substep in the last MG_HIST steps and the last solution Q = P + pshift
+// that minimises |g - A B|_2 (FP64 normal equations, Cholesky with
+// vectors that add nothing new dropped).  B goes to the guess rows.
+struct WarmSh {
+    double red[16 * 9];
+    double coef[MG_HIST + 1];
+};
+
+DEV void body_warm_base(const BatchDev &bd, int inst, WarmSh &sh)
+{
+    InstState &S = bd.st[inst];
+    if (S.status || !bd.hist || !bd.guess) return;
+    CaseP cp = case_ptrs(bd, inst);
+    const InstParam &P = *cp.P;
+    const int n = P.n, s = bd.msub - 1;
+    const double *__restrict__ PA = cp.A(NS_PA), *__restrict__ PB = cp.A(NS_PB), *__restrict__ PCc = cp.A(NS_PC);
+    const double *__restrict__ PG = cp.A(NS_PG), *__restrict__ PP = cp.A(NS_P);
+    double *B = bd.guess + P.gs_off;
+    const double *Hs = bd.hist + P.hist_off + (size_t)s * MG_HIST * (size_t)n;
+    const int cnt = S.hist_cnt[s], head = S.hist_head[s];
+    const double pq = S.pshift;
+    const int m = cnt + 1;   // history vectors (most recent first), then Q
+    auto val = [&](int j, int row) -> double {
+        if (row < 1 || row > n) return 0.0;
+        if (j < cnt) return Hs[(size_t)((head - 1 - j + 2 * MG_HIST) % MG_HIST) * n + (row - 1)];
+        return PP[row] + pq;
+    };
+    auto aval = [&](int j, int row) -> double {   // (A v_j)(row), rows 1..n of the POISSON operator
+        double v = PB[row] * val(j, row);
+        if (row > 1) v = v + PA[row] * val(j, row - 1);
+        if (row < n) v = v + PCc[row] * val(j, row + 1);
+        return v;
+    };
+    // G_ij = <A v_i, A v_j>, h_i = <A v_i, g>
+    double sums[9];
+    red_sumN<9>(1, n + 1, [&](int row, double *v) {
+        double w[MG_HIST + 1];
+        for (int j = 0; j < MG_HIST + 1; j++) w[j] = (j < m) ? aval(j, row) : 0.0;
+        const double g = PG[row];
+        v[0] = w[0] * w[0]; v[1] = w[0] * w[1]; v[2] = w[0] * w[2];
+        v[3] = w[1] * w[1]; v[4] = w[1] * w[2]; v[5] = w[2] * w[2];
+        v[6] = w[0] * g; v[7] = w[1] * g; v[8] = w[2] * g;
+    }, sh.red, sums);
+    SERIAL {
+        double G[3][3] = {{sums[0], sums[1], sums[2]}, {sums[1], sums[3], sums[4]}, {sums[2], sums[4], sums[5]}};
+        const double h[3] = {sums[6], sums[7], sums[8]};
+        double Lm[3][3] = {{0.0}}, y[3] = {0.0}, c[3] = {0.0};
+        int idx[3], nk = 0;
+        for (int j = 0; j < m; j++) {
+            double row[3] = {0.0};
+            for (int t = 0; t < nk; t++) {
+                double v = G[j][idx[t]];
+                for (int u = 0; u < t; u++) v = v - row[u] * Lm[t][u];
+                row[t] = v / Lm[t][t];
+            }
+            double d = G[j][j];
+            for (int t = 0; t < nk; t++) d = d - row[t] * row[t];
+            if (G[j][j] > 0.0 && d > 1.0e-12 * G[j][j]) {
+                for (int t = 0; t < nk; t++) Lm[nk][t] = row[t];
+                Lm[nk][nk] = sqrt(d);
+                idx[nk++] = j;
+            }
+        }
+        for (int t = 0; t < nk; t++) {
+            double v = h[idx[t]];
+            for (int u = 0; u < t; u++) v = v - Lm[t][u] * y[u];
+            y[t] = v / Lm[t][t];
+        }
+        for (int t = nk - 1; t >= 0; t--) {
+            double v = y[t];
+            for (int u = t + 1; u < nk; u++) v = v - Lm[u][t] * c[u];
+            c[t] = v / Lm[t][t];
+        }
+        for (int j = 0; j < MG_HIST + 1; j++) sh.coef[j] = 0.0;
+        for (int t = 0; t < nk; t++) sh.coef[idx[t]] = c[t];
+    }
+    SERIAL_END
+    PFOR(row, 1, n + 1) {
+        double v = 0.0;
+        for (int j = 0; j < m; j++) v = v + sh.coef[j] * val(j, row);
+        B[row] = v;
     }
     PFOR_END
 }

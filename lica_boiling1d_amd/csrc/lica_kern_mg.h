@@ -8,7 +8,8 @@
 // equation from a zero start in
 //   MG64:    FP64 (IOLDV=0, lica.in:8: the iteration starts from x = 0),
 //   MIXED:   FP32 (matrices, smoother, transfers; the coarsest solve in FP64),
-//   LEARNED: FP32, the iteration started from the network's guess,
+//   LEARNED: FP32, the iteration started from the warm-start network's guess,
+//   PROJECTED: FP32, started from the projection on the solution history,
 // until the normwise backward error max|g - A x| / (||A|| max|x| + max|g|)
 // reaches rtol.  The solution replaces P(1:N) as the direct solve does.
 //
@@ -291,42 +292,59 @@ DEV void mg_update(const MgFine &F, const MgLev<T> &Z, bool first, const double 
     }, sh.red, rmax, xmax);
 }
 
+// residual of the starting point; when it already meets rtol the solve ends
+// there, before any level is built
 template <class T>
-DEV int mg_iterate(const MgFine &F, double *scr, int n, double rtol, int maxc, double fmax, double anorm, MgSh &sh,
-                   double *p, double *res_out)
+DEV double mg_start(const MgFine &F, int n, double fmax, double anorm, bool warm, MgSh &sh, double **cur, double **oth,
+                    double *scr)
 {
     MgLev<T> Z = mg_lev<T>(scr, n, 0, sh);
-    double *cur = F.x0, *oth = F.x1;
     double rmax, xmax;
-    mg_update<T>(F, Z, true, cur, oth, sh, &rmax, &xmax);
-    { double *t = cur; cur = oth; oth = t; }
-    double den = anorm * xmax + fmax;
-    double res = (den > 0.0) ? rmax / den : rmax;
+    mg_update<T>(F, Z, true, *cur, *oth, sh, &rmax, &xmax);
+    { double *t = *cur; *cur = *oth; *oth = t; }
+    if (warm && rmax > fmax) {
+        // a starting guess with a larger residual than zero's is dropped
+        PFOR(i, 0, n) { (*cur)[i] = 0.0; }
+        PFOR_END
+        mg_update<T>(F, Z, true, *cur, *oth, sh, &rmax, &xmax);
+        { double *t = *cur; *cur = *oth; *oth = t; }
+    }
+    const double den = anorm * xmax + fmax;
+    return (den > 0.0) ? rmax / den : rmax;
+}
+
+// V-cycles until rtol; *cur is the iterate on return (the update alternates
+// the two buffers)
+template <class T>
+DEV int mg_iterate(const MgFine &F, double *scr, int n, double rtol, int maxc, double fmax, double anorm, MgSh &sh,
+                   double **cur, double **oth, double res, double *res_out)
+{
+    MgLev<T> Z = mg_lev<T>(scr, n, 0, sh);
+    double rmax, xmax;
     int cyc = 0;
     while (res > rtol && cyc < maxc) {
         mg_vcycle<T>(scr, n, sh);
-        mg_update<T>(F, Z, false, cur, oth, sh, &rmax, &xmax);
-        { double *t = cur; cur = oth; oth = t; }
+        mg_update<T>(F, Z, false, *cur, *oth, sh, &rmax, &xmax);
+        { double *t = *cur; *cur = *oth; *oth = t; }
         cyc++;
-        den = anorm * xmax + fmax;
+        const double den = anorm * xmax + fmax;
         res = (den > 0.0) ? rmax / den : rmax;
     }
-    PFOR(i, 0, n) { p[i + 1] = cur[i]; }
-    PFOR_END
     *res_out = res;
     return (res > rtol) ? -1 : cyc;
 }
 
 // Solve rows 1..n of a, b, c, g (a(1), c(n) ignored) into p(1..n).
-// mode 2: FP64 cycles; 3/4: FP32 cycles (x0 holds the initial guess for mode
-// 4).  Returns the cycles, or -1 when maxc cycles did not reach rtol; *res
+// mode 2: FP64 cycles; 3-5: FP32 cycles (x0 holds the initial guess for
+// modes 4 and 5).  The levels are built only when the starting point misses
+// rtol.  Returns the cycles, or -1 when maxc cycles did not reach rtol; *res
 // the final backward error.
 DEV int mg_solve(const double *a, const double *b, const double *c, const double *g, double *p, int n,
                  double *scr, int mode, double rtol, int maxc, const double *x0, MgSh &sh, double *res_out)
 {
     const int L = mg_levels(n);
     const bool f32 = (mode != LICA_MG_FP64_MODE);
-    const bool guess = (mode == LICA_MG_LEARNED_MODE && x0);
+    const bool guess = (LICA_MG_WARM(mode) && x0);
     MgFine F = mg_fine(scr, n);
     MgLev<float> Z32 = mg_lev<float>(scr, n, 0, sh);
     float *a32 = (float *)Z32.a;
@@ -342,10 +360,20 @@ DEV int mg_solve(const double *a, const double *b, const double *c, const double
         fm = fabs(g[i + 1]);
         an = fabs(ai) + fabs(bi) + fabs(ci);
     }, sh.red, &fmax, &anorm);
-    for (int l = 0; l + 1 < L; l++)
-        mg_coarsen(mg_lev<double>(scr, n, l, sh), mg_lev<double>(scr, n, l + 1, sh),
-                   (float *)mg_lev<float>(scr, n, l + 1, sh).a, f32);
-    mg_coarsest_factor(mg_lev<double>(scr, n, L - 1, sh), sh);
-    if (f32) return mg_iterate<float>(F, scr, n, rtol, maxc, fmax, anorm, sh, p, res_out);
-    return mg_iterate<double>(F, scr, n, rtol, maxc, fmax, anorm, sh, p, res_out);
+    double *cur = F.x0, *oth = F.x1, res;
+    if (f32) res = mg_start<float>(F, n, fmax, anorm, guess, sh, &cur, &oth, scr);
+    else res = mg_start<double>(F, n, fmax, anorm, guess, sh, &cur, &oth, scr);
+    int cyc = 0;
+    if (res > rtol) {
+        for (int l = 0; l + 1 < L; l++)
+            mg_coarsen(mg_lev<double>(scr, n, l, sh), mg_lev<double>(scr, n, l + 1, sh),
+                       (float *)mg_lev<float>(scr, n, l + 1, sh).a, f32);
+        mg_coarsest_factor(mg_lev<double>(scr, n, L - 1, sh), sh);
+        if (f32) cyc = mg_iterate<float>(F, scr, n, rtol, maxc, fmax, anorm, sh, &cur, &oth, res, &res);
+        else cyc = mg_iterate<double>(F, scr, n, rtol, maxc, fmax, anorm, sh, &cur, &oth, res, &res);
+    }
+    PFOR(i, 0, n) { p[i + 1] = cur[i]; }
+    PFOR_END
+    *res_out = res;
+    return (res > rtol) ? -1 : cyc;
 }

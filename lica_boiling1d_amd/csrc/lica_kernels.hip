@@ -107,6 +107,12 @@ int lica_launch_mg(int nsys, int n, const double *a, const double *b, const doub
     return (int)hipGetLastError();
 }
 
+__global__ void __launch_bounds__(256) k_warm_base(BatchDev bd)
+{
+    __shared__ WarmSh sh;
+    body_warm_base(bd, bd.order[blockIdx.x], sh);
+}
+
 __global__ void __launch_bounds__(1024) k_capture(BatchDev bd, int which)
 {
     body_capture(bd, bd.order[blockIdx.x], which);
@@ -186,6 +192,7 @@ int lica_launch(int which, const BatchDev &bd, int threads, void *stream)
     case KL_CAPTURE_POST:
         hipLaunchKernelGGL(k_capture, dim3(bd.ninst), dim3(256), 0, (hipStream_t)stream, bd, 1);
         break;
+    case KL_WARM_BASE: LAUNCH(k_warm_base, 256); break;
     default: return -1;
     }
     return (int)hipGetLastError();
@@ -293,6 +300,7 @@ int lica_launch(int which, const BatchDev &bd, int threads, void *stream)
     case KL_CAPTURE_POST:
         for (int b = 0; b < bd.ninst; b++) body_capture(bd, bd.order[b], 1);
         break;
+    case KL_WARM_BASE: emul_run<WarmSh>(bd, 256, body_warm_base); break;
     default: return -1;
     }
     return 0;

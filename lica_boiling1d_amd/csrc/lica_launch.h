@@ -5,7 +5,8 @@
 
 enum KernelId {
     KL_INIT = 0, KL_DT, KL_CONT_ENERGY, KL_TEMP_GRAD, KL_MOMENTUM,
-    KL_POISSON_DIRECT, KL_UCALC, KL_TRANSPORT, KL_POISSON_MG, KL_CAPTURE_PRE, KL_CAPTURE_POST, KL_COUNT
+    KL_POISSON_DIRECT, KL_UCALC, KL_TRANSPORT, KL_POISSON_MG, KL_CAPTURE_PRE, KL_CAPTURE_POST, KL_WARM_BASE,
+    KL_COUNT
 };
 
 int lica_launch(int which, const BatchDev &bd, int threads, void *stream);
@@ -19,22 +20,26 @@ long long lica_mg_lds_max();                   // largest dynamic LDS of a multi
 
 // warm-start network (lica_net.hip): one POISSON system per case
 struct NetCase {
-    const double *a, *b, *c, *g, *p;   // rows 1..n of the case (index k)
-    const double *pshift;              // P + *pshift is the case's last solution
-    double *guess;                     // rows 1..n: receives P + s_p * network
-    long long off;                     // first cell of the case in the activation arena
-    int n;
+    const double *a, *b, *c, *g;       // rows 1..n of the case's POISSON system (index k)
+    double *guess;                     // rows 1..n: the warm-start base B, corrected by s * network
+    const double *place;               // interface height of the case (its state)
+    double dzi;                        // 1 / NS cell height (uniform grid)
+    long long off;                     // first cell of the case's window in the activation arena
+    int n, win;                        // rows; cells of the corrected window around the interface
 };
 // one case's captured POISSON system (rows 1..n): a, b, c, right-hand side,
-// P before (p0) and after (p1) the solve
+// the starting point (p0) and the solution (p1)
 struct NetPairSrc {
     const double *a, *b, *c, *g, *p0, *p1;
     int n;
+    int kif;     // NS row of the interface (windows are centred there, with jitter)
 };
 struct lica_net;
 // append up to wpc windows of len cells of every listed case to the network's
 // training set (features as k_net_features, target (p1 - p0) / max|p0|)
 int lica_net_append(lica_net *t, const NetPairSrc *hsrc, int ncase, int64_t wpc, int64_t len, uint64_t seed,
                     int64_t *added);
-int lica_net_guess(lica_net *t, const NetCase *dcases, int ncase, const int64_t *lens, double *dscale, void *stream);
+int64_t lica_net_window();   // cells of the corrected window around the interface
+int lica_net_guess(lica_net *t, const NetCase *dcases, int ncase, const int64_t *lens, double *dinfo, double rtol,
+                   void *stream);
 int lica_net_prepare(lica_net *t, const int64_t *lens, int ncase, const NetCase *dcases);

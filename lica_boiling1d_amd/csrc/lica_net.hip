@@ -1,9 +1,9 @@
 // Convolutional warm start of the POISSON solve (gfx950, BF16 MFMA).
 //
-// The network maps per-cell features of a POISSON system (1d.f:3529-3543) --
-// the right-hand side, the two off-diagonal coefficients and the previous
-// substep's pressure, each scaled per system -- to a correction of the
-// previous pressure; the multigrid solve then starts from that guess.
+// The network maps per-cell features of a POISSON system (1d.f:3529-3543)
+// and of the warm-start base B -- B's residual and the three diagonals, each
+// scaled per system -- to a correction of B; the multigrid solve then starts
+// from the corrected guess.
 // Layers: NET_LAYERS 1-D convolutions of NET_TAPS taps over the cell index,
 // NET_C channels (features padded to NET_C), ReLU between layers, one output
 // channel.  Weights and activations are stored in BF16, every product is
@@ -27,7 +27,8 @@
 #define NET_TAPS 21
 #define NET_HALF (NET_TAPS / 2)
 #define NET_LAYERS 4
-#define NET_TILE 64          // cells per workgroup in the convolution kernels
+#define NET_TILE 256         // cells per workgroup in the convolution kernels
+#define NET_WIN 256          // cells around the interface the warm-start network corrects
 #define NET_NF 4             // input features per cell
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -44,19 +45,23 @@ struct NetSeq {
 
 // ---------------------------------------------------------------------
 // Y[cell][co] = act(bias[co] + sum_t sum_ci W[t][co][ci] X[cell + t - HALF][ci])
-// grid: one workgroup per tile (tile -> sequence via seq_of_tile), 256 threads.
-// cout is 32 or 16 (the last layer, one meaningful channel); relu 0/1;
+// grid: one workgroup per tile of NET_TILE cells (tile -> sequence via
+// seq_of_tile), 256 threads: wave w computes rows 64w..64w+63 of the tile as
+// four 16-row blocks, so one staging of the weights serves 256 cells.
+// COUT is 32, or 16 for the last layer (one meaningful channel); relu 0/1;
 // yf (optional) receives the FP32 pre-activation of channel 0.
+template <int COUT>
 __global__ void __launch_bounds__(256) k_net_conv(const __bf16 *__restrict__ X, const __bf16 *__restrict__ W,
                                                   const float *__restrict__ bias, __bf16 *__restrict__ Y,
                                                   float *__restrict__ yf, const NetSeq *__restrict__ seqs,
-                                                  const int *__restrict__ seq_of_tile, int cout, int relu)
+                                                  const int *__restrict__ seq_of_tile, int relu)
 {
     __shared__ __attribute__((aligned(16))) __bf16 xs[(NET_TILE + NET_TAPS) * NET_C];
-    __shared__ __attribute__((aligned(16))) __bf16 ws[NET_TAPS * NET_C * NET_C];
+    __shared__ __attribute__((aligned(16))) __bf16 ws[NET_TAPS * COUT * NET_C];
     const int tile = blockIdx.x;
     const NetSeq sq = seqs[seq_of_tile[tile]];
     const int c0 = (tile - sq.tile0) * NET_TILE;      // first cell of the tile in the sequence
+    if (c0 >= sq.len) return;                         // nothing of the sequence in this tile
     const int tid = threadIdx.x;
     // stage the input rows c0-HALF .. c0+TILE+HALF-1 (zero outside the sequence)
     for (int i = tid; i < (NET_TILE + NET_TAPS - 1) * NET_C / 8; i += blockDim.x) {
@@ -67,35 +72,46 @@ __global__ void __launch_bounds__(256) k_net_conv(const __bf16 *__restrict__ X, 
         else for (int j = 0; j < 8; j++) v[j] = to_bf16(0.0f);
         *(bf16x8 *)(xs + row * NET_C + part * 8) = v;
     }
-    for (int i = tid; i < NET_TAPS * cout * NET_C / 8; i += blockDim.x)
+    for (int i = tid; i < NET_TAPS * COUT * NET_C / 8; i += blockDim.x)
         *(bf16x8 *)(ws + i * 8) = *(const bf16x8 *)(W + i * 8);
     __syncthreads();
     const int wave = tid >> 6, lane = tid & 63;
     const int m = lane & 15, kq = lane >> 4;                  // operand row / k group
-    const int ntiles = cout / 16;
-    f32x4 acc[2];
-    for (int nt = 0; nt < 2; nt++) acc[nt] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
-    const int r0 = wave * 16;                                 // first tile row of this wave
+    constexpr int NT = COUT / 16;
+    f32x4 acc[4][NT];
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) acc[rb][nt] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+    const int r0 = wave * 64;                                 // first tile row of this wave
     for (int t = 0; t < NET_TAPS; t++) {
-        const bf16x8 a = *(const bf16x8 *)(xs + (r0 + m + t) * NET_C + kq * 8);
-        for (int nt = 0; nt < ntiles; nt++) {
-            const bf16x8 b = *(const bf16x8 *)(ws + (t * cout + nt * 16 + m) * NET_C + kq * 8);
-            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[nt], 0, 0, 0);
+        bf16x8 b[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) b[nt] = *(const bf16x8 *)(ws + (t * COUT + nt * 16 + m) * NET_C + kq * 8);
+#pragma unroll
+        for (int rb = 0; rb < 4; rb++) {
+            const bf16x8 a = *(const bf16x8 *)(xs + (r0 + rb * 16 + m + t) * NET_C + kq * 8);
+#pragma unroll
+            for (int nt = 0; nt < NT; nt++) acc[rb][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[nt], acc[rb][nt], 0, 0, 0);
         }
     }
-    // epilogue: lane holds col (l%16), rows 4*(l/16)+i
-    for (int nt = 0; nt < ntiles; nt++) {
-        const int co = nt * 16 + m;
-        for (int i = 0; i < 4; i++) {
-            const int cell = c0 + r0 + kq * 4 + i;
-            if (cell >= sq.len) continue;
-            float v = acc[nt][i] + bias[co];
-            if (yf && co == 0) yf[sq.off + cell] = v;
-            if (relu) v = fmaxf(v, 0.0f);
-            Y[(sq.off + cell) * NET_C + co] = to_bf16(v);
+    // epilogue: lane holds col (l%16), rows 4*(l/16)+i of each block
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) {
+            const int co = nt * 16 + m;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int cell = c0 + r0 + rb * 16 + kq * 4 + i;
+                if (cell >= sq.len) continue;
+                float v = acc[rb][nt][i] + bias[co];
+                if (yf && co == 0) yf[sq.off + cell] = v;
+                if (relu) v = fmaxf(v, 0.0f);
+                Y[(sq.off + cell) * NET_C + co] = to_bf16(v);
+            }
         }
-    }
-    if (cout == 16) {   // zero the padding channels 16..31 of the output rows
+    if (COUT == 16) {   // zero the padding channels 16..31 of the output rows
         for (int i = tid; i < NET_TILE * 16; i += blockDim.x) {
             const int row = i / 16, ch = 16 + i % 16;
             const int cell = c0 + row;
@@ -139,12 +155,9 @@ __global__ void __launch_bounds__(256) k_net_wgrad(const __bf16 *__restrict__ X,
             for (int nt = 0; nt < 2; nt++) {
                 f32x4 acc = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
                 for (int kb = 0; kb < NET_TILE; kb += 32) {
-                    bf16x8 a, b;
-                    for (int j = 0; j < 8; j++) {
-                        const int r = kb + kq * 8 + j;                       // cell of the tile
-                        a[j] = zt[(mt * 16 + m) * NET_TILE + r];
-                        b[j] = xs[(r + t) * NET_C + nt * 16 + m];
-                    }
+                    const bf16x8 a = *(const bf16x8 *)(zt + (mt * 16 + m) * NET_TILE + kb + kq * 8);
+                    bf16x8 b;
+                    for (int j = 0; j < 8; j++) b[j] = xs[(kb + kq * 8 + j + t) * NET_C + nt * 16 + m];
                     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
                 }
                 for (int i = 0; i < 4; i++) {
@@ -233,92 +246,158 @@ __global__ void k_net_adam(float *__restrict__ w, float *__restrict__ m, float *
 }
 
 // ---------------------------------------------------------------------
-// Features of the POISSON system of every case of a batch (one workgroup per
-// case): with Q = P + pshift the last solution (UCALC's shift of P undone),
-// scales s_g = max|g|, s_b = max|b|, s_q = max|Q| (1 where the maximum is 0),
-// features g/s_g, a/s_b, c/s_b, Q/s_q.
+// Features of a POISSON system and a starting point B (the warm-start base):
+// with r = g - A B the base's residual (rows 1..n of the POISSON operator:
+// a(1) and c(n) do not couple), s_r = max|r| and s_b = max|b| (1 where 0),
+// the features are r/s_r, a/s_b, b/s_b, c/s_b.  The network's output y over
+// a window of rows around the interface -- where the error left by the
+// projection is largest, as a and c jump with the density -- is a direction
+// d; its length comes from the residual: guess = B + alpha d with
+// alpha = <A d, r> / <A d, A d>, so a useless direction costs nothing.
+// Training targets are (p - B) scaled to unit maximum over the window.
 
-// the three scales over rows 1..n (256-thread workgroup); q = p + pq
-__device__ void net_scales(const double *g, const double *b, const double *p, double pq, int n, double *sg_out,
-                           double *sb_out, double *sp_out)
+__device__ __forceinline__ double net_res(const double *a, const double *b, const double *c, const double *g,
+                                          const double *x, int n, int k)
 {
-    __shared__ double red[3][8];
-    const int tid = threadIdx.x;
-    double sg = 0.0, sb = 0.0, sp = 0.0;
-    for (int k = 1 + tid; k <= n; k += blockDim.x) {
-        sg = fmax(sg, fabs(g[k]));
-        sb = fmax(sb, fabs(b[k]));
-        sp = fmax(sp, fabs(p[k] + pq));
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        sg = fmax(sg, __shfl_xor(sg, off, 64));
-        sb = fmax(sb, __shfl_xor(sb, off, 64));
-        sp = fmax(sp, __shfl_xor(sp, off, 64));
-    }
-    if ((tid & 63) == 0) { red[0][tid >> 6] = sg; red[1][tid >> 6] = sb; red[2][tid >> 6] = sp; }
-    __syncthreads();
-    sg = 0.0; sb = 0.0; sp = 0.0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); w++) { sg = fmax(sg, red[0][w]); sb = fmax(sb, red[1][w]); sp = fmax(sp, red[2][w]); }
-    *sg_out = (sg == 0.0) ? 1.0 : sg;
-    *sb_out = (sb == 0.0) ? 1.0 : sb;
-    *sp_out = (sp == 0.0) ? 1.0 : sp;
+    double v = b[k] * x[k];
+    if (k > 1) v = v + a[k] * x[k - 1];
+    if (k < n) v = v + c[k] * x[k + 1];
+    return g[k] - v;
 }
 
+// block reduction of NV maxima or sums (256-thread workgroup, all threads
+// get the result)
+template <int NV, bool SUM>
+__device__ void net_reduce(double *v)
+{
+    __shared__ double red[NV][8];
+    const int tid = threadIdx.x;
+    for (int off = 32; off > 0; off >>= 1)
+        for (int q = 0; q < NV; q++) {
+            const double o = __shfl_xor(v[q], off, 64);
+            v[q] = SUM ? v[q] + o : fmax(v[q], o);
+        }
+    if ((tid & 63) == 0)
+        for (int q = 0; q < NV; q++) red[q][tid >> 6] = v[q];
+    __syncthreads();
+    for (int q = 0; q < NV; q++) {
+        double r = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) r = SUM ? r + red[q][w] : fmax(r, red[q][w]);
+        v[q] = r;
+    }
+    __syncthreads();
+}
+
+// s_r, s_b over rows 1..n; with want_berr also the backward error of x,
+// max|r| / (max(|a|+|b|+|c|) max|x| + max|g|)
+__device__ void net_scales(const double *a, const double *b, const double *c, const double *g, const double *x, int n,
+                           double *sr_out, double *sb_out, double *berr_out)
+{
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};   // max|r|, max|b|, max(|a|+|b|+|c|), max|x|, max|g|
+    for (int k = 1 + (int)threadIdx.x; k <= n; k += blockDim.x) {
+        v[0] = fmax(v[0], fabs(net_res(a, b, c, g, x, n, k)));
+        v[1] = fmax(v[1], fabs(b[k]));
+        const double ak = (k > 1) ? fabs(a[k]) : 0.0, ck = (k < n) ? fabs(c[k]) : 0.0;
+        v[2] = fmax(v[2], ak + fabs(b[k]) + ck);
+        v[3] = fmax(v[3], fabs(x[k]));
+        v[4] = fmax(v[4], fabs(g[k]));
+    }
+    net_reduce<5, false>(v);
+    const double den = v[2] * v[3] + v[4];
+    if (berr_out) *berr_out = (den > 0.0) ? v[0] / den : v[0];
+    *sr_out = (v[0] == 0.0) ? 1.0 : v[0];
+    *sb_out = (v[1] == 0.0) ? 1.0 : v[1];
+}
+
+__device__ __forceinline__ void net_row(const double *a, const double *b, const double *c, const double *g,
+                                        const double *x, int n, int k, double sr, double sb, __bf16 *row)
+{
+    row[0] = to_bf16((float)(net_res(a, b, c, g, x, n, k) / sr));
+    row[1] = to_bf16((float)(a[k] / sb));
+    row[2] = to_bf16((float)(b[k] / sb));
+    row[3] = to_bf16((float)(c[k] / sb));
+}
+
+// first row of a case's window: cs.win rows centred on the interface
+__device__ __forceinline__ int net_k0(const NetCase &cs)
+{
+    const int kif = (int)(*cs.place * cs.dzi) + 1;
+    return min(max(kif - cs.win / 2, 1), cs.n - cs.win + 1);
+}
+
+// features of the window of every case of a batch (one workgroup per case).
+// A case whose base already meets rtol gets an empty sequence (its tiles do
+// no work) and keeps B; info[2i] = first row of the window.
 __global__ void __launch_bounds__(256) k_net_features(const NetCase *__restrict__ cases, __bf16 *__restrict__ X,
-                                                      double *__restrict__ scale)
+                                                      NetSeq *__restrict__ seqs, double *__restrict__ info,
+                                                      double rtol)
 {
     const NetCase cs = cases[blockIdx.x];
     const int tid = threadIdx.x;
-    const double pq = *cs.pshift;
-    double sg, sb, sp;
-    net_scales(cs.g, cs.b, cs.p, pq, cs.n, &sg, &sb, &sp);
-    for (int k = 1 + tid; k <= cs.n; k += blockDim.x) {
-        __bf16 *row = X + (cs.off + k - 1) * NET_C;
-        row[0] = to_bf16((float)(cs.g[k] / sg));
-        row[1] = to_bf16((float)(cs.a[k] / sb));
-        row[2] = to_bf16((float)(cs.c[k] / sb));
-        row[3] = to_bf16((float)((cs.p[k] + pq) / sp));
-        for (int ch = NET_NF; ch < NET_C; ch++) row[ch] = to_bf16(0.0f);
+    double sr, sb, berr;
+    net_scales(cs.a, cs.b, cs.c, cs.g, cs.guess, cs.n, &sr, &sb, &berr);
+    const int k0 = net_k0(cs);
+    const bool skip = !(berr > rtol);
+    if (!skip)
+        for (int c = tid; c < cs.win; c += blockDim.x) {
+            __bf16 *row = X + (cs.off + c) * NET_C;
+            net_row(cs.a, cs.b, cs.c, cs.g, cs.guess, cs.n, k0 + c, sr, sb, row);
+            for (int ch = NET_NF; ch < NET_C; ch++) row[ch] = to_bf16(0.0f);
+        }
+    if (tid == 0) {
+        seqs[blockIdx.x].len = skip ? 0 : cs.win;
+        info[2 * blockIdx.x] = (double)k0;
     }
-    if (tid == 0) scale[blockIdx.x] = sp;
 }
 
 // training windows from captured systems (one workgroup per case): window w
 // of case i starts at row 1 + woff[i*wpc + w] (-1: none) and goes to data-set
 // slot slot[i*wpc + w]; features as above ([cell][NET_NF] BF16), target
-// (p1 - p0) / s_p (FP32)
+// (p1 - p0) / max over the window of |p1 - p0| (FP32; 0 where p1 = p0)
 __global__ void __launch_bounds__(256) k_net_pairs(const NetPairSrc *__restrict__ src, const int *__restrict__ woff,
                                                    const long long *__restrict__ slot, int wpc, int len,
                                                    __bf16 *__restrict__ dx, float *__restrict__ dt)
 {
     const NetPairSrc s = src[blockIdx.x];
-    double sg, sb, sp;
-    net_scales(s.g, s.b, s.p0, 0.0, s.n, &sg, &sb, &sp);
+    double sr, sb;
+    net_scales(s.a, s.b, s.c, s.g, s.p0, s.n, &sr, &sb, nullptr);
     for (int w = 0; w < wpc; w++) {
         const int o = woff[blockIdx.x * wpc + w];
         if (o < 0) continue;
+        double em[1] = {0.0};
+        for (int c = threadIdx.x; c < len; c += blockDim.x) em[0] = fmax(em[0], fabs(s.p1[1 + o + c] - s.p0[1 + o + c]));
+        net_reduce<1, false>(em);
+        const double se = (em[0] > 0.0) ? em[0] : 1.0;
         const long long base = slot[blockIdx.x * wpc + w] * len;
         for (int c = threadIdx.x; c < len; c += blockDim.x) {
             const int k = 1 + o + c;
-            __bf16 *row = dx + (base + c) * NET_NF;
-            row[0] = to_bf16((float)(s.g[k] / sg));
-            row[1] = to_bf16((float)(s.a[k] / sb));
-            row[2] = to_bf16((float)(s.c[k] / sb));
-            row[3] = to_bf16((float)(s.p0[k] / sp));
-            dt[base + c] = (float)((s.p1[k] - s.p0[k]) / sp);
+            net_row(s.a, s.b, s.c, s.g, s.p0, s.n, k, sr, sb, dx + (base + c) * NET_NF);
+            dt[base + c] = (float)((s.p1[k] - s.p0[k]) / se);
         }
     }
 }
 
-// guess(k) = Q(k) + s_q * y(k)
+// guess = B + alpha d over the window, alpha = <A d, r> / <A d, A d> (the
+// rows where A d is nonzero), in place; skipped cases keep B
 __global__ void __launch_bounds__(256) k_net_guess(const NetCase *__restrict__ cases, const float *__restrict__ y,
-                                                   const double *__restrict__ scale)
+                                                   const NetSeq *__restrict__ seqs, const double *__restrict__ info)
 {
     const NetCase cs = cases[blockIdx.x];
-    const double sp = scale[blockIdx.x];
-    const double pq = *cs.pshift;
-    for (int k = 1 + threadIdx.x; k <= cs.n; k += blockDim.x)
-        cs.guess[k] = (cs.p[k] + pq) + sp * (double)y[cs.off + k - 1];
+    if (seqs[blockIdx.x].len == 0) return;
+    const int k0 = (int)info[2 * blockIdx.x], n = cs.n, w = cs.win;
+    auto d = [&](int k) -> double { return (k >= k0 && k < k0 + w) ? (double)y[cs.off + (k - k0)] : 0.0; };
+    double v[2] = {0.0, 0.0};
+    for (int k = max(k0 - 1, 1) + (int)threadIdx.x; k <= min(k0 + w, n); k += blockDim.x) {
+        double ad = cs.b[k] * d(k);
+        if (k > 1) ad = ad + cs.a[k] * d(k - 1);
+        if (k < n) ad = ad + cs.c[k] * d(k + 1);
+        const double r = net_res(cs.a, cs.b, cs.c, cs.g, cs.guess, n, k);
+        v[0] += ad * r;
+        v[1] += ad * ad;
+    }
+    net_reduce<2, true>(v);
+    const double alpha = (v[1] > 0.0) ? v[0] / v[1] : 0.0;
+    for (int c = threadIdx.x; c < w; c += blockDim.x) cs.guess[k0 + c] = cs.guess[k0 + c] + alpha * (double)y[cs.off + c];
 }
 
 // ---------------------------------------------------------------------
@@ -434,9 +513,12 @@ int net_forward(lica_net *t)
     int rc = net_cast_weights(t);
     for (int l = 0; l < NET_LAYERS && !rc; l++) {
         const bool last = (l == NET_LAYERS - 1);
-        hipLaunchKernelGGL(k_net_conv, dim3(t->ntiles), dim3(256), 0, t->stream, t->act[l], t->wb[l], t->b[l],
-                           t->act[l + 1], last ? t->yf : (float *)nullptr, t->seqs, t->seq_of_tile, NET_COUT[l],
-                           last ? 0 : 1);
+        if (last)
+            hipLaunchKernelGGL(k_net_conv<16>, dim3(t->ntiles), dim3(256), 0, t->stream, t->act[l], t->wb[l], t->b[l],
+                               t->act[l + 1], t->yf, t->seqs, t->seq_of_tile, 0);
+        else
+            hipLaunchKernelGGL(k_net_conv<NET_C>, dim3(t->ntiles), dim3(256), 0, t->stream, t->act[l], t->wb[l],
+                               t->b[l], t->act[l + 1], (float *)nullptr, t->seqs, t->seq_of_tile, 1);
         rc = hip_ok(hipGetLastError());
     }
     return rc;
@@ -462,8 +544,8 @@ int net_backward(lica_net *t, double *loss_out)
             const int n = NET_TAPS * NET_C * NET_C;
             hipLaunchKernelGGL(k_net_flip, dim3((n + 255) / 256), dim3(256), 0, t->stream, t->w[l], t->wt[l],
                                NET_COUT[l], NET_C);
-            hipLaunchKernelGGL(k_net_conv, dim3(t->ntiles), dim3(256), 0, t->stream, t->dz[cur], t->wt[l],
-                               t->zero_bias, t->dz[1 - cur], (float *)nullptr, t->seqs, t->seq_of_tile, NET_C, 0);
+            hipLaunchKernelGGL(k_net_conv<NET_C>, dim3(t->ntiles), dim3(256), 0, t->stream, t->dz[cur], t->wt[l],
+                               t->zero_bias, t->dz[1 - cur], (float *)nullptr, t->seqs, t->seq_of_tile, 0);
             const long long ne = nc * NET_C;
             hipLaunchKernelGGL(k_net_relu_mask, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, t->stream,
                                t->dz[1 - cur], t->act[l], ne);
@@ -734,7 +816,8 @@ int lica_net_train(lica_net *t, int64_t iters, int64_t batch, double lr, uint64_
 // warm start inside a batch: features of every case's POISSON system, the
 // network, guess = P + s_p * y (called by the batch driver before the
 // multigrid kernel of a substep)
-int lica_net_guess(lica_net *t, const NetCase *dcases, int ncase, const int64_t *lens, double *dscale, void *stream)
+int lica_net_guess(lica_net *t, const NetCase *dcases, int ncase, const int64_t *lens, double *dinfo, double rtol,
+                   void *stream)
 {
     int rc = 0;
     if (t->layout_owner != (const void *)dcases) {
@@ -745,12 +828,14 @@ int lica_net_guess(lica_net *t, const NetCase *dcases, int ncase, const int64_t 
     hipStream_t s = (hipStream_t)stream;
     hipStream_t saved = t->stream;
     t->stream = s;
-    hipLaunchKernelGGL(k_net_features, dim3(ncase), dim3(256), 0, s, dcases, t->act[0], dscale);
+    hipLaunchKernelGGL(k_net_features, dim3(ncase), dim3(256), 0, s, dcases, t->act[0], t->seqs, dinfo, rtol);
     rc = net_forward(t);
-    hipLaunchKernelGGL(k_net_guess, dim3(ncase), dim3(256), 0, s, dcases, t->yf, dscale);
+    hipLaunchKernelGGL(k_net_guess, dim3(ncase), dim3(256), 0, s, dcases, t->yf, t->seqs, dinfo);
     t->stream = saved;
     return rc ? rc : hip_ok(hipGetLastError());
 }
+
+int64_t lica_net_window() { return NET_WIN; }
 
 // layout of a batch's cases (called when the batch adopts the network, so
 // that the per-substep guess launches no copies and can be graph-captured)
@@ -773,11 +858,14 @@ int lica_net_append(lica_net *t, const NetPairSrc *hsrc, int ncase, int64_t wpc,
     std::vector<int> woff((size_t)ncase * (size_t)wpc, -1);
     std::vector<long long> slot((size_t)ncase * (size_t)wpc, 0);
     long long n_new = 0;
+    std::uniform_int_distribution<int64_t> jit(-(int64_t)len / 4, (int64_t)len / 4);
     for (int i = 0; i < ncase; i++) {
         if (hsrc[i].n < len) continue;
         std::uniform_int_distribution<int64_t> pick(0, hsrc[i].n - len);
         for (int64_t w = 0; w < wpc; w++) {
-            woff[(size_t)i * wpc + w] = (int)pick(rng);
+            int64_t o = (hsrc[i].kif >= 1) ? hsrc[i].kif - 1 - len / 2 + jit(rng) : pick(rng);
+            o = std::min<int64_t>(std::max<int64_t>(o, 0), hsrc[i].n - len);
+            woff[(size_t)i * wpc + w] = (int)o;
             slot[(size_t)i * wpc + w] = t->data_n + n_new++;
         }
     }

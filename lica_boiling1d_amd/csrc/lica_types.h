@@ -38,6 +38,7 @@ struct InstParam {
     long long ns_off, rf_off, nsi_off, rfi_off;
     long long mg_off;            // multigrid scratch (doubles), multigrid modes only
     long long gs_off;            // guess / capture rows (n+2 per case)
+    long long hist_off;          // solution history of the warm start (3 substeps x MG_HIST x n)
     // properties (INPUT_SIMULATION 1d.f:434-480)
     double re, prm, denr, ja, tsat, cflmax, dt0, z0, tslope;
     double rei, denm, denp, den_diff, vism, visp, denscm, densc_diff;
@@ -58,6 +59,7 @@ struct InstState {
     double n_reinit, n_bandgen, n_masscor;
     double poisson_cycles, poisson_solves, poisson_resid;
     double pshift;        // p(1) removed by the last UCALC: P + pshift is the last solution
+    int hist_head[3], hist_cnt[3];   // warm-start history ring per substep: next slot, entries held
     int sf_lo, sf_hi;     // refined window of nonzero SFGAS/SFGAS2
     int im_lo, im_hi;     // NS window of IMASK_PH==1
     int bn[LICA_MLAYER + 1];
@@ -81,13 +83,18 @@ struct BatchDev {
     double mg_rtol;      // relative residual target
     double *mg;          // multigrid scratch of the batch
     long long mg_lds;    // dynamic LDS bytes of the multigrid kernel (x, f of the coarse levels)
-    const double *guess; // initial guesses of the learned mode (rows at gs_off)
+    double *guess;       // initial guesses of the warm-started modes (rows at gs_off)
+    double *hist;        // solution history of the warm-started modes
     double *cap;         // P before / after the last substep's solve, its RHS (3 x (n+2) at 3*gs_off)
 };
 
 #define LICA_MG_FP64_MODE 2
 #define LICA_MG_MIXED_MODE 3
 #define LICA_MG_LEARNED_MODE 4
+#define LICA_MG_PROJ_MODE 5
+#define MG_HIST 2            // solutions per substep kept for the warm start
+// modes whose V-cycles start from the warm-start guess
+#define LICA_MG_WARM(mode) ((mode) == LICA_MG_LEARNED_MODE || (mode) == LICA_MG_PROJ_MODE)
 
 // status codes shared with include/lica_mi355x.h
 #define ST_OK 0

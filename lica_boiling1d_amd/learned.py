@@ -1,21 +1,26 @@
 """Learned warm start of the Poisson solve.
 
-A 1-D convolutional network (4 layers of 21 taps, 32 channels, ReLU; BF16
-weights and activations, FP32 accumulation on MFMA; about 46k parameters)
-maps, cell by cell, the features of a POISSON system (1d.f:3529-3543)
+The POISSON solution of a substep (1d.f:3529-3543) is, to about 1e-6, a
+combination of the case's solutions at the same substep in the last steps:
+``LICA_POISSON_PROJECTED`` starts the mixed-precision V-cycles from B, the
+combination of those two and the last solution that minimises |g - A B|_2.
+What is left, p - B, sits at the interface.  ``LICA_POISSON_LEARNED`` adds a
+1-D convolutional network's correction (4 layers of 21 taps, 32 channels,
+ReLU; BF16 weights and activations, FP32 accumulation on MFMA; about 46k
+parameters) that maps, cell by cell,
 
-    g / max|g|,  a / max|b|,  c / max|b|,  Q / max|Q|
+    r / max|r|,  a / max|b|,  b / max|b|,  c / max|b|      (r = g - A B)
 
-(g the right-hand side, a/b/c the three diagonals, Q the previous substep's
-solution: the pressure held before the solve plus the p(1) that UCALC
-subtracted from it, 1d.f:3433-3470) to the correction (p - Q) / max|Q| of the
-solution p.  ``LICA_POISSON_LEARNED`` starts the mixed-precision V-cycles from
-Q + max|Q| * network; the V-cycles always continue to the same residual
-tolerance, so multigrid decides the answer and the network only saves
-cycles.  Training pairs come from the solver itself (``collect_pairs``): a
-batch runs with capture on in a multigrid mode, and after every step windows
-of every case's last POISSON system go from its capture rows straight into
-the network's data set in device memory.
+to a direction d over a window of 256 rows around the interface, where the
+error p - B left by the projection is largest (a and c jump with the
+density there); the residual fixes its length, guess = B + alpha d with
+alpha = <A d, r> / <A d, A d>, so a useless direction costs nothing.  Cases
+whose B already meets the tolerance skip the network.  The V-cycles always
+continue to the same backward-error tolerance, so multigrid decides the
+answer and the network only saves cycles.  Training pairs come
+from the solver itself (``collect_pairs``): a batch runs with capture on and
+after every step windows of every case's captured POISSON system and
+starting point go straight into the network's data set in device memory.
 """
 import ctypes
 
@@ -122,29 +127,43 @@ class Net:
         return loss.value
 
 
-def features(a, b, c, g, q):
-    """Per-cell features and the solution scale of one POISSON system (rows
-    1..n, q the previous solution); the same arithmetic as the device
-    feature kernels, before the BF16 rounding."""
-    sg = np.max(np.abs(g)) or 1.0
+def features(a, b, c, g, base):
+    """Per-cell features of one POISSON system (rows 1..n) and starting
+    point; the arithmetic of the device feature kernels, before the BF16
+    rounding."""
+    ab = a * np.r_[0.0, base[:-1]]
+    cb = c * np.r_[base[1:], 0.0]
+    r = g - ((b * base + ab) + cb)
+    sr = np.max(np.abs(r)) or 1.0
     sb = np.max(np.abs(b)) or 1.0
-    sq = np.max(np.abs(q)) or 1.0
-    f = np.stack([g / sg, a / sb, c / sb, q / sq], axis=1)
-    return f.astype(np.float32), sq
+    f = np.stack([r / sr, a / sb, b / sb, c / sb], axis=1)
+    return f.astype(np.float32)
 
 
-def collect_pairs(net, cases, steps, windows_per_case=4, window=WINDOW, seed=0, mode=None, rtol=1e-10, lib=None):
-    """Run the cases `steps` steps in a multigrid mode (mixed precision by
-    default) with capture on and append, after every step, windows of every
-    healthy case's last POISSON system to the network's data set.  Returns the
-    number of windows added."""
+def target(p, base):
+    """Training target of a window: p - base scaled to unit maximum (zero
+    where they agree)."""
+    e = p - base
+    se = np.max(np.abs(e)) or 1.0
+    return (e / se).astype(np.float32)
+
+
+def collect_pairs(net, cases, steps, windows_per_case=4, window=WINDOW, seed=0, mode=None, rtol=1e-10, lib=None,
+                  warmup=2):
+    """Run the cases in the projected warm-start mode (or `mode`) with
+    capture on, `warmup` steps to fill the solution history, then `steps`
+    steps after each of which windows of every healthy case's captured
+    POISSON system (substeps 1, 2, 3 in turn) join the network's data set.
+    Returns the number of windows added."""
     from .solver import Batch
     from . import _abi as A
     b = Batch(cases, lib=lib if lib is not None else net._lib)
-    b.set_poisson(A.POISSON_MIXED if mode is None else mode, rtol, 200)
-    b.set_capture(True)
+    b.set_poisson(A.POISSON_PROJECTED if mode is None else mode, rtol, 200)
+    if warmup:
+        b.run_all(warmup)
     total = 0
     for s in range(steps):
+        b.set_capture(True, substep=1 + s % 3)
         b.run_all(1)
         total += net.add_pairs(b, windows_per_case, window, seed * 1000003 + s)
     b.close()

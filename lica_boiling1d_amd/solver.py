@@ -180,7 +180,10 @@ class Batch(_Handle):
         """GPU ms per kernel class of the last run (timing mode) and the wall ms."""
         out = np.zeros(16)
         m = self._lib.lica_batch_last_timings(self._h, _dptr(out), 16)
-        names = ["init", "dt", "cont_energy", "temp_grad", "momentum", "poisson", "ucalc", "transport", "poisson_mg"]
+        # the work between the POISSON matrix and its solve (warm-start base,
+        # capture, network) is timed in the slot after poisson_mg
+        names = ["init", "dt", "cont_energy", "temp_grad", "momentum", "poisson", "ucalc", "transport", "poisson_mg",
+                 "warm_start"]
         res = {names[i]: float(out[i]) for i in range(min(m - 1, len(names)))}
         res["wall"] = float(out[m - 1])
         return res

@@ -171,3 +171,28 @@ def test_batched_poisson_multigrid_cpu(emul, mode, n):
         assert np.max(np.abs(r)) <= 1e-10 * (anorm * np.max(np.abs(p[s])) + np.max(np.abs(g[s])))
         x = np.linalg.solve(A, g[s])
         assert np.max(np.abs(p[s] - x)) <= 1e-6 * np.max(np.abs(x))
+
+
+def test_projected_warm_start_cpu(emul):
+    """The history projection (LICA_POISSON_PROJECTED) on the deck case: every
+    solve meets the backward-error tolerance, the interface stays within 1e-6
+    relative of the oracle, and once the history holds the pressure field
+    built up over the first steps the V-cycles per solve fall far below the
+    zero start's (7 on this case)."""
+    case = L.deck_case()
+    o = Oracle(case)
+    assert o.step(30) == 0
+    cyc = {}
+    for mode in (L.POISSON_MIXED, L.POISSON_PROJECTED):
+        b = L.Batch([case], lib=emul)
+        b.set_poisson(mode, 1e-10, 100)
+        b.run(20)
+        s0 = b.scalars()
+        b.run(10)
+        s = b.scalars()
+        assert 0 < s["POISSON_RESID"] <= 1e-10
+        zo = o.scalar_array()[3]
+        assert abs(s["PLACE"] - zo) <= 1e-6 * zo
+        cyc[mode] = (s["POISSON_CYCLES"] - s0["POISSON_CYCLES"]) / (s["POISSON_SOLVES"] - s0["POISSON_SOLVES"])
+    assert cyc[L.POISSON_MIXED] >= 5
+    assert cyc[L.POISSON_PROJECTED] <= 2, cyc

@@ -4,15 +4,15 @@ The MFMA kernels (BF16 operands, FP32 accumulation) are compared with the
 NumPy restatement in net_ref.py, which quantises at the same points and
 accumulates exactly: the forward output and the gradient agree to within
 the accumulation-order differences of FP32 (a rare BF16 rounding flip of one
-activation included).  The learned mode only chooses the initial guess of the
-mixed-precision V-cycles, so its results meet the same backward-error
-tolerance as the other multigrid modes.
+activation included).  The projected and learned modes only choose the
+initial guess of the mixed-precision V-cycles, so their results meet the same
+backward-error tolerance as the other multigrid modes.
 """
 import numpy as np
 import pytest
 
 import lica_boiling1d_amd as L
-from lica_boiling1d_amd.learned import Net, collect_pairs, features
+from lica_boiling1d_amd.learned import Net, collect_pairs, features, target
 from oracle import Oracle
 import net_ref as R
 
@@ -95,26 +95,29 @@ def test_training_fits_a_learnable_map():
     assert l1 <= 0.05 * l0, (l0, l1)
 
 
-def test_device_pairs_equal_host_features():
+@pytest.mark.parametrize("mode", [L.POISSON_MIXED, L.POISSON_PROJECTED])
+def test_device_pairs_equal_host_features(mode):
     """Windows cut on the device from the capture rows equal the features and
     targets computed on the host from the same rows (window = whole column,
-    so its offset is 0)."""
-    _say("device training pairs, 3 cases n=128")
+    so its offset is 0); the starting point is the last solution in the
+    mixed mode and the projection base in the projected mode."""
+    _say(f"device training pairs, 3 cases n=128, mode {mode}")
     cases = [L.deck_case(128, 2).copy(z0=z) for z in (0.05, 0.0607, 0.08)]
     net = Net(seed=0)
     b = L.Batch(cases)
-    b.set_poisson(L.POISSON_MIXED, 1e-10, 100)
-    b.set_capture(True)
+    b.set_poisson(mode, 1e-10, 100)
     b.run(3)
+    b.set_capture(True, substep=2)
+    b.run(1)
     assert net.add_pairs(b, 1, 128, seed=0) == 3
     x, t = net.data()
     assert x.shape == (3, 128, 4) and t.shape == (3, 128)
     for i in range(3):
         a, bb, c, g = (b.field(k, i) for k in ("PA", "PB", "PC", "PRHS"))
         q, p = b.field("PPREV", i), b.field("PSOL", i)
-        f, sq = features(a, bb, c, g, q)
+        f = features(a, bb, c, g, q)
         assert np.array_equal(x[i], R.bf16(f).astype(np.float32)), i
-        assert np.array_equal(t[i], ((p - q) / sq).astype(np.float32)), i
+        assert np.array_equal(t[i], target(p, q)), i
     # appending keeps what is held; the window length is fixed by the data set
     b.run(1)
     assert net.add_pairs(b, 2, 128, seed=1) == 6
@@ -141,22 +144,51 @@ def test_add_pairs_needs_capture_and_multigrid():
         net.add_pairs(b, 1, 64)
 
 
+def _cycles(b, steps):
+    s0 = b.scalars()
+    b.run(steps)
+    s = b.scalars()
+    return (s["POISSON_CYCLES"] - s0["POISSON_CYCLES"]) / (s["POISSON_SOLVES"] - s0["POISSON_SOLVES"]), s
+
+
+def test_projected_mode_large_case():
+    """History projection on the large case (N=8192, r=4): every solve meets
+    the tolerance, the interface stays within 1e-6 relative of the FP64
+    oracle, and after the start-up transient the V-cycles per solve fall
+    below the zero start's."""
+    case = L.single_large_case(1)
+    _say("projected warm start, large case, 30 steps")
+    o = Oracle(case)
+    assert o.step(30) == 0
+    cyc = {}
+    for mode in (L.POISSON_MIXED, L.POISSON_PROJECTED):
+        b = L.Batch([case])
+        b.set_poisson(mode, 1e-10, 100)
+        b.run(20)
+        cyc[mode], s = _cycles(b, 10)
+        assert 0 < s["POISSON_RESID"] <= 1e-10
+        zo = o.scalar_array()[3]
+        assert abs(s["PLACE"] - zo) <= 1e-6 * zo
+    _say(f"cycles/solve: zero start {cyc[L.POISSON_MIXED]:.2f}, projected {cyc[L.POISSON_PROJECTED]:.2f}")
+    assert cyc[L.POISSON_PROJECTED] < cyc[L.POISSON_MIXED]
+
+
 def test_learned_mode_meets_tolerance():
     """Train on the solver's own POISSON pairs of the large case, then run the
     learned mode: every solve meets the backward-error tolerance and the
     interface stays within 1e-6 relative of the FP64 oracle."""
     case = L.single_large_case(1)
-    _say("learned mode: collect pairs, train, 6 steps of the large case")
+    _say("learned mode: collect pairs, train, 8 steps of the large case")
     net = Net(seed=0)
     assert collect_pairs(net, [case], steps=6, windows_per_case=16) == 6 * 16
     net.train(100, batch=32, lr=1e-3)
     b, o = L.Batch([case]), Oracle(case)
     b.set_net(net)
     b.set_poisson(L.POISSON_LEARNED, 1e-10, 100)
-    b.run(6)
-    assert o.step(6) == 0
+    b.run(8)
+    assert o.step(8) == 0
     s = b.scalars()
-    assert s["POISSON_SOLVES"] == 18
+    assert s["POISSON_SOLVES"] == 24
     assert 0 < s["POISSON_RESID"] <= 1e-10
     zo = o.scalar_array()[3]
     assert abs(s["PLACE"] - zo) <= 1e-6 * zo
