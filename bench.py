@@ -271,7 +271,11 @@ def main():
                                   "vcycles_per_solve": pr.get("vcycles_per_solve"),
                                   "poisson_backward_error_max": pr.get("resid_max"),
                                   "kernel_ms_per_launch": roofline(L, cases, L.POISSON_PROJECTED, args, pr)[1]}
-        out["network_gain_vs_projected"] = res["value"] / pr["value"]
+        if pr["nbad"]:
+            st = pr["status"]
+            codes, counts = np.unique(st[st != 0], return_counts=True)
+            out["mixed_projected"]["stopped_cases"] = {L.status_string(int(k)): int(v) for k, v in zip(codes, counts)}
+        out["network_gain_vs_projected"] = res["value"] / pr["value"] if pr["value"] > 0 else None
         out["vcycles_saved_by_network_per_solve"] = pr.get("vcycles_per_solve", 0.0) - res.get("vcycles_per_solve", 0.0)
     if mode >= L.POISSON_MIXED and not args.no_fp64:
         say("timed run, plain FP64 V-cycles from zero")
@@ -282,7 +286,11 @@ def main():
                              "vcycles_per_solve": ref.get("vcycles_per_solve"),
                              "poisson_backward_error_max": ref.get("resid_max"),
                              "kernel_ms_per_launch": roofline(L, cases, L.POISSON_MG64, args, ref)[1]}
-        out["speedup_vs_plain_fp64"] = res["value"] / ref["value"]
+        if ref["nbad"]:
+            st = ref["status"]
+            codes, counts = np.unique(st[st != 0], return_counts=True)
+            out["plain_fp64"]["stopped_cases"] = {L.status_string(int(k)): int(v) for k, v in zip(codes, counts)}
+        out["speedup_vs_plain_fp64"] = res["value"] / ref["value"] if ref["value"] > 0 else None
         out["vcycles_saved_per_solve"] = ref.get("vcycles_per_solve", 0.0) - res.get("vcycles_per_solve", 0.0)
         out["interface_rel_diff_vs_plain_fp64_max"] = float(rel.max()) if rel.size else None
     if not args.no_cpu:
