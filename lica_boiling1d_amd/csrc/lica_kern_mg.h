@@ -14,7 +14,8 @@
 // reaches rtol.  The solution replaces P(1:N) as the direct solve does.
 //
 // Memory: the iterate, the fine matrix and the matrices of every level sit in
-// a per-case global scratch (read-only during the cycles); x and f of the
+// a per-case global scratch (read-only during the cycles), one packed row
+// (a, b, c, 1/b) per unknown so that a row is one wide load; x and f of the
 // levels live in LDS from the coarsest level up as far as the launch's
 // dynamic LDS allows (the rest in the scratch), so that the passes of a cycle
 // -- each one barrier -- do not wait on global stores.
@@ -48,15 +49,16 @@ HOSTDEV long long mg_rows_before(int n, int l)
 }
 HOSTDEV long long mg_rows_total(int n) { return mg_rows_before(n, mg_levels(n)); }
 
-// Per-case scratch, in doubles:
-//   7n         fine level FP64: a, b, c, 1/b, g, iterate, its double buffer
-//   4(tot-n)   FP64 a, b, c, 1/b of levels 1..L-1
-//   2 tot      FP32 a, b, c, 1/b of every level
+// Per-case scratch, in doubles (every block 32-byte aligned when the case's
+// scratch is):
+//   4 tot      FP64 rows (a, b, c, 1/b) of every level, level 0 first
+//   2 tot      FP32 rows of every level
+//   4n         g, iterate, its double buffer (and one spare row)
 //   2 tot      x, f of the levels kept outside LDS (room for FP64)
 HOSTDEV long long mg_scratch_doubles(int n)
 {
     const long long tot = mg_rows_total(n);
-    return 7LL * n + 4LL * (tot - n) + 2LL * tot + 2LL * tot + 8;
+    return (4LL * tot + 2LL * tot + 4LL * n + 2LL * tot + 8 + 3) / 4 * 4;
 }
 // LDS that holds x, f of every level in elements of tsize bytes
 HOSTDEV long long mg_lds_bytes(int n, int tsize) { return 2LL * tsize * mg_rows_total(n); }
@@ -69,27 +71,43 @@ struct MgSh {
     long long lds_bytes;
 };
 
+// one packed matrix row
+#ifdef LICA_EMUL
+template <class T>
+struct Row4 {
+    T a, b, c, ib;
+};
+#else
+template <class T>
+struct alignas(4 * sizeof(T)) Row4 {
+    T a, b, c, ib;
+};
+#endif
+
 template <class T>
 struct MgLev {
-    const T *a, *b, *c, *ib;
+    Row4<T> *m;   // rows (a, b, c, 1/b); written only while the levels are built
     T *x, *f;
     int n;
 };
 struct MgFine {
-    double *a, *b, *c, *ib, *g, *x0, *x1;
+    Row4<double> *m;
+    double *g, *x0, *x1;
     int n;
 };
 
 DEV MgFine mg_fine(double *scr, int n)
 {
+    const long long tot = mg_rows_total(n);
     MgFine F;
     F.n = n;
-    F.a = scr; F.b = scr + n; F.c = scr + 2 * n; F.ib = scr + 3 * n;
-    F.g = scr + 4 * n; F.x0 = scr + 5 * n; F.x1 = scr + 6 * n;
+    F.m = (Row4<double> *)scr;
+    double *v = scr + 6 * tot;
+    F.g = v; F.x0 = v + n; F.x1 = v + 2 * n;
     return F;
 }
 
-// matrices of level l (FP64 or FP32); x, f in LDS when levels l..L-1 fit
+// rows of level l (FP64 or FP32); x, f in LDS when levels l..L-1 fit
 template <class T>
 DEV MgLev<T> mg_lev(double *scr, int n, int l, const MgSh &sh)
 {
@@ -97,18 +115,12 @@ DEV MgLev<T> mg_lev(double *scr, int n, int l, const MgSh &sh)
     const long long pre = mg_rows_before(n, l), tot = mg_rows_total(n);
     MgLev<T> v;
     v.n = m;
-    if constexpr (sizeof(T) == 8) {
-        const double *q = (l == 0) ? scr : scr + 7LL * n + 4 * (pre - n);
-        const long long s = (l == 0) ? n : m;
-        v.a = q; v.b = q + s; v.c = q + 2 * s; v.ib = q + 3 * s;
-    } else {
-        const float *q = (const float *)(scr + 7LL * n + 4 * (tot - n)) + 4 * pre;
-        v.a = q; v.b = q + m; v.c = q + 2 * m; v.ib = q + 3 * m;
-    }
+    if constexpr (sizeof(T) == 8) v.m = (Row4<T> *)scr + pre;
+    else v.m = (Row4<T> *)(scr + 4 * tot) + pre;
     const long long suffix = tot - pre;   // rows of levels l..L-1
     T *q;
     if ((long long)(2 * sizeof(T)) * suffix <= sh.lds_bytes) q = (T *)sh.lds + 2 * (suffix - m);
-    else q = (T *)(scr + 7LL * n + 4 * (tot - n) + 2 * tot) + 2 * pre;
+    else q = (T *)(scr + 6 * tot + 4LL * n) + 2 * pre;
     v.x = q;
     v.f = q + m;
     return v;
@@ -126,17 +138,17 @@ DEV double mg_pw(int k, int J, int nf, int nc)
 
 // Galerkin operator of level l+1 from level l (FP64), with 1/b, and the FP32
 // copy of level l+1 when the cycles run in FP32
-DEV void mg_coarsen(const MgLev<double> &F, const MgLev<double> &C, float *a32, bool f32)
+DEV void mg_coarsen(const MgLev<double> &F, const MgLev<double> &C, Row4<float> *c32, bool f32)
 {
     const int nf = F.n, nc = C.n;
-    double *ca = (double *)C.a, *cb = (double *)C.b, *cc = (double *)C.c, *cib = (double *)C.ib;
     PFOR(I, 0, nc) {
         double acc[3] = {0.0, 0.0, 0.0};
         for (int t = 0; t < 3; t++) {
             const int i = 2 * I - 1 + t;
             const double wi = mg_pw(i, I, nf, nc);
             if (wi == 0.0) continue;
-            const double ai = F.a[i], ci = F.c[i], bi = F.b[i];
+            const Row4<double> ri = F.m[i];
+            const double ai = ri.a, ci = ri.c, bi = ri.b;
             for (int d = -1; d <= 1; d++) {
                 const int J = I + d;
                 double apj = ai * mg_pw(i - 1, J, nf, nc) + bi * mg_pw(i, J, nf, nc) + ci * mg_pw(i + 1, J, nf, nc);
@@ -144,9 +156,13 @@ DEV void mg_coarsen(const MgLev<double> &F, const MgLev<double> &C, float *a32, 
             }
         }
         const double va = (I == 0) ? 0.0 : acc[0], vc = (I == nc - 1) ? 0.0 : acc[2], vb = acc[1];
-        ca[I] = va; cb[I] = vb; cc[I] = vc; cib[I] = 1.0 / vb;
+        Row4<double> r;
+        r.a = va; r.b = vb; r.c = vc; r.ib = 1.0 / vb;
+        C.m[I] = r;
         if (f32) {
-            a32[I] = (float)va; a32[nc + I] = (float)vb; a32[2 * nc + I] = (float)vc; a32[3 * nc + I] = 1.0f / (float)vb;
+            Row4<float> q;
+            q.a = (float)va; q.b = (float)vb; q.c = (float)vc; q.ib = 1.0f / (float)vb;
+            c32[I] = q;
         }
     }
     PFOR_END
@@ -163,12 +179,13 @@ DEV void mg_smooth(const MgLev<T> &L, int sweeps, bool zero_start)
             const bool z = zero_start && s == 0 && color == 0;
             PFOR(h, 0, (n - color + 1) / 2) {
                 const int i = 2 * h + color;
+                const Row4<T> r = L.m[i];
                 T v = L.f[i];
                 if (!z) {
-                    if (i > 0) v = v - L.a[i] * L.x[i - 1];
-                    if (i < n - 1) v = v - L.c[i] * L.x[i + 1];
+                    if (i > 0) v = v - r.a * L.x[i - 1];
+                    if (i < n - 1) v = v - r.c * L.x[i + 1];
                 }
-                L.x[i] = v * L.ib[i];
+                L.x[i] = v * r.ib;
             }
             PFOR_END
         }
@@ -178,9 +195,10 @@ DEV void mg_smooth(const MgLev<T> &L, int sweeps, bool zero_start)
 template <class T>
 DEV T mg_res_at(const MgLev<T> &F, int i)
 {
-    T v = F.f[i] - F.b[i] * F.x[i];
-    if (i > 0) v = v - F.a[i] * F.x[i - 1];
-    if (i < F.n - 1) v = v - F.c[i] * F.x[i + 1];
+    const Row4<T> r = F.m[i];
+    T v = F.f[i] - r.b * F.x[i];
+    if (i > 0) v = v - r.a * F.x[i - 1];
+    if (i < F.n - 1) v = v - r.c * F.x[i + 1];
     return v;
 }
 
@@ -217,16 +235,17 @@ DEV void mg_coarsest_factor(const MgLev<double> &L, MgSh &sh)
 {
     SERIAL {
         const int n = L.n;
-        double ib = 1.0 / L.b[0];
+        double ib = 1.0 / L.m[0].b;
         sh.cib[0] = ib;
         sh.ca[0] = 0.0;
         sh.cg[0] = 0.0;
         for (int j = 1; j < n; j++) {
-            const double g = L.c[j - 1] * ib;
-            ib = 1.0 / (L.b[j] - L.a[j] * g);
+            const Row4<double> r = L.m[j];
+            const double g = L.m[j - 1].c * ib;
+            ib = 1.0 / (r.b - r.a * g);
             sh.cg[j] = g;
             sh.cib[j] = ib;
-            sh.ca[j] = L.a[j];
+            sh.ca[j] = r.a;
         }
     }
     SERIAL_END
@@ -281,10 +300,11 @@ DEV void mg_update(const MgFine &F, const MgLev<T> &Z, bool first, const double 
     const int n = F.n;
     auto xn = [&](int k) -> double { return first ? xa[k] : xa[k] + (double)Z.x[k]; };
     red_max2(0, n, [&](int i, double &ra, double &xa_) {
+        const Row4<double> r = F.m[i];
         const double xi = xn(i);
-        double v = F.g[i] - F.b[i] * xi;
-        if (i > 0) v = v - F.a[i] * xn(i - 1);
-        if (i < n - 1) v = v - F.c[i] * xn(i + 1);
+        double v = F.g[i] - r.b * xi;
+        if (i > 0) v = v - r.a * xn(i - 1);
+        if (i < n - 1) v = v - r.c * xn(i + 1);
         xb[i] = xi;
         Z.f[i] = (T)v;
         ra = fabs(v);
@@ -346,16 +366,19 @@ DEV int mg_solve(const double *a, const double *b, const double *c, const double
     const bool f32 = (mode != LICA_MG_FP64_MODE);
     const bool guess = (LICA_MG_WARM(mode) && x0);
     MgFine F = mg_fine(scr, n);
-    MgLev<float> Z32 = mg_lev<float>(scr, n, 0, sh);
-    float *a32 = (float *)Z32.a;
+    Row4<float> *m32 = mg_lev<float>(scr, n, 0, sh).m;
     double fmax, anorm;
     red_max2(0, n, [&](int i, double &fm, double &an) {
         const double ai = (i == 0) ? 0.0 : a[i + 1], bi = b[i + 1], ci = (i == n - 1) ? 0.0 : c[i + 1];
-        F.a[i] = ai; F.b[i] = bi; F.c[i] = ci; F.ib[i] = 1.0 / bi;
+        Row4<double> r;
+        r.a = ai; r.b = bi; r.c = ci; r.ib = 1.0 / bi;
+        F.m[i] = r;
         F.g[i] = g[i + 1];
         F.x0[i] = guess ? x0[i + 1] : 0.0;
         if (f32) {
-            a32[i] = (float)ai; a32[n + i] = (float)bi; a32[2 * n + i] = (float)ci; a32[3 * n + i] = 1.0f / (float)bi;
+            Row4<float> q;
+            q.a = (float)ai; q.b = (float)bi; q.c = (float)ci; q.ib = 1.0f / (float)bi;
+            m32[i] = q;
         }
         fm = fabs(g[i + 1]);
         an = fabs(ai) + fabs(bi) + fabs(ci);
@@ -367,7 +390,7 @@ DEV int mg_solve(const double *a, const double *b, const double *c, const double
     if (res > rtol) {
         for (int l = 0; l + 1 < L; l++)
             mg_coarsen(mg_lev<double>(scr, n, l, sh), mg_lev<double>(scr, n, l + 1, sh),
-                       (float *)mg_lev<float>(scr, n, l + 1, sh).a, f32);
+                       mg_lev<float>(scr, n, l + 1, sh).m, f32);
         mg_coarsest_factor(mg_lev<double>(scr, n, L - 1, sh), sh);
         if (f32) cyc = mg_iterate<float>(F, scr, n, rtol, maxc, fmax, anorm, sh, &cur, &oth, res, &res);
         else cyc = mg_iterate<double>(F, scr, n, rtol, maxc, fmax, anorm, sh, &cur, &oth, res, &res);
