@@ -68,11 +68,15 @@ int d2h(void *h, const void *d, size_t n, hipStream_t) { memcpy(h, d, n); return
 int dsync(hipStream_t) { return 0; }
 int have_device() { return 1; }
 #else
+// zeroed device memory.  hipMemset runs on the null stream, asynchronously
+// to the host, and the batch streams are non-blocking (not ordered after the
+// null stream): without the wait a late fill could overwrite what a batch
+// stream has uploaded or computed in the buffer since.
 int dmalloc(void **p, size_t n)
 {
     if (hipMalloc(p, n ? n : 1) != hipSuccess) { *p = nullptr; return LICA_HIP_ERROR; }
     if (hipMemset(*p, 0, n ? n : 1) != hipSuccess) return LICA_HIP_ERROR;
-    return 0;
+    return hipStreamSynchronize(nullptr) == hipSuccess ? 0 : LICA_HIP_ERROR;
 }
 void dfree(void *p) { if (p) (void)hipFree(p); }
 int h2d(void *d, const void *h, size_t n, hipStream_t s)

@@ -445,10 +445,13 @@ struct lica_net {
 };
 
 namespace {
+// zeroed device memory; the fill (null stream, asynchronous to the host) is
+// waited for, as the network's streams are not ordered after the null stream
 int net_alloc(void **p, size_t bytes)
 {
     if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess) { *p = nullptr; return LICA_HIP_ERROR; }
-    return hip_ok(hipMemset(*p, 0, bytes ? bytes : 1));
+    int rc = hip_ok(hipMemset(*p, 0, bytes ? bytes : 1));
+    return rc ? rc : hip_ok(hipStreamSynchronize(nullptr));
 }
 
 // room for ncell cells in nseq sequences
