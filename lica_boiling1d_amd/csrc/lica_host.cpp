@@ -421,7 +421,9 @@ BatchDev make_bd(const lica_batch *b, int msub)
     bd.ns = b->d_ns; bd.rf = b->d_rf; bd.nsi = b->d_nsi; bd.rfi = b->d_rfi;
     bd.order = b->d_order;
     bd.ninst = (int)b->ninst;
-    bd.tri_mode = (b->poisson_mode == LICA_POISSON_THOMAS) ? 0 : 1;
+    // the multigrid modes leave only diagonally dominant rows (energy,
+    // momentum) to the tridiagonal solver: no refinement steps there
+    bd.tri_mode = (b->poisson_mode == LICA_POISSON_THOMAS) ? 0 : (b->poisson_mode >= LICA_POISSON_MG64) ? 2 : 1;
     bd.msub = msub;
     bd.pmode = b->poisson_mode;
     bd.mg = b->d_mg;

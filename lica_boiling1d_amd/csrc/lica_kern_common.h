@@ -485,9 +485,12 @@ DEV void tri_solve(double *a, double *b, double *c, double *g, int l1, int l2, i
     double *G = g + l1;
     // The partitioned elimination is backward stable but, on the nearly
     // singular POISSON rows, its forward error grows with the chunk count;
-    // TRI_NREF refinement steps with residuals in double-double bring the
-    // solution back to (or below) TRDIAG3's error.  a, b and c are left
-    // untouched; g receives the solution.
+    // in mode 1 TRI_NREF refinement steps with residuals in double-double
+    // bring the solution back to (or below) TRDIAG3's error.  Mode 2 (the
+    // diagonally dominant energy and momentum rows when multigrid takes the
+    // POISSON rows) does without them.  a, b and c are left untouched; g
+    // receives the solution.
+    const int nref = (mode == 2) ? 0 : TRI_NREF;
 #ifndef LICA_EMUL
     const int lmax = (nrow + nc - 1) / nc;
     if (lmax > TRI_RPT) {
@@ -507,7 +510,7 @@ DEV void tri_solve(double *a, double *b, double *c, double *g, int l1, int l2, i
         });
     }
     part_pass_reg(A, B, C, x, act, ch, nc, L, s, ts);
-    for (int it = 0; it < TRI_NREF; it++) {
+    for (int it = 0; it < nref; it++) {
         if (act) {
             RA[2 * ch] = x[0];
             static_for<TRI_RPT>([&](auto jc) {
@@ -578,7 +581,7 @@ DEV void tri_solve(double *a, double *b, double *c, double *g, int l1, int l2, i
     };
     for (int i = 0; i < nrow; i++) wx[i] = G[i];
     pass(wx);
-    for (int it = 0; it < TRI_NREF; it++) {
+    for (int it = 0; it < nref; it++) {
         for (int i = 0; i < nrow; i++) {
             const double xm = (i == 0) ? 0.0 : wx[i - 1], xp = (i == nrow - 1) ? 0.0 : wx[i + 1];
             const double am = (i == 0) ? 0.0 : A[i], cp = (i == nrow - 1) ? 0.0 : C[i];

@@ -69,7 +69,31 @@ struct MgSh {
     double cy[MG_COARSE];                                  // its forward sweep
     void *lds;                                             // x, f of the coarse levels
     long long lds_bytes;
+    // level table of the current solve (filled once: the loops that give a
+    // level's rows and offset would otherwise run before every pass)
+    long long lv_pre[MG_MAXLEV + 1];                       // rows of the levels before l
+    int lv_n[MG_MAXLEV];                                   // rows of level l
+    int nlev;
 };
+
+// fill the level table for n fine rows (one thread, then a barrier)
+DEV void mg_table(int n, MgSh &sh)
+{
+    SERIAL {
+        const int L = mg_levels(n);
+        long long pre = 0;
+        int m = n;
+        for (int l = 0; l < L; l++) {
+            sh.lv_n[l] = m;
+            sh.lv_pre[l] = pre;
+            pre += m;
+            m = (m + 1) / 2;
+        }
+        sh.lv_pre[L] = pre;
+        sh.nlev = L;
+    }
+    SERIAL_END
+}
 
 // one packed matrix row
 #ifdef LICA_EMUL
@@ -96,9 +120,9 @@ struct MgFine {
     int n;
 };
 
-DEV MgFine mg_fine(double *scr, int n)
+DEV MgFine mg_fine(double *scr, int n, const MgSh &sh)
 {
-    const long long tot = mg_rows_total(n);
+    const long long tot = sh.lv_pre[sh.nlev];
     MgFine F;
     F.n = n;
     F.m = (Row4<double> *)scr;
@@ -111,8 +135,8 @@ DEV MgFine mg_fine(double *scr, int n)
 template <class T>
 DEV MgLev<T> mg_lev(double *scr, int n, int l, const MgSh &sh)
 {
-    const int m = mg_rows(n, l);
-    const long long pre = mg_rows_before(n, l), tot = mg_rows_total(n);
+    const int m = sh.lv_n[l];
+    const long long pre = sh.lv_pre[l], tot = sh.lv_pre[sh.nlev];
     MgLev<T> v;
     v.n = m;
     if constexpr (sizeof(T) == 8) v.m = (Row4<T> *)scr + pre;
@@ -277,7 +301,7 @@ DEV void mg_coarsest(const MgLev<T> &L, MgSh &sh)
 template <class T>
 DEV void mg_vcycle(double *scr, int n, MgSh &sh)
 {
-    const int L = mg_levels(n);
+    const int L = sh.nlev;
     for (int l = 0; l + 1 < L; l++) {
         MgLev<T> F = mg_lev<T>(scr, n, l, sh), C = mg_lev<T>(scr, n, l + 1, sh);
         mg_smooth<T>(F, MG_NU, true);
@@ -362,10 +386,11 @@ DEV int mg_iterate(const MgFine &F, double *scr, int n, double rtol, int maxc, d
 DEV int mg_solve(const double *a, const double *b, const double *c, const double *g, double *p, int n,
                  double *scr, int mode, double rtol, int maxc, const double *x0, MgSh &sh, double *res_out)
 {
-    const int L = mg_levels(n);
+    mg_table(n, sh);
+    const int L = sh.nlev;
     const bool f32 = (mode != LICA_MG_FP64_MODE);
     const bool guess = (LICA_MG_WARM(mode) && x0);
-    MgFine F = mg_fine(scr, n);
+    MgFine F = mg_fine(scr, n, sh);
     Row4<float> *m32 = mg_lev<float>(scr, n, 0, sh).m;
     double fmax, anorm;
     red_max2(0, n, [&](int i, double &fm, double &an) {

@@ -76,7 +76,7 @@ struct BatchDev {
     int *nsi, *rfi;
     const int *order;    // workgroup -> case (largest cases first)
     int ninst;
-    int tri_mode;        // 0: serial Thomas, 1: partitioned
+    int tri_mode;        // 0: serial Thomas, 1: partitioned + refinement, 2: partitioned
     int msub;            // RK3 substep 1..3
     int pmode;           // Poisson mode (0/1 direct, 2 MG64, 3 mixed, 4 learned)
     int mg_maxc;         // V-cycle cap per solve
