@@ -37,21 +37,15 @@ DEV void xshift(const double *a, double *out)
         constexpr int lo = floor_div(t, CPL);
         constexpr int sp = t - lo * CPL;
         double v = a[sp];
-        if constexpr (lo > 0) v = __shfl_down(v, lo, 64);
-        if constexpr (lo < 0) v = __shfl_up(v, -lo, 64);
+        if constexpr (lo == 1) v = lane_dn1(v);
+        else if constexpr (lo > 1) v = __shfl_down(v, lo, 64);
+        if constexpr (lo == -1) v = lane_up1(v);
+        else if constexpr (lo < -1) v = __shfl_up(v, -lo, 64);
         out[s] = v;
     });
 }
-DEV double wave_max(double m)
-{
-    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
-    return m;
-}
-DEV double wave_min(double m)
-{
-    for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off, 64));
-    return m;
-}
+DEV double wave_max(double m) { return wave_red_max(m); }
+DEV double wave_min(double m) { return wave_red_min(m); }
 #endif
 
 DEV double weno_m5(double v1, double v2, double v3, double v4, double v5)

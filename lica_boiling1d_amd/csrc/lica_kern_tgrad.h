@@ -88,20 +88,16 @@ template <int CPL>
 DEV void xneighbors(const double *a, double *l, double *r)
 {
     if constexpr (CPL == 1) {
-        l[0] = __shfl_up(a[0], 1, 64);
-        r[0] = __shfl_down(a[0], 1, 64);
+        l[0] = lane_up1(a[0]);
+        r[0] = lane_dn1(a[0]);
     } else {
-        double up = __shfl_up(a[1], 1, 64);
-        double dn = __shfl_down(a[0], 1, 64);
+        double up = lane_up1(a[1]);
+        double dn = lane_dn1(a[0]);
         l[0] = up; r[0] = a[1];
         l[1] = a[0]; r[1] = dn;
     }
 }
-DEV double xwave_max(double m)
-{
-    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
-    return m;
-}
+DEV double xwave_max(double m) { return wave_red_max(m); }
 #endif
 
 // one forward-Euler stage on a (cells outside [ia,ib] stay as they are);

@@ -56,6 +56,48 @@ extern int lica_emul_nthreads;
 #define SERIAL if (threadIdx.x == 0)
 #define SERIAL_END __syncthreads();
 #define NTHREADS ((int)blockDim.x)
+
+// Lane moves of a 64-lane wave on DPP (data moved inside the VALU, no LDS
+// crossbar): CTRL 0x138 wave_shr:1 (lane i gets lane i-1), 0x130 wave_shl:1
+// (lane i gets lane i+1), 0x121..0x128 row_ror:1..8 (rotation within each
+// row of 16 lanes).  A lane without a source keeps its own value, as
+// __shfl_up/__shfl_down by one lane do at the wave's edge.
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    int lo = (int)(unsigned)(b & 0xffffffffull), hi = (int)(unsigned)(b >> 32);
+    lo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo);
+}
+__device__ __forceinline__ double lane_up1(double v) { return dpp_mov<0x138>(v); }   // = __shfl_up(v, 1)
+__device__ __forceinline__ double lane_dn1(double v) { return dpp_mov<0x130>(v); }   // = __shfl_down(v, 1)
+__device__ __forceinline__ double lane_read(double v, int lane)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffull), lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+// maximum / minimum over the 64 lanes (exact in any order), the same value
+// in every lane
+__device__ __forceinline__ double wave_red_max(double m)
+{
+    m = fmax(m, dpp_mov<0x121>(m));
+    m = fmax(m, dpp_mov<0x122>(m));
+    m = fmax(m, dpp_mov<0x124>(m));
+    m = fmax(m, dpp_mov<0x128>(m));
+    return fmax(fmax(lane_read(m, 0), lane_read(m, 16)), fmax(lane_read(m, 32), lane_read(m, 48)));
+}
+__device__ __forceinline__ double wave_red_min(double m)
+{
+    m = fmin(m, dpp_mov<0x121>(m));
+    m = fmin(m, dpp_mov<0x122>(m));
+    m = fmin(m, dpp_mov<0x124>(m));
+    m = fmin(m, dpp_mov<0x128>(m));
+    return fmin(fmin(lane_read(m, 0), lane_read(m, 16)), fmin(lane_read(m, 32), lane_read(m, 48)));
+}
 #endif
 
 // exact max/min over f(i), i in [lo,hi), combined with init
@@ -71,7 +113,7 @@ DEV double red_max(int lo, int hi, double init, F f, double *scr)
 #else
     double m = init;
     for (int i = lo + (int)threadIdx.x; i < hi; i += (int)blockDim.x) m = fmax(m, f(i));
-    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    m = wave_red_max(m);
     const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
     if ((threadIdx.x & 63) == 0) scr[w] = m;
     __syncthreads();
@@ -94,7 +136,7 @@ DEV double red_min(int lo, int hi, double init, F f, double *scr)
 #else
     double m = init;
     for (int i = lo + (int)threadIdx.x; i < hi; i += (int)blockDim.x) m = fmin(m, f(i));
-    for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off, 64));
+    m = wave_red_min(m);
     const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
     if ((threadIdx.x & 63) == 0) scr[w] = m;
     __syncthreads();
@@ -130,10 +172,8 @@ DEV void red_max2(int lo, int hi, F f, double *scr, double *m0, double *m1)
         a = fmax(a, u);
         b = fmax(b, v);
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        a = fmax(a, __shfl_xor(a, off, 64));
-        b = fmax(b, __shfl_xor(b, off, 64));
-    }
+    a = wave_red_max(a);
+    b = wave_red_max(b);
     const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
     if ((threadIdx.x & 63) == 0) { scr[2 * w] = a; scr[2 * w + 1] = b; }
     __syncthreads();
