@@ -260,9 +260,12 @@ DEV void reinit_band(BandSh &B, const InstParam &P, InstState &S, int nn1, int n
     (void)nw;
     reinit_regs<WMAX + 8>(B, P, nn1, nn2, ilimit);
 #else
-    if (nw <= 64) reinit_regs<1>(B, P, nn1, nn2, ilimit);
-    else if (nw <= 128) reinit_regs<2>(B, P, nn1, nn2, ilimit);
-    else reinit_regs<4>(B, P, nn1, nn2, ilimit);
+    // the first wave holds the window (a second wave of the workgroup waits)
+    if (threadIdx.x < 64) {
+        if (nw <= 64) reinit_regs<1>(B, P, nn1, nn2, ilimit);
+        else if (nw <= 128) reinit_regs<2>(B, P, nn1, nn2, ilimit);
+        else reinit_regs<4>(B, P, nn1, nn2, ilimit);
+    }
 #endif
     BARRIER();
     SERIAL { S.n_reinit += 1.0; }

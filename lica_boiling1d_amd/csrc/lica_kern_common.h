@@ -907,7 +907,6 @@ struct BandSh {
     // temperature extrapolation window (NS cells [nlo,nhi])
     int nlo, nhi;
     double t0[NSWMAX], t1[NSWMAX], nphi[NSWMAX], psih[NSWMAX];
-    double dtn[NSWMAX];
     double psitmp[NSWMAX];
     int mcell[2 * LICA_MAXENT];
     // per-plane tables of the window for the register-resident stages

@@ -1,7 +1,7 @@
-// TEMP_GRAD (chphase.f:55-368) on one wave per case: phase marking of the
+// TEMP_GRAD (chphase.f:55-368) on two waves per case: phase marking of the
 // NS cells under the band, the pseudo-time temperature extrapolation
-// TEMP_EXTP (467-803) on the NS window under the band, TF_GRAD (959-1159)
-// and VOL_TOT_MFDOT (1328-1493).
+// TEMP_EXTP (467-803) on the NS window under the band (one wave per phase),
+// TF_GRAD (959-1159) and VOL_TOT_MFDOT (1328-1493).
 #pragma once
 #include "lica_kern_band.h"
 
@@ -13,6 +13,8 @@ struct TgradSh {
     int tmk1[WMAX];
     double pmw2[WMAX];
     int tmk2[WMAX];
+    double xit[2], xhv[2];   // TEMP_EXTP iterations / halvings of phases 0, 1
+    int xst[2];
 };
 
 // One point of TEMP_EXTP_RK3 (chphase.f:806-956): the upwind pseudo-time
@@ -58,9 +60,9 @@ DEV double extp_point(double a, double al, double ar, double np, double npl, dou
 
 // The pseudo-time loops of TEMP_EXTP (chphase.f:619-679 for dT/dn, 704-769
 // for T) held in registers: window cell i = CPL*lane + s of slot s of the
-// lane (one or two cells per lane of the one wave; the CPU logic-check build
-// keeps the whole window in one lane), neighbours by lane shuffles, RAT by a
-// wave reduction.  Every cell update equals extp_point in the reference's
+// lane (one or two cells per lane of the phase's wave; the CPU logic-check
+// build keeps the whole window in one lane), neighbours by lane moves, the
+// RAT test by a wave vote.  Every cell update equals extp_point in the reference's
 // Jacobi order (all cells from the previous stage); the parts of it that
 // depend only on NPHI/PSIH are evaluated once per loop.
 template <int CPL>
@@ -80,10 +82,10 @@ DEV void xneighbors(const double *a, double *l, double *r)
         r[s] = s + 1 < CPL ? a[s + 1] : a[s];
     }
 }
-DEV double xwave_max(double m) { return m; }
+DEV bool xwave_any(bool f) { return f; }
 #else
 template <int CPL>
-DEV int xcell(int s) { return CPL * (int)threadIdx.x + s; }
+DEV int xcell(int s) { return CPL * (int)(threadIdx.x & 63) + s; }
 template <int CPL>
 DEV void xneighbors(const double *a, double *l, double *r)
 {
@@ -97,52 +99,101 @@ DEV void xneighbors(const double *a, double *l, double *r)
         l[1] = a[0]; r[1] = dn;
     }
 }
-DEV double xwave_max(double m) { return wave_red_max(m); }
+DEV bool xwave_any(bool f) { return __any((int)f) != 0; }
 #endif
 
 // one forward-Euler stage on a (cells outside [ia,ib] stay as they are);
-// the dT/dn pass (nd=0) uses the array being advanced as its DTEMPN
-template <int CPL>
-DEV void extp_stage(double *a, const XCells<CPL> &c, int im, int nd, double redtl, double inv_sdx1, double fmn,
-                    double fmx)
+// the dT/dn pass (ND=0) uses the array being advanced as its DTEMPN.  The
+// x/y terms of extp_point, min/max(nx,0) * (v-v)/sdx1 summed twice, are a
+// zero of one sign z0 for finite v and NaN otherwise, i.e. exactly
+// (v-v)*z0, and AX+AY+AZ = (z+z)+az = z+az; the z factors min/max(nz,0)
+// are -1/0 (IM=1) or 0/1 (IM=0).
+template <int CPL, int IM, int ND>
+DEV void extp_stage(double *a, const XCells<CPL> &c, double redtl, double z0)
 {
     double l[CPL], r[CPL], tn[CPL];
     xneighbors<CPL>(a, l, r);
-    const double ndd = (double)nd;
     for (int s = 0; s < CPL; s++) {
-        double v = a[s];
+        const double v = a[s];
+        double t = v;
         if (c.upd[s] && c.live[s]) {
-            // extp_point with nx=ny=0 (or -0), nz=+-1: fmn/fmx are min/max(nx,0),
-            // the z factors are min/max(nz,0)
-            double apxm = (v - v) * inv_sdx1;
-            double apxp = apxm;
-            double apzm = (v - l[s]) * c.vv[s];
-            double apzp = (r[s] - v) * c.vv[s];
-            double ax = fmn * apxp + fmx * apxm;
-            double az = (im == 1) ? ((-1.0) * apzp + 0.0 * apzm) : (0.0 * apzp + 1.0 * apzm);
-            double dt = (nd == 0) ? v : c.dtn[s];
-            double sum = ax + ax + az;
-            double drhs = (im == 1) ? sum + ndd * dt : sum - ndd * dt;
-            double t = v - redtl * drhs * c.coef[s];
-            if (c.small[s]) t = v - redtl * sum;
-            v = t;
+            const double z = (v - v) * z0;
+            const double apzm = (v - l[s]) * c.vv[s];
+            const double apzp = (r[s] - v) * c.vv[s];
+            const double az = IM ? (-apzp) + 0.0 * apzm : 0.0 * apzp + apzm;
+            const double sum = z + az;
+            double drhs;
+            if (ND == 0) drhs = IM ? sum + 0.0 * v : sum - 0.0 * v;
+            else drhs = IM ? sum + c.dtn[s] : sum - c.dtn[s];
+            t = v - redtl * drhs * c.coef[s];
+            if (ND == 0 && c.small[s]) t = v - redtl * sum;
         }
-        tn[s] = v;
+        tn[s] = t;
     }
     for (int s = 0; s < CPL; s++) a[s] = tn[s];
 }
 
-// TEMP_EXTP's two pseudo-time loops for phase im on the LDS window tw (the
-// dT/dn field is in B.dtn on entry).  Returns ST_EXTRAP after 20 halvings.
-template <int CPL>
-DEV int temp_extp_loops(BandSh &B, const InstParam &P, const double *VVDZ, double *tw, int im, int ia, int ib,
-                        double sm_grid, double eps, double *iters, double *halv)
+// fabs(x / rk - 1.0) >= eps for rk != 0, decided without the division where
+// the margin is clear: for |x| <= 1e290 and 1e-290 <= |rk| <= 1e290,
+// |x - rk| below (above) eps |rk| (1 -+ 2^-30) puts the exact quotient's
+// distance to 1 below (above) eps (1 -+ 2^-31), which the rounding of the
+// quotient (2^-53 relative) cannot cross; everything else takes the division
+DEV bool rel_change_fails(double x, double rk, double eps)
 {
+    const double ar = fabs(rk);
+    if (ar >= 1.0e-290 && ar <= 1.0e290 && fabs(x) <= 1.0e290) {
+        const double d = fabs(x - rk), th = eps * ar;
+        if (d < th * (1.0 - 0x1p-30)) return false;
+        if (d > th * (1.0 + 0x1p-30)) return true;
+    }
+    return fabs(x / rk - 1.0) >= eps;
+}
+
+// DTEMPN of phase IM at the start of TEMP_EXTP (chphase.f:510-596) for window
+// cell i (NS cell k in [ia,ib]).  NPHI vanishes on the three cells at either
+// end of the window, so the neighbours i-1, i+1 are read only inside it.
+template <int IM>
+DEV double extp_dtn_start(const BandSh &B, const InstParam &P, const double *PCN, const double *VVDZ,
+                          const double *tw, int i, int k, double sm_grid)
+{
+    const double np = B.nphi[i], pc = PCN[k];
+    double v = 0.0;
+    if (((fabs(np) > 1.5 * sm_grid) && (np != 0.0)) || ((fabs(pc - 0.5) == 0.5) && (np != 0.0))) {
+        const double npl = B.nphi[i - 1], npr = B.nphi[i + 1];
+        double dpx = 0.5 * (np - np) * (1.0 / P.sdx1);
+        double dtdx = 0.5 * (tw[i] - tw[i]) * (1.0 / P.sdx1);
+        double dpz, dtdz;
+        if (npl == 0.0) { dpz = (npr - np) * VVDZ[k + 1]; dtdz = (tw[i + 1] - tw[i]) * VVDZ[k + 1]; }
+        else if (npr == 0.0) { dpz = (np - npl) * VVDZ[k]; dtdz = (tw[i] - tw[i - 1]) * VVDZ[k]; }
+        else { dpz = 0.5 * (npr - npl) * VVDZ[k]; dtdz = 0.5 * (tw[i + 1] - tw[i - 1]) * VVDZ[k]; }
+        double dalphi = dpx * dpx + dpx * dpx + dpz * dpz;
+        if (dalphi != 0.0) {
+            double nx = 0.0, ny = 0.0, nz = 1.0;
+            if (IM == 1) v = nx * dtdx + ny * dtdx + nz * dtdz * B.psih[i];
+            else v = nx * dtdx + ny * dtdx + nz * dtdz * (1.0 - B.psih[i]);
+        }
+    } else if (((fabs(np) <= 1.5 * sm_grid) || (fabs(pc - 0.5) != 0.5)) && ((np != 1.0e-7) && (np != 0.0))) {
+        if (IM == 1) v = (tw[i] - P.tsat) / np * B.psih[i];
+        else v = (tw[i] - P.tsat) / np * (1.0 - B.psih[i]);
+    }
+    return v;
+}
+
+// TEMP_EXTP's two pseudo-time loops for phase IM on the LDS window tw, in
+// registers of one wave.  Returns ST_EXTRAP after 20 halvings.
+// RAT < EPS over the updated cells (a NaN ratio never counts, as with MAX)
+// is the wave's vote that no cell's ratio reaches EPS.
+template <int CPL, int IM>
+DEV int temp_extp_loops(BandSh &B, const InstParam &P, const double *PCN, const double *VVDZ, double *tw, int ia,
+                        int ib, double sm_grid, double eps, double *iters, double *halv)
+{
+    const int im = IM;
     const int nlo = B.nlo, nw = B.nhi - B.nlo + 1;
     const double inv_sdx1 = 1.0 / P.sdx1;
     double nx = 0.0;
     if (im == 1) nx = (-1.0) * nx;
     const double fmn = fmin(nx, 0.0), fmx = fmax(nx, 0.0);
+    const double z0 = fmn * (0.0 * inv_sdx1) + fmx * (0.0 * inv_sdx1);
     XCells<CPL> c;
     double a[CPL], rk[CPL];
     for (int s = 0; s < CPL; s++) {
@@ -168,8 +219,8 @@ DEV int temp_extp_loops(BandSh &B, const InstParam &P, const double *VVDZ, doubl
         c.coef[s] = (im == 1) ? (1.0 - ps) : ps;
         c.small[s] = fabs(np) <= sm_grid;
         c.live[s] = live;
-        c.dtn[s] = ok ? B.dtn[i] : 0.0;
         c.upd[s] = ok && k >= ia && k <= ib;
+        c.dtn[s] = c.upd[s] ? extp_dtn_start<IM>(B, P, PCN, VVDZ, tw, i, k, sm_grid) : 0.0;
         a[s] = c.dtn[s];
     }
     // (a) dT/dn (ND=0), continuing from the current field after a halving
@@ -179,21 +230,20 @@ DEV int temp_extp_loops(BandSh &B, const InstParam &P, const double *VVDZ, doubl
         int conv = 0;
         for (int it = 1; it <= 100; it++) {
             for (int s = 0; s < CPL; s++) rk[s] = a[s];
-            extp_stage<CPL>(a, c, im, 0, redtl, inv_sdx1, fmn, fmx);
-            extp_stage<CPL>(a, c, im, 0, redtl, inv_sdx1, fmn, fmx);
+            extp_stage<CPL, IM, 0>(a, c, redtl, z0);
+            extp_stage<CPL, IM, 0>(a, c, redtl, z0);
             for (int s = 0; s < CPL; s++) if (c.upd[s]) a[s] = 0.75 * rk[s] + 0.25 * a[s];
-            extp_stage<CPL>(a, c, im, 0, redtl, inv_sdx1, fmn, fmx);
-            double m = 0.0;
+            extp_stage<CPL, IM, 0>(a, c, redtl, z0);
+            bool fail = false;
             for (int s = 0; s < CPL; s++) {
-                if (c.upd[s]) a[s] = 1.0 / 3.0 * rk[s] + 2.0 / 3.0 * a[s];
-                double v = 0.0;
-                if (rk[s] != 0.0) v = fabs(a[s] / rk[s] - 1.0);
-                else if (a[s] != 0.0) v = fabs(rk[s] / a[s] - 1.0);
-                if (c.upd[s]) m = fmax(m, v);
+                if (!c.upd[s]) continue;
+                a[s] = 1.0 / 3.0 * rk[s] + 2.0 / 3.0 * a[s];
+                // rk = 0: |rk/a - 1| is 1 for any a but 0 and NaN
+                if (rk[s] != 0.0) fail = fail || rel_change_fails(a[s], rk[s], eps);
+                else fail = fail || (a[s] != 0.0 && a[s] == a[s]);
             }
-            double rat = xwave_max(m);
             *iters += 1.0;
-            if (fabs(rat) < eps) { conv = 1; break; }
+            if (!xwave_any(fail)) { conv = 1; break; }
         }
         if (conv) break;
         if (++halvings > 20) return ST_EXTRAP;
@@ -210,19 +260,19 @@ DEV int temp_extp_loops(BandSh &B, const InstParam &P, const double *VVDZ, doubl
         int conv = 0;
         for (int it = 1; it <= 100; it++) {
             for (int s = 0; s < CPL; s++) { rk[s] = t[s]; a[s] = t[s]; }
-            extp_stage<CPL>(a, c, im, 1, redtl, inv_sdx1, fmn, fmx);
-            extp_stage<CPL>(a, c, im, 1, redtl, inv_sdx1, fmn, fmx);
+            extp_stage<CPL, IM, 1>(a, c, redtl, z0);
+            extp_stage<CPL, IM, 1>(a, c, redtl, z0);
             for (int s = 0; s < CPL; s++) if (c.upd[s]) a[s] = 0.75 * rk[s] + 0.25 * a[s];
-            extp_stage<CPL>(a, c, im, 1, redtl, inv_sdx1, fmn, fmx);
-            double m = 0.0;
+            extp_stage<CPL, IM, 1>(a, c, redtl, z0);
+            bool fail = false;
             for (int s = 0; s < CPL; s++) {
-                if (c.upd[s]) { a[s] = 1.0 / 3.0 * rk[s] + 2.0 / 3.0 * a[s]; t[s] = a[s]; }
-                double v = (rk[s] != 0.0) ? fabs(t[s] / rk[s] - 1.0) : 0.0;
-                if (c.upd[s]) m = fmax(m, v);
+                if (!c.upd[s]) continue;
+                a[s] = 1.0 / 3.0 * rk[s] + 2.0 / 3.0 * a[s];
+                t[s] = a[s];
+                if (rk[s] != 0.0) fail = fail || rel_change_fails(t[s], rk[s], eps);
             }
-            double rat = xwave_max(m);
             *iters += 1.0;
-            if (fabs(rat) < eps) { conv = 1; break; }
+            if (!xwave_any(fail)) { conv = 1; break; }
         }
         if (conv) break;
         if (++halvings > 20) return ST_EXTRAP;
@@ -231,7 +281,7 @@ DEV int temp_extp_loops(BandSh &B, const InstParam &P, const double *VVDZ, doubl
     }
     for (int s = 0; s < CPL; s++) {
         int i = xcell<CPL>(s);
-        if (i < nw) { tw[i] = t[s]; B.dtn[i] = c.dtn[s]; }
+        if (i < nw) tw[i] = t[s];
     }
     return ST_OK;
 }
@@ -368,52 +418,51 @@ DEV void body_temp_grad(const BatchDev &bd, int inst, TgradSh &TS)
     const double sm_grid = fmin(fmin(2.0 * P.sdxf, 2.0 * P.sdyf), 2.0 * P.sdzf);
     const double eps = 1.0e-4;
     const int ia = nlo < 1 ? 1 : nlo, ib = nhi > n ? n : nhi;
-    for (int im = 0; im <= 1; im++) {
-        double *tw = (im == 0) ? B.t0 : B.t1;
-        PFOR(k, nlo, nhi + 1) { B.dtn[k - nlo] = 0.0; }
-        PFOR_END
-        PFOR(k, ia, ib + 1) {
-            int i = k - nlo;
-            double np = B.nphi[i], pc = PCN[k];
-            double v = 0.0;
-            if (((fabs(np) > 1.5 * sm_grid) && (np != 0.0)) || ((fabs(pc - 0.5) == 0.5) && (np != 0.0))) {
-                double dpx = 0.5 * (np - np) * (1.0 / P.sdx1);
-                double dtdx = 0.5 * (tw[i] - tw[i]) * (1.0 / P.sdx1);
-                double dpz, dtdz;
-                if (B.nphi[i - 1] == 0.0) { dpz = (B.nphi[i + 1] - B.nphi[i]) * VVDZ[k + 1]; dtdz = (tw[i + 1] - tw[i]) * VVDZ[k + 1]; }
-                else if (B.nphi[i + 1] == 0.0) { dpz = (B.nphi[i] - B.nphi[i - 1]) * VVDZ[k]; dtdz = (tw[i] - tw[i - 1]) * VVDZ[k]; }
-                else { dpz = 0.5 * (B.nphi[i + 1] - B.nphi[i - 1]) * VVDZ[k]; dtdz = 0.5 * (tw[i + 1] - tw[i - 1]) * VVDZ[k]; }
-                double dalphi = dpx * dpx + dpx * dpx + dpz * dpz;
-                if (dalphi != 0.0) {
-                    double nx = 0.0, ny = 0.0, nz = 1.0;
-                    if (im == 1) v = nx * dtdx + ny * dtdx + nz * dtdz * B.psih[i];
-                    else v = nx * dtdx + ny * dtdx + nz * dtdz * (1.0 - B.psih[i]);
-                }
-            } else if (((fabs(np) <= 1.5 * sm_grid) || (fabs(pc - 0.5) != 0.5)) && ((np != 1.0e-7) && (np != 0.0))) {
-                if (im == 1) v = (tw[i] - P.tsat) / np * B.psih[i];
-                else v = (tw[i] - P.tsat) / np * (1.0 - B.psih[i]);
-            }
-            B.dtn[i] = v;
-        }
-        PFOR_END
-        // the two pseudo-time loops, in registers of the one wave
-        BARRIER();
-        double iters = 0.0, halv = 0.0;
+    // the two phases are independent (phase im advances its own copy of the
+    // window, B.t0 or B.t1, from its own DTEMPN): on the GPU wave im runs
+    // phase im.  Counters and status combine in the reference's order: a
+    // failed phase 0 ends TEMP_GRAD before phase 1 starts, so phase 1's
+    // counts are dropped then.
 #ifdef LICA_EMUL
-        int xst = temp_extp_loops<NSWMAX>(B, P, VVDZ, tw, im, ia, ib, sm_grid, eps, &iters, &halv);
-#else
-        int xst = (nhi - nlo + 1 <= 64) ? temp_extp_loops<1>(B, P, VVDZ, tw, im, ia, ib, sm_grid, eps, &iters, &halv)
-                                        : temp_extp_loops<2>(B, P, VVDZ, tw, im, ia, ib, sm_grid, eps, &iters, &halv);
-#endif
-        BARRIER();
-        SERIAL {
-            S.n_extrap_iters += iters;
-            S.n_halvings += halv;
-            if (xst) S.status = xst;
-        }
-        SERIAL_END
-        if (xst) return;
+    for (int im = 0; im <= 1; im++) {
+        double iters = 0.0, halv = 0.0;
+        TS.xst[im] = im ? temp_extp_loops<NSWMAX, 1>(B, P, PCN, VVDZ, B.t1, ia, ib, sm_grid, eps, &iters, &halv)
+                        : temp_extp_loops<NSWMAX, 0>(B, P, PCN, VVDZ, B.t0, ia, ib, sm_grid, eps, &iters, &halv);
+        TS.xit[im] = iters;
+        TS.xhv[im] = halv;
     }
+#else
+    {
+        const int im = (int)(threadIdx.x >> 6);
+        double iters = 0.0, halv = 0.0;
+        int xst;
+        if (nhi - nlo + 1 <= 64)
+            xst = im ? temp_extp_loops<1, 1>(B, P, PCN, VVDZ, B.t1, ia, ib, sm_grid, eps, &iters, &halv)
+                     : temp_extp_loops<1, 0>(B, P, PCN, VVDZ, B.t0, ia, ib, sm_grid, eps, &iters, &halv);
+        else
+            xst = im ? temp_extp_loops<2, 1>(B, P, PCN, VVDZ, B.t1, ia, ib, sm_grid, eps, &iters, &halv)
+                     : temp_extp_loops<2, 0>(B, P, PCN, VVDZ, B.t0, ia, ib, sm_grid, eps, &iters, &halv);
+        if ((threadIdx.x & 63) == 0) {
+            TS.xst[im] = xst;
+            TS.xit[im] = iters;
+            TS.xhv[im] = halv;
+        }
+    }
+#endif
+    BARRIER();
+    SERIAL {
+        S.n_extrap_iters += TS.xit[0];
+        S.n_halvings += TS.xhv[0];
+        if (TS.xst[0]) {
+            S.status = TS.xst[0];
+        } else {
+            S.n_extrap_iters += TS.xit[1];
+            S.n_halvings += TS.xhv[1];
+            if (TS.xst[1]) S.status = TS.xst[1];
+        }
+    }
+    SERIAL_END
+    if (TS.xst[0] || TS.xst[1]) return;
     // (3) TF on the T-band, (4)-(6) TF_GRAD
     PFOR(e, B.off[1], B.off[LICA_NN1 + 1]) {
         int k = B.ent_k[e];

@@ -31,7 +31,7 @@ __global__ void __launch_bounds__(1024) k_cont_energy(BatchDev bd)
     tri_attach(sh);
     body_cont_energy(bd, bd.order[blockIdx.x], sh);
 }
-__global__ void __launch_bounds__(64) k_temp_grad(BatchDev bd)
+__global__ void __launch_bounds__(128) k_temp_grad(BatchDev bd)
 {
     __shared__ TgradSh sh;
     body_temp_grad(bd, bd.order[blockIdx.x], sh);
@@ -178,7 +178,7 @@ int lica_launch(int which, const BatchDev &bd, int threads, void *stream)
     case KL_INIT: LAUNCH(k_init, threads); break;
     case KL_DT: LAUNCH(k_dt, threads); break;
     case KL_CONT_ENERGY: LAUNCH_TRI(k_cont_energy, threads); break;
-    case KL_TEMP_GRAD: LAUNCH(k_temp_grad, 64); break;
+    case KL_TEMP_GRAD: LAUNCH(k_temp_grad, 128); break;
     case KL_MOMENTUM: LAUNCH_TRI(k_momentum, threads); break;
     case KL_POISSON_DIRECT: LAUNCH_TRI(k_poisson_direct, threads); break;
     case KL_UCALC: LAUNCH(k_ucalc, threads); break;
