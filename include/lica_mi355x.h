@@ -182,6 +182,12 @@ int lica_batch_set_net(lica_batch *b, lica_net *net);
  * and LEARNED modes, else the last solution -- the new solution and the
  * right-hand side (fields LICA_F_PPREV, LICA_F_PSOL, LICA_F_PRHS)          */
 int lica_batch_set_capture(lica_batch *b, int64_t on);
+/* No counterpart in the reference (diagnostic of the warm start): per case
+ * 4 doubles -- first row of the network's window, backward error of the
+ * warm-start base, its residual scale s_r and row scale s_b -- of the last
+ * network correction in LICA_POISSON_LEARNED (zeros before the first);
+ * len >= 4 * cases                                                        */
+int lica_batch_warm_info(lica_batch *b, double *out, int64_t len);
 
 /* ---- single-case API mirroring the reference call sequence ----------- */
 int lica_init(lica_batch **ctx, const char *lica_in_path, const char *grid_in_path);

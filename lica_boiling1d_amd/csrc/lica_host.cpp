@@ -432,6 +432,7 @@ BatchDev make_bd(const lica_batch *b, int msub)
     bd.guess = b->d_guess;
     bd.hist = b->d_hist;
     bd.cap = b->capture ? b->d_cap : nullptr;
+    bd.wscale = (b->poisson_mode == LICA_POISSON_LEARNED && b->net) ? b->d_scale : nullptr;
     bd.mg_lds = mg_lds(b);
     return bd;
 }
@@ -721,7 +722,7 @@ int lica_batch_create(int64_t ninst, const lica_case_desc *cases, lica_batch **o
     rc = rc ? rc : dmalloc((void **)&b->d_order, sizeof(int) * (size_t)ninst);
     rc = rc ? rc : dmalloc((void **)&b->d_guess, sizeof(double) * (size_t)gs_total);
     rc = rc ? rc : dmalloc((void **)&b->d_cap, sizeof(double) * 3 * (size_t)gs_total);
-    rc = rc ? rc : dmalloc((void **)&b->d_scale, sizeof(double) * 2 * (size_t)ninst);
+    rc = rc ? rc : dmalloc((void **)&b->d_scale, sizeof(double) * 4 * (size_t)ninst);
     if (rc) { lica_batch_destroy(b); return rc; }
     // workgroup order: largest cases first (longest-processing-time first)
     std::vector<int> order((size_t)ninst);
@@ -1065,6 +1066,16 @@ int lica_batch_set_capture(lica_batch *b, int64_t on)
     b->capture = (int)on;
     b->cap_valid = false;
     return LICA_OK;
+}
+
+// per case [first row of the network's window, backward error of the
+// warm-start base, s_r, s_b] of the last network correction
+int lica_batch_warm_info(lica_batch *b, double *out, int64_t len)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (!out || len < 4 * b->ninst) return LICA_BAD_ARG;
+    return d2h(out, b->d_scale, sizeof(double) * 4 * (size_t)b->ninst, b->stream);
 }
 
 // training windows from the captured systems of every healthy case; needs a

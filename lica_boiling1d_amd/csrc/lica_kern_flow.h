@@ -227,7 +227,12 @@ struct WarmSh {
 DEV void body_warm_base(const BatchDev &bd, int inst, WarmSh &sh)
 {
     InstState &S = bd.st[inst];
-    if (S.status || !bd.hist || !bd.guess) return;
+    if (S.status || !bd.hist || !bd.guess) {
+        // a stopped case gets no network correction
+        SERIAL { if (bd.wscale) bd.wscale[4 * inst + 1] = 0.0; }
+        SERIAL_END
+        return;
+    }
     CaseP cp = case_ptrs(bd, inst);
     const InstParam &P = *cp.P;
     const int n = P.n, s = bd.msub - 1;
@@ -293,12 +298,40 @@ DEV void body_warm_base(const BatchDev &bd, int inst, WarmSh &sh)
         for (int t = 0; t < nk; t++) sh.coef[idx[t]] = c[t];
     }
     SERIAL_END
-    PFOR(row, 1, n + 1) {
+    auto base = [&](int row) -> double {
         double v = 0.0;
         for (int j = 0; j < m; j++) v = v + sh.coef[j] * val(j, row);
-        B[row] = v;
+        return v;
+    };
+    if (!bd.wscale) {
+        PFOR(row, 1, n + 1) { B[row] = base(row); }
+        PFOR_END
+        return;
     }
-    PFOR_END
+    // with the network attached, the scales of its features in the same
+    // pass: max|r| of B's residual, max|b|, max(|a|+|b|+|c|), max|B|,
+    // max|g| (neighbours of B recomputed, bit for bit the stored values)
+    double mx[5];
+    red_maxN<5>(1, n + 1, [&](int row, double *v) {
+        const double x = base(row);
+        B[row] = x;
+        double ax = PB[row] * x;
+        if (row > 1) ax = ax + PA[row] * base(row - 1);
+        if (row < n) ax = ax + PCc[row] * base(row + 1);
+        const double ak = (row > 1) ? fabs(PA[row]) : 0.0, ck = (row < n) ? fabs(PCc[row]) : 0.0;
+        v[0] = fabs(PG[row] - ax);
+        v[1] = fabs(PB[row]);
+        v[2] = ak + fabs(PB[row]) + ck;
+        v[3] = fabs(x);
+        v[4] = fabs(PG[row]);
+    }, sh.red, mx);
+    SERIAL {
+        const double den = mx[2] * mx[3] + mx[4];
+        bd.wscale[4 * inst + 1] = (den > 0.0) ? mx[0] / den : mx[0];
+        bd.wscale[4 * inst + 2] = (mx[0] == 0.0) ? 1.0 : mx[0];
+        bd.wscale[4 * inst + 3] = (mx[1] == 0.0) ? 1.0 : mx[1];
+    }
+    SERIAL_END
 }
 
 // UCALC (1d.f:3582-3627), p -= p(1) (3433-3470), CONVRGE1 (3306-3362), the

@@ -68,7 +68,7 @@ DEV double extp_point(double a, double al, double ar, double np, double npl, dou
 template <int CPL>
 struct XCells {
     double vv[CPL], coef[CPL], dtn[CPL];
-    int upd[CPL], live[CPL], small[CPL];
+    int upd[CPL], act[CPL], small[CPL];   // act: updated and on a live cell
 };
 
 #ifdef LICA_EMUL
@@ -113,22 +113,21 @@ DEV void extp_stage(double *a, const XCells<CPL> &c, double redtl, double z0)
 {
     double l[CPL], r[CPL], tn[CPL];
     xneighbors<CPL>(a, l, r);
+    // every slot computes the update (no branch between the slots' chains);
+    // cells that do not move keep their value
     for (int s = 0; s < CPL; s++) {
         const double v = a[s];
-        double t = v;
-        if (c.upd[s] && c.live[s]) {
-            const double z = (v - v) * z0;
-            const double apzm = (v - l[s]) * c.vv[s];
-            const double apzp = (r[s] - v) * c.vv[s];
-            const double az = IM ? (-apzp) + 0.0 * apzm : 0.0 * apzp + apzm;
-            const double sum = z + az;
-            double drhs;
-            if (ND == 0) drhs = IM ? sum + 0.0 * v : sum - 0.0 * v;
-            else drhs = IM ? sum + c.dtn[s] : sum - c.dtn[s];
-            t = v - redtl * drhs * c.coef[s];
-            if (ND == 0 && c.small[s]) t = v - redtl * sum;
-        }
-        tn[s] = t;
+        const double z = (v - v) * z0;
+        const double apzm = (v - l[s]) * c.vv[s];
+        const double apzp = (r[s] - v) * c.vv[s];
+        const double az = IM ? (-apzp) + 0.0 * apzm : 0.0 * apzp + apzm;
+        const double sum = z + az;
+        double drhs;
+        if (ND == 0) drhs = IM ? sum + 0.0 * v : sum - 0.0 * v;
+        else drhs = IM ? sum + c.dtn[s] : sum - c.dtn[s];
+        double t = v - redtl * drhs * c.coef[s];
+        if (ND == 0) t = c.small[s] ? v - redtl * sum : t;
+        tn[s] = c.act[s] ? t : v;
     }
     for (int s = 0; s < CPL; s++) a[s] = tn[s];
 }
@@ -218,7 +217,7 @@ DEV int temp_extp_loops(BandSh &B, const InstParam &P, const double *PCN, const 
         c.vv[s] = vv;
         c.coef[s] = (im == 1) ? (1.0 - ps) : ps;
         c.small[s] = fabs(np) <= sm_grid;
-        c.live[s] = live;
+        c.act[s] = live && ok && k >= ia && k <= ib;
         c.upd[s] = ok && k >= ia && k <= ib;
         c.dtn[s] = c.upd[s] ? extp_dtn_start<IM>(B, P, PCN, VVDZ, tw, i, k, sm_grid) : 0.0;
         a[s] = c.dtn[s];
@@ -232,12 +231,12 @@ DEV int temp_extp_loops(BandSh &B, const InstParam &P, const double *PCN, const 
             for (int s = 0; s < CPL; s++) rk[s] = a[s];
             extp_stage<CPL, IM, 0>(a, c, redtl, z0);
             extp_stage<CPL, IM, 0>(a, c, redtl, z0);
-            for (int s = 0; s < CPL; s++) if (c.upd[s]) a[s] = 0.75 * rk[s] + 0.25 * a[s];
+            for (int s = 0; s < CPL; s++) a[s] = c.upd[s] ? 0.75 * rk[s] + 0.25 * a[s] : a[s];
             extp_stage<CPL, IM, 0>(a, c, redtl, z0);
             bool fail = false;
             for (int s = 0; s < CPL; s++) {
+                a[s] = c.upd[s] ? 1.0 / 3.0 * rk[s] + 2.0 / 3.0 * a[s] : a[s];
                 if (!c.upd[s]) continue;
-                a[s] = 1.0 / 3.0 * rk[s] + 2.0 / 3.0 * a[s];
                 // rk = 0: |rk/a - 1| is 1 for any a but 0 and NaN
                 if (rk[s] != 0.0) fail = fail || rel_change_fails(a[s], rk[s], eps);
                 else fail = fail || (a[s] != 0.0 && a[s] == a[s]);
@@ -262,13 +261,13 @@ DEV int temp_extp_loops(BandSh &B, const InstParam &P, const double *PCN, const 
             for (int s = 0; s < CPL; s++) { rk[s] = t[s]; a[s] = t[s]; }
             extp_stage<CPL, IM, 1>(a, c, redtl, z0);
             extp_stage<CPL, IM, 1>(a, c, redtl, z0);
-            for (int s = 0; s < CPL; s++) if (c.upd[s]) a[s] = 0.75 * rk[s] + 0.25 * a[s];
+            for (int s = 0; s < CPL; s++) a[s] = c.upd[s] ? 0.75 * rk[s] + 0.25 * a[s] : a[s];
             extp_stage<CPL, IM, 1>(a, c, redtl, z0);
             bool fail = false;
             for (int s = 0; s < CPL; s++) {
+                a[s] = c.upd[s] ? 1.0 / 3.0 * rk[s] + 2.0 / 3.0 * a[s] : a[s];
+                t[s] = c.upd[s] ? a[s] : t[s];
                 if (!c.upd[s]) continue;
-                a[s] = 1.0 / 3.0 * rk[s] + 2.0 / 3.0 * a[s];
-                t[s] = a[s];
                 if (rk[s] != 0.0) fail = fail || rel_change_fails(t[s], rk[s], eps);
             }
             *iters += 1.0;

@@ -36,10 +36,14 @@ struct NetPairSrc {
 };
 struct lica_net;
 // append up to wpc windows of len cells of every listed case to the network's
-// training set (features as k_net_features, target (p1 - p0) / max|p0|)
+// training set (features as k_net_features, target (p1 - p0) over its
+// maximum magnitude in the window)
 int lica_net_append(lica_net *t, const NetPairSrc *hsrc, int ncase, int64_t wpc, int64_t len, uint64_t seed,
                     int64_t *added);
 int64_t lica_net_window();   // cells of the corrected window around the interface
+// network correction of the warm-start guesses of a batch; dinfo holds 4
+// doubles per case, [1..3] = backward error, s_r, s_b of the base (written
+// by the warm-start base kernel), [0] = window start (written here)
 int lica_net_guess(lica_net *t, const NetCase *dcases, int ncase, const int64_t *lens, double *dinfo, double rtol,
                    void *stream);
 int lica_net_prepare(lica_net *t, const int64_t *lens, int ncase, const NetCase *dcases);

@@ -326,16 +326,18 @@ __device__ __forceinline__ int net_k0(const NetCase &cs)
 }
 
 // features of the window of every case of a batch (one workgroup per case).
-// A case whose base already meets rtol gets an empty sequence (its tiles do
-// no work) and keeps B; info[2i] = first row of the window.
+// info[4i+1..3] hold the backward error of case i's base B and the scales
+// s_r, s_b (net_scales over all rows, written by the warm-start base kernel
+// in its pass over B).  A case whose base already meets rtol gets an empty
+// sequence (its tiles do no work) and keeps B; info[4i] = first row of the
+// window.
 __global__ void __launch_bounds__(256) k_net_features(const NetCase *__restrict__ cases, __bf16 *__restrict__ X,
                                                       NetSeq *__restrict__ seqs, double *__restrict__ info,
                                                       double rtol)
 {
     const NetCase cs = cases[blockIdx.x];
     const int tid = threadIdx.x;
-    double sr, sb, berr;
-    net_scales(cs.a, cs.b, cs.c, cs.g, cs.guess, cs.n, &sr, &sb, &berr);
+    const double berr = info[4 * blockIdx.x + 1], sr = info[4 * blockIdx.x + 2], sb = info[4 * blockIdx.x + 3];
     const int k0 = net_k0(cs);
     const bool skip = !(berr > rtol);
     if (!skip)
@@ -346,7 +348,7 @@ __global__ void __launch_bounds__(256) k_net_features(const NetCase *__restrict_
         }
     if (tid == 0) {
         seqs[blockIdx.x].len = skip ? 0 : cs.win;
-        info[2 * blockIdx.x] = (double)k0;
+        info[4 * blockIdx.x] = (double)k0;
     }
 }
 
@@ -384,7 +386,7 @@ __global__ void __launch_bounds__(256) k_net_guess(const NetCase *__restrict__ c
 {
     const NetCase cs = cases[blockIdx.x];
     if (seqs[blockIdx.x].len == 0) return;
-    const int k0 = (int)info[2 * blockIdx.x], n = cs.n, w = cs.win;
+    const int k0 = (int)info[4 * blockIdx.x], n = cs.n, w = cs.win;
     auto d = [&](int k) -> double { return (k >= k0 && k < k0 + w) ? (double)y[cs.off + (k - k0)] : 0.0; };
     double v[2] = {0.0, 0.0};
     for (int k = max(k0 - 1, 1) + (int)threadIdx.x; k <= min(k0 + w, n); k += blockDim.x) {

@@ -185,6 +185,44 @@ DEV void red_max2(int lo, int hi, F f, double *scr, double *m0, double *m1)
 #endif
 }
 
+// N maxima at once over f(i, v[N]) (v >= 0), i in [lo,hi); f may also write
+// memory, which is visible to the whole workgroup on return
+template <int N, class F>
+DEV void red_maxN(int lo, int hi, F f, double *scr, double *out)
+{
+#ifdef LICA_EMUL
+    (void)scr;
+    lica_emul_check();
+    double acc[N];
+    for (int q = 0; q < N; q++) acc[q] = 0.0;
+    for (int i = lo; i < hi; ++i) {
+        double v[N];
+        f(i, v);
+        for (int q = 0; q < N; q++) acc[q] = fmax(acc[q], v[q]);
+    }
+    for (int q = 0; q < N; q++) out[q] = acc[q];
+#else
+    double acc[N];
+    for (int q = 0; q < N; q++) acc[q] = 0.0;
+    for (int i = lo + (int)threadIdx.x; i < hi; i += (int)blockDim.x) {
+        double v[N];
+        f(i, v);
+        for (int q = 0; q < N; q++) acc[q] = fmax(acc[q], v[q]);
+    }
+    for (int q = 0; q < N; q++) acc[q] = wave_red_max(acc[q]);
+    const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int q = 0; q < N; q++) scr[w * N + q] = acc[q];
+    __syncthreads();
+    for (int q = 0; q < N; q++) {
+        double r = 0.0;
+        for (int i = 0; i < nw; ++i) r = fmax(r, scr[i * N + q]);
+        out[q] = r;
+    }
+    __syncthreads();
+#endif
+}
+
 // N sums at once over f(i, v[N]), i in [lo,hi); the order of the additions
 // is fixed for a given workgroup size
 template <int N, class F>

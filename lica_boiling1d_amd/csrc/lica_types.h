@@ -86,6 +86,8 @@ struct BatchDev {
     double *guess;       // initial guesses of the warm-started modes (rows at gs_off)
     double *hist;        // solution history of the warm-started modes
     double *cap;         // P before / after the last substep's solve, its RHS (3 x (n+2) at 3*gs_off)
+    double *wscale;      // per case 4 doubles: network window start, backward error of the warm-start
+                         // base, its residual scale s_r, the row scale s_b (null without the network)
 };
 
 #define LICA_MG_FP64_MODE 2

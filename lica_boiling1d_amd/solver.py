@@ -146,6 +146,13 @@ class Batch(_Handle):
         """Keep the POISSON system of substep `substep` (1..3) of every step."""
         _check(self._lib.lica_batch_set_capture(self._h, int(substep) if on else 0), "set_capture")
 
+    def warm_info(self):
+        """Per case [first row of the network window, backward error of the
+        warm-start base, s_r, s_b] of the last learned-mode correction."""
+        out = np.zeros((len(self.cases), 4))
+        _check(self._lib.lica_batch_warm_info(self._h, _dptr(out), out.size), "warm_info")
+        return out
+
     def set_timing(self, on=True):
         _check(self._lib.lica_batch_set_timing(self._h, 1 if on else 0), "set_timing")
 

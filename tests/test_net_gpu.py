@@ -129,6 +129,45 @@ def test_device_pairs_equal_host_features(mode):
     b.close()
 
 
+def _base_scales(a, b, c, g, x):
+    """Backward error of x and the feature scales s_r, s_b of one system
+    (rows 1..n), in the residual's order of operations on the device."""
+    v = b * x
+    v[1:] = v[1:] + a[1:] * x[:-1]
+    v[:-1] = v[:-1] + c[:-1] * x[1:]
+    r = g - v
+    ak = np.r_[0.0, np.abs(a[1:])]
+    ck = np.r_[np.abs(c[:-1]), 0.0]
+    m0, m1 = np.max(np.abs(r)), np.max(np.abs(b))
+    m2, m3, m4 = np.max((ak + np.abs(b)) + ck), np.max(np.abs(x)), np.max(np.abs(g))
+    den = m2 * m3 + m4
+    return (m0 / den if den > 0 else m0), (m0 or 1.0), (m1 or 1.0)
+
+
+def test_learned_base_scales_equal_host():
+    """The warm-start base kernel's pass over B also yields the backward
+    error and the scales the network's features use; they equal, bit for bit,
+    the ones computed on the host from the captured rows of the same substep
+    (the base B before the network's correction, the matrix, the RHS)."""
+    _say("learned-mode base scales, 3 cases n=200")
+    cases = [L.deck_case(200, 2).copy(z0=z) for z in (0.05, 0.0607, 0.08)]
+    net = Net(seed=0)
+    b = L.Batch(cases)
+    b.set_net(net)
+    b.set_poisson(L.POISSON_LEARNED, 1e-10, 100)
+    b.run(3)
+    b.set_capture(True, substep=3)
+    b.run(1)
+    info = b.warm_info()
+    for i in range(3):
+        a, bb, c, g = (b.field(k, i) for k in ("PA", "PB", "PC", "PRHS"))
+        x = b.field("PPREV", i)
+        assert tuple(info[i, 1:]) == _base_scales(a, bb, c, g, x), i
+        assert info[i, 0] == 1.0   # the window covers the whole column
+    b.close()
+    net.close()
+
+
 def test_add_pairs_needs_capture_and_multigrid():
     cases = [L.deck_case(128, 2)]
     net = Net(seed=0)
