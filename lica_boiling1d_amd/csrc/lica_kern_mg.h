@@ -64,7 +64,7 @@ HOSTDEV long long mg_scratch_doubles(int n)
 HOSTDEV long long mg_lds_bytes(int n, int tsize) { return 2LL * tsize * mg_rows_total(n); }
 
 struct MgSh {
-    double red[32];                                        // two-value reductions
+    double red[64];                                        // reductions of up to four values
     double cg[MG_COARSE], cib[MG_COARSE], ca[MG_COARSE];   // coarsest-level elimination factors
     double cy[MG_COARSE];                                  // its forward sweep
     void *lds;                                             // x, f of the coarse levels
@@ -386,10 +386,35 @@ DEV int mg_iterate(const MgFine &F, double *scr, int n, double rtol, int maxc, d
 DEV int mg_solve(const double *a, const double *b, const double *c, const double *g, double *p, int n,
                  double *scr, int mode, double rtol, int maxc, const double *x0, MgSh &sh, double *res_out)
 {
+    const bool guess = (LICA_MG_WARM(mode) && x0);
+    if (guess) {
+        // a warm start usually meets rtol as it is: its backward error from the
+        // rows themselves, before anything is packed (the maxima of the packed
+        // path below, so the same decision and, when it is taken, p = x0)
+        double mx[4];   // max|g|, max(|a|+|b|+|c|), max|r|, max|x0|
+        red_maxN<4>(0, n, [&](int i, double *v) {
+            const double ai = (i == 0) ? 0.0 : a[i + 1], bi = b[i + 1], ci = (i == n - 1) ? 0.0 : c[i + 1];
+            const double xi = x0[i + 1];
+            double r = g[i + 1] - bi * xi;
+            if (i > 0) r = r - ai * x0[i];
+            if (i < n - 1) r = r - ci * x0[i + 2];
+            v[0] = fabs(g[i + 1]);
+            v[1] = fabs(ai) + fabs(bi) + fabs(ci);
+            v[2] = fabs(r);
+            v[3] = fabs(xi);
+        }, sh.red, mx);
+        const double den = mx[1] * mx[3] + mx[0];
+        const double res0 = (den > 0.0) ? mx[2] / den : mx[2];
+        if (!(mx[2] > mx[0]) && !(res0 > rtol)) {
+            PFOR(i, 0, n) { p[i + 1] = x0[i + 1]; }
+            PFOR_END
+            *res_out = res0;
+            return 0;
+        }
+    }
     mg_table(n, sh);
     const int L = sh.nlev;
     const bool f32 = (mode != LICA_MG_FP64_MODE);
-    const bool guess = (LICA_MG_WARM(mode) && x0);
     MgFine F = mg_fine(scr, n, sh);
     Row4<float> *m32 = mg_lev<float>(scr, n, 0, sh).m;
     double fmax, anorm;
