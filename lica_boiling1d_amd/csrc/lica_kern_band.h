@@ -90,6 +90,34 @@ DEV Fill fill_params(const BandSh &B, const InstParam &P)
         f.tmp1 = (phi_get(B, khigh, P.n3f) < 0.0) ? 0.0 : 1.0;
         f.tmp_outer = (phi_get(B, f.kc_low + 1, P.n3f) < 0.0) ? 0.0 : 1.0;
     }
+    f.khigh = khigh;
+    return f;
+}
+
+// fill_params with the scan as workgroup reductions (call from every
+// thread; every thread gets the result): the count of sign-change planes,
+// the highest (the downward scan's first) and the lowest (its last)
+DEV Fill fill_params_all(BandSh &B, const InstParam &P)
+{
+    const int kt = B.whi < P.n3fm ? B.whi : P.n3fm, kb = B.wlo < 1 ? 1 : B.wlo;
+    double hi, lo, cnt[1];
+    red_max2(kb, kt + 1, [&](int k, double &u, double &v) {
+        const bool c = phi_get(B, k, P.n3f) * phi_get(B, k + 1, P.n3f) < 0.0;
+        u = c ? (double)k : 0.0;
+        v = c ? (double)(kt + 1 - k) : 0.0;
+    }, B.red, &hi, &lo);
+    red_sumN<1>(kb, kt + 1, [&](int k, double *v) {
+        v[0] = (phi_get(B, k, P.n3f) * phi_get(B, k + 1, P.n3f) < 0.0) ? 1.0 : 0.0;
+    }, B.red, cnt);
+    Fill f;
+    f.knum = (int)cnt[0];
+    f.kc_low = 0; f.tmp1 = 1.0; f.tmp_outer = 1.0; f.khigh = 0;
+    if (f.knum > 0) {
+        f.khigh = (int)hi;
+        f.kc_low = kt + 1 - (int)lo;
+        f.tmp1 = (phi_get(B, f.khigh, P.n3f) < 0.0) ? 0.0 : 1.0;
+        f.tmp_outer = (phi_get(B, f.kc_low + 1, P.n3f) < 0.0) ? 0.0 : 1.0;
+    }
     return f;
 }
 

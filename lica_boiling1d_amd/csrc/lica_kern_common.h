@@ -916,6 +916,7 @@ struct BandSh {
 struct Fill {           // CAL_PSIF_MTP column fill outside the band (two_phase.f:3011-3075)
     int knum, kc_low;
     double tmp1, tmp_outer;
+    int khigh;          // highest sign-change plane (0 when knum == 0)
 };
 
 DEV double phi_get(const BandSh &B, int k, int n3f)

@@ -5,6 +5,9 @@
 #pragma once
 #include "lica_kern_band.h"
 
+// the N-tube entries (layers 1..NN2) precede the outermost layer's
+static_assert(LICA_NN2 + 1 <= LICA_MLAYER, "layer order of the band entries");
+
 struct TgradSh {
     BandSh B;
     int nbk[LICA_MLAYER + 1][LICA_MAXP];
@@ -352,31 +355,38 @@ DEV void body_temp_grad(const BatchDev &bd, int inst, TgradSh &TS)
     double *SFGAS = cp.R(RF_SFGAS), *SFGAS2 = cp.R(RF_SFGAS2);
     const double *VPREF = cp.R(RF_VPREFIX);
 
+    // the band of the case: counts first, then the cells (in parallel),
+    // then the entry tables
     SERIAL {
         B.status = 0;
-        for (int nl = 0; nl <= LICA_MLAYER + 1; nl++) B.bn[nl] = 0;
-        for (int nl = 1; nl <= LICA_MLAYER; nl++) {
-            B.bn[nl] = S.bn[nl];
-            for (int q = 0; q < S.bn[nl]; q++) B.bk[nl][q] = S.bk[nl][q];
-        }
-        band_index(B, P, 1);
+        for (int nl = 0; nl <= LICA_MLAYER + 1; nl++) B.bn[nl] = (nl >= 1 && nl <= LICA_MLAYER) ? S.bn[nl] : 0;
         for (int i = 0; i < 3; i++) B.gl[i] = 0.0;
         for (int i = 0; i < 4; i++) B.gu[i] = 0.0;
-        // NS window under the band
-        int kmin = 1 << 30, kmax = -1;
-        for (int e = 0; e < B.off[LICA_MLAYER + 1]; e++) {
-            int kk = KCMV[B.ent_k[e]];
-            if (kk < kmin) kmin = kk;
-            if (kk > kmax) kmax = kk;
-        }
-        if (kmax < 0) { kmin = 1; kmax = 1; }
-        B.nlo = kmin - 3 < 0 ? 0 : kmin - 3;
-        B.nhi = kmax + 3 > n3 ? n3 : kmax + 3;
-        if (B.nhi - B.nlo + 1 > NSWMAX) B.status = ST_WINDOW;
-        for (int nl = 0; nl <= LICA_MLAYER; nl++)
-            for (int q = 0; q < LICA_MAXP; q++) S.mflvs[nl][q] = 0.0;
     }
     SERIAL_END
+    PFOR(j, 0, LICA_MLAYER * LICA_MAXP) {
+        const int nl = 1 + j / LICA_MAXP, q = j % LICA_MAXP;
+        if (q < B.bn[nl]) B.bk[nl][q] = S.bk[nl][q];
+    }
+    PFOR_END
+    PFOR(j, 0, (LICA_MLAYER + 1) * LICA_MAXP) { S.mflvs[j / LICA_MAXP][j % LICA_MAXP] = 0.0; }
+    PFOR_END
+    SERIAL { band_index(B, P, 1); }
+    SERIAL_END
+    {
+        // NS window under the band
+        const int ne = B.off[LICA_MLAYER + 1];
+        const double kmn = red_min(0, ne, 1073741824.0, [&](int e) { return (double)KCMV[B.ent_k[e]]; }, B.red);
+        const double kmx = red_max(0, ne, -1.0, [&](int e) { return (double)KCMV[B.ent_k[e]]; }, B.red);
+        SERIAL {
+            int kmin = (int)kmn, kmax = (int)kmx;
+            if (kmax < 0) { kmin = 1; kmax = 1; }
+            B.nlo = kmin - 3 < 0 ? 0 : kmin - 3;
+            B.nhi = kmax + 3 > n3 ? n3 : kmax + 3;
+            if (B.nhi - B.nlo + 1 > NSWMAX) B.status = ST_WINDOW;
+        }
+        SERIAL_END
+    }
     if (B.status) { SERIAL { S.status = B.status; } SERIAL_END return; }
     const int nlo = B.nlo, nhi = B.nhi;
     PFOR(i, 0, WMAX) { B.phw[i] = 0.0; B.tf0[i] = 0.0; B.tf1[i] = 0.0; }
@@ -388,28 +398,49 @@ DEV void body_temp_grad(const BatchDev &bd, int inst, TgradSh &TS)
         B.t0[i] = T[k]; B.t1[i] = T[k]; B.nphi[i] = 0.0; B.psih[i] = 0.0;
     }
     PFOR_END
-    // (1) phase of the NS cells under the band (chphase.f:174-211)
+    // (1) phase of the NS cells under the band (chphase.f:174-211).  The
+    // lookups and PHINSINTP of every entry run in parallel (PHINSINTP is a
+    // function of the NS cell; xe = the phase flag it sets, -1 if it leaves
+    // IC as it is); the reference's order of updates -- which entry sets a
+    // cell, the running flag IC -- is replayed serially from LDS.  B.pm1,
+    // B.pm2 and B.mcell are free until step (3).
+    const int e0 = B.off[1], e1 = B.off[LICA_NN2 + 1], e2 = B.off[LICA_MLAYER], e3 = B.off[LICA_MLAYER + 1];
+    int *xk = B.mcell, *xe = B.mcell + LICA_MAXENT;
+    PFOR(e, e0, e1) {
+        const int kk = KCMV[B.ent_k[e]];
+        int ic = -1;
+        B.pm1[e] = phinsintp(B, cp, kk, &ic);
+        B.pm2[e] = PCN[kk];
+        xk[e] = kk;
+        xe[e] = ic;
+    }
+    PFOR_END
+    PFOR(e, e2, e3) {
+        const int k = B.ent_k[e];
+        const int kk = KCMV[k];
+        const double dz = ZPF[k] - ZP[kk];
+        const double d2 = P.dxy2_min + dz * dz;
+        B.pm1[e] = phi_get(B, k, n3f) + sqrt(d2);
+        B.pm2[e] = PCN[kk];
+        xk[e] = kk;
+    }
+    PFOR_END
     SERIAL {
         int ic = 0;
-        for (int e = B.off[1]; e < B.off[LICA_NN2 + 1]; e++) {
-            int kk = KCMV[B.ent_k[e]];
-            int i = kk - nlo;
+        for (int e = e0; e < e1; e++) {
+            const int i = xk[e] - nlo;
             if (B.nphi[i] == 0.0) {
-                double v = phinsintp(B, cp, kk, &ic);
-                B.nphi[i] = v;
-                if (fabs(PCN[kk] - 0.5) != 0.5) B.psih[i] = (double)ic;
-                else B.psih[i] = PCN[kk];
+                if (xe[e] >= 0) ic = xe[e];
+                B.nphi[i] = B.pm1[e];
+                const double pc = B.pm2[e];
+                if (fabs(pc - 0.5) != 0.5) B.psih[i] = (double)ic;
+                else B.psih[i] = pc;
             }
         }
-        for (int e = B.off[LICA_MLAYER]; e < B.off[LICA_MLAYER + 1]; e++) {
-            int k = B.ent_k[e];
-            int kk = KCMV[k];
-            int i = kk - nlo;
-            double dz = ZPF[k] - ZP[kk];
-            double d2 = P.dxy2_min + dz * dz;
-            double pre = phi_get(B, k, n3f) + sqrt(d2);
-            B.nphi[i] = fmin(B.nphi[i], pre);
-            B.psih[i] = PCN[kk];
+        for (int e = e2; e < e3; e++) {
+            const int i = xk[e] - nlo;
+            B.nphi[i] = fmin(B.nphi[i], B.pm1[e]);
+            B.psih[i] = B.pm2[e];
         }
     }
     SERIAL_END
@@ -470,13 +501,7 @@ DEV void body_temp_grad(const BatchDev &bd, int inst, TgradSh &TS)
     }
     PFOR_END
     psif_mtp_band(B, P, LICA_NN1, B.pm1);
-    SERIAL {
-        Fill f = fill_params(B, P);
-        B.v[2] = f.tmp1; B.v[3] = f.tmp_outer; B.iv[2] = f.knum; B.iv[3] = f.kc_low;
-    }
-    SERIAL_END
-    Fill fill1;
-    fill1.tmp1 = B.v[2]; fill1.tmp_outer = B.v[3]; fill1.knum = B.iv[2]; fill1.kc_low = B.iv[3];
+    const Fill fill1 = fill_params_all(B, P);
     PFOR(e, B.off[1], B.off[LICA_NN1 - 2 + 1]) {
         int k = B.ent_k[e];
         double gx = 0.5 * (0.0 - 0.0) * (1.0 / P.sdxf);
@@ -513,13 +538,11 @@ DEV void body_temp_grad(const BatchDev &bd, int inst, TgradSh &TS)
     reinit_band(B, P, S, LICA_NN1, LICA_NN2, LICA_NN2);
     psif_mtp_band(B, P, LICA_NN1, B.pm2);
     psif_view(B, LICA_NN1, B.pm2, TS.pmw2, TS.tmk2);
+    // the interface plane: the ascending scan keeps the last sign change,
+    // the highest one of the fill scan
+    const Fill f2 = fill_params_all(B, P);
     SERIAL {
-        double place = 0.0;
-        int kt = B.whi < P.n3fm ? B.whi : P.n3fm;
-        for (int k = (B.wlo < 1 ? 1 : B.wlo); k <= kt; k++)
-            if (phi_get(B, k, n3f) * phi_get(B, k + 1, n3f) < 0.0) place = ZPF[k] + fabs(phi_get(B, k, n3f));
-        S.place = place;
-        Fill f2 = fill_params(B, P);
+        S.place = (f2.knum > 0) ? ZPF[f2.khigh] + fabs(phi_get(B, f2.khigh, n3f)) : 0.0;
         S.vol_tot_iphs = vol_implicit(B, P, VPREF, f2, TS.pmw2, TS.tmk2);
     }
     SERIAL_END
