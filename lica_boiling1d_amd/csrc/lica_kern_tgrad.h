@@ -8,14 +8,12 @@
 // the N-tube entries (layers 1..NN2) precede the outermost layer's
 static_assert(LICA_NN2 + 1 <= LICA_MLAYER, "layer order of the band entries");
 
+// LDS of a workgroup (the band, its windows, the TEMP_EXTP window); the
+// window views of psif_mtp that VOL_TOT_MFDOT and CAL_PSI_TMP need reuse
+// B.tf0/B.mask_s and B.tf1/B.slot, free by then (TF_GRAD and the
+// reinitialisation are done), which keeps six workgroups on a CU
 struct TgradSh {
     BandSh B;
-    int nbk[LICA_MLAYER + 1][LICA_MAXP];
-    int nbn[LICA_MLAYER + 1];
-    double pmw1[WMAX];
-    int tmk1[WMAX];
-    double pmw2[WMAX];
-    int tmk2[WMAX];
     double xit[2], xhv[2];   // TEMP_EXTP iterations / halvings of phases 0, 1
     int xst[2];
 };
@@ -383,7 +381,8 @@ DEV void body_temp_grad(const BatchDev &bd, int inst, TgradSh &TS)
             if (kmax < 0) { kmin = 1; kmax = 1; }
             B.nlo = kmin - 3 < 0 ? 0 : kmin - 3;
             B.nhi = kmax + 3 > n3 ? n3 : kmax + 3;
-            if (B.nhi - B.nlo + 1 > NSWMAX) B.status = ST_WINDOW;
+            // CAL_PSI_TMP's rows reach one cell beyond either end
+            if (B.nhi - B.nlo + 3 > NSWMAX) B.status = ST_WINDOW;
         }
         SERIAL_END
     }
@@ -537,13 +536,15 @@ DEV void body_temp_grad(const BatchDev &bd, int inst, TgradSh &TS)
     PFOR_END
     reinit_band(B, P, S, LICA_NN1, LICA_NN2, LICA_NN2);
     psif_mtp_band(B, P, LICA_NN1, B.pm2);
-    psif_view(B, LICA_NN1, B.pm2, TS.pmw2, TS.tmk2);
+    double *pmw2 = B.tf0;
+    int *tmk2 = B.mask_s;
+    psif_view(B, LICA_NN1, B.pm2, pmw2, tmk2);
     // the interface plane: the ascending scan keeps the last sign change,
     // the highest one of the fill scan
     const Fill f2 = fill_params_all(B, P);
     SERIAL {
         S.place = (f2.knum > 0) ? ZPF[f2.khigh] + fabs(phi_get(B, f2.khigh, n3f)) : 0.0;
-        S.vol_tot_iphs = vol_implicit(B, P, VPREF, f2, TS.pmw2, TS.tmk2);
+        S.vol_tot_iphs = vol_implicit(B, P, VPREF, f2, pmw2, tmk2);
     }
     SERIAL_END
     // SFGAS/SFGAS2: clear the previous window, then write the new one
@@ -575,10 +576,12 @@ DEV void body_temp_grad(const BatchDev &bd, int inst, TgradSh &TS)
     PFOR(i, 0, WMAX) { B.phw[i] = B.phs[i]; }
     PFOR_END
     // (7) CAL_PSI_TMP(1) and IMASK_PH (chphase.f:300-358)
-    psif_view(B, LICA_NN1, B.pm1, TS.pmw1, TS.tmk1);
+    double *pmw1 = B.tf1;
+    int *tmk1 = B.slot;
+    psif_view(B, LICA_NN1, B.pm1, pmw1, tmk1);
     const int ka = (nlo - 1) < 1 ? 1 : nlo - 1, kb = (nhi + 1) > n ? n : nhi + 1;
     PFOR(k, ka, kb + 1) {
-        auto fpm = [&](int kk) { return psif_implicit(B, fill1, TS.pmw1, TS.tmk1, kk); };
+        auto fpm = [&](int kk) { return psif_implicit(B, fill1, pmw1, tmk1, kk); };
         double ps = box_avg(ZF, P.ssdzf, fpm, Z[k], Z[k + 1], KCOU2[k], KCOU1[k + 1]);
         if (ps <= 1.0e-3) ps = 0.0;
         else if (ps >= 0.999) ps = 1.0;
