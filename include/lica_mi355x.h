@@ -183,10 +183,12 @@ int lica_batch_set_net(lica_batch *b, lica_net *net);
  * right-hand side (fields LICA_F_PPREV, LICA_F_PSOL, LICA_F_PRHS)          */
 int lica_batch_set_capture(lica_batch *b, int64_t on);
 /* No counterpart in the reference (diagnostic of the warm start): per case
- * 4 doubles -- first row of the network's window, backward error of the
- * warm-start base, its residual scale s_r and row scale s_b -- of the last
- * network correction in LICA_POISSON_LEARNED (zeros before the first);
- * len >= 4 * cases                                                        */
+ * 7 doubles of the last learned-mode substep -- first row of the network's
+ * window, backward error of the warm-start base, its residual scale s_r and
+ * row scale s_b, the network step alpha (0 when the network did not run),
+ * the base's backward error as the multigrid start test computes it, and 1
+ * if the solve took the guess as it was (zeros before the first);
+ * len >= 7 * cases                                                         */
 int lica_batch_warm_info(lica_batch *b, double *out, int64_t len);
 
 /* ---- single-case API mirroring the reference call sequence ----------- */

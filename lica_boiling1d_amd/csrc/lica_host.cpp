@@ -28,8 +28,7 @@ int lica_emul_nthreads = 256;
 int lica_emul_depth = 0;
 typedef void *hipStream_t;
 // the warm-start network runs on MFMA only; the CPU logic-check build has none
-int lica_net_guess(lica_net *, const NetCase *, int, const int64_t *, double *, double, void *) { return LICA_UNSUPPORTED_DECK; }
-int lica_net_prepare(lica_net *, const int64_t *, int, const NetCase *) { return LICA_UNSUPPORTED_DECK; }
+int lica_net_guess(lica_net *, const NetCase *, int, double *, double, void *) { return LICA_UNSUPPORTED_DECK; }
 int lica_net_append(lica_net *, const NetPairSrc *, int, int64_t, int64_t, uint64_t, int64_t *)
 {
     return LICA_UNSUPPORTED_DECK;
@@ -370,7 +369,6 @@ struct lica_batch {
     double *d_guess = nullptr, *d_scale = nullptr, *d_cap = nullptr, *d_hist = nullptr;
     NetCase *d_netcases = nullptr;
     std::vector<int64_t> lens;
-    std::vector<int64_t> net_lens;   // window of every case in the network's layout
     int capture = 0;             // substep whose POISSON solve is captured (0: none)
     bool cap_valid = false;      // the capture rows hold the last run's last substep
     long long mg_lds_cap = -1;   // dynamic LDS cap of the multigrid kernel (-1: by batch size)
@@ -432,13 +430,19 @@ BatchDev make_bd(const lica_batch *b, int msub)
     bd.guess = b->d_guess;
     bd.hist = b->d_hist;
     bd.cap = b->capture ? b->d_cap : nullptr;
+    // the learned mode's extra pass over B (feature scales, the solve's
+    // start verdict); the projected mode writes B alone (the multigrid
+    // kernel's own start test is then cheaper than the pass)
     bd.wscale = (b->poisson_mode == LICA_POISSON_LEARNED && b->net) ? b->d_scale : nullptr;
     bd.mg_lds = mg_lds(b);
     return bd;
 }
 
 // threads per workgroup: the requested count, raised in the partitioned mode
-// until every case's rows fit the per-thread register chunks
+// until every case's rows fit the per-thread register chunks.  One size for
+// the whole launch: per-size launches one after the other were measured
+// slower (each drains before the next starts, and the largest-first order
+// no longer balances the tail).
 int block_threads(const lica_batch *b)
 {
     int t = b->threads;
@@ -468,7 +472,7 @@ int pre_poisson(lica_batch *b, int msub)
     if (!rc && b->capture == msub) rc = launch(b, KL_CAPTURE_PRE, msub);
     if (!rc && b->poisson_mode == LICA_POISSON_LEARNED) {
         if (!b->net) return LICA_BAD_ARG;
-        if (lica_net_guess(b->net, b->d_netcases, (int)b->ninst, b->net_lens.data(), b->d_scale, b->rtol,
+        if (lica_net_guess(b->net, b->d_netcases, (int)b->ninst, b->d_scale, b->rtol,
                            (void *)b->stream))
             rc = LICA_HIP_ERROR;
     }
@@ -722,7 +726,7 @@ int lica_batch_create(int64_t ninst, const lica_case_desc *cases, lica_batch **o
     rc = rc ? rc : dmalloc((void **)&b->d_order, sizeof(int) * (size_t)ninst);
     rc = rc ? rc : dmalloc((void **)&b->d_guess, sizeof(double) * (size_t)gs_total);
     rc = rc ? rc : dmalloc((void **)&b->d_cap, sizeof(double) * 3 * (size_t)gs_total);
-    rc = rc ? rc : dmalloc((void **)&b->d_scale, sizeof(double) * 4 * (size_t)ninst);
+    rc = rc ? rc : dmalloc((void **)&b->d_scale, sizeof(double) * LICA_WARM_INFO * (size_t)ninst);
     if (rc) { lica_batch_destroy(b); return rc; }
     // workgroup order: largest cases first (longest-processing-time first)
     std::vector<int> order((size_t)ninst);
@@ -1032,8 +1036,6 @@ int lica_batch_set_net(lica_batch *b, lica_net *net)
     if (rc) return rc;
     if (!net) return LICA_BAD_ARG;
     std::vector<NetCase> nc((size_t)b->ninst);
-    std::vector<int64_t> wl((size_t)b->ninst);
-    long long off = 0;
     for (int64_t i = 0; i < b->ninst; i++) {
         const InstParam &p = b->par[(size_t)i];
         double *base = b->d_ns + p.ns_off + 2;
@@ -1045,15 +1047,10 @@ int lica_batch_set_net(lica_batch *b, lica_net *net)
         c.dzi = 1.0 / ((double)p.r * p.sdzf);
         c.n = p.n;
         c.win = (int)std::min<int64_t>(p.n, lica_net_window());
-        c.off = off;
-        wl[(size_t)i] = c.win;
-        off += c.win;
     }
     if (!b->d_netcases) rc = dmalloc((void **)&b->d_netcases, sizeof(NetCase) * nc.size());
     rc = rc ? rc : h2d(b->d_netcases, nc.data(), sizeof(NetCase) * nc.size(), b->stream);
     if (rc) return rc;
-    b->net_lens = wl;
-    if (lica_net_prepare(net, b->net_lens.data(), (int)b->ninst, b->d_netcases)) return LICA_HIP_ERROR;
     b->net = net;
     return LICA_OK;
 }
@@ -1069,13 +1066,13 @@ int lica_batch_set_capture(lica_batch *b, int64_t on)
 }
 
 // per case [first row of the network's window, backward error of the
-// warm-start base, s_r, s_b] of the last network correction
+// warm-start base, s_r, s_b, alpha] of the last network correction
 int lica_batch_warm_info(lica_batch *b, double *out, int64_t len)
 {
     int rc = check_batch(b);
     if (rc) return rc;
-    if (!out || len < 4 * b->ninst) return LICA_BAD_ARG;
-    return d2h(out, b->d_scale, sizeof(double) * 4 * (size_t)b->ninst, b->stream);
+    if (!out || len < LICA_WARM_INFO * b->ninst) return LICA_BAD_ARG;
+    return d2h(out, b->d_scale, sizeof(double) * LICA_WARM_INFO * (size_t)b->ninst, b->stream);
 }
 
 // training windows from the captured systems of every healthy case; needs a

@@ -314,6 +314,26 @@ struct TransportSh {
     int nbn[LICA_MLAYER + 1];
 };
 
+// the band of a case from its state into LDS: counts, then the cells (in
+// parallel), then the entry tables and the window (call from every thread)
+DEV void band_load(BandSh &B, const InstParam &P, const InstState &S)
+{
+    SERIAL {
+        B.status = 0;
+        for (int nl = 0; nl <= LICA_MLAYER + 1; nl++) B.bn[nl] = (nl >= 1 && nl <= LICA_MLAYER) ? S.bn[nl] : 0;
+        for (int i = 0; i < 3; i++) B.gl[i] = 0.0;
+        for (int i = 0; i < 4; i++) B.gu[i] = 0.0;
+    }
+    SERIAL_END
+    PFOR(j, 0, LICA_MLAYER * LICA_MAXP) {
+        const int nl = 1 + j / LICA_MAXP, q = j % LICA_MAXP;
+        if (q < B.bn[nl]) B.bk[nl][q] = S.bk[nl][q];
+    }
+    PFOR_END
+    SERIAL { band_index(B, P, 1); }
+    SERIAL_END
+}
+
 DEV void body_transport(const BatchDev &bd, int inst, TransportSh &T)
 {
     InstState &S = bd.st[inst];
@@ -322,18 +342,7 @@ DEV void body_transport(const BatchDev &bd, int inst, TransportSh &T)
     CaseP cp = case_ptrs(bd, inst);
     const InstParam &P = *cp.P;
     const int n = P.n, n3 = P.n3, n3f = P.n3f, n3fm = P.n3fm;
-    SERIAL {
-        B.status = 0;
-        for (int nl = 0; nl <= LICA_MLAYER + 1; nl++) B.bn[nl] = 0;
-        for (int nl = 1; nl <= LICA_MLAYER; nl++) {
-            B.bn[nl] = S.bn[nl];
-            for (int q = 0; q < S.bn[nl]; q++) B.bk[nl][q] = S.bk[nl][q];
-        }
-        band_index(B, P, 1);
-        for (int i = 0; i < 3; i++) B.gl[i] = 0.0;
-        for (int i = 0; i < 4; i++) B.gu[i] = 0.0;
-    }
-    SERIAL_END
+    band_load(B, P, S);
     if (B.status) { SERIAL { S.status = B.status; } SERIAL_END return; }
     PFOR(i, 0, WMAX) { B.phw[i] = 0.0; }
     PFOR_END
@@ -424,23 +433,21 @@ DEV void body_transport(const BatchDev &bd, int inst, TransportSh &T)
     const int *KCOU1 = cp.AI(NSI_KCOU1), *KCOU2 = cp.AI(NSI_KCOU2);
     const int *KCOUMP1 = cp.AI(NSI_KCOUMP1), *KCOUMP2 = cp.AI(NSI_KCOUMP2);
     const int *KCMV = cp.RI(RFI_KCOUMP_VEL);
-    SERIAL {
-        int nm = 0;
-        for (int e = B.off[1]; e < B.off[LICA_NN1 + 1]; e++) {
-            int kk = KCMV[B.ent_k[e]];
-            int dup = 0;
-            for (int j = 0; j < nm; j++) if (B.mcell[j] == kk) { dup = 1; break; }
-            if (!dup) B.mcell[nm++] = kk;
+    // each NS cell under the T-band once: the entry that meets it first
+    // (the coupled values depend only on the cell)
+    const int e0 = B.off[1], e1 = B.off[LICA_NN1 + 1];
+    int *xk = B.mcell;
+    PFOR(e, e0, e1) { xk[e - e0] = KCMV[B.ent_k[e]]; }
+    PFOR_END
+    PFOR(e, e0, e1) {
+        const int k = xk[e - e0];
+        bool first = true;
+        for (int j = 0; j < e - e0 && first; j++) first = (xk[j] != k);
+        if (first) {
+            auto fpsif = [&](int kk) { return PSIF[kk]; };
+            if (k >= 2) PZN[k] = snap3(box_avg(ZF, P.ssdzf, fpsif, ZP[k - 1], ZP[k], KCOUMP2[k - 1], KCOUMP1[k]));
+            PCN[k] = snap3(box_avg(ZF, P.ssdzf, fpsif, Z[k], Z[k + 1], KCOU2[k], KCOU1[k + 1]));
         }
-        B.iv[0] = nm;
-    }
-    SERIAL_END
-    const int nm = B.iv[0];
-    PFOR(j, 0, nm) {
-        int k = B.mcell[j];
-        auto fpsif = [&](int kk) { return PSIF[kk]; };
-        if (k >= 2) PZN[k] = snap3(box_avg(ZF, P.ssdzf, fpsif, ZP[k - 1], ZP[k], KCOUMP2[k - 1], KCOUMP1[k]));
-        PCN[k] = snap3(box_avg(ZF, P.ssdzf, fpsif, Z[k], Z[k + 1], KCOU2[k], KCOU1[k + 1]));
     }
     PFOR_END
     SERIAL {

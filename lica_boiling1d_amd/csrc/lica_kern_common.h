@@ -888,11 +888,14 @@ DEV void body_cont_energy(const BatchDev &bd, int inst, BlkSh &sh)
 
 // ---------------------------------------------------------------------
 // Band context of the wave kernels (narrow band + refined window in LDS)
+static_assert(LICA_MLAYER < 256 && LICA_MAXP <= 256, "band entry tables hold layer and slot in bytes");
+
 struct BandSh {
     int bn[LICA_MLAYER + 2];
     int bk[LICA_MLAYER + 2][LICA_MAXP];
     int off[LICA_MLAYER + 2];           // entries of layer n: [off[n], off[n+1])
-    int ent_k[LICA_MAXENT], ent_n[LICA_MAXENT], ent_q[LICA_MAXENT];
+    int ent_k[LICA_MAXENT];
+    unsigned char ent_n[LICA_MAXENT], ent_q[LICA_MAXENT];   // layer (<= LICA_MLAYER), slot (< LICA_MAXP)
     int wlo, whi, dlo;                  // window [wlo,whi] of z-planes, work base dlo=wlo-4
     int status;
     double phw[WMAX], gl[3], gu[4];
@@ -904,10 +907,6 @@ struct BandSh {
     double red[16];
     double v[24];
     int iv[24];
-    // temperature extrapolation window (NS cells [nlo,nhi])
-    int nlo, nhi;
-    double t0[NSWMAX], t1[NSWMAX], nphi[NSWMAX], psih[NSWMAX];
-    double psitmp[NSWMAX];
     int mcell[2 * LICA_MAXENT];
     // per-plane tables of the window for the register-resident stages
     int lay[WMAX + 8], slot[WMAX + 8], dmk[WMAX + 8];

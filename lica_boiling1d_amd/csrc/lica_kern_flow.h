@@ -164,8 +164,19 @@ DEV void body_poisson_mg(const BatchDev &bd, int inst, MgSh &sh)
     const double *PA = cp.A(NS_PA), *PB = cp.A(NS_PB), *PCc = cp.A(NS_PC), *PG = cp.A(NS_PG);
     const bool warm = LICA_MG_WARM(bd.pmode) && bd.guess && bd.hist;
     const double *x0 = warm ? bd.guess + P.gs_off : nullptr;
+    // the warm-start pass's verdict on B (still the guess unless the
+    // network moved it, which clears the verdict)
+    const double *pre = (warm && bd.wscale) ? bd.wscale + (size_t)LICA_WARM_INFO * inst : nullptr;
     double res = 0.0;
-    int cyc = mg_solve(PA, PB, PCc, PG, PP, n, bd.mg + P.mg_off, bd.pmode, bd.mg_rtol, bd.mg_maxc, x0, sh, &res);
+    int cyc;
+    if (pre && pre[6] != 0.0) {
+        PFOR(i, 1, n + 1) { PP[i] = x0[i]; }
+        PFOR_END
+        res = pre[5];
+        cyc = 0;
+    } else {
+        cyc = mg_solve(PA, PB, PCc, PG, PP, n, bd.mg + P.mg_off, bd.pmode, bd.mg_rtol, bd.mg_maxc, x0, sh, &res);
+    }
     if (warm) {
         // the solution joins this substep's history ring
         const int s = bd.msub - 1, head = S.hist_head[s];
@@ -220,7 +231,7 @@ DEV void body_capture(const BatchDev &bd, int inst, int which)
 // that minimises |g - A B|_2 (FP64 normal equations, Cholesky with
 // vectors that add nothing new dropped).  B goes to the guess rows.
 struct WarmSh {
-    double red[16 * 9];
+    double red[16 * ((MG_HIST + 1) * (MG_HIST + 2) / 2 + MG_HIST + 1)];
     double coef[MG_HIST + 1];
 };
 
@@ -229,7 +240,12 @@ DEV void body_warm_base(const BatchDev &bd, int inst, WarmSh &sh)
     InstState &S = bd.st[inst];
     if (S.status || !bd.hist || !bd.guess) {
         // a stopped case gets no network correction
-        SERIAL { if (bd.wscale) bd.wscale[4 * inst + 1] = 0.0; }
+        SERIAL {
+            if (bd.wscale) {
+                bd.wscale[LICA_WARM_INFO * inst + 1] = 0.0;
+                bd.wscale[LICA_WARM_INFO * inst + 6] = 0.0;
+            }
+        }
         SERIAL_END
         return;
     }
@@ -254,23 +270,30 @@ DEV void body_warm_base(const BatchDev &bd, int inst, WarmSh &sh)
         if (row < n) v = v + PCc[row] * val(j, row + 1);
         return v;
     };
-    // G_ij = <A v_i, A v_j>, h_i = <A v_i, g>
-    double sums[9];
-    red_sumN<9>(1, n + 1, [&](int row, double *v) {
-        double w[MG_HIST + 1];
-        for (int j = 0; j < MG_HIST + 1; j++) w[j] = (j < m) ? aval(j, row) : 0.0;
+    // G_ij = <A v_i, A v_j> (i <= j, row by row), h_i = <A v_i, g>
+    constexpr int MV = MG_HIST + 1, NG = MV * (MV + 1) / 2;
+    double sums[NG + MV];
+    red_sumN<NG + MV>(1, n + 1, [&](int row, double *v) {
+        double w[MV];
+        for (int j = 0; j < MV; j++) w[j] = (j < m) ? aval(j, row) : 0.0;
         const double g = PG[row];
-        v[0] = w[0] * w[0]; v[1] = w[0] * w[1]; v[2] = w[0] * w[2];
-        v[3] = w[1] * w[1]; v[4] = w[1] * w[2]; v[5] = w[2] * w[2];
-        v[6] = w[0] * g; v[7] = w[1] * g; v[8] = w[2] * g;
+        int q = 0;
+        for (int i = 0; i < MV; i++)
+            for (int j = i; j < MV; j++) v[q++] = w[i] * w[j];
+        for (int j = 0; j < MV; j++) v[NG + j] = w[j] * g;
     }, sh.red, sums);
     SERIAL {
-        double G[3][3] = {{sums[0], sums[1], sums[2]}, {sums[1], sums[3], sums[4]}, {sums[2], sums[4], sums[5]}};
-        const double h[3] = {sums[6], sums[7], sums[8]};
-        double Lm[3][3] = {{0.0}}, y[3] = {0.0}, c[3] = {0.0};
-        int idx[3], nk = 0;
+        double G[MV][MV], h[MV];
+        {
+            int q = 0;
+            for (int i = 0; i < MV; i++)
+                for (int j = i; j < MV; j++) { G[i][j] = sums[q]; G[j][i] = sums[q]; q++; }
+            for (int j = 0; j < MV; j++) h[j] = sums[NG + j];
+        }
+        double Lm[MV][MV] = {{0.0}}, y[MV] = {0.0}, c[MV] = {0.0};
+        int idx[MV], nk = 0;
         for (int j = 0; j < m; j++) {
-            double row[3] = {0.0};
+            double row[MV] = {0.0};
             for (int t = 0; t < nk; t++) {
                 double v = G[j][idx[t]];
                 for (int u = 0; u < t; u++) v = v - row[u] * Lm[t][u];
@@ -308,28 +331,41 @@ DEV void body_warm_base(const BatchDev &bd, int inst, WarmSh &sh)
         PFOR_END
         return;
     }
-    // with the network attached, the scales of its features in the same
-    // pass: max|r| of B's residual, max|b|, max(|a|+|b|+|c|), max|B|,
-    // max|g| (neighbours of B recomputed, bit for bit the stored values)
-    double mx[5];
-    red_maxN<5>(1, n + 1, [&](int row, double *v) {
+    // In the same pass (neighbours of B recomputed, bit for bit the stored
+    // values): the scales of the network's features -- max|r| of B's
+    // residual g - (bB + aB- + cB+), max|b| -- and the maxima of the
+    // multigrid solve's start test -- max|g|, max(|a|+|b|+|c|), max|B| and
+    // max|r| with r accumulated as that test does, ((g - bB) - aB-) - cB+ --
+    // so that the solve need not read the rows again when B already meets
+    // the tolerance.
+    double mx[6];
+    red_maxN<6>(1, n + 1, [&](int row, double *v) {
         const double x = base(row);
         B[row] = x;
+        const double xl = (row > 1) ? base(row - 1) : 0.0, xr = (row < n) ? base(row + 1) : 0.0;
         double ax = PB[row] * x;
-        if (row > 1) ax = ax + PA[row] * base(row - 1);
-        if (row < n) ax = ax + PCc[row] * base(row + 1);
+        if (row > 1) ax = ax + PA[row] * xl;
+        if (row < n) ax = ax + PCc[row] * xr;
+        double rm = PG[row] - PB[row] * x;
+        if (row > 1) rm = rm - PA[row] * xl;
+        if (row < n) rm = rm - PCc[row] * xr;
         const double ak = (row > 1) ? fabs(PA[row]) : 0.0, ck = (row < n) ? fabs(PCc[row]) : 0.0;
         v[0] = fabs(PG[row] - ax);
         v[1] = fabs(PB[row]);
         v[2] = ak + fabs(PB[row]) + ck;
         v[3] = fabs(x);
         v[4] = fabs(PG[row]);
+        v[5] = fabs(rm);
     }, sh.red, mx);
     SERIAL {
+        double *inf = bd.wscale + (size_t)LICA_WARM_INFO * inst;
         const double den = mx[2] * mx[3] + mx[4];
-        bd.wscale[4 * inst + 1] = (den > 0.0) ? mx[0] / den : mx[0];
-        bd.wscale[4 * inst + 2] = (mx[0] == 0.0) ? 1.0 : mx[0];
-        bd.wscale[4 * inst + 3] = (mx[1] == 0.0) ? 1.0 : mx[1];
+        inf[1] = (den > 0.0) ? mx[0] / den : mx[0];
+        inf[2] = (mx[0] == 0.0) ? 1.0 : mx[0];
+        inf[3] = (mx[1] == 0.0) ? 1.0 : mx[1];
+        const double res0 = (den > 0.0) ? mx[5] / den : mx[5];
+        inf[5] = res0;
+        inf[6] = (!(mx[5] > mx[4]) && !(res0 > bd.mg_rtol)) ? 1.0 : 0.0;
     }
     SERIAL_END
 }

@@ -12,8 +12,16 @@ static_assert(LICA_NN2 + 1 <= LICA_MLAYER, "layer order of the band entries");
 // window views of psif_mtp that VOL_TOT_MFDOT and CAL_PSI_TMP need reuse
 // B.tf0/B.mask_s and B.tf1/B.slot, free by then (TF_GRAD and the
 // reinitialisation are done), which keeps six workgroups on a CU
+// TEMP_EXTP's window of NS cells [nlo, nhi] under the band
+struct ExtpWin {
+    int nlo, nhi;
+    double t0[NSWMAX], t1[NSWMAX], nphi[NSWMAX], psih[NSWMAX];
+    double psitmp[NSWMAX];   // CAL_PSI_TMP of cells nlo-1 .. nhi+1
+};
+
 struct TgradSh {
     BandSh B;
+    ExtpWin X;
     double xit[2], xhv[2];   // TEMP_EXTP iterations / halvings of phases 0, 1
     int xst[2];
 };
@@ -153,13 +161,13 @@ DEV bool rel_change_fails(double x, double rk, double eps)
 // cell i (NS cell k in [ia,ib]).  NPHI vanishes on the three cells at either
 // end of the window, so the neighbours i-1, i+1 are read only inside it.
 template <int IM>
-DEV double extp_dtn_start(const BandSh &B, const InstParam &P, const double *PCN, const double *VVDZ,
+DEV double extp_dtn_start(const ExtpWin &X, const InstParam &P, const double *PCN, const double *VVDZ,
                           const double *tw, int i, int k, double sm_grid)
 {
-    const double np = B.nphi[i], pc = PCN[k];
+    const double np = X.nphi[i], pc = PCN[k];
     double v = 0.0;
     if (((fabs(np) > 1.5 * sm_grid) && (np != 0.0)) || ((fabs(pc - 0.5) == 0.5) && (np != 0.0))) {
-        const double npl = B.nphi[i - 1], npr = B.nphi[i + 1];
+        const double npl = X.nphi[i - 1], npr = X.nphi[i + 1];
         double dpx = 0.5 * (np - np) * (1.0 / P.sdx1);
         double dtdx = 0.5 * (tw[i] - tw[i]) * (1.0 / P.sdx1);
         double dpz, dtdz;
@@ -169,12 +177,12 @@ DEV double extp_dtn_start(const BandSh &B, const InstParam &P, const double *PCN
         double dalphi = dpx * dpx + dpx * dpx + dpz * dpz;
         if (dalphi != 0.0) {
             double nx = 0.0, ny = 0.0, nz = 1.0;
-            if (IM == 1) v = nx * dtdx + ny * dtdx + nz * dtdz * B.psih[i];
-            else v = nx * dtdx + ny * dtdx + nz * dtdz * (1.0 - B.psih[i]);
+            if (IM == 1) v = nx * dtdx + ny * dtdx + nz * dtdz * X.psih[i];
+            else v = nx * dtdx + ny * dtdx + nz * dtdz * (1.0 - X.psih[i]);
         }
     } else if (((fabs(np) <= 1.5 * sm_grid) || (fabs(pc - 0.5) != 0.5)) && ((np != 1.0e-7) && (np != 0.0))) {
-        if (IM == 1) v = (tw[i] - P.tsat) / np * B.psih[i];
-        else v = (tw[i] - P.tsat) / np * (1.0 - B.psih[i]);
+        if (IM == 1) v = (tw[i] - P.tsat) / np * X.psih[i];
+        else v = (tw[i] - P.tsat) / np * (1.0 - X.psih[i]);
     }
     return v;
 }
@@ -184,11 +192,11 @@ DEV double extp_dtn_start(const BandSh &B, const InstParam &P, const double *PCN
 // RAT < EPS over the updated cells (a NaN ratio never counts, as with MAX)
 // is the wave's vote that no cell's ratio reaches EPS.
 template <int CPL, int IM>
-DEV int temp_extp_loops(BandSh &B, const InstParam &P, const double *PCN, const double *VVDZ, double *tw, int ia,
+DEV int temp_extp_loops(const ExtpWin &X, const InstParam &P, const double *PCN, const double *VVDZ, double *tw, int ia,
                         int ib, double sm_grid, double eps, double *iters, double *halv)
 {
     const int im = IM;
-    const int nlo = B.nlo, nw = B.nhi - B.nlo + 1;
+    const int nlo = X.nlo, nw = X.nhi - X.nlo + 1;
     const double inv_sdx1 = 1.0 / P.sdx1;
     double nx = 0.0;
     if (im == 1) nx = (-1.0) * nx;
@@ -199,11 +207,11 @@ DEV int temp_extp_loops(BandSh &B, const InstParam &P, const double *PCN, const 
     for (int s = 0; s < CPL; s++) {
         const int i = xcell<CPL>(s), k = nlo + i;
         const bool ok = i < nw;
-        const double np = ok ? B.nphi[i] : 0.0;
-        const double npl = (ok && i > 0) ? B.nphi[i - 1] : 0.0;
-        const double npr = (ok && i + 1 < nw) ? B.nphi[i + 1] : 0.0;
+        const double np = ok ? X.nphi[i] : 0.0;
+        const double npl = (ok && i > 0) ? X.nphi[i - 1] : 0.0;
+        const double npr = (ok && i + 1 < nw) ? X.nphi[i + 1] : 0.0;
         const double vv = ok ? VVDZ[k] : 0.0, vvp = ok ? VVDZ[k + 1] : 0.0;
-        const double ps = ok ? B.psih[i] : 0.0;
+        const double ps = ok ? X.psih[i] : 0.0;
         // the static part of extp_point
         int live = 0;
         if (np != 0.0) {
@@ -220,7 +228,7 @@ DEV int temp_extp_loops(BandSh &B, const InstParam &P, const double *PCN, const 
         c.small[s] = fabs(np) <= sm_grid;
         c.act[s] = live && ok && k >= ia && k <= ib;
         c.upd[s] = ok && k >= ia && k <= ib;
-        c.dtn[s] = c.upd[s] ? extp_dtn_start<IM>(B, P, PCN, VVDZ, tw, i, k, sm_grid) : 0.0;
+        c.dtn[s] = c.upd[s] ? extp_dtn_start<IM>(X, P, PCN, VVDZ, tw, i, k, sm_grid) : 0.0;
         a[s] = c.dtn[s];
     }
     // (a) dT/dn (ND=0), continuing from the current field after a halving
@@ -307,7 +315,7 @@ DEV double phinsintp(const BandSh &B, const CaseP &cp, int ko, int *ic)
 }
 
 // TEMPASSIGN (chphase.f:372-463): x/y weights are zero on the flat column
-DEV double tempassign(const BandSh &B, const CaseP &cp, int k, const double *tw)
+DEV double tempassign(int nlo, const CaseP &cp, int k, const double *tw)
 {
     const double *ZP = cp.A(NS_ZP), *ZPF = cp.R(RF_ZPF), *VDZ = cp.A(NS_VDZ);
     const int *KCMV = cp.RI(RFI_KCOUMP_VEL);
@@ -321,7 +329,7 @@ DEV double tempassign(const BandSh &B, const CaseP &cp, int k, const double *tw)
         zd = (kk1 == kk2) ? 0.0 : (ZPF[k] - ZP[kk1]) / (ZP[kk2] - ZP[kk1]);
     }
     double xd = 0.0, yd = 0.0;
-    double t_kk = tw[kk - B.nlo], t_k2 = tw[kk2 - B.nlo];
+    double t_kk = tw[kk - nlo], t_k2 = tw[kk2 - nlo];
     double c00 = t_kk * (1.0 - xd) + t_kk * xd;
     double c10 = t_kk * (1.0 - xd) + t_kk * xd;
     double c01 = t_k2 * (1.0 - xd) + t_k2 * xd;
@@ -342,6 +350,7 @@ DEV void body_temp_grad(const BatchDev &bd, int inst, TgradSh &TS)
     InstState &S = bd.st[inst];
     if (S.status || S.m == 1) return;   // TEMP_GRAD runs from the second step (1d.f:3654-3667)
     BandSh &B = TS.B;
+    ExtpWin &X = TS.X;
     CaseP cp = case_ptrs(bd, inst);
     const InstParam &P = *cp.P;
     const int n = P.n, n3 = P.n3, n3f = P.n3f;
@@ -353,24 +362,9 @@ DEV void body_temp_grad(const BatchDev &bd, int inst, TgradSh &TS)
     double *SFGAS = cp.R(RF_SFGAS), *SFGAS2 = cp.R(RF_SFGAS2);
     const double *VPREF = cp.R(RF_VPREFIX);
 
-    // the band of the case: counts first, then the cells (in parallel),
-    // then the entry tables
-    SERIAL {
-        B.status = 0;
-        for (int nl = 0; nl <= LICA_MLAYER + 1; nl++) B.bn[nl] = (nl >= 1 && nl <= LICA_MLAYER) ? S.bn[nl] : 0;
-        for (int i = 0; i < 3; i++) B.gl[i] = 0.0;
-        for (int i = 0; i < 4; i++) B.gu[i] = 0.0;
-    }
-    SERIAL_END
-    PFOR(j, 0, LICA_MLAYER * LICA_MAXP) {
-        const int nl = 1 + j / LICA_MAXP, q = j % LICA_MAXP;
-        if (q < B.bn[nl]) B.bk[nl][q] = S.bk[nl][q];
-    }
-    PFOR_END
+    band_load(B, P, S);
     PFOR(j, 0, (LICA_MLAYER + 1) * LICA_MAXP) { S.mflvs[j / LICA_MAXP][j % LICA_MAXP] = 0.0; }
     PFOR_END
-    SERIAL { band_index(B, P, 1); }
-    SERIAL_END
     {
         // NS window under the band
         const int ne = B.off[LICA_MLAYER + 1];
@@ -379,22 +373,22 @@ DEV void body_temp_grad(const BatchDev &bd, int inst, TgradSh &TS)
         SERIAL {
             int kmin = (int)kmn, kmax = (int)kmx;
             if (kmax < 0) { kmin = 1; kmax = 1; }
-            B.nlo = kmin - 3 < 0 ? 0 : kmin - 3;
-            B.nhi = kmax + 3 > n3 ? n3 : kmax + 3;
+            X.nlo = kmin - 3 < 0 ? 0 : kmin - 3;
+            X.nhi = kmax + 3 > n3 ? n3 : kmax + 3;
             // CAL_PSI_TMP's rows reach one cell beyond either end
-            if (B.nhi - B.nlo + 3 > NSWMAX) B.status = ST_WINDOW;
+            if (X.nhi - X.nlo + 3 > NSWMAX) B.status = ST_WINDOW;
         }
         SERIAL_END
     }
     if (B.status) { SERIAL { S.status = B.status; } SERIAL_END return; }
-    const int nlo = B.nlo, nhi = B.nhi;
+    const int nlo = X.nlo, nhi = X.nhi;
     PFOR(i, 0, WMAX) { B.phw[i] = 0.0; B.tf0[i] = 0.0; B.tf1[i] = 0.0; }
     PFOR_END
     PFOR(e, 0, B.off[LICA_MLAYER + 1]) { B.phw[B.ent_k[e] - B.wlo] = S.alphi[B.ent_n[e]][B.ent_q[e]]; }
     PFOR_END
     PFOR(k, nlo, nhi + 1) {
         int i = k - nlo;
-        B.t0[i] = T[k]; B.t1[i] = T[k]; B.nphi[i] = 0.0; B.psih[i] = 0.0;
+        X.t0[i] = T[k]; X.t1[i] = T[k]; X.nphi[i] = 0.0; X.psih[i] = 0.0;
     }
     PFOR_END
     // (1) phase of the NS cells under the band (chphase.f:174-211).  The
@@ -428,18 +422,18 @@ DEV void body_temp_grad(const BatchDev &bd, int inst, TgradSh &TS)
         int ic = 0;
         for (int e = e0; e < e1; e++) {
             const int i = xk[e] - nlo;
-            if (B.nphi[i] == 0.0) {
+            if (X.nphi[i] == 0.0) {
                 if (xe[e] >= 0) ic = xe[e];
-                B.nphi[i] = B.pm1[e];
+                X.nphi[i] = B.pm1[e];
                 const double pc = B.pm2[e];
-                if (fabs(pc - 0.5) != 0.5) B.psih[i] = (double)ic;
-                else B.psih[i] = pc;
+                if (fabs(pc - 0.5) != 0.5) X.psih[i] = (double)ic;
+                else X.psih[i] = pc;
             }
         }
         for (int e = e2; e < e3; e++) {
             const int i = xk[e] - nlo;
-            B.nphi[i] = fmin(B.nphi[i], B.pm1[e]);
-            B.psih[i] = B.pm2[e];
+            X.nphi[i] = fmin(X.nphi[i], B.pm1[e]);
+            X.psih[i] = B.pm2[e];
         }
     }
     SERIAL_END
@@ -448,15 +442,15 @@ DEV void body_temp_grad(const BatchDev &bd, int inst, TgradSh &TS)
     const double eps = 1.0e-4;
     const int ia = nlo < 1 ? 1 : nlo, ib = nhi > n ? n : nhi;
     // the two phases are independent (phase im advances its own copy of the
-    // window, B.t0 or B.t1, from its own DTEMPN): on the GPU wave im runs
+    // window, X.t0 or X.t1, from its own DTEMPN): on the GPU wave im runs
     // phase im.  Counters and status combine in the reference's order: a
     // failed phase 0 ends TEMP_GRAD before phase 1 starts, so phase 1's
     // counts are dropped then.
 #ifdef LICA_EMUL
     for (int im = 0; im <= 1; im++) {
         double iters = 0.0, halv = 0.0;
-        TS.xst[im] = im ? temp_extp_loops<NSWMAX, 1>(B, P, PCN, VVDZ, B.t1, ia, ib, sm_grid, eps, &iters, &halv)
-                        : temp_extp_loops<NSWMAX, 0>(B, P, PCN, VVDZ, B.t0, ia, ib, sm_grid, eps, &iters, &halv);
+        TS.xst[im] = im ? temp_extp_loops<NSWMAX, 1>(X, P, PCN, VVDZ, X.t1, ia, ib, sm_grid, eps, &iters, &halv)
+                        : temp_extp_loops<NSWMAX, 0>(X, P, PCN, VVDZ, X.t0, ia, ib, sm_grid, eps, &iters, &halv);
         TS.xit[im] = iters;
         TS.xhv[im] = halv;
     }
@@ -466,11 +460,11 @@ DEV void body_temp_grad(const BatchDev &bd, int inst, TgradSh &TS)
         double iters = 0.0, halv = 0.0;
         int xst;
         if (nhi - nlo + 1 <= 64)
-            xst = im ? temp_extp_loops<1, 1>(B, P, PCN, VVDZ, B.t1, ia, ib, sm_grid, eps, &iters, &halv)
-                     : temp_extp_loops<1, 0>(B, P, PCN, VVDZ, B.t0, ia, ib, sm_grid, eps, &iters, &halv);
+            xst = im ? temp_extp_loops<1, 1>(X, P, PCN, VVDZ, X.t1, ia, ib, sm_grid, eps, &iters, &halv)
+                     : temp_extp_loops<1, 0>(X, P, PCN, VVDZ, X.t0, ia, ib, sm_grid, eps, &iters, &halv);
         else
-            xst = im ? temp_extp_loops<2, 1>(B, P, PCN, VVDZ, B.t1, ia, ib, sm_grid, eps, &iters, &halv)
-                     : temp_extp_loops<2, 0>(B, P, PCN, VVDZ, B.t0, ia, ib, sm_grid, eps, &iters, &halv);
+            xst = im ? temp_extp_loops<2, 1>(X, P, PCN, VVDZ, X.t1, ia, ib, sm_grid, eps, &iters, &halv)
+                     : temp_extp_loops<2, 0>(X, P, PCN, VVDZ, X.t0, ia, ib, sm_grid, eps, &iters, &halv);
         if ((threadIdx.x & 63) == 0) {
             TS.xst[im] = xst;
             TS.xit[im] = iters;
@@ -495,8 +489,8 @@ DEV void body_temp_grad(const BatchDev &bd, int inst, TgradSh &TS)
     // (3) TF on the T-band, (4)-(6) TF_GRAD
     PFOR(e, B.off[1], B.off[LICA_NN1 + 1]) {
         int k = B.ent_k[e];
-        B.tf0[k - B.wlo] = tempassign(B, cp, k, B.t0);
-        B.tf1[k - B.wlo] = tempassign(B, cp, k, B.t1);
+        B.tf0[k - B.wlo] = tempassign(X.nlo, cp, k, X.t0);
+        B.tf1[k - B.wlo] = tempassign(X.nlo, cp, k, X.t1);
     }
     PFOR_END
     psif_mtp_band(B, P, LICA_NN1, B.pm1);
@@ -585,7 +579,7 @@ DEV void body_temp_grad(const BatchDev &bd, int inst, TgradSh &TS)
         double ps = box_avg(ZF, P.ssdzf, fpm, Z[k], Z[k + 1], KCOU2[k], KCOU1[k + 1]);
         if (ps <= 1.0e-3) ps = 0.0;
         else if (ps >= 0.999) ps = 1.0;
-        B.psitmp[k - ka] = ps;
+        X.psitmp[k - ka] = ps;
     }
     PFOR_END
     const int iml = S.im_lo, imh = S.im_hi;
@@ -593,7 +587,7 @@ DEV void body_temp_grad(const BatchDev &bd, int inst, TgradSh &TS)
     PFOR_END
     const int ja = (ka + 1) < 2 ? 2 : ka + 1;
     PFOR(k, ja, kb + 1) {
-        if (fabs(B.psitmp[k - ka] - B.psitmp[k - 1 - ka]) != 0.0) IMASK[k] = 1;
+        if (fabs(X.psitmp[k - ka] - X.psitmp[k - 1 - ka]) != 0.0) IMASK[k] = 1;
     }
     PFOR_END
     SERIAL {

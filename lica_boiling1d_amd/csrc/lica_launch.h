@@ -24,7 +24,6 @@ struct NetCase {
     double *guess;                     // rows 1..n: the warm-start base B, corrected by s * network
     const double *place;               // interface height of the case (its state)
     double dzi;                        // 1 / NS cell height (uniform grid)
-    long long off;                     // first cell of the case's window in the activation arena
     int n, win;                        // rows; cells of the corrected window around the interface
 };
 // one case's captured POISSON system (rows 1..n): a, b, c, right-hand side,
@@ -41,9 +40,9 @@ struct lica_net;
 int lica_net_append(lica_net *t, const NetPairSrc *hsrc, int ncase, int64_t wpc, int64_t len, uint64_t seed,
                     int64_t *added);
 int64_t lica_net_window();   // cells of the corrected window around the interface
-// network correction of the warm-start guesses of a batch; dinfo holds 4
-// doubles per case, [1..3] = backward error, s_r, s_b of the base (written
-// by the warm-start base kernel), [0] = window start (written here)
-int lica_net_guess(lica_net *t, const NetCase *dcases, int ncase, const int64_t *lens, double *dinfo, double rtol,
-                   void *stream);
-int lica_net_prepare(lica_net *t, const int64_t *lens, int ncase, const NetCase *dcases);
+// network correction of the warm-start guesses of a batch (one launch);
+// dinfo holds LICA_WARM_INFO doubles per case, [1..3] = backward error, s_r,
+// s_b of the base (written by the warm-start base kernel), [0] = window
+// start and [4] = alpha (written here), [6] = the multigrid start verdict
+// on the base (cleared here for the cases whose guess moves)
+int lica_net_guess(lica_net *t, const NetCase *dcases, int ncase, double *dinfo, double rtol, void *stream);

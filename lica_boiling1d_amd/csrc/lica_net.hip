@@ -30,6 +30,7 @@
 #define NET_TILE 256         // cells per workgroup in the convolution kernels
 #define NET_WIN 256          // cells around the interface the warm-start network corrects
 #define NET_NF 4             // input features per cell
+static_assert(NET_WIN <= NET_TILE, "a case's window is one tile of the per-case kernel");
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -42,6 +43,35 @@ struct NetSeq {
     int len;         // cells
     int tile0;       // first tile of the sequence in the tile list
 };
+
+// ---------------------------------------------------------------------
+// The FP32 sums of one layer over a tile: xs holds the tile's NET_TILE input
+// rows with NET_HALF halo rows on either side, ws the layer's weights;
+// wave w gets rows 64w..64w+63 as four 16-row blocks (acc[rb][nt]: block
+// rb, output channels 16nt..16nt+15).
+template <int COUT>
+__device__ __forceinline__ void net_tile_mfma(const __bf16 *xs, const __bf16 *ws, f32x4 (&acc)[4][COUT / 16])
+{
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int m = lane & 15, kq = lane >> 4;                  // operand row / k group
+    constexpr int NT = COUT / 16;
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) acc[rb][nt] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+    const int r0 = wave * 64;                                 // first tile row of this wave
+    for (int t = 0; t < NET_TAPS; t++) {
+        bf16x8 b[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) b[nt] = *(const bf16x8 *)(ws + (t * COUT + nt * 16 + m) * NET_C + kq * 8);
+#pragma unroll
+        for (int rb = 0; rb < 4; rb++) {
+            const bf16x8 a = *(const bf16x8 *)(xs + (r0 + rb * 16 + m + t) * NET_C + kq * 8);
+#pragma unroll
+            for (int nt = 0; nt < NT; nt++) acc[rb][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[nt], acc[rb][nt], 0, 0, 0);
+        }
+    }
+}
 
 // ---------------------------------------------------------------------
 // Y[cell][co] = act(bias[co] + sum_t sum_ci W[t][co][ci] X[cell + t - HALF][ci])
@@ -79,22 +109,8 @@ __global__ void __launch_bounds__(256) k_net_conv(const __bf16 *__restrict__ X, 
     const int m = lane & 15, kq = lane >> 4;                  // operand row / k group
     constexpr int NT = COUT / 16;
     f32x4 acc[4][NT];
-#pragma unroll
-    for (int rb = 0; rb < 4; rb++)
-#pragma unroll
-        for (int nt = 0; nt < NT; nt++) acc[rb][nt] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+    net_tile_mfma<COUT>(xs, ws, acc);
     const int r0 = wave * 64;                                 // first tile row of this wave
-    for (int t = 0; t < NET_TAPS; t++) {
-        bf16x8 b[NT];
-#pragma unroll
-        for (int nt = 0; nt < NT; nt++) b[nt] = *(const bf16x8 *)(ws + (t * COUT + nt * 16 + m) * NET_C + kq * 8);
-#pragma unroll
-        for (int rb = 0; rb < 4; rb++) {
-            const bf16x8 a = *(const bf16x8 *)(xs + (r0 + rb * 16 + m + t) * NET_C + kq * 8);
-#pragma unroll
-            for (int nt = 0; nt < NT; nt++) acc[rb][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[nt], acc[rb][nt], 0, 0, 0);
-        }
-    }
     // epilogue: lane holds col (l%16), rows 4*(l/16)+i of each block
 #pragma unroll
     for (int rb = 0; rb < 4; rb++)
@@ -191,6 +207,23 @@ __global__ void k_net_cast(const float *__restrict__ w, __bf16 *__restrict__ wb,
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) wb[i] = to_bf16(w[i]);
+}
+
+// the same for every layer in one launch: element i of the concatenation
+// belongs to the first layer l with i < end[l]
+struct NetCast {
+    const float *src[NET_LAYERS];
+    __bf16 *dst[NET_LAYERS];
+    long long end[NET_LAYERS];
+};
+__global__ void k_net_cast_all(NetCast c)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long b = 0;
+    for (int l = 0; l < NET_LAYERS; l++) {
+        if (i < c.end[l]) { c.dst[l][i - b] = to_bf16(c.src[l][i - b]); return; }
+        b = c.end[l];
+    }
 }
 
 // dZ_prev = dX * (Yprev > 0)   (ReLU of the previous layer), channels-last
@@ -325,33 +358,6 @@ __device__ __forceinline__ int net_k0(const NetCase &cs)
     return min(max(kif - cs.win / 2, 1), cs.n - cs.win + 1);
 }
 
-// features of the window of every case of a batch (one workgroup per case).
-// info[4i+1..3] hold the backward error of case i's base B and the scales
-// s_r, s_b (net_scales over all rows, written by the warm-start base kernel
-// in its pass over B).  A case whose base already meets rtol gets an empty
-// sequence (its tiles do no work) and keeps B; info[4i] = first row of the
-// window.
-__global__ void __launch_bounds__(256) k_net_features(const NetCase *__restrict__ cases, __bf16 *__restrict__ X,
-                                                      NetSeq *__restrict__ seqs, double *__restrict__ info,
-                                                      double rtol)
-{
-    const NetCase cs = cases[blockIdx.x];
-    const int tid = threadIdx.x;
-    const double berr = info[4 * blockIdx.x + 1], sr = info[4 * blockIdx.x + 2], sb = info[4 * blockIdx.x + 3];
-    const int k0 = net_k0(cs);
-    const bool skip = !(berr > rtol);
-    if (!skip)
-        for (int c = tid; c < cs.win; c += blockDim.x) {
-            __bf16 *row = X + (cs.off + c) * NET_C;
-            net_row(cs.a, cs.b, cs.c, cs.g, cs.guess, cs.n, k0 + c, sr, sb, row);
-            for (int ch = NET_NF; ch < NET_C; ch++) row[ch] = to_bf16(0.0f);
-        }
-    if (tid == 0) {
-        seqs[blockIdx.x].len = skip ? 0 : cs.win;
-        info[4 * blockIdx.x] = (double)k0;
-    }
-}
-
 // training windows from captured systems (one workgroup per case): window w
 // of case i starts at row 1 + woff[i*wpc + w] (-1: none) and goes to data-set
 // slot slot[i*wpc + w]; features as above ([cell][NET_NF] BF16), target
@@ -379,17 +385,97 @@ __global__ void __launch_bounds__(256) k_net_pairs(const NetPairSrc *__restrict_
     }
 }
 
-// guess = B + alpha d over the window, alpha = <A d, r> / <A d, A d> (the
-// rows where A d is nonzero), in place; skipped cases keep B
-__global__ void __launch_bounds__(256) k_net_guess(const NetCase *__restrict__ cases, const float *__restrict__ y,
-                                                   const NetSeq *__restrict__ seqs, const double *__restrict__ info)
+// parameters of the layers: BF16 weights (cast from the FP32 masters by
+// k_net_cast) and FP32 biases
+struct NetParams {
+    const __bf16 *w[NET_LAYERS];
+    const float *b[NET_LAYERS];
+};
+
+// One layer of the per-case network in LDS: the layer's BF16 weights into
+// ws, the FP32 sums over the tile, then bias (+ ReLU) into the next
+// activation rows (BF16, the halo and the rows beyond the window stay zero)
+// or, for the last layer, channel 0's FP32 pre-activation into ys.
+template <int COUT>
+__device__ __forceinline__ void net_case_layer(const __bf16 *xin, __bf16 *xout, __bf16 *ws, float *ys,
+                                               const __bf16 *__restrict__ wl, const float *__restrict__ bl, int w,
+                                               bool last)
 {
+    const int tid = threadIdx.x;
+    for (int i = tid; i < NET_TAPS * COUT * NET_C / 8; i += blockDim.x)
+        *(bf16x8 *)(ws + i * 8) = *(const bf16x8 *)(wl + i * 8);
+    __syncthreads();
+    constexpr int NT = COUT / 16;
+    f32x4 acc[4][NT];
+    net_tile_mfma<COUT>(xin, ws, acc);
+    const int wave = tid >> 6, lane = tid & 63, m = lane & 15, kq = lane >> 4, r0 = wave * 64;
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) {
+            const int co = nt * 16 + m;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int cell = r0 + rb * 16 + kq * 4 + i;
+                if (cell >= w) continue;
+                const float v = acc[rb][nt][i] + bl[co];
+                if (last) {
+                    if (co == 0) ys[cell] = v;
+                } else {
+                    xout[(NET_HALF + cell) * NET_C + co] = to_bf16(fmaxf(v, 0.0f));
+                }
+            }
+        }
+    __syncthreads();   // ws and xin are free, xout / ys complete
+}
+
+// The network's correction of a batch's warm-start guesses, one workgroup
+// per case (the window, <= NET_TILE rows, is one tile): the features of the
+// window's rows from B (scales in info, written by the warm-start base pass),
+// the NET_LAYERS layers with activations and weights in LDS, then
+// guess = B + alpha d over the window with alpha = <A d, r> / <A d, A d> (the
+// rows where A d is nonzero).  A case whose base already meets rtol keeps B.
+// The arithmetic is k_net_conv's layer by layer (BF16 activations and
+// weights, FP32 sums in the same order).  info[LICA_WARM_INFO i]: window
+// start (written), backward error, s_r, s_b (read), alpha (written), the
+// multigrid start verdict (cleared when the guess moves).
+__global__ void __launch_bounds__(256) k_net_case(const NetCase *__restrict__ cases, NetParams prm,
+                                                  double *__restrict__ info, double rtol)
+{
+    constexpr int ROWS = NET_TILE + NET_TAPS - 1;
+    __shared__ __attribute__((aligned(16))) __bf16 xs[2][ROWS * NET_C];
+    __shared__ __attribute__((aligned(16))) __bf16 ws[NET_TAPS * NET_C * NET_C];
+    __shared__ float ys[NET_TILE];
     const NetCase cs = cases[blockIdx.x];
-    if (seqs[blockIdx.x].len == 0) return;
-    const int k0 = (int)info[4 * blockIdx.x], n = cs.n, w = cs.win;
-    auto d = [&](int k) -> double { return (k >= k0 && k < k0 + w) ? (double)y[cs.off + (k - k0)] : 0.0; };
+    const int tid = threadIdx.x;
+    double *inf = info + (size_t)LICA_WARM_INFO * blockIdx.x;
+    const double berr = inf[1], sr = inf[2], sb = inf[3];
+    const int k0 = net_k0(cs), n = cs.n, w = cs.win;
+    if (tid == 0) { inf[0] = (double)k0; inf[4] = 0.0; }
+    if (!(berr > rtol)) return;
+    if (tid == 0) inf[6] = 0.0;   // the guess moves: the solve tests it afresh
+    // layer 0 input (features, channels NET_NF.. zero) and zero rows outside
+    // the window in both activation buffers
+    for (int i = tid; i < ROWS; i += blockDim.x) {
+        const int c = i - NET_HALF;
+        __bf16 *row = xs[0] + i * NET_C;
+        int ch0 = 0;
+        if (c >= 0 && c < w) {
+            net_row(cs.a, cs.b, cs.c, cs.g, cs.guess, n, k0 + c, sr, sb, row);
+            ch0 = NET_NF;
+        } else {
+            for (int ch = 0; ch < NET_C; ch++) xs[1][i * NET_C + ch] = to_bf16(0.0f);
+        }
+        for (int ch = ch0; ch < NET_C; ch++) row[ch] = to_bf16(0.0f);
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int l = 0; l < NET_LAYERS - 1; l++, cur ^= 1)
+        net_case_layer<NET_C>(xs[cur], xs[cur ^ 1], ws, ys, prm.w[l], prm.b[l], w, false);
+    net_case_layer<16>(xs[cur], xs[cur ^ 1], ws, ys, prm.w[NET_LAYERS - 1], prm.b[NET_LAYERS - 1], w, true);
+    auto d = [&](int k) -> double { return (k >= k0 && k < k0 + w) ? (double)ys[k - k0] : 0.0; };
     double v[2] = {0.0, 0.0};
-    for (int k = max(k0 - 1, 1) + (int)threadIdx.x; k <= min(k0 + w, n); k += blockDim.x) {
+    for (int k = max(k0 - 1, 1) + tid; k <= min(k0 + w, n); k += blockDim.x) {
         double ad = cs.b[k] * d(k);
         if (k > 1) ad = ad + cs.a[k] * d(k - 1);
         if (k < n) ad = ad + cs.c[k] * d(k + 1);
@@ -399,7 +485,8 @@ __global__ void __launch_bounds__(256) k_net_guess(const NetCase *__restrict__ c
     }
     net_reduce<2, true>(v);
     const double alpha = (v[1] > 0.0) ? v[0] / v[1] : 0.0;
-    for (int c = threadIdx.x; c < w; c += blockDim.x) cs.guess[k0 + c] = cs.guess[k0 + c] + alpha * (double)y[cs.off + c];
+    for (int c = tid; c < w; c += blockDim.x) cs.guess[k0 + c] = cs.guess[k0 + c] + alpha * (double)ys[c];
+    if (tid == 0) inf[4] = alpha;
 }
 
 // ---------------------------------------------------------------------
@@ -443,7 +530,6 @@ struct lica_net {
     long long *idx = nullptr;
     long long data_n = 0, data_cap = 0; // windows held / room for
     int data_len = 0;
-    const void *layout_owner = nullptr;   // batch whose cases the current layout holds
 };
 
 namespace {
@@ -499,7 +585,6 @@ int net_layout(lica_net *t, const int64_t *lens, int nseq)
     t->nseq = nseq;
     t->ntiles = (int)sot.size();
     t->ncell = off;
-    t->layout_owner = nullptr;
     return rc ? rc : hip_ok(hipStreamSynchronize(t->stream));
 }
 
@@ -818,38 +903,23 @@ int lica_net_train(lica_net *t, int64_t iters, int64_t batch, double lr, uint64_
 }  // extern "C"
 
 // ---------------------------------------------------------------------
-// warm start inside a batch: features of every case's POISSON system, the
-// network, guess = P + s_p * y (called by the batch driver before the
-// multigrid kernel of a substep)
-int lica_net_guess(lica_net *t, const NetCase *dcases, int ncase, const int64_t *lens, double *dinfo, double rtol,
-                   void *stream)
+// warm start inside a batch: the network's correction of every case's
+// guess, one workgroup per case (called by the batch driver before the
+// multigrid kernel of a substep; no allocation, so it can be graph-captured)
+int lica_net_guess(lica_net *t, const NetCase *dcases, int ncase, double *dinfo, double rtol, void *stream)
 {
-    int rc = 0;
-    if (t->layout_owner != (const void *)dcases) {
-        rc = net_layout(t, lens, ncase);
-        if (rc) return rc;
-        t->layout_owner = (const void *)dcases;
-    }
-    hipStream_t s = (hipStream_t)stream;
-    hipStream_t saved = t->stream;
-    t->stream = s;
-    hipLaunchKernelGGL(k_net_features, dim3(ncase), dim3(256), 0, s, dcases, t->act[0], t->seqs, dinfo, rtol);
-    rc = net_forward(t);
-    hipLaunchKernelGGL(k_net_guess, dim3(ncase), dim3(256), 0, s, dcases, t->yf, t->seqs, dinfo);
-    t->stream = saved;
-    return rc ? rc : hip_ok(hipGetLastError());
+    NetParams prm;
+    for (int l = 0; l < NET_LAYERS; l++) { prm.w[l] = t->wb[l]; prm.b[l] = t->b[l]; }
+    // the BF16 copies follow the masters (one launch for all layers)
+    NetCast nc;
+    long long tot = 0;
+    for (int l = 0; l < NET_LAYERS; l++) { nc.src[l] = t->w[l]; nc.dst[l] = t->wb[l]; nc.end[l] = (tot += (long long)wsize(l)); }
+    hipLaunchKernelGGL(k_net_cast_all, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, (hipStream_t)stream, nc);
+    hipLaunchKernelGGL(k_net_case, dim3(ncase), dim3(256), 0, (hipStream_t)stream, dcases, prm, dinfo, rtol);
+    return hip_ok(hipGetLastError());
 }
 
 int64_t lica_net_window() { return NET_WIN; }
-
-// layout of a batch's cases (called when the batch adopts the network, so
-// that the per-substep guess launches no copies and can be graph-captured)
-int lica_net_prepare(lica_net *t, const int64_t *lens, int ncase, const NetCase *dcases)
-{
-    int rc = net_layout(t, lens, ncase);
-    if (!rc) t->layout_owner = (const void *)dcases;
-    return rc;
-}
 
 // training windows from a batch's captured systems (see lica_net_add_pairs)
 int lica_net_append(lica_net *t, const NetPairSrc *hsrc, int ncase, int64_t wpc, int64_t len, uint64_t seed,

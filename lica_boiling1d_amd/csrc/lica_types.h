@@ -86,15 +86,20 @@ struct BatchDev {
     double *guess;       // initial guesses of the warm-started modes (rows at gs_off)
     double *hist;        // solution history of the warm-started modes
     double *cap;         // P before / after the last substep's solve, its RHS (3 x (n+2) at 3*gs_off)
-    double *wscale;      // per case 4 doubles: network window start, backward error of the warm-start
-                         // base, its residual scale s_r, the row scale s_b (null without the network)
+    double *wscale;      // per case LICA_WARM_INFO doubles (learned mode): network window start,
+                         // backward error of the warm-start base, its residual scale s_r, the row scale
+                         // s_b, the network step alpha, the base's backward error as the multigrid start
+                         // test computes it, 1 if the solve may take the guess as it is
 };
+#define LICA_WARM_INFO 7
 
 #define LICA_MG_FP64_MODE 2
 #define LICA_MG_MIXED_MODE 3
 #define LICA_MG_LEARNED_MODE 4
 #define LICA_MG_PROJ_MODE 5
-#define MG_HIST 2            // solutions per substep kept for the warm start
+#ifndef MG_HIST
+#define MG_HIST 4            // solutions per substep kept for the warm start
+#endif
 // modes whose V-cycles start from the warm-start guess
 #define LICA_MG_WARM(mode) ((mode) == LICA_MG_LEARNED_MODE || (mode) == LICA_MG_PROJ_MODE)
 

@@ -148,8 +148,9 @@ class Batch(_Handle):
 
     def warm_info(self):
         """Per case [first row of the network window, backward error of the
-        warm-start base, s_r, s_b] of the last learned-mode correction."""
-        out = np.zeros((len(self.cases), 4))
+        warm-start base, s_r, s_b, alpha, start-test backward error, 1 if the
+        solve took the guess as it was] of the last learned-mode substep."""
+        out = np.zeros((len(self.cases), 7))
         _check(self._lib.lica_batch_warm_info(self._h, _dptr(out), out.size), "warm_info")
         return out
 

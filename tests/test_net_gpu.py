@@ -144,17 +144,33 @@ def _base_scales(a, b, c, g, x):
     return (m0 / den if den > 0 else m0), (m0 or 1.0), (m1 or 1.0)
 
 
-def test_learned_base_scales_equal_host():
+def _start_test(a, b, c, g, x, rtol):
+    """The multigrid start test on x: backward error with the residual
+    accumulated as ((g - b x) - a x-) - c x+, and whether x is taken as is."""
+    r = g - b * x
+    r[1:] = r[1:] - a[1:] * x[:-1]
+    r[:-1] = r[:-1] - c[:-1] * x[1:]
+    ak = np.r_[0.0, np.abs(a[1:])]
+    ck = np.r_[np.abs(c[:-1]), 0.0]
+    rm, fm = np.max(np.abs(r)), np.max(np.abs(g))
+    den = np.max((ak + np.abs(b)) + ck) * np.max(np.abs(x)) + fm
+    res = rm / den if den > 0 else rm
+    return res, (not rm > fm) and (not res > rtol)
+
+
+@pytest.mark.parametrize("mode", [L.POISSON_LEARNED])
+def test_learned_base_scales_equal_host(mode):
     """The warm-start base kernel's pass over B also yields the backward
-    error and the scales the network's features use; they equal, bit for bit,
-    the ones computed on the host from the captured rows of the same substep
-    (the base B before the network's correction, the matrix, the RHS)."""
-    _say("learned-mode base scales, 3 cases n=200")
+    error and the scales the network's features use, and the multigrid
+    solve's start test on B; they equal, bit for bit, the ones computed on
+    the host from the captured rows of the same substep (the base B before
+    the network's correction, the matrix, the RHS)."""
+    _say(f"warm-start base scales, 3 cases n=200, mode {mode}")
     cases = [L.deck_case(200, 2).copy(z0=z) for z in (0.05, 0.0607, 0.08)]
     net = Net(seed=0)
     b = L.Batch(cases)
     b.set_net(net)
-    b.set_poisson(L.POISSON_LEARNED, 1e-10, 100)
+    b.set_poisson(mode, 1e-10, 100)
     b.run(3)
     b.set_capture(True, substep=3)
     b.run(1)
@@ -162,8 +178,52 @@ def test_learned_base_scales_equal_host():
     for i in range(3):
         a, bb, c, g = (b.field(k, i) for k in ("PA", "PB", "PC", "PRHS"))
         x = b.field("PPREV", i)
-        assert tuple(info[i, 1:]) == _base_scales(a, bb, c, g, x), i
-        assert info[i, 0] == 1.0   # the window covers the whole column
+        assert tuple(info[i, 1:4]) == _base_scales(a, bb, c, g, x), i
+        if mode == L.POISSON_LEARNED:
+            assert info[i, 0] == 1.0   # the window covers the whole column
+        res, take = _start_test(a, bb, c, g, x, 1e-10)
+        assert info[i, 5] == res, i
+        # the network clears the verdict of the cases whose guess it moved
+        moved = mode == L.POISSON_LEARNED and info[i, 1] > 1e-10
+        assert info[i, 6] == (1.0 if take and not moved else 0.0), i
+    b.close()
+    net.close()
+
+
+def test_learned_correction_matches_layer_kernels():
+    """The per-case kernel of the learned mode (features, the four layers in
+    LDS, the step alpha) against the layer-by-layer training-path kernels:
+    the host computes the features of the captured base, runs them through
+    lica_net_forward and fits alpha = <A d, r> / <A d, A d> in FP64; the
+    device's alpha agrees to summation order."""
+    _say("learned correction vs layer kernels, 3 cases n=200")
+    cases = [L.deck_case(200, 2).copy(z0=z) for z in (0.05, 0.0607, 0.08)]
+    net = Net(seed=0)
+    assert collect_pairs(net, cases, steps=3, windows_per_case=2, window=128) > 0
+    net.train(50, batch=4, lr=1e-3)
+    b = L.Batch(cases)
+    b.set_net(net)
+    b.set_poisson(L.POISSON_LEARNED, 1e-10, 100)
+    b.run(2)
+    b.set_capture(True, substep=3)
+    b.run(1)
+    info = b.warm_info()
+    applied = 0
+    for i in range(3):
+        a, bb, c, g = (b.field(k, i) for k in ("PA", "PB", "PC", "PRHS"))
+        x = b.field("PPREV", i)
+        if not info[i, 1] > 1e-10:
+            assert info[i, 4] == 0.0
+            continue
+        applied += 1
+        d = net.forward([features(a, bb, c, g, x)])[0].astype(np.float64)
+        ad = bb * d
+        ad[1:] += a[1:] * d[:-1]
+        ad[:-1] += c[:-1] * d[1:]
+        r = g - ((bb * x + np.r_[0.0, a[1:] * x[:-1]]) + np.r_[c[:-1] * x[1:], 0.0])
+        alpha = np.dot(ad, r) / np.dot(ad, ad)
+        assert abs(info[i, 4] - alpha) <= 1e-9 * abs(alpha), (i, info[i, 4], alpha)
+    assert applied > 0
     b.close()
     net.close()
 
