@@ -22,7 +22,9 @@
 #pragma once
 #include "lica_kern_common.h"
 
+#ifndef MG_COARSE
 #define MG_COARSE 32       // levels stop at this many rows
+#endif
 #define MG_MAXLEV 20
 #define MG_NU 2            // pre- and post-smoothing sweeps
 

@@ -84,11 +84,18 @@ long long lica_mg_lds_max()
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess || optin <= 0)
             optin = 65536;
-        cap = (long long)optin - 4096;
+        // the static part (MgSh) of the larger of the two kernels comes off
         const void *ks[2] = {(const void *)k_poisson_mg, (const void *)k_mg_systems};
+        long long stat = 4096;
+        for (int i = 0; i < 2; i++) {
+            hipFuncAttributes fa;
+            if (hipFuncGetAttributes(&fa, ks[i]) == hipSuccess)
+                stat = std::max(stat, (long long)fa.sharedSizeBytes);
+        }
+        cap = ((long long)optin - stat) / 256 * 256;
         for (int i = 0; i < 2; i++)
             if (hipFuncSetAttribute(ks[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)cap) != hipSuccess) {
-                cap = 65536 - 4096;
+                cap = (65536 - stat) / 256 * 256;
                 break;
             }
     }

@@ -49,21 +49,27 @@ struct NetSeq {
 // rows with NET_HALF halo rows on either side, ws the layer's weights;
 // wave w gets rows 64w..64w+63 as four 16-row blocks (acc[rb][nt]: block
 // rb, output channels 16nt..16nt+15).
+// Taps t0 .. t1-1 only, with ws holding those taps (ws[t - t0]) and acc
+// carried over from the previous taps when t0 > 0: the sums accumulate in
+// the same order whatever the grouping.
 template <int COUT>
-__device__ __forceinline__ void net_tile_mfma(const __bf16 *xs, const __bf16 *ws, f32x4 (&acc)[4][COUT / 16])
+__device__ __forceinline__ void net_tile_mfma(const __bf16 *xs, const __bf16 *ws, f32x4 (&acc)[4][COUT / 16],
+                                              int t0 = 0, int t1 = NET_TAPS)
 {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int m = lane & 15, kq = lane >> 4;                  // operand row / k group
     constexpr int NT = COUT / 16;
+    if (t0 == 0) {
 #pragma unroll
-    for (int rb = 0; rb < 4; rb++)
+        for (int rb = 0; rb < 4; rb++)
 #pragma unroll
-        for (int nt = 0; nt < NT; nt++) acc[rb][nt] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+            for (int nt = 0; nt < NT; nt++) acc[rb][nt] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+    }
     const int r0 = wave * 64;                                 // first tile row of this wave
-    for (int t = 0; t < NET_TAPS; t++) {
+    for (int t = t0; t < t1; t++) {
         bf16x8 b[NT];
 #pragma unroll
-        for (int nt = 0; nt < NT; nt++) b[nt] = *(const bf16x8 *)(ws + (t * COUT + nt * 16 + m) * NET_C + kq * 8);
+        for (int nt = 0; nt < NT; nt++) b[nt] = *(const bf16x8 *)(ws + ((t - t0) * COUT + nt * 16 + m) * NET_C + kq * 8);
 #pragma unroll
         for (int rb = 0; rb < 4; rb++) {
             const bf16x8 a = *(const bf16x8 *)(xs + (r0 + rb * 16 + m + t) * NET_C + kq * 8);
@@ -392,22 +398,28 @@ struct NetParams {
     const float *b[NET_LAYERS];
 };
 
-// One layer of the per-case network in LDS: the layer's BF16 weights into
-// ws, the FP32 sums over the tile, then bias (+ ReLU) into the next
-// activation rows (BF16, the halo and the rows beyond the window stay zero)
-// or, for the last layer, channel 0's FP32 pre-activation into ys.
+// One layer of the per-case network in LDS: the layer's BF16 weights pass
+// through ws NET_TGRP taps at a time, the FP32 sums over the tile stay in
+// registers, then bias (+ ReLU) goes back into the same activation rows
+// (BF16; the halo and the rows beyond the window stay zero) or, for the
+// last layer, channel 0's FP32 pre-activation into ys.
+#define NET_TGRP 7
+static_assert(NET_TAPS % NET_TGRP == 0, "tap groups");
 template <int COUT>
-__device__ __forceinline__ void net_case_layer(const __bf16 *xin, __bf16 *xout, __bf16 *ws, float *ys,
-                                               const __bf16 *__restrict__ wl, const float *__restrict__ bl, int w,
-                                               bool last)
+__device__ __forceinline__ void net_case_layer(__bf16 *xs, __bf16 *ws, float *ys, const __bf16 *__restrict__ wl,
+                                               const float *__restrict__ bl, int w, bool last)
 {
     const int tid = threadIdx.x;
-    for (int i = tid; i < NET_TAPS * COUT * NET_C / 8; i += blockDim.x)
-        *(bf16x8 *)(ws + i * 8) = *(const bf16x8 *)(wl + i * 8);
-    __syncthreads();
     constexpr int NT = COUT / 16;
     f32x4 acc[4][NT];
-    net_tile_mfma<COUT>(xin, ws, acc);
+    for (int t0 = 0; t0 < NET_TAPS; t0 += NET_TGRP) {
+        const __bf16 *src = wl + (size_t)t0 * COUT * NET_C;
+        for (int i = tid; i < NET_TGRP * COUT * NET_C / 8; i += blockDim.x)
+            *(bf16x8 *)(ws + i * 8) = *(const bf16x8 *)(src + i * 8);
+        __syncthreads();
+        net_tile_mfma<COUT>(xs, ws, acc, t0, t0 + NET_TGRP);
+        __syncthreads();   // ws (and, after the last group, xs) free
+    }
     const int wave = tid >> 6, lane = tid & 63, m = lane & 15, kq = lane >> 4, r0 = wave * 64;
 #pragma unroll
     for (int rb = 0; rb < 4; rb++)
@@ -422,11 +434,11 @@ __device__ __forceinline__ void net_case_layer(const __bf16 *xin, __bf16 *xout, 
                 if (last) {
                     if (co == 0) ys[cell] = v;
                 } else {
-                    xout[(NET_HALF + cell) * NET_C + co] = to_bf16(fmaxf(v, 0.0f));
+                    xs[(NET_HALF + cell) * NET_C + co] = to_bf16(fmaxf(v, 0.0f));
                 }
             }
         }
-    __syncthreads();   // ws and xin are free, xout / ys complete
+    __syncthreads();   // the next layer's input / ys complete
 }
 
 // The network's correction of a batch's warm-start guesses, one workgroup
@@ -443,8 +455,8 @@ __global__ void __launch_bounds__(256) k_net_case(const NetCase *__restrict__ ca
                                                   double *__restrict__ info, double rtol)
 {
     constexpr int ROWS = NET_TILE + NET_TAPS - 1;
-    __shared__ __attribute__((aligned(16))) __bf16 xs[2][ROWS * NET_C];
-    __shared__ __attribute__((aligned(16))) __bf16 ws[NET_TAPS * NET_C * NET_C];
+    __shared__ __attribute__((aligned(16))) __bf16 xs[ROWS * NET_C];
+    __shared__ __attribute__((aligned(16))) __bf16 ws[NET_TGRP * NET_C * NET_C];
     __shared__ float ys[NET_TILE];
     const NetCase cs = cases[blockIdx.x];
     const int tid = threadIdx.x;
@@ -454,25 +466,21 @@ __global__ void __launch_bounds__(256) k_net_case(const NetCase *__restrict__ ca
     if (tid == 0) { inf[0] = (double)k0; inf[4] = 0.0; }
     if (!(berr > rtol)) return;
     if (tid == 0) inf[6] = 0.0;   // the guess moves: the solve tests it afresh
-    // layer 0 input (features, channels NET_NF.. zero) and zero rows outside
-    // the window in both activation buffers
+    // layer 0 input (features, channels NET_NF.. zero), zero rows outside
+    // the window
     for (int i = tid; i < ROWS; i += blockDim.x) {
         const int c = i - NET_HALF;
-        __bf16 *row = xs[0] + i * NET_C;
+        __bf16 *row = xs + i * NET_C;
         int ch0 = 0;
         if (c >= 0 && c < w) {
             net_row(cs.a, cs.b, cs.c, cs.g, cs.guess, n, k0 + c, sr, sb, row);
             ch0 = NET_NF;
-        } else {
-            for (int ch = 0; ch < NET_C; ch++) xs[1][i * NET_C + ch] = to_bf16(0.0f);
         }
         for (int ch = ch0; ch < NET_C; ch++) row[ch] = to_bf16(0.0f);
     }
     __syncthreads();
-    int cur = 0;
-    for (int l = 0; l < NET_LAYERS - 1; l++, cur ^= 1)
-        net_case_layer<NET_C>(xs[cur], xs[cur ^ 1], ws, ys, prm.w[l], prm.b[l], w, false);
-    net_case_layer<16>(xs[cur], xs[cur ^ 1], ws, ys, prm.w[NET_LAYERS - 1], prm.b[NET_LAYERS - 1], w, true);
+    for (int l = 0; l < NET_LAYERS - 1; l++) net_case_layer<NET_C>(xs, ws, ys, prm.w[l], prm.b[l], w, false);
+    net_case_layer<16>(xs, ws, ys, prm.w[NET_LAYERS - 1], prm.b[NET_LAYERS - 1], w, true);
     auto d = [&](int k) -> double { return (k >= k0 && k < k0 + w) ? (double)ys[k - k0] : 0.0; };
     double v[2] = {0.0, 0.0};
     for (int k = max(k0 - 1, 1) + tid; k <= min(k0 + w, n); k += blockDim.x) {
