@@ -594,6 +594,66 @@ DEV void tri_solve(double *a, double *b, double *c, double *g, int l1, int l2, i
 #endif
 }
 
+// A tridiagonal solve whose rows come from a function instead of arrays:
+// row(k, a, b, c, g) gives row k of l1..l2 (called once per row, by the
+// thread whose chunk holds it), out(k, x) receives its solution (after every
+// row was read).  In mode 2 with the rows in the per-thread register chunks
+// nothing goes through global memory: the chunks are filled by row() and the
+// solve is tri_solve's register path, operation for operation.  Otherwise
+// the rows go to the arrays a, b, c, g and tri_solve runs on them.
+template <class RowF, class OutF>
+DEV void tri_solve_rows(double *a, double *b, double *c, double *g, int l1, int l2, int mode, TriSh &ts, RowF row,
+                        OutF out)
+{
+    const int nrow = l2 - l1 + 1;
+    int nc = NTHREADS;
+    if (nc > nrow / 2) nc = nrow / 2;
+#ifndef LICA_EMUL
+    const bool regs = (mode == 2 && nrow >= 4 && (nrow + nc - 1) / nc <= TRI_RPT);
+#else
+    const bool regs = false;
+#endif
+    if (!regs) {
+        PFOR(k, l1, l2 + 1) { row(k, a[k], b[k], c[k], g[k]); }
+        PFOR_END
+        tri_solve(a, b, c, g, l1, l2, mode, ts);
+        PFOR(k, l1, l2 + 1) { out(k, g[k]); }
+        PFOR_END
+        return;
+    }
+#ifndef LICA_EMUL
+    double *RA = ts.ws, *RB = ts.ws + ts.ws_rows, *RC = ts.ws + 2 * ts.ws_rows, *RG = ts.ws + 3 * ts.ws_rows;
+    const int ch = (int)threadIdx.x;
+    const bool act = ch < nc;
+    int s = 0, L = 0;
+    double xa[TRI_RPT], xb[TRI_RPT], xc[TRI_RPT], x[TRI_RPT];
+    if (act) {
+        chunk_bounds(nrow, nc, ch, &s, &L);
+        static_for<TRI_RPT>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            if (j < L) {
+                row(l1 + s + j, xa[j], xb[j], xc[j], x[j]);
+            } else {
+                xa[j] = 0.0; xb[j] = 1.0; xc[j] = 0.0; x[j] = 0.0;
+            }
+        });
+        double r[8];
+        chunk_reduce_reg(xa, xb, xc, x, L, r);
+        put_end_rows(RA, RB, RC, RG, ch, nc, r);
+    }
+    BARRIER();
+    lds_tri_solve(ts, 2 * nc);
+    if (act) {
+        chunk_recover_reg(xa, xc, x, L, RG[2 * ch], RG[2 * ch + 1]);
+        static_for<TRI_RPT>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            if (j < L) out(l1 + s + j, x[j]);
+        });
+    }
+    BARRIER();
+#endif
+}
+
 // ---------------------------------------------------------------------
 // Region classes of CAL_CONTINUITY / CAL_CONTINUITY2 (1d.f:2037-2056):
 // 2 within two cells of a jump of PSI_CN, 1 pure liquid, 0 otherwise.
@@ -841,7 +901,7 @@ DEV void body_cont_energy(const BatchDev &bd, int inst, BlkSh &sh)
     const double g = P.gamma[msub - 1], ro = P.ro[msub - 1];
     const double den_diffi = (P.den_diff == 0.0) ? 1.0 : 1.0 / P.den_diff;
     const double acops = alpha * dt / (P.re * P.prm);
-    PFOR(k, 1, n + 1) {
+    auto lhsps_row = [&](int k, double &ta, double &tb, double &tc, double &tg) {
         double rhs;
         if (m != 1) {
             rhs = (g * T[k] * (W[k + 1] - W[k]) * VVDZ[k]) * dt;
@@ -869,15 +929,12 @@ DEV void body_cont_energy(const BatchDev &bd, int inst, BlkSh &sh)
             if (pcn <= 0.5) ck = 0.0;
             else if (pcn > 0.5) ak = 0.0;
         }
-        TA[k] = ak; TB[k] = bk; TC[k] = ck; TG[k] = rhs;
-    }
-    PFOR_END
-    tri_solve(TA, TB, TC, TG, 1, n, bd.tri_mode, sh.tri);
-    PFOR(k, 1, n + 1) {
-        RHSPS[k] = TG[k];
-        T[k] = TG[k] + T[k];
-    }
-    PFOR_END
+        ta = ak; tb = bk; tc = ck; tg = rhs;
+    };
+    tri_solve_rows(TA, TB, TC, TG, 1, n, bd.tri_mode, sh.tri, lhsps_row, [&](int k, double x) {
+        RHSPS[k] = x;
+        T[k] = x + T[k];
+    });
     SERIAL {
         T[0] = 0.0;
         T[n] = 1.0;

@@ -73,7 +73,7 @@ DEV void body_momentum(const BatchDev &bd, int inst, BlkSh &sh)
     }
     // RHSZ (1d.f:1603-1745), RHS1/DENF_Z - W (1557-1560), LHSW (1749-1809)
     const double g = P.gamma[msub - 1], ro = P.ro[msub - 1], a2 = 2.0 * alpha;
-    PFOR(k, 2, n + 1) {
+    auto lhsw_row = [&](int k, double &ta, double &tb, double &tc, double &tg) {
         int kp1 = k + 1, km1 = k - 1;
         double wt = 0.5 * (W[kp1] + W[k]);
         double wb = 0.5 * (W[k] + W[km1]);
@@ -98,12 +98,9 @@ DEV void body_momentum(const BatchDev &bd, int inst, BlkSh &sh)
         double dk1 = 2.0 * acoef * dzi * VFZ[km1] * AKW[k];
         double dk3 = 2.0 * acoef * dzi * VFZ[k] * CKW[k];
         double dk2 = -dk1 - dk3;
-        TA[k] = dk1; TC[k] = dk3; TB[k] = 1.0 + dk2; TG[k] = rhs1;
-    }
-    PFOR_END
-    tri_solve(TA, TB, TC, TG, 2, n, bd.tri_mode, sh.tri);
-    PFOR(k, 2, n + 1) { W[k] = TG[k] + W[k]; }
-    PFOR_END
+        ta = dk1; tc = dk3; tb = 1.0 + dk2; tg = rhs1;
+    };
+    tri_solve_rows(TA, TB, TC, TG, 2, n, bd.tri_mode, sh.tri, lhsw_row, [&](int k, double x) { W[k] = x + W[k]; });
     // BC + BC_FLUX (1d.f:1812-1923)
     SERIAL {
         W[1] = 0.0;

@@ -215,6 +215,33 @@ def test_multigrid_modes(mode, which, steps):
 
 
 @pytest.mark.parametrize("mode", [L.POISSON_MG64, L.POISSON_MIXED])
+def test_multigrid_modes_equal_logic_check_build(mode):
+    """The same kernel source compiled for the CPU (work items in order, the
+    logic-check build the CPU tests pin against the oracle) gives the GPU's
+    fields and scalars bit for bit in the multigrid modes: the partitioned
+    energy and momentum solves (rows in registers on the GPU, in arrays on
+    the CPU) and the V-cycles (FP64, or FP32 in FP64 defect correction) do
+    the same operations in the same order."""
+    import emul_build
+    from lica_boiling1d_amd import _abi
+    if not os.path.exists(emul_build.LIB):
+        pytest.fail("the logic-check build is missing: run __graft_entry__.build() first")
+    emul = _abi.load(emul_build.LIB)
+    cases = [L.deck_case(n, 2) for n in (100, 257, 600)]
+    _say(f"multigrid mode {mode}: GPU vs logic-check build, n = 100, 257, 600, 12 steps")
+    g, e = L.Batch(cases), L.Batch(cases, lib=emul)
+    for b in (g, e):
+        b.set_poisson(mode, 1e-10, 100)
+        b.run(12)
+    for i in range(len(cases)):
+        for f in ("W", "P", "T", "PSI_CN", "PSI_ZN", "PSIF", "PHI"):
+            assert np.array_equal(g.field(f, i), e.field(f, i)), (i, f)
+        assert np.array_equal(g.scalar_array(i), e.scalar_array(i)), i
+    g.close()
+    e.close()
+
+
+@pytest.mark.parametrize("mode", [L.POISSON_MG64, L.POISSON_MIXED])
 def test_batched_poisson_multigrid(mode):
     """lica_poisson by V-cycles on the POISSON systems of the deck case (the
     last substep of steps 1..8): backward error <= 1e-10 and the solution
