@@ -440,9 +440,9 @@ BatchDev make_bd(const lica_batch *b, int msub)
 
 // threads per workgroup: the requested count, raised in the partitioned mode
 // until every case's rows fit the per-thread register chunks.  One size for
-// the whole launch: per-size launches one after the other were measured
-// slower (each drains before the next starts, and the largest-first order
-// no longer balances the tail).
+// the whole launch: launches per size class were measured slower, one after
+// the other (each drains before the next starts, and the largest-first
+// order no longer balances the tail) and on two streams at once alike.
 int block_threads(const lica_batch *b)
 {
     int t = b->threads;
