@@ -100,7 +100,15 @@ DEV void body_momentum(const BatchDev &bd, int inst, BlkSh &sh)
         double dk2 = -dk1 - dk3;
         ta = dk1; tc = dk3; tb = 1.0 + dk2; tg = rhs1;
     };
-    tri_solve_rows(TA, TB, TC, TG, 2, n, bd.tri_mode, sh.tri, lhsw_row, [&](int k, double x) { W[k] = x + W[k]; });
+    // the rows go through the scratch arrays: assembled in the coalesced
+    // order, the solve reads them back by chunks (assembling straight into
+    // the chunks reads the fifteen input arrays chunk-strided, which moved
+    // 65% more HBM bytes for no gain in time)
+    PFOR(k, 2, n + 1) { lhsw_row(k, TA[k], TB[k], TC[k], TG[k]); }
+    PFOR_END
+    tri_solve(TA, TB, TC, TG, 2, n, bd.tri_mode, sh.tri);
+    PFOR(k, 2, n + 1) { W[k] = TG[k] + W[k]; }
+    PFOR_END
     // BC + BC_FLUX (1d.f:1812-1923)
     SERIAL {
         W[1] = 0.0;
