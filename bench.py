@@ -107,6 +107,11 @@ def cpu_baseline(cases, seconds=15.0):
     return {"value": steps / el, "unit": "instance-steps/s", "cores": 1, "kind": "port", "sample": sample}
 
 
+def lanes_of(cases, args):
+    """Case sets run side by side on two streams (2) or one sequence (1)."""
+    return args.lanes if args.lanes else (2 if len(cases) >= 256 else 1)
+
+
 def timed_run(L, cases, mode, args, net=None):
     """One timed run: warmup steps, then args.steps steps with per-kernel HIP
     events; only the steps the cases completed are counted (a case that
@@ -118,6 +123,7 @@ def timed_run(L, cases, mode, args, net=None):
         b.set_net(net)
     b.set_poisson(mode, args.rtol, 200)
     b.set_block(args.threads)
+    b.set_lanes(lanes_of(cases, args))
     if args.warmup:
         b.run_all(args.warmup)
     b.sync()
@@ -171,12 +177,15 @@ def roofline(L, cases, mode, args, res):
     tim = dict(res["timings"])
     if mode >= L.POISSON_MG64:
         tim["poisson"] = tim.get("poisson_mg", 0.0)
-    per = {k: tim.get(k, 0.0) / (args.steps * nl) for k, nl in launches.items()}   # ms per launch
+    # with two lanes every kernel is launched twice per substep, each launch
+    # on half of the cases: ms per launch and bytes per launch both halve
+    nla = lanes_of(cases, args)
+    per = {k: tim.get(k, 0.0) / (args.steps * nl * nla) for k, nl in launches.items()}   # ms per launch
     # the warm-start slot (projection, network) is reported, not rated
     dom = max((k for k in per if k != "warm_start"), key=lambda k: per[k])
     vc = res.get("vcycles_per_solve", 0.0)
     live = [c for c, g in zip(cases, res["ok"]) if g]
-    bytes_launch = sum(kernel_bytes(c, dom, mode, vc) for c in live)
+    bytes_launch = sum(kernel_bytes(c, dom, mode, vc) for c in live) / nla
     ach = bytes_launch / (per[dom] * 1e-3) / 1e9
     return ({"bound": "hbm", "kernel": "k_" + ("poisson_mg" if dom == "poisson" and mode >= L.POISSON_MG64 else dom),
              "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
@@ -193,6 +202,7 @@ def main():
     ap.add_argument("--mode", choices=["learned", "projected", "direct", "thomas", "mg64", "mixed"], default="learned")
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--threads", type=int, default=256)
+    ap.add_argument("--lanes", type=int, default=0, help="case sets on two streams (2) or one (1); 0: 2 for >= 256 cases")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--train-cases", type=int, default=4096)
     ap.add_argument("--train-steps", type=int, default=25)
@@ -227,6 +237,7 @@ def main():
     mode = {"learned": L.POISSON_LEARNED, "projected": L.POISSON_PROJECTED, "direct": L.POISSON_DIRECT,
             "thomas": L.POISSON_THOMAS, "mg64": L.POISSON_MG64, "mixed": L.POISSON_MIXED}[args.mode]
     wl["poisson"] = args.mode
+    wl["lanes"] = lanes_of(cases, args)
     if mode >= L.POISSON_MG64:
         wl["poisson_backward_error"] = args.rtol
 

@@ -28,7 +28,8 @@ int lica_emul_nthreads = 256;
 int lica_emul_depth = 0;
 typedef void *hipStream_t;
 // the warm-start network runs on MFMA only; the CPU logic-check build has none
-int lica_net_guess(lica_net *, const NetCase *, int, double *, double, void *) { return LICA_UNSUPPORTED_DECK; }
+int lica_net_guess(lica_net *, const NetCase *, const int *, int, double *, double, void *) { return LICA_UNSUPPORTED_DECK; }
+int lica_net_cast(lica_net *, void *) { return LICA_UNSUPPORTED_DECK; }
 int lica_net_append(lica_net *, const NetPairSrc *, int, int64_t, int64_t, uint64_t, int64_t *)
 {
     return LICA_UNSUPPORTED_DECK;
@@ -385,11 +386,25 @@ struct lica_batch {
     hipStream_t stream = nullptr;
 #ifndef LICA_EMUL
     hipGraphExec_t graph = nullptr;
-    int graph_threads = -1, graph_mode = -1;
+    int graph_threads = -1, graph_mode = -1, graph_lanes = -1;
     int graph_capture = 0;
     const lica_net *graph_net = nullptr;
     long long graph_mg_lds = -1;
-    std::vector<hipEvent_t> ev;
+    std::vector<hipEvent_t> ev[2];     // timing events of the lanes
+#endif
+    // Lanes: the cases split into two sets (alternate cases of the
+    // largest-first order) whose steps run as independent launch sequences
+    // on two streams, so that a latency-bound kernel of one set shares the
+    // CUs with a bandwidth-bound kernel of the other.  In timing mode each
+    // lane records its own events (a kernel's time is then its duration
+    // while the other lane runs beside it).
+    int lanes = 1;
+    int cur_lane = -1;                 // set being enqueued (-1: all cases)
+    int lane_lo[2] = {0, 0}, lane_hi[2] = {0, 0};
+    int *d_order_lanes = nullptr;      // lane-major workgroup order
+    hipStream_t lane_stream[2] = {nullptr, nullptr};
+#ifndef LICA_EMUL
+    hipEvent_t fork = nullptr, join = nullptr;
 #endif
     double timings[KL_COUNT] = {0};
     double wall_ms = 0.0;
@@ -455,10 +470,16 @@ int block_threads(const lica_batch *b)
     return t;
 }
 
+hipStream_t cur_stream(const lica_batch *b) { return b->cur_lane < 0 ? b->stream : b->lane_stream[b->cur_lane]; }
+
 int launch(lica_batch *b, int which, int msub)
 {
     BatchDev bd = make_bd(b, msub);
-    int rc = lica_launch(which, bd, block_threads(b), (void *)b->stream);
+    if (b->cur_lane >= 0) {
+        bd.order = b->d_order_lanes + b->lane_lo[b->cur_lane];
+        bd.ninst = b->lane_hi[b->cur_lane] - b->lane_lo[b->cur_lane];
+    }
+    int rc = lica_launch(which, bd, block_threads(b), (void *)cur_stream(b));
     return rc == 0 ? 0 : LICA_HIP_ERROR;
 }
 
@@ -472,15 +493,46 @@ int pre_poisson(lica_batch *b, int msub)
     if (!rc && b->capture == msub) rc = launch(b, KL_CAPTURE_PRE, msub);
     if (!rc && b->poisson_mode == LICA_POISSON_LEARNED) {
         if (!b->net) return LICA_BAD_ARG;
-        if (lica_net_guess(b->net, b->d_netcases, (int)b->ninst, b->d_scale, b->rtol,
-                           (void *)b->stream))
+        const int *ord = (b->cur_lane >= 0) ? b->d_order_lanes + b->lane_lo[b->cur_lane] : nullptr;
+        const int cnt = (b->cur_lane >= 0) ? b->lane_hi[b->cur_lane] - b->lane_lo[b->cur_lane] : (int)b->ninst;
+        if (lica_net_guess(b->net, b->d_netcases, ord, cnt, b->d_scale, b->rtol, (void *)cur_stream(b)))
             rc = LICA_HIP_ERROR;
     }
     return rc;
 }
 
-// the launches of one time step (1d.f:196-295)
+int enqueue_step_cases(lica_batch *b);
+
+// the launches of one time step (1d.f:196-295); with two lanes the two case
+// sets run their own sequences on two streams, forked from and joined back
+// into the batch's stream
 int enqueue_step(lica_batch *b)
+{
+    if (b->poisson_mode == LICA_POISSON_LEARNED) {
+        if (!b->net) return LICA_BAD_ARG;
+        if (lica_net_cast(b->net, (void *)b->stream)) return LICA_HIP_ERROR;
+    }
+#ifndef LICA_EMUL
+    if (b->lanes == 2 && b->cur_lane < 0) {
+        if (hipEventRecord(b->fork, b->stream) != hipSuccess ||
+            hipStreamWaitEvent(b->lane_stream[1], b->fork, 0) != hipSuccess)
+            return LICA_HIP_ERROR;
+        int rc = 0;
+        for (int l = 0; l < 2 && !rc; l++) {
+            b->cur_lane = l;
+            rc = enqueue_step_cases(b);
+        }
+        b->cur_lane = -1;
+        if (hipEventRecord(b->join, b->lane_stream[1]) != hipSuccess ||
+            hipStreamWaitEvent(b->stream, b->join, 0) != hipSuccess)
+            rc = rc ? rc : LICA_HIP_ERROR;
+        return rc;
+    }
+#endif
+    return enqueue_step_cases(b);
+}
+
+int enqueue_step_cases(lica_batch *b)
 {
     int rc = launch(b, KL_DT, 1);
     const int kp = (b->poisson_mode >= LICA_POISSON_MG64) ? KL_POISSON_MG : KL_POISSON_DIRECT;
@@ -499,17 +551,19 @@ int enqueue_step(lica_batch *b)
 
 #ifndef LICA_EMUL
 // same launches with an event after each one (per-kernel GPU time)
-int enqueue_step_timed(lica_batch *b, std::vector<int> &kinds)
+// (one lane's launches when b->cur_lane >= 0; ev[0] is recorded by the
+// caller on the same stream)
+int enqueue_step_timed(lica_batch *b, std::vector<hipEvent_t> &ev, std::vector<int> &kinds)
 {
     auto rec = [&](int kind) -> int {
         kinds.push_back(kind);
         size_t i = kinds.size();
-        while (b->ev.size() <= i) {
+        while (ev.size() <= i) {
             hipEvent_t e;
             if (hipEventCreate(&e) != hipSuccess) return LICA_HIP_ERROR;
-            b->ev.push_back(e);
+            ev.push_back(e);
         }
-        return hipEventRecord(b->ev[i], b->stream) == hipSuccess ? 0 : LICA_HIP_ERROR;
+        return hipEventRecord(ev[i], cur_stream(b)) == hipSuccess ? 0 : LICA_HIP_ERROR;
     };
     int rc = launch(b, KL_DT, 1);
     if (!rc) rc = rec(KL_DT);
@@ -542,6 +596,7 @@ int build_graph(lica_batch *b)
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) { b->graph = nullptr; return LICA_HIP_ERROR; }
     b->graph_threads = b->threads;
+    b->graph_lanes = b->lanes;
     b->graph_mode = b->poisson_mode;
     b->graph_capture = b->capture;
     b->graph_net = b->net;
@@ -715,6 +770,13 @@ int lica_batch_create(int64_t ninst, const lica_case_desc *cases, lica_batch **o
     }
 #ifndef LICA_EMUL
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) { delete b; return LICA_HIP_ERROR; }
+    b->lane_stream[0] = b->stream;
+    if (hipStreamCreateWithFlags(&b->lane_stream[1], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess) {
+        lica_batch_destroy(b);
+        return LICA_HIP_ERROR;
+    }
 #endif
     int rc = 0;
     rc = rc ? rc : dmalloc((void **)&b->d_par, sizeof(InstParam) * (size_t)ninst);
@@ -733,6 +795,17 @@ int lica_batch_create(int64_t ninst, const lica_case_desc *cases, lica_batch **o
     for (int64_t i = 0; i < ninst; i++) order[(size_t)i] = (int)i;
     std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return b->par[(size_t)a].n > b->par[(size_t)c].n; });
     rc = h2d(b->d_order, order.data(), sizeof(int) * order.size(), b->stream);
+    {
+        // lane 0: cases 0, 2, 4, ... of the order, lane 1: cases 1, 3, 5, ...
+        std::vector<int> lo;
+        for (int pass = 0; pass < 2; pass++)
+            for (size_t j = (size_t)pass; j < order.size(); j += 2) lo.push_back(order[j]);
+        b->lane_lo[0] = 0; b->lane_hi[0] = (int)((order.size() + 1) / 2);
+        b->lane_lo[1] = b->lane_hi[0]; b->lane_hi[1] = (int)order.size();
+    b->lanes = (ninst >= 256) ? 2 : 1;
+        rc = rc ? rc : dmalloc((void **)&b->d_order_lanes, sizeof(int) * lo.size());
+        rc = rc ? rc : h2d(b->d_order_lanes, lo.data(), sizeof(int) * lo.size(), b->stream);
+    }
     rc = rc ? rc : h2d(b->d_par, b->par.data(), sizeof(InstParam) * b->par.size(), b->stream);
     if (rc) { lica_batch_destroy(b); return rc; }
     *out = b;
@@ -775,25 +848,43 @@ int lica_batch_run(lica_batch *b, int64_t nsteps)
     for (int64_t s = 0; s < nsteps && !rc; s++) rc = enqueue_step(b);
 #else
     if (b->timing) {
+        const int nl = (b->lanes == 2) ? 2 : 1;
         for (int64_t s = 0; s < nsteps && !rc; s++) {
-            std::vector<int> kinds;
-            if (b->ev.empty()) {
-                hipEvent_t e;
-                if (hipEventCreate(&e) != hipSuccess) return LICA_HIP_ERROR;
-                b->ev.push_back(e);
+            std::vector<int> kinds[2];
+            if (b->poisson_mode == LICA_POISSON_LEARNED) {
+                if (!b->net) return LICA_BAD_ARG;
+                if (lica_net_cast(b->net, (void *)b->stream)) return LICA_HIP_ERROR;
             }
-            if (hipEventRecord(b->ev[0], b->stream) != hipSuccess) return LICA_HIP_ERROR;
-            rc = enqueue_step_timed(b, kinds);
+            if (nl == 2 && (hipEventRecord(b->fork, b->stream) != hipSuccess ||
+                            hipStreamWaitEvent(b->lane_stream[1], b->fork, 0) != hipSuccess))
+                return LICA_HIP_ERROR;
+            for (int l = 0; l < nl && !rc; l++) {
+                b->cur_lane = (nl == 2) ? l : -1;
+                if (b->ev[l].empty()) {
+                    hipEvent_t e;
+                    if (hipEventCreate(&e) != hipSuccess) { rc = LICA_HIP_ERROR; break; }
+                    b->ev[l].push_back(e);
+                }
+                if (hipEventRecord(b->ev[l][0], cur_stream(b)) != hipSuccess) { rc = LICA_HIP_ERROR; break; }
+                rc = enqueue_step_timed(b, b->ev[l], kinds[l]);
+            }
+            b->cur_lane = -1;
+            if (nl == 2 && (hipEventRecord(b->join, b->lane_stream[1]) != hipSuccess ||
+                            hipStreamWaitEvent(b->stream, b->join, 0) != hipSuccess))
+                rc = rc ? rc : LICA_HIP_ERROR;
             if (rc) break;
-            if (hipEventSynchronize(b->ev[kinds.size()]) != hipSuccess) return LICA_HIP_ERROR;
-            for (size_t j = 0; j < kinds.size(); j++) {
-                float ms = 0.0f;
-                if (hipEventElapsedTime(&ms, b->ev[j], b->ev[j + 1]) != hipSuccess) return LICA_HIP_ERROR;
-                b->timings[kinds[j]] += (double)ms;
+            for (int l = 0; l < nl; l++) {
+                if (hipEventSynchronize(b->ev[l][kinds[l].size()]) != hipSuccess) return LICA_HIP_ERROR;
+                for (size_t j = 0; j < kinds[l].size(); j++) {
+                    float ms = 0.0f;
+                    if (hipEventElapsedTime(&ms, b->ev[l][j], b->ev[l][j + 1]) != hipSuccess) return LICA_HIP_ERROR;
+                    b->timings[kinds[l][j]] += (double)ms;
+                }
             }
         }
     } else if (b->use_graph) {
-        if (!b->graph || b->graph_threads != b->threads || b->graph_mode != b->poisson_mode ||
+        if (!b->graph || b->graph_threads != b->threads || b->graph_lanes != b->lanes ||
+            b->graph_mode != b->poisson_mode ||
             b->graph_capture != b->capture || b->graph_net != b->net || b->graph_mg_lds != mg_lds(b))
             rc = build_graph(b);
         for (int64_t s = 0; s < nsteps && !rc; s++)
@@ -841,6 +932,15 @@ int lica_batch_set_mg_lds(lica_batch *b, int64_t bytes)
     int rc = check_batch(b);
     if (rc) return rc;
     b->mg_lds_cap = (bytes < 0) ? -1 : (long long)bytes;
+    return LICA_OK;
+}
+
+int lica_batch_set_lanes(lica_batch *b, int64_t lanes)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (lanes != 1 && lanes != 2) return LICA_BAD_ARG;
+    b->lanes = (b->ninst >= 2) ? (int)lanes : 1;
     return LICA_OK;
 }
 
@@ -1018,11 +1118,16 @@ void lica_batch_destroy(lica_batch *b)
     if (!b) return;
 #ifndef LICA_EMUL
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->lane_stream[1]) (void)hipStreamSynchronize(b->lane_stream[1]);
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
-    for (hipEvent_t e : b->ev) (void)hipEventDestroy(e);
+    for (int l = 0; l < 2; l++)
+        for (hipEvent_t e : b->ev[l]) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->join) (void)hipEventDestroy(b->join);
+    if (b->lane_stream[1]) (void)hipStreamDestroy(b->lane_stream[1]);
 #endif
     dfree(b->d_par); dfree(b->d_st); dfree(b->d_ns); dfree(b->d_rf);
-    dfree(b->d_nsi); dfree(b->d_rfi); dfree(b->d_order); dfree(b->d_mg);
+    dfree(b->d_nsi); dfree(b->d_rfi); dfree(b->d_order); dfree(b->d_order_lanes); dfree(b->d_mg);
     dfree(b->d_guess); dfree(b->d_scale); dfree(b->d_cap); dfree(b->d_netcases); dfree(b->d_hist);
 #ifndef LICA_EMUL
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1146,6 +1251,10 @@ int lica_rhsnlhs(lica_batch *ctx, int64_t msub, double *w, double *p, double *ps
     if (rc) return rc;
     if (msub < 1 || msub > 3) return LICA_BAD_ARG;
     ctx->last_msub = (int)msub;
+    if (ctx->poisson_mode == LICA_POISSON_LEARNED) {
+        if (!ctx->net) return LICA_BAD_ARG;
+        if (lica_net_cast(ctx->net, (void *)ctx->stream)) return LICA_HIP_ERROR;
+    }
     const int kp = (ctx->poisson_mode >= LICA_POISSON_MG64) ? KL_POISSON_MG : KL_POISSON_DIRECT;
     const int seq[5] = {KL_CONT_ENERGY, KL_TEMP_GRAD, KL_MOMENTUM, kp, KL_UCALC};
     for (int j = 0; j < 5 && !rc; j++) {

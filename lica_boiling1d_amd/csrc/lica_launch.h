@@ -45,4 +45,7 @@ int64_t lica_net_window();   // cells of the corrected window around the interfa
 // s_b of the base (written by the warm-start base kernel), [0] = window
 // start and [4] = alpha (written here), [6] = the multigrid start verdict
 // on the base (cleared here for the cases whose guess moves)
-int lica_net_guess(lica_net *t, const NetCase *dcases, int ncase, double *dinfo, double rtol, void *stream);
+// (order: the cases of the launch, ncase of them; null for cases 0..ncase-1)
+int lica_net_guess(lica_net *t, const NetCase *dcases, const int *order, int ncase, double *dinfo, double rtol,
+                   void *stream);
+int lica_net_cast(lica_net *t, void *stream);   // BF16 weights from the FP32 masters

@@ -451,16 +451,17 @@ __device__ __forceinline__ void net_case_layer(__bf16 *xs, __bf16 *ws, float *ys
 // weights, FP32 sums in the same order).  info[LICA_WARM_INFO i]: window
 // start (written), backward error, s_r, s_b (read), alpha (written), the
 // multigrid start verdict (cleared when the guess moves).
-__global__ void __launch_bounds__(256) k_net_case(const NetCase *__restrict__ cases, NetParams prm,
-                                                  double *__restrict__ info, double rtol)
+__global__ void __launch_bounds__(256) k_net_case(const NetCase *__restrict__ cases, const int *__restrict__ order,
+                                                  NetParams prm, double *__restrict__ info, double rtol)
 {
+    const int ci = order ? order[blockIdx.x] : (int)blockIdx.x;   // the case
     constexpr int ROWS = NET_TILE + NET_TAPS - 1;
     __shared__ __attribute__((aligned(16))) __bf16 xs[ROWS * NET_C];
     __shared__ __attribute__((aligned(16))) __bf16 ws[NET_TGRP * NET_C * NET_C];
     __shared__ float ys[NET_TILE];
-    const NetCase cs = cases[blockIdx.x];
+    const NetCase cs = cases[ci];
     const int tid = threadIdx.x;
-    double *inf = info + (size_t)LICA_WARM_INFO * blockIdx.x;
+    double *inf = info + (size_t)LICA_WARM_INFO * ci;
     const double berr = inf[1], sr = inf[2], sb = inf[3];
     const int k0 = net_k0(cs), n = cs.n, w = cs.win;
     if (tid == 0) { inf[0] = (double)k0; inf[4] = 0.0; }
@@ -914,16 +915,24 @@ int lica_net_train(lica_net *t, int64_t iters, int64_t batch, double lr, uint64_
 // warm start inside a batch: the network's correction of every case's
 // guess, one workgroup per case (called by the batch driver before the
 // multigrid kernel of a substep; no allocation, so it can be graph-captured)
-int lica_net_guess(lica_net *t, const NetCase *dcases, int ncase, double *dinfo, double rtol, void *stream)
+int lica_net_guess(lica_net *t, const NetCase *dcases, const int *order, int ncase, double *dinfo, double rtol,
+                   void *stream)
 {
     NetParams prm;
     for (int l = 0; l < NET_LAYERS; l++) { prm.w[l] = t->wb[l]; prm.b[l] = t->b[l]; }
-    // the BF16 copies follow the masters (one launch for all layers)
+    hipLaunchKernelGGL(k_net_case, dim3(ncase), dim3(256), 0, (hipStream_t)stream, dcases, order, prm, dinfo, rtol);
+    return hip_ok(hipGetLastError());
+}
+
+// the BF16 copies of the weights from the FP32 masters, one launch for all
+// layers (the batch driver calls it once per step, before the step's
+// launches, so that every set of cases reads the same copies)
+int lica_net_cast(lica_net *t, void *stream)
+{
     NetCast nc;
     long long tot = 0;
     for (int l = 0; l < NET_LAYERS; l++) { nc.src[l] = t->w[l]; nc.dst[l] = t->wb[l]; nc.end[l] = (tot += (long long)wsize(l)); }
     hipLaunchKernelGGL(k_net_cast_all, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, (hipStream_t)stream, nc);
-    hipLaunchKernelGGL(k_net_case, dim3(ncase), dim3(256), 0, (hipStream_t)stream, dcases, prm, dinfo, rtol);
     return hip_ok(hipGetLastError());
 }
 

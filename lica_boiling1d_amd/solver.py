@@ -146,6 +146,10 @@ class Batch(_Handle):
         """Keep the POISSON system of substep `substep` (1..3) of every step."""
         _check(self._lib.lica_batch_set_capture(self._h, int(substep) if on else 0), "set_capture")
 
+    def set_lanes(self, lanes=2):
+        """Run the cases as two sets on two streams at once (1: one set)."""
+        _check(self._lib.lica_batch_set_lanes(self._h, int(lanes)), "set_lanes")
+
     def warm_info(self):
         """Per case [first row of the network window, backward error of the
         warm-start base, s_r, s_b, alpha, start-test backward error, 1 if the

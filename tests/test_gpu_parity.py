@@ -85,6 +85,34 @@ def test_graph_replay_equals_stream_launches():
             assert np.array_equal(a.scalar_array(i), g.scalar_array(i))
 
 
+@pytest.mark.parametrize("mode", [L.POISSON_THOMAS, L.POISSON_PROJECTED])
+def test_two_lanes_equal_one(mode):
+    """The cases run as two sets on two streams (stream launches, graph
+    replay and timing mode) give the single sequence's fields bit for bit."""
+    cases = L.batched_cases(6, seed=5, nmin=128, nmax=512)
+    _say(f"two lanes vs one, 6 cases, mode {mode}, 2 x 4 steps")
+    runs = []
+    for lanes, graph, timing in ((1, False, False), (2, False, False), (2, True, False), (2, False, True)):
+        b = L.Batch(cases)
+        b.set_poisson(mode, 1e-10, 100)
+        b.set_lanes(lanes)
+        b.use_graph(graph)
+        b.set_timing(timing)
+        runs.append(b)
+    for _ in range(2):
+        for b in runs:
+            b.run(4)
+        for i in range(len(cases)):
+            for f in ("W", "P", "T", "PSIF", "PHI"):
+                ref = runs[0].field(f, i)
+                for b in runs[1:]:
+                    assert np.array_equal(ref, b.field(f, i)), (f, i)
+            for b in runs[1:]:
+                assert np.array_equal(runs[0].scalar_array(i), b.scalar_array(i)), i
+    for b in runs:
+        b.close()
+
+
 def test_block_size_does_not_change_results():
     case = L.deck_case(300, 2)
     _say("block sizes 64/256/1024, 10 steps")
