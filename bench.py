@@ -108,7 +108,7 @@ def cpu_baseline(cases, seconds=15.0):
 
 
 def lanes_of(cases, args):
-    """Case sets run side by side on two streams (2) or one sequence (1)."""
+    """Case sets run side by side on as many streams (2..4) or one sequence (1)."""
     return args.lanes if args.lanes else (2 if len(cases) >= 256 else 1)
 
 
@@ -177,8 +177,8 @@ def roofline(L, cases, mode, args, res):
     tim = dict(res["timings"])
     if mode >= L.POISSON_MG64:
         tim["poisson"] = tim.get("poisson_mg", 0.0)
-    # with two lanes every kernel is launched twice per substep, each launch
-    # on half of the cases: ms per launch and bytes per launch both halve
+    # with L lanes every kernel is launched L times per substep, each launch
+    # on 1/L of the cases: ms per launch and bytes per launch both divide by L
     nla = lanes_of(cases, args)
     per = {k: tim.get(k, 0.0) / (args.steps * nl * nla) for k, nl in launches.items()}   # ms per launch
     # the warm-start slot (projection, network) is reported, not rated
@@ -202,7 +202,7 @@ def main():
     ap.add_argument("--mode", choices=["learned", "projected", "direct", "thomas", "mg64", "mixed"], default="learned")
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--threads", type=int, default=256)
-    ap.add_argument("--lanes", type=int, default=0, help="case sets on two streams (2) or one (1); 0: 2 for >= 256 cases")
+    ap.add_argument("--lanes", type=int, default=0, help="case sets on as many streams (1..4); 0: 2 for >= 256 cases")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--train-cases", type=int, default=4096)
     ap.add_argument("--train-steps", type=int, default=25)

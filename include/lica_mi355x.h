@@ -126,7 +126,7 @@ int lica_batch_status(lica_batch *b, int64_t *first_bad_inst);
 int lica_batch_get_progress(lica_batch *b, int64_t *status_out, int64_t *steps_out);
 int64_t lica_batch_size(lica_batch *b);
 /* GPU time of the last lica_batch_run, per kernel class (ms), summed over
- * the launches (two per kernel and substep with two lanes); returns count */
+ * the launches (L per kernel and substep with L lanes); returns count */
 int lica_batch_last_timings(lica_batch *b, double *ms_out, int64_t cap);
 void lica_batch_destroy(lica_batch *b);
 
@@ -176,12 +176,12 @@ int lica_net_train(lica_net *net, int64_t iters, int64_t batch, double lr, uint6
  * workgroup keeps its coarse levels in (-1: 60 KB for batches of >= 512
  * cases, else the device limit); levels beyond the cap use global memory */
 int lica_batch_set_mg_lds(lica_batch *b, int64_t bytes);
-/* No counterpart in the reference (scheduling): 2 runs the batch's cases as
- * two sets (alternate cases of the largest-first order) whose step launch
- * sequences go to two streams at once, so that one set's latency-bound
- * kernels share the CUs with the other's bandwidth-bound ones; 1 runs one
- * sequence.  Default: 2 for batches of 256 cases or more.  Results do not
- * change (cases are independent)                                          */
+/* No counterpart in the reference (scheduling): L = 2..4 runs the batch's
+ * cases as L sets (every L-th case of the largest-first order) whose step
+ * launch sequences go to L streams at once, so that one set's latency-bound
+ * kernels share the CUs with another's bandwidth-bound ones; 1 runs one
+ * sequence.  L is capped at the number of cases.  Default: 2 for batches of
+ * 256 cases or more.  Results do not change (cases are independent)      */
 int lica_batch_set_lanes(lica_batch *b, int64_t lanes);
 /* the network a batch uses in LICA_POISSON_LEARNED (not owned by the batch) */
 int lica_batch_set_net(lica_batch *b, lica_net *net);

@@ -353,6 +353,19 @@ struct LineReader {
     }
 };
 
+// most case sets a batch's step can be split into (one stream each)
+constexpr int LICA_MAX_LANES = 4;
+
+// the split over L lanes: lane l holds positions l, l + L, l + 2L, ... of the
+// largest-first order (n >= L cases), stored lane after lane
+int lane_count(int64_t n, int L, int l) { return (int)((n - l + L - 1) / L); }
+int lane_first(int64_t n, int L, int l)
+{
+    int s = 0;
+    for (int j = 0; j < l; j++) s += lane_count(n, L, j);
+    return s;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------
@@ -390,21 +403,21 @@ struct lica_batch {
     int graph_capture = 0;
     const lica_net *graph_net = nullptr;
     long long graph_mg_lds = -1;
-    std::vector<hipEvent_t> ev[2];     // timing events of the lanes
+    std::vector<hipEvent_t> ev[LICA_MAX_LANES];     // timing events of the lanes
 #endif
-    // Lanes: the cases split into two sets (alternate cases of the
+    // Lanes: the cases split into L sets (every L-th case of the
     // largest-first order) whose steps run as independent launch sequences
-    // on two streams, so that a latency-bound kernel of one set shares the
-    // CUs with a bandwidth-bound kernel of the other.  In timing mode each
-    // lane records its own events (a kernel's time is then its duration
-    // while the other lane runs beside it).
+    // on L streams, so that a latency-bound kernel of one set shares the
+    // CUs with a bandwidth-bound kernel of another and one set's tail of
+    // workgroups overlaps the next set's launch.  In timing mode each lane
+    // records its own events (a kernel's time is then its duration while
+    // the other lanes run beside it).
     int lanes = 1;
     int cur_lane = -1;                 // set being enqueued (-1: all cases)
-    int lane_lo[2] = {0, 0}, lane_hi[2] = {0, 0};
-    int *d_order_lanes = nullptr;      // lane-major workgroup order
-    hipStream_t lane_stream[2] = {nullptr, nullptr};
+    int *d_order_lanes = nullptr;      // per lane count L = 1..LICA_MAX_LANES: that split's lane-major order
+    hipStream_t lane_stream[LICA_MAX_LANES] = {};
 #ifndef LICA_EMUL
-    hipEvent_t fork = nullptr, join = nullptr;
+    hipEvent_t fork = nullptr, join[LICA_MAX_LANES] = {};
 #endif
     double timings[KL_COUNT] = {0};
     double wall_ms = 0.0;
@@ -472,12 +485,18 @@ int block_threads(const lica_batch *b)
 
 hipStream_t cur_stream(const lica_batch *b) { return b->cur_lane < 0 ? b->stream : b->lane_stream[b->cur_lane]; }
 
+// workgroup order of the lane being enqueued
+const int *lane_order(const lica_batch *b)
+{
+    return b->d_order_lanes + (size_t)(b->lanes - 1) * (size_t)b->ninst + lane_first(b->ninst, b->lanes, b->cur_lane);
+}
+
 int launch(lica_batch *b, int which, int msub)
 {
     BatchDev bd = make_bd(b, msub);
     if (b->cur_lane >= 0) {
-        bd.order = b->d_order_lanes + b->lane_lo[b->cur_lane];
-        bd.ninst = b->lane_hi[b->cur_lane] - b->lane_lo[b->cur_lane];
+        bd.order = lane_order(b);
+        bd.ninst = lane_count(b->ninst, b->lanes, b->cur_lane);
     }
     int rc = lica_launch(which, bd, block_threads(b), (void *)cur_stream(b));
     return rc == 0 ? 0 : LICA_HIP_ERROR;
@@ -493,8 +512,8 @@ int pre_poisson(lica_batch *b, int msub)
     if (!rc && b->capture == msub) rc = launch(b, KL_CAPTURE_PRE, msub);
     if (!rc && b->poisson_mode == LICA_POISSON_LEARNED) {
         if (!b->net) return LICA_BAD_ARG;
-        const int *ord = (b->cur_lane >= 0) ? b->d_order_lanes + b->lane_lo[b->cur_lane] : nullptr;
-        const int cnt = (b->cur_lane >= 0) ? b->lane_hi[b->cur_lane] - b->lane_lo[b->cur_lane] : (int)b->ninst;
+        const int *ord = (b->cur_lane >= 0) ? lane_order(b) : nullptr;
+        const int cnt = (b->cur_lane >= 0) ? lane_count(b->ninst, b->lanes, b->cur_lane) : (int)b->ninst;
         if (lica_net_guess(b->net, b->d_netcases, ord, cnt, b->d_scale, b->rtol, (void *)cur_stream(b)))
             rc = LICA_HIP_ERROR;
     }
@@ -503,8 +522,29 @@ int pre_poisson(lica_batch *b, int msub)
 
 int enqueue_step_cases(lica_batch *b);
 
-// the launches of one time step (1d.f:196-295); with two lanes the two case
-// sets run their own sequences on two streams, forked from and joined back
+#ifndef LICA_EMUL
+// lanes 1.. start after what the batch's stream holds, and the batch's stream
+// goes on after every lane's launches
+int fork_lanes(lica_batch *b)
+{
+    if (hipEventRecord(b->fork, b->stream) != hipSuccess) return 1;
+    for (int l = 1; l < b->lanes; l++)
+        if (hipStreamWaitEvent(b->lane_stream[l], b->fork, 0) != hipSuccess) return 1;
+    return 0;
+}
+
+int join_lanes(lica_batch *b)
+{
+    for (int l = 1; l < b->lanes; l++)
+        if (hipEventRecord(b->join[l], b->lane_stream[l]) != hipSuccess ||
+            hipStreamWaitEvent(b->stream, b->join[l], 0) != hipSuccess)
+            return 1;
+    return 0;
+}
+#endif
+
+// the launches of one time step (1d.f:196-295); with L > 1 lanes the L case
+// sets run their own sequences on L streams, forked from and joined back
 // into the batch's stream
 int enqueue_step(lica_batch *b)
 {
@@ -513,19 +553,15 @@ int enqueue_step(lica_batch *b)
         if (lica_net_cast(b->net, (void *)b->stream)) return LICA_HIP_ERROR;
     }
 #ifndef LICA_EMUL
-    if (b->lanes == 2 && b->cur_lane < 0) {
-        if (hipEventRecord(b->fork, b->stream) != hipSuccess ||
-            hipStreamWaitEvent(b->lane_stream[1], b->fork, 0) != hipSuccess)
-            return LICA_HIP_ERROR;
+    if (b->lanes > 1 && b->cur_lane < 0) {
+        if (fork_lanes(b)) return LICA_HIP_ERROR;
         int rc = 0;
-        for (int l = 0; l < 2 && !rc; l++) {
+        for (int l = 0; l < b->lanes && !rc; l++) {
             b->cur_lane = l;
             rc = enqueue_step_cases(b);
         }
         b->cur_lane = -1;
-        if (hipEventRecord(b->join, b->lane_stream[1]) != hipSuccess ||
-            hipStreamWaitEvent(b->stream, b->join, 0) != hipSuccess)
-            rc = rc ? rc : LICA_HIP_ERROR;
+        if (join_lanes(b)) rc = rc ? rc : LICA_HIP_ERROR;
         return rc;
     }
 #endif
@@ -771,9 +807,11 @@ int lica_batch_create(int64_t ninst, const lica_case_desc *cases, lica_batch **o
 #ifndef LICA_EMUL
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) { delete b; return LICA_HIP_ERROR; }
     b->lane_stream[0] = b->stream;
-    if (hipStreamCreateWithFlags(&b->lane_stream[1], hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess) {
+    bool ok = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
+    for (int l = 1; l < LICA_MAX_LANES && ok; l++)
+        ok = hipStreamCreateWithFlags(&b->lane_stream[l], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&b->join[l], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
         lica_batch_destroy(b);
         return LICA_HIP_ERROR;
     }
@@ -796,13 +834,12 @@ int lica_batch_create(int64_t ninst, const lica_case_desc *cases, lica_batch **o
     std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return b->par[(size_t)a].n > b->par[(size_t)c].n; });
     rc = h2d(b->d_order, order.data(), sizeof(int) * order.size(), b->stream);
     {
-        // lane 0: cases 0, 2, 4, ... of the order, lane 1: cases 1, 3, 5, ...
+        // the splits over 1..LICA_MAX_LANES lanes, one after the other
         std::vector<int> lo;
-        for (int pass = 0; pass < 2; pass++)
-            for (size_t j = (size_t)pass; j < order.size(); j += 2) lo.push_back(order[j]);
-        b->lane_lo[0] = 0; b->lane_hi[0] = (int)((order.size() + 1) / 2);
-        b->lane_lo[1] = b->lane_hi[0]; b->lane_hi[1] = (int)order.size();
-    b->lanes = (ninst >= 256) ? 2 : 1;
+        for (int L = 1; L <= LICA_MAX_LANES; L++)
+            for (int l = 0; l < L; l++)
+                for (size_t j = (size_t)l; j < order.size(); j += (size_t)L) lo.push_back(order[j]);
+        b->lanes = (ninst >= 256) ? 2 : 1;
         rc = rc ? rc : dmalloc((void **)&b->d_order_lanes, sizeof(int) * lo.size());
         rc = rc ? rc : h2d(b->d_order_lanes, lo.data(), sizeof(int) * lo.size(), b->stream);
     }
@@ -848,18 +885,16 @@ int lica_batch_run(lica_batch *b, int64_t nsteps)
     for (int64_t s = 0; s < nsteps && !rc; s++) rc = enqueue_step(b);
 #else
     if (b->timing) {
-        const int nl = (b->lanes == 2) ? 2 : 1;
+        const int nl = b->lanes;
         for (int64_t s = 0; s < nsteps && !rc; s++) {
-            std::vector<int> kinds[2];
+            std::vector<int> kinds[LICA_MAX_LANES];
             if (b->poisson_mode == LICA_POISSON_LEARNED) {
                 if (!b->net) return LICA_BAD_ARG;
                 if (lica_net_cast(b->net, (void *)b->stream)) return LICA_HIP_ERROR;
             }
-            if (nl == 2 && (hipEventRecord(b->fork, b->stream) != hipSuccess ||
-                            hipStreamWaitEvent(b->lane_stream[1], b->fork, 0) != hipSuccess))
-                return LICA_HIP_ERROR;
+            if (nl > 1 && fork_lanes(b)) return LICA_HIP_ERROR;
             for (int l = 0; l < nl && !rc; l++) {
-                b->cur_lane = (nl == 2) ? l : -1;
+                b->cur_lane = (nl > 1) ? l : -1;
                 if (b->ev[l].empty()) {
                     hipEvent_t e;
                     if (hipEventCreate(&e) != hipSuccess) { rc = LICA_HIP_ERROR; break; }
@@ -869,9 +904,7 @@ int lica_batch_run(lica_batch *b, int64_t nsteps)
                 rc = enqueue_step_timed(b, b->ev[l], kinds[l]);
             }
             b->cur_lane = -1;
-            if (nl == 2 && (hipEventRecord(b->join, b->lane_stream[1]) != hipSuccess ||
-                            hipStreamWaitEvent(b->stream, b->join, 0) != hipSuccess))
-                rc = rc ? rc : LICA_HIP_ERROR;
+            if (nl > 1 && join_lanes(b)) rc = rc ? rc : LICA_HIP_ERROR;
             if (rc) break;
             for (int l = 0; l < nl; l++) {
                 if (hipEventSynchronize(b->ev[l][kinds[l].size()]) != hipSuccess) return LICA_HIP_ERROR;
@@ -939,8 +972,8 @@ int lica_batch_set_lanes(lica_batch *b, int64_t lanes)
 {
     int rc = check_batch(b);
     if (rc) return rc;
-    if (lanes != 1 && lanes != 2) return LICA_BAD_ARG;
-    b->lanes = (b->ninst >= 2) ? (int)lanes : 1;
+    if (lanes < 1 || lanes > LICA_MAX_LANES) return LICA_BAD_ARG;
+    b->lanes = (int)std::min<int64_t>(lanes, b->ninst);
     return LICA_OK;
 }
 
@@ -1118,13 +1151,16 @@ void lica_batch_destroy(lica_batch *b)
     if (!b) return;
 #ifndef LICA_EMUL
     if (b->stream) (void)hipStreamSynchronize(b->stream);
-    if (b->lane_stream[1]) (void)hipStreamSynchronize(b->lane_stream[1]);
+    for (int l = 1; l < LICA_MAX_LANES; l++)
+        if (b->lane_stream[l]) (void)hipStreamSynchronize(b->lane_stream[l]);
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
-    for (int l = 0; l < 2; l++)
+    for (int l = 0; l < LICA_MAX_LANES; l++)
         for (hipEvent_t e : b->ev[l]) (void)hipEventDestroy(e);
     if (b->fork) (void)hipEventDestroy(b->fork);
-    if (b->join) (void)hipEventDestroy(b->join);
-    if (b->lane_stream[1]) (void)hipStreamDestroy(b->lane_stream[1]);
+    for (int l = 1; l < LICA_MAX_LANES; l++) {
+        if (b->join[l]) (void)hipEventDestroy(b->join[l]);
+        if (b->lane_stream[l]) (void)hipStreamDestroy(b->lane_stream[l]);
+    }
 #endif
     dfree(b->d_par); dfree(b->d_st); dfree(b->d_ns); dfree(b->d_rf);
     dfree(b->d_nsi); dfree(b->d_rfi); dfree(b->d_order); dfree(b->d_order_lanes); dfree(b->d_mg);

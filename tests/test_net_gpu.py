@@ -243,6 +243,38 @@ def test_add_pairs_needs_capture_and_multigrid():
         net.add_pairs(b, 1, 64)
 
 
+def test_learned_two_lanes_equal_one():
+    """With the cases split into two or four sets on as many streams, the per-case
+    network kernel reads each lane's case order: the fields and the warm-start
+    info equal the single sequence's bit for bit (stream launches and graph
+    replay)."""
+    cases = L.batched_cases(6, seed=7, nmin=128, nmax=512)
+    _say("learned mode, two lanes vs one, 6 cases, 2 x 3 steps")
+    net = Net(seed=0)
+    runs = []
+    for lanes, graph in ((1, False), (2, False), (2, True), (4, False)):
+        b = L.Batch(cases)
+        b.set_net(net)
+        b.set_poisson(L.POISSON_LEARNED, 1e-10, 100)
+        b.set_lanes(lanes)
+        b.use_graph(graph)
+        runs.append(b)
+    for _ in range(2):
+        for b in runs:
+            b.run(3)
+        ref_info = runs[0].warm_info()
+        for b in runs[1:]:
+            assert np.array_equal(ref_info, b.warm_info())
+        for i in range(len(cases)):
+            for f in ("W", "P", "T", "PHI"):
+                ref = runs[0].field(f, i)
+                for b in runs[1:]:
+                    assert np.array_equal(ref, b.field(f, i)), (f, i)
+    for b in runs:
+        b.close()
+    net.close()
+
+
 def _cycles(b, steps):
     s0 = b.scalars()
     b.run(steps)
