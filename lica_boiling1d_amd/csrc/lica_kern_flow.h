@@ -240,6 +240,48 @@ struct WarmSh {
     double coef[MG_HIST + 1];
 };
 
+// Least squares over the vectors 0..m-1 from their normal equations G c = h:
+// Cholesky in vector order, dropping a vector whose pivot falls below
+// 1e-12 of its diagonal (nearly dependent on the ones kept before it); the
+// dropped ones get c = 0.  Every index is a compile-time constant (the
+// kept set is a mask), so G, L and the solves stay in registers.
+template <int MV>
+DEV void normal_solve(const double (&G)[MV][MV], const double (&h)[MV], int m, double (&c)[MV])
+{
+    double L[MV][MV];
+    bool keep[MV];
+#pragma unroll
+    for (int j = 0; j < MV; j++) {
+#pragma unroll
+        for (int i = 0; i < j; i++) {
+            double v = G[j][i];
+#pragma unroll
+            for (int u = 0; u < i; u++) v = (keep[u] && keep[i]) ? v - L[j][u] * L[i][u] : v;
+            L[j][i] = keep[i] ? v / L[i][i] : 0.0;
+        }
+        double d = G[j][j];
+#pragma unroll
+        for (int i = 0; i < j; i++) d = keep[i] ? d - L[j][i] * L[j][i] : d;
+        keep[j] = j < m && G[j][j] > 0.0 && d > 1.0e-12 * G[j][j];
+        L[j][j] = keep[j] ? sqrt(d) : 1.0;
+    }
+    double y[MV];
+#pragma unroll
+    for (int i = 0; i < MV; i++) {
+        double v = h[i];
+#pragma unroll
+        for (int u = 0; u < i; u++) v = (keep[u] && keep[i]) ? v - L[i][u] * y[u] : v;
+        y[i] = keep[i] ? v / L[i][i] : 0.0;
+    }
+#pragma unroll
+    for (int i = MV - 1; i >= 0; i--) {
+        double v = y[i];
+#pragma unroll
+        for (int u = i + 1; u < MV; u++) v = (keep[u] && keep[i]) ? v - L[u][i] * c[u] : v;
+        c[i] = keep[i] ? v / L[i][i] : 0.0;
+    }
+}
+
 DEV void body_warm_base(const BatchDev &bd, int inst, WarmSh &sh)
 {
     InstState &S = bd.st[inst];
@@ -295,35 +337,9 @@ DEV void body_warm_base(const BatchDev &bd, int inst, WarmSh &sh)
                 for (int j = i; j < MV; j++) { G[i][j] = sums[q]; G[j][i] = sums[q]; q++; }
             for (int j = 0; j < MV; j++) h[j] = sums[NG + j];
         }
-        double Lm[MV][MV] = {{0.0}}, y[MV] = {0.0}, c[MV] = {0.0};
-        int idx[MV], nk = 0;
-        for (int j = 0; j < m; j++) {
-            double row[MV] = {0.0};
-            for (int t = 0; t < nk; t++) {
-                double v = G[j][idx[t]];
-                for (int u = 0; u < t; u++) v = v - row[u] * Lm[t][u];
-                row[t] = v / Lm[t][t];
-            }
-            double d = G[j][j];
-            for (int t = 0; t < nk; t++) d = d - row[t] * row[t];
-            if (G[j][j] > 0.0 && d > 1.0e-12 * G[j][j]) {
-                for (int t = 0; t < nk; t++) Lm[nk][t] = row[t];
-                Lm[nk][nk] = sqrt(d);
-                idx[nk++] = j;
-            }
-        }
-        for (int t = 0; t < nk; t++) {
-            double v = h[idx[t]];
-            for (int u = 0; u < t; u++) v = v - Lm[t][u] * y[u];
-            y[t] = v / Lm[t][t];
-        }
-        for (int t = nk - 1; t >= 0; t--) {
-            double v = y[t];
-            for (int u = t + 1; u < nk; u++) v = v - Lm[u][t] * c[u];
-            c[t] = v / Lm[t][t];
-        }
-        for (int j = 0; j < MG_HIST + 1; j++) sh.coef[j] = 0.0;
-        for (int t = 0; t < nk; t++) sh.coef[idx[t]] = c[t];
+        double c[MV];
+        normal_solve<MV>(G, h, m, c);
+        for (int j = 0; j < MV; j++) sh.coef[j] = c[j];
     }
     SERIAL_END
     auto base = [&](int row) -> double {

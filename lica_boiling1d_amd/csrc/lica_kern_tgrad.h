@@ -13,18 +13,27 @@ static_assert(LICA_NN2 + 1 <= LICA_MLAYER, "layer order of the band entries");
 // B.tf0/B.mask_s and B.tf1/B.slot, free by then (TF_GRAD and the
 // reinitialisation are done), which keeps six workgroups on a CU
 // TEMP_EXTP's window of NS cells [nlo, nhi] under the band
+// The NS window of TEMP_EXTP, cells nlo..nhi (psitmp: CAL_PSI_TMP of cells
+// nlo-1 .. nhi+1).  Only t0 has LDS of its own; the other arrays use band
+// arrays that are dead while they are live, which keeps the workgroup
+// within 20 KB of LDS (eight workgroups per CU): nphi and psih live from the
+// window's set-up to the end of the extrapolation, in B.phs and B.mfd
+// (both written afresh by VOL_TOT_MFDOT before they are read); t1 in B.wf
+// (TEMP_GRAD has no use for it); psitmp, after TEMPASSIGN has read t0, in t0.
 struct ExtpWin {
     int nlo, nhi;
-    double t0[NSWMAX], t1[NSWMAX], nphi[NSWMAX], psih[NSWMAX];
-    double psitmp[NSWMAX];   // CAL_PSI_TMP of cells nlo-1 .. nhi+1
+    double *t0, *t1, *nphi, *psih, *psitmp;
 };
 
 struct TgradSh {
     BandSh B;
-    ExtpWin X;
+    int nlo, nhi;
+    double t0[NSWMAX];
     double xit[2], xhv[2];   // TEMP_EXTP iterations / halvings of phases 0, 1
     int xst[2];
 };
+static_assert(NSWMAX <= WMAX && NSWMAX <= LICA_MAXENT && NSWMAX <= WMAX + 8,
+              "the window arrays must fit the band arrays they use");
 
 // One point of TEMP_EXTP_RK3 (chphase.f:806-956): the upwind pseudo-time
 // update of cell value a with neighbours al/ar; the normal is fixed to +z
@@ -350,7 +359,6 @@ DEV void body_temp_grad(const BatchDev &bd, int inst, TgradSh &TS)
     InstState &S = bd.st[inst];
     if (S.status || S.m == 1) return;   // TEMP_GRAD runs from the second step (1d.f:3654-3667)
     BandSh &B = TS.B;
-    ExtpWin &X = TS.X;
     CaseP cp = case_ptrs(bd, inst);
     const InstParam &P = *cp.P;
     const int n = P.n, n3 = P.n3, n3f = P.n3f;
@@ -373,15 +381,16 @@ DEV void body_temp_grad(const BatchDev &bd, int inst, TgradSh &TS)
         SERIAL {
             int kmin = (int)kmn, kmax = (int)kmx;
             if (kmax < 0) { kmin = 1; kmax = 1; }
-            X.nlo = kmin - 3 < 0 ? 0 : kmin - 3;
-            X.nhi = kmax + 3 > n3 ? n3 : kmax + 3;
+            TS.nlo = kmin - 3 < 0 ? 0 : kmin - 3;
+            TS.nhi = kmax + 3 > n3 ? n3 : kmax + 3;
             // CAL_PSI_TMP's rows reach one cell beyond either end
-            if (X.nhi - X.nlo + 3 > NSWMAX) B.status = ST_WINDOW;
+            if (TS.nhi - TS.nlo + 3 > NSWMAX) B.status = ST_WINDOW;
         }
         SERIAL_END
     }
     if (B.status) { SERIAL { S.status = B.status; } SERIAL_END return; }
-    const int nlo = X.nlo, nhi = X.nhi;
+    const int nlo = TS.nlo, nhi = TS.nhi;
+    const ExtpWin X{nlo, nhi, TS.t0, B.wf, B.phs, B.mfd, TS.t0};
     PFOR(i, 0, WMAX) { B.phw[i] = 0.0; B.tf0[i] = 0.0; B.tf1[i] = 0.0; }
     PFOR_END
     PFOR(e, 0, B.off[LICA_MLAYER + 1]) { B.phw[B.ent_k[e] - B.wlo] = S.alphi[B.ent_n[e]][B.ent_q[e]]; }

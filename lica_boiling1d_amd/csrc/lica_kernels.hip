@@ -31,7 +31,7 @@ __global__ void __launch_bounds__(1024) k_cont_energy(BatchDev bd)
     tri_attach(sh);
     body_cont_energy(bd, bd.order[blockIdx.x], sh);
 }
-__global__ void __launch_bounds__(128) k_temp_grad(BatchDev bd)
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k_temp_grad(BatchDev bd)
 {
     __shared__ TgradSh sh;
     body_temp_grad(bd, bd.order[blockIdx.x], sh);
