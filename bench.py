@@ -20,8 +20,11 @@ from a disjoint batch (seed 20261203); the training is not timed.
 
 The JSON line also carries
   roofline      the dominant kernel's algorithmic bytes per launch over its
-                mean launch time (HIP events between the launches on the
-                library's stream, over the timed steps), against 8 TB/s;
+                mean launch time (HIP events between the launches on each
+                lane's stream, over the timed steps), against 8 TB/s; with
+                two lanes (the cases split into two sets whose launch
+                sequences run on two streams at once) a launch covers half
+                of the cases, and bytes and times are both per launch;
   cpu_baseline  the oracle (serial FP64 restatement, one core) on a bounded
                 sample of the same cases.
 ``--workload large`` / ``deck`` and ``--mode`` select the single large case,
