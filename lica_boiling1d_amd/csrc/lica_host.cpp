@@ -546,12 +546,18 @@ int join_lanes(lica_batch *b)
 // the launches of one time step (1d.f:196-295); with L > 1 lanes the L case
 // sets run their own sequences on L streams, forked from and joined back
 // into the batch's stream
+// the network's BF16 weights for the learned mode (they do not change while
+// a batch runs)
+int cast_net(lica_batch *b)
+{
+    if (b->poisson_mode != LICA_POISSON_LEARNED) return 0;
+    if (!b->net) return LICA_BAD_ARG;
+    return lica_net_cast(b->net, (void *)b->stream) ? LICA_HIP_ERROR : 0;
+}
+
 int enqueue_step(lica_batch *b)
 {
-    if (b->poisson_mode == LICA_POISSON_LEARNED) {
-        if (!b->net) return LICA_BAD_ARG;
-        if (lica_net_cast(b->net, (void *)b->stream)) return LICA_HIP_ERROR;
-    }
+    if (int rc = cast_net(b)) return rc;
 #ifndef LICA_EMUL
     if (b->lanes > 1 && b->cur_lane < 0) {
         if (fork_lanes(b)) return LICA_HIP_ERROR;
@@ -884,35 +890,37 @@ int lica_batch_run(lica_batch *b, int64_t nsteps)
 #ifdef LICA_EMUL
     for (int64_t s = 0; s < nsteps && !rc; s++) rc = enqueue_step(b);
 #else
-    if (b->timing) {
+    if (b->timing || !b->use_graph) {
+        // the lanes are forked once for the whole run: each lane's steps
+        // follow each other on its stream with no join in between (the cases
+        // are independent), so the lanes drift apart and one lane's launches
+        // fill the other's idle CUs.  Timing events are read after the run.
         const int nl = b->lanes;
-        for (int64_t s = 0; s < nsteps && !rc; s++) {
-            std::vector<int> kinds[LICA_MAX_LANES];
-            if (b->poisson_mode == LICA_POISSON_LEARNED) {
-                if (!b->net) return LICA_BAD_ARG;
-                if (lica_net_cast(b->net, (void *)b->stream)) return LICA_HIP_ERROR;
+        std::vector<int> kinds[LICA_MAX_LANES];
+        rc = cast_net(b);
+        if (!rc && nl > 1 && fork_lanes(b)) rc = LICA_HIP_ERROR;
+        for (int l = 0; l < nl && b->timing && !rc; l++) {
+            b->cur_lane = (nl > 1) ? l : -1;
+            if (b->ev[l].empty()) {
+                hipEvent_t e;
+                if (hipEventCreate(&e) != hipSuccess) { rc = LICA_HIP_ERROR; break; }
+                b->ev[l].push_back(e);
             }
-            if (nl > 1 && fork_lanes(b)) return LICA_HIP_ERROR;
+            if (hipEventRecord(b->ev[l][0], cur_stream(b)) != hipSuccess) rc = LICA_HIP_ERROR;
+        }
+        for (int64_t s = 0; s < nsteps && !rc; s++)
             for (int l = 0; l < nl && !rc; l++) {
                 b->cur_lane = (nl > 1) ? l : -1;
-                if (b->ev[l].empty()) {
-                    hipEvent_t e;
-                    if (hipEventCreate(&e) != hipSuccess) { rc = LICA_HIP_ERROR; break; }
-                    b->ev[l].push_back(e);
-                }
-                if (hipEventRecord(b->ev[l][0], cur_stream(b)) != hipSuccess) { rc = LICA_HIP_ERROR; break; }
-                rc = enqueue_step_timed(b, b->ev[l], kinds[l]);
+                rc = b->timing ? enqueue_step_timed(b, b->ev[l], kinds[l]) : enqueue_step_cases(b);
             }
-            b->cur_lane = -1;
-            if (nl > 1 && join_lanes(b)) rc = rc ? rc : LICA_HIP_ERROR;
-            if (rc) break;
-            for (int l = 0; l < nl; l++) {
-                if (hipEventSynchronize(b->ev[l][kinds[l].size()]) != hipSuccess) return LICA_HIP_ERROR;
-                for (size_t j = 0; j < kinds[l].size(); j++) {
-                    float ms = 0.0f;
-                    if (hipEventElapsedTime(&ms, b->ev[l][j], b->ev[l][j + 1]) != hipSuccess) return LICA_HIP_ERROR;
-                    b->timings[kinds[l][j]] += (double)ms;
-                }
+        b->cur_lane = -1;
+        if (nl > 1 && join_lanes(b)) rc = rc ? rc : LICA_HIP_ERROR;
+        for (int l = 0; l < nl && b->timing && !rc; l++) {
+            if (hipEventSynchronize(b->ev[l][kinds[l].size()]) != hipSuccess) return LICA_HIP_ERROR;
+            for (size_t j = 0; j < kinds[l].size(); j++) {
+                float ms = 0.0f;
+                if (hipEventElapsedTime(&ms, b->ev[l][j], b->ev[l][j + 1]) != hipSuccess) return LICA_HIP_ERROR;
+                b->timings[kinds[l][j]] += (double)ms;
             }
         }
     } else if (b->use_graph) {
@@ -922,8 +930,6 @@ int lica_batch_run(lica_batch *b, int64_t nsteps)
             rc = build_graph(b);
         for (int64_t s = 0; s < nsteps && !rc; s++)
             if (hipGraphLaunch(b->graph, b->stream) != hipSuccess) rc = LICA_HIP_ERROR;
-    } else {
-        for (int64_t s = 0; s < nsteps && !rc; s++) rc = enqueue_step(b);
     }
 #endif
     if (!rc) rc = dsync(b->stream);
