@@ -22,9 +22,10 @@ The JSON line also carries
   roofline      the dominant kernel's algorithmic bytes per launch over its
                 mean launch time (HIP events between the launches on each
                 lane's stream, over the timed steps), against 8 TB/s; with
-                two lanes (the cases split into two sets whose launch
-                sequences run on two streams at once) a launch covers half
-                of the cases, and bytes and times are both per launch;
+                four lanes (the cases split into four sets whose launch
+                sequences run on four streams at once) a launch covers a
+                quarter of the cases, and bytes and times are both per
+                launch;
   cpu_baseline  the oracle (serial FP64 restatement, one core) on a bounded
                 sample of the same cases.
 ``--workload large`` / ``deck`` and ``--mode`` select the single large case,
@@ -38,6 +39,15 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+
+# Lanes run on streams of their own, and streams beyond the process's
+# hardware queues share a queue, which serialises them; HIP reads the queue
+# count once, when it starts, and defaults to 4.  Four lanes, the batch's
+# own stream and the host-side copies need more than that: ask for 8 before
+# anything starts HIP.
+LANES_BATCH = 4
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 HBM_PEAK_GBS = 8000.0
 
@@ -112,7 +122,7 @@ def cpu_baseline(cases, seconds=15.0):
 
 def lanes_of(cases, args):
     """Case sets run side by side on as many streams (2..4) or one sequence (1)."""
-    return args.lanes if args.lanes else (2 if len(cases) >= 256 else 1)
+    return args.lanes if args.lanes else (LANES_BATCH if len(cases) >= 256 else 1)
 
 
 def timed_run(L, cases, mode, args, net=None):
@@ -205,7 +215,7 @@ def main():
     ap.add_argument("--mode", choices=["learned", "projected", "direct", "thomas", "mg64", "mixed"], default="learned")
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--threads", type=int, default=256)
-    ap.add_argument("--lanes", type=int, default=0, help="case sets on as many streams (1..4); 0: 2 for >= 256 cases")
+    ap.add_argument("--lanes", type=int, default=0, help="case sets on as many streams (1..4); 0: 4 for >= 256 cases")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--train-cases", type=int, default=4096)
     ap.add_argument("--train-steps", type=int, default=25)

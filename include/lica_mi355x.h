@@ -181,7 +181,9 @@ int lica_batch_set_mg_lds(lica_batch *b, int64_t bytes);
  * launch sequences go to L streams at once, so that one set's latency-bound
  * kernels share the CUs with another's bandwidth-bound ones; 1 runs one
  * sequence.  L is capped at the number of cases.  Default: 2 for batches of
- * 256 cases or more.  Results do not change (cases are independent)      */
+ * 256 cases or more.  Results do not change (cases are independent).  Each
+ * lane needs a hardware queue of its own (GPU_MAX_HW_QUEUES, 4 by default in
+ * HIP: set 8 before HIP starts for 3 or 4 lanes)                          */
 int lica_batch_set_lanes(lica_batch *b, int64_t lanes);
 /* the network a batch uses in LICA_POISSON_LEARNED (not owned by the batch) */
 int lica_batch_set_net(lica_batch *b, lica_net *net);

@@ -249,7 +249,7 @@ def test_learned_two_lanes_equal_one():
     info equal the single sequence's bit for bit (stream launches and graph
     replay)."""
     cases = L.batched_cases(6, seed=7, nmin=128, nmax=512)
-    _say("learned mode, two lanes vs one, 6 cases, 2 x 3 steps")
+    _say("learned mode, 2 and 4 lanes vs one, 6 cases, 2 x 3 steps")
     net = Net(seed=0)
     runs = []
     for lanes, graph in ((1, False), (2, False), (2, True), (4, False)):
