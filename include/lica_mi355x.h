@@ -176,14 +176,14 @@ int lica_net_train(lica_net *net, int64_t iters, int64_t batch, double lr, uint6
  * workgroup keeps its coarse levels in (-1: 60 KB for batches of >= 512
  * cases, else the device limit); levels beyond the cap use global memory */
 int lica_batch_set_mg_lds(lica_batch *b, int64_t bytes);
-/* No counterpart in the reference (scheduling): L = 2..4 runs the batch's
+/* No counterpart in the reference (scheduling): L = 2..8 runs the batch's
  * cases as L sets (every L-th case of the largest-first order) whose step
  * launch sequences go to L streams at once, so that one set's latency-bound
  * kernels share the CUs with another's bandwidth-bound ones; 1 runs one
  * sequence.  L is capped at the number of cases.  Default: 2 for batches of
  * 256 cases or more.  Results do not change (cases are independent).  Each
  * lane needs a hardware queue of its own (GPU_MAX_HW_QUEUES, 4 by default in
- * HIP: set 8 before HIP starts for 3 or 4 lanes)                          */
+ * HIP: set it to at least L + 4 before HIP starts for more than 2 lanes)  */
 int lica_batch_set_lanes(lica_batch *b, int64_t lanes);
 /* the network a batch uses in LICA_POISSON_LEARNED (not owned by the batch) */
 int lica_batch_set_net(lica_batch *b, lica_net *net);

@@ -354,7 +354,7 @@ struct LineReader {
 };
 
 // most case sets a batch's step can be split into (one stream each)
-constexpr int LICA_MAX_LANES = 4;
+constexpr int LICA_MAX_LANES = 8;
 
 // the split over L lanes: lane l holds positions l, l + L, l + 2L, ... of the
 // largest-first order (n >= L cases), stored lane after lane
@@ -521,6 +521,22 @@ int pre_poisson(lica_batch *b, int msub)
 }
 
 int enqueue_step_cases(lica_batch *b);
+
+// streams and join events of lanes 1..L-1, created when first needed
+int ensure_lanes(lica_batch *b, int L)
+{
+#ifndef LICA_EMUL
+    for (int l = 1; l < L; l++) {
+        if (!b->lane_stream[l] && hipStreamCreateWithFlags(&b->lane_stream[l], hipStreamNonBlocking) != hipSuccess)
+            return LICA_HIP_ERROR;
+        if (!b->join[l] && hipEventCreateWithFlags(&b->join[l], hipEventDisableTiming) != hipSuccess)
+            return LICA_HIP_ERROR;
+    }
+#else
+    (void)b; (void)L;
+#endif
+    return 0;
+}
 
 #ifndef LICA_EMUL
 // lanes 1.. start after what the batch's stream holds, and the batch's stream
@@ -813,11 +829,7 @@ int lica_batch_create(int64_t ninst, const lica_case_desc *cases, lica_batch **o
 #ifndef LICA_EMUL
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) { delete b; return LICA_HIP_ERROR; }
     b->lane_stream[0] = b->stream;
-    bool ok = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
-    for (int l = 1; l < LICA_MAX_LANES && ok; l++)
-        ok = hipStreamCreateWithFlags(&b->lane_stream[l], hipStreamNonBlocking) == hipSuccess &&
-             hipEventCreateWithFlags(&b->join[l], hipEventDisableTiming) == hipSuccess;
-    if (!ok) {
+    if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) {
         lica_batch_destroy(b);
         return LICA_HIP_ERROR;
     }
@@ -846,6 +858,7 @@ int lica_batch_create(int64_t ninst, const lica_case_desc *cases, lica_batch **o
             for (int l = 0; l < L; l++)
                 for (size_t j = (size_t)l; j < order.size(); j += (size_t)L) lo.push_back(order[j]);
         b->lanes = (ninst >= 256) ? 2 : 1;
+        rc = rc ? rc : ensure_lanes(b, b->lanes);
         rc = rc ? rc : dmalloc((void **)&b->d_order_lanes, sizeof(int) * lo.size());
         rc = rc ? rc : h2d(b->d_order_lanes, lo.data(), sizeof(int) * lo.size(), b->stream);
     }
@@ -979,7 +992,9 @@ int lica_batch_set_lanes(lica_batch *b, int64_t lanes)
     int rc = check_batch(b);
     if (rc) return rc;
     if (lanes < 1 || lanes > LICA_MAX_LANES) return LICA_BAD_ARG;
-    b->lanes = (int)std::min<int64_t>(lanes, b->ninst);
+    const int L = (int)std::min<int64_t>(lanes, b->ninst);
+    if (int e = ensure_lanes(b, L)) return e;
+    b->lanes = L;
     return LICA_OK;
 }
 

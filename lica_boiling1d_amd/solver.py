@@ -147,7 +147,7 @@ class Batch(_Handle):
         _check(self._lib.lica_batch_set_capture(self._h, int(substep) if on else 0), "set_capture")
 
     def set_lanes(self, lanes=2):
-        """Run the cases as `lanes` sets (1..4) on as many streams at once."""
+        """Run the cases as `lanes` sets (1..8) on as many streams at once."""
         _check(self._lib.lica_batch_set_lanes(self._h, int(lanes)), "set_lanes")
 
     def warm_info(self):

@@ -87,14 +87,14 @@ def test_graph_replay_equals_stream_launches():
 
 @pytest.mark.parametrize("mode", [L.POISSON_THOMAS, L.POISSON_PROJECTED])
 def test_two_lanes_equal_one(mode):
-    """The cases run as 2, 3 or 4 sets on as many streams (stream launches,
-    graph replay and timing mode) give the single sequence's fields bit for
-    bit."""
+    """The cases run as 2, 3, 4 or 6 sets (8 asked for, capped at the 6 cases)
+    on as many streams (stream launches, graph replay and timing mode) give
+    the single sequence's fields bit for bit."""
     cases = L.batched_cases(6, seed=5, nmin=128, nmax=512)
-    _say(f"2-4 lanes vs one, 6 cases, mode {mode}, 2 x 4 steps")
+    _say(f"2-6 lanes vs one, 6 cases, mode {mode}, 2 x 4 steps")
     runs = []
     for lanes, graph, timing in ((1, False, False), (2, False, False), (2, True, False), (2, False, True),
-                                 (3, False, True), (4, True, False), (4, False, False)):
+                                 (3, False, True), (4, True, False), (4, False, False), (8, False, False)):
         b = L.Batch(cases)
         b.set_poisson(mode, 1e-10, 100)
         b.set_lanes(lanes)
