@@ -352,18 +352,19 @@ DEV void body_warm_base(const BatchDev &bd, int inst, WarmSh &sh)
         PFOR_END
         return;
     }
-    // In the same pass (neighbours of B recomputed, bit for bit the stored
-    // values): the scales of the network's features -- max|r| of B's
-    // residual g - (bB + aB- + cB+), max|b| -- and the maxima of the
+    // B, then (reading B back, its neighbours written by other threads
+    // before the barrier) the scales of the network's features -- max|r| of
+    // B's residual g - (bB + aB- + cB+), max|b| -- and the maxima of the
     // multigrid solve's start test -- max|g|, max(|a|+|b|+|c|), max|B| and
     // max|r| with r accumulated as that test does, ((g - bB) - aB-) - cB+ --
     // so that the solve need not read the rows again when B already meets
     // the tolerance.
+    PFOR(row, 1, n + 1) { B[row] = base(row); }
+    PFOR_END
     double mx[6];
     red_maxN<6>(1, n + 1, [&](int row, double *v) {
-        const double x = base(row);
-        B[row] = x;
-        const double xl = (row > 1) ? base(row - 1) : 0.0, xr = (row < n) ? base(row + 1) : 0.0;
+        const double x = B[row];
+        const double xl = (row > 1) ? B[row - 1] : 0.0, xr = (row < n) ? B[row + 1] : 0.0;
         double ax = PB[row] * x;
         if (row > 1) ax = ax + PA[row] * xl;
         if (row < n) ax = ax + PCc[row] * xr;
