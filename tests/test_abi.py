@@ -79,3 +79,14 @@ def test_no_gpu_means_no_run():
     with pytest.raises(L.LicaError) as e:
         L.Batch([L.deck_case()])
     assert e.value.status == abi.NO_DEVICE
+
+
+def test_bench_measures_one_gpu_only():
+    """`bench.py --gpus N` with N > 1 exits with a message instead of
+    running: the batch's cases would shard by case, one GPU is measured."""
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+    assert r.stdout == ""
