@@ -453,7 +453,10 @@ DEV void body_transport(const BatchDev &bd, int inst, TransportSh &T)
     SERIAL {
         PCN[0] = PCN[1];
         PCN[n3] = PCN[n];
-        if (bd.msub == 3) S.time = S.time + S.dt;
+        if (bd.msub == 3) {
+            S.time = S.time + S.dt;
+            if (S.status == 0 && S.nf_pending) S.status = ST_NONFINITE;
+        }
     }
     SERIAL_END
 }

@@ -169,19 +169,9 @@ DEV void body_poisson_mg(const BatchDev &bd, int inst, MgSh &sh)
     const double *PA = cp.A(NS_PA), *PB = cp.A(NS_PB), *PCc = cp.A(NS_PC), *PG = cp.A(NS_PG);
     const bool warm = LICA_MG_WARM(bd.pmode) && bd.guess && bd.hist;
     const double *x0 = warm ? bd.guess + P.gs_off : nullptr;
-    // the warm-start pass's verdict on B (still the guess unless the
-    // network moved it, which clears the verdict)
-    const double *pre = (warm && bd.wscale) ? bd.wscale + (size_t)LICA_WARM_INFO * inst : nullptr;
     double res = 0.0;
-    int cyc;
-    if (pre && pre[6] != 0.0) {
-        PFOR(i, 1, n + 1) { PP[i] = x0[i]; }
-        PFOR_END
-        res = pre[5];
-        cyc = 0;
-    } else {
-        cyc = mg_solve(PA, PB, PCc, PG, PP, n, bd.mg + P.mg_off, bd.pmode, bd.mg_rtol, bd.mg_maxc, x0, sh, &res);
-    }
+    const int cyc = mg_solve(PA, PB, PCc, PG, PP, n, bd.mg + P.mg_off, bd.pmode, bd.mg_rtol, bd.mg_maxc, x0, sh,
+                             &res);
     if (warm) {
         // the solution joins this substep's history ring
         const int s = bd.msub - 1, head = S.hist_head[s];
@@ -435,13 +425,16 @@ DEV void body_ucalc(const BatchDev &bd, int inst, BlkSh &sh)
         return fabs(d);
     }, sh.red);
     double c3max = red_max(1, n + 1, 0.0, [&](int k) { return fabs(W[k] + W[k + 1]) * SSDZ[k]; }, sh.red);
+    // the end-of-step finiteness check of MAIN (W, T and P no longer change
+    // after the last substep's UCALC) takes effect after that substep's
+    // TRANSPORT, whose own STOPs come first
     int bad = 0;
     if (msub == 3)
         bad = red_or(0, n3 + 1, [&](int k) { return !(isfinite(W[k]) && isfinite(T[k]) && isfinite(PP[k])); }, sh.red);
     SERIAL {
         S.dvmax = dvmax;
         S.c3max = c3max;
-        if (bad) S.status = ST_NONFINITE;
+        S.nf_pending = bad;
     }
     SERIAL_END
     const int first = (m == 1 && msub == 1);

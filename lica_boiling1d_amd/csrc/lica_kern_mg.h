@@ -1,24 +1,52 @@
 // Multigrid solve of the POISSON system (1d.f:3529-3543) of a substep, one
-// workgroup per case: V(2,2)-cycles with red-black Gauss-Seidel smoothing,
-// Galerkin coarse operators P^T A P (linear interpolation, coarse points at
-// the even fine rows, zero beyond the top row as in the reference's implicit
-// p = 0 above the last face) and an exact FP64 elimination on the coarsest
-// level.  Every mode is FP64 defect correction x += V(g - A x): the residual,
-// the update and the stopping test are FP64, the V-cycle runs on the residual
-// equation from a zero start in
-//   MG64:    FP64 (IOLDV=0, lica.in:8: the iteration starts from x = 0),
-//   MIXED:   FP32 (matrices, smoother, transfers; the coarsest solve in FP64),
-//   LEARNED: FP32, the iteration started from the warm-start network's guess,
-//   PROJECTED: FP32, started from the projection on the solution history,
-// until the normwise backward error max|g - A x| / (||A|| max|x| + max|g|)
-// reaches rtol.  The solution replaces P(1:N) as the direct solve does.
+// workgroup per case.
 //
-// Memory: the iterate, the fine matrix and the matrices of every level sit in
-// a per-case global scratch (read-only during the cycles), one packed row
-// (a, b, c, 1/b) per unknown so that a row is one wide load; x and f of the
-// levels live in LDS from the coarsest level up as far as the launch's
-// dynamic LDS allows (the rest in the scratch), so that the passes of a cycle
-// -- each one barrier -- do not wait on global stores.
+// The V-cycle is a matrix-dependent ("ideal") two-level scheme applied
+// recursively: the odd rows of a level are its fine (F) points and the even
+// rows its coarse (C) points; the coarse operator is the Schur complement of
+// the F rows (Galerkin with the operator-dependent interpolation
+// P_F = -A_FF^-1 A_FC), the restriction is R = [I, -A_CF A_FF^-1], and after
+// the coarse correction the F rows are relaxed exactly.  On a tridiagonal
+// operator that cycle is exact in exact arithmetic (it is cyclic reduction
+// written as a V-cycle), so its contraction per cycle is set by rounding
+// alone.
+//
+// The POISSON rows are a nearly singular M-matrix: b = -(a + c) rounded, so a
+// row sums to the rounding error of that addition, except the last row,
+// which carries the Dirichlet coefficient of the implicit p = 0 above the top
+// face.  Computing a coarse pivot as b - a c'/b' - c a'/b' cancels almost all
+// of its digits.  The setup instead carries every level as (a, c, s, 1/b)
+// with s the row sum -- exact on the finest level by error-free additions --
+// and forms a coarse row as s' = s - (a/b_l) s_l - (c/b_u) s_u,
+// a' = -(a/b_l) a_l, c' = -(c/b_u) c_u, b' = (s' - a') - c': every sum then
+// adds terms of one sign (Grassmann-Taksar-Heyman elimination), so each
+// level is accurate to a few ulps of its own entries, in FP64 and in the FP32
+// copy the mixed-precision cycles read.  The coarsest level (<= MG_COARSE
+// rows) is eliminated the same way in FP64.
+//
+// Every mode is defect correction x += V(g - A x) in FP64 with the residual
+// in double-double (error-free products and sums, rounded once); the cycle V
+// runs in
+//   MG64:      FP64 (IOLDV=0, lica.in:8: the iteration starts from x = 0),
+//   MIXED:     FP32 (the coarsest solve in FP64),
+//   LEARNED:   FP32, started from the warm-start network's guess,
+//   PROJECTED: FP32, started from the projection on the solution history.
+// A solve ends when the normwise backward error
+// max|g - A x| / (||A|| max|x| + max|g|) <= rtol AND the last correction
+// max|dx| <= rtol max|x|.  The second test bounds the forward error: the
+// cycle contracts the error by 1e-3 or more per cycle in FP32 (to rounding in
+// FP64), so the error left after a correction of relative size rtol is far
+// below rtol.  A starting point is never accepted without one cycle.
+// Non-finite residuals or iterates never pass a test (their maxima read as
+// infinity), a non-finite guess is replaced by zero, and a system with
+// non-finite rows yields a non-finite solution, as the direct solve would.
+//
+// Memory: setup rows of the coarse levels, cycle rows of every level (FP64
+// or FP32) and the iterate sit in a per-case global scratch; the cycle's
+// vectors -- level 0's right-hand side r and correction e, and one vector per
+// coarser level (its f on the way down, its x on the way up) -- live in LDS
+// from the coarsest level up as far as the launch's dynamic LDS allows (the
+// rest in the scratch).
 #pragma once
 #include "lica_kern_common.h"
 
@@ -26,7 +54,7 @@
 #define MG_COARSE 32       // levels stop at this many rows
 #endif
 #define MG_MAXLEV 20
-#define MG_NU 2            // pre- and post-smoothing sweeps
+#define MG_NRED 5         // values reduced at once (mg_correct), per wave of up to 16
 
 // rows of level l for n fine rows
 HOSTDEV int mg_rows(int n, int l)
@@ -53,23 +81,24 @@ HOSTDEV long long mg_rows_total(int n) { return mg_rows_before(n, mg_levels(n));
 
 // Per-case scratch, in doubles (every block 32-byte aligned when the case's
 // scratch is):
-//   4 tot      FP64 rows (a, b, c, 1/b) of every level, level 0 first
-//   2 tot      FP32 rows of every level
-//   4n         g, iterate, its double buffer (and one spare row)
-//   2 tot      x, f of the levels kept outside LDS (room for FP64)
+//   4 (tot - n)  FP64 setup rows (a, c, s, 1/b) of levels 1..L-1
+//   4 tot        FP64 cycle rows of every level
+//   2 tot        FP32 cycle rows of every level
+//   2 n          the iterate and its double buffer
+//   n + tot      the cycle's vectors kept outside LDS (room for FP64)
 HOSTDEV long long mg_scratch_doubles(int n)
 {
     const long long tot = mg_rows_total(n);
-    return (4LL * tot + 2LL * tot + 4LL * n + 2LL * tot + 8 + 3) / 4 * 4;
+    return (4LL * (tot - n) + 4LL * tot + 2LL * tot + 2LL * n + (n + tot) + 8 + 3) / 4 * 4;
 }
-// LDS that holds x, f of every level in elements of tsize bytes
-HOSTDEV long long mg_lds_bytes(int n, int tsize) { return 2LL * tsize * mg_rows_total(n); }
+// LDS that holds every vector of a cycle in elements of tsize bytes
+HOSTDEV long long mg_lds_bytes(int n, int tsize) { return (long long)tsize * (n + mg_rows_total(n)); }
 
 struct MgSh {
-    double red[64];                                        // reductions of up to four values
-    double cg[MG_COARSE], cib[MG_COARSE], ca[MG_COARSE];   // coarsest-level elimination factors
+    double red[16 * MG_NRED];                              // per-wave partials: 16 waves x MG_NRED values
+    double cm[MG_COARSE], cid[MG_COARSE], cc[MG_COARSE];   // coarsest level: multipliers, 1/pivots, c
     double cy[MG_COARSE];                                  // its forward sweep
-    void *lds;                                             // x, f of the coarse levels
+    void *lds;                                             // the cycle's vectors (dynamic LDS)
     long long lds_bytes;
     // level table of the current solve (filled once: the loops that give a
     // level's rows and offset would otherwise run before every pass)
@@ -97,358 +126,316 @@ DEV void mg_table(int n, MgSh &sh)
     SERIAL_END
 }
 
-// one packed matrix row
+// setup row of a level: off-diagonals, row sum, 1/diagonal
 #ifdef LICA_EMUL
+struct MgSet {
+    double a, c, s, ib;
+};
 template <class T>
-struct Row4 {
-    T a, b, c, ib;
+struct MgCyc {
+    T w0, w1, w2, pad;
 };
 #else
+struct alignas(32) MgSet {
+    double a, c, s, ib;
+};
 template <class T>
-struct alignas(4 * sizeof(T)) Row4 {
-    T a, b, c, ib;
+struct alignas(4 * sizeof(T)) MgCyc {
+    T w0, w1, w2, pad;
 };
 #endif
+// cycle row of a level:
+//   C row i (even): f_c = f_i + w0 f_{i-1} + w1 f_{i+1}   (w0 = -a_i/b_{i-1}, w1 = -c_i/b_{i+1})
+//   F row k (odd):  x_k = w0 f_k + w1 x_{k-1} + w2 x_{k+1} (w0 = 1/b_k, w1 = -a_k/b_k, w2 = -c_k/b_k)
 
-template <class T>
-struct MgLev {
-    Row4<T> *m;   // rows (a, b, c, 1/b); written only while the levels are built
-    T *x, *f;
-    int n;
-};
-struct MgFine {
-    Row4<double> *m;
-    double *g, *x0, *x1;
-    int n;
-};
-
-DEV MgFine mg_fine(double *scr, int n, const MgSh &sh)
+// row i (0..n-1) of the finest level from the POISSON rows k = i+1 (a(1) and
+// c(n) are not part of the system): the row sum exactly, by error-free
+// additions, rounded once
+DEV MgSet mg_row0(const double *a, const double *b, const double *c, int n, int i)
 {
-    const long long tot = sh.lv_pre[sh.nlev];
-    MgFine F;
-    F.n = n;
-    F.m = (Row4<double> *)scr;
-    double *v = scr + 6 * tot;
-    F.g = v; F.x0 = v + n; F.x1 = v + 2 * n;
-    return F;
+    const double ai = (i == 0) ? 0.0 : a[i + 1], bi = b[i + 1], ci = (i == n - 1) ? 0.0 : c[i + 1];
+    double t, e1, s, e2;
+    two_sum(ai, ci, &t, &e1);
+    two_sum(t, bi, &s, &e2);
+    MgSet r;
+    r.a = ai; r.c = ci; r.s = s + (e1 + e2); r.ib = 1.0 / bi;
+    return r;
 }
 
-// rows of level l (FP64 or FP32); x, f in LDS when levels l..L-1 fit
-template <class T>
-DEV MgLev<T> mg_lev(double *scr, int n, int l, const MgSh &sh)
+struct MgView {
+    double *scr;
+    int n;
+    long long tot;
+    MgSet *set;                 // setup rows of levels >= 1 (level l row i at lv_pre[l] - n + i)
+    MgCyc<double> *c64;         // cycle rows (level l row i at lv_pre[l] + i)
+    MgCyc<float> *c32;
+    double *x0, *x1;            // iterate buffers
+    double *gv;                 // global home of the cycle's vectors
+};
+
+DEV MgView mg_view(double *scr, int n, const MgSh &sh)
 {
-    const int m = sh.lv_n[l];
-    const long long pre = sh.lv_pre[l], tot = sh.lv_pre[sh.nlev];
-    MgLev<T> v;
-    v.n = m;
-    if constexpr (sizeof(T) == 8) v.m = (Row4<T> *)scr + pre;
-    else v.m = (Row4<T> *)(scr + 4 * tot) + pre;
-    const long long suffix = tot - pre;   // rows of levels l..L-1
-    T *q;
-    if ((long long)(2 * sizeof(T)) * suffix <= sh.lds_bytes) q = (T *)sh.lds + 2 * (suffix - m);
-    else q = (T *)(scr + 6 * tot + 4LL * n) + 2 * pre;
-    v.x = q;
-    v.f = q + m;
+    MgView v;
+    v.scr = scr;
+    v.n = n;
+    v.tot = sh.lv_pre[sh.nlev];
+    v.set = (MgSet *)scr;
+    v.c64 = (MgCyc<double> *)(scr + 4 * (v.tot - n));
+    v.c32 = (MgCyc<float> *)(scr + 4 * (v.tot - n) + 4 * v.tot);
+    v.x0 = scr + 4 * (v.tot - n) + 6 * v.tot;
+    v.x1 = v.x0 + n;
+    v.gv = v.x1 + n;
     return v;
 }
 
-// interpolation weight of coarse point J at fine row k (fine rows 0..nf-1,
-// coarse J <-> fine 2J; odd fine rows average their two coarse neighbours)
-DEV double mg_pw(int k, int J, int nf, int nc)
+// The cycle's vectors in one index space of n + tot elements: level 0's r at
+// [0, n), its e at [n, 2n), level l >= 1 at [n + lv_pre[l], n + lv_pre[l+1]).
+// The tail of that space from Q = max(0, n + tot - LDS elements) lives in LDS;
+// a vector that starts below Q stays in global memory.
+template <class T>
+DEV T *mg_vec(const MgView &V, const MgSh &sh, long long start)
 {
-    if (k < 0 || k >= nf || J < 0 || J >= nc) return 0.0;
-    if (k == 2 * J) return 1.0;
-    if (k == 2 * J - 1 || k == 2 * J + 1) return 0.5;
-    return 0.0;
+    const long long cap = sh.lds_bytes / (long long)sizeof(T);
+    const long long q0 = (V.n + V.tot > cap) ? V.n + V.tot - cap : 0;
+    if (start >= q0) return (T *)sh.lds + (start - q0);
+    return (T *)V.gv + start;
+}
+template <class T>
+DEV T *mg_v(const MgView &V, const MgSh &sh, int l) { return mg_vec<T>(V, sh, V.n + sh.lv_pre[l]); }
+template <class T>
+DEV T *mg_r0(const MgView &V, const MgSh &sh) { return mg_vec<T>(V, sh, 0); }
+template <class T>
+DEV T *mg_e0(const MgView &V, const MgSh &sh) { return mg_vec<T>(V, sh, V.n); }
+
+template <class T>
+DEV MgCyc<T> *mg_cyc(const MgView &V, const MgSh &sh, int l)
+{
+    if constexpr (sizeof(T) == 8) return V.c64 + sh.lv_pre[l];
+    else return V.c32 + sh.lv_pre[l];
 }
 
-// Galerkin operator of level l+1 from level l (FP64), with 1/b, and the FP32
-// copy of level l+1 when the cycles run in FP32
-DEV void mg_coarsen(const MgLev<double> &F, const MgLev<double> &C, Row4<float> *c32, bool f32)
+// Setup: the coarse levels' rows (row-sum form), the cycle rows of every
+// level, and the coarsest level's elimination factors.
+template <class T>
+DEV void mg_setup(const double *a, const double *b, const double *c, const MgView &V, MgSh &sh)
 {
-    const int nf = F.n, nc = C.n;
-    PFOR(I, 0, nc) {
-        double acc[3] = {0.0, 0.0, 0.0};
-        for (int t = 0; t < 3; t++) {
-            const int i = 2 * I - 1 + t;
-            const double wi = mg_pw(i, I, nf, nc);
-            if (wi == 0.0) continue;
-            const Row4<double> ri = F.m[i];
-            const double ai = ri.a, ci = ri.c, bi = ri.b;
-            for (int d = -1; d <= 1; d++) {
-                const int J = I + d;
-                double apj = ai * mg_pw(i - 1, J, nf, nc) + bi * mg_pw(i, J, nf, nc) + ci * mg_pw(i + 1, J, nf, nc);
-                acc[d + 1] += wi * apj;
+    const int L = sh.nlev, n = V.n;
+    for (int l = 0; l + 1 < L; l++) {
+        const int nl = sh.lv_n[l], nc = sh.lv_n[l + 1];
+        MgSet *fine = (l == 0) ? nullptr : V.set + (sh.lv_pre[l] - n);
+        MgSet *crs = V.set + (sh.lv_pre[l + 1] - n);
+        MgCyc<T> *cy = mg_cyc<T>(V, sh, l);
+        PFOR(J, 0, nc) {
+            const int i = 2 * J;
+            auto row = [&](int k) -> MgSet { return fine ? fine[k] : mg_row0(a, b, c, n, k); };
+            const MgSet R = row(i);
+            double sv = R.s, av = 0.0, cv = 0.0, w0 = 0.0, w1 = 0.0;
+            if (i >= 1) {
+                const MgSet Lo = row(i - 1);
+                const double q = R.a * Lo.ib;
+                av = -q * Lo.a;
+                sv = sv - q * Lo.s;
+                w0 = -q;
             }
-        }
-        const double va = (I == 0) ? 0.0 : acc[0], vc = (I == nc - 1) ? 0.0 : acc[2], vb = acc[1];
-        Row4<double> r;
-        r.a = va; r.b = vb; r.c = vc; r.ib = 1.0 / vb;
-        C.m[I] = r;
-        if (f32) {
-            Row4<float> q;
-            q.a = (float)va; q.b = (float)vb; q.c = (float)vc; q.ib = 1.0f / (float)vb;
-            c32[I] = q;
-        }
-    }
-    PFOR_END
-}
-
-// red-black Gauss-Seidel; zero_start: x is taken as zero before the first
-// half-sweep (its stale values are never read)
-template <class T>
-DEV void mg_smooth(const MgLev<T> &L, int sweeps, bool zero_start)
-{
-    const int n = L.n;
-    for (int s = 0; s < sweeps; s++) {
-        for (int color = 0; color < 2; color++) {
-            const bool z = zero_start && s == 0 && color == 0;
-            PFOR(h, 0, (n - color + 1) / 2) {
-                const int i = 2 * h + color;
-                const Row4<T> r = L.m[i];
-                T v = L.f[i];
-                if (!z) {
-                    if (i > 0) v = v - r.a * L.x[i - 1];
-                    if (i < n - 1) v = v - r.c * L.x[i + 1];
-                }
-                L.x[i] = v * r.ib;
+            if (i + 1 < nl) {
+                const MgSet Up = row(i + 1);
+                const double q = R.c * Up.ib;
+                cv = -q * Up.c;
+                sv = sv - q * Up.s;
+                w1 = -q;
+                MgCyc<T> f;
+                f.w0 = (T)Up.ib; f.w1 = (T)(-Up.a * Up.ib); f.w2 = (T)(-Up.c * Up.ib); f.pad = (T)0;
+                cy[i + 1] = f;
             }
-            PFOR_END
+            MgSet C;
+            C.a = av; C.c = cv; C.s = sv; C.ib = 1.0 / ((sv - av) - cv);
+            crs[J] = C;
+            MgCyc<T> e;
+            e.w0 = (T)w0; e.w1 = (T)w1; e.w2 = (T)0; e.pad = (T)0;
+            cy[i] = e;
         }
+        PFOR_END
     }
-}
-
-template <class T>
-DEV T mg_res_at(const MgLev<T> &F, int i)
-{
-    const Row4<T> r = F.m[i];
-    T v = F.f[i] - r.b * F.x[i];
-    if (i > 0) v = v - r.a * F.x[i - 1];
-    if (i < F.n - 1) v = v - r.c * F.x[i + 1];
-    return v;
-}
-
-// f_c = P^T (f - A x) with the residual evaluated on the fly
-template <class T>
-DEV void mg_restrict(const MgLev<T> &F, const MgLev<T> &C)
-{
-    PFOR(J, 0, C.n) {
-        const int i = 2 * J;
-        T v = mg_res_at(F, i);
-        if (i - 1 >= 0) v = v + (T)0.5 * mg_res_at(F, i - 1);
-        if (i + 1 < F.n) v = v + (T)0.5 * mg_res_at(F, i + 1);
-        C.f[J] = v;
-    }
-    PFOR_END
-}
-
-// x_f += P x_c
-template <class T>
-DEV void mg_prolong(const MgLev<T> &F, const MgLev<T> &C)
-{
-    PFOR(i, 0, F.n) {
-        const int J = i / 2;
-        T e;
-        if (i % 2 == 0) e = C.x[J];
-        else e = (T)0.5 * (C.x[J] + ((J + 1 < C.n) ? C.x[J + 1] : (T)0.0));
-        F.x[i] = F.x[i] + e;
-    }
-    PFOR_END
-}
-
-// elimination factors of the coarsest level (FP64, once per solve)
-DEV void mg_coarsest_factor(const MgLev<double> &L, MgSh &sh)
-{
+    // coarsest level: Thomas elimination in row-sum form, sigma_j the row sum
+    // of row j after the rows below it are eliminated, pivot sigma_j - c_j
     SERIAL {
-        const int n = L.n;
-        double ib = 1.0 / L.m[0].b;
-        sh.cib[0] = ib;
-        sh.ca[0] = 0.0;
-        sh.cg[0] = 0.0;
-        for (int j = 1; j < n; j++) {
-            const Row4<double> r = L.m[j];
-            const double g = L.m[j - 1].c * ib;
-            ib = 1.0 / (r.b - r.a * g);
-            sh.cg[j] = g;
-            sh.cib[j] = ib;
-            sh.ca[j] = r.a;
+        const int l = L - 1, m = sh.lv_n[l];
+        const MgSet *rows = (l == 0) ? nullptr : V.set + (sh.lv_pre[l] - n);
+        double sig = 0.0, piv = 1.0;
+        for (int j = 0; j < m; j++) {
+            const MgSet r = rows ? rows[j] : mg_row0(a, b, c, n, j);
+            const double mu = (j == 0) ? 0.0 : r.a / piv;
+            sig = r.s - mu * sig;
+            piv = sig - r.c;
+            sh.cm[j] = mu;
+            sh.cid[j] = 1.0 / piv;
+            sh.cc[j] = r.c;
         }
     }
     SERIAL_END
 }
 
-// exact solve on the coarsest level in FP64 (one thread)
+// exact FP64 solve on the coarsest level: f in, x out (may alias)
 template <class T>
-DEV void mg_coarsest(const MgLev<T> &L, MgSh &sh)
+DEV void mg_coarsest(const T *f, T *x, int m, MgSh &sh)
 {
     SERIAL {
-        const int n = L.n;
-        double y = (double)L.f[0] * sh.cib[0];
+        double y = (double)f[0];
         sh.cy[0] = y;
-        for (int j = 1; j < n; j++) {
-            y = ((double)L.f[j] - sh.ca[j] * y) * sh.cib[j];
+        for (int j = 1; j < m; j++) {
+            y = (double)f[j] - sh.cm[j] * y;
             sh.cy[j] = y;
         }
-        double x = sh.cy[n - 1];
-        L.x[n - 1] = (T)x;
-        for (int j = n - 2; j >= 0; j--) {
-            x = sh.cy[j] - sh.cg[j + 1] * x;
-            L.x[j] = (T)x;
+        double v = sh.cy[m - 1] * sh.cid[m - 1];
+        x[m - 1] = (T)v;
+        for (int j = m - 2; j >= 0; j--) {
+            v = (sh.cy[j] - sh.cc[j] * v) * sh.cid[j];
+            x[j] = (T)v;
         }
     }
     SERIAL_END
 }
 
-// one V-cycle on the residual equation held in level 0's f; result in its x
+// one cycle on level 0's r; the correction in level 0's e
 template <class T>
-DEV void mg_vcycle(double *scr, int n, MgSh &sh)
+DEV void mg_cycle(const MgView &V, MgSh &sh)
 {
     const int L = sh.nlev;
+    T *r0 = mg_r0<T>(V, sh), *e0 = mg_e0<T>(V, sh);
+    if (L == 1) {
+        mg_coarsest<T>(r0, e0, sh.lv_n[0], sh);
+        return;
+    }
     for (int l = 0; l + 1 < L; l++) {
-        MgLev<T> F = mg_lev<T>(scr, n, l, sh), C = mg_lev<T>(scr, n, l + 1, sh);
-        mg_smooth<T>(F, MG_NU, true);
-        mg_restrict<T>(F, C);
-    }
-    mg_coarsest<T>(mg_lev<T>(scr, n, L - 1, sh), sh);
-    for (int l = L - 2; l >= 0; l--) {
-        MgLev<T> F = mg_lev<T>(scr, n, l, sh), C = mg_lev<T>(scr, n, l + 1, sh);
-        mg_prolong<T>(F, C);
-        mg_smooth<T>(F, MG_NU, false);
-    }
-}
-
-// xn = x + e (e = Z.x, absent on the first pass) into the other buffer,
-// r = g - A xn in FP64, Z.f = r in T; *rmax = max|r|, *xmax = max|xn|
-template <class T>
-DEV void mg_update(const MgFine &F, const MgLev<T> &Z, bool first, const double *xa, double *xb, MgSh &sh,
-                   double *rmax, double *xmax)
-{
-    const int n = F.n;
-    auto xn = [&](int k) -> double { return first ? xa[k] : xa[k] + (double)Z.x[k]; };
-    red_max2(0, n, [&](int i, double &ra, double &xa_) {
-        const Row4<double> r = F.m[i];
-        const double xi = xn(i);
-        double v = F.g[i] - r.b * xi;
-        if (i > 0) v = v - r.a * xn(i - 1);
-        if (i < n - 1) v = v - r.c * xn(i + 1);
-        xb[i] = xi;
-        Z.f[i] = (T)v;
-        ra = fabs(v);
-        xa_ = fabs(xi);
-    }, sh.red, rmax, xmax);
-}
-
-// residual of the starting point; when it already meets rtol the solve ends
-// there, before any level is built
-template <class T>
-DEV double mg_start(const MgFine &F, int n, double fmax, double anorm, bool warm, MgSh &sh, double **cur, double **oth,
-                    double *scr)
-{
-    MgLev<T> Z = mg_lev<T>(scr, n, 0, sh);
-    double rmax, xmax;
-    mg_update<T>(F, Z, true, *cur, *oth, sh, &rmax, &xmax);
-    { double *t = *cur; *cur = *oth; *oth = t; }
-    if (warm && rmax > fmax) {
-        // a starting guess with a larger residual than zero's is dropped
-        PFOR(i, 0, n) { (*cur)[i] = 0.0; }
+        const int nl = sh.lv_n[l], nc = sh.lv_n[l + 1];
+        const T *fin = (l == 0) ? r0 : mg_v<T>(V, sh, l);
+        T *fout = mg_v<T>(V, sh, l + 1);
+        const MgCyc<T> *cy = mg_cyc<T>(V, sh, l);
+        PFOR(J, 0, nc) {
+            const int i = 2 * J;
+            const MgCyc<T> w = cy[i];
+            T v = fin[i];
+            if (i >= 1) v = v + w.w0 * fin[i - 1];
+            if (i + 1 < nl) v = v + w.w1 * fin[i + 1];
+            fout[J] = v;
+        }
         PFOR_END
-        mg_update<T>(F, Z, true, *cur, *oth, sh, &rmax, &xmax);
-        { double *t = *cur; *cur = *oth; *oth = t; }
     }
-    const double den = anorm * xmax + fmax;
+    {
+        T *vc = mg_v<T>(V, sh, L - 1);
+        mg_coarsest<T>(vc, vc, sh.lv_n[L - 1], sh);
+    }
+    for (int l = L - 2; l >= 0; l--) {
+        const int nl = sh.lv_n[l], nc = sh.lv_n[l + 1];
+        const T *xc = mg_v<T>(V, sh, l + 1);
+        const T *fx = (l == 0) ? r0 : mg_v<T>(V, sh, l);
+        T *xo = (l == 0) ? e0 : mg_v<T>(V, sh, l);
+        const MgCyc<T> *cy = mg_cyc<T>(V, sh, l);
+        PFOR(J, 0, nc) {
+            const T xl = xc[J];
+            xo[2 * J] = xl;
+            const int k = 2 * J + 1;
+            if (k < nl) {
+                const MgCyc<T> w = cy[k];
+                const T xr = (J + 1 < nc) ? xc[J + 1] : (T)0;
+                xo[k] = w.w0 * fx[k] + w.w1 * xl + w.w2 * xr;
+            }
+        }
+        PFOR_END
+    }
+}
+
+DEV double mg_mag(double v) { return isfinite(v) ? fabs(v) : INFINITY; }
+
+// x_new = x + e (e absent on the first pass) into xb, r = g - A x_new in
+// double-double into level 0's r; mx = {max|r|, max|x_new|, max|e|,
+// max|g|, ||A||_inf} (non-finite values as infinity; the last two on the
+// first pass only)
+template <class T>
+DEV void mg_correct(const double *a, const double *b, const double *c, const double *g, const MgView &V,
+                    const double *xa, double *xb, bool first, MgSh &sh, double *mx)
+{
+    const int n = V.n;
+    T *r0 = mg_r0<T>(V, sh);
+    const T *e0 = mg_e0<T>(V, sh);
+    auto xn = [&](int k) -> double { return first ? xa[k] : xa[k] + (double)e0[k]; };
+    red_maxN<MG_NRED>(0, n, [&](int i, double *v) {
+        const double ai = (i == 0) ? 0.0 : a[i + 1], bi = b[i + 1], ci = (i == n - 1) ? 0.0 : c[i + 1];
+        const double x = xn(i), xl = (i > 0) ? xn(i - 1) : 0.0, xr = (i < n - 1) ? xn(i + 1) : 0.0;
+        const double r = dd_residual(g[i + 1], ai, xl, bi, x, ci, xr);
+        xb[i] = x;
+        r0[i] = (T)r;
+        v[0] = mg_mag(r);
+        v[1] = mg_mag(x);
+        v[2] = first ? 0.0 : mg_mag((double)e0[i]);
+        v[3] = first ? mg_mag(g[i + 1]) : 0.0;
+        v[4] = first ? mg_mag(fabs(ai) + fabs(bi) + fabs(ci)) : 0.0;
+    }, sh.red, mx);
+}
+
+DEV double mg_berr(double rmax, double xmax, double gmax, double anorm)
+{
+    const double den = anorm * xmax + gmax;
     return (den > 0.0) ? rmax / den : rmax;
 }
 
-// V-cycles until rtol; *cur is the iterate on return (the update alternates
-// the two buffers)
 template <class T>
-DEV int mg_iterate(const MgFine &F, double *scr, int n, double rtol, int maxc, double fmax, double anorm, MgSh &sh,
-                   double **cur, double **oth, double res, double *res_out)
+DEV int mg_run(const double *a, const double *b, const double *c, const double *g, const MgView &V, double rtol,
+               int maxc, const double *x0, MgSh &sh, double **cur, double *res_out, bool *nonfinite_rows)
 {
-    MgLev<T> Z = mg_lev<T>(scr, n, 0, sh);
-    double rmax, xmax;
-    int cyc = 0;
-    while (res > rtol && cyc < maxc) {
-        mg_vcycle<T>(scr, n, sh);
-        mg_update<T>(F, Z, false, *cur, *oth, sh, &rmax, &xmax);
-        { double *t = *cur; *cur = *oth; *oth = t; }
-        cyc++;
-        const double den = anorm * xmax + fmax;
-        res = (den > 0.0) ? rmax / den : rmax;
+    const int n = V.n;
+    mg_setup<T>(a, b, c, V, sh);
+    double *xa = V.x0, *xb = V.x1;
+    PFOR(i, 0, n) { xa[i] = x0 ? x0[i + 1] : 0.0; }
+    PFOR_END
+    double mx[MG_NRED];
+    mg_correct<T>(a, b, c, g, V, xa, xb, true, sh, mx);
+    const double gmax = mx[3], anorm = mx[4];
+    *nonfinite_rows = !(gmax < INFINITY) || !(anorm < INFINITY);
+    if (*nonfinite_rows) { *cur = xb; *res_out = NAN; return 0; }
+    if (x0 && !(mx[1] < INFINITY && mx[0] < INFINITY)) {
+        // a non-finite guess is dropped: start from zero
+        PFOR(i, 0, n) { xa[i] = 0.0; }
+        PFOR_END
+        mg_correct<T>(a, b, c, g, V, xa, xb, true, sh, mx);
     }
+    { double *t = xa; xa = xb; xb = t; }
+    double res = mg_berr(mx[0], mx[1], gmax, anorm);
+    int cyc = 0;
+    bool conv = false;
+    while (cyc < maxc) {
+        mg_cycle<T>(V, sh);
+        mg_correct<T>(a, b, c, g, V, xa, xb, false, sh, mx);
+        { double *t = xa; xa = xb; xb = t; }
+        cyc++;
+        res = mg_berr(mx[0], mx[1], gmax, anorm);
+        if (!(mx[1] < INFINITY)) break;
+        if (res <= rtol && mx[2] <= rtol * mx[1]) { conv = true; break; }
+    }
+    *cur = xa;
     *res_out = res;
-    return (res > rtol) ? -1 : cyc;
+    return conv ? cyc : -1;
 }
 
 // Solve rows 1..n of a, b, c, g (a(1), c(n) ignored) into p(1..n).
-// mode 2: FP64 cycles; 3-5: FP32 cycles (x0 holds the initial guess for
-// modes 4 and 5).  The levels are built only when the starting point misses
-// rtol.  Returns the cycles, or -1 when maxc cycles did not reach rtol; *res
-// the final backward error.
+// mode 2: FP64 cycles; 3-5: FP32 cycles (x0, rows 1..n, holds the initial
+// guess for modes 4 and 5).  Returns the cycles, or -1 when maxc cycles did
+// not meet the stopping test; *res_out the final backward error.  Rows with
+// non-finite entries give a non-finite p and 0 cycles.
 DEV int mg_solve(const double *a, const double *b, const double *c, const double *g, double *p, int n,
                  double *scr, int mode, double rtol, int maxc, const double *x0, MgSh &sh, double *res_out)
 {
     const bool guess = (LICA_MG_WARM(mode) && x0);
-    if (guess) {
-        // a warm start usually meets rtol as it is: its backward error from the
-        // rows themselves, before anything is packed (the maxima of the packed
-        // path below, so the same decision and, when it is taken, p = x0)
-        double mx[4];   // max|g|, max(|a|+|b|+|c|), max|r|, max|x0|
-        red_maxN<4>(0, n, [&](int i, double *v) {
-            const double ai = (i == 0) ? 0.0 : a[i + 1], bi = b[i + 1], ci = (i == n - 1) ? 0.0 : c[i + 1];
-            const double xi = x0[i + 1];
-            double r = g[i + 1] - bi * xi;
-            if (i > 0) r = r - ai * x0[i];
-            if (i < n - 1) r = r - ci * x0[i + 2];
-            v[0] = fabs(g[i + 1]);
-            v[1] = fabs(ai) + fabs(bi) + fabs(ci);
-            v[2] = fabs(r);
-            v[3] = fabs(xi);
-        }, sh.red, mx);
-        const double den = mx[1] * mx[3] + mx[0];
-        const double res0 = (den > 0.0) ? mx[2] / den : mx[2];
-        if (!(mx[2] > mx[0]) && !(res0 > rtol)) {
-            PFOR(i, 0, n) { p[i + 1] = x0[i + 1]; }
-            PFOR_END
-            *res_out = res0;
-            return 0;
-        }
-    }
     mg_table(n, sh);
-    const int L = sh.nlev;
-    const bool f32 = (mode != LICA_MG_FP64_MODE);
-    MgFine F = mg_fine(scr, n, sh);
-    Row4<float> *m32 = mg_lev<float>(scr, n, 0, sh).m;
-    double fmax, anorm;
-    red_max2(0, n, [&](int i, double &fm, double &an) {
-        const double ai = (i == 0) ? 0.0 : a[i + 1], bi = b[i + 1], ci = (i == n - 1) ? 0.0 : c[i + 1];
-        Row4<double> r;
-        r.a = ai; r.b = bi; r.c = ci; r.ib = 1.0 / bi;
-        F.m[i] = r;
-        F.g[i] = g[i + 1];
-        F.x0[i] = guess ? x0[i + 1] : 0.0;
-        if (f32) {
-            Row4<float> q;
-            q.a = (float)ai; q.b = (float)bi; q.c = (float)ci; q.ib = 1.0f / (float)bi;
-            m32[i] = q;
-        }
-        fm = fabs(g[i + 1]);
-        an = fabs(ai) + fabs(bi) + fabs(ci);
-    }, sh.red, &fmax, &anorm);
-    double *cur = F.x0, *oth = F.x1, res;
-    if (f32) res = mg_start<float>(F, n, fmax, anorm, guess, sh, &cur, &oth, scr);
-    else res = mg_start<double>(F, n, fmax, anorm, guess, sh, &cur, &oth, scr);
-    int cyc = 0;
-    if (res > rtol) {
-        for (int l = 0; l + 1 < L; l++)
-            mg_coarsen(mg_lev<double>(scr, n, l, sh), mg_lev<double>(scr, n, l + 1, sh),
-                       mg_lev<float>(scr, n, l + 1, sh).m, f32);
-        mg_coarsest_factor(mg_lev<double>(scr, n, L - 1, sh), sh);
-        if (f32) cyc = mg_iterate<float>(F, scr, n, rtol, maxc, fmax, anorm, sh, &cur, &oth, res, &res);
-        else cyc = mg_iterate<double>(F, scr, n, rtol, maxc, fmax, anorm, sh, &cur, &oth, res, &res);
-    }
-    PFOR(i, 0, n) { p[i + 1] = cur[i]; }
+    const MgView V = mg_view(scr, n, sh);
+    double *cur;
+    bool bad = false;
+    int cyc;
+    if (mode == LICA_MG_FP64_MODE)
+        cyc = mg_run<double>(a, b, c, g, V, rtol, maxc, guess ? x0 : nullptr, sh, &cur, res_out, &bad);
+    else
+        cyc = mg_run<float>(a, b, c, g, V, rtol, maxc, guess ? x0 : nullptr, sh, &cur, res_out, &bad);
+    PFOR(i, 0, n) { p[i + 1] = bad ? NAN : cur[i]; }
     PFOR_END
-    *res_out = res;
-    return (res > rtol) ? -1 : cyc;
+    return cyc;
 }

@@ -54,6 +54,8 @@ struct InstState {
     double time, dt, alpha, acoef, dtconst, dtconsti;
     long long m;
     int msub, status;
+    int nf_pending;       // W, T or P non-finite after the step's last UCALC: ST_NONFINITE once TRANSPORT ends
+    int pad0;
     double vol_tot_ori, vol_tot_iphs, place, dvmax, cflm, c3max;
     double n_subcycle_warn, n_mass_warn, n_halvings, n_extrap_iters;
     double n_reinit, n_bandgen, n_masscor;

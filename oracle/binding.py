@@ -28,15 +28,16 @@ class OrcCase(ctypes.Structure):
 
 
 def build(force=False, extra=()):
-    """gcc -O2 -ffp-contract=off: every operation rounds on its own, in the
-    serial order the reference's loops run."""
+    """gcc -O3 -fopenmp -ffp-contract=off: every operation rounds on its own,
+    in the serial order the reference's loops run; OpenMP only spreads
+    independent cases over threads (orc_run_cases)."""
     os.makedirs(BUILD, exist_ok=True)
     src = os.path.join(HERE, "lica_oracle.c")
     if not force and os.path.exists(LIB) and os.path.getmtime(LIB) >= max(
             os.path.getmtime(src), os.path.getmtime(os.path.join(HERE, "lica_oracle.h"))):
         return LIB
     tmp = LIB + ".tmp"
-    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-std=c11", "-Wall", *extra, src,
+    cmd = ["gcc", "-O3", "-fopenmp", "-fPIC", "-shared", "-ffp-contract=off", "-std=c11", "-Wall", *extra, src,
            "-o", tmp, "-lm"]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
@@ -75,6 +76,9 @@ def load():
         lib.orc_psif_cell.argtypes = [c_dbl, c_dbl]
         lib.orc_psif_cell.restype = c_dbl
         lib.orc_grid_nodes.argtypes = [c_i64, c_dbl, P_dbl]
+        P_i64 = ctypes.POINTER(c_i64)
+        lib.orc_run_cases.argtypes = [ctypes.POINTER(OrcCase), c_i64, c_i64, P_i64, P_i64, P_dbl, P_dbl]
+        lib.orc_run_cases.restype = ctypes.c_int
         _lib = lib
     return _lib
 
@@ -141,3 +145,21 @@ def grid_nodes(n, zl):
     z = np.zeros(n + 1)
     load().orc_grid_nodes(n, zl, _p(z))
     return z
+
+
+def run_cases(cases, nsteps):
+    """Set up and advance every case nsteps steps (or until it stops), one
+    case per OpenMP thread.  Returns (status, steps entered, set-up seconds,
+    stepping seconds, threads)."""
+    lib = load()
+    arr = (OrcCase * len(cases))()
+    for i, c in enumerate(cases):
+        for f, _ in OrcCase._fields_:
+            setattr(arr[i], f, getattr(c, f))
+    st = np.zeros(len(cases), dtype=np.int64)
+    steps = np.zeros(len(cases), dtype=np.int64)
+    t_set, t_step = ctypes.c_double(), ctypes.c_double()
+    P_i64 = ctypes.POINTER(c_i64)
+    thr = lib.orc_run_cases(arr, len(cases), int(nsteps), st.ctypes.data_as(P_i64), steps.ctypes.data_as(P_i64),
+                            ctypes.byref(t_set), ctypes.byref(t_step))
+    return st, steps, t_set.value, t_step.value, int(thr)

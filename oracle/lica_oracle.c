@@ -50,6 +50,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define M_MAX 11
 #define NN1 8       /* N_MAX-3, two_phase.f:744 */
@@ -646,7 +649,7 @@ static double weno_p(const orc_inst *s, int64_t k)
 static void reinit_rk3(orc_inst *s, double redtl, int nn1, int nn2)
 {
     double eps = s->sdzf * s->sdzf;  /* MIN(SDXF,SDYF,SDZF)**2 */
-    static double newv[M_MAX + 1][MAXP];
+    double newv[M_MAX + 1][MAXP];
     if (nn1 != 0) dphi_z(s, nn1);
     double wxm = weno_flat_m(), wxp = weno_flat_p();
     for (int n = 1; n <= nn1; n++)
@@ -863,7 +866,7 @@ static void rein_trigger(orc_inst *s, double *dmax, double *dmin)
 /* TRANSPORT_RK3 two_phase.f:1133-1495 (WENO branch: FUNCBODY=100>CRI) */
 static void transport_rk3(orc_inst *s, double dtlvs, const double (*alphi_old)[MAXP], const int64_t *bn_old)
 {
-    static double newv[M_MAX + 1][MAXP];
+    double newv[M_MAX + 1][MAXP];
     dphi_z(s, NN1);
     double wxm = weno_flat_m();
     for (int n = 1; n <= NN1; n++)
@@ -1795,4 +1798,51 @@ int64_t orc_last_poisson(orc_inst *s, double *a, double *b, double *c, double *g
         if (p) p[k - 1] = D(s->p, k);
     }
     return s->n3m;
+}
+
+/* CPU baseline of the benchmark: nc independent cases, each set up and then
+ * advanced nsteps steps or until it stops, one case per thread (OpenMP,
+ * dynamic schedule over the cases in the order given).  The set-up (geometry,
+ * LVSINIT, GRID_COUPLING(1)) and the steps are timed apart; steps_out[i] is
+ * the number of steps case i entered (the one it stopped in included).
+ * Returns the threads used. */
+int orc_run_cases(const orc_case *cases, int64_t nc, int64_t nsteps, int64_t *status, int64_t *steps_out,
+                  double *setup_s, double *step_s)
+{
+    orc_inst **inst = (orc_inst **)calloc((size_t)(nc > 0 ? nc : 1), sizeof(orc_inst *));
+    int threads = 1;
+#ifdef _OPENMP
+    threads = omp_get_max_threads();
+    double t0 = omp_get_wtime();
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int64_t i = 0; i < nc; i++) {
+        inst[i] = orc_new(&cases[i]);
+        status[i] = inst[i] ? orc_init(inst[i]) : ORC_GRID;
+        steps_out[i] = 0;
+    }
+#ifdef _OPENMP
+    double t1 = omp_get_wtime();
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int64_t i = 0; i < nc; i++) {
+        if (status[i]) continue;
+        for (int64_t it = 0; it < nsteps; it++) {
+            steps_out[i] += 1;
+            status[i] = orc_step(inst[i], 1);
+            if (status[i]) break;
+        }
+    }
+#ifdef _OPENMP
+    double t2 = omp_get_wtime();
+    *setup_s = t1 - t0;
+    *step_s = t2 - t1;
+#else
+    *setup_s = 0.0;
+    *step_s = 0.0;
+#endif
+    for (int64_t i = 0; i < nc; i++)
+        if (inst[i]) orc_free(inst[i]);
+    free(inst);
+    return threads;
 }

@@ -74,6 +74,12 @@ int64_t orc_n(orc_inst *s);
 /* the Poisson inputs (a, b, c, g of POISSON) of the last substep */
 int64_t orc_last_poisson(orc_inst *s, double *a, double *b, double *c, double *g, double *p);
 
+/* CPU baseline: nc cases set up and advanced nsteps steps each (or until
+ * they stop), one case per OpenMP thread; set-up and steps timed apart
+ * (seconds, wall clock); returns the threads used */
+int orc_run_cases(const orc_case *cases, int64_t nc, int64_t nsteps, int64_t *status, int64_t *steps_out,
+                  double *setup_s, double *step_s);
+
 /* unit-level restatements used by the tests */
 void orc_trdiag3(int64_t n, const double *a, const double *b, const double *c,
                  double *g);

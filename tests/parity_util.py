@@ -1,6 +1,8 @@
 """Comparison helpers of the parity tests (library vs oracle)."""
 import numpy as np
 
+from oracle import Oracle
+
 # the fields and scalars both sides expose, in the order they are checked
 FIELDS = ("W", "P", "T", "PSI_CN", "PSI_ZN", "PSIF", "PHI", "SFGAS", "WO", "BAND", "ALPHI", "MFLVS")
 # TIME, DT, STEP, PLACE, DVMAX, CFLM, VOL_ORI, VOL_IPHS, QWALL, STATUS, counters
@@ -31,4 +33,19 @@ def max_rel(batch, inst, orc, fields=("W", "P", "T", "PSI_CN", "PSI_ZN", "PSIF",
             continue
         scale = max(np.max(np.abs(c)), 1e-300)
         out[f] = float(np.max(np.abs(g - c)) / scale) if g.size else 0.0
+    return out
+
+
+def ulp_sensitivity(case, steps, fields):
+    """max relative change of each field when the oracle's own input z0
+    moves by one ulp: the part of a field that rounding decides."""
+    o1, o2 = Oracle(case), Oracle(case.copy(z0=np.nextafter(case.z0, 1.0)))
+    o1.step(steps)
+    o2.step(steps)
+    out = {}
+    for f in fields:
+        x, y = o1.field(f), o2.field(f)
+        out[f] = float(np.max(np.abs(x - y)) / max(np.max(np.abs(x)), 1e-300))
+    o1.close()
+    o2.close()
     return out

@@ -13,7 +13,7 @@ import pytest
 
 import lica_boiling1d_amd as L
 from oracle import Oracle
-from parity_util import exact_mismatches, max_rel
+from parity_util import exact_mismatches, max_rel, ulp_sensitivity
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -130,19 +130,6 @@ def test_block_size_does_not_change_results():
             assert all(np.array_equal(x, y) for x, y in zip(ref, cur))
 
 
-def _ulp_sensitivity(case, steps, fields):
-    """max relative change of each field when the oracle's own input z0
-    moves by one ulp: the part of a field that rounding decides."""
-    o1, o2 = Oracle(case), Oracle(case.copy(z0=np.nextafter(case.z0, 1.0)))
-    o1.step(steps)
-    o2.step(steps)
-    out = {}
-    for f in fields:
-        x, y = o1.field(f), o2.field(f)
-        out[f] = float(np.max(np.abs(x - y)) / max(np.max(np.abs(x)), 1e-300))
-    return out
-
-
 @pytest.mark.parametrize("which,steps", [("deck", 50), ("large", 10)])
 def test_direct_mode_within_tolerance(which, steps):
     """Partitioned tridiagonal solves with double-double refinement.
@@ -160,7 +147,7 @@ def test_direct_mode_within_tolerance(which, steps):
     assert o.step(steps) == 0
     fields = ("W", "P", "T", "PSI_CN", "PSI_ZN", "PSIF", "PHI")
     rel = max_rel(b, 0, o, fields)
-    sens = _ulp_sensitivity(case, steps, fields)
+    sens = ulp_sensitivity(case, steps, fields)
     for f in fields:
         assert rel[f] <= max(1e-10, 10.0 * sens[f]), (f, rel[f], sens[f])
     zo = o.scalar_array()[3]
@@ -224,12 +211,14 @@ def test_checkpoint_restart_bit_exact(tmp_path):
         assert np.array_equal(a.scalar_array(i), b.scalar_array(i))
 
 
-@pytest.mark.parametrize("mode", [L.POISSON_MG64, L.POISSON_MIXED])
+@pytest.mark.parametrize("mode", [L.POISSON_MG64, L.POISSON_MIXED, L.POISSON_PROJECTED])
 @pytest.mark.parametrize("which,steps", [("deck", 60), ("large", 6)])
 def test_multigrid_modes(mode, which, steps):
-    """V-cycles to a backward error of 1e-10 (FP64, or FP32 corrections in
-    FP64 defect correction): every Poisson solve meets the tolerance and the
-    interface stays within 1e-6 relative of the FP64 oracle."""
+    """V-cycles (FP64, or FP32 corrections in FP64 defect correction, from
+    zero or from the history projection) until the backward error and the
+    last correction are both within 1e-10.  The direct mode's bar: every
+    field within 1e-10 relative of the oracle or 10x the oracle's own change
+    under a one-ulp perturbation of its input, the interface within 1e-10."""
     case = L.deck_case() if which == "deck" else L.single_large_case(1)
     _say(f"multigrid mode {mode}, {which} case, {steps} steps")
     b, o = L.Batch([case]), Oracle(case)
@@ -239,9 +228,15 @@ def test_multigrid_modes(mode, which, steps):
     s = b.scalars()
     assert s["POISSON_SOLVES"] == 3 * steps
     assert 0 < s["POISSON_RESID"] <= 1e-10
-    assert s["POISSON_CYCLES"] / s["POISSON_SOLVES"] < 30
+    assert s["POISSON_CYCLES"] / s["POISSON_SOLVES"] <= (2.5 if mode == L.POISSON_MG64 else 6)
+    fields = ("W", "P", "T", "PSI_CN", "PSI_ZN", "PSIF", "PHI")
+    rel = max_rel(b, 0, o, fields)
+    sens = ulp_sensitivity(case, steps, fields)
+    _say("  " + " ".join(f"{f}={rel[f]:.1e}/{sens[f]:.1e}" for f in fields))
+    for f in fields:
+        assert rel[f] <= max(1e-10, 10.0 * sens[f]), (f, rel[f], sens[f])
     zo = o.scalar_array()[3]
-    assert abs(s["PLACE"] - zo) <= 1e-6 * zo
+    assert abs(s["PLACE"] - zo) <= 1e-10 * zo
 
 
 @pytest.mark.parametrize("mode", [L.POISSON_MG64, L.POISSON_MIXED])
@@ -275,7 +270,7 @@ def test_multigrid_modes_equal_logic_check_build(mode):
 def test_batched_poisson_multigrid(mode):
     """lica_poisson by V-cycles on the POISSON systems of the deck case (the
     last substep of steps 1..8): backward error <= 1e-10 and the solution
-    close to TRDIAG3's."""
+    within 1e-12 of TRDIAG3's (whose own error on these rows is about 2e-15)."""
     _say(f"batched poisson multigrid mode {mode}")
     o = Oracle(L.deck_case())
     rows = []
@@ -289,4 +284,4 @@ def test_batched_poisson_multigrid(mode):
         r = g[s] - (b[s] * p[s] + np.r_[0.0, a[s, 1:] * p[s, :-1]] + np.r_[c[s, :-1] * p[s, 1:], 0.0])
         anorm = np.max(np.abs(a[s]) + np.abs(b[s]) + np.abs(c[s]))
         assert np.max(np.abs(r)) <= 1e-10 * (anorm * np.max(np.abs(p[s])) + np.max(np.abs(g[s])))
-        assert np.max(np.abs(p[s] - x[s])) <= 1e-4 * np.max(np.abs(x[s]))
+        assert np.max(np.abs(p[s] - x[s])) <= 1e-12 * np.max(np.abs(x[s]))

@@ -123,12 +123,13 @@ def test_writefield_file(emul, tmp_path):
     assert np.allclose(t[:, 0, 0], b.field("T"), rtol=1e-11, atol=1e-300)
 
 
-@pytest.mark.parametrize("mode", [L.POISSON_MG64, L.POISSON_MIXED])
+@pytest.mark.parametrize("mode", [L.POISSON_MG64, L.POISSON_MIXED, L.POISSON_PROJECTED])
 def test_multigrid_modes_close_to_oracle(emul, mode):
-    """V-cycles to a backward error of 1e-10 on the deck case: every solve
-    meets it, the interface stays within 1e-6 relative of the oracle, and
-    where the levels' x, f live (all in LDS, split, all in global memory)
-    does not change a bit of the result."""
+    """V-cycles on the deck case until the backward error and the last
+    correction are both within 1e-10: every field within 1e-9 relative of the
+    oracle's direct solve (the partitioned direct mode's level on this case),
+    the interface within 1e-12, and where the cycle's vectors live (all in
+    LDS, split, all in global memory) does not change a bit of the result."""
     case = L.deck_case()
     o = Oracle(case)
     assert o.step(8) == 0
@@ -141,44 +142,101 @@ def test_multigrid_modes_close_to_oracle(emul, mode):
         s = b.scalars()
         assert s["POISSON_SOLVES"] == 24
         assert 0 < s["POISSON_RESID"] <= 1e-10
-        assert s["POISSON_CYCLES"] / s["POISSON_SOLVES"] < 30
+        assert s["POISSON_CYCLES"] / s["POISSON_SOLVES"] <= (2 if mode == L.POISSON_MG64 else 5)
         zo = o.scalar_array()[3]
-        assert abs(s["PLACE"] - zo) <= 1e-6 * zo
+        assert abs(s["PLACE"] - zo) <= 1e-12 * zo
+        rel = max_rel(b, 0, o, ("W", "P", "T", "PSI_CN", "PSI_ZN", "PSIF", "PHI"))
+        assert all(v <= 1e-9 for v in rel.values()), rel
         runs.append([b.field(f) for f in ("W", "P", "T", "PHI")])
     for other in runs[1:]:
         assert all(np.array_equal(x, y) for x, y in zip(runs[0], other))
 
 
+def _dense(a, b, c):
+    return np.diag(b) + np.diag(a[1:], -1) + np.diag(c[:-1], 1)
+
+
 @pytest.mark.parametrize("mode", [L.POISSON_MG64, L.POISSON_MIXED])
-@pytest.mark.parametrize("n", [20, 33, 257, 1000])
+@pytest.mark.parametrize("n", [1, 20, 33, 257, 1000])
 def test_batched_poisson_multigrid_cpu(emul, mode, n):
-    """lica_poisson by V-cycles against a dense solve: backward error within
-    1e-10 (one level for n <= 32, odd level sizes above)."""
+    """lica_poisson by V-cycles against a dense solve (one level for n <= 32,
+    odd level sizes above), on random diagonally dominant M-matrices of both
+    sign patterns: the forward error within 1e-12 of the solution's maximum."""
     rng = np.random.default_rng(n)
     nsys = 3
-    a = -rng.uniform(0.5, 1.5, (nsys, n))
-    c = -rng.uniform(0.5, 1.5, (nsys, n))
-    a[:, 0] = 0.0
-    # diagonally dominant M-matrices (the POISSON rows' sign pattern, strict
-    # dominance so that Gauss-Seidel smoothing converges on random coefficients)
-    b = -(a + c) + rng.uniform(0.01, 0.1, (nsys, n))
-    g = rng.standard_normal((nsys, n))
-    p = L.poisson(a, b, c, g, mode=mode, rtol=1e-10, max_cycles=100, lib=emul)
-    for s in range(nsys):
-        A = np.diag(b[s]) + np.diag(a[s, 1:], -1) + np.diag(c[s, :-1], 1)
-        r = g[s] - A @ p[s]
-        anorm = np.max(np.abs(a[s]) + np.abs(b[s]) + np.abs(c[s]))
-        assert np.max(np.abs(r)) <= 1e-10 * (anorm * np.max(np.abs(p[s])) + np.max(np.abs(g[s])))
-        x = np.linalg.solve(A, g[s])
-        assert np.max(np.abs(p[s] - x)) <= 1e-6 * np.max(np.abs(x))
+    for sign in (1.0, -1.0):
+        a = -sign * rng.uniform(0.5, 1.5, (nsys, n))
+        c = -sign * rng.uniform(0.5, 1.5, (nsys, n))
+        a[:, 0] = 0.0
+        b = -(a + c) + sign * rng.uniform(0.01, 0.1, (nsys, n))
+        g = rng.standard_normal((nsys, n))
+        p = L.poisson(a, b, c, g, mode=mode, rtol=1e-10, max_cycles=100, lib=emul)
+        for s in range(nsys):
+            x = np.linalg.solve(_dense(a[s], b[s], c[s]), g[s])
+            assert np.max(np.abs(p[s] - x)) <= 1e-12 * np.max(np.abs(x))
+
+
+def _thomas_longdouble(a, b, c, g):
+    """Thomas elimination in x87 extended precision (64-bit significand)."""
+    a, b, c, g = (np.asarray(v, dtype=np.longdouble) for v in (a, b, c, g))
+    n = len(g)
+    cp, gp = np.zeros(n, np.longdouble), np.zeros(n, np.longdouble)
+    cp[0], gp[0] = c[0] / b[0], g[0] / b[0]
+    for k in range(1, n):
+        m = b[k] - a[k] * cp[k - 1]
+        cp[k] = c[k] / m if k < n - 1 else 0.0
+        gp[k] = (g[k] - a[k] * gp[k - 1]) / m
+    x = np.zeros(n, np.longdouble)
+    x[-1] = gp[-1]
+    for k in range(n - 2, -1, -1):
+        x[k] = gp[k] - cp[k] * x[k + 1]
+    return x.astype(np.float64)
+
+
+@pytest.mark.parametrize("mode,tol", [(L.POISSON_MG64, 1e-14), (L.POISSON_MIXED, 1e-11)])
+def test_poisson_multigrid_nearly_singular_rows(emul, mode, tol):
+    """The POISSON sign pattern with rows that sum to their rounding error
+    (b = -(a + c) in floating point, the interior rows of 1d.f:3535-3541),
+    a density jump of 1605 (1/rho of the vapour and the liquid) and the
+    Dirichlet row on top: the solution within tol of an extended-precision
+    elimination.  (A normwise backward error of 1e-10 alone would allow an
+    error of order 1e-2 here, and TRDIAG3 in FP64 is off by about 2e-8 on
+    these rows because its pivots lose the row sums.)"""
+    rng = np.random.default_rng(5)
+    n = 3000
+    rho = np.where(np.arange(n + 1) < 900, 1.0, 1605.36)
+    coef = (1.0 / rho) * rng.uniform(0.99, 1.01, n + 1)
+    a = coef[:-1].copy()
+    c = coef[1:].copy()
+    a[0] = 0.0
+    b = -(a + c)
+    g = rng.standard_normal(n) * 1e3
+    p = L.poisson(a, b, c, g, mode=mode, rtol=1e-10, max_cycles=100, lib=emul)[0]
+    x = _thomas_longdouble(a, b, c, g)
+    assert np.max(np.abs(p - x)) <= tol * np.max(np.abs(x))
+
+
+def test_poisson_multigrid_nonfinite_rows(emul):
+    """A system with a non-finite right-hand side gives a non-finite solution
+    (as the direct solve does) and is never reported converged with a finite
+    residual."""
+    n = 300
+    a = np.full(n, 1.0)
+    c = np.full(n, 1.0)
+    a[0] = 0.0
+    b = -(a + c) - 0.01
+    g = np.ones(n)
+    g[17] = np.nan
+    p = L.poisson(a, b, c, g, mode=L.POISSON_MIXED, rtol=1e-10, max_cycles=10, lib=emul)
+    assert not np.all(np.isfinite(p))
 
 
 def test_projected_warm_start_cpu(emul):
     """The history projection (LICA_POISSON_PROJECTED) on the deck case: every
-    solve meets the backward-error tolerance, the interface stays within 1e-6
-    relative of the oracle, and once the history holds the pressure field
-    built up over the first steps the V-cycles per solve fall far below the
-    zero start's (7 on this case)."""
+    solve meets the stopping test, the interface stays within 1e-12 relative
+    of the oracle, and once the history holds the pressure field built up
+    over the first steps the V-cycles per solve fall below the zero start's
+    (four on this case)."""
     case = L.deck_case()
     o = Oracle(case)
     assert o.step(30) == 0
@@ -192,7 +250,7 @@ def test_projected_warm_start_cpu(emul):
         s = b.scalars()
         assert 0 < s["POISSON_RESID"] <= 1e-10
         zo = o.scalar_array()[3]
-        assert abs(s["PLACE"] - zo) <= 1e-6 * zo
+        assert abs(s["PLACE"] - zo) <= 1e-12 * zo
         cyc[mode] = (s["POISSON_CYCLES"] - s0["POISSON_CYCLES"]) / (s["POISSON_SOLVES"] - s0["POISSON_SOLVES"])
-    assert cyc[L.POISSON_MIXED] >= 5
-    assert cyc[L.POISSON_PROJECTED] <= 2, cyc
+    assert cyc[L.POISSON_MIXED] >= 3.5
+    assert cyc[L.POISSON_PROJECTED] <= 3, cyc
