@@ -5,35 +5,46 @@ kernel -- base grid N ~ U{128..4096}, superheat U(2,20) K, film thickness
 U(0.05,0.5) mm, refinement r=2, drawn from SplitMix64 seed 20261202.  A
 "step" is one full RK3 time step of every case (1d.f:196-295); the value is
 completed instance-steps/s with the state resident in HBM, every Poisson
-solve taken by V-cycles to a normwise backward error of 1e-10.
+solve taken by V-cycles until its normwise backward error and its last
+correction are both within 1e-10.
 
-Three timed runs of the same batch from the same initial state, each over
-the same window of steps after the same untimed warmup (past the start-up
-transient in which the pressure field builds up from zero):
+Timed runs of the same batch from the same initial state, each over the same
+window of steps after the same untimed warmup:
   mixed+learned    FP32 V-cycle corrections in FP64 defect correction,
                    started from the history projection corrected by the
                    warm-start network (the value);
   mixed+projected  the same without the network (what the network adds);
-  plain FP64       FP64 V-cycles from a zero initial guess (IOLDV=0).
+  plain FP64       FP64 V-cycles from a zero initial guess (IOLDV=0);
+  thomas           the reference's own direct solve (TRDIAG3 order, bit for
+                   bit the FP64 oracle): the yardstick for which cases stop.
 Before them the network is trained on the GPU on pairs the solver generates
 from a disjoint batch (seed 20261203); the training is not timed.
 
+Every run reports its stopped cases (the reference's STOP branches) and how
+they compare with the Thomas run's, case by case (status and step), and the
+interface position's largest relative difference from the Thomas run over
+every case alive there (a case alive there but stopped here counts apart).
+
 The JSON line also carries
-  roofline      the dominant kernel's algorithmic bytes per launch over its
-                mean launch time (HIP events between the launches on each
-                lane's stream, over the timed steps), against 8 TB/s; with
-                four lanes (the cases split into four sets whose launch
-                sequences run on four streams at once) a launch covers a
-                quarter of the cases, and bytes and times are both per
-                launch;
-  cpu_baseline  the oracle (serial FP64 restatement, one core) on a bounded
-                sample of the same cases.
+  roofline      the dominant kernel of the headline mode: its algorithmic
+                bytes per launch (distinct arrays x elements x size, twice
+                when read and written) over its exclusive launch duration,
+                taken with HIP events on a one-lane run of the same batch
+                and steps (one launch per kernel and substep, no concurrent
+                launches), against 8 TB/s; `traffic` is the HBM bytes per
+                launch that rocprofv3's FETCH_SIZE/WRITE_SIZE counters gave
+                for that kernel on this workload, read from the committed
+                profile summary (null when there is none);
+  cpu_baseline  the FP64 oracle (the serial restatement, built -O3 -fopenmp)
+                on the GPU box's host cores: one case per OpenMP thread over
+                a bounded sample of the same batch, set-up excluded.
 ``--workload large`` / ``deck`` and ``--mode`` select the single large case,
 the shipped deck and the other Poisson modes.
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -50,13 +61,24 @@ if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
     os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 HBM_PEAK_GBS = 8000.0
+PROFILES = os.path.join(ROOT, "profiles")
 
 
-def kernel_bytes(case, kernel, mode, vcycles=0.0):
+def mg_rows_total(n, coarse=32):
+    """rows of every level of the V-cycle hierarchy (levels halve down to 32 rows)"""
+    tot, m = n, n
+    while m > coarse:
+        m = (m + 1) // 2
+        tot += m
+    return tot
+
+
+def kernel_bytes(case, kernel, mode):
     """Algorithmic HBM bytes of one launch for one case: every distinct
-    global array the kernel touches, elements touched x 8 B (4 B for int
-    arrays), twice when read and written.  Band-sized work (a few dozen
-    refined planes) is counted with its own arrays."""
+    global array the kernel touches, elements touched x element size, twice
+    when read and written (ghost cells, reuse through caches and re-reads
+    not counted).  Band-sized work (a few dozen refined planes) is counted
+    with its own arrays."""
     n = int(case.n)
     per_cell = {
         # PSI_CN read, RK3ZO written
@@ -67,19 +89,23 @@ def kernel_bytes(case, kernel, mode, vcycles=0.0):
         # W,PSI_Z,PSI_C,RK3ZO r/w; PSI_CN,DF_Z,VF_Z,SSDZ,VVDZ,P,PSI_ZN,AKW,CKW read;
         # IMASK (int); KCOU1/KCOU2 (int); tridiagonal work r/w; Poisson matrix written
         "momentum": 4 * 16 + 9 * 8 + 4 + 8 + 4 * 16 + 4 * 8,
-        # Thomas: a r/w (pivots), b,c read, g r/w, P written; partitioned: a,b,c,g r/w;
-        # multigrid: read a,b,c,g and write P, build the level hierarchy (about
-        # 2n rows of a,b,c,x,f,r), then per V-cycle 384 B per fine cell in FP64
-        # (2+2 smoothing sweeps, residual, transfers over all levels), half of
-        # that for the FP32 cycles plus the FP64 defect correction (32 B)
-        "poisson": ((16 + 8 + 8 + 16 + 8) if mode == 0 else
-                    (4 * 16 + 8) if mode == 1 else
-                    int(40 + 2 * 48 + vcycles * (384 if mode == 2 else 192 + 32))),
         # W,P,WO r/w; PSI_Z,VVDZ,SSDZ read; KCOU1/KCOU2; WT written
         "ucalc": 3 * 16 + 3 * 8 + 8 + 8,
     }
     if kernel in per_cell:
         return per_cell[kernel] * n
+    if kernel == "poisson":
+        if mode == 0:   # Thomas: a r/w (pivots), b,c read, g r/w, P written
+            return (16 + 8 + 8 + 16 + 8) * n
+        if mode == 1:   # partitioned: a,b,c,g r/w, P written
+            return (4 * 16 + 8) * n
+        # V-cycles: a,b,c,g read, P written, the two iterate buffers r/w,
+        # the coarse levels' setup rows (4 doubles) r/w, the cycle rows of
+        # every level (4 values of the cycle's precision) r/w, the warm-start
+        # guess read
+        tot = mg_rows_total(n)
+        cyc = 32 if mode == 2 else 16
+        return (4 * 8 + 8 + 2 * 16) * n + 2 * 32 * (tot - n) + 2 * cyc * tot + (8 * n if mode in (4, 5) else 0)
     # band kernels: 11 layers of <=16 planes, refined window of ~200 planes of
     # PSIF/SFGAS/geometry, NS window of ~100 cells of T/PSI_CN/geometry
     r = int(case.r)
@@ -91,45 +117,58 @@ def kernel_bytes(case, kernel, mode, vcycles=0.0):
     return 0
 
 
-def cpu_baseline(cases, seconds=15.0):
-    """Oracle (serial FP64 restatement, one core) on a bounded sample: the
-    cases in order, a few steps each (every step of a single case), until
-    `seconds` have passed."""
-    from oracle import Oracle
-    per_case = 3 if len(cases) > 1 else 100000
-    t0 = time.perf_counter()
-    steps, ncase, cells = 0, 0, 0
-    for c in cases:
-        o = Oracle(c)
-        ncase += 1
-        for _ in range(per_case):
-            st = o.step(1)
-            steps += 1
-            cells += int(c.n)
-            if st != 0 or time.perf_counter() - t0 >= seconds:
-                break
-        o.close()
-        if time.perf_counter() - t0 >= seconds:
-            break
-    el = time.perf_counter() - t0
+def host_info():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "cpus_available": avail,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_baseline(cases, workload, steps=3):
+    """The FP64 oracle on the host cores (gcc -O3 -fopenmp -ffp-contract=off).
+    Batch: mode (ii) of the reference-style comparison -- one case per OpenMP
+    thread over every case of the batch for `steps` steps, set-up excluded.
+    A single case: its steps on one core (its loops are serial)."""
+    from oracle.binding import run_cases
+    info = host_info()
     if len(cases) == 1:
-        sample = f"{steps} steps of the same case (N={cases[0].n}, r={cases[0].r}) on one host core, {el:.1f} s"
-    else:
-        sample = (f"first {ncase} cases of the same batch, up to {per_case} steps each ({steps} instance-steps, "
-                  f"mean N={cells / max(steps, 1):.0f}) on one host core, {el:.1f} s, case set-up included")
-    return {"value": steps / el, "unit": "instance-steps/s", "cores": 1, "kind": "port", "sample": sample}
+        st, done, t_set, t_step, thr = run_cases(cases * 1, 200 if cases[0].n <= 1024 else 20)
+        v = float(done.sum()) / t_step
+        return {"value": v, "unit": "instance-steps/s", "cores": 1, "kind": "port",
+                "sample": f"{int(done.sum())} steps of the {workload} (N={cases[0].n}, r={cases[0].r}) by the "
+                          f"FP64 oracle on one host core, {t_step:.1f} s, set-up excluded",
+                **info}
+    st, done, t_set, t_step, thr = run_cases(cases, steps)
+    v = float(done.sum()) / t_step
+    return {"value": v, "unit": "instance-steps/s", "cores": thr, "kind": "port",
+            "sample": f"all {len(cases)} cases of the same batch, {steps} steps each ({int(done.sum())} "
+                      f"instance-steps, {int((st != 0).sum())} cases stopped) by the FP64 oracle, one case per "
+                      f"OpenMP thread on {thr} threads, {t_step:.1f} s of stepping; set-up ({t_set:.1f} s) "
+                      "excluded",
+            "mode": "instance-parallel", **info}
 
 
 def lanes_of(cases, args):
-    """Case sets run side by side on as many streams (2..4) or one sequence (1)."""
+    """Case sets run side by side on as many streams (2..8) or one sequence (1)."""
     return args.lanes if args.lanes else (LANES_BATCH if len(cases) >= 256 else 1)
 
 
 def timed_run(L, cases, mode, args, net=None):
-    """One timed run: warmup steps, then args.steps steps with per-kernel HIP
-    events; only the steps the cases completed are counted (a case that
-    reaches one of the reference's STOP conditions keeps its status and is
-    left behind)."""
+    """Warmup steps, then args.steps timed steps (host clock, synchronised
+    on both sides); only the steps the cases completed are counted (a case
+    that reaches one of the reference's STOP conditions keeps its status and
+    is left behind)."""
     import numpy as np
     b = L.Batch(cases)
     if net is not None:
@@ -140,30 +179,104 @@ def timed_run(L, cases, mode, args, net=None):
     if args.warmup:
         b.run_all(args.warmup)
     b.sync()
-    s0 = [b.scalar_array(i) for i in range(len(cases))]
+    s0 = np.array([b.scalar_array(i) for i in range(len(cases))])
     _, steps0 = b.progress()
-    b.set_timing(True)
     b.sync()
     t0 = time.perf_counter()
     nbad = b.run_all(args.steps)
     b.sync()
     el = time.perf_counter() - t0
-    tim = b.timings()
     st, steps1 = b.progress()
-    s1 = [b.scalar_array(i) for i in range(len(cases))]
+    s1 = np.array([b.scalar_array(i) for i in range(len(cases))])
     b.close()
     done = steps1 - steps0
     ok = (st == 0)
-    res = {"el": el, "done": int(done.sum()), "value": float(done.sum()) / el, "timings": tim,
+    res = {"el": el, "done": int(done.sum()), "value": float(done.sum()) / el,
            "cells": float(sum(int(c.n) * int(d) for c, d in zip(cases, done))) / el,
-           "ok": ok, "place": np.array([s[3] for s in s1]), "nbad": nbad, "status": st}
-    if mode >= L.POISSON_MG64:
+           "ok": ok, "place": s1[:, 3], "nbad": nbad, "status": st, "steps": steps1}
+    if mode >= L.POISSON_MG64 and ok.any():
         ci, si, ri = (L.SCALARS.index(k) for k in ("POISSON_CYCLES", "POISSON_SOLVES", "POISSON_RESID"))
-        cyc = sum(a[ci] - z[ci] for a, z, g in zip(s1, s0, ok) if g)
-        sol = sum(a[si] - z[si] for a, z, g in zip(s1, s0, ok) if g)
-        res["vcycles_per_solve"] = cyc / max(sol, 1.0)
-        res["resid_max"] = max((a[ri] for a, g in zip(s1, ok) if g), default=0.0)
+        res["vcycles_per_solve"] = float((s1[ok, ci] - s0[ok, ci]).sum() / max((s1[ok, si] - s0[ok, si]).sum(), 1.0))
+        res["resid_max"] = float(s1[ok, ri].max())
     return res
+
+
+def kernel_profile(L, cases, mode, args, net=None):
+    """Exclusive GPU ms per launch of every kernel class: the same batch and
+    steps on one lane with HIP events between the launches (one launch per
+    kernel and substep, nothing running beside it)."""
+    b = L.Batch(cases)
+    if net is not None:
+        b.set_net(net)
+    b.set_poisson(mode, args.rtol, 200)
+    b.set_block(args.threads)
+    b.set_lanes(1)
+    if args.warmup:
+        b.run_all(args.warmup)
+    b.sync()
+    b.set_timing(True)
+    b.run_all(args.steps)
+    b.sync()
+    tim = b.timings()
+    st, _ = b.progress()
+    b.close()
+    launches = {"dt": 1, "cont_energy": 3, "temp_grad": 3, "momentum": 3, "poisson": 3, "poisson_mg": 3, "ucalc": 3,
+                "transport": 3, "warm_start": 3}
+    per = {k: tim.get(k, 0.0) / (args.steps * nl) for k, nl in launches.items()}
+    if mode >= L.POISSON_MG64:
+        per["poisson"] = per.pop("poisson_mg")
+    else:
+        per.pop("poisson_mg")
+    return per, (st == 0)
+
+
+def profile_traffic(kernel, workload_key):
+    """HBM bytes per launch of `kernel` on this workload from the committed
+    rocprofv3 counter summary (profiles/<kernel>_<workload>_hbm_traffic.md,
+    line 'traffic_bytes_per_launch: <bytes>'), or None."""
+    path = os.path.join(PROFILES, f"{kernel}_{workload_key}_hbm_traffic.md")
+    try:
+        with open(path) as f:
+            m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+def roofline(L, cases, mode, per, alive, workload_key):
+    """Dominant kernel by exclusive launch time: algorithmic bytes of one
+    launch (the cases still running) over that time."""
+    dom = max((k for k in per if k != "warm_start"), key=lambda k: per[k])
+    live = [c for c, g in zip(cases, alive) if g]
+    bytes_launch = sum(kernel_bytes(c, dom, mode) for c in live)
+    ach = bytes_launch / (per[dom] * 1e-3) / 1e9
+    kname = "k_" + ("poisson_mg" if dom == "poisson" and mode >= L.POISSON_MG64 else
+                    "poisson_direct" if dom == "poisson" else dom)
+    return {"bound": "hbm", "kernel": kname, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS, "traffic": profile_traffic(kname, workload_key),
+            "ms_per_launch": per[dom], "bytes_per_launch": bytes_launch,
+            "timing": "exclusive launch duration, HIP events, one lane"}
+
+
+def stopped(L, res):
+    import numpy as np
+    st = res["status"]
+    codes, counts = np.unique(st[st != 0], return_counts=True)
+    return {L.status_string(int(k)): int(v) for k, v in zip(codes, counts)}
+
+
+def versus_thomas(res, ref):
+    """Stops and interface of a run against the Thomas run of the same batch."""
+    import numpy as np
+    same = (res["status"] == ref["status"]) & (res["steps"] == ref["steps"])
+    alive_ref = ref["status"] == 0
+    both = alive_ref & (res["status"] == 0)
+    rel = np.abs(res["place"][both] - ref["place"][both]) / np.abs(ref["place"][both])
+    diff = np.where(~same)[0]
+    return {"cases_with_other_stop_than_thomas": int(diff.size),
+            "those_cases": [int(i) for i in diff[:20]],
+            "alive_in_thomas_but_stopped_here": int((alive_ref & (res["status"] != 0)).sum()),
+            "interface_rel_diff_max_over_cases_alive_in_both": float(rel.max()) if rel.size else None}
 
 
 def train_net(L, args):
@@ -182,27 +295,14 @@ def train_net(L, args):
                  "collect_s": t1 - t0, "train_s": t2 - t1}
 
 
-def roofline(L, cases, mode, args, res):
-    """Dominant kernel of a timed run: algorithmic bytes per launch (the cases
-    still running) over its mean launch time."""
-    launches = {"dt": 1, "cont_energy": 3, "temp_grad": 3, "momentum": 3, "poisson": 3, "ucalc": 3, "transport": 3,
-                "warm_start": 3}
-    tim = dict(res["timings"])
-    if mode >= L.POISSON_MG64:
-        tim["poisson"] = tim.get("poisson_mg", 0.0)
-    # with L lanes every kernel is launched L times per substep, each launch
-    # on 1/L of the cases: ms per launch and bytes per launch both divide by L
-    nla = lanes_of(cases, args)
-    per = {k: tim.get(k, 0.0) / (args.steps * nl * nla) for k, nl in launches.items()}   # ms per launch
-    # the warm-start slot (projection, network) is reported, not rated
-    dom = max((k for k in per if k != "warm_start"), key=lambda k: per[k])
-    vc = res.get("vcycles_per_solve", 0.0)
-    live = [c for c, g in zip(cases, res["ok"]) if g]
-    bytes_launch = sum(kernel_bytes(c, dom, mode, vc) for c in live) / nla
-    ach = bytes_launch / (per[dom] * 1e-3) / 1e9
-    return ({"bound": "hbm", "kernel": "k_" + ("poisson_mg" if dom == "poisson" and mode >= L.POISSON_MG64 else dom),
-             "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
-             "ms_per_launch": per[dom], "bytes_per_launch": bytes_launch}, per)
+def run_summary(L, res, args):
+    out = {"value": res["value"], "ms_per_step": res["el"] * 1000.0 / args.steps}
+    if "vcycles_per_solve" in res:
+        out["vcycles_per_solve"] = res["vcycles_per_solve"]
+        out["poisson_backward_error_max"] = res["resid_max"]
+    if res["nbad"]:
+        out["stopped_cases"] = stopped(L, res)
+    return out
 
 
 def main():
@@ -215,14 +315,14 @@ def main():
     ap.add_argument("--mode", choices=["learned", "projected", "direct", "thomas", "mg64", "mixed"], default="learned")
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--threads", type=int, default=256)
-    ap.add_argument("--lanes", type=int, default=0, help="case sets on as many streams (1..4); 0: 4 for >= 256 cases")
+    ap.add_argument("--lanes", type=int, default=0, help="case sets on as many streams (1..8); 0: 4 for >= 256 cases")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--train-cases", type=int, default=4096)
     ap.add_argument("--train-steps", type=int, default=25)
     ap.add_argument("--train-windows", type=int, default=1)
     ap.add_argument("--train-iters", type=int, default=3000)
-    ap.add_argument("--no-fp64", action="store_true", help="skip the plain-FP64 comparison run")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the comparison runs")
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -230,29 +330,31 @@ def main():
               "independent, so more GPUs would shard the batch by case with no exchange.", file=sys.stderr)
         sys.exit(2)
     if args.steps is None:
-        args.steps = 60 if args.workload == "batch" else 200
+        args.steps = 20 if args.workload == "batch" else 200
     if args.warmup is None:
-        args.warmup = 20
+        args.warmup = 5
 
-    import numpy as np
     import lica_boiling1d_amd as L
     if args.workload == "large":
         cases = [L.single_large_case(seed=1)]
         wl = {"workload": "single large case", "n": 8192, "r": 4, "superheat_K": 10.0, "film_mm": 0.2,
               "perturbation": "1% sin, seed 1", "instances": 1}
+        wkey = "large"
     elif args.workload == "deck":
         cases = [L.deck_case()]
         wl = {"workload": "shipped deck case", "n": 256, "r": 2, "instances": 1}
+        wkey = "deck"
     else:
         cases = L.batched_cases(args.batch)
         wl = {"workload": "batched cases", "instances": args.batch, "n": "U{128..4096}", "r": 2,
               "superheat_K": "U(2,20)", "film_mm": "U(0.05,0.5)", "seed": 20261202}
+        wkey = f"batch{args.batch}"
     mode = {"learned": L.POISSON_LEARNED, "projected": L.POISSON_PROJECTED, "direct": L.POISSON_DIRECT,
             "thomas": L.POISSON_THOMAS, "mg64": L.POISSON_MG64, "mixed": L.POISSON_MIXED}[args.mode]
     wl["poisson"] = args.mode
     wl["lanes"] = lanes_of(cases, args)
     if mode >= L.POISSON_MG64:
-        wl["poisson_backward_error"] = args.rtol
+        wl["poisson_stop"] = f"backward error <= {args.rtol:g} and last correction <= {args.rtol:g} max|p|"
 
     def say(m):
         print("bench: " + m, file=sys.stderr, flush=True)
@@ -271,55 +373,54 @@ def main():
         print("bench: no case advanced", file=sys.stderr)
         sys.exit(1)
     say(f"{res['value']:.1f} instance-steps/s")
+    dtype = ("f64+f32+bf16" if mode == L.POISSON_LEARNED else
+             "f64+f32" if mode in (L.POISSON_MIXED, L.POISSON_PROJECTED) else "f64")
     out = {"metric": "instance-steps/s at Poisson resid 1e-10: mixed+learned vs plain FP64 (1 GPU)",
            "value": res["value"], "unit": "instance-steps/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": res["el"] * 1000.0 / args.steps, "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None,
-           "dtype": ("f64+f32+bf16" if mode == L.POISSON_LEARNED else
-                     "f64+f32" if mode in (L.POISSON_MIXED, L.POISSON_PROJECTED) else "f64"),
-           "data": "synthetic", "config": wl, "cell_updates_per_s": res["cells"]}
+           "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": wl,
+           "cell_updates_per_s": res["cells"]}
     if "vcycles_per_solve" in res:
         out["vcycles_per_solve"] = res["vcycles_per_solve"]
         out["poisson_backward_error_max"] = res["resid_max"]
-    out["roofline"], out["kernel_ms_per_launch"] = roofline(L, cases, mode, args, res)
     if res["nbad"]:
-        st = res["status"]
-        codes, counts = np.unique(st[st != 0], return_counts=True)
-        out["stopped_cases"] = {L.status_string(int(k)): int(v) for k, v in zip(codes, counts)}
+        out["stopped_cases"] = stopped(L, res)
+    say("one-lane kernel profile")
+    per, alive = kernel_profile(L, cases, mode, args, net)
+    out["roofline"] = roofline(L, cases, mode, per, alive, wkey)
+    out["kernel_ms_per_launch"] = per
     if train is not None:
         out["training"] = train
-    if mode == L.POISSON_LEARNED and not args.no_fp64:
-        say("timed run, mixed+projected (no network)")
-        pr = timed_run(L, cases, L.POISSON_PROJECTED, args)
-        out["mixed_projected"] = {"value": pr["value"], "ms_per_step": pr["el"] * 1000.0 / args.steps,
-                                  "vcycles_per_solve": pr.get("vcycles_per_solve"),
-                                  "poisson_backward_error_max": pr.get("resid_max"),
-                                  "kernel_ms_per_launch": roofline(L, cases, L.POISSON_PROJECTED, args, pr)[1]}
-        if pr["nbad"]:
-            st = pr["status"]
-            codes, counts = np.unique(st[st != 0], return_counts=True)
-            out["mixed_projected"]["stopped_cases"] = {L.status_string(int(k)): int(v) for k, v in zip(codes, counts)}
-        out["network_gain_vs_projected"] = res["value"] / pr["value"] if pr["value"] > 0 else None
-        out["vcycles_saved_by_network_per_solve"] = pr.get("vcycles_per_solve", 0.0) - res.get("vcycles_per_solve", 0.0)
-    if mode >= L.POISSON_MIXED and not args.no_fp64:
-        say("timed run, plain FP64 V-cycles from zero")
-        ref = timed_run(L, cases, L.POISSON_MG64, args)
-        both = res["ok"] & ref["ok"]
-        rel = np.abs(res["place"][both] - ref["place"][both]) / np.abs(ref["place"][both])
-        out["plain_fp64"] = {"value": ref["value"], "ms_per_step": ref["el"] * 1000.0 / args.steps,
-                             "vcycles_per_solve": ref.get("vcycles_per_solve"),
-                             "poisson_backward_error_max": ref.get("resid_max"),
-                             "kernel_ms_per_launch": roofline(L, cases, L.POISSON_MG64, args, ref)[1]}
-        if ref["nbad"]:
-            st = ref["status"]
-            codes, counts = np.unique(st[st != 0], return_counts=True)
-            out["plain_fp64"]["stopped_cases"] = {L.status_string(int(k)): int(v) for k, v in zip(codes, counts)}
-        out["speedup_vs_plain_fp64"] = res["value"] / ref["value"] if ref["value"] > 0 else None
-        out["vcycles_saved_per_solve"] = ref.get("vcycles_per_solve", 0.0) - res.get("vcycles_per_solve", 0.0)
-        out["interface_rel_diff_vs_plain_fp64_max"] = float(rel.max()) if rel.size else None
+    if not args.no_fp64:
+        runs = {}
+        if mode == L.POISSON_LEARNED:
+            say("timed run, mixed+projected (no network)")
+            runs["mixed_projected"] = timed_run(L, cases, L.POISSON_PROJECTED, args)
+        if mode != L.POISSON_MG64:
+            say("timed run, plain FP64 V-cycles from zero")
+            runs["plain_fp64"] = timed_run(L, cases, L.POISSON_MG64, args)
+        if mode != L.POISSON_THOMAS:
+            say("timed run, Thomas (the reference's direct solve)")
+            runs["thomas"] = timed_run(L, cases, L.POISSON_THOMAS, args)
+        ref = runs.get("thomas", res)
+        for k, r in runs.items():
+            out[k] = run_summary(L, r, args)
+        if "mixed_projected" in runs:
+            pr = runs["mixed_projected"]
+            out["network_gain_vs_projected"] = res["value"] / pr["value"] if pr["value"] > 0 else None
+            out["vcycles_saved_by_network_per_solve"] = pr.get("vcycles_per_solve", 0.0) - res.get(
+                "vcycles_per_solve", 0.0)
+        if "plain_fp64" in runs:
+            pf = runs["plain_fp64"]
+            out["speedup_vs_plain_fp64"] = res["value"] / pf["value"] if pf["value"] > 0 else None
+            out["vcycles_saved_per_solve"] = pf.get("vcycles_per_solve", 0.0) - res.get("vcycles_per_solve", 0.0)
+        out["versus_thomas"] = {"this_run": versus_thomas(res, ref)}
+        for k, r in runs.items():
+            if k != "thomas":
+                out["versus_thomas"][k] = versus_thomas(r, ref)
     if not args.no_cpu:
         say("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(cases, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(cases, wl["workload"], args.cpu_steps)
     print(json.dumps(out), flush=True)
 
 

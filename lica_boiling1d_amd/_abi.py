@@ -142,6 +142,16 @@ def load(path=None):
     return _lib
 
 
+# statuses a single case can end with (the reference's STOPs and a Poisson
+# solve that hits its cycle cap); 20..31 are errors of the call itself
+CASE_STATUSES = frozenset(range(1, 20)) | {32}
+
+
+def is_case_status(status):
+    """True for a status that a case of a batch ends with (not a call error)."""
+    return int(status) in CASE_STATUSES
+
+
 def status_string(status):
     try:
         return load().lica_status_string(int(status)).decode()

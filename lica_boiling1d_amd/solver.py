@@ -182,7 +182,9 @@ class Batch(_Handle):
         reference STOPs there) keeps its status and is left behind.
         Returns the number of cases with a non-zero status."""
         rc = self._lib.lica_batch_run(self._h, int(nsteps))
-        if rc not in (0,) and rc < 20:
+        if rc != 0 and _abi.is_case_status(rc):
+            # the status of the first stopped case (a reference STOP or a
+            # Poisson solve at its cycle cap); the others are in progress()
             st, _ = self.progress()
             return int(np.count_nonzero(st))
         _check(rc, "lica_batch_run")
