@@ -305,6 +305,21 @@ def run_summary(L, res, args):
     return out
 
 
+def calibrate_copy(mib):
+    """A plain device-to-device copy of `mib` MiB (torch's copy kernel), five
+    times: each dispatch reads and writes exactly mib MiB, so the ratio of
+    a counter's reading to that size scales the same counter elsewhere."""
+    import torch
+    n = mib * (1 << 20) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes": n * 8, "dispatches": 5}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -324,7 +339,13 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the comparison runs")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device 5 times (the known-size kernel that scales the HBM "
+                         "byte counters of a profile) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the cases of a batch are "
               "independent, so more GPUs would shard the batch by case with no exchange.", file=sys.stderr)
