@@ -66,6 +66,7 @@ enum {
     LICA_BAD_ARG = 20,
     LICA_UNSUPPORTED_DECK = 21,
     LICA_IO = 22,
+    LICA_TOO_LARGE = 23,           /* beyond the fixed workspace of lica_poisson_noalloc */
     LICA_HIP_ERROR = 30,
     LICA_NO_DEVICE = 31,
     LICA_NOT_CONVERGED = 32        /* Poisson iteration hit its cycle cap */
@@ -216,6 +217,21 @@ int lica_end_step(lica_batch *ctx);   /* TIME=TIME+DT (1d.f:260) */
 int lica_poisson(int64_t nsys, int64_t n, const double *a, const double *b, const double *c,
                  const double *g, double *p, int64_t mode, double rtol, int64_t max_cycles,
                  int64_t *cycles_out);
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): lica_poisson on DEVICE arrays a, b, c, g -> p (nsys x n,
+ * contiguous per system, the same meaning and results as lica_poisson) with
+ * no allocation of any kind.  The kernels' workspace is one fixed 64 MiB
+ * device array per GPU, held until the process ends: the multigrid modes fit
+ * nsys * (11 * n + a few) doubles, the direct modes 4 * nsys * n doubles;
+ * larger requests return LICA_TOO_LARGE.  Work goes to `stream` (a
+ * hipStream_t; NULL for the null stream) and the call returns when it is
+ * done.  Every call on one GPU uses that GPU's workspace: calls that overlap
+ * on one GPU race.                                                          */
+int lica_poisson_noalloc(int64_t nsys, int64_t n, const double *a, const double *b, const double *c,
+                         const double *g, double *p, int64_t mode, double rtol, int64_t max_cycles,
+                         int64_t *cycles_out, void *stream);
 
 const char *lica_status_string(int status);
 

@@ -102,6 +102,8 @@ def _declare(lib):
         "lica_grid_coupling": (ctypes.c_int, [V, c_i64, P_dbl, P_dbl]),
         "lica_end_step": (ctypes.c_int, [V]),
         "lica_poisson": (ctypes.c_int, [c_i64, c_i64, P_dbl, P_dbl, P_dbl, P_dbl, P_dbl, c_i64, c_dbl, c_i64, P_i64]),
+        # device addresses in and out, a hipStream_t (or None)
+        "lica_poisson_noalloc": (ctypes.c_int, [c_i64, c_i64, V, V, V, V, V, c_i64, c_dbl, c_i64, P_i64, V]),
         "lica_status_string": (ctypes.c_char_p, [ctypes.c_int]),
         "lica_batch_set_net": (ctypes.c_int, [V, V]),
         "lica_batch_set_capture": (ctypes.c_int, [V, c_i64]),

@@ -65,6 +65,7 @@ int dmalloc(void **p, size_t n) { *p = calloc(1, n ? n : 1); return *p ? 0 : LIC
 void dfree(void *p) { free(p); }
 int h2d(void *d, const void *h, size_t n, hipStream_t) { memcpy(d, h, n); return 0; }
 int d2h(void *h, const void *d, size_t n, hipStream_t) { memcpy(h, d, n); return 0; }
+int d2d(void *d, const void *s, size_t n, hipStream_t) { memmove(d, s, n); return 0; }
 int dsync(hipStream_t) { return 0; }
 int have_device() { return 1; }
 #else
@@ -88,6 +89,11 @@ int d2h(void *h, const void *d, size_t n, hipStream_t s)
 {
     if (hipMemcpyAsync(h, d, n, hipMemcpyDeviceToHost, s) != hipSuccess) return LICA_HIP_ERROR;
     return hipStreamSynchronize(s) == hipSuccess ? 0 : LICA_HIP_ERROR;
+}
+// device-to-device copy, ordered on the stream (no wait)
+int d2d(void *d, const void *s, size_t n, hipStream_t st)
+{
+    return hipMemcpyAsync(d, s, n, hipMemcpyDeviceToDevice, st) == hipSuccess ? 0 : LICA_HIP_ERROR;
 }
 int dsync(hipStream_t s) { return hipStreamSynchronize(s) == hipSuccess ? 0 : LICA_HIP_ERROR; }
 int have_device()
@@ -702,6 +708,7 @@ const char *lica_status_string(int s)
     case LICA_HIP_ERROR: return "HIP runtime error";
     case LICA_NO_DEVICE: return "no GPU device";
     case LICA_NOT_CONVERGED: return "Poisson iteration hit its cycle cap";
+    case LICA_TOO_LARGE: return "systems exceed the fixed workspace of the allocation-free entry point";
     }
     return "unknown status";
 }
@@ -1422,6 +1429,63 @@ extern "C" int lica_poisson(int64_t nsys, int64_t n, const double *a, const doub
 #ifndef LICA_EMUL
     (void)hipStreamDestroy(st);
 #endif
+    if (cycles_out) *cycles_out = maxcyc;
+    return rc;
+}
+
+// The same solves without any allocation: device arrays in and out, the
+// kernels' workspace in the fixed per-GPU array (lica_noalloc_scratch), the
+// caller's stream.  Nothing in the workspace needs to start at zero: the
+// kernels write every element before they read it.  Calls that overlap on
+// one GPU share the workspace and race.
+extern "C" int lica_poisson_noalloc(int64_t nsys, int64_t n, const double *a, const double *b, const double *c,
+                                    const double *g, double *p, int64_t mode, double rtol, int64_t max_cycles,
+                                    int64_t *cycles_out, void *stream)
+{
+    if (nsys <= 0 || n < 1 || !a || !b || !c || !g || !p) return LICA_BAD_ARG;
+    if (mode < LICA_POISSON_THOMAS || mode > LICA_POISSON_MIXED) return LICA_UNSUPPORTED_DECK;
+    if (mode >= LICA_POISSON_MG64 && (!(rtol > 0.0) || max_cycles < 1)) return LICA_BAD_ARG;
+    if (nsys > INT32_MAX || n > INT32_MAX) return LICA_TOO_LARGE;
+    if (!have_device()) return LICA_NO_DEVICE;
+    double *scr = nullptr;
+    if (lica_noalloc_scratch(&scr)) return LICA_HIP_ERROR;
+    const size_t cap = LICA_NOALLOC_SCRATCH_BYTES / sizeof(double);
+    const size_t tot = (size_t)nsys * (size_t)n, bytes = tot * sizeof(double);
+    hipStream_t st = (hipStream_t)stream;
+    int64_t maxcyc = 0;
+    int rc = 0;
+    if (mode >= LICA_POISSON_MG64) {
+        const size_t sc = (size_t)lica_mg_scratch_doubles((int)n);
+        if (sc > cap / (size_t)nsys || (size_t)nsys * (sc + 2) > cap) return LICA_TOO_LARGE;
+        double *dmg = scr, *dres = scr + (size_t)nsys * sc;
+        int *dcyc = (int *)(dres + (size_t)nsys);
+        if (lica_launch_mg((int)nsys, (int)n, a, b, c, g, p, dmg, (int)mode, rtol, (int)max_cycles, dcyc, dres, 256,
+                           (void *)st))
+            return LICA_HIP_ERROR;
+        // the cycle counts come back through a fixed host buffer
+        int buf[256];
+        bool capped = false;
+        for (int64_t i0 = 0; i0 < nsys && !rc; i0 += 256) {
+            const int64_t m = std::min<int64_t>(256, nsys - i0);
+            rc = d2h(buf, dcyc + i0, sizeof(int) * (size_t)m, st);
+            for (int64_t i = 0; i < m && !rc; i++) {
+                if (buf[i] < 0) { maxcyc = max_cycles; capped = true; }
+                else maxcyc = std::max<int64_t>(maxcyc, buf[i]);
+            }
+        }
+        if (!rc && capped) rc = LICA_NOT_CONVERGED;
+    } else {
+        // the direct solves work in place: a, b, c, g go to the workspace first
+        if (tot > cap / 4) return LICA_TOO_LARGE;
+        double *w[4] = {scr, scr + tot, scr + 2 * tot, scr + 3 * tot};
+        const double *src[4] = {a, b, c, g};
+        for (int i = 0; i < 4 && !rc; i++) rc = d2d(w[i], src[i], bytes, st);
+        if (!rc && lica_launch_tri((int)nsys, (int)n, w[0], w[1], w[2], w[3], mode == LICA_POISSON_THOMAS ? 0 : 1,
+                                   256, (void *)st))
+            rc = LICA_HIP_ERROR;
+        rc = rc ? rc : d2d(p, w[3], bytes, st);
+        rc = rc ? rc : dsync(st);
+    }
     if (cycles_out) *cycles_out = maxcyc;
     return rc;
 }

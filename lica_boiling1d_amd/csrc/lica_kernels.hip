@@ -1,9 +1,40 @@
 // Kernel entry points and launchers of the per-step path (gfx950).
 #include "lica_kern_flow.h"
 #include "lica_launch.h"
+#include <mutex>
 
 long long lica_mg_scratch_doubles(int n) { return mg_scratch_doubles(n); }
 long long lica_mg_lds_need(int n, int tsize) { return mg_lds_bytes(n, tsize); }
+
+// Fixed device workspace of lica_poisson_noalloc (callers that cannot
+// allocate): one array per GPU, held until the process ends.  Host code
+// never uses the array's name as an address: the address of each GPU's copy
+// comes from hipGetSymbolAddress with that device current, once per device.
+#ifdef LICA_EMUL
+static double scratch[LICA_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+int lica_noalloc_scratch(double **out)
+{
+    *out = scratch;
+    return 0;
+}
+#else
+static __device__ __align__(256) double scratch[LICA_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+int lica_noalloc_scratch(double **out)
+{
+    static std::mutex mu;
+    static double *addr[LICA_NOALLOC_MAX_DEVICES] = {};
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= LICA_NOALLOC_MAX_DEVICES) return -1;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return -1;
+        addr[dev] = (double *)ptr;
+    }
+    *out = addr[dev];
+    return 0;
+}
+#endif
 
 #ifndef LICA_EMUL
 // LDS workspace of the partitioned tridiagonal solves, sized by the launch

@@ -15,6 +15,11 @@ int lica_launch_mg(int nsys, int n, const double *a, const double *b, const doub
                    double *scratch, int mode, double rtol, int maxc, int *cycles, double *resid, int threads,
                    void *stream);
 long long lica_mg_scratch_doubles(int n);
+// the fixed workspace of lica_poisson_noalloc: this device's address of the
+// array (0, or -1 when HIP cannot give it)
+#define LICA_NOALLOC_SCRATCH_BYTES (64ull << 20)
+#define LICA_NOALLOC_MAX_DEVICES 64
+int lica_noalloc_scratch(double **out);
 long long lica_mg_lds_need(int n, int tsize);   // LDS for x, f of every level
 long long lica_mg_lds_max();                   // largest dynamic LDS of a multigrid workgroup
 
