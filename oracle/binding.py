@@ -79,6 +79,15 @@ def load():
         P_i64 = ctypes.POINTER(c_i64)
         lib.orc_run_cases.argtypes = [ctypes.POINTER(OrcCase), c_i64, c_i64, P_i64, P_i64, P_dbl, P_dbl]
         lib.orc_run_cases.restype = ctypes.c_int
+        lib.orc_rk3_coefficients.argtypes = [V, P_dbl, P_dbl]
+        lib.orc_test_box_avg.argtypes = [V, P_dbl, c_dbl, c_dbl, c_i64, c_i64]
+        lib.orc_test_box_avg.restype = c_dbl
+        lib.orc_test_determine_dt.argtypes = [V, P_dbl, P_dbl, c_dbl, P_dbl, P_dbl]
+        lib.orc_test_mass_correction.argtypes = [V, c_dbl, P_i64]
+        lib.orc_test_mass_correction.restype = ctypes.c_int
+        lib.orc_test_vel_intp.argtypes = [V, P_dbl, P_dbl]
+        lib.orc_test_temp_grad.argtypes = [V, P_dbl, c_dbl, P_dbl, P_dbl, P_dbl, P_dbl, P_dbl, P_dbl]
+        lib.orc_test_temp_grad.restype = ctypes.c_int
         _lib = lib
     return _lib
 
@@ -126,6 +135,47 @@ class Oracle:
         out = np.zeros(NSCAL)
         self.lib.orc_scalars(self.h, _p(out))
         return out
+
+    # unit-level hooks (pinning tests)
+    def rk3_coefficients(self):
+        g, r = np.zeros(3), np.zeros(3)
+        self.lib.orc_rk3_coefficients(self.h, _p(g), _p(r))
+        return g, r
+
+    def box_avg(self, f, z1, z2, k1, k2):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        assert f.size == self.n * self.r
+        return self.lib.orc_test_box_avg(self.h, _p(f), float(z1), float(z2), int(k1), int(k2))
+
+    def determine_dt(self, w, psi_cn, dt):
+        w, psi_cn = (np.ascontiguousarray(x, dtype=np.float64) for x in (w, psi_cn))
+        assert w.size == self.n + 2 and psi_cn.size == self.n + 2
+        dto, cflm = ctypes.c_double(), ctypes.c_double()
+        self.lib.orc_test_determine_dt(self.h, _p(w), _p(psi_cn), float(dt), ctypes.byref(dto), ctypes.byref(cflm))
+        return dto.value, cflm.value
+
+    def mass_correction(self, vol_target):
+        passes = ctypes.c_int64()
+        st = self.lib.orc_test_mass_correction(self.h, float(vol_target), ctypes.byref(passes))
+        return st, passes.value
+
+    def vel_intp(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        assert w.size == self.n + 2
+        wf = np.zeros(self.n * self.r)
+        self.lib.orc_test_vel_intp(self.h, _p(w), _p(wf))
+        return wf
+
+    def temp_grad(self, t, dtconst):
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        assert t.size == self.n + 2
+        t0, t1 = np.zeros(self.n + 2), np.zeros(self.n + 2)
+        mfl = np.zeros(11 * 16)
+        sf2 = np.zeros(self.n * self.r)
+        place, vol = ctypes.c_double(), ctypes.c_double()
+        st = self.lib.orc_test_temp_grad(self.h, _p(t), float(dtconst), _p(t0), _p(t1), _p(mfl), _p(sf2),
+                                         ctypes.byref(place), ctypes.byref(vol))
+        return st, t0, t1, mfl, sf2, place.value, vol.value
 
     def last_poisson(self):
         n = self.n

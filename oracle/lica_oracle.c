@@ -1846,3 +1846,80 @@ int orc_run_cases(const orc_case *cases, int64_t nc, int64_t nsteps, int64_t *st
     free(inst);
     return threads;
 }
+
+/* ------------------------------------------------------------------ */
+/* Unit-level hooks for the pinning tests: each runs one restated routine
+ * on an initialised instance with inputs the test prescribes.            */
+
+void orc_rk3_coefficients(const orc_inst *s, double *gamma, double *ro)
+{
+    for (int i = 0; i < 3; i++) { gamma[i] = s->gamma[i]; ro[i] = s->ro[i]; }
+}
+
+/* INTER_PSI_FACE box average of a refined field f (cells 1..n3fm) over
+ * [z1,z2] with the refined index range k1..k2 */
+double orc_test_box_avg(orc_inst *s, const double *f, double z1, double z2, int64_t k1, int64_t k2)
+{
+    darr a = dnew(s->m3l, s->m3u);
+    for (int64_t k = 1; k <= s->n3fm; k++) D(a, k) = f[k - 1];
+    double v = box_avg(s, &a, z1, z2, k1, k2);
+    free(a.d);
+    return v;
+}
+
+/* DETERMINE_DT on prescribed W and PSI_CN (index 0..n3) and step dt */
+void orc_test_determine_dt(orc_inst *s, const double *w, const double *psi_cn, double dt, double *dt_out,
+                           double *cflm_out)
+{
+    for (int64_t k = 0; k <= s->n3; k++) { D(s->w, k) = w[k]; D(s->psi_cn, k) = psi_cn[k]; }
+    s->dt = dt;
+    determine_dt(s);
+    *dt_out = s->dt;
+    *cflm_out = s->cflm;
+}
+
+/* GLOBAL_MASS_CORRECTION of the current band toward the vapour volume
+ * vol_target (VOL_TOT_IPHS); ALPHI takes the corrected band values.
+ * Returns the status; *passes the volume evaluations it made. */
+int orc_test_mass_correction(orc_inst *s, double vol_target, int64_t *passes)
+{
+    phi_from_band(s);
+    s->vol_tot_iphs = vol_target;
+    double before = s->n_masscor;
+    int st = global_mass_correction(s, NN1);
+    *passes = (int64_t)(s->n_masscor - before);
+    for (int nl = 1; nl <= M_MAX; nl++)
+        for (int64_t q = 0; q < s->bn[nl]; q++) s->alphi[nl][q] = D(s->phi, s->bk[nl][q]);
+    return st;
+}
+
+/* VEL_INTP with W = WO = w (index 0..n3): refined velocity at the band
+ * planes, written to wf (refined index 1..n3fm at wf[k-1]) */
+void orc_test_vel_intp(orc_inst *s, const double *w, double *wf)
+{
+    for (int64_t k = 0; k <= s->n3; k++) { D(s->w, k) = w[k]; D(s->wo, k) = w[k]; }
+    dzero(&s->wf);
+    vel_intp(s, 1);
+    for (int64_t k = 1; k <= s->n3fm; k++) wf[k - 1] = D(s->wf, k);
+}
+
+/* TEMP_GRAD (TEMP_EXTP, TF_GRAD, VOL_TOT_MFDOT) on a prescribed T (index
+ * 0..n3) with DTCONST dtconst.  Outputs: the extended temperatures of each
+ * phase (0..n3), the band's MFLVS in the BAND field's slot order (layers
+ * 1..11, slots 0..bn-1), SFGAS2 (refined 1..n3fm), the provisional
+ * interface position and vapour volume.  Returns the status. */
+int orc_test_temp_grad(orc_inst *s, const double *t, double dtconst, double *temp0, double *temp1, double *mflvs,
+                       double *sfgas2, double *place, double *vol_iphs)
+{
+    for (int64_t k = 0; k <= s->n3; k++) D(s->t, k) = t[k];
+    s->dtconst = dtconst;
+    int st = temp_grad(s);
+    for (int64_t k = 0; k <= s->n3; k++) { temp0[k] = D(s->temp0, k); temp1[k] = D(s->temp1, k); }
+    int64_t m = 0;
+    for (int nl = 1; nl <= M_MAX; nl++)
+        for (int64_t q = 0; q < s->bn[nl]; q++) mflvs[m++] = s->mflvs[nl][q];
+    for (int64_t k = 1; k <= s->n3fm; k++) sfgas2[k - 1] = D(s->sfgas2, k);
+    *place = s->place;
+    *vol_iphs = s->vol_tot_iphs;
+    return st;
+}

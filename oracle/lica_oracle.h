@@ -80,6 +80,17 @@ int64_t orc_last_poisson(orc_inst *s, double *a, double *b, double *c, double *g
 int orc_run_cases(const orc_case *cases, int64_t nc, int64_t nsteps, int64_t *status, int64_t *steps_out,
                   double *setup_s, double *step_s);
 
+/* unit-level hooks of the pinning tests (each runs one restated routine on
+ * an initialised instance with inputs the test prescribes; see the .c file) */
+void orc_rk3_coefficients(const orc_inst *s, double *gamma, double *ro);
+double orc_test_box_avg(orc_inst *s, const double *f, double z1, double z2, int64_t k1, int64_t k2);
+void orc_test_determine_dt(orc_inst *s, const double *w, const double *psi_cn, double dt, double *dt_out,
+                           double *cflm_out);
+int orc_test_mass_correction(orc_inst *s, double vol_target, int64_t *passes);
+void orc_test_vel_intp(orc_inst *s, const double *w, double *wf);
+int orc_test_temp_grad(orc_inst *s, const double *t, double dtconst, double *temp0, double *temp1, double *mflvs,
+                       double *sfgas2, double *place, double *vol_iphs);
+
 /* unit-level restatements used by the tests */
 void orc_trdiag3(int64_t n, const double *a, const double *b, const double *c,
                  double *g);
