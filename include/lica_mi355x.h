@@ -212,6 +212,52 @@ int lica_transport(lica_batch *ctx);
 int lica_grid_coupling(lica_batch *ctx, int64_t ini_psi, double *psi_zn, double *psi_cn);
 int lica_end_step(lica_batch *ctx);   /* TIME=TIME+DT (1d.f:260) */
 
+/* ---- seams with the reference's own argument lists -------------------- */
+/* Arrays are the reference's REAL*8 (0:M1,0:M2,0:M3) of the 1-D
+ * configuration, M1 = M2 = 2 (1dmodulesX.f:5), in Fortran order
+ * ((I,J,K) at I + 3 (J + 3 K)), scalars by reference, so a Fortran driver
+ * passes its own arrays unchanged.  Each call copies the columns I = J = 1
+ * it reads into the context at entry and writes the ones the subroutine
+ * changes back at exit; the context owns the module state (T, the band,
+ * PSIF, SFGAS, WO, ...).  x/y arrays, QVOL_ORI, VOL1 and ANPSO are accepted
+ * and not used (identically zero or unused in this configuration).
+ *   lica_f_determine_dt  DETERMINE_DT(CFLM,U,V,W,PSI_CN) 1d.f:683 (caller
+ *                        1d.f:201); DT, a module variable there, in and out
+ *                        as the last argument
+ *   lica_f_rhsnlhs       RHSNLHS(ITRACKING,U,V,W,P,PSI_XN,PSI_YN,PSI_ZN,PSI_CN,
+ *                        RK3XO,RK3YO,RK3ZO,ANPSO,QVOL_ORI,VOL1) 1d.f:1444
+ *                        (caller 1d.f:227); MSUB, MAIN's loop variable, first
+ *   lica_f_convrge1      CONVRGE1(DVMAX,U,V,W) 1d.f:3306 (caller 1d.f:234)
+ *   lica_f_transport     TRANSPORT(U,V,W,PSI_CN,VOL_TOT_ORI) two_phase.f:701
+ *                        (caller 1d.f:242); W must be the W the preceding
+ *                        RHSNLHS returned (LICA_BAD_ARG otherwise)
+ *   lica_f_grid_coupling GRID_COUPLING(INI_PSI,PSI_XN,PSI_YN,PSI_ZN,PSI_CN)
+ *                        two_phase.f:4863 (caller 1d.f:244), INI_PSI = 0
+ *   lica_f_poisson       POISSON(P,DIVGSUM,DENF_XI,DENF_YI,DENF_ZI,DENF_Y,DENF_C)
+ *                        1d.f:3492 (caller 1d.f:3428): DIVGSUM(1,1,1:N3M),
+ *                        the matrix from DENF_ZI and the context's geometry,
+ *                        solved in the context's Poisson mode (not the
+ *                        warm-started ones)
+ *   lica_f_temp_grad     TEMP_GRAD(PSI_CN) chphase.f:55 (caller 1d.f:3667)  */
+int lica_f_determine_dt(lica_batch *ctx, double *cflm, const double *u, const double *v, const double *w,
+                        const double *psi_cn, double *dt);
+int lica_f_rhsnlhs(lica_batch *ctx, const int64_t *msub, const int64_t *itracking, double *u, double *v, double *w,
+                   double *p, double *psi_xn, double *psi_yn, double *psi_zn, double *psi_cn, double *rk3xo,
+                   double *rk3yo, double *rk3zo, double *anpso, double *qvol_ori, double *vol1);
+int lica_f_convrge1(lica_batch *ctx, double *dvmax, const double *u, const double *v, const double *w);
+int lica_f_transport(lica_batch *ctx, const double *u, const double *v, const double *w, double *psi_cn,
+                     double *vol_tot_ori);
+int lica_f_grid_coupling(lica_batch *ctx, const int64_t *ini_psi, double *psi_xn, double *psi_yn, double *psi_zn,
+                         double *psi_cn);
+int lica_f_poisson(lica_batch *ctx, double *p, const double *divgsum, const double *denf_xi, const double *denf_yi,
+                   const double *denf_zi, const double *denf_y, const double *denf_c);
+int lica_f_temp_grad(lica_batch *ctx, const double *psi_cn);
+/* the context's module state into the caller's arrays (after lica_init: the
+ * state MAKEFLD, LVSINIT and GRID_COUPLING(1) leave, 1d.f:70-190); NULL
+ * pointers are skipped */
+int lica_f_state(lica_batch *ctx, double *w, double *p, double *t, double *psi_zn, double *psi_cn,
+                 double *vol_tot_ori, double *time, double *dt);
+
 /* batched POISSON system of 1d.f:3535-3564: rows k=1..n of each of nsys
  * systems (a(1) is ignored as in the reference), contiguous per system.  */
 int lica_poisson(int64_t nsys, int64_t n, const double *a, const double *b, const double *c,

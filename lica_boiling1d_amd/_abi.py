@@ -102,6 +102,15 @@ def _declare(lib):
         "lica_grid_coupling": (ctypes.c_int, [V, c_i64, P_dbl, P_dbl]),
         "lica_end_step": (ctypes.c_int, [V]),
         "lica_poisson": (ctypes.c_int, [c_i64, c_i64, P_dbl, P_dbl, P_dbl, P_dbl, P_dbl, c_i64, c_dbl, c_i64, P_i64]),
+        # seams with the reference's argument lists: (3, 3, N3+1) Fortran-order arrays
+        "lica_f_determine_dt": (ctypes.c_int, [V, P_dbl, P_dbl, P_dbl, P_dbl, P_dbl, P_dbl]),
+        "lica_f_rhsnlhs": (ctypes.c_int, [V, P_i64, P_i64] + [P_dbl] * 14),
+        "lica_f_convrge1": (ctypes.c_int, [V, P_dbl, P_dbl, P_dbl, P_dbl]),
+        "lica_f_transport": (ctypes.c_int, [V, P_dbl, P_dbl, P_dbl, P_dbl, P_dbl]),
+        "lica_f_grid_coupling": (ctypes.c_int, [V, P_i64, P_dbl, P_dbl, P_dbl, P_dbl]),
+        "lica_f_poisson": (ctypes.c_int, [V] + [P_dbl] * 7),
+        "lica_f_temp_grad": (ctypes.c_int, [V, P_dbl]),
+        "lica_f_state": (ctypes.c_int, [V] + [P_dbl] * 8),
         # device addresses in and out, a hipStream_t (or None)
         "lica_poisson_noalloc": (ctypes.c_int, [c_i64, c_i64, V, V, V, V, V, c_i64, c_dbl, c_i64, P_i64, V]),
         "lica_status_string": (ctypes.c_char_p, [ctypes.c_int]),

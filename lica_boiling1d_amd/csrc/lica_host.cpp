@@ -1375,6 +1375,241 @@ int lica_end_step(lica_batch *ctx)
     return rc ? rc : first_status(ctx, &bad);
 }
 
+// ---------------------------------------------------------------------
+// Seams with the reference's own argument lists.  The arrays are the
+// reference's REAL*8 (0:M1,0:M2,0:M3) of the 1-D configuration, M1 = M2 = 2
+// (1dmodulesX.f:5), in Fortran order: (I,J,K) at I + 3 (J + 3 K).  The flow,
+// the energy and the Poisson solve use the column I = J = 1 (1d.f:1554-1555,
+// 3529-3530); each call copies the columns it reads into the context at
+// entry and writes the ones the subroutine changes back at exit.  The x/y
+// arrays (U, V, PSI_XN, PSI_YN, RK3XO, RK3YO) are identically zero in this
+// configuration and are accepted and left alone, as are the channel-flow
+// scalars QVOL_ORI and VOL1 (ICH = 0) and ANPSO (D7: read as zero).
+
+static size_t fidx(int64_t k) { return (size_t)(1 + 3 * (1 + 3 * k)); }
+
+static int fcol_in(lica_batch *b, int f, const double *a)
+{
+    if (!a) return 0;
+    const InstParam &p = b->par[0];
+    std::vector<double> col((size_t)p.n3 + 1);
+    for (int64_t k = 0; k <= p.n3; k++) col[(size_t)k] = a[fidx(k)];
+    return h2d(b->d_ns + p.ns_off + (size_t)f * p.ln + 2, col.data(), sizeof(double) * col.size(), b->stream);
+}
+
+static int fcol_get(lica_batch *b, int f, std::vector<double> &col)
+{
+    const InstParam &p = b->par[0];
+    col.resize((size_t)p.n3 + 2);
+    return d2h(col.data(), b->d_ns + p.ns_off + (size_t)f * p.ln + 2, sizeof(double) * col.size(), b->stream);
+}
+
+static int fcol_out(lica_batch *b, int f, double *a)
+{
+    if (!a) return 0;
+    std::vector<double> col;
+    int rc = fcol_get(b, f, col);
+    if (rc) return rc;
+    for (int64_t k = 0; k <= b->par[0].n3; k++) a[fidx(k)] = col[(size_t)k];
+    return 0;
+}
+
+static int put_state(lica_batch *b, const InstState *s) { return h2d(b->d_st, s, sizeof(InstState), b->stream); }
+
+static int f_single(lica_batch *b) { return (check_batch(b) || b->ninst != 1) ? LICA_BAD_ARG : 0; }
+
+int lica_f_determine_dt(lica_batch *ctx, double *cflm, const double *u, const double *v, const double *w,
+                        const double *psi_cn, double *dt)
+{
+    (void)u; (void)v;
+    int rc = f_single(ctx);
+    if (rc) return rc;
+    rc = fcol_in(ctx, NS_W, w);
+    rc = rc ? rc : fcol_in(ctx, NS_PSI_CN, psi_cn);
+    InstState s;
+    rc = rc ? rc : get_state(ctx, 0, &s);
+    if (rc) return rc;
+    if (s.status) return s.status;
+    if (w) {
+        // the CFL input of DETERMINE_DT from the caller's W (1d.f:1325-1373),
+        // max |W(k)+W(k+1)| SSDZ(k) as the projection records it
+        std::vector<double> ssdz;
+        rc = fcol_get(ctx, NS_SSDZ, ssdz);
+        if (rc) return rc;
+        double c3 = 0.0;
+        for (int64_t k = 1; k <= ctx->par[0].n; k++)
+            c3 = fmax(c3, fabs(w[fidx(k)] + w[fidx(k + 1)]) * ssdz[(size_t)k]);
+        s.c3max = c3;
+    }
+    if (dt) s.dt = *dt;
+    rc = put_state(ctx, &s);
+    rc = rc ? rc : launch(ctx, KL_DT, 1);
+    rc = rc ? rc : dsync(ctx->stream);
+    rc = rc ? rc : get_state(ctx, 0, &s);
+    if (rc) return rc;
+    if (cflm) *cflm = s.cflm;
+    if (dt) *dt = s.dt;
+    return s.status;
+}
+
+int lica_f_rhsnlhs(lica_batch *ctx, const int64_t *msub, const int64_t *itracking, double *u, double *v, double *w,
+                   double *p, double *psi_xn, double *psi_yn, double *psi_zn, double *psi_cn, double *rk3xo,
+                   double *rk3yo, double *rk3zo, double *anpso, double *qvol_ori, double *vol1)
+{
+    (void)itracking; (void)u; (void)v; (void)psi_xn; (void)psi_yn; (void)rk3xo; (void)rk3yo; (void)anpso;
+    (void)qvol_ori; (void)vol1;
+    int rc = f_single(ctx);
+    if (rc) return rc;
+    if (!msub) return LICA_BAD_ARG;
+    rc = fcol_in(ctx, NS_W, w);
+    rc = rc ? rc : fcol_in(ctx, NS_P, p);
+    rc = rc ? rc : fcol_in(ctx, NS_PSI_ZN, psi_zn);
+    rc = rc ? rc : fcol_in(ctx, NS_PSI_CN, psi_cn);
+    rc = rc ? rc : fcol_in(ctx, NS_RK3ZO, rk3zo);
+    if (rc) return rc;
+    const int st = lica_rhsnlhs(ctx, *msub, nullptr, nullptr, nullptr, nullptr);
+    if (st && !(st > 0 && st < 20)) return st;
+    rc = fcol_out(ctx, NS_W, w);
+    rc = rc ? rc : fcol_out(ctx, NS_P, p);
+    rc = rc ? rc : fcol_out(ctx, NS_RK3ZO, rk3zo);
+    return rc ? rc : st;
+}
+
+int lica_f_convrge1(lica_batch *ctx, double *dvmax, const double *u, const double *v, const double *w)
+{
+    (void)u; (void)v;
+    int rc = f_single(ctx);
+    if (rc) return rc;
+    rc = fcol_in(ctx, NS_W, w);
+    rc = rc ? rc : launch(ctx, KL_CONVRGE1, ctx->last_msub);
+    rc = rc ? rc : dsync(ctx->stream);
+    InstState s;
+    rc = rc ? rc : get_state(ctx, 0, &s);
+    if (rc) return rc;
+    if (dvmax) *dvmax = s.dvmax;
+    return s.status;
+}
+
+int lica_f_transport(lica_batch *ctx, const double *u, const double *v, const double *w, double *psi_cn,
+                     double *vol_tot_ori)
+{
+    (void)u; (void)v; (void)psi_cn;
+    int rc = f_single(ctx);
+    if (rc) return rc;
+    if (w) {
+        // VEL_INTP's time-averaged velocity was prepared by the projection
+        // from the W the last RHSNLHS returned; TRANSPORT must get that W
+        std::vector<double> cur;
+        rc = fcol_get(ctx, NS_W, cur);
+        if (rc) return rc;
+        for (int64_t k = 0; k <= ctx->par[0].n3; k++)
+            if (!(w[fidx(k)] == cur[(size_t)k])) return LICA_BAD_ARG;
+    }
+    if (vol_tot_ori) {
+        InstState s;
+        rc = get_state(ctx, 0, &s);
+        if (rc) return rc;
+        s.vol_tot_ori = *vol_tot_ori;
+        rc = put_state(ctx, &s);
+        if (rc) return rc;
+    }
+    const int st = lica_transport(ctx);
+    if (st && !(st > 0 && st < 20)) return st;
+    if (vol_tot_ori) {
+        InstState s;
+        rc = get_state(ctx, 0, &s);
+        if (rc) return rc;
+        *vol_tot_ori = s.vol_tot_ori;
+    }
+    return st;
+}
+
+int lica_f_grid_coupling(lica_batch *ctx, const int64_t *ini_psi, double *psi_xn, double *psi_yn, double *psi_zn,
+                         double *psi_cn)
+{
+    (void)psi_xn; (void)psi_yn;
+    int rc = f_single(ctx);
+    if (rc) return rc;
+    if (!ini_psi || *ini_psi != 0) return LICA_BAD_ARG;   // GRID_COUPLING(1) belongs to lica_init
+    rc = fcol_out(ctx, NS_PSI_ZN, psi_zn);
+    return rc ? rc : fcol_out(ctx, NS_PSI_CN, psi_cn);
+}
+
+int lica_f_poisson(lica_batch *ctx, double *p, const double *divgsum, const double *denf_xi, const double *denf_yi,
+                   const double *denf_zi, const double *denf_y, const double *denf_c)
+{
+    (void)denf_xi; (void)denf_yi; (void)denf_y; (void)denf_c;
+    int rc = f_single(ctx);
+    if (rc) return rc;
+    if (!p || !divgsum || !denf_zi) return LICA_BAD_ARG;
+    if (ctx->poisson_mode == LICA_POISSON_LEARNED || ctx->poisson_mode == LICA_POISSON_PROJECTED)
+        return LICA_UNSUPPORTED_DECK;   // the warm starts belong to the substep sequence
+    const InstParam &ip = ctx->par[0];
+    const int64_t n = ip.n;
+    std::vector<double> vvdz, ssdz;
+    rc = fcol_get(ctx, NS_VVDZ, vvdz);
+    rc = rc ? rc : fcol_get(ctx, NS_SSDZ, ssdz);
+    if (rc) return rc;
+    // the rows of 1d.f:3529-3543: a(1) = 0 (FIXKL(1) = 1), c from DENF_ZI
+    // above each face, b = -(a + c), g = DIVGSUM(1,1,K), K = 1..N3M
+    std::vector<double> a((size_t)n + 2, 0.0), bb((size_t)n + 2, 0.0), c((size_t)n + 2, 0.0), g((size_t)n + 2, 0.0);
+    for (int64_t k = 1; k <= n; k++) {
+        const double fixkl = (k == 1) ? 1.0 : 0.0;
+        const double ak = vvdz[(size_t)k] * ssdz[(size_t)k] * denf_zi[fidx(k)] * (1.0 - fixkl);
+        const double ck = vvdz[(size_t)k + 1] * ssdz[(size_t)k] * denf_zi[fidx(k + 1)];
+        a[(size_t)k] = ak; c[(size_t)k] = ck; bb[(size_t)k] = -(ak + ck); g[(size_t)k] = divgsum[k - 1];
+    }
+    const size_t bytes = sizeof(double) * ((size_t)n + 2);
+    double *base = ctx->d_ns + ip.ns_off + 2;
+    rc = h2d(base + (size_t)NS_PA * ip.ln, a.data(), bytes, ctx->stream);
+    rc = rc ? rc : h2d(base + (size_t)NS_PB * ip.ln, bb.data(), bytes, ctx->stream);
+    rc = rc ? rc : h2d(base + (size_t)NS_PC * ip.ln, c.data(), bytes, ctx->stream);
+    rc = rc ? rc : h2d(base + (size_t)NS_PG * ip.ln, g.data(), bytes, ctx->stream);
+    const int kp = (ctx->poisson_mode >= LICA_POISSON_MG64) ? KL_POISSON_MG : KL_POISSON_DIRECT;
+    rc = rc ? rc : launch(ctx, kp, ctx->last_msub);
+    rc = rc ? rc : dsync(ctx->stream);
+    InstState s;
+    rc = rc ? rc : get_state(ctx, 0, &s);
+    if (rc) return rc;
+    if (s.status) return s.status;
+    std::vector<double> col;
+    rc = fcol_get(ctx, NS_P, col);
+    if (rc) return rc;
+    for (int64_t k = 0; k <= ip.n3; k++) p[fidx(k)] = col[(size_t)k];
+    return 0;
+}
+
+int lica_f_state(lica_batch *ctx, double *w, double *p, double *t, double *psi_zn, double *psi_cn,
+                 double *vol_tot_ori, double *time, double *dt)
+{
+    int rc = f_single(ctx);
+    rc = rc ? rc : dsync(ctx->stream);
+    rc = rc ? rc : fcol_out(ctx, NS_W, w);
+    rc = rc ? rc : fcol_out(ctx, NS_P, p);
+    rc = rc ? rc : fcol_out(ctx, NS_T, t);
+    rc = rc ? rc : fcol_out(ctx, NS_PSI_ZN, psi_zn);
+    rc = rc ? rc : fcol_out(ctx, NS_PSI_CN, psi_cn);
+    InstState s;
+    rc = rc ? rc : get_state(ctx, 0, &s);
+    if (rc) return rc;
+    if (vol_tot_ori) *vol_tot_ori = s.vol_tot_ori;
+    if (time) *time = s.time;
+    if (dt) *dt = s.dt;
+    return 0;
+}
+
+int lica_f_temp_grad(lica_batch *ctx, const double *psi_cn)
+{
+    int rc = f_single(ctx);
+    if (rc) return rc;
+    rc = fcol_in(ctx, NS_PSI_CN, psi_cn);
+    rc = rc ? rc : launch(ctx, KL_TEMP_GRAD, ctx->last_msub);
+    rc = rc ? rc : dsync(ctx->stream);
+    if (rc) return rc;
+    int64_t bad;
+    return first_status(ctx, &bad);
+}
+
 }  // extern "C"
 
 extern "C" int lica_poisson(int64_t nsys, int64_t n, const double *a, const double *b, const double *c,

@@ -382,6 +382,48 @@ DEV void body_warm_base(const BatchDev &bd, int inst, WarmSh &sh)
     SERIAL_END
 }
 
+// CONVRGE1 (1d.f:3306-3362): max |dw/dz - S (rho_l - rho_v)/rho_l| over the
+// cells (S the cell's box average of SFGAS), and the CFL input
+// max |W(k)+W(k+1)| SSDZ(k) of the next DETERMINE_DT (1d.f:1325-1373)
+DEV void conv_maxima(const CaseP &cp, const InstState &S, BlkSh &sh, double *dvmax, double *c3max)
+{
+    const InstParam &P = *cp.P;
+    const int n = P.n;
+    const double *__restrict__ W = cp.A(NS_W);
+    const double *__restrict__ Z = cp.A(NS_Z), *__restrict__ SSDZ = cp.A(NS_SSDZ), *__restrict__ ZF = cp.R(RF_ZF);
+    const double *__restrict__ SFGAS = cp.R(RF_SFGAS);
+    const int *__restrict__ KCOU1 = cp.AI(NSI_KCOU1), *__restrict__ KCOU2 = cp.AI(NSI_KCOU2);
+    const int sfl = S.sf_lo, sfh = S.sf_hi;
+    stage_refwin(sh.win, sfl - 2 * P.r - 4, sfh + 2 * P.r + 4, P.m3l, P.n3f + P.r, ZF, SFGAS, nullptr);
+    const RefWin &win = sh.win;
+    auto zfa = [&](int kk) { return (kk >= win.lo && kk <= win.hi + 1) ? win.zf[kk - win.lo] : ZF[kk]; };
+    auto fsf = [&](int kk) { return (kk >= win.lo && kk <= win.hi) ? win.f0[kk - win.lo] : SFGAS[kk]; };
+    *dvmax = red_max(1, n + 1, 0.0, [&](int k) {
+        double d = (W[k + 1] - W[k]) * SSDZ[k];
+        bool t = !(KCOU1[k + 1] < sfl || KCOU2[k] > sfh);
+        double ss = t ? box_avg_acc(zfa, P.ssdzf, fsf, Z[k], Z[k + 1], KCOU2[k], KCOU1[k + 1]) : 0.0;
+        d = d - ss * (P.den_diff / P.denp);
+        return fabs(d);
+    }, sh.red);
+    *c3max = red_max(1, n + 1, 0.0, [&](int k) { return fabs(W[k] + W[k + 1]) * SSDZ[k]; }, sh.red);
+}
+
+// CONVRGE1 on its own (the seam with the reference's argument list, after
+// the caller's W was copied in)
+DEV void body_convrge1(const BatchDev &bd, int inst, BlkSh &sh)
+{
+    InstState &S = bd.st[inst];
+    if (S.status) return;
+    CaseP cp = case_ptrs(bd, inst);
+    double dvmax, c3max;
+    conv_maxima(cp, S, sh, &dvmax, &c3max);
+    SERIAL {
+        S.dvmax = dvmax;
+        S.c3max = c3max;
+    }
+    SERIAL_END
+}
+
 // UCALC (1d.f:3582-3627), p -= p(1) (3433-3470), CONVRGE1 (3306-3362), the
 // CFL input of the next DETERMINE_DT, and VEL_INTP's UT/UO (two_phase.f:5570-5616)
 DEV void body_ucalc(const BatchDev &bd, int inst, BlkSh &sh)
@@ -413,18 +455,8 @@ DEV void body_ucalc(const BatchDev &bd, int inst, BlkSh &sh)
     PFOR_END
     SERIAL { PP[0] = PP[1]; }
     SERIAL_END
-    stage_refwin(sh.win, sfl - 2 * P.r - 4, sfh + 2 * P.r + 4, P.m3l, P.n3f + P.r, ZF, SFGAS, nullptr);
-    const RefWin &win = sh.win;
-    auto zfa = [&](int kk) { return (kk >= win.lo && kk <= win.hi + 1) ? win.zf[kk - win.lo] : ZF[kk]; };
-    auto fsf = [&](int kk) { return (kk >= win.lo && kk <= win.hi) ? win.f0[kk - win.lo] : SFGAS[kk]; };
-    double dvmax = red_max(1, n + 1, 0.0, [&](int k) {
-        double d = (W[k + 1] - W[k]) * SSDZ[k];
-        bool t = !(KCOU1[k + 1] < sfl || KCOU2[k] > sfh);
-        double ss = t ? box_avg_acc(zfa, P.ssdzf, fsf, Z[k], Z[k + 1], KCOU2[k], KCOU1[k + 1]) : 0.0;
-        d = d - ss * (P.den_diff / P.denp);
-        return fabs(d);
-    }, sh.red);
-    double c3max = red_max(1, n + 1, 0.0, [&](int k) { return fabs(W[k] + W[k + 1]) * SSDZ[k]; }, sh.red);
+    double dvmax, c3max;
+    conv_maxima(cp, S, sh, &dvmax, &c3max);
     // the end-of-step finiteness check of MAIN (W, T and P no longer change
     // after the last substep's UCALC) takes effect after that substep's
     // TRANSPORT, whose own STOPs come first

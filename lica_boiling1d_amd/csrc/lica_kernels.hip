@@ -165,6 +165,11 @@ __global__ void __launch_bounds__(1024) k_ucalc(BatchDev bd)
     __shared__ BlkSh sh;
     body_ucalc(bd, bd.order[blockIdx.x], sh);
 }
+__global__ void __launch_bounds__(1024) k_convrge1(BatchDev bd)
+{
+    __shared__ BlkSh sh;
+    body_convrge1(bd, bd.order[blockIdx.x], sh);
+}
 __global__ void __launch_bounds__(64) k_transport(BatchDev bd)
 {
     __shared__ TransportSh sh;
@@ -225,6 +230,7 @@ int lica_launch(int which, const BatchDev &bd, int threads, void *stream)
     case KL_POISSON_DIRECT: LAUNCH_TRI(k_poisson_direct, threads); break;
     case KL_UCALC: LAUNCH(k_ucalc, threads); break;
     case KL_TRANSPORT: LAUNCH(k_transport, 64); break;
+    case KL_CONVRGE1: LAUNCH(k_convrge1, threads); break;
     case KL_POISSON_MG:
         hipLaunchKernelGGL(k_poisson_mg, dim3(bd.ninst), dim3(threads), (size_t)bd.mg_lds, (hipStream_t)stream, bd);
         break;
@@ -343,6 +349,7 @@ int lica_launch(int which, const BatchDev &bd, int threads, void *stream)
         for (int b = 0; b < bd.ninst; b++) body_capture(bd, bd.order[b], 1);
         break;
     case KL_WARM_BASE: emul_run<WarmSh>(bd, 256, body_warm_base); break;
+    case KL_CONVRGE1: emul_run<BlkSh>(bd, threads, body_convrge1); break;
     default: return -1;
     }
     return 0;

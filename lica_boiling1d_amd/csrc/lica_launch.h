@@ -6,7 +6,7 @@
 enum KernelId {
     KL_INIT = 0, KL_DT, KL_CONT_ENERGY, KL_TEMP_GRAD, KL_MOMENTUM,
     KL_POISSON_DIRECT, KL_UCALC, KL_TRANSPORT, KL_POISSON_MG, KL_CAPTURE_PRE, KL_CAPTURE_POST, KL_WARM_BASE,
-    KL_COUNT
+    KL_CONVRGE1, KL_COUNT
 };
 
 int lica_launch(int which, const BatchDev &bd, int threads, void *stream);

@@ -55,6 +55,9 @@ class _Handle:
     def __exit__(self, *a):
         self.close()
 
+    def set_poisson(self, mode, rtol=1.0e-10, max_cycles=100):
+        _check(self._lib.lica_batch_set_poisson(self._h, int(mode), float(rtol), int(max_cycles)), "set_poisson")
+
     # getters shared by Batch and Context
     def field(self, name, inst=0):
         f = FIELDS[name] if isinstance(name, str) else int(name)
@@ -123,9 +126,6 @@ class Batch(_Handle):
 
     def run(self, nsteps):
         _check(self._lib.lica_batch_run(self._h, int(nsteps)), "lica_batch_run")
-
-    def set_poisson(self, mode, rtol=1.0e-10, max_cycles=100):
-        _check(self._lib.lica_batch_set_poisson(self._h, int(mode), float(rtol), int(max_cycles)), "set_poisson")
 
     def set_mg_lds(self, nbytes):
         """Cap on the multigrid kernel's LDS (bytes; -1 restores the default)."""
@@ -240,6 +240,76 @@ class Context(_Handle):
 
     def end_step(self):
         _check(self._lib.lica_end_step(self._h), "lica_end_step")
+
+
+def ref_array(n):
+    """A zeroed array shaped like the reference's REAL*8 (0:2,0:2,0:N3) with
+    N3 = n+1, in Fortran memory order: numpy shape (N3+1, 3, 3) indexed
+    [K, J, I]; the column the 1-D path uses is [:, 1, 1]."""
+    return np.zeros((n + 2, 3, 3))
+
+
+class ReferenceSeams(Context):
+    """One case driven through the seams with the reference's own argument
+    lists (DETERMINE_DT 1d.f:683, RHSNLHS 1d.f:1444, CONVRGE1 1d.f:3306,
+    TRANSPORT two_phase.f:701, GRID_COUPLING two_phase.f:4863, POISSON
+    1d.f:3492, TEMP_GRAD chphase.f:55) on arrays from ref_array: each call
+    reads and writes the caller's arrays as the subroutine does."""
+
+    def _a(self, x):
+        if x is None:
+            return None
+        assert x.shape == (self.n + 2, 3, 3) and x.dtype == np.float64 and x.flags.c_contiguous
+        return _dptr(x)
+
+    def state(self):
+        """W, P, T, PSI_ZN, PSI_CN arrays and VOL_TOT_ORI, TIME, DT of the context"""
+        w, p, t, zn, cn = (ref_array(self.n) for _ in range(5))
+        vol, tim, dt = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(self._lib.lica_f_state(self._h, *[self._a(x) for x in (w, p, t, zn, cn)], ctypes.byref(vol),
+                                      ctypes.byref(tim), ctypes.byref(dt)), "lica_f_state")
+        return {"W": w, "P": p, "T": t, "PSI_ZN": zn, "PSI_CN": cn, "VOL_TOT_ORI": vol.value, "TIME": tim.value,
+                "DT": dt.value}
+
+    def DETERMINE_DT(self, u, v, w, psi_cn, dt):
+        cflm, d = ctypes.c_double(), ctypes.c_double(dt)
+        _check(self._lib.lica_f_determine_dt(self._h, ctypes.byref(cflm), self._a(u), self._a(v), self._a(w),
+                                             self._a(psi_cn), ctypes.byref(d)), "lica_f_determine_dt")
+        return cflm.value, d.value
+
+    def RHSNLHS(self, msub, itracking, u, v, w, p, psi_xn, psi_yn, psi_zn, psi_cn, rk3xo, rk3yo, rk3zo, anpso,
+                qvol_ori=0.0, vol1=0.0):
+        ms, it = ctypes.c_int64(msub), ctypes.c_int64(itracking)
+        q, v1 = ctypes.c_double(qvol_ori), ctypes.c_double(vol1)
+        arrs = [self._a(x) for x in (u, v, w, p, psi_xn, psi_yn, psi_zn, psi_cn, rk3xo, rk3yo, rk3zo, anpso)]
+        _check(self._lib.lica_f_rhsnlhs(self._h, ctypes.byref(ms), ctypes.byref(it), *arrs,
+                                        ctypes.byref(q), ctypes.byref(v1)), "lica_f_rhsnlhs")
+
+    def CONVRGE1(self, u, v, w):
+        d = ctypes.c_double()
+        _check(self._lib.lica_f_convrge1(self._h, ctypes.byref(d), self._a(u), self._a(v), self._a(w)),
+               "lica_f_convrge1")
+        return d.value
+
+    def TRANSPORT(self, u, v, w, psi_cn, vol_tot_ori):
+        vol = ctypes.c_double(vol_tot_ori)
+        _check(self._lib.lica_f_transport(self._h, self._a(u), self._a(v), self._a(w), self._a(psi_cn),
+                                          ctypes.byref(vol)), "lica_f_transport")
+        return vol.value
+
+    def GRID_COUPLING(self, ini_psi, psi_xn, psi_yn, psi_zn, psi_cn):
+        ip = ctypes.c_int64(ini_psi)
+        _check(self._lib.lica_f_grid_coupling(self._h, ctypes.byref(ip), self._a(psi_xn), self._a(psi_yn),
+                                              self._a(psi_zn), self._a(psi_cn)), "lica_f_grid_coupling")
+
+    def POISSON(self, p, divgsum, denf_xi, denf_yi, denf_zi, denf_y, denf_c):
+        divgsum = np.ascontiguousarray(divgsum, dtype=np.float64)
+        assert divgsum.size == self.n
+        _check(self._lib.lica_f_poisson(self._h, self._a(p), _dptr(divgsum), self._a(denf_xi), self._a(denf_yi),
+                                        self._a(denf_zi), self._a(denf_y), self._a(denf_c)), "lica_f_poisson")
+
+    def TEMP_GRAD(self, psi_cn):
+        _check(self._lib.lica_f_temp_grad(self._h, self._a(psi_cn)), "lica_f_temp_grad")
 
 
 def poisson(a, b, c, g, mode=_abi.POISSON_THOMAS, rtol=1.0e-10, max_cycles=100, lib=None):

@@ -88,6 +88,8 @@ def load():
         lib.orc_test_vel_intp.argtypes = [V, P_dbl, P_dbl]
         lib.orc_test_temp_grad.argtypes = [V, P_dbl, c_dbl, P_dbl, P_dbl, P_dbl, P_dbl, P_dbl, P_dbl]
         lib.orc_test_temp_grad.restype = ctypes.c_int
+        lib.orc_last_poisson_inputs.argtypes = [V, P_dbl, P_dbl]
+        lib.orc_last_poisson_inputs.restype = c_i64
         _lib = lib
     return _lib
 
@@ -176,6 +178,12 @@ class Oracle:
         st = self.lib.orc_test_temp_grad(self.h, _p(t), float(dtconst), _p(t0), _p(t1), _p(mfl), _p(sf2),
                                          ctypes.byref(place), ctypes.byref(vol))
         return st, t0, t1, mfl, sf2, place.value, vol.value
+
+    def last_poisson_inputs(self):
+        """DENF_ZI (0..n+1) and DIVGSUM (1..n) of the last substep's POISSON"""
+        dzi, dg = np.zeros(self.n + 2), np.zeros(self.n)
+        self.lib.orc_last_poisson_inputs(self.h, _p(dzi), _p(dg))
+        return dzi, dg
 
     def last_poisson(self):
         n = self.n

@@ -1923,3 +1923,12 @@ int orc_test_temp_grad(orc_inst *s, const double *t, double dtconst, double *tem
     *vol_iphs = s->vol_tot_iphs;
     return st;
 }
+
+/* the inputs of the last substep's POISSON (1d.f:3492): DENF_ZI (0..n3)
+ * and DIVGSUM (1..n) */
+int64_t orc_last_poisson_inputs(orc_inst *s, double *denf_zi, double *divgsum)
+{
+    for (int64_t k = 0; k <= s->n3; k++) denf_zi[k] = D(s->denf_zi, k);
+    for (int64_t k = 1; k <= s->n3m; k++) divgsum[k - 1] = D(s->divgsum, k);
+    return s->n3m;
+}

@@ -91,6 +91,8 @@ void orc_test_vel_intp(orc_inst *s, const double *w, double *wf);
 int orc_test_temp_grad(orc_inst *s, const double *t, double dtconst, double *temp0, double *temp1, double *mflvs,
                        double *sfgas2, double *place, double *vol_iphs);
 
+int64_t orc_last_poisson_inputs(orc_inst *s, double *denf_zi, double *divgsum);
+
 /* unit-level restatements used by the tests */
 void orc_trdiag3(int64_t n, const double *a, const double *b, const double *c,
                  double *g);

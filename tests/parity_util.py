@@ -49,3 +49,52 @@ def ulp_sensitivity(case, steps, fields):
     o1.close()
     o2.close()
     return out
+
+
+def reference_main(seams, steps, ref_array):
+    """MAIN's time loop (1d.f:196-295) through the seams with the reference's
+    argument lists, on arrays shaped like the reference's: DETERMINE_DT, the
+    zeroing of RK3ZO/ANPSO (1d.f:206-218), then per substep RHSNLHS,
+    CONVRGE1, TRANSPORT and GRID_COUPLING(0), and TIME = TIME + DT."""
+    n = seams.n
+    st = seams.state()
+    w, p, zn, cn = st["W"], st["P"], st["PSI_ZN"], st["PSI_CN"]
+    vol, dt = st["VOL_TOT_ORI"], st["DT"]
+    u, v, xn, yn, rx, ry, rz, an = (ref_array(n) for _ in range(8))
+    dvmax = cflm = 0.0
+    for _ in range(steps):
+        cflm, dt = seams.DETERMINE_DT(u, v, w, cn, dt)
+        rz[:] = 0.0
+        an[:] = 0.0
+        for msub in (1, 2, 3):
+            seams.RHSNLHS(msub, 2, u, v, w, p, xn, yn, zn, cn, rx, ry, rz, an)
+            dvmax = seams.CONVRGE1(u, v, w)
+            vol = seams.TRANSPORT(u, v, w, cn, vol)
+            seams.GRID_COUPLING(0, xn, yn, zn, cn)
+        seams.end_step()
+    untouched = all(not np.any(x) for x in (u, v, xn, yn, rx, ry))
+    return {"W": w, "P": p, "PSI_ZN": zn, "PSI_CN": cn, "VOL_TOT_ORI": vol, "DVMAX": dvmax, "CFLM": cflm,
+            "DT": dt, "untouched": untouched}
+
+
+def check_reference_main(seams, orc, steps, ref_array):
+    """The seam-driven run against the oracle, bit for bit: the caller's
+    columns I = J = 1, the other columns left zero, the context's own state."""
+    out = reference_main(seams, steps, ref_array)
+    assert orc.step(steps) == 0
+    bad = []
+    for f in ("W", "P", "PSI_ZN", "PSI_CN"):
+        arr = out[f]
+        if not np.array_equal(arr[:, 1, 1], orc.field(f)):
+            bad.append(f)
+        rest = arr.copy()
+        rest[:, 1, 1] = 0.0
+        if np.any(rest):
+            bad.append(f + " outside the column")
+    so = orc.scalar_array()
+    for name, i in (("DVMAX", 4), ("VOL_TOT_ORI", 6), ("DT", 1), ("CFLM", 5)):
+        if out[name] != so[i]:
+            bad.append(name)
+    if not out["untouched"]:
+        bad.append("x/y arrays written")
+    return bad + exact_mismatches(seams, 0, orc)

@@ -285,3 +285,40 @@ def test_batched_poisson_multigrid(mode):
         anorm = np.max(np.abs(a[s]) + np.abs(b[s]) + np.abs(c[s]))
         assert np.max(np.abs(r)) <= 1e-10 * (anorm * np.max(np.abs(p[s])) + np.max(np.abs(g[s])))
         assert np.max(np.abs(p[s] - x[s])) <= 1e-12 * np.max(np.abs(x[s]))
+
+
+def test_reference_seams_bit_exact():
+    """MAIN's loop through the seams with the reference's own argument lists
+    (arrays shaped (0:2,0:2,0:N3) in Fortran order, copied in at entry and
+    out at exit), 3 steps of the deck: the oracle's fields and scalars bit
+    for bit, the x/y arrays left zero."""
+    from parity_util import check_reference_main
+    _say("reference-signature seams, deck case, 3 steps")
+    seams = L.ReferenceSeams(os.path.join(GOLDEN, "lica.in"), os.path.join(GOLDEN, "grid.in"))
+    assert check_reference_main(seams, Oracle(L.deck_case()), 3, L.ref_array) == []
+    seams.close()
+
+
+@pytest.mark.parametrize("mode,tol", [(L.POISSON_THOMAS, 0.0), (L.POISSON_DIRECT, 1e-12), (L.POISSON_MG64, 1e-12),
+                                      (L.POISSON_MIXED, 1e-11)])
+def test_reference_poisson_seam(mode, tol):
+    """POISSON(P,DIVGSUM,DENF_XI,DENF_YI,DENF_ZI,DENF_Y,DENF_C) on the inputs
+    of the oracle's last substep: P(1,1,1:N) the oracle's TRDIAG3 solution
+    (bit for bit in Thomas mode), P(1,1,0) and P(1,1,N3) its end copies."""
+    from oracle.binding import trdiag3
+    _say(f"POISSON seam, mode {mode}")
+    o = Oracle(L.deck_case())
+    assert o.step(3) == 0
+    dzi, dg = o.last_poisson_inputs()
+    a, b, c, g, _ = o.last_poisson()
+    x = trdiag3(a, b, c, g)
+    seams = L.ReferenceSeams(os.path.join(GOLDEN, "lica.in"), os.path.join(GOLDEN, "grid.in"))
+    seams.set_poisson(mode, 1e-10, 100)
+    n = seams.n
+    P, DZI = L.ref_array(n), L.ref_array(n)
+    DZI[:, 1, 1] = dzi
+    seams.POISSON(P, dg, None, None, DZI, None, None)
+    col = P[:, 1, 1]
+    assert np.max(np.abs(col[1:n + 1] - x)) <= tol * np.max(np.abs(x))
+    assert col[0] == col[1] and col[n + 1] == col[n]
+    seams.close()

@@ -254,3 +254,39 @@ def test_projected_warm_start_cpu(emul):
         cyc[mode] = (s["POISSON_CYCLES"] - s0["POISSON_CYCLES"]) / (s["POISSON_SOLVES"] - s0["POISSON_SOLVES"])
     assert cyc[L.POISSON_MIXED] >= 3.5
     assert cyc[L.POISSON_PROJECTED] <= 3, cyc
+
+
+def test_reference_seams_bit_exact(emul):
+    """MAIN's loop through the seams with the reference's argument lists
+    (arrays shaped (0:2,0:2,0:N3) in Fortran order) gives the oracle's
+    fields and scalars bit for bit."""
+    from parity_util import check_reference_main
+    from conftest import GOLDEN
+    import os
+    seams = L.ReferenceSeams(os.path.join(GOLDEN, "lica.in"), os.path.join(GOLDEN, "grid.in"), lib=emul)
+    assert check_reference_main(seams, Oracle(L.deck_case()), 3, L.ref_array) == []
+
+
+@pytest.mark.parametrize("mode,tol", [(L.POISSON_THOMAS, 0.0), (L.POISSON_MG64, 1e-12)])
+def test_reference_poisson_seam(emul, mode, tol):
+    """POISSON(P,DIVGSUM,DENF_XI,DENF_YI,DENF_ZI,DENF_Y,DENF_C) on the inputs
+    of the oracle's last substep: the matrix assembled from DENF_ZI and the
+    geometry, P(1,1,1:N) the oracle's TRDIAG3 solution (bit for bit in
+    Thomas mode) and P(1,1,0), P(1,1,N3) its end copies (1d.f:3565-3572)."""
+    from conftest import GOLDEN
+    from oracle.binding import trdiag3
+    import os
+    o = Oracle(L.deck_case())
+    assert o.step(3) == 0
+    dzi, dg = o.last_poisson_inputs()
+    a, b, c, g, _ = o.last_poisson()
+    x = trdiag3(a, b, c, g)
+    seams = L.ReferenceSeams(os.path.join(GOLDEN, "lica.in"), os.path.join(GOLDEN, "grid.in"), lib=emul)
+    seams.set_poisson(mode, 1e-10, 100)
+    n = seams.n
+    P, DZI = L.ref_array(n), L.ref_array(n)
+    DZI[:, 1, 1] = dzi
+    seams.POISSON(P, dg, None, None, DZI, None, None)
+    col = P[:, 1, 1]
+    assert np.max(np.abs(col[1:n + 1] - x)) <= tol * np.max(np.abs(x))
+    assert col[0] == col[1] and col[n + 1] == col[n]
