@@ -25,7 +25,7 @@
  *   lica_batch_write_fld WRITEFIELD 1d.f:1121-1232 (the fld###### restart/post file)
  *   lica_batch_save/load no counterpart: the reference's fld restart keeps 12
  *                        digits and omits PSIF/TEMP/WO/MFLVS; these keep every
- *                        byte of the device state
+ *                        byte of the device state a later step reads
  * Errors are returned as status codes (the reference STOPs).
  */
 #ifndef LICA_MI355X_H
@@ -132,7 +132,10 @@ int lica_batch_last_timings(lica_batch *b, double *ms_out, int64_t cap);
 void lica_batch_destroy(lica_batch *b);
 
 /* ---- checkpoint / restart -------------------------------------------- */
-/* binary checkpoint of the whole device state: a restart continues bit for bit */
+/* binary checkpoint of the device state a later step reads (fields, band,
+ * scalars, counters, the Poisson mode, tolerance and cycle cap, and in the
+ * warm-started modes the solution history): a restart continues bit for
+ * bit; the learned mode's network is not kept (set it again after loading) */
 int lica_batch_save(lica_batch *b, const char *path);
 int lica_batch_load(const char *path, lica_batch **out);
 /* WRITEFIELD text file (1d.f:1121-1232, read by post.f READFLD 238-401) of one case */

@@ -1727,12 +1727,31 @@ extern "C" int lica_poisson_noalloc(int64_t nsys, int64_t n, const double *a, co
 
 // ---------------------------------------------------------------------
 // Checkpoint / restart.  The binary checkpoint holds every byte of the
-// batch's device state (fields, band, scalars, counters) so that a restart
-// continues bit for bit; the field file is the text format of WRITEFIELD
-// (1d.f:1121-1232), which post.f's READFLD (post.f:238-401) reads.
+// batch's device state a later step reads (fields, band, scalars, counters,
+// the Poisson mode with its tolerance and cycle cap, and in the
+// warm-started modes the solution history and the guess rows), so that a
+// restart continues bit for bit; the multigrid scratch, the network's
+// per-substep scales and the capture rows are rebuilt by every substep and
+// are not kept, nor is the network (a batch does not own it: a restart in
+// the learned mode needs lica_batch_set_net before it runs).  The field
+// file is the text format of WRITEFIELD (1d.f:1121-1232), which post.f's
+// READFLD (post.f:238-401) reads.
 
 namespace {
-const char CKPT_MAGIC[8] = {'L', 'I', 'C', 'A', 'C', 'K', 'P', '1'};
+const char CKPT_MAGIC[8] = {'L', 'I', 'C', 'A', 'C', 'K', 'P', '2'};
+
+long long guess_doubles(const lica_batch *b)
+{
+    long long t = 0;
+    for (const InstParam &p : b->par) t += p.n + 2;
+    return t;
+}
+long long hist_doubles(const lica_batch *b)
+{
+    long long t = 0;
+    for (const InstParam &p : b->par) t += 3LL * MG_HIST * p.n;
+    return t;
+}
 
 template <class T>
 bool put(FILE *f, const T *p, size_t n) { return fwrite(p, sizeof(T), n, f) == n; }
@@ -1763,7 +1782,17 @@ extern "C" int lica_batch_save(lica_batch *b, const char *path)
     bool ok = put(f, CKPT_MAGIC, 8) && put(f, hdr, 6) && put(f, b->cases.data(), b->cases.size()) &&
               put(f, st.data(), st.size()) && put(f, ns.data(), ns.size()) && put(f, rf.data(), rf.size()) &&
               put(f, nsi.data(), nsi.size()) && put(f, rfi.data(), rfi.size());
+    // the Poisson solve's setting and, when kept, the warm-start history
+    const int64_t pm[3] = {b->poisson_mode, b->max_cycles, b->d_hist ? 1 : 0};
+    ok = ok && put(f, pm, 3) && put(f, &b->rtol, 1);
+    if (ok && b->d_hist) {
+        std::vector<double> hist((size_t)hist_doubles(b)), guess((size_t)guess_doubles(b));
+        rc = d2h(hist.data(), b->d_hist, sizeof(double) * hist.size(), b->stream);
+        rc = rc ? rc : d2h(guess.data(), b->d_guess, sizeof(double) * guess.size(), b->stream);
+        ok = !rc && put(f, hist.data(), hist.size()) && put(f, guess.data(), guess.size());
+    }
     ok = (fclose(f) == 0) && ok;
+    if (rc) return rc;
     return ok ? LICA_OK : LICA_IO;
 }
 
@@ -1796,8 +1825,29 @@ extern "C" int lica_batch_load(const char *path, lica_batch **out)
     std::vector<int> nsi(b->nsi_total), rfi(b->rfi_total);
     bool ok = get(f, st.data(), st.size()) && get(f, ns.data(), ns.size()) && get(f, rf.data(), rf.size()) &&
               get(f, nsi.data(), nsi.size()) && get(f, rfi.data(), rfi.size());
+    int64_t pm[3] = {0, 0, 0};
+    double rtol = 0.0;
+    ok = ok && get(f, pm, 3) && get(f, &rtol, 1) && pm[0] >= LICA_POISSON_THOMAS && pm[0] <= LICA_POISSON_PROJECTED &&
+         pm[1] >= 1 && rtol > 0.0;
+    std::vector<double> hist, guess;
+    if (ok && pm[2]) {
+        hist.resize((size_t)hist_doubles(b));
+        guess.resize((size_t)guess_doubles(b));
+        ok = get(f, hist.data(), hist.size()) && get(f, guess.data(), guess.size());
+    }
     fclose(f);
     if (!ok) { lica_batch_destroy(b); return LICA_IO; }
+    // the mode as lica_batch_set_poisson would set it (its scratch and
+    // history allocated); the learned mode's network is the caller's to set
+    const int64_t mode = pm[0];
+    rc = lica_batch_set_poisson(b, mode == LICA_POISSON_LEARNED ? LICA_POISSON_PROJECTED : mode, rtol, pm[1]);
+    if (!rc && mode == LICA_POISSON_LEARNED) b->poisson_mode = LICA_POISSON_LEARNED;
+    if (!rc && pm[2]) {
+        if (!b->d_hist) rc = LICA_IO;
+        rc = rc ? rc : h2d(b->d_hist, hist.data(), sizeof(double) * hist.size(), b->stream);
+        rc = rc ? rc : h2d(b->d_guess, guess.data(), sizeof(double) * guess.size(), b->stream);
+    }
+    if (rc) { lica_batch_destroy(b); return rc; }
     rc = h2d(b->d_st, st.data(), sizeof(InstState) * st.size(), b->stream);
     rc = rc ? rc : h2d(b->d_ns, ns.data(), sizeof(double) * ns.size(), b->stream);
     rc = rc ? rc : h2d(b->d_rf, rf.data(), sizeof(double) * rf.size(), b->stream);

@@ -195,11 +195,14 @@ def test_batched_poisson_systems(mode, n):
         assert np.max(np.abs(p[s] - x)) <= 1e-10 * np.max(np.abs(x))
 
 
-def test_checkpoint_restart_bit_exact(tmp_path):
-    """Binary checkpoint of the device state: save, load, continue == uninterrupted."""
+@pytest.mark.parametrize("mode", [L.POISSON_THOMAS, L.POISSON_PROJECTED])
+def test_checkpoint_restart_bit_exact(tmp_path, mode):
+    """Binary checkpoint of the device state: save, load, continue ==
+    uninterrupted (in the projected mode with its solution history)."""
     cases = L.batched_cases(4, seed=9, nmin=128, nmax=600)
-    _say("checkpoint/restart, 4 cases")
+    _say(f"checkpoint/restart, 4 cases, mode {mode}")
     a = L.Batch(cases)
+    a.set_poisson(mode, 1e-10, 100)
     a.run(6)
     a.save(tmp_path / "ck.bin")
     a.run(6)

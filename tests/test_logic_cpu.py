@@ -76,10 +76,13 @@ def test_stopped_case_keeps_status_others_continue(emul):
     assert nbad == 1 and st[1] == st_o != 0 and st[0] == 0 and steps[0] == 3
 
 
-def test_checkpoint_restart_bit_exact(emul, tmp_path):
-    """save after 7 steps, load, 8 more == 15 uninterrupted steps"""
+@pytest.mark.parametrize("mode", [L.POISSON_THOMAS, L.POISSON_PROJECTED])
+def test_checkpoint_restart_bit_exact(emul, tmp_path, mode):
+    """save after 7 steps, load, 8 more == 15 uninterrupted steps (the
+    projected warm start's solution history and mode travel with the file)"""
     cases = L.batched_cases(3, seed=5, nmin=128, nmax=300)
     a = L.Batch(cases, lib=emul)
+    a.set_poisson(mode, 1e-10, 100)
     a.run(7)
     a.save(tmp_path / "ck.bin")
     a.run(8)
