@@ -176,6 +176,8 @@ def timed_run(L, cases, mode, args, net=None):
     b.set_poisson(mode, args.rtol, 200)
     b.set_block(args.threads)
     b.set_lanes(lanes_of(cases, args))
+    if getattr(args, "graph", False):
+        b.use_graph(True)
     if args.warmup:
         b.run_all(args.warmup)
     b.sync()
@@ -331,6 +333,7 @@ def main():
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--threads", type=int, default=256)
     ap.add_argument("--lanes", type=int, default=0, help="case sets on as many streams (1..8); 0: 4 for >= 256 cases")
+    ap.add_argument("--graph", action="store_true", help="replay each step as a captured hipGraph")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--train-cases", type=int, default=4096)
     ap.add_argument("--train-steps", type=int, default=25)
@@ -374,6 +377,7 @@ def main():
             "thomas": L.POISSON_THOMAS, "mg64": L.POISSON_MG64, "mixed": L.POISSON_MIXED}[args.mode]
     wl["poisson"] = args.mode
     wl["lanes"] = lanes_of(cases, args)
+    wl["graph"] = bool(args.graph)
     if mode >= L.POISSON_MG64:
         wl["poisson_stop"] = f"backward error <= {args.rtol:g} and last correction <= {args.rtol:g} max|p|"
 

@@ -435,11 +435,8 @@ DEV void body_ucalc(const BatchDev &bd, int inst, BlkSh &sh)
     const int n = P.n, n3 = P.n3, msub = bd.msub;
     double *__restrict__ W = cp.A(NS_W), *__restrict__ PP = cp.A(NS_P), *__restrict__ PZ = cp.A(NS_PSI_Z), *__restrict__ WO = cp.A(NS_WO), *__restrict__ WT = cp.A(NS_WT);
     const double *__restrict__ T = cp.A(NS_T);
-    const double *__restrict__ Z = cp.A(NS_Z), *__restrict__ SSDZ = cp.A(NS_SSDZ), *__restrict__ VVDZ = cp.A(NS_VVDZ), *__restrict__ ZF = cp.R(RF_ZF);
-    const double *__restrict__ SFGAS = cp.R(RF_SFGAS);
-    const int *__restrict__ KCOU1 = cp.AI(NSI_KCOU1), *__restrict__ KCOU2 = cp.AI(NSI_KCOU2);
+    const double *__restrict__ VVDZ = cp.A(NS_VVDZ);
     const double dtconst = S.dtconst;
-    const int sfl = S.sf_lo, sfh = S.sf_hi;
     const long long m = S.m;
     PFOR(k, 2, n + 1) { W[k] = W[k] - dtconst * (PP[k] - PP[k - 1]) * VVDZ[k] * denf_zi(P, PZ, k); }
     PFOR_END

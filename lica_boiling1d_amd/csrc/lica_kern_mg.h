@@ -41,8 +41,10 @@
 // infinity), a non-finite guess is replaced by zero, and a system with
 // non-finite rows yields a non-finite solution, as the direct solve would.
 //
-// Memory: setup rows of the coarse levels, cycle rows of every level (FP64
-// or FP32) and the iterate sit in a per-case global scratch; the cycle's
+// Memory: setup rows of the coarse levels, the pair weights of every level
+// (FP64 or FP32) and the iterate sit in a per-case global scratch; the
+// finest level's setup is fused with the first residual pass (one read of
+// the rows for both); the cycle's
 // vectors -- level 0's right-hand side r and correction e, and one vector per
 // coarser level (its f on the way down, its x on the way up) -- live in LDS
 // from the coarsest level up as far as the launch's dynamic LDS allows (the
@@ -80,16 +82,23 @@ HOSTDEV long long mg_rows_before(int n, int l)
 HOSTDEV long long mg_rows_total(int n) { return mg_rows_before(n, mg_levels(n)); }
 
 // Per-case scratch, in doubles (every block 32-byte aligned when the case's
-// scratch is):
+// scratch is); a level's rows pair up as (C row 2J, F row 2J+1), one pair per
+// row J of the next level, tot - n pairs in all:
 //   4 (tot - n)  FP64 setup rows (a, c, s, 1/b) of levels 1..L-1
-//   4 tot        FP64 cycle rows of every level
-//   2 tot        FP32 cycle rows of every level
+//   2 (tot - n)  FP64 C weights (w0, w1) of every pair
+//   4 (tot - n)  FP64 F weights (w0, w1, w2, pad) of every pair
+//   1 (tot - n)  FP32 C weights
+//   2 (tot - n)  FP32 F weights
 //   2 n          the iterate and its double buffer
 //   n + tot      the cycle's vectors kept outside LDS (room for FP64)
+// (the C and F weights are apart so that the downward pass, which reads
+// only the C weights, and the upward pass, which reads only the F weights,
+// each pull only the bytes they use)
+HOSTDEV long long mg_ru4(long long x) { return (x + 3) / 4 * 4; }
 HOSTDEV long long mg_scratch_doubles(int n)
 {
-    const long long tot = mg_rows_total(n);
-    return (4LL * (tot - n) + 4LL * tot + 2LL * tot + 2LL * n + (n + tot) + 8 + 3) / 4 * 4;
+    const long long tot = mg_rows_total(n), np = tot - n;
+    return mg_ru4(4 * np + mg_ru4(2 * np) + 4 * np + mg_ru4(np) + mg_ru4(2 * np) + mg_ru4(2LL * n) + (n + tot) + 8);
 }
 // LDS that holds every vector of a cycle in elements of tsize bytes
 HOSTDEV long long mg_lds_bytes(int n, int tsize) { return (long long)tsize * (n + mg_rows_total(n)); }
@@ -126,13 +135,20 @@ DEV void mg_table(int n, MgSh &sh)
     SERIAL_END
 }
 
-// setup row of a level: off-diagonals, row sum, 1/diagonal
+// setup row of a level: off-diagonals, row sum, 1/diagonal; the weights of
+// a pair of rows of a level:
+//   C row i = 2J:    f_c(J) = f_i + w0 f_{i-1} + w1 f_{i+1}   (w0 = -a_i/b_{i-1}, w1 = -c_i/b_{i+1})
+//   F row k = 2J+1:  x_k = w0 f_k + w1 x_{k-1} + w2 x_{k+1}   (w0 = 1/b_k, w1 = -a_k/b_k, w2 = -c_k/b_k)
 #ifdef LICA_EMUL
 struct MgSet {
     double a, c, s, ib;
 };
 template <class T>
-struct MgCyc {
+struct MgCw {
+    T w0, w1;
+};
+template <class T>
+struct MgFw {
     T w0, w1, w2, pad;
 };
 #else
@@ -140,13 +156,14 @@ struct alignas(32) MgSet {
     double a, c, s, ib;
 };
 template <class T>
-struct alignas(4 * sizeof(T)) MgCyc {
+struct alignas(2 * sizeof(T)) MgCw {
+    T w0, w1;
+};
+template <class T>
+struct alignas(4 * sizeof(T)) MgFw {
     T w0, w1, w2, pad;
 };
 #endif
-// cycle row of a level:
-//   C row i (even): f_c = f_i + w0 f_{i-1} + w1 f_{i+1}   (w0 = -a_i/b_{i-1}, w1 = -c_i/b_{i+1})
-//   F row k (odd):  x_k = w0 f_k + w1 x_{k-1} + w2 x_{k+1} (w0 = 1/b_k, w1 = -a_k/b_k, w2 = -c_k/b_k)
 
 // row i (0..n-1) of the finest level from the POISSON rows k = i+1 (a(1) and
 // c(n) are not part of the system): the row sum exactly, by error-free
@@ -167,8 +184,10 @@ struct MgView {
     int n;
     long long tot;
     MgSet *set;                 // setup rows of levels >= 1 (level l row i at lv_pre[l] - n + i)
-    MgCyc<double> *c64;         // cycle rows (level l row i at lv_pre[l] + i)
-    MgCyc<float> *c32;
+    MgCw<double> *cw64;         // weights of the pairs of level l at lv_pre[l+1] - n + J
+    MgFw<double> *fw64;
+    MgCw<float> *cw32;
+    MgFw<float> *fw32;
     double *x0, *x1;            // iterate buffers
     double *gv;                 // global home of the cycle's vectors
 };
@@ -179,12 +198,16 @@ DEV MgView mg_view(double *scr, int n, const MgSh &sh)
     v.scr = scr;
     v.n = n;
     v.tot = sh.lv_pre[sh.nlev];
-    v.set = (MgSet *)scr;
-    v.c64 = (MgCyc<double> *)(scr + 4 * (v.tot - n));
-    v.c32 = (MgCyc<float> *)(scr + 4 * (v.tot - n) + 4 * v.tot);
-    v.x0 = scr + 4 * (v.tot - n) + 6 * v.tot;
-    v.x1 = v.x0 + n;
-    v.gv = v.x1 + n;
+    const long long np = v.tot - n;
+    double *q = scr;
+    v.set = (MgSet *)q;             q += 4 * np;
+    v.cw64 = (MgCw<double> *)q;     q += mg_ru4(2 * np);
+    v.fw64 = (MgFw<double> *)q;     q += 4 * np;
+    v.cw32 = (MgCw<float> *)q;      q += mg_ru4(np);
+    v.fw32 = (MgFw<float> *)q;      q += mg_ru4(2 * np);
+    v.x0 = q;
+    v.x1 = q + n;                   q += mg_ru4(2LL * n);
+    v.gv = q;
     return v;
 }
 
@@ -208,51 +231,67 @@ template <class T>
 DEV T *mg_e0(const MgView &V, const MgSh &sh) { return mg_vec<T>(V, sh, V.n); }
 
 template <class T>
-DEV MgCyc<T> *mg_cyc(const MgView &V, const MgSh &sh, int l)
+DEV MgCw<T> *mg_cw(const MgView &V, const MgSh &sh, int l)
 {
-    if constexpr (sizeof(T) == 8) return V.c64 + sh.lv_pre[l];
-    else return V.c32 + sh.lv_pre[l];
+    if constexpr (sizeof(T) == 8) return V.cw64 + (sh.lv_pre[l + 1] - V.n);
+    else return V.cw32 + (sh.lv_pre[l + 1] - V.n);
+}
+template <class T>
+DEV MgFw<T> *mg_fw(const MgView &V, const MgSh &sh, int l)
+{
+    if constexpr (sizeof(T) == 8) return V.fw64 + (sh.lv_pre[l + 1] - V.n);
+    else return V.fw32 + (sh.lv_pre[l + 1] - V.n);
 }
 
-// Setup: the coarse levels' rows (row-sum form), the cycle rows of every
-// level, and the coarsest level's elimination factors.
+// One pair (C row i = 2J, F row i+1) of a level: the coarse row J in
+// row-sum form and the pair's weights (rows of the level from row(k))
+template <class T, class RowF>
+DEV MgSet mg_pair(RowF row, int i, int nl, MgCw<T> &cw, MgFw<T> &fw, bool &has_f)
+{
+    const MgSet R = row(i);
+    double sv = R.s, av = 0.0, cv = 0.0, w0 = 0.0, w1 = 0.0;
+    if (i >= 1) {
+        const MgSet Lo = row(i - 1);
+        const double q = R.a * Lo.ib;
+        av = -q * Lo.a;
+        sv = sv - q * Lo.s;
+        w0 = -q;
+    }
+    has_f = (i + 1 < nl);
+    if (has_f) {
+        const MgSet Up = row(i + 1);
+        const double q = R.c * Up.ib;
+        cv = -q * Up.c;
+        sv = sv - q * Up.s;
+        w1 = -q;
+        fw.w0 = (T)Up.ib; fw.w1 = (T)(-Up.a * Up.ib); fw.w2 = (T)(-Up.c * Up.ib); fw.pad = (T)0;
+    }
+    cw.w0 = (T)w0; cw.w1 = (T)w1;
+    MgSet C;
+    C.a = av; C.c = cv; C.s = sv; C.ib = 1.0 / ((sv - av) - cv);
+    return C;
+}
+
+// Setup of the levels below level 1 (level 1's rows come from mg_first):
+// level l+1's rows and level l's pair weights for l = 1..L-2, then the
+// coarsest level's elimination factors.
 template <class T>
-DEV void mg_setup(const double *a, const double *b, const double *c, const MgView &V, MgSh &sh)
+DEV void mg_setup_rest(const double *a, const double *b, const double *c, const MgView &V, MgSh &sh)
 {
     const int L = sh.nlev, n = V.n;
-    for (int l = 0; l + 1 < L; l++) {
+    for (int l = 1; l + 1 < L; l++) {
         const int nl = sh.lv_n[l], nc = sh.lv_n[l + 1];
-        MgSet *fine = (l == 0) ? nullptr : V.set + (sh.lv_pre[l] - n);
+        const MgSet *fine = V.set + (sh.lv_pre[l] - n);
         MgSet *crs = V.set + (sh.lv_pre[l + 1] - n);
-        MgCyc<T> *cy = mg_cyc<T>(V, sh, l);
+        MgCw<T> *cw = mg_cw<T>(V, sh, l);
+        MgFw<T> *fw = mg_fw<T>(V, sh, l);
         PFOR(J, 0, nc) {
-            const int i = 2 * J;
-            auto row = [&](int k) -> MgSet { return fine ? fine[k] : mg_row0(a, b, c, n, k); };
-            const MgSet R = row(i);
-            double sv = R.s, av = 0.0, cv = 0.0, w0 = 0.0, w1 = 0.0;
-            if (i >= 1) {
-                const MgSet Lo = row(i - 1);
-                const double q = R.a * Lo.ib;
-                av = -q * Lo.a;
-                sv = sv - q * Lo.s;
-                w0 = -q;
-            }
-            if (i + 1 < nl) {
-                const MgSet Up = row(i + 1);
-                const double q = R.c * Up.ib;
-                cv = -q * Up.c;
-                sv = sv - q * Up.s;
-                w1 = -q;
-                MgCyc<T> f;
-                f.w0 = (T)Up.ib; f.w1 = (T)(-Up.a * Up.ib); f.w2 = (T)(-Up.c * Up.ib); f.pad = (T)0;
-                cy[i + 1] = f;
-            }
-            MgSet C;
-            C.a = av; C.c = cv; C.s = sv; C.ib = 1.0 / ((sv - av) - cv);
-            crs[J] = C;
-            MgCyc<T> e;
-            e.w0 = (T)w0; e.w1 = (T)w1; e.w2 = (T)0; e.pad = (T)0;
-            cy[i] = e;
+            MgCw<T> w;
+            MgFw<T> f;
+            bool has_f;
+            crs[J] = mg_pair<T>([&](int k) { return fine[k]; }, 2 * J, nl, w, f, has_f);
+            cw[J] = w;
+            if (has_f) fw[J] = f;
         }
         PFOR_END
     }
@@ -310,10 +349,10 @@ DEV void mg_cycle(const MgView &V, MgSh &sh)
         const int nl = sh.lv_n[l], nc = sh.lv_n[l + 1];
         const T *fin = (l == 0) ? r0 : mg_v<T>(V, sh, l);
         T *fout = mg_v<T>(V, sh, l + 1);
-        const MgCyc<T> *cy = mg_cyc<T>(V, sh, l);
+        const MgCw<T> *cw = mg_cw<T>(V, sh, l);
         PFOR(J, 0, nc) {
             const int i = 2 * J;
-            const MgCyc<T> w = cy[i];
+            const MgCw<T> w = cw[J];
             T v = fin[i];
             if (i >= 1) v = v + w.w0 * fin[i - 1];
             if (i + 1 < nl) v = v + w.w1 * fin[i + 1];
@@ -330,13 +369,13 @@ DEV void mg_cycle(const MgView &V, MgSh &sh)
         const T *xc = mg_v<T>(V, sh, l + 1);
         const T *fx = (l == 0) ? r0 : mg_v<T>(V, sh, l);
         T *xo = (l == 0) ? e0 : mg_v<T>(V, sh, l);
-        const MgCyc<T> *cy = mg_cyc<T>(V, sh, l);
+        const MgFw<T> *fw = mg_fw<T>(V, sh, l);
         PFOR(J, 0, nc) {
             const T xl = xc[J];
             xo[2 * J] = xl;
             const int k = 2 * J + 1;
             if (k < nl) {
-                const MgCyc<T> w = cy[k];
+                const MgFw<T> w = fw[J];
                 const T xr = (J + 1 < nc) ? xc[J + 1] : (T)0;
                 xo[k] = w.w0 * fx[k] + w.w1 * xl + w.w2 * xr;
             }
@@ -373,6 +412,43 @@ DEV void mg_correct(const double *a, const double *b, const double *c, const dou
     }, sh.red, mx);
 }
 
+// The finest level's pass of the setup together with the first residual
+// (one read of a, b, c, g and the starting point; needs L >= 2): level 1's
+// rows and level 0's pair weights, x = x0 (rows 1..n; zero when x0 is null)
+// into xb, r = g - A x in double-double into level 0's r, and the maxima of
+// mg_correct's first pass
+template <class T>
+DEV void mg_first(const double *a, const double *b, const double *c, const double *g, const MgView &V,
+                  const double *x0, double *xb, MgSh &sh, double *mx)
+{
+    const int n = V.n, nc = sh.lv_n[1];
+    MgSet *crs = V.set;
+    MgCw<T> *cw = mg_cw<T>(V, sh, 0);
+    MgFw<T> *fw = mg_fw<T>(V, sh, 0);
+    T *r0 = mg_r0<T>(V, sh);
+    auto xv = [&](int k) -> double { return x0 ? x0[k + 1] : 0.0; };
+    red_maxN<MG_NRED>(0, nc, [&](int J, double *v) {
+        MgCw<T> w;
+        MgFw<T> f;
+        bool has_f;
+        crs[J] = mg_pair<T>([&](int k) { return mg_row0(a, b, c, n, k); }, 2 * J, n, w, f, has_f);
+        cw[J] = w;
+        if (has_f) fw[J] = f;
+        for (int q = 0; q < 5; q++) v[q] = 0.0;
+        for (int k = 2 * J; k <= 2 * J + 1 && k < n; k++) {
+            const double ai = (k == 0) ? 0.0 : a[k + 1], bi = b[k + 1], ci = (k == n - 1) ? 0.0 : c[k + 1];
+            const double x = xv(k), xl = (k > 0) ? xv(k - 1) : 0.0, xr = (k < n - 1) ? xv(k + 1) : 0.0;
+            const double r = dd_residual(g[k + 1], ai, xl, bi, x, ci, xr);
+            xb[k] = x;
+            r0[k] = (T)r;
+            v[0] = fmax(v[0], mg_mag(r));
+            v[1] = fmax(v[1], mg_mag(x));
+            v[3] = fmax(v[3], mg_mag(g[k + 1]));
+            v[4] = fmax(v[4], mg_mag(fabs(ai) + fabs(bi) + fabs(ci)));
+        }
+    }, sh.red, mx);
+}
+
 DEV double mg_berr(double rmax, double xmax, double gmax, double anorm)
 {
     const double den = anorm * xmax + gmax;
@@ -384,12 +460,17 @@ DEV int mg_run(const double *a, const double *b, const double *c, const double *
                int maxc, const double *x0, MgSh &sh, double **cur, double *res_out, bool *nonfinite_rows)
 {
     const int n = V.n;
-    mg_setup<T>(a, b, c, V, sh);
     double *xa = V.x0, *xb = V.x1;
-    PFOR(i, 0, n) { xa[i] = x0 ? x0[i + 1] : 0.0; }
-    PFOR_END
     double mx[MG_NRED];
-    mg_correct<T>(a, b, c, g, V, xa, xb, true, sh, mx);
+    if (sh.nlev >= 2) {
+        mg_first<T>(a, b, c, g, V, x0, xb, sh, mx);
+        mg_setup_rest<T>(a, b, c, V, sh);
+    } else {
+        mg_setup_rest<T>(a, b, c, V, sh);
+        PFOR(i, 0, n) { xa[i] = x0 ? x0[i + 1] : 0.0; }
+        PFOR_END
+        mg_correct<T>(a, b, c, g, V, xa, xb, true, sh, mx);
+    }
     const double gmax = mx[3], anorm = mx[4];
     *nonfinite_rows = !(gmax < INFINITY) || !(anorm < INFINITY);
     if (*nonfinite_rows) { *cur = xb; *res_out = NAN; return 0; }
