@@ -176,10 +176,12 @@ def timed_run(L, cases, mode, args, net=None):
     b.set_poisson(mode, args.rtol, 200)
     b.set_block(args.threads)
     b.set_lanes(lanes_of(cases, args))
+    b.set_lane_split(args.split == "binned")
     if getattr(args, "graph", False):
         b.use_graph(True)
     if args.warmup:
         b.run_all(args.warmup)
+    tuning = b.autotune(args.tune) if args.tune > 0 else b.tuning()
     b.sync()
     s0 = np.array([b.scalar_array(i) for i in range(len(cases))])
     _, steps0 = b.progress()
@@ -195,7 +197,7 @@ def timed_run(L, cases, mode, args, net=None):
     ok = (st == 0)
     res = {"el": el, "done": int(done.sum()), "value": float(done.sum()) / el,
            "cells": float(sum(int(c.n) * int(d) for c, d in zip(cases, done))) / el,
-           "ok": ok, "place": s1[:, 3], "nbad": nbad, "status": st, "steps": steps1}
+           "ok": ok, "place": s1[:, 3], "nbad": nbad, "status": st, "steps": steps1, "tuning": tuning}
     if mode >= L.POISSON_MG64 and ok.any():
         ci, si, ri = (L.SCALARS.index(k) for k in ("POISSON_CYCLES", "POISSON_SOLVES", "POISSON_RESID"))
         res["vcycles_per_solve"] = float((s1[ok, ci] - s0[ok, ci]).sum() / max((s1[ok, si] - s0[ok, si]).sum(), 1.0))
@@ -215,6 +217,8 @@ def kernel_profile(L, cases, mode, args, net=None):
     b.set_lanes(1)
     if args.warmup:
         b.run_all(args.warmup)
+    if args.tune > 0:
+        b.autotune(args.tune)
     b.sync()
     b.set_timing(True)
     b.run_all(args.steps)
@@ -334,6 +338,10 @@ def main():
     ap.add_argument("--threads", type=int, default=256)
     ap.add_argument("--lanes", type=int, default=0, help="case sets on as many streams (1..8); 0: 4 for >= 256 cases")
     ap.add_argument("--graph", action="store_true", help="replay each step as a captured hipGraph")
+    ap.add_argument("--split", choices=("binned", "interleaved"), default="binned",
+                    help="lanes as size bins of equal work (own launch shapes) or every L-th case")
+    ap.add_argument("--tune", type=int, default=2,
+                    help="trial steps per launch shape of the autotuner, after the warmup (0: default shapes)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--train-cases", type=int, default=4096)
     ap.add_argument("--train-steps", type=int, default=25)
@@ -377,6 +385,8 @@ def main():
             "thomas": L.POISSON_THOMAS, "mg64": L.POISSON_MG64, "mixed": L.POISSON_MIXED}[args.mode]
     wl["poisson"] = args.mode
     wl["lanes"] = lanes_of(cases, args)
+    wl["lane_split"] = args.split
+    wl["autotune_steps"] = args.tune
     wl["graph"] = bool(args.graph)
     if mode >= L.POISSON_MG64:
         wl["poisson_stop"] = f"backward error <= {args.rtol:g} and last correction <= {args.rtol:g} max|p|"
@@ -410,6 +420,7 @@ def main():
         out["poisson_backward_error_max"] = res["resid_max"]
     if res["nbad"]:
         out["stopped_cases"] = stopped(L, res)
+    out["launch_shapes"] = res["tuning"]
     say("one-lane kernel profile")
     per, alive = kernel_profile(L, cases, mode, args, net)
     out["roofline"] = roofline(L, cases, mode, per, alive, wkey)

@@ -189,6 +189,31 @@ int lica_batch_set_mg_lds(lica_batch *b, int64_t bytes);
  * lane needs a hardware queue of its own (GPU_MAX_HW_QUEUES, 4 by default in
  * HIP: set it to at least L + 4 before HIP starts for more than 2 lanes)  */
 int lica_batch_set_lanes(lica_batch *b, int64_t lanes);
+/* No counterpart in the reference (scheduling): split 1 makes the lanes
+ * contiguous size bins of the largest-first order holding equal work (n
+ * plus 512 rows per case), each launched with threads and multigrid LDS
+ * fitted to its own largest case; 0 (default) every L-th case.  Results do
+ * not change.                                                              */
+int lica_batch_set_lane_split(lica_batch *b, int64_t split);
+/* No counterpart in the reference (tuning): coordinate descent on the
+ * batch's step time.  For the kernel classes whose shape is free
+ * (the multigrid or direct POISSON solve, the momentum and energy stages,
+ * UCALC, DETERMINE_DT, in that order) and for each lane, tries 64..1024
+ * threads per workgroup, then multigrid LDS sizes per lane, each trial
+ * `steps` (1..16) steps of every lane at once as in a run, and keeps a
+ * change that gains more than 1 %.  The batch's device state is copied
+ * before and put back after every trial, so the run continues from where
+ * it was; no result depends on the shapes chosen (maxima only, chunks of
+ * the partitioned solves set by the row count).  Call after
+ * lica_batch_set_lanes / set_lane_split / set_poisson, which drop it.      */
+int lica_batch_autotune(lica_batch *b, int64_t steps);
+/* per lane (3 + LICA_KERNEL_CLASSES values): cases, largest n, multigrid LDS
+ * bytes, then threads per workgroup of every kernel class in the library's
+ * order (init, dt, cont_energy, temp_grad, momentum, poisson_direct, ucalc,
+ * transport, poisson_mg, capture_pre, capture_post, warm_base, convrge1);
+ * *count receives the number of values (out may be null to ask for it)    */
+#define LICA_KERNEL_CLASSES 13
+int lica_batch_get_tuning(lica_batch *b, int64_t *out, int64_t cap, int64_t *count);
 /* the network a batch uses in LICA_POISSON_LEARNED (not owned by the batch) */
 int lica_batch_set_net(lica_batch *b, lica_net *net);
 /* keep, for every case, the starting point of substep msub's POISSON solve

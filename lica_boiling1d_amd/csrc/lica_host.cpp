@@ -372,6 +372,22 @@ int lane_first(int64_t n, int L, int l)
     return s;
 }
 
+// launch shape of the kernels on one set of cases: its largest case, and
+// tuned threads per workgroup per kernel class and multigrid LDS (0 and -1:
+// the default rule).  The tuned classes (DETERMINE_DT, the energy and
+// momentum stages, the POISSON solves, UCALC) do not change a result with
+// their shape: every reduction in them is a maximum or in an order set by
+// the case, the partitioned solves' chunks are set by the row count, and
+// where the multigrid vectors live does not change their arithmetic.  The
+// band kernels and the warm-start base keep fixed shapes (one or two waves
+// per case; a projection whose sums are ordered by the workgroup size).
+static_assert(KL_COUNT == LICA_KERNEL_CLASSES, "kernel classes of the tuning table");
+struct LaneCfg {
+    int nmax = 0;
+    int thr[KL_COUNT] = {};
+    long long mg_lds = -1;
+};
+
 }  // namespace
 
 // ---------------------------------------------------------------------
@@ -409,6 +425,7 @@ struct lica_batch {
     int graph_capture = 0;
     const lica_net *graph_net = nullptr;
     long long graph_mg_lds = -1;
+    int graph_cfg = -1;
     std::vector<hipEvent_t> ev[LICA_MAX_LANES];     // timing events of the lanes
 #endif
     // Lanes: the cases split into L sets (every L-th case of the
@@ -422,6 +439,16 @@ struct lica_batch {
     int cur_lane = -1;                 // set being enqueued (-1: all cases)
     int *d_order_lanes = nullptr;      // per lane count L = 1..LICA_MAX_LANES: that split's lane-major order
     hipStream_t lane_stream[LICA_MAX_LANES] = {};
+    // split of the cases over the lanes: 0 every L-th case of the
+    // largest-first order, 1 contiguous size bins of that order with equal
+    // work (lane l holds positions bin_lo[l] .. bin_lo[l+1]-1), so that each
+    // lane's launch shape fits its own largest case
+    int lane_split = 0;
+    int bin_lo[LICA_MAX_LANES + 1] = {};
+    std::vector<int> order_h;          // the largest-first order
+    LaneCfg lcfg[LICA_MAX_LANES];      // per lane (lane 0 is every case with one lane)
+    LaneCfg all_cfg;                   // launches over every case while there are several lanes
+    int cfg_version = 0;               // changes with any launch shape
 #ifndef LICA_EMUL
     hipEvent_t fork = nullptr, join[LICA_MAX_LANES] = {};
 #endif
@@ -431,20 +458,28 @@ struct lica_batch {
 
 namespace {
 
-// dynamic LDS of the multigrid kernel: x, f of every level of the largest
-// case in the cycles' precision, capped (by default at 60 KB for batches of
-// many cases, so that several workgroups share a CU, else at the device's
-// per-workgroup limit); levels that do not fit stay in the global scratch
-long long mg_lds(const lica_batch *b)
+// launch shape of the set being enqueued
+const LaneCfg &cur_cfg(const lica_batch *b)
+{
+    if (b->cur_lane >= 0) return b->lcfg[b->cur_lane];
+    return (b->lanes == 1) ? b->lcfg[0] : b->all_cfg;
+}
+
+// dynamic LDS of the multigrid kernel: x, f of every level of the set's
+// largest case in the cycles' precision, capped (tuned, else the batch's
+// cap, else 60 KB for batches of many cases, so that several workgroups
+// share a CU, else the device's per-workgroup limit); levels that do not
+// fit stay in the global scratch
+long long mg_lds(const lica_batch *b, const LaneCfg &cfg)
 {
     if (b->poisson_mode < LICA_POISSON_MG64) return 0;
-    int nmax = 0;
-    for (const InstParam &p : b->par) nmax = std::max(nmax, p.n);
-    const long long need = lica_mg_lds_need(nmax, b->poisson_mode == LICA_POISSON_MG64 ? 8 : 4);
-    long long cap = (b->mg_lds_cap >= 0) ? b->mg_lds_cap : (b->ninst >= 512 ? 65536 - 4096 : lica_mg_lds_max());
+    const long long need = lica_mg_lds_need(cfg.nmax, b->poisson_mode == LICA_POISSON_MG64 ? 8 : 4);
+    long long cap = (cfg.mg_lds >= 0) ? cfg.mg_lds
+                  : (b->mg_lds_cap >= 0) ? b->mg_lds_cap : (b->ninst >= 512 ? 65536 - 4096 : lica_mg_lds_max());
     cap = std::min(cap, lica_mg_lds_max());
     return std::min(need, cap) / 16 * 16;
 }
+long long mg_lds(const lica_batch *b) { return mg_lds(b, cur_cfg(b)); }
 
 BatchDev make_bd(const lica_batch *b, int msub)
 {
@@ -472,29 +507,71 @@ BatchDev make_bd(const lica_batch *b, int msub)
     return bd;
 }
 
-// threads per workgroup: the requested count, raised in the partitioned mode
-// until every case's rows fit the per-thread register chunks.  One size for
-// the whole launch: launches per size class were measured slower, one after
-// the other (each drains before the next starts, and the largest-first
-// order no longer balances the tail) and on two streams at once alike.
-int block_threads(const lica_batch *b)
+// threads per workgroup of kernel class `which` on the set being enqueued:
+// the tuned count, else the requested one raised (outside the Thomas mode)
+// until the set's largest case has a thread per chunk of its partitioned
+// solves; a tuned count of a kernel that solves systems is raised likewise
+int block_threads(const lica_batch *b, int which)
 {
-    int t = b->threads;
-    if (b->poisson_mode != LICA_POISSON_THOMAS) {
-        int nmax = 0;
-        for (const InstParam &p : b->par) nmax = std::max(nmax, p.n);
-        const int need = (nmax + 7) / 8;   // rows per thread of the register level
-        while (t < need && t < 1024) t *= 2;
+    const LaneCfg &cfg = cur_cfg(b);
+    const bool part = b->poisson_mode != LICA_POISSON_THOMAS;
+    const int need = part ? lica_tri_chunks(cfg.nmax) : 0;
+    const bool solves = which == KL_CONT_ENERGY || which == KL_MOMENTUM || which == KL_POISSON_DIRECT;
+    int t = cfg.thr[which];
+    if (t > 0) {
+        while (solves && t < need && t < 1024) t *= 2;
+        return t;
     }
+    t = b->threads;
+    while (part && t < need && t < 1024) t *= 2;
     return t;
 }
 
 hipStream_t cur_stream(const lica_batch *b) { return b->cur_lane < 0 ? b->stream : b->lane_stream[b->cur_lane]; }
 
-// workgroup order of the lane being enqueued
+// cases of lane l, and the workgroup order of the lane being enqueued
+int lane_cases(const lica_batch *b, int l)
+{
+    return b->lane_split ? b->bin_lo[l + 1] - b->bin_lo[l] : lane_count(b->ninst, b->lanes, l);
+}
 const int *lane_order(const lica_batch *b)
 {
+    if (b->lane_split) return b->d_order + b->bin_lo[b->cur_lane];
     return b->d_order_lanes + (size_t)(b->lanes - 1) * (size_t)b->ninst + lane_first(b->ninst, b->lanes, b->cur_lane);
+}
+
+// the lanes' case sets and their largest cases for the current lane count
+// and split; tuned launch shapes are dropped.  The binned split gives lane l
+// the positions of the largest-first order whose cumulative work (n plus a
+// fixed 512 rows for the band kernels, whose cost does not grow with n)
+// reaches l/L of the total, at least one case each.
+void plan_lanes(lica_batch *b)
+{
+    const int L = b->lanes;
+    const int64_t N = b->ninst;
+    auto nof = [&](int pos) { return b->par[(size_t)b->order_h[(size_t)pos]].n; };
+    if (b->lane_split) {
+        double tot = 0.0;
+        for (int64_t i = 0; i < N; i++) tot += nof((int)i) + 512.0;
+        b->bin_lo[0] = 0;
+        double acc = 0.0;
+        int64_t pos = 0;
+        for (int l = 1; l < L; l++) {
+            const double goal = tot * l / L;
+            while (pos < N - (L - l) && (pos < b->bin_lo[l - 1] + 1 || acc + (nof((int)pos) + 512.0) * 0.5 < goal)) {
+                acc += nof((int)pos) + 512.0;
+                pos++;
+            }
+            b->bin_lo[l] = (int)pos;
+        }
+        b->bin_lo[L] = (int)N;
+    }
+    for (int l = 0; l < LICA_MAX_LANES; l++) b->lcfg[l] = LaneCfg();
+    for (int l = 0; l < L; l++)
+        b->lcfg[l].nmax = (N > 0) ? nof(b->lane_split ? b->bin_lo[l] : l) : 0;
+    b->all_cfg = LaneCfg();
+    b->all_cfg.nmax = (N > 0) ? nof(0) : 0;
+    b->cfg_version++;
 }
 
 int launch(lica_batch *b, int which, int msub)
@@ -502,9 +579,9 @@ int launch(lica_batch *b, int which, int msub)
     BatchDev bd = make_bd(b, msub);
     if (b->cur_lane >= 0) {
         bd.order = lane_order(b);
-        bd.ninst = lane_count(b->ninst, b->lanes, b->cur_lane);
+        bd.ninst = lane_cases(b, b->cur_lane);
     }
-    int rc = lica_launch(which, bd, block_threads(b), (void *)cur_stream(b));
+    int rc = lica_launch(which, bd, block_threads(b, which), (void *)cur_stream(b));
     return rc == 0 ? 0 : LICA_HIP_ERROR;
 }
 
@@ -519,7 +596,7 @@ int pre_poisson(lica_batch *b, int msub)
     if (!rc && b->poisson_mode == LICA_POISSON_LEARNED) {
         if (!b->net) return LICA_BAD_ARG;
         const int *ord = (b->cur_lane >= 0) ? lane_order(b) : nullptr;
-        const int cnt = (b->cur_lane >= 0) ? lane_count(b->ninst, b->lanes, b->cur_lane) : (int)b->ninst;
+        const int cnt = (b->cur_lane >= 0) ? lane_cases(b, b->cur_lane) : (int)b->ninst;
         if (lica_net_guess(b->net, b->d_netcases, ord, cnt, b->d_scale, b->rtol, (void *)cur_stream(b)))
             rc = LICA_HIP_ERROR;
     }
@@ -664,7 +741,8 @@ int build_graph(lica_batch *b)
     b->graph_mode = b->poisson_mode;
     b->graph_capture = b->capture;
     b->graph_net = b->net;
-    b->graph_mg_lds = mg_lds(b);
+    b->graph_mg_lds = b->mg_lds_cap;
+    b->graph_cfg = b->cfg_version;
     return 0;
 }
 #endif
@@ -865,6 +943,8 @@ int lica_batch_create(int64_t ninst, const lica_case_desc *cases, lica_batch **o
             for (int l = 0; l < L; l++)
                 for (size_t j = (size_t)l; j < order.size(); j += (size_t)L) lo.push_back(order[j]);
         b->lanes = (ninst >= 256) ? 2 : 1;
+        b->order_h = order;
+        plan_lanes(b);
         rc = rc ? rc : ensure_lanes(b, b->lanes);
         rc = rc ? rc : dmalloc((void **)&b->d_order_lanes, sizeof(int) * lo.size());
         rc = rc ? rc : h2d(b->d_order_lanes, lo.data(), sizeof(int) * lo.size(), b->stream);
@@ -946,7 +1026,8 @@ int lica_batch_run(lica_batch *b, int64_t nsteps)
     } else if (b->use_graph) {
         if (!b->graph || b->graph_threads != b->threads || b->graph_lanes != b->lanes ||
             b->graph_mode != b->poisson_mode ||
-            b->graph_capture != b->capture || b->graph_net != b->net || b->graph_mg_lds != mg_lds(b))
+            b->graph_capture != b->capture || b->graph_net != b->net || b->graph_mg_lds != b->mg_lds_cap ||
+            b->graph_cfg != b->cfg_version)
             rc = build_graph(b);
         for (int64_t s = 0; s < nsteps && !rc; s++)
             if (hipGraphLaunch(b->graph, b->stream) != hipSuccess) rc = LICA_HIP_ERROR;
@@ -980,7 +1061,10 @@ int lica_batch_set_poisson(lica_batch *b, int64_t mode, double rtol, int64_t max
         rc = rc ? rc : h2d(b->d_par, b->par.data(), sizeof(InstParam) * b->par.size(), b->stream);
         if (rc) return rc;
     }
-    b->poisson_mode = (int)mode;
+    if (b->poisson_mode != (int)mode) {
+        b->poisson_mode = (int)mode;
+        plan_lanes(b);   // tuned shapes belong to a mode
+    }
     b->rtol = rtol;
     b->max_cycles = (int)max_cycles;
     return LICA_OK;
@@ -1002,6 +1086,7 @@ int lica_batch_set_lanes(lica_batch *b, int64_t lanes)
     const int L = (int)std::min<int64_t>(lanes, b->ninst);
     if (int e = ensure_lanes(b, L)) return e;
     b->lanes = L;
+    plan_lanes(b);
     return LICA_OK;
 }
 
@@ -1011,6 +1096,211 @@ int lica_batch_set_block(lica_batch *b, int64_t threads)
     if (rc) return rc;
     if (threads < 64 || threads > 1024 || (threads & (threads - 1))) return LICA_BAD_ARG;
     b->threads = (int)threads;
+    return LICA_OK;
+}
+
+int lica_batch_set_lane_split(lica_batch *b, int64_t split)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (split != 0 && split != 1) return LICA_BAD_ARG;
+    b->lane_split = (int)split;
+    plan_lanes(b);
+    return LICA_OK;
+}
+
+}  // extern "C"
+
+namespace {
+long long guess_doubles(const lica_batch *b);
+long long hist_doubles(const lica_batch *b);
+
+// a device copy of everything a step reads and writes, to run trial steps
+// from and return to the same state
+struct StateCopy {
+    struct Buf { void *dev; void *copy; size_t bytes; };
+    std::vector<Buf> bufs;
+    ~StateCopy() { for (Buf &x : bufs) dfree(x.copy); }
+    int add(void *dev, size_t bytes)
+    {
+        if (!dev || !bytes) return 0;
+        Buf x{dev, nullptr, bytes};
+        if (dmalloc(&x.copy, bytes)) return LICA_HIP_ERROR;
+        bufs.push_back(x);
+        return 0;
+    }
+    int save(hipStream_t st)
+    {
+        for (Buf &x : bufs)
+            if (d2d(x.copy, x.dev, x.bytes, st)) return LICA_HIP_ERROR;
+        return dsync(st);
+    }
+    int restore(hipStream_t st)
+    {
+        for (Buf &x : bufs)
+            if (d2d(x.dev, x.copy, x.bytes, st)) return LICA_HIP_ERROR;
+        return dsync(st);
+    }
+};
+
+#ifndef LICA_EMUL
+// GPU time of `steps` steps of every lane, the lanes on their streams at
+// once as in a run (events on the batch's stream before the fork and after
+// the join)
+int time_steps(lica_batch *b, int64_t steps, double *ms)
+{
+    hipEvent_t e[2];
+    if (hipEventCreate(&e[0]) != hipSuccess) return LICA_HIP_ERROR;
+    if (hipEventCreate(&e[1]) != hipSuccess) { (void)hipEventDestroy(e[0]); return LICA_HIP_ERROR; }
+    const int nl = b->lanes;
+    int rc = (hipEventRecord(e[0], b->stream) == hipSuccess) ? 0 : LICA_HIP_ERROR;
+    if (!rc && nl > 1 && fork_lanes(b)) rc = LICA_HIP_ERROR;
+    for (int64_t s = 0; s < steps && !rc; s++)
+        for (int l = 0; l < nl && !rc; l++) {
+            b->cur_lane = (nl > 1) ? l : -1;
+            rc = enqueue_step_cases(b);
+        }
+    b->cur_lane = -1;
+    if (nl > 1 && join_lanes(b)) rc = rc ? rc : LICA_HIP_ERROR;
+    float t = 0.0f;
+    if (!rc && (hipEventRecord(e[1], b->stream) != hipSuccess || hipEventSynchronize(e[1]) != hipSuccess ||
+                hipEventElapsedTime(&t, e[0], e[1]) != hipSuccess))
+        rc = LICA_HIP_ERROR;
+    (void)hipEventDestroy(e[0]);
+    (void)hipEventDestroy(e[1]);
+    *ms = (double)t;
+    return rc;
+}
+#endif
+}  // namespace
+
+extern "C" {
+
+int lica_batch_autotune(lica_batch *b, int64_t steps)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (!b->initialized || steps < 1 || steps > 16) return LICA_BAD_ARG;
+    for (int l = 0; l < b->lanes; l++) {
+        for (int k = 0; k < KL_COUNT; k++) b->lcfg[l].thr[k] = 0;
+        b->lcfg[l].mg_lds = -1;
+    }
+    b->cfg_version++;
+#ifndef LICA_EMUL
+    if (int e = cast_net(b)) return e;
+    StateCopy sc;
+    rc = sc.add(b->d_st, sizeof(InstState) * (size_t)b->ninst);
+    rc = rc ? rc : sc.add(b->d_ns, sizeof(double) * b->ns_total);
+    rc = rc ? rc : sc.add(b->d_rf, sizeof(double) * b->rf_total);
+    rc = rc ? rc : sc.add(b->d_nsi, sizeof(int) * b->nsi_total);
+    rc = rc ? rc : sc.add(b->d_rfi, sizeof(int) * b->rfi_total);
+    rc = rc ? rc : sc.add(b->d_guess, sizeof(double) * (size_t)guess_doubles(b));
+    rc = rc ? rc : sc.add(b->d_scale, sizeof(double) * LICA_WARM_INFO * (size_t)b->ninst);
+    rc = rc ? rc : sc.add(b->d_cap, sizeof(double) * 3 * (size_t)guess_doubles(b));
+    if (b->d_hist) rc = rc ? rc : sc.add(b->d_hist, sizeof(double) * (size_t)hist_doubles(b));
+    rc = rc ? rc : sc.save(b->stream);
+    if (rc) return rc;
+    const bool cap_valid = b->cap_valid;
+    // a trial: `steps` steps of every lane from the saved state, back to it after
+    auto trial = [&](double *ms) -> int {
+        b->cfg_version++;
+        int e = time_steps(b, steps, ms);
+        int e2 = sc.restore(b->stream);
+        return e ? e : e2;
+    };
+    // Coordinate descent on the whole batch's step time: one kernel class
+    // (the longest first) of one lane at a time, each candidate timed with
+    // every lane running as in a run; a change is kept only when it gains
+    // more than 1 % (below that the difference is within the timing's own
+    // spread).  The first trial runs the default shapes once untimed.
+    double best = 0.0;
+    rc = trial(&best);
+    rc = rc ? rc : trial(&best);
+    const int classes[5] = {KL_POISSON_MG, KL_MOMENTUM, KL_CONT_ENERGY, KL_UCALC, KL_DT};
+    const int cand[5] = {64, 128, 256, 512, 1024};
+    for (int ci = 0; ci < 5 && !rc; ci++) {
+        const int k = (classes[ci] == KL_POISSON_MG && b->poisson_mode < LICA_POISSON_MG64) ? KL_POISSON_DIRECT
+                                                                                          : classes[ci];
+        for (int l = 0; l < b->lanes && !rc; l++) {
+            LaneCfg &cfg = b->lcfg[l];
+            b->cur_lane = (b->lanes > 1) ? l : -1;
+            const int cur = block_threads(b, k);
+            b->cur_lane = -1;
+            int keep = cfg.thr[k];
+            for (int j = 0; j < 5 && !rc; j++) {
+                if (cand[j] == cur) continue;
+                cfg.thr[k] = cand[j];
+                b->cur_lane = (b->lanes > 1) ? l : -1;
+                const bool raised = block_threads(b, k) != cand[j];   // below a solving kernel's need
+                b->cur_lane = -1;
+                if (raised) continue;
+                double t = 0.0;
+                rc = trial(&t);
+                if (!rc && t < best * 0.99) { best = t; keep = cand[j]; }
+            }
+            cfg.thr[k] = keep;
+        }
+    }
+    if (b->poisson_mode >= LICA_POISSON_MG64) {
+        // LDS of the multigrid workgroup: room for every vector of the lane's
+        // largest case, or a cap that lets more workgroups share a CU (the
+        // finest levels then use global memory)
+        for (int l = 0; l < b->lanes && !rc; l++) {
+            LaneCfg &cfg = b->lcfg[l];
+            const long long need = std::min(lica_mg_lds_need(cfg.nmax, b->poisson_mode == LICA_POISSON_MG64 ? 8 : 4),
+                                            lica_mg_lds_max()) / 16 * 16;
+            b->cur_lane = (b->lanes > 1) ? l : -1;
+            const long long cur = mg_lds(b);
+            b->cur_lane = -1;
+            const long long caps[6] = {need, 65536 - 4096, 49152, 32768, 20480, 12288};
+            long long keep = cfg.mg_lds;
+            for (int j = 0; j < 6 && !rc; j++) {
+                const long long c = std::min(caps[j], need);
+                if (c == cur || (j > 0 && c == need)) continue;
+                cfg.mg_lds = c;
+                double t = 0.0;
+                rc = trial(&t);
+                if (!rc && t < best * 0.99) { best = t; keep = c; }
+            }
+            cfg.mg_lds = keep;
+        }
+    }
+    b->cfg_version++;
+    b->cap_valid = cap_valid;
+    for (int k = 0; k < KL_COUNT; k++) b->timings[k] = 0.0;
+    if (rc) {
+        for (int l = 0; l < b->lanes; l++) {
+            for (int k = 0; k < KL_COUNT; k++) b->lcfg[l].thr[k] = 0;
+            b->lcfg[l].mg_lds = -1;
+        }
+        b->cfg_version++;
+    }
+    return rc;
+#else
+    (void)steps;
+    return LICA_OK;
+#endif
+}
+
+int lica_batch_get_tuning(lica_batch *b, int64_t *out, int64_t cap, int64_t *count)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    const int64_t per = 3 + KL_COUNT;
+    const int64_t need = per * b->lanes;
+    if (count) *count = need;
+    if (!out) return LICA_OK;
+    if (cap < need) return LICA_BAD_ARG;
+    for (int l = 0; l < b->lanes; l++) {
+        const int saved = b->cur_lane;
+        b->cur_lane = (b->lanes > 1) ? l : -1;
+        int64_t *o = out + per * l;
+        o[0] = lane_cases(b, l);
+        o[1] = b->lcfg[l].nmax;
+        o[2] = mg_lds(b);
+        for (int k = 0; k < KL_COUNT; k++) o[3 + k] = block_threads(b, k);
+        b->cur_lane = saved;
+    }
     return LICA_OK;
 }
 

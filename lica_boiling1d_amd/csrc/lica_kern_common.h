@@ -187,6 +187,24 @@ DEV double box_avg(const double *zf, double ssdzf, F f, double z1, double z2, in
 #define TRI_LCHUNK 8     // rows per chunk of the LDS levels
 #define TRI_SERIAL 64    // end-row systems this small are eliminated by one thread
 
+#ifndef TRI_CHUNK_ROWS
+#define TRI_CHUNK_ROWS 8
+#endif
+// chunks of the partitioned solve of nrow >= 4 rows: TRI_CHUNK_ROWS rows
+// each while that needs at most 1024 chunks, else TRI_RPT rows each (at
+// least two chunks).  The partition depends on the row count alone, so a
+// case's result does not depend on the workgroup size; a workgroup with
+// fewer threads than chunks eliminates serially instead (the host gives the
+// kernels that solve systems tri_chunks(nmax) threads or more).
+HOSTDEV int tri_chunks(int nrow)
+{
+    int nc = (nrow + TRI_CHUNK_ROWS - 1) / TRI_CHUNK_ROWS;
+    if (nc > 1024) nc = (nrow + TRI_RPT - 1) / TRI_RPT;
+    if (nc < 2) nc = 2;
+    if (nc > nrow / 2) nc = nrow / 2;
+    return nc;
+}
+
 // rows of the LDS workspace (4 arrays) the end-row levels of nthreads chunks need
 HOSTDEV int tri_ws_rows(int nthreads)
 {
@@ -478,9 +496,13 @@ DEV void tri_solve(double *a, double *b, double *c, double *g, int l1, int l2, i
         tri_thomas(a, b, c, g, l1, l2);
         return;
     }
-    double *RA = ts.ws, *RB = ts.ws + ts.ws_rows, *RC = ts.ws + 2 * ts.ws_rows, *RG = ts.ws + 3 * ts.ws_rows;
-    int nc = NTHREADS;
-    if (nc > nrow / 2) nc = nrow / 2;
+    const int nc = tri_chunks(nrow);
+    if (nc > NTHREADS) {
+        // fewer threads than chunks: the serial elimination (exact reference order)
+        tri_thomas(a, b, c, g, l1, l2);
+        return;
+    }
+    double *RA = ts.ws;
     const double *A = a + l1, *B = b + l1, *C = c + l1;
     double *G = g + l1;
     // The partitioned elimination is backward stable but, on the nearly
@@ -492,12 +514,6 @@ DEV void tri_solve(double *a, double *b, double *c, double *g, int l1, int l2, i
     // receives the solution.
     const int nref = (mode == 2) ? 0 : TRI_NREF;
 #ifndef LICA_EMUL
-    const int lmax = (nrow + nc - 1) / nc;
-    if (lmax > TRI_RPT) {
-        // rows beyond registers: the serial elimination (exact reference order)
-        tri_thomas(a, b, c, g, l1, l2);
-        return;
-    }
     const int ch = (int)threadIdx.x;
     const bool act = ch < nc;
     int s = 0, L = 0;
@@ -555,6 +571,7 @@ DEV void tri_solve(double *a, double *b, double *c, double *g, int l1, int l2, i
     BARRIER();
 #else
     // the same passes over copies of the rows (CPU logic-check build)
+    double *RB = ts.ws + ts.ws_rows, *RC = ts.ws + 2 * ts.ws_rows, *RG = ts.ws + 3 * ts.ws_rows;
     static double *wa = nullptr, *wb, *wc, *wx, *wd;
     static int cap = 0;
     if (nrow > cap) {
@@ -606,10 +623,10 @@ DEV void tri_solve_rows(double *a, double *b, double *c, double *g, int l1, int 
                         OutF out)
 {
     const int nrow = l2 - l1 + 1;
-    int nc = NTHREADS;
-    if (nc > nrow / 2) nc = nrow / 2;
+    const int nc = (nrow >= 4) ? tri_chunks(nrow) : 0;
+    const bool fits = nc <= NTHREADS;
 #ifndef LICA_EMUL
-    const bool regs = (mode == 2 && nrow >= 4 && (nrow + nc - 1) / nc <= TRI_RPT);
+    const bool regs = (mode == 2 && nrow >= 4 && fits);
 #else
     const bool regs = false;
 #endif

@@ -4,6 +4,7 @@
 #include <mutex>
 
 long long lica_mg_scratch_doubles(int n) { return mg_scratch_doubles(n); }
+int lica_tri_chunks(int nrow) { return nrow >= 4 ? tri_chunks(nrow) : 1; }
 long long lica_mg_lds_need(int n, int tsize) { return mg_lds_bytes(n, tsize); }
 
 // Fixed device workspace of lica_poisson_noalloc (callers that cannot

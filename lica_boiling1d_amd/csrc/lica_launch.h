@@ -9,6 +9,11 @@ enum KernelId {
     KL_CONVRGE1, KL_COUNT
 };
 
+// chunks of the partitioned tridiagonal solve of nrow rows: a kernel that
+// solves systems of up to nmax rows needs lica_tri_chunks(nmax) threads to
+// take the partitioned path
+int lica_tri_chunks(int nrow);
+
 int lica_launch(int which, const BatchDev &bd, int threads, void *stream);
 int lica_launch_tri(int nsys, int n, double *a, double *b, double *c, double *g, int mode, int threads, void *stream);
 int lica_launch_mg(int nsys, int n, const double *a, const double *b, const double *c, const double *g, double *p,

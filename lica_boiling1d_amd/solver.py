@@ -150,6 +150,38 @@ class Batch(_Handle):
         """Run the cases as `lanes` sets (1..8) on as many streams at once."""
         _check(self._lib.lica_batch_set_lanes(self._h, int(lanes)), "set_lanes")
 
+    def set_lane_split(self, binned=True):
+        """Lanes as contiguous size bins of the largest-first order holding
+        equal work, each launched with shapes fitted to its own largest case
+        (binned), or as every L-th case of that order (the default)."""
+        _check(self._lib.lica_batch_set_lane_split(self._h, 1 if binned else 0), "set_lane_split")
+
+    def autotune(self, steps=2):
+        """Coordinate descent on the step time: for every kernel class whose
+        shape is free and every lane, 64..1024 threads per workgroup, then
+        multigrid LDS sizes, each trial `steps` steps of all lanes at once;
+        a change is kept when it gains more than 1 %.  The batch returns to
+        the state it had; no result depends on the choice.  Returns
+        tuning()."""
+        _check(self._lib.lica_batch_autotune(self._h, int(steps)), "autotune")
+        return self.tuning()
+
+    def tuning(self):
+        """Launch shape per lane: cases, largest n, multigrid LDS bytes and
+        threads per workgroup of each kernel class."""
+        cnt = ctypes.c_int64(0)
+        _check(self._lib.lica_batch_get_tuning(self._h, None, 0, ctypes.byref(cnt)), "get_tuning")
+        out = np.zeros(cnt.value, dtype=np.int64)
+        _check(self._lib.lica_batch_get_tuning(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), out.size,
+                                               ctypes.byref(cnt)), "get_tuning")
+        per = 3 + len(KERNEL_CLASSES)
+        res = []
+        for l in range(out.size // per):
+            o = out[l * per:(l + 1) * per]
+            res.append({"cases": int(o[0]), "nmax": int(o[1]), "mg_lds": int(o[2]),
+                        "threads": {k: int(o[3 + i]) for i, k in enumerate(KERNEL_CLASSES) if k in TUNED_CLASSES}})
+        return res
+
     def warm_info(self):
         """Per case [first row of the network window, backward error of the
         warm-start base, s_r, s_b, alpha, start-test backward error, 1 if the
@@ -201,6 +233,13 @@ class Batch(_Handle):
         res = {names[i]: float(out[i]) for i in range(min(m - 1, len(names)))}
         res["wall"] = float(out[m - 1])
         return res
+
+
+# kernel classes in the library's order, and those whose launch shape the
+# autotuner sets
+KERNEL_CLASSES = ["init", "dt", "cont_energy", "temp_grad", "momentum", "poisson_direct", "ucalc", "transport",
+                  "poisson_mg", "capture_pre", "capture_post", "warm_base", "convrge1"]
+TUNED_CLASSES = ("dt", "cont_energy", "momentum", "poisson_direct", "ucalc", "poisson_mg")
 
 
 class Context(_Handle):

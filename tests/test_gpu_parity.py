@@ -115,12 +115,17 @@ def test_two_lanes_equal_one(mode):
         b.close()
 
 
-def test_block_size_does_not_change_results():
+@pytest.mark.parametrize("mode", [L.POISSON_THOMAS, L.POISSON_DIRECT, L.POISSON_MIXED])
+def test_block_size_does_not_change_results(mode):
+    """Every reduction of the step is a maximum or ordered by the case, and
+    the partitioned solves' chunks are set by the row count: 64, 256 and
+    1024 threads per workgroup give the same bits in every mode."""
     case = L.deck_case(300, 2)
-    _say("block sizes 64/256/1024, 10 steps")
+    _say(f"block sizes 64/256/1024, mode {mode}, 10 steps")
     ref = None
     for t in (64, 256, 1024):
         b = L.Batch([case])
+        b.set_poisson(mode, 1e-10, 100)
         b.set_block(t)
         b.run(10)
         cur = [b.field(f) for f in ("W", "P", "T", "PSIF", "PHI")]
@@ -128,6 +133,42 @@ def test_block_size_does_not_change_results():
             ref = cur
         else:
             assert all(np.array_equal(x, y) for x, y in zip(ref, cur))
+
+
+@pytest.mark.parametrize("mode", [L.POISSON_THOMAS, L.POISSON_MIXED, L.POISSON_PROJECTED])
+def test_binned_lanes_and_autotune_do_not_change_results(mode):
+    """Four lanes as size bins of equal work, each with the launch shapes the
+    autotuner picked by timing trial steps (threads per workgroup of every
+    free kernel class, multigrid LDS), give the one-sequence default run's
+    fields, scalars and per-case status bit for bit, and the autotuner
+    leaves the batch in the state it found it in."""
+    cases = L.batched_cases(48, seed=7, nmin=128, nmax=2048)
+    _say(f"binned lanes + autotune vs one sequence, 48 cases, mode {mode}, 2 + 4 steps")
+    ref, b = L.Batch(cases), L.Batch(cases)
+    for x in (ref, b):
+        x.set_poisson(mode, 1e-10, 100)
+    ref.set_lanes(1)
+    b.set_lanes(4)
+    b.set_lane_split(True)
+    for x in (ref, b):
+        x.run_all(2)
+    tab = b.autotune(1)
+    _say("  " + str(tab))
+    assert sum(t["cases"] for t in tab) == len(cases)
+    assert [t["nmax"] for t in tab] == sorted((t["nmax"] for t in tab), reverse=True)
+    for t in tab:
+        assert all(64 <= v <= 1024 for v in t["threads"].values())
+    for x in (ref, b):
+        x.run_all(4)
+    st_r, m_r = ref.progress()
+    st_b, m_b = b.progress()
+    assert np.array_equal(st_r, st_b) and np.array_equal(m_r, m_b)
+    for i in range(len(cases)):
+        for f in ("W", "P", "T", "PSIF", "PHI"):
+            assert np.array_equal(ref.field(f, i), b.field(f, i), equal_nan=True), (f, i)
+        assert np.array_equal(ref.scalar_array(i), b.scalar_array(i), equal_nan=True), i
+    ref.close()
+    b.close()
 
 
 @pytest.mark.parametrize("which,steps", [("deck", 50), ("large", 10)])

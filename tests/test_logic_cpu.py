@@ -293,3 +293,31 @@ def test_reference_poisson_seam(emul, mode, tol):
     col = P[:, 1, 1]
     assert np.max(np.abs(col[1:n + 1] - x)) <= tol * np.max(np.abs(x))
     assert col[0] == col[1] and col[n + 1] == col[n]
+
+
+def test_lane_bins_hold_equal_work(emul):
+    """The binned lane split: consecutive runs of the largest-first order,
+    each lane's work (n + 512 rows per case) within two cases' work of the
+    others', every lane's largest case recorded; the solving kernels get a
+    thread per chunk of the lane's largest case outside the Thomas mode; the
+    interleaved split gives lane l the l-th largest case."""
+    cases = L.batched_cases(200, seed=3)
+    b = L.Batch(cases, init=False, lib=emul)
+    b.set_lanes(4)
+    b.set_lane_split(True)
+    b.set_poisson(L.POISSON_MIXED, 1e-10, 100)
+    ns = sorted((int(c.n) for c in cases), reverse=True)
+    tab = b.tuning()
+    assert len(tab) == 4
+    pos, works = 0, []
+    for t in tab:
+        assert t["cases"] >= 1 and t["nmax"] == ns[pos]
+        works.append(sum(n + 512 for n in ns[pos:pos + t["cases"]]))
+        assert t["threads"]["momentum"] >= -(-t["nmax"] // 8)
+        assert t["threads"]["cont_energy"] >= -(-t["nmax"] // 8)
+        pos += t["cases"]
+    assert pos == len(cases)
+    assert max(works) - min(works) <= 2 * (ns[0] + 512)
+    b.set_lane_split(False)
+    assert [t["nmax"] for t in b.tuning()] == ns[:4]
+    b.close()
