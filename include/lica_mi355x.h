@@ -140,6 +140,21 @@ int lica_batch_save(lica_batch *b, const char *path);
 int lica_batch_load(const char *path, lica_batch **out);
 /* WRITEFIELD text file (1d.f:1121-1232, read by post.f READFLD 238-401) of one case */
 int lica_batch_write_fld(lica_batch *b, int64_t inst, const char *path, int64_t ihist);
+/* Restart of case inst of an initialized batch from a field file, as MAIN
+ * does with IREAD=1 and ITRACKING=1 (1d.f:117-143): PREFLD's reads
+ * (1d.f:972-1097: W, P, T, the band and VOL_TOT_ORI of the 1-D column; U
+ * and V must be zero, x/y planes uniform), then LVSINIT from the band read
+ * (two_phase.f:146-189) and GRID_COUPLING(1) on the device.  flags: the
+ * deck's LICA_FLD_IRESET (TIME = 0), LICA_FLD_IPZERO (P = 0),
+ * LICA_FLD_IDTOLD (DT = the file's DT, else the deck's).  The step counter
+ * starts again at 1, as MAIN's loop does; the case's Poisson history is
+ * cleared.  LICA_IO for a file that is not such a field file of this
+ * case's grid; otherwise the case's status (0, or a reference STOP such as
+ * LICA_PSIF_BUG found while rebuilding the level set).                   */
+#define LICA_FLD_IRESET 1
+#define LICA_FLD_IPZERO 2
+#define LICA_FLD_IDTOLD 4
+int lica_batch_read_fld(lica_batch *b, int64_t inst, const char *path, int64_t flags);
 
 /* ---- warm start of the Poisson solve ----------------------------------- */
 /* No counterpart in the reference.  LICA_POISSON_PROJECTED starts the

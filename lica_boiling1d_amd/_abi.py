@@ -120,6 +120,7 @@ def _declare(lib):
         "lica_batch_warm_info": (ctypes.c_int, [V, P_dbl, c_i64]),
         "lica_batch_set_lanes": (ctypes.c_int, [V, c_i64]),
         "lica_batch_set_lane_split": (ctypes.c_int, [V, c_i64]),
+        "lica_batch_read_fld": (ctypes.c_int, [V, c_i64, ctypes.c_char_p, c_i64]),
         "lica_batch_autotune": (ctypes.c_int, [V, c_i64]),
         "lica_batch_get_tuning": (ctypes.c_int, [V, P_i64, c_i64, P_i64]),
         "lica_net_param_count": (c_i64, []),

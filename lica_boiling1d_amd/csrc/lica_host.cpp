@@ -2236,3 +2236,218 @@ extern "C" int lica_batch_write_fld(lica_batch *b, int64_t inst, const char *pat
     fld_reals(f, v, 5);
     return (fclose(f) == 0) ? LICA_OK : LICA_IO;
 }
+
+// ---------------------------------------------------------------------
+// Restart of one case from a field file, as PREFLD reads it
+// (1d.f:972-1097) and LVSINIT rebuilds the level set from the band read
+// (ITRACKING=1, two_phase.f:146-189), followed by GRID_COUPLING(1) as MAIN
+// calls it (1d.f:117-143).  Each READ of PREFLD starts a new line; a list
+// longer than its format continues on the next line (format reversion):
+// 5ES20.12 lists hold five values a line, 8I8 eight, and so on.
+
+namespace {
+struct FldIn {
+    FILE *f = nullptr;
+    std::string line;
+    bool ok = true;
+    bool next()
+    {
+        line.clear();
+        int ch;
+        bool any = false;
+        while ((ch = fgetc(f)) != EOF) {
+            any = true;
+            if (ch == '\n') break;
+            if (ch != '\r') line.push_back((char)ch);
+        }
+        if (!any) ok = false;
+        return ok;
+    }
+    // the text of field i of width w on the current line (blank beyond its end)
+    std::string field(size_t at, int w) const
+    {
+        if (at >= line.size()) return std::string();
+        return line.substr(at, (size_t)w);
+    }
+    static bool real(const std::string &t0, double *v)
+    {
+        std::string t;
+        for (char c : t0)
+            if (c != ' ') t.push_back(c);
+        if (t.empty()) { *v = 0.0; return true; }   // a blank field reads as zero
+        // Fortran drops the E of a three-digit exponent (1.0-100)
+        if (t.find_first_of("EeDd") == std::string::npos) {
+            size_t p = t.find_last_of("+-");
+            if (p != std::string::npos && p > 0) t.insert(p, "E");
+        }
+        for (char &c : t)
+            if (c == 'D' || c == 'd') c = 'E';
+        char *end = nullptr;
+        *v = strtod(t.c_str(), &end);
+        return end && *end == '\0' && std::isfinite(*v);
+    }
+    static bool integer(const std::string &t0, long long *v)
+    {
+        std::string t;
+        for (char c : t0)
+            if (c != ' ') t.push_back(c);
+        if (t.empty()) { *v = 0; return true; }
+        char *end = nullptr;
+        *v = strtoll(t.c_str(), &end, 10);
+        return end && *end == '\0';
+    }
+    // one READ of a list of n reals, `per` fields of width w a line
+    bool reals(std::vector<double> &out, size_t n, int per, int w = 20)
+    {
+        out.assign(n, 0.0);
+        size_t i = 0;
+        do {
+            if (!next()) return false;
+            for (int j = 0; j < per && i < n; j++, i++)
+                if (!real(field((size_t)j * w, w), &out[i])) return ok = false;
+        } while (i < n);
+        return true;
+    }
+    bool ints(std::vector<long long> &out, size_t n, int per, int w = 8)
+    {
+        out.assign(n, 0);
+        size_t i = 0;
+        do {
+            if (!next()) return false;
+            for (int j = 0; j < per && i < n; j++, i++)
+                if (!integer(field((size_t)j * w, w), &out[i])) return ok = false;
+        } while (i < n);
+        return true;
+    }
+    // one line of ni I8 fields then nr ES20.12 fields
+    bool mixed(int ni, long long *iv, int nr, double *rv)
+    {
+        if (!next()) return false;
+        for (int j = 0; j < ni; j++)
+            if (!integer(field((size_t)j * 8, 8), &iv[j])) return ok = false;
+        for (int j = 0; j < nr; j++)
+            if (!real(field((size_t)ni * 8 + (size_t)j * 20, 20), &rv[j])) return ok = false;
+        return true;
+    }
+};
+
+// a (0:N1,0:N2) plane list of the 1-D column must hold one value per k
+bool column_of(const std::vector<double> &v, size_t per_k, size_t nk, std::vector<double> &col)
+{
+    col.assign(nk, 0.0);
+    for (size_t k = 0; k < nk; k++) {
+        const double *p = v.data() + k * per_k;
+        for (size_t j = 1; j < per_k; j++)
+            if (p[j] != p[0]) return false;
+        col[k] = p[0];
+    }
+    return true;
+}
+}  // namespace
+
+extern "C" int lica_batch_read_fld(lica_batch *b, int64_t inst, const char *path, int64_t flags)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (inst < 0 || inst >= b->ninst || !path || !b->initialized || (flags & ~7LL)) return LICA_BAD_ARG;
+    const InstParam &p = b->par[(size_t)inst];
+    const int n = p.n, n3 = p.n3, r = p.r;
+    FldIn in;
+    in.f = fopen(path, "r");
+    if (!in.f) return LICA_IO;
+    long long iv[5];
+    double rv[6];
+    std::vector<double> U, V, W, Pf, T, vol, misc;
+    std::vector<long long> numa, contact;
+    bool ok = in.mixed(3, iv, 0, rv);
+    const long long N1 = iv[0], N2 = iv[1], N3 = iv[2];
+    // the 1-D column: one interior cell in x and y (N1M = N2M = 1), the case's N3
+    ok = ok && N1 == 2 && N2 == 2 && N3 == n3;
+    long long ihist_m[2];
+    double time_dt[2];
+    ok = ok && in.mixed(2, ihist_m, 2, time_dt);
+    long long ips[4];
+    double pra;
+    ok = ok && in.mixed(4, ips, 1, &pra);
+    const long long ipss = ips[0];
+    ok = ok && (ipss == 0 || ipss == 1);
+    ok = ok && in.reals(misc, 1, 1);                                              // QVOL_ORI
+    ok = ok && in.reals(U, (size_t)(N1 * (N2 + 1) * (N3 + 1)), 5);
+    ok = ok && in.reals(V, (size_t)((N1 + 1) * N2 * (N3 + 1)), 5);
+    ok = ok && in.reals(W, (size_t)((N1 + 1) * (N2 + 1) * N3), 5);
+    ok = ok && in.reals(Pf, (size_t)((N1 - 1) * (N2 - 1) * (N3 - 1)), 5);
+    for (double x : U) ok = ok && x == 0.0;
+    for (double x : V) ok = ok && x == 0.0;
+    long long nf[3];
+    ok = ok && in.mixed(3, nf, 0, rv) && nf[2] == p.n3f;
+    ok = ok && in.reals(misc, 6, 6);                                              // DENR .. RE_AIR
+    long long nl_nmax[2];
+    ok = ok && in.mixed(2, nl_nmax, 0, rv) && nl_nmax[0] == 1 && nl_nmax[1] == LICA_MLAYER;
+    ok = ok && in.reals(vol, 1, 8);
+    ok = ok && in.ints(numa, LICA_MLAYER, 8);
+    // the band: every plane as r x r cells (I, J = 1..r) of one value
+    InstState s;
+    memset((void *)&s, 0, sizeof(s));
+    for (int nl = 1; nl <= LICA_MLAYER && ok; nl++) {
+        const long long cells = numa[(size_t)nl - 1];
+        if (cells < 0 || cells % ((long long)r * r) != 0 || cells / ((long long)r * r) > LICA_MAXP) { ok = false; break; }
+        s.bn[nl] = (int)(cells / ((long long)r * r));
+        for (int q = 0; q < s.bn[nl] && ok; q++)
+            for (int c = 0; c < r * r && ok; c++) {
+                long long e[3];
+                double a;
+                ok = in.mixed(3, e, 1, &a);
+                if (!ok) break;
+                if (c == 0) {
+                    s.bk[nl][q] = (int)e[2];
+                    s.alphi[nl][q] = a;
+                    ok = e[2] >= 1 && e[2] <= p.n3fm;
+                }
+                ok = ok && e[0] == 1 + c % r && e[1] == 1 + c / r && e[2] == s.bk[nl][q] && a == s.alphi[nl][q];
+            }
+    }
+    ok = ok && in.ints(contact, 1, 4) && in.ints(contact, 1, 4);                  // CONTACT, CONTACT_OLD
+    ok = ok && in.reals(misc, 1, 5) && in.reals(misc, 1, 5);                      // CONTACT_START, _TIME
+    ok = ok && in.ints(contact, 1, 4) && contact[0] == 1;                          // LVSON
+    ok = ok && in.reals(misc, 1, 5);                                               // T_CON
+    std::vector<double> tcol((size_t)n3 + 1, 0.0);
+    if (ok && ipss == 1) {
+        ok = in.reals(misc, 3, 3) && in.reals(T, (size_t)((N1 + 1) * (N2 + 1) * (N3 + 1)), 5);
+        ok = ok && column_of(T, (size_t)((N1 + 1) * (N2 + 1)), (size_t)N3 + 1, tcol);
+    }
+    fclose(in.f);
+    std::vector<double> wcol, pcol;
+    ok = ok && column_of(W, (size_t)((N1 + 1) * (N2 + 1)), (size_t)N3, wcol);     // W(k), k = 1..N3
+    ok = ok && column_of(Pf, 1, (size_t)n, pcol);                                 // P(k), k = 1..N3M
+    if (!ok) return LICA_IO;
+    // fresh geometry, then the fields read (PREFLD), IPZERO / IRESET / IDTOLD
+    // as MAIN applies them (1d.f:1070, 183-186, 1078-1083)
+    std::vector<double> ns = b->img_ns[(size_t)inst];
+    auto at = [&](int fld, int k) -> double & { return ns[(size_t)fld * p.ln + (size_t)(k + 2)]; };
+    for (int k = 1; k <= n3; k++) at(NS_W, k) = wcol[(size_t)k - 1];
+    for (int k = 1; k <= n; k++) at(NS_P, k) = (flags & 2) ? 0.0 : pcol[(size_t)k - 1];
+    for (int k = 0; k <= n3; k++) at(NS_T, k) = tcol[(size_t)k];
+    s.dt = (flags & 4) ? time_dt[1] : p.dt0;
+    s.time = (flags & 1) ? 0.0 : time_dt[0];
+    s.m = 0;
+    s.vol_tot_ori = vol[0];
+    s.vol_tot_iphs = vol[0];
+    s.sf_lo = 1; s.sf_hi = 0; s.im_lo = 1; s.im_hi = 0;
+    rc = h2d(b->d_ns + p.ns_off, ns.data(), sizeof(double) * ns.size(), b->stream);
+    rc = rc ? rc : h2d(b->d_rf + p.rf_off, b->img_rf[(size_t)inst].data(), sizeof(double) * b->img_rf[(size_t)inst].size(), b->stream);
+    rc = rc ? rc : h2d(b->d_nsi + p.nsi_off, b->img_nsi[(size_t)inst].data(), sizeof(int) * b->img_nsi[(size_t)inst].size(), b->stream);
+    rc = rc ? rc : h2d(b->d_rfi + p.rfi_off, b->img_rfi[(size_t)inst].data(), sizeof(int) * b->img_rfi[(size_t)inst].size(), b->stream);
+    rc = rc ? rc : h2d(b->d_st + inst, &s, sizeof(InstState), b->stream);
+    if (rc) return rc;
+    // the case's own workgroup: its place in the largest-first order
+    size_t pos = 0;
+    while (pos < b->order_h.size() && b->order_h[pos] != (int)inst) pos++;
+    BatchDev bd = make_bd(b, 1);
+    bd.order = b->d_order + pos;
+    bd.ninst = 1;
+    if (lica_launch_init_fld(bd, (void *)b->stream)) return LICA_HIP_ERROR;
+    rc = dsync(b->stream);
+    if (rc) return rc;
+    rc = get_state(b, inst, &s);
+    return rc ? rc : s.status;
+}

@@ -460,3 +460,68 @@ DEV void body_transport(const BatchDev &bd, int inst, TransportSh &T)
     }
     SERIAL_END
 }
+
+// The level set of a case restarted from a field file: LVSINIT with the
+// band read (ITRACKING=1, two_phase.f:146-189) -- ALPHI_EXT from the band,
+// ALPHI_BC, CAL_PSIF(1,N_MAX) (the band formula on every layer and the
+// column fill elsewhere, two_phase.f:2835-3164) -- then GRID_COUPLING(1)
+// (4863-5118) and the CFL input of the first DETERMINE_DT.  The host has
+// put W, P, T, the band (S.bn, S.bk, S.alphi), the time, DT and
+// VOL_TOT_ORI in place, as PREFLD reads them (1d.f:972-1097).
+DEV void body_init_fld(const BatchDev &bd, int inst, TransportSh &T)
+{
+    InstState &S = bd.st[inst];
+    BandSh &B = T.B;
+    CaseP cp = case_ptrs(bd, inst);
+    const InstParam &P = *cp.P;
+    const int n = P.n, n3 = P.n3, n3f = P.n3f, n3fm = P.n3fm;
+    band_load(B, P, S);
+    if (B.status) { SERIAL { S.status = B.status; } SERIAL_END return; }
+    PFOR(i, 0, WMAX) { B.phw[i] = 0.0; }
+    PFOR_END
+    PFOR(e, 0, B.off[LICA_MLAYER + 1]) { B.phw[B.ent_k[e] - B.wlo] = S.alphi[B.ent_n[e]][B.ent_q[e]]; }
+    PFOR_END
+    SERIAL { alphi_bc(B, P); }
+    SERIAL_END
+    double *PSIF = cp.R(RF_PSIF);
+    psif_mtp_band(B, P, LICA_MLAYER, B.pm1);
+    SERIAL {
+        const Fill f = fill_params(B, P);
+        B.iv[0] = f.knum; B.iv[1] = f.kc_low; B.iv[2] = f.khigh;
+        B.v[0] = f.tmp1; B.v[1] = f.tmp_outer;
+    }
+    SERIAL_END
+    Fill f;
+    f.knum = B.iv[0]; f.kc_low = B.iv[1]; f.khigh = B.iv[2]; f.tmp1 = B.v[0]; f.tmp_outer = B.v[1];
+    PFOR(k, 1, n3fm + 1) { PSIF[k] = fill_val(f, k); }
+    PFOR_END
+    PFOR(e, 0, B.off[LICA_MLAYER + 1]) { PSIF[B.ent_k[e]] = B.pm1[e]; }
+    PFOR_END
+    SERIAL {
+        if (B.status) S.status = B.status;
+        for (int k = 0; k <= P.r; k++) { PSIF[-k] = PSIF[1]; PSIF[n3f + k] = PSIF[n3fm]; }
+    }
+    SERIAL_END
+    if (S.status) return;
+    // GRID_COUPLING(1): every NS cell
+    double *PZN = cp.A(NS_PSI_ZN), *PCN = cp.A(NS_PSI_CN);
+    const double *W = cp.A(NS_W), *SSDZ = cp.A(NS_SSDZ);
+    const double *ZP = cp.A(NS_ZP), *Z = cp.A(NS_Z), *ZF = cp.R(RF_ZF);
+    const int *KCOU1 = cp.AI(NSI_KCOU1), *KCOU2 = cp.AI(NSI_KCOU2);
+    const int *KCOUMP1 = cp.AI(NSI_KCOUMP1), *KCOUMP2 = cp.AI(NSI_KCOUMP2);
+    auto fpsif = [&](int kk) { return PSIF[kk]; };
+    PFOR(k, 1, n + 1) {
+        if (k >= 2) PZN[k] = snap3(box_avg(ZF, P.ssdzf, fpsif, ZP[k - 1], ZP[k], KCOUMP2[k - 1], KCOUMP1[k]));
+        else PZN[k] = snap3(PZN[k]);
+        PCN[k] = snap3(box_avg(ZF, P.ssdzf, fpsif, Z[k], Z[k + 1], KCOU2[k], KCOU1[k + 1]));
+    }
+    PFOR_END
+    SERIAL {
+        PCN[0] = PCN[1];
+        PCN[n3] = PCN[n];
+    }
+    SERIAL_END
+    double c3max = red_max(1, n + 1, 0.0, [&](int k) { return fabs(W[k] + W[k + 1]) * SSDZ[k]; }, B.red);
+    SERIAL { S.c3max = c3max; }
+    SERIAL_END
+}

@@ -176,6 +176,18 @@ __global__ void __launch_bounds__(64) k_transport(BatchDev bd)
     __shared__ TransportSh sh;
     body_transport(bd, bd.order[blockIdx.x], sh);
 }
+__global__ void __launch_bounds__(64) k_init_fld(BatchDev bd)
+{
+    __shared__ TransportSh sh;
+    body_init_fld(bd, bd.order[blockIdx.x], sh);
+}
+
+int lica_launch_init_fld(const BatchDev &bd, void *stream)
+{
+    if (bd.ninst <= 0) return 0;
+    hipLaunchKernelGGL(k_init_fld, dim3(bd.ninst), dim3(64), 0, (hipStream_t)stream, bd);
+    return (int)hipGetLastError();
+}
 
 static int tri_lds_setup();
 
@@ -353,6 +365,13 @@ int lica_launch(int which, const BatchDev &bd, int threads, void *stream)
     case KL_CONVRGE1: emul_run<BlkSh>(bd, threads, body_convrge1); break;
     default: return -1;
     }
+    return 0;
+}
+
+int lica_launch_init_fld(const BatchDev &bd, void *stream)
+{
+    (void)stream;
+    emul_run<TransportSh>(bd, 64, body_init_fld);
     return 0;
 }
 #endif

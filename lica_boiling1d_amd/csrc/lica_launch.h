@@ -15,6 +15,8 @@ enum KernelId {
 int lica_tri_chunks(int nrow);
 
 int lica_launch(int which, const BatchDev &bd, int threads, void *stream);
+// the level set of cases restarted from a field file (one wave per case)
+int lica_launch_init_fld(const BatchDev &bd, void *stream);
 int lica_launch_tri(int nsys, int n, double *a, double *b, double *c, double *g, int mode, int threads, void *stream);
 int lica_launch_mg(int nsys, int n, const double *a, const double *b, const double *c, const double *g, double *p,
                    double *scratch, int mode, double rtol, int maxc, int *cycles, double *resid, int threads,

@@ -121,6 +121,18 @@ class Batch(_Handle):
         """WRITEFIELD text file of one case (1d.f:1121-1232)."""
         _check(self._lib.lica_batch_write_fld(self._h, int(inst), str(path).encode(), int(ihist)), "write_fld")
 
+    def read_fld(self, path, inst=0, ireset=True, ipzero=False, idtold=False):
+        """Restart case `inst` from a field file as MAIN does with IREAD=1,
+        ITRACKING=1 (PREFLD 1d.f:972-1097, LVSINIT two_phase.f:146-189,
+        GRID_COUPLING(1)); ireset/ipzero/idtold are the deck's IRESET,
+        IPZERO, IDTOLD (lica.in line 2; the shipped deck has 1, 0, 0).
+        Returns the case's status (0, or a reference STOP)."""
+        flags = (1 if ireset else 0) | (2 if ipzero else 0) | (4 if idtold else 0)
+        rc = self._lib.lica_batch_read_fld(self._h, int(inst), str(path).encode(), flags)
+        if rc != 0 and not _abi.is_case_status(rc):
+            _check(rc, "read_fld")
+        return int(rc)
+
     def init(self):
         _check(self._lib.lica_batch_init(self._h), "lica_batch_init")
 

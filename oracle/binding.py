@@ -90,12 +90,108 @@ def load():
         lib.orc_test_temp_grad.restype = ctypes.c_int
         lib.orc_last_poisson_inputs.argtypes = [V, P_dbl, P_dbl]
         lib.orc_last_poisson_inputs.restype = c_i64
+        lib.orc_init_fld.argtypes = [V, P_dbl, P_dbl, P_dbl, c_dbl, c_dbl, c_dbl, P_i64, P_i64, P_dbl]
+        lib.orc_init_fld.restype = ctypes.c_int
         _lib = lib
     return _lib
 
 
 def _p(a):
     return a.ctypes.data_as(P_dbl)
+
+
+ORC_M_MAX, ORC_MAXP = 11, 16
+
+
+def _fortran_real(t):
+    t = t.strip()
+    if not t:
+        return 0.0
+    u = t.upper().replace("D", "E")
+    if "E" not in u:
+        # a three-digit exponent written without its E (1.0-100)
+        j = max(u.rfind("+"), u.rfind("-"))
+        if j > 0:
+            u = u[:j] + "E" + u[j:]
+    return float(u)
+
+
+class _Records:
+    """Fortran formatted READs of a text file: every READ starts a new line,
+    a list longer than its format continues on the next one."""
+
+    def __init__(self, path):
+        with open(path) as f:
+            self.lines = f.read().split("\n")
+        self.i = 0
+
+    def line(self):
+        s = self.lines[self.i]
+        self.i += 1
+        return s
+
+    def read(self, widths, kinds, count=None):
+        """One READ: `count` values (default len(widths)) in fields of the
+        given widths and kinds ('i' or 'r'), the format reused per line."""
+        count = len(widths) if count is None else count
+        out = []
+        while True:
+            s = self.line()
+            at = 0
+            for w, kd in zip(widths, kinds):
+                if len(out) == count:
+                    break
+                fld = s[at:at + w]
+                at += w
+                out.append(int(fld) if kd == "i" and fld.strip() else (0 if kd == "i" else _fortran_real(fld)))
+            if len(out) >= count:
+                return out
+
+
+def read_fld(path):
+    """The lists PREFLD reads from a field file (1d.f:972-1097), for the
+    1-D column: N1..N3, IHIST, M, TIME, DT, IPSS, W(k), P(k), T(k),
+    VOL_TOT_ORI and the band as ordered planes per layer (each the r*r
+    list entries of one z-plane)."""
+    r = _Records(path)
+    n1, n2, n3 = r.read([8] * 3, "iii")
+    ihist, m, time, dt = r.read([8, 8, 20, 20], "iirr")
+    ipss = r.read([8, 8, 8, 8, 20], "iiiir")[0]
+    r.read([20], "r")
+    u = r.read([20] * 5, "r" * 5, n1 * (n2 + 1) * (n3 + 1))
+    v = r.read([20] * 5, "r" * 5, (n1 + 1) * n2 * (n3 + 1))
+    w = r.read([20] * 5, "r" * 5, (n1 + 1) * (n2 + 1) * n3)
+    p = r.read([20] * 5, "r" * 5, (n1 - 1) * (n2 - 1) * (n3 - 1))
+    n1f, n2f, n3f = r.read([8] * 3, "iii")
+    r.read([20] * 6, "r" * 6)
+    nlvs, nmax = r.read([8, 8], "ii")
+    vol = r.read([20] * 8, "r" * 8, nlvs)
+    numa = r.read([8] * 8, "i" * 8, nmax * nlvs)
+    rr = n1f - 1
+    band = []
+    for nl in range(nmax):
+        planes = []
+        for q in range(numa[nl] // (rr * rr)):
+            cells = [r.read([8, 8, 8, 20], "iiir") for _ in range(rr * rr)]
+            assert all(c[2] == cells[0][2] and c[3] == cells[0][3] for c in cells)
+            planes.append((cells[0][2], cells[0][3]))
+        band.append(planes)
+    r.read([8] * 4, "i" * 4, nlvs * nlvs)        # CONTACT
+    r.read([8] * 4, "i" * 4, nlvs * nlvs)        # CONTACT_OLD
+    r.read([20] * 5, "r" * 5, nlvs * nlvs)       # CONTACT_START
+    r.read([20] * 5, "r" * 5, nlvs * nlvs)       # CONTACT_TIME
+    r.read([8] * 4, "i" * 4, nlvs)               # LVSON
+    r.read([20] * 5, "r" * 5, nlvs * nlvs)       # T_CON
+    t = None
+    if ipss == 1:
+        r.read([20] * 3, "rrr")
+        t = r.read([20] * 5, "r" * 5, (n1 + 1) * (n2 + 1) * (n3 + 1))
+    per = (n1 + 1) * (n2 + 1)
+    col = lambda a, nk: np.array([a[k * per] for k in range(nk)])
+    return {"n3": n3, "n3f": n3f, "ihist": ihist, "m": m, "time": time, "dt": dt, "ipss": ipss,
+            "u0": not any(u) and not any(v),
+            "w": col(w, n3), "p": np.array(p), "t": col(t, n3 + 1) if t is not None else np.zeros(n3 + 1),
+            "vol": vol[0], "band": band}
 
 
 class Oracle:
@@ -125,6 +221,31 @@ class Oracle:
 
     def step(self, nsteps=1):
         return self.lib.orc_step(self.h, int(nsteps))
+
+    def init_fld(self, path, ireset=True, ipzero=False, idtold=False):
+        """In place of init: MAIN with IREAD=1, ITRACKING=1 from a field
+        file (PREFLD, LVSINIT from the band read, GRID_COUPLING(1)), with
+        the deck's IRESET / IPZERO / IDTOLD."""
+        d = read_fld(path)
+        n3 = d["n3"]
+        w = np.zeros(n3 + 1)
+        w[1:] = d["w"]
+        p = np.zeros(n3 + 1)
+        if not ipzero:
+            p[1:n3] = d["p"]
+        t = np.array(d["t"], dtype=np.float64)
+        bn = np.zeros(ORC_M_MAX + 1, dtype=np.int64)
+        bk = np.zeros((ORC_M_MAX + 1, ORC_MAXP), dtype=np.int64)
+        al = np.zeros((ORC_M_MAX + 1, ORC_MAXP))
+        for nl, planes in enumerate(d["band"], start=1):
+            bn[nl] = len(planes)
+            for q, (k, a) in enumerate(planes):
+                bk[nl, q] = k
+                al[nl, q] = a
+        P_i64 = ctypes.POINTER(c_i64)
+        return self.lib.orc_init_fld(self.h, _p(w), _p(p), _p(t), 0.0 if ireset else d["time"],
+                                     d["dt"] if idtold else float(self.case.dt0), d["vol"],
+                                     bn.ctypes.data_as(P_i64), bk.ctypes.data_as(P_i64), _p(al))
 
     def field(self, name):
         f = ORC_FIELDS[name] if isinstance(name, str) else int(name)

@@ -54,10 +54,10 @@
 #include <omp.h>
 #endif
 
-#define M_MAX 11
+#define M_MAX ORC_M_MAX
 #define NN1 8       /* N_MAX-3, two_phase.f:744 */
 #define NN2 10      /* N_MAX-1, two_phase.f:745 */
-#define MAXP 16     /* z-planes stored per band layer */
+#define MAXP ORC_MAXP  /* z-planes stored per band layer */
 #define ORC_MAX_HALVINGS 20
 
 #ifdef ORC_CHECK
@@ -1704,6 +1704,38 @@ int orc_init(orc_inst *s)
         for (int64_t q = 0; q < s->bn[nl]; q++) s->alphi[nl][q] = D(s->phi, s->bk[nl][q]);
     grid_coupling(s, 1);
     s->vol_tot_iphs = s->vol_tot_ori;
+    return ORC_OK;
+}
+
+/* MAIN with IREAD=1, ITRACKING=1 (1d.f:117-143): the fields PREFLD read
+ * (1d.f:972-1097; w, p, t indexed 0..N3, IPZERO/IRESET/IDTOLD already
+ * applied by the caller), the band (bn[1..M_MAX], bk/alphi M_MAX+1 rows of
+ * MAXP), LVSINIT from that band (two_phase.f:146-189: ALPHI_EXT from the
+ * band, ALPHI_BC, CAL_PSIF(1,N_MAX)) and GRID_COUPLING(1).  On a fresh
+ * instance, in place of orc_init.                                         */
+int orc_init_fld(orc_inst *s, const double *w, const double *p, const double *t, double time, double dt,
+                 double vol_tot_ori, const int64_t *bn, const int64_t *bk, const double *alphi)
+{
+    if (s->status) return s->status;
+    s->dt = dt;
+    s->time = time;
+    s->m = 0;
+    for (int64_t k = 0; k <= s->n3; k++) { D(s->w, k) = w[k]; D(s->p, k) = p[k]; D(s->t, k) = t[k]; }
+    for (int nl = 1; nl <= M_MAX; nl++) {
+        if (bn[nl] < 0 || bn[nl] > MAXP) return s->status = ORC_BAND_CAPACITY;
+        s->bn[nl] = bn[nl];
+        for (int64_t q = 0; q < bn[nl]; q++) {
+            s->bk[nl][q] = bk[(size_t)nl * MAXP + (size_t)q];
+            s->alphi[nl][q] = alphi[(size_t)nl * MAXP + (size_t)q];
+        }
+    }
+    phi_from_band(s);
+    alphi_bc(s);
+    int st = cal_psif(s, 1, M_MAX);
+    if (st) return s->status = st;
+    s->vol_tot_ori = vol_tot_ori;
+    s->vol_tot_iphs = vol_tot_ori;
+    grid_coupling(s, 1);
     return ORC_OK;
 }
 

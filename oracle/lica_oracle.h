@@ -35,6 +35,9 @@ typedef struct {
 
 typedef struct orc_inst orc_inst;
 
+#define ORC_M_MAX 11    /* band layers (N_MAX)            */
+#define ORC_MAXP 16     /* z-planes stored per band layer */
+
 /* status codes */
 enum {
     ORC_OK = 0,
@@ -67,6 +70,11 @@ enum {
 orc_inst *orc_new(const orc_case *c);
 void orc_free(orc_inst *s);
 int orc_init(orc_inst *s);
+/* restart from a field file's contents (1d.f:117-143 with IREAD=1,
+ * ITRACKING=1): w, p, t indexed 0..N3; bn[1..M_MAX]; bk and alphi rows
+ * nl = 0..M_MAX of ORC_MAXP slots; call on a fresh instance              */
+int orc_init_fld(orc_inst *s, const double *w, const double *p, const double *t, double time, double dt,
+                 double vol_tot_ori, const int64_t *bn, const int64_t *bk, const double *alphi);
 int orc_step(orc_inst *s, int64_t nsteps);
 int64_t orc_get(orc_inst *s, int field, double *out, int64_t cap);
 void orc_scalars(orc_inst *s, double *out);

@@ -366,3 +366,57 @@ def test_reference_poisson_seam(mode, tol):
     assert np.max(np.abs(col[1:n + 1] - x)) <= tol * np.max(np.abs(x))
     assert col[0] == col[1] and col[n + 1] == col[n]
     seams.close()
+
+
+def test_fld_writer_and_restart(tmp_path):
+    """The field file the GPU writes after 6 steps of the deck case holds the
+    oracle's W, P, T and band as WRITEFIELD prints them (ES20.12: every
+    value parses back to the oracle's value printed that way).  Restarting
+    from it (IREAD=1, ITRACKING=1: PREFLD, LVSINIT from the band read,
+    GRID_COUPLING(1)) as the second case of a three-case batch gives the
+    oracle's restart fields, and 6 steps later its fields and scalars, bit
+    for bit, while the batch's other cases run on as in a batch that was
+    never touched."""
+    from oracle.binding import read_fld
+    case = L.deck_case()
+    _say("field file of the deck case after 6 steps: writer vs oracle, restart as case 1 of 3")
+    g, o = L.Batch([case]), Oracle(case)
+    g.run(6)
+    assert o.step(6) == 0
+    path = tmp_path / "fld000006"
+    g.write_fld(path, 0, 6)
+    d = read_fld(path)
+    es = lambda v: float("%20.12E" % v)
+    assert d["n3"] == case.n + 1 and d["u0"]
+    assert np.array_equal(d["w"], [es(x) for x in o.field("W")[1:]])
+    assert np.array_equal(d["p"], [es(x) for x in o.field("P")[1:case.n + 1]])
+    assert np.array_equal(d["t"], [es(x) for x in o.field("T")])
+    bandf, alf = o.field("BAND"), o.field("ALPHI")
+    pos, ai = 0, 0
+    for planes in d["band"]:
+        cnt = int(bandf[pos])
+        assert [k for k, _ in planes] == [int(x) for x in bandf[pos + 1:pos + 1 + cnt]]
+        assert [a for _, a in planes] == [es(x) for x in alf[ai:ai + cnt]]
+        pos += 1 + cnt
+        ai += cnt
+    g.close()
+    others = [L.deck_case(300, 2), case, L.deck_case(200, 2)]
+    b, ref = L.Batch(others), L.Batch(others)
+    for x in (b, ref):
+        x.run(2)
+    assert b.read_fld(path, 1, ireset=False, idtold=True) == 0
+    o2 = Oracle(case, init=False)
+    assert o2.init_fld(path, ireset=False, idtold=True) == 0
+    fields = ("W", "P", "T", "PSI_CN", "PSI_ZN", "PSIF", "PHI")
+    for f in fields:
+        assert np.array_equal(b.field(f, 1), o2.field(f)), f
+    for x in (b, ref):
+        x.run(6)
+    assert o2.step(6) == 0
+    for f in fields:
+        assert np.array_equal(b.field(f, 1), o2.field(f)), f
+        for i in (0, 2):
+            assert np.array_equal(b.field(f, i), ref.field(f, i)), (f, i)
+    assert np.array_equal(b.scalar_array(1)[:9], o2.scalar_array()[:9])
+    b.close()
+    ref.close()

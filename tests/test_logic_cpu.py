@@ -321,3 +321,91 @@ def test_lane_bins_hold_equal_work(emul):
     b.set_lane_split(False)
     assert [t["nmax"] for t in b.tuning()] == ns[:4]
     b.close()
+
+
+@pytest.mark.parametrize("ireset,ipzero,idtold", [(False, False, True), (True, True, False)])
+def test_fld_restart_matches_oracle(emul, tmp_path, ireset, ipzero, idtold):
+    """Restart from a field file (IREAD=1, ITRACKING=1): PREFLD's reads,
+    LVSINIT from the band read, GRID_COUPLING(1).  The CPU logic-check
+    build and the oracle -- each with its own parser of the file -- give the
+    same fields at the restart and five steps later, bit for bit; writing
+    the restarted case again gives the file back but for the step counter
+    (MAIN's loop starts again at M=1)."""
+    from oracle.binding import read_fld
+    case = L.deck_case()
+    a = L.Batch([case], lib=emul)
+    a.run(5)
+    f1, f2 = tmp_path / "fld000005", tmp_path / "again"
+    a.write_fld(f1, 0, 5)
+    b = L.Batch([case], lib=emul)
+    assert b.read_fld(f1, 0, ireset=ireset, ipzero=ipzero, idtold=idtold) == 0
+    o = Oracle(case, init=False)
+    assert o.init_fld(f1, ireset=ireset, ipzero=ipzero, idtold=idtold) == 0
+    fields = ("W", "P", "T", "PSI_CN", "PSI_ZN", "PSIF", "PHI")
+    for f in fields:
+        assert np.array_equal(b.field(f), o.field(f)), f
+    d = read_fld(f1)
+    s = b.scalars()
+    assert s["TIME"] == (0.0 if ireset else d["time"])
+    assert s["DT"] == (d["dt"] if idtold else case.dt0)
+    assert s["VOL_ORI"] == d["vol"]
+    if not ipzero:
+        b.write_fld(f2, 0, 5)
+        x, y = open(f1).read().split("\n"), open(f2).read().split("\n")
+        assert len(x) == len(y) and [i for i in range(len(x)) if x[i] != y[i]] == [1]
+    b.run(5)
+    assert o.step(5) == 0
+    for f in fields:
+        assert np.array_equal(b.field(f), o.field(f)), f
+    assert np.array_equal(b.scalar_array()[:9], o.scalar_array()[:9])
+
+
+def test_fld_reader_formats_and_refusals(emul, tmp_path):
+    """Fortran's ES20.12 drops the E of a three-digit exponent
+    (-1.234567890123-100); both parsers read it.  Files that are not a field
+    file of the case's grid -- another N3, U not zero, x/y planes not
+    uniform, cut short -- are refused with LICA_IO (status 22) and leave
+    the batch usable."""
+    from oracle.binding import _fortran_real, read_fld
+    assert _fortran_real("-1.234567890123-100") == -1.234567890123e-100
+    assert _fortran_real(" 2.500000000000E+00") == 2.5
+    case = L.deck_case()
+    a = L.Batch([case], lib=emul)
+    a.run(3)
+    good = tmp_path / "good"
+    a.write_fld(good, 0, 3)
+    lines = open(good).read().split("\n")
+    n3 = case.n + 1
+    # T(0:2,0:2,0) are the first nine values of the last list: give them a
+    # three-digit exponent in Fortran's form
+    nt = 9 * (n3 + 1)
+    first = len(lines) - 1 - (nt + 4) // 5
+    tiny = "%20s" % "1.000000000000-100"
+    lines[first] = tiny * 5
+    lines[first + 1] = tiny * 4 + lines[first + 1][80:]
+    fortran = tmp_path / "fortran"
+    open(fortran, "w").write("\n".join(lines))
+    b = L.Batch([case], lib=emul)
+    assert b.read_fld(fortran, 0, ireset=False, idtold=True) == 0
+    assert b.field("T")[0] == 1e-100
+    assert read_fld(fortran)["t"][0] == 1e-100
+    bad = {}
+    other = L.Batch([L.deck_case(200, 2)], lib=emul)
+    other.write_fld(tmp_path / "other", 0, 0)
+    bad["other grid"] = tmp_path / "other"
+    lines = open(good).read().split("\n")
+    u = list(lines)
+    u[4] = "%20.12E" % 1.0 + u[4][20:]
+    bad["U"] = tmp_path / "u"
+    open(bad["U"], "w").write("\n".join(u))
+    bad["cut"] = tmp_path / "cut"
+    open(bad["cut"], "w").write("\n".join(lines[:len(lines) // 2]))
+    nu = list(lines)
+    nu[first] = nu[first][:20] + "%20.12E" % 0.5 + nu[first][40:]
+    bad["T not uniform in x/y"] = tmp_path / "nonuniform"
+    open(bad["T not uniform in x/y"], "w").write("\n".join(nu))
+    for why, path in bad.items():
+        with pytest.raises(L.LicaError) as e:
+            b.read_fld(path, 0)
+        assert e.value.status == 22, why
+    b.run(1)
