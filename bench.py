@@ -151,12 +151,26 @@ def cpu_baseline(cases, workload, steps=3):
                 **info}
     st, done, t_set, t_step, thr = run_cases(cases, steps)
     v = float(done.sum()) / t_step
-    return {"value": v, "unit": "instance-steps/s", "cores": thr, "kind": "port",
-            "sample": f"all {len(cases)} cases of the same batch, {steps} steps each ({int(done.sum())} "
-                      f"instance-steps, {int((st != 0).sum())} cases stopped) by the FP64 oracle, one case per "
-                      f"OpenMP thread on {thr} threads, {t_step:.1f} s of stepping; set-up ({t_set:.1f} s) "
-                      "excluded",
-            "mode": "instance-parallel", **info}
+    out = {"value": v, "unit": "instance-steps/s", "cores": thr, "kind": "port",
+           "sample": f"all {len(cases)} cases of the same batch, {steps} steps each ({int(done.sum())} "
+                     f"instance-steps, {int((st != 0).sum())} cases stopped) by the FP64 oracle, one case per "
+                     f"OpenMP thread on {thr} threads, {t_step:.1f} s of stepping; set-up ({t_set:.1f} s) "
+                     "excluded",
+           "mode": "instance-parallel", **info}
+    # mode (i), the reference's own style: one case at a time, its per-cell
+    # extrapolation loops shared by the threads; on a sample of the batch
+    # (every 64th case), with one thread and with all of them
+    from oracle.binding import run_single
+    sample = cases[::64]
+    mode_i = {}
+    for t in (1, thr):
+        st1, done1, _, t1, used = run_single(sample, steps, t)
+        mode_i[f"threads_{used}"] = float(done1.sum()) / t1 if t1 > 0 else None
+    out["mode_i"] = {"unit": "instance-steps/s", **mode_i,
+                     "sample": f"{len(sample)} cases (every 64th of the batch), {steps} steps each, one case at a "
+                               "time with the pseudo-time extrapolation's per-cell loops shared by the OpenMP "
+                               "threads (the reference's !$OMP PARALLEL DO style)"}
+    return out
 
 
 def lanes_of(cases, args):

@@ -90,6 +90,9 @@ def load():
         lib.orc_test_temp_grad.restype = ctypes.c_int
         lib.orc_last_poisson_inputs.argtypes = [V, P_dbl, P_dbl]
         lib.orc_last_poisson_inputs.restype = c_i64
+        lib.orc_set_loop_threads.argtypes = [V, ctypes.c_int]
+        lib.orc_run_single.argtypes = [ctypes.POINTER(OrcCase), c_i64, c_i64, P_i64, P_i64, P_dbl, P_dbl]
+        lib.orc_run_single.restype = ctypes.c_int
         lib.orc_init_fld.argtypes = [V, P_dbl, P_dbl, P_dbl, c_dbl, c_dbl, c_dbl, P_i64, P_i64, P_dbl]
         lib.orc_init_fld.restype = ctypes.c_int
         _lib = lib
@@ -222,6 +225,11 @@ class Oracle:
     def step(self, nsteps=1):
         return self.lib.orc_step(self.h, int(nsteps))
 
+    def set_loop_threads(self, on=True):
+        """Share the per-cell loops of the pseudo-time extrapolation among
+        the OpenMP threads (the reference's style within one case)."""
+        self.lib.orc_set_loop_threads(self.h, 1 if on else 0)
+
     def init_fld(self, path, ireset=True, ipzero=False, idtold=False):
         """In place of init: MAIN with IREAD=1, ITRACKING=1 from a field
         file (PREFLD, LVSINIT from the band read, GRID_COUPLING(1)), with
@@ -342,3 +350,26 @@ def run_cases(cases, nsteps):
     thr = lib.orc_run_cases(arr, len(cases), int(nsteps), st.ctypes.data_as(P_i64), steps.ctypes.data_as(P_i64),
                             ctypes.byref(t_set), ctypes.byref(t_step))
     return st, steps, t_set.value, t_step.value, int(thr)
+
+
+def run_single(cases, nsteps, threads=0):
+    """The reference's style: the cases one after the other, each with its
+    per-cell extrapolation loops shared by `threads` OpenMP threads (0: the
+    default count).  Returns (status, steps entered, set-up seconds,
+    stepping seconds, threads)."""
+    lib = load()
+    st = np.zeros(len(cases), dtype=np.int64)
+    steps = np.zeros(len(cases), dtype=np.int64)
+    t_set = t_step = 0.0
+    thr = 1
+    P_i64 = ctypes.POINTER(c_i64)
+    for i, c in enumerate(cases):
+        oc = OrcCase()
+        for f, _ in OrcCase._fields_:
+            setattr(oc, f, getattr(c, f))
+        s1, s2 = ctypes.c_double(), ctypes.c_double()
+        thr = lib.orc_run_single(ctypes.byref(oc), int(nsteps), int(threads), st[i:].ctypes.data_as(P_i64),
+                                 steps[i:].ctypes.data_as(P_i64), ctypes.byref(s1), ctypes.byref(s2))
+        t_set += s1.value
+        t_step += s2.value
+    return st, steps, t_set, t_step, int(thr)

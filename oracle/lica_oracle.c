@@ -60,6 +60,18 @@
 #define MAXP ORC_MAXP  /* z-planes stored per band layer */
 #define ORC_MAX_HALVINGS 20
 
+/* The reference's OpenMP style within one case (!$OMP PARALLEL DO over K,
+ * e.g. chphase.f:620-670): with s->omp set, the per-cell loops of the
+ * pseudo-time extrapolation -- the oracle's dominant work -- are shared by
+ * the threads.  Every such loop writes only its own cell and reads nothing
+ * the loop writes, and its maximum is combined by fmax (order-free), so
+ * the results do not change by a bit.                                    */
+#ifdef _OPENMP
+#define ORC_PFOR _Pragma("omp parallel for schedule(static) if (s->omp)")
+#else
+#define ORC_PFOR
+#endif
+
 #ifdef ORC_CHECK
 #define CHK(c) do { if (!(c)) { fprintf(stderr, "oracle bounds: %s at %s:%d\n", #c, __FILE__, __LINE__); abort(); } } while (0)
 #else
@@ -120,6 +132,7 @@ struct orc_inst {
     iarr mask_s, maskc;
     /* last Poisson system (diagnostic) */
     darr pa, pb, pc, pg;
+    int omp;    /* per-cell loops shared by the OpenMP threads (one case at a time) */
 };
 
 /* ------------------------------------------------------------------ */
@@ -1214,7 +1227,9 @@ static void temp_extp_rk3(orc_inst *s, darr *tmpo, int im, double redtl, int nd,
 {
     int64_t n = s->n3m;
     double sm_grid = fmin(fmin(2.0 * s->sdxf, 2.0 * s->sdyf), 2.0 * s->sdzf);
+    ORC_PFOR
     for (int64_t k = 1; k <= n; k++) D(s->temp_new, k) = D(*tmpo, k);
+    ORC_PFOR
     for (int64_t k = 1; k <= n; k++) {
         double np = D(s->nphi, k);
         if (np == 0.0) continue;
@@ -1244,7 +1259,36 @@ static void temp_extp_rk3(orc_inst *s, darr *tmpo, int im, double redtl, int nd,
             D(s->temp_new, k) = D(*tmpo, k) - redtl * drhs;
         }
     }
+    ORC_PFOR
     for (int64_t k = 1; k <= n; k++) D(*tmpo, k) = D(s->temp_new, k);
+}
+
+/* max over cells 1..n of f(k) >= 0 (NaN ignored, as fmax does), the
+ * threads' partial maxima combined by fmax                               */
+#define ORC_PMAX(out, n, expr)                                              \
+    do {                                                                    \
+        double acc_ = 0.0;                                                  \
+        _Pragma("omp parallel if (s->omp)")                                 \
+        {                                                                   \
+            double part_ = 0.0;                                             \
+            _Pragma("omp for schedule(static) nowait")                      \
+            for (int64_t k = 1; k <= (n); k++) part_ = fmax((expr), part_); \
+            _Pragma("omp critical")                                         \
+            acc_ = fmax(part_, acc_);                                       \
+        }                                                                   \
+        (out) = acc_;                                                       \
+    } while (0)
+
+static double rat_dtn(const orc_inst *s, int64_t k)
+{
+    if (D(s->dtempn_rk3, k) != 0.0) return fabs(D(s->dtempn, k) / D(s->dtempn_rk3, k) - 1.0);
+    if (D(s->dtempn, k) != 0.0) return fabs(D(s->dtempn_rk3, k) / D(s->dtempn, k) - 1.0);
+    return 0.0;
+}
+static double rat_t(const orc_inst *s, const darr *temp, int64_t k)
+{
+    if (D(s->temp_rk3, k) != 0.0) return fabs(D(*temp, k) / D(s->temp_rk3, k) - 1.0);
+    return 0.0;
 }
 
 /* TEMP_EXTP chphase.f:467-803 */
@@ -1257,6 +1301,7 @@ static int temp_extp(orc_inst *s)
         darr *temp = (im == 0) ? &s->temp0 : &s->temp1;
         double redtl = 0.2 * sm_grid;
         dzero(&s->dtempn); dzero(&s->dtempn_rk3);
+        ORC_PFOR
         for (int64_t k = 1; k <= n; k++) {
             double np = D(s->nphi, k), pc = D(s->psi_cn, k);
             D(s->dtempn, k) = 0.0;
@@ -1281,17 +1326,17 @@ static int temp_extp(orc_inst *s)
         for (;;) {
             int conv = 0;
             for (int it = 1; it <= 100; it++) {
+                ORC_PFOR
                 for (int64_t k = 1; k <= n; k++) D(s->dtempn_rk3, k) = D(s->dtempn, k);
                 temp_extp_rk3(s, &s->dtempn, im, redtl, 0, &s->dtempn);
                 temp_extp_rk3(s, &s->dtempn, im, redtl, 0, &s->dtempn);
+                ORC_PFOR
                 for (int64_t k = 1; k <= n; k++) D(s->dtempn, k) = 0.75 * D(s->dtempn_rk3, k) + 0.25 * D(s->dtempn, k);
                 temp_extp_rk3(s, &s->dtempn, im, redtl, 0, &s->dtempn);
+                ORC_PFOR
                 for (int64_t k = 1; k <= n; k++) D(s->dtempn, k) = 1.0 / 3.0 * D(s->dtempn_rk3, k) + 2.0 / 3.0 * D(s->dtempn, k);
-                double rat = 0.0;
-                for (int64_t k = 1; k <= n; k++) {
-                    if (D(s->dtempn_rk3, k) != 0.0) rat = fmax(fabs(D(s->dtempn, k) / D(s->dtempn_rk3, k) - 1.0), rat);
-                    else if (D(s->dtempn, k) != 0.0) rat = fmax(fabs(D(s->dtempn_rk3, k) / D(s->dtempn, k) - 1.0), rat);
-                }
+                double rat;
+                ORC_PMAX(rat, n, rat_dtn(s, k));
                 s->n_extrap_iters += 1;
                 if (fabs(rat) < eps) { conv = 1; break; }
             }
@@ -1306,18 +1351,20 @@ static int temp_extp(orc_inst *s)
         for (;;) {
             int conv = 0;
             for (int it = 1; it <= 100; it++) {
+                ORC_PFOR
                 for (int64_t k = 0; k <= s->n3; k++) { D(s->temp_rk3, k) = D(*temp, k); D(s->tempo, k) = D(*temp, k); }
                 temp_extp_rk3(s, &s->tempo, im, redtl, 1, &s->dtempn);
                 temp_extp_rk3(s, &s->tempo, im, redtl, 1, &s->dtempn);
+                ORC_PFOR
                 for (int64_t k = 1; k <= n; k++) D(s->tempo, k) = 0.75 * D(s->temp_rk3, k) + 0.25 * D(s->tempo, k);
                 temp_extp_rk3(s, &s->tempo, im, redtl, 1, &s->dtempn);
+                ORC_PFOR
                 for (int64_t k = 1; k <= n; k++) {
                     D(s->tempo, k) = 1.0 / 3.0 * D(s->temp_rk3, k) + 2.0 / 3.0 * D(s->tempo, k);
                     D(*temp, k) = D(s->tempo, k);
                 }
-                double rat = 0.0;
-                for (int64_t k = 1; k <= n; k++)
-                    if (D(s->temp_rk3, k) != 0.0) rat = fmax(fabs(D(*temp, k) / D(s->temp_rk3, k) - 1.0), rat);
+                double rat;
+                ORC_PMAX(rat, n, rat_t(s, temp, k));
                 s->n_extrap_iters += 1;
                 if (fabs(rat) < eps) { conv = 1; break; }
             }
@@ -1877,6 +1924,43 @@ int orc_run_cases(const orc_case *cases, int64_t nc, int64_t nsteps, int64_t *st
         if (inst[i]) orc_free(inst[i]);
     free(inst);
     return threads;
+}
+
+void orc_set_loop_threads(orc_inst *s, int on) { s->omp = on ? 1 : 0; }
+
+/* One case at a time with its per-cell loops shared by `threads` OpenMP
+ * threads (the reference's own style, mode (i)); returns the threads used */
+int orc_run_single(const orc_case *c, int64_t nsteps, int64_t threads, int64_t *status, int64_t *steps_out,
+                   double *setup_s, double *step_s)
+{
+    int used = 1;
+    *setup_s = 0.0;
+    *step_s = 0.0;
+#ifdef _OPENMP
+    const int saved = omp_get_max_threads();
+    used = threads > 0 ? (int)threads : saved;
+    omp_set_num_threads(used);
+    double t0 = omp_get_wtime();
+#endif
+    orc_inst *s = orc_new(c);
+    *status = s ? orc_init(s) : ORC_GRID;
+    *steps_out = 0;
+    if (s) s->omp = 1;
+#ifdef _OPENMP
+    double t1 = omp_get_wtime();
+#endif
+    for (int64_t it = 0; s && !*status && it < nsteps; it++) {
+        *steps_out += 1;
+        *status = orc_step(s, 1);
+    }
+#ifdef _OPENMP
+    double t2 = omp_get_wtime();
+    *setup_s = t1 - t0;
+    *step_s = t2 - t1;
+    omp_set_num_threads(saved);
+#endif
+    if (s) orc_free(s);
+    return used;
 }
 
 /* ------------------------------------------------------------------ */

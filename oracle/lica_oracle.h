@@ -85,6 +85,12 @@ int64_t orc_last_poisson(orc_inst *s, double *a, double *b, double *c, double *g
 /* CPU baseline: nc cases set up and advanced nsteps steps each (or until
  * they stop), one case per OpenMP thread; set-up and steps timed apart
  * (seconds, wall clock); returns the threads used */
+/* share (on = 1) or not the per-cell loops of one instance among the OpenMP threads */
+void orc_set_loop_threads(orc_inst *s, int on);
+/* one case with its per-cell loops shared by `threads` OpenMP threads
+ * (0: the default count); returns the threads used                        */
+int orc_run_single(const orc_case *c, int64_t nsteps, int64_t threads, int64_t *status, int64_t *steps_out,
+                   double *setup_s, double *step_s);
 int orc_run_cases(const orc_case *cases, int64_t nc, int64_t nsteps, int64_t *status, int64_t *steps_out,
                   double *setup_s, double *step_s);
 

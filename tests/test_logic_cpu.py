@@ -409,3 +409,16 @@ def test_fld_reader_formats_and_refusals(emul, tmp_path):
             b.read_fld(path, 0)
         assert e.value.status == 22, why
     b.run(1)
+
+
+def test_oracle_loop_threads_do_not_change_results():
+    """The oracle's reference-style mode (one case, the pseudo-time
+    extrapolation's per-cell loops shared by the OpenMP threads, maxima
+    combined by fmax) gives the serial oracle's fields bit for bit."""
+    case = L.batched_cases(4096)[7]
+    a, b = Oracle(case), Oracle(case)
+    b.set_loop_threads(True)
+    assert a.step(3) == b.step(3)
+    for f in ("W", "P", "T", "PSIF", "PHI", "PSI_CN"):
+        assert np.array_equal(a.field(f), b.field(f)), f
+    assert np.array_equal(a.scalar_array(), b.scalar_array())
