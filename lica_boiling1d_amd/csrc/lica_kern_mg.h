@@ -85,10 +85,8 @@ HOSTDEV long long mg_rows_total(int n) { return mg_rows_before(n, mg_levels(n));
 // scratch is); a level's rows pair up as (C row 2J, F row 2J+1), one pair per
 // row J of the next level, tot - n pairs in all:
 //   4 (tot - n)  FP64 setup rows (a, c, s, 1/b) of levels 1..L-1
-//   2 (tot - n)  FP64 C weights (w0, w1) of every pair
-//   4 (tot - n)  FP64 F weights (w0, w1, w2, pad) of every pair
-//   1 (tot - n)  FP32 C weights
-//   2 (tot - n)  FP32 F weights
+//   2 (tot - n)  C weights (w0, w1) of every pair (FP64, or FP32 in its first half)
+//   4 (tot - n)  F weights (w0, w1, w2, pad) of every pair (likewise)
 //   2 n          the iterate and its double buffer
 //   n + tot      the cycle's vectors kept outside LDS (room for FP64)
 // (the C and F weights are apart so that the downward pass, which reads
@@ -98,7 +96,7 @@ HOSTDEV long long mg_ru4(long long x) { return (x + 3) / 4 * 4; }
 HOSTDEV long long mg_scratch_doubles(int n)
 {
     const long long tot = mg_rows_total(n), np = tot - n;
-    return mg_ru4(4 * np + mg_ru4(2 * np) + 4 * np + mg_ru4(np) + mg_ru4(2 * np) + mg_ru4(2LL * n) + (n + tot) + 8);
+    return mg_ru4(4 * np + mg_ru4(2 * np) + 4 * np + mg_ru4(2LL * n) + (n + tot) + 8);
 }
 // LDS that holds every vector of a cycle in elements of tsize bytes
 HOSTDEV long long mg_lds_bytes(int n, int tsize) { return (long long)tsize * (n + mg_rows_total(n)); }
@@ -186,7 +184,7 @@ struct MgView {
     MgSet *set;                 // setup rows of levels >= 1 (level l row i at lv_pre[l] - n + i)
     MgCw<double> *cw64;         // weights of the pairs of level l at lv_pre[l+1] - n + J
     MgFw<double> *fw64;
-    MgCw<float> *cw32;
+    MgCw<float> *cw32;          // the same memory: a solve uses one precision
     MgFw<float> *fw32;
     double *x0, *x1;            // iterate buffers
     double *gv;                 // global home of the cycle's vectors
@@ -201,10 +199,10 @@ DEV MgView mg_view(double *scr, int n, const MgSh &sh)
     const long long np = v.tot - n;
     double *q = scr;
     v.set = (MgSet *)q;             q += 4 * np;
-    v.cw64 = (MgCw<double> *)q;     q += mg_ru4(2 * np);
-    v.fw64 = (MgFw<double> *)q;     q += 4 * np;
-    v.cw32 = (MgCw<float> *)q;      q += mg_ru4(np);
-    v.fw32 = (MgFw<float> *)q;      q += mg_ru4(2 * np);
+    v.cw64 = (MgCw<double> *)q;
+    v.cw32 = (MgCw<float> *)q;      q += mg_ru4(2 * np);
+    v.fw64 = (MgFw<double> *)q;
+    v.fw32 = (MgFw<float> *)q;      q += 4 * np;
     v.x0 = q;
     v.x1 = q + n;                   q += mg_ru4(2LL * n);
     v.gv = q;
