@@ -184,8 +184,12 @@ DEV double box_avg(const double *zf, double ssdzf, F f, double z1, double z2, in
 //           TRDIAG3's accuracy on the Laplacian-like rows of POISSON.
 
 #define TRI_RPT 8        // rows per thread held in registers at the first level
-#define TRI_LCHUNK 8     // rows per chunk of the LDS levels
-#define TRI_SERIAL 64    // end-row systems this small are eliminated by one thread
+#define TRI_LCHUNK 8     // rows per chunk of the LDS levels (at most)
+#define TRI_SERIAL 16    // end-row systems this small are eliminated by one thread
+
+// rows of the next LDS level of an end-row system of m rows: two end rows
+// of each of its ceil(m / TRI_LCHUNK) chunks
+HOSTDEV int tri_next(int m) { return 2 * ((m + TRI_LCHUNK - 1) / TRI_LCHUNK); }
 
 #ifndef TRI_CHUNK_ROWS
 #define TRI_CHUNK_ROWS 8
@@ -211,7 +215,7 @@ HOSTDEV int tri_ws_rows(int nthreads)
     int m = 2 * nthreads, tot = 0;
     while (m > TRI_SERIAL) {
         tot += m;
-        m = 2 * (m / TRI_LCHUNK);
+        m = tri_next(m);
     }
     return tot + m;
 }
@@ -229,7 +233,10 @@ DEV void chunk_bounds(int nrow, int nc, int ch, int *s, int *L)
 }
 
 // reduce rows s..s+L-1 (L >= 2) in place; r = start row (a,b,c,g), end row (a,b,c,g)
-DEV void chunk_reduce(double *a, double *b, double *c, double *g, int s, int L, double *r)
+// (the four arrays never overlap: the compiler may keep a row it has just
+// written in registers and load the next one early)
+DEV void chunk_reduce(double *__restrict__ a, double *__restrict__ b, double *__restrict__ c, double *__restrict__ g,
+                      int s, int L, double *__restrict__ r)
 {
     for (int j = 2; j <= L - 1; j++) {
         int i = s + j;
@@ -265,7 +272,8 @@ DEV void chunk_reduce(double *a, double *b, double *c, double *g, int s, int L, 
     r[4] = a[e]; r[5] = b[e]; r[6] = c[e]; r[7] = g[e];
 }
 
-DEV void chunk_recover(const double *a, const double *c, double *g, int s, int L, double xs, double xe)
+DEV void chunk_recover(const double *__restrict__ a, const double *__restrict__ c, double *__restrict__ g, int s, int L,
+                       double xs, double xe)
 {
     for (int j = 1; j <= L - 2; j++) {
         int i = s + j;
@@ -370,21 +378,23 @@ DEV void put_end_rows(double *ra, double *rb, double *rc, double *rg, int ch, in
 DEV void lds_level(int m, int l, int *off, int *rows)
 {
     int o = 0, cur = m;
-    for (int i = 0; i < l; i++) { o += cur; cur = 2 * (cur / TRI_LCHUNK); }
+    for (int i = 0; i < l; i++) { o += cur; cur = tri_next(cur); }
     *off = o;
     *rows = cur;
 }
 
-// the end-row system of m rows at row 0 of the LDS workspace; solution in rg
+// the end-row system of m rows at row 0 of the LDS workspace; solution in
+// rg.  Each level splits its rows into ceil(rows / TRI_LCHUNK) chunks of at
+// most TRI_LCHUNK rows.
 DEV void lds_tri_solve(TriSh &ts, int m)
 {
     double *RA = ts.ws, *RB = ts.ws + ts.ws_rows, *RC = ts.ws + 2 * ts.ws_rows, *RG = ts.ws + 3 * ts.ws_rows;
     int nlev = 0;
-    for (int cur = m; cur > TRI_SERIAL; cur = 2 * (cur / TRI_LCHUNK)) nlev++;
+    for (int cur = m; cur > TRI_SERIAL; cur = tri_next(cur)) nlev++;
     for (int l = 0; l < nlev; l++) {
         int bo, rows;
         lds_level(m, l, &bo, &rows);
-        const int nc = rows / TRI_LCHUNK, no = bo + rows;
+        const int nc = (rows + TRI_LCHUNK - 1) / TRI_LCHUNK, no = bo + rows;
         PFOR(ch, 0, nc) {
             int s, L;
             chunk_bounds(rows, nc, ch, &s, &L);
@@ -397,7 +407,8 @@ DEV void lds_tri_solve(TriSh &ts, int m)
     int o, cur;
     lds_level(m, nlev, &o, &cur);
     SERIAL {
-        double *ra = RA + o, *rb = RB + o, *rc = RC + o, *rg = RG + o;
+        double *__restrict__ ra = RA + o, *__restrict__ rb = RB + o, *__restrict__ rc = RC + o,
+               *__restrict__ rg = RG + o;
         double bet = 1.0 / rb[0];
         rg[0] = rg[0] * bet;
         for (int j = 1; j < cur; j++) {
@@ -412,7 +423,7 @@ DEV void lds_tri_solve(TriSh &ts, int m)
     for (int l = nlev - 1; l >= 0; l--) {
         int bo, rows;
         lds_level(m, l, &bo, &rows);
-        const int nc = rows / TRI_LCHUNK, no = bo + rows;
+        const int nc = (rows + TRI_LCHUNK - 1) / TRI_LCHUNK, no = bo + rows;
         PFOR(ch, 0, nc) {
             int s, L;
             chunk_bounds(rows, nc, ch, &s, &L);
