@@ -89,7 +89,9 @@ DEV void mg_attach(MgSh &sh, long long lds_bytes)
     if (threadIdx.x == 0) { sh.lds = (void *)lica_tri_lds; sh.lds_bytes = lds_bytes; }
     __syncthreads();
 }
-__global__ void __launch_bounds__(1024) k_poisson_mg(BatchDev bd)
+// six waves per SIMD (80 VGPRs, a few spills): three 512-thread workgroups
+// per CU instead of two
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) k_poisson_mg(BatchDev bd)
 {
     __shared__ MgSh sh;
     mg_attach(sh, bd.mg_lds);
