@@ -57,9 +57,7 @@ __global__ void __launch_bounds__(1024) k_dt(BatchDev bd)
     __shared__ BlkSh sh;
     body_dt(bd, bd.order[blockIdx.x], sh);
 }
-// five waves per SIMD (96 VGPRs, at the cost of 132 B of spills per lane):
-// measured 10 % faster than four
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5))) k_cont_energy(BatchDev bd)
+__global__ void __launch_bounds__(1024) k_cont_energy(BatchDev bd)
 {
     __shared__ BlkSh sh;
     tri_attach(sh);
