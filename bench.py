@@ -100,12 +100,12 @@ def kernel_bytes(case, kernel, mode):
         if mode == 1:   # partitioned: a,b,c,g r/w, P written
             return (4 * 16 + 8) * n
         # V-cycles: a,b,c,g read, P written, the two iterate buffers r/w,
-        # the coarse levels' setup rows (4 doubles) r/w, the cycle rows of
-        # every level (4 values of the cycle's precision) r/w, the warm-start
-        # guess read
+        # the coarse levels' setup rows (4 doubles) r/w, the weights of every
+        # level's pairs of rows (2 for the C row, 3 for the F row, in the
+        # cycle's precision) r/w, the warm-start guess read
         tot = mg_rows_total(n)
-        cyc = 32 if mode == 2 else 16
-        return (4 * 8 + 8 + 2 * 16) * n + 2 * 32 * (tot - n) + 2 * cyc * tot + (8 * n if mode in (4, 5) else 0)
+        w = 8 if mode == 2 else 4
+        return (4 * 8 + 8 + 2 * 16) * n + 2 * 32 * (tot - n) + 2 * 5 * w * (tot - n) + (8 * n if mode in (4, 5) else 0)
     # band kernels: 11 layers of <=16 planes, refined window of ~200 planes of
     # PSIF/SFGAS/geometry, NS window of ~100 cells of T/PSI_CN/geometry
     r = int(case.r)
