@@ -500,6 +500,9 @@ DEV void part_pass_reg(const double *A, const double *B, const double *C, double
 }
 #endif
 
+// NREF_MAX bounds the refinement steps at compile time: a kernel that never
+// refines (0) carries none of the refinement's registers
+template <int NREF_MAX = TRI_NREF>
 DEV void tri_solve(double *a, double *b, double *c, double *g, int l1, int l2, int mode, TriSh &ts)
 {
     const int nrow = l2 - l1 + 1;
@@ -523,7 +526,7 @@ DEV void tri_solve(double *a, double *b, double *c, double *g, int l1, int l2, i
     // diagonally dominant energy and momentum rows when multigrid takes the
     // POISSON rows) does without them.  a, b and c are left untouched; g
     // receives the solution.
-    const int nref = (mode == 2) ? 0 : TRI_NREF;
+    const int nref = (mode == 2) ? 0 : (NREF_MAX < TRI_NREF ? NREF_MAX : TRI_NREF);
 #ifndef LICA_EMUL
     const int ch = (int)threadIdx.x;
     const bool act = ch < nc;
@@ -710,14 +713,28 @@ DEV int region_jf(const unsigned char *jf, const double *pc, int n, int k)
     return (pc[k] == 1.0) ? 1 : 0;
 }
 
+// The workgroup's LDS: a few reductions' partials in the static part, and
+// in the dynamic part (blk_dyn_bytes) the refined window of the box
+// averages and the jump flags of PSI_CN, which the partitioned solve's
+// workspace overlays: a kernel stages the window again after its solve if
+// it reads it there.  The overlay keeps a 512-thread workgroup of the
+// momentum stage at 44 KB (three per CU) instead of 61 KB (two).
 struct BlkSh {
     double red[16];
     double v[16];
     int iv[16];
-    TriSh tri;     // points at the kernel's dynamic LDS (tri_ws_rows(blockDim) x 4 doubles)
-    RefWin win;    // refined window of the box averages
-    unsigned char jf[JFMAX];   // jump flags of PSI_CN (n <= JFMAX - 1)
+    TriSh tri;          // ws at the start of the dynamic LDS (tri_ws_rows(blockDim) x 4 doubles)
+    RefWin *win;        // refined window of the box averages (dynamic LDS, under the solve's workspace)
+    unsigned char *jf;  // jump flags of PSI_CN (n <= JFMAX - 1), after the window
 };
+// dynamic LDS of a BlkSh kernel with `threads` per workgroup (tri: it runs
+// partitioned solves)
+HOSTDEV long long blk_dyn_bytes(int threads, int tri)
+{
+    const long long win = ((long long)sizeof(RefWin) + JFMAX + 15) / 16 * 16;
+    const long long ws = tri ? 32LL * tri_ws_rows(threads) : 0;
+    return ws > win ? ws : win;
+}
 
 // DETERMINE_DT (1d.f:683-817) at the start of a step, with the per-step
 // reset of RK3ZO (1d.f:206-218); the CFL maximum |W(k)+W(k+1)|*SSDZ(k) was
@@ -810,9 +827,9 @@ DEV void body_cont_energy(const BatchDev &bd, int inst, BlkSh &sh)
             lo = KCOU1[ka] - 2;
             hi = KCOU2[kb] + 2;
         }
-        stage_refwin(sh.win, lo, hi, P.m3l, P.n3f + P.r, ZF, PSIF, nullptr);
+        stage_refwin(*sh.win, lo, hi, P.m3l, P.n3f + P.r, ZF, PSIF, nullptr);
     }
-    const RefWin &win = sh.win;
+    const RefWin &win = *sh.win;
     auto zfa = [&](int kk) { return (kk >= win.lo && kk <= win.hi + 1) ? win.zf[kk - win.lo] : ZF[kk]; };
     auto psa = [&](int kk) { return (kk >= win.lo && kk <= win.hi) ? win.f0[kk - win.lo] : PSIF[kk]; };
     // PSIFACE_Z / PSIFACE_C (1d.f:2349-2549) and the face densities

@@ -13,6 +13,10 @@ DEV double denf_zi(const InstParam &P, const double *PZ, int k)
     return 1.0 / d;
 }
 
+// NREF: the most refinement steps its partitioned solve may take (TRI_NREF
+// for the direct Poisson mode's kernel, 0 for the others, whose kernel then
+// carries no refinement code and needs fewer registers)
+template <int NREF>
 DEV void body_momentum(const BatchDev &bd, int inst, BlkSh &sh)
 {
     InstState &S = bd.st[inst];
@@ -35,8 +39,8 @@ DEV void body_momentum(const BatchDev &bd, int inst, BlkSh &sh)
     // a refined range [k1,k2] that misses the nonzero SFGAS window averages to 0;
     // the boxes that touch it read ZF/SFGAS/SFGAS2 staged in LDS
     auto touches = [&](int k1, int k2) { return !(k2 < sfl || k1 > sfh); };
-    stage_refwin(sh.win, sfl - 2 * P.r - 4, sfh + 2 * P.r + 4, P.m3l, P.n3f + P.r, ZF, SFGAS, SFGAS2);
-    const RefWin &win = sh.win;
+    stage_refwin(*sh.win, sfl - 2 * P.r - 4, sfh + 2 * P.r + 4, P.m3l, P.n3f + P.r, ZF, SFGAS, SFGAS2);
+    const RefWin &win = *sh.win;
     auto zfa = [&](int kk) { return (kk >= win.lo && kk <= win.hi + 1) ? win.zf[kk - win.lo] : ZF[kk]; };
     auto fsf = [&](int kk) { return (kk >= win.lo && kk <= win.hi) ? win.f0[kk - win.lo] : SFGAS[kk]; };
     auto fsf2 = [&](int kk) { return (kk >= win.lo && kk <= win.hi) ? win.f1[kk - win.lo] : SFGAS2[kk]; };
@@ -106,7 +110,9 @@ DEV void body_momentum(const BatchDev &bd, int inst, BlkSh &sh)
     // 65% more HBM bytes for no gain in time)
     PFOR(k, 2, n + 1) { lhsw_row(k, TA[k], TB[k], TC[k], TG[k]); }
     PFOR_END
-    tri_solve(TA, TB, TC, TG, 2, n, bd.tri_mode, sh.tri);
+    tri_solve<NREF>(TA, TB, TC, TG, 2, n, bd.tri_mode, sh.tri);
+    // the solve's workspace overlays the window: DIVGS reads it again
+    stage_refwin(*sh.win, sfl - 2 * P.r - 4, sfh + 2 * P.r + 4, P.m3l, P.n3f + P.r, ZF, SFGAS, SFGAS2);
     PFOR(k, 2, n + 1) { W[k] = TG[k] + W[k]; }
     PFOR_END
     // BC + BC_FLUX (1d.f:1812-1923)
@@ -394,8 +400,8 @@ DEV void conv_maxima(const CaseP &cp, const InstState &S, BlkSh &sh, double *dvm
     const double *__restrict__ SFGAS = cp.R(RF_SFGAS);
     const int *__restrict__ KCOU1 = cp.AI(NSI_KCOU1), *__restrict__ KCOU2 = cp.AI(NSI_KCOU2);
     const int sfl = S.sf_lo, sfh = S.sf_hi;
-    stage_refwin(sh.win, sfl - 2 * P.r - 4, sfh + 2 * P.r + 4, P.m3l, P.n3f + P.r, ZF, SFGAS, nullptr);
-    const RefWin &win = sh.win;
+    stage_refwin(*sh.win, sfl - 2 * P.r - 4, sfh + 2 * P.r + 4, P.m3l, P.n3f + P.r, ZF, SFGAS, nullptr);
+    const RefWin &win = *sh.win;
     auto zfa = [&](int kk) { return (kk >= win.lo && kk <= win.hi + 1) ? win.zf[kk - win.lo] : ZF[kk]; };
     auto fsf = [&](int kk) { return (kk >= win.lo && kk <= win.hi) ? win.f0[kk - win.lo] : SFGAS[kk]; };
     *dvmax = red_max(1, n + 1, 0.0, [&](int k) {

@@ -38,12 +38,19 @@ int lica_noalloc_scratch(double **out)
 #endif
 
 #ifndef LICA_EMUL
-// LDS workspace of the partitioned tridiagonal solves, sized by the launch
+// dynamic LDS, sized by the launch: the partitioned solves' workspace over
+// the refined window and the jump flags (BlkSh kernels, blk_dyn_bytes), or
+// the multigrid cycle's vectors
 extern __shared__ double lica_tri_lds[];
 
 DEV void tri_attach(BlkSh &sh)
 {
-    if (threadIdx.x == 0) { sh.tri.ws = lica_tri_lds; sh.tri.ws_rows = tri_ws_rows((int)blockDim.x); }
+    if (threadIdx.x == 0) {
+        sh.tri.ws = lica_tri_lds;
+        sh.tri.ws_rows = tri_ws_rows((int)blockDim.x);
+        sh.win = (RefWin *)lica_tri_lds;
+        sh.jf = (unsigned char *)(sh.win + 1);
+    }
     __syncthreads();
 }
 
@@ -55,6 +62,7 @@ __global__ void __launch_bounds__(1024) k_init(BatchDev bd)
 __global__ void __launch_bounds__(1024) k_dt(BatchDev bd)
 {
     __shared__ BlkSh sh;
+    tri_attach(sh);
     body_dt(bd, bd.order[blockIdx.x], sh);
 }
 __global__ void __launch_bounds__(1024) k_cont_energy(BatchDev bd)
@@ -70,11 +78,21 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k
     __shared__ TgradSh sh;
     body_temp_grad(bd, bd.order[blockIdx.x], sh);
 }
-__global__ void __launch_bounds__(1024) k_momentum(BatchDev bd)
+// the momentum stage of the Thomas and multigrid modes (no refinement in
+// its solve): 79 VGPRs at six waves per SIMD (a few spills), so that three
+// 512-thread workgroups share a CU (44 KB of LDS each)
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) k_momentum(BatchDev bd)
 {
     __shared__ BlkSh sh;
     tri_attach(sh);
-    body_momentum(bd, bd.order[blockIdx.x], sh);
+    body_momentum<0>(bd, bd.order[blockIdx.x], sh);
+}
+// the direct Poisson mode's momentum stage (refined partitioned solve)
+__global__ void __launch_bounds__(1024) k_momentum_refine(BatchDev bd)
+{
+    __shared__ BlkSh sh;
+    tri_attach(sh);
+    body_momentum<TRI_NREF>(bd, bd.order[blockIdx.x], sh);
 }
 __global__ void __launch_bounds__(1024) k_poisson_direct(BatchDev bd)
 {
@@ -164,11 +182,13 @@ __global__ void __launch_bounds__(1024) k_capture(BatchDev bd, int which)
 __global__ void __launch_bounds__(1024) k_ucalc(BatchDev bd)
 {
     __shared__ BlkSh sh;
+    tri_attach(sh);
     body_ucalc(bd, bd.order[blockIdx.x], sh);
 }
 __global__ void __launch_bounds__(1024) k_convrge1(BatchDev bd)
 {
     __shared__ BlkSh sh;
+    tri_attach(sh);
     body_convrge1(bd, bd.order[blockIdx.x], sh);
 }
 __global__ void __launch_bounds__(64) k_transport(BatchDev bd)
@@ -204,26 +224,27 @@ int lica_launch_tri(int nsys, int n, double *a, double *b, double *c, double *g,
 {
     if (nsys <= 0) return 0;
     if (int rc = tri_lds_setup()) return rc;
-    const size_t lds = sizeof(double) * 4 * (size_t)tri_ws_rows(threads);
+    const size_t lds = (size_t)blk_dyn_bytes(threads, 1);
     hipLaunchKernelGGL(k_tri_systems, dim3(nsys), dim3(threads), lds, (hipStream_t)stream, a, b, c, g, n, mode);
     return (int)hipGetLastError();
 }
 
 #define LAUNCH(K, threads) \
     hipLaunchKernelGGL(K, dim3(bd.ninst), dim3(threads), 0, (hipStream_t)stream, bd)
+#define LAUNCH_BLK(K, threads) \
+    hipLaunchKernelGGL(K, dim3(bd.ninst), dim3(threads), (size_t)blk_dyn_bytes(threads, 0), (hipStream_t)stream, bd)
 #define LAUNCH_TRI(K, threads) \
-    hipLaunchKernelGGL(K, dim3(bd.ninst), dim3(threads), sizeof(double) * 4 * (size_t)tri_ws_rows(threads), \
-                       (hipStream_t)stream, bd)
+    hipLaunchKernelGGL(K, dim3(bd.ninst), dim3(threads), (size_t)blk_dyn_bytes(threads, 1), (hipStream_t)stream, bd)
 
 // allow the dynamic LDS of the partitioned solves beyond 64 KB (1024 threads)
 static int tri_lds_setup()
 {
     static int done = 0, rc = 0;
     if (!done) {
-        const int bytes = (int)(sizeof(double) * 4 * (size_t)tri_ws_rows(1024));
-        const void *ks[4] = {(const void *)k_cont_energy, (const void *)k_momentum, (const void *)k_poisson_direct,
-                             (const void *)k_tri_systems};
-        for (int i = 0; i < 4 && !rc; i++)
+        const int bytes = (int)blk_dyn_bytes(1024, 1);
+        const void *ks[5] = {(const void *)k_cont_energy, (const void *)k_momentum, (const void *)k_momentum_refine,
+                             (const void *)k_poisson_direct, (const void *)k_tri_systems};
+        for (int i = 0; i < 5 && !rc; i++)
             rc = (int)hipFuncSetAttribute(ks[i], hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         done = 1;
     }
@@ -236,14 +257,17 @@ int lica_launch(int which, const BatchDev &bd, int threads, void *stream)
     if (int rc = tri_lds_setup()) return rc;
     switch (which) {
     case KL_INIT: LAUNCH(k_init, threads); break;
-    case KL_DT: LAUNCH(k_dt, threads); break;
+    case KL_DT: LAUNCH_BLK(k_dt, threads); break;
     case KL_CONT_ENERGY: LAUNCH_TRI(k_cont_energy, threads); break;
     case KL_TEMP_GRAD: LAUNCH(k_temp_grad, 128); break;
-    case KL_MOMENTUM: LAUNCH_TRI(k_momentum, threads); break;
+    case KL_MOMENTUM:
+        if (bd.tri_mode == 1) LAUNCH_TRI(k_momentum_refine, threads);
+        else LAUNCH_TRI(k_momentum, threads);
+        break;
     case KL_POISSON_DIRECT: LAUNCH_TRI(k_poisson_direct, threads); break;
-    case KL_UCALC: LAUNCH(k_ucalc, threads); break;
+    case KL_UCALC: LAUNCH_BLK(k_ucalc, threads); break;
     case KL_TRANSPORT: LAUNCH(k_transport, 64); break;
-    case KL_CONVRGE1: LAUNCH(k_convrge1, threads); break;
+    case KL_CONVRGE1: LAUNCH_BLK(k_convrge1, threads); break;
     case KL_POISSON_MG:
         hipLaunchKernelGGL(k_poisson_mg, dim3(bd.ninst), dim3(threads), (size_t)bd.mg_lds, (hipStream_t)stream, bd);
         break;
@@ -265,9 +289,19 @@ long long lica_mg_lds_max() { return 160 * 1024 - 4096; }
 static double *emul_tri_ws(int threads)
 {
     static double *ws = nullptr;
-    if (!ws) ws = new double[4 * (size_t)tri_ws_rows(1024)];
+    if (!ws) ws = new double[(size_t)blk_dyn_bytes(1024, 1) / 8 + 1];
     (void)threads;
     return ws;
+}
+
+// the same overlay as on the GPU: the window and the jump flags under the
+// solve's workspace
+static void attach(BlkSh &sh, int threads)
+{
+    sh.tri.ws = emul_tri_ws(threads);
+    sh.tri.ws_rows = tri_ws_rows(threads);
+    sh.win = (RefWin *)sh.tri.ws;
+    sh.jf = (unsigned char *)(sh.win + 1);
 }
 
 int lica_launch_tri(int nsys, int n, double *a, double *b, double *c, double *g, int mode, int threads, void *stream)
@@ -275,8 +309,7 @@ int lica_launch_tri(int nsys, int n, double *a, double *b, double *c, double *g,
     (void)stream;
     static BlkSh *sh = nullptr;
     if (!sh) sh = new BlkSh();
-    sh->tri.ws = emul_tri_ws(threads);
-    sh->tri.ws_rows = tri_ws_rows(threads);
+    attach(*sh, threads);
     int saved = lica_emul_nthreads;
     lica_emul_nthreads = threads;
     for (int s = 0; s < nsys; s++) {
@@ -313,7 +346,6 @@ int lica_launch_mg(int nsys, int n, const double *a, const double *b, const doub
     return 0;
 }
 
-static void attach(BlkSh &sh, int threads) { sh.tri.ws = emul_tri_ws(threads); sh.tri.ws_rows = tri_ws_rows(threads); }
 static double *emul_mg_lds = nullptr;
 static long long emul_mg_lds_bytes = 0;
 static void attach(MgSh &sh, int) { sh.lds = emul_mg_lds; sh.lds_bytes = emul_mg_lds_bytes; }
@@ -343,7 +375,10 @@ int lica_launch(int which, const BatchDev &bd, int threads, void *stream)
     case KL_DT: emul_run<BlkSh>(bd, threads, body_dt); break;
     case KL_CONT_ENERGY: emul_run<BlkSh>(bd, threads, body_cont_energy); break;
     case KL_TEMP_GRAD: emul_run<TgradSh>(bd, 64, body_temp_grad); break;
-    case KL_MOMENTUM: emul_run<BlkSh>(bd, threads, body_momentum); break;
+    case KL_MOMENTUM:
+        if (bd.tri_mode == 1) emul_run<BlkSh>(bd, threads, body_momentum<TRI_NREF>);
+        else emul_run<BlkSh>(bd, threads, body_momentum<0>);
+        break;
     case KL_POISSON_DIRECT: emul_run<BlkSh>(bd, threads, body_poisson_direct); break;
     case KL_UCALC: emul_run<BlkSh>(bd, threads, body_ucalc); break;
     case KL_TRANSPORT: emul_run<TransportSh>(bd, 64, body_transport); break;
