@@ -59,6 +59,22 @@ def test_single_large_case_bit_exact():
     assert exact_mismatches(b, 0, o) == []
 
 
+def test_large_case_1000_steps_bit_exact():
+    """The single large case (N=8192, r=4, 10 K, 0.2 mm film with a seeded
+    1 % perturbation) over its full 1000 steps in the reference's Thomas
+    mode: every field, the band and the scalars equal the oracle's bit for
+    bit at steps 250, 500 and 1000."""
+    case = L.single_large_case(seed=1)
+    b, o = L.Batch([case]), Oracle(case)
+    done = 0
+    for upto in (250, 500, 1000):
+        _say(f"single large case N=8192 r=4, steps {done + 1}..{upto}")
+        b.run(upto - done)
+        assert o.step(upto - done) == 0
+        done = upto
+        assert exact_mismatches(b, 0, o) == [], upto
+
+
 def test_heterogeneous_batch_bit_exact():
     """12 cases of different N, superheat and film thickness in one batch."""
     cases = L.batched_cases(12, seed=7, nmin=128, nmax=1024)
