@@ -696,42 +696,39 @@ DEV int region_of(const double *pc, int n, int k)
     return (pc[k] == 1.0) ? 1 : 0;
 }
 
-// The same classes from jump flags jf[j] = (PSI_CN(j+1) != PSI_CN(j)),
-// j = 0..n, held in LDS (PSI_CN lies in [0,1], so a nonzero difference and
-// inequality coincide).
-#define JFMAX 8200
-DEV void jump_flags(unsigned char *jf, const double *pc, int n)
+// lowest and highest j in [0, n] with PSI_CN(j+1) != PSI_CN(j) (*jhi < 0:
+// none), in one pass; a cell outside [jlo - 3, jhi + 3] is in region 0/1 and
+// so is its upper neighbour
+DEV void jump_range(const double *pc, int n, double *red, int *jlo, int *jhi)
 {
-    PFOR(j, 0, n + 1) { jf[j] = (pc[j + 1] - pc[j] != 0.0) ? 1 : 0; }
-    PFOR_END
-}
-DEV int region_jf(const unsigned char *jf, const double *pc, int n, int k)
-{
-    int j0 = k - 2 < 1 ? 1 : k - 2, j1 = k + 2 > n ? n : k + 2;
-    for (int j = j0 - 1; j <= j1; j++)
-        if (jf[j]) return 2;
-    return (pc[k] == 1.0) ? 1 : 0;
+    double u, v;
+    red_max2(0, n + 1, [&](int j, double &a, double &b) {
+        const bool c = pc[j + 1] != pc[j];
+        a = c ? 1.0e9 - (double)j : 0.0;
+        b = c ? (double)j + 1.0 : 0.0;
+    }, red, &u, &v);
+    *jhi = (int)v - 1;
+    *jlo = (v > 0.0) ? (int)(1.0e9 - u) : (1 << 30);
 }
 
 // The workgroup's LDS: a few reductions' partials in the static part, and
 // in the dynamic part (blk_dyn_bytes) the refined window of the box
-// averages and the jump flags of PSI_CN, which the partitioned solve's
-// workspace overlays: a kernel stages the window again after its solve if
-// it reads it there.  The overlay keeps a 512-thread workgroup of the
-// momentum stage at 44 KB (three per CU) instead of 61 KB (two).
+// averages, which the partitioned solve's workspace overlays: a kernel
+// stages the window again after its solve if it reads it there.  The
+// overlay keeps a 512-thread workgroup of the momentum stage at 44 KB
+// (three per CU) instead of 53 KB (two).
 struct BlkSh {
     double red[16];
     double v[16];
     int iv[16];
     TriSh tri;          // ws at the start of the dynamic LDS (tri_ws_rows(blockDim) x 4 doubles)
     RefWin *win;        // refined window of the box averages (dynamic LDS, under the solve's workspace)
-    unsigned char *jf;  // jump flags of PSI_CN (n <= JFMAX - 1), after the window
 };
 // dynamic LDS of a BlkSh kernel with `threads` per workgroup (tri: it runs
 // partitioned solves)
 HOSTDEV long long blk_dyn_bytes(int threads, int tri)
 {
-    const long long win = ((long long)sizeof(RefWin) + JFMAX + 15) / 16 * 16;
+    const long long win = ((long long)sizeof(RefWin) + 15) / 16 * 16;
     const long long ws = tri ? 32LL * tri_ws_rows(threads) : 0;
     return ws > win ? ws : win;
 }
@@ -818,12 +815,12 @@ DEV void body_cont_energy(const BatchDev &bd, int inst, BlkSh &sh)
 
     // the donor boxes of region-2 cells lie within a few NS cells of the
     // jumps of PSI_CN: stage that stretch of ZF and PSIF in LDS
+    int jlo, jhi;
+    jump_range(PCN, n, sh.red, &jlo, &jhi);
     {
-        const double jlo = red_min(0, n + 1, 1.0e9, [&](int k) { return (PCN[k + 1] != PCN[k]) ? (double)k : 1.0e9; }, sh.red);
-        const double jhi = red_max(0, n + 1, -1.0, [&](int k) { return (PCN[k + 1] != PCN[k]) ? (double)k : -1.0; }, sh.red);
         int lo = 1, hi = 0;
-        if (jhi >= 0.0) {
-            const int ka = (int)jlo - 5 < -1 ? -1 : (int)jlo - 5, kb = (int)jhi + 6 > n3 + 2 ? n3 + 2 : (int)jhi + 6;
+        if (jhi >= 0) {
+            const int ka = jlo - 5 < -1 ? -1 : jlo - 5, kb = jhi + 6 > n3 + 2 ? n3 + 2 : jhi + 6;
             lo = KCOU1[ka] - 2;
             hi = KCOU2[kb] + 2;
         }
@@ -872,13 +869,11 @@ DEV void body_cont_energy(const BatchDev &bd, int inst, BlkSh &sh)
         return cface2(f);
     };
 
-    const bool use_jf = n + 1 <= JFMAX;
-    if (use_jf) jump_flags(sh.jf, PCN, n);
-    auto region = [&](int k) { return use_jf ? region_jf(sh.jf, PCN, n, k) : region_of(PCN, n, k); };
+    auto region = [&](int k) { return region_of(PCN, n, k); };
     // CONTI_Z / CONTI_C in closed form: the serial loops leave on each face
     // the value written last, i.e. the next cell's constant when that cell
     // is region 0/1 and otherwise the value the face's own cell computed.
-    PFOR(k, 1, n + 1) {
+    auto conti_cell = [&](int k) {
         const int rk = region(k);
         const int rn = (k < n) ? region(k + 1) : -1;
         double vf, df;
@@ -933,6 +928,47 @@ DEV void body_cont_energy(const BatchDev &bd, int inst, BlkSh &sh)
         if (pc > 1.0) pc = 1.0;
         else if (pc < 0.0) pc = 0.0;
         PC[k] = pc;
+    };
+    // Cells 2..n-1 away from every jump (k outside [jlo - 3, jhi + 3]): they
+    // and their upper neighbours are region 0/1, so the faces take the upper
+    // cell's constants and PSI_Z/PSI_C keep PSI_ZN/PSI_CN (clamped) -- four
+    // cells per trip, their loads in flight together.  The cells within
+    // reach of a jump and the two ends take the full rules.
+    const int sa = (jhi >= 0) ? (jlo - 3 < 1 ? 1 : jlo - 3) : (1 << 30);
+    const int sb = (jhi >= 0) ? (jhi + 3 > n ? n : jhi + 3) : -1;
+    PLOOP {
+        constexpr int U = 4;
+        for (int k0 = 2 + PTID; k0 <= n - 1; k0 += U * PSTRIDE) {
+            double pzn[U], pcn[U], pcn1[U];
+#pragma unroll
+            for (int j = 0; j < U; j++) {
+                const int kc = (k0 + j * PSTRIDE < n - 1) ? k0 + j * PSTRIDE : n - 1;
+                pzn[j] = PZN[kc];
+                pcn[j] = PCN[kc];
+                pcn1[j] = PCN[kc + 1];
+            }
+#pragma unroll
+            for (int j = 0; j < U; j++) {
+                const int k = k0 + j * PSTRIDE;
+                if (k > n - 1 || (k >= sa && k <= sb)) continue;
+                const bool liq = (pcn1[j] == 1.0);   // region of cell k + 1
+                DFZ[k] = liq ? P.denp : P.denm;
+                VFZ[k] = liq ? P.visp : P.vism;
+                double pz = pzn[j];
+                if (pz > 1.0) pz = 1.0;
+                else if (pz < 0.0) pz = 0.0;
+                PZ[k] = pz;
+                DFC[k] = liq ? P.denp : P.denm;
+                double pc = pcn[j];
+                if (pc > 1.0) pc = 1.0;
+                else if (pc < 0.0) pc = 0.0;
+                PC[k] = pc;
+            }
+        }
+    }
+    PFOR(i, 0, (sb >= sa ? sb - sa + 1 : 0) + 2) {
+        const int k = (i == 0) ? 1 : (i == 1) ? n : sa + i - 2;
+        if (!(i == 1 && n == 1) && !(i >= 2 && (k == 1 || k == n))) conti_cell(k);
     }
     PFOR_END
     SERIAL {

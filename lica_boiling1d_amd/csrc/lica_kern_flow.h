@@ -46,11 +46,17 @@ DEV void body_momentum(const BatchDev &bd, int inst, BlkSh &sh)
     auto fsf2 = [&](int kk) { return (kk >= win.lo && kk <= win.hi) ? win.f1[kk - win.lo] : SFGAS2[kk]; };
 
     if (m != 1) {
-        // CAL_CONTINUITY2 (1d.f:3006-3169)
-        const bool use_jf = n + 1 <= JFMAX;
-        if (use_jf) jump_flags(sh.jf, PCN, n);
-        PFOR(k, 1, n + 1) {
-            int rg = use_jf ? region_jf(sh.jf, PCN, n, k) : region_of(PCN, n, k);
+        // CAL_CONTINUITY2 (1d.f:3006-3169).  Outside region 2 -- cells more
+        // than two from every jump of PSI_CN -- it only clamps PSI_Z and
+        // PSI_C to [0, 1], which CONTI_Z/C of this substep already did (a
+        // NaN stays NaN either way): the loop runs over the cells within
+        // reach of a jump.
+        int jlo, jhi;
+        jump_range(PCN, n, sh.red, &jlo, &jhi);
+        const int sa = (jhi >= 0) ? (jlo - 2 < 1 ? 1 : jlo - 2) : 1;
+        const int sb = (jhi >= 0) ? (jhi + 3 > n ? n : jhi + 3) : 0;
+        PFOR(k, sa, sb + 1) {
+            int rg = region_of(PCN, n, k);
             if (k >= 2) {
                 double pz = PZ[k];
                 if (rg == 2 && touches(KCOU2[k], KCOU1[k + 1]))

@@ -39,7 +39,7 @@ int lica_noalloc_scratch(double **out)
 
 #ifndef LICA_EMUL
 // dynamic LDS, sized by the launch: the partitioned solves' workspace over
-// the refined window and the jump flags (BlkSh kernels, blk_dyn_bytes), or
+// the refined window (BlkSh kernels, blk_dyn_bytes), or
 // the multigrid cycle's vectors
 extern __shared__ double lica_tri_lds[];
 
@@ -49,7 +49,6 @@ DEV void tri_attach(BlkSh &sh)
         sh.tri.ws = lica_tri_lds;
         sh.tri.ws_rows = tri_ws_rows((int)blockDim.x);
         sh.win = (RefWin *)lica_tri_lds;
-        sh.jf = (unsigned char *)(sh.win + 1);
     }
     __syncthreads();
 }
@@ -294,14 +293,12 @@ static double *emul_tri_ws(int threads)
     return ws;
 }
 
-// the same overlay as on the GPU: the window and the jump flags under the
-// solve's workspace
+// the same overlay as on the GPU: the window under the solve's workspace
 static void attach(BlkSh &sh, int threads)
 {
     sh.tri.ws = emul_tri_ws(threads);
     sh.tri.ws_rows = tri_ws_rows(threads);
     sh.win = (RefWin *)sh.tri.ws;
-    sh.jf = (unsigned char *)(sh.win + 1);
 }
 
 int lica_launch_tri(int nsys, int n, double *a, double *b, double *c, double *g, int mode, int threads, void *stream)

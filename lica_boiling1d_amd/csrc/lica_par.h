@@ -43,6 +43,10 @@ struct EmulScope {
 #define PFOR_END
 #define SERIAL if (EmulScope lica_scope_{}; true)
 #define SERIAL_END
+// a work item's first index and the stride between its indices
+#define PTID 0
+#define PSTRIDE 1
+#define PLOOP if (EmulScope lica_scope_{}; true)
 extern int lica_emul_nthreads;
 #define NTHREADS (lica_emul_nthreads)
 #else
@@ -56,6 +60,9 @@ extern int lica_emul_nthreads;
 #define SERIAL if (threadIdx.x == 0)
 #define SERIAL_END __syncthreads();
 #define NTHREADS ((int)blockDim.x)
+#define PTID ((int)threadIdx.x)
+#define PSTRIDE ((int)blockDim.x)
+#define PLOOP
 
 // Lane moves of a 64-lane wave on DPP (data moved inside the VALU, no LDS
 // crossbar): CTRL 0x138 wave_shr:1 (lane i gets lane i-1), 0x130 wave_shl:1
