@@ -717,8 +717,11 @@ DEV void jump_range(const double *pc, int n, double *red, int *jlo, int *jhi)
 // stages the window again after its solve if it reads it there.  The
 // overlay keeps a 512-thread workgroup of the momentum stage at 44 KB
 // (three per CU) instead of 53 KB (two).
+// The per-wave partials of the workgroup reductions: up to 16 waves (1024
+// threads) x up to three values (red_max2, red_maxN<3>).
+#define BLK_RED_SLOTS (16 * 3)
 struct BlkSh {
-    double red[16];
+    double red[BLK_RED_SLOTS];
     double v[16];
     int iv[16];
     TriSh tri;          // ws at the start of the dynamic LDS (tri_ws_rows(blockDim) x 4 doubles)
@@ -1042,7 +1045,7 @@ struct BandSh {
     double pm1[LICA_MAXENT], pm2[LICA_MAXENT], mfd[LICA_MAXENT];
     double tf0[WMAX], tf1[WMAX];
     int mask_s[WMAX];
-    double red[16];
+    double red[BLK_RED_SLOTS];
     double v[24];
     int iv[24];
     int mcell[2 * LICA_MAXENT];

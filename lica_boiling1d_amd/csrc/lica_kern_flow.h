@@ -119,22 +119,28 @@ DEV void body_momentum(const BatchDev &bd, int inst, BlkSh &sh)
     tri_solve<NREF>(TA, TB, TC, TG, 2, n, bd.tri_mode, sh.tri);
     // the solve's workspace overlays the window: DIVGS reads it again
     stage_refwin(*sh.win, sfl - 2 * P.r - 4, sfh + 2 * P.r + 4, P.m3l, P.n3f + P.r, ZF, SFGAS, SFGAS2);
-    PFOR(k, 2, n + 1) { W[k] = TG[k] + W[k]; }
-    PFOR_END
-    // BC + BC_FLUX (1d.f:1812-1923)
-    SERIAL {
-        W[1] = 0.0;
-        W[n3] = W[n];
-        double qq3 = 0.0 + (W[n3] - W[1]) * P.sdx1 * P.sdy1;
+    // The solve's correction, BC + BC_FLUX (1d.f:1812-1923) and FSM_CHOI
+    // (1d.f:3365-3400) in one pass: FSM_CHOI reads W only at the cell it
+    // updates, and the boundary values need only the corrected W(n):
+    // W(1) = 0, W(n3) = W(n) + UBC with UBC = -(W(n3) - W(1)) SDX1 SDY1 / AA.
+    auto bc = [&](double wn) {
+        const double w1 = 0.0, wn3 = wn;
+        double qq3 = 0.0 + (wn3 - w1) * P.sdx1 * P.sdy1;
         double aa = 0.0 + 2.0 * P.xl * P.yl;
         double ubc = -qq3 / aa;
         W[1] = 0.0;
-        W[n3] = W[n3] + ubc;
+        W[n3] = wn3 + ubc;
+    };
+    PFOR(k, 2, n + 1) {
+        const double wk = TG[k] + W[k];
+        if (k == n) bc(wk);
+        W[k] = wk + dtconst * ((PP[k] - PP[k - 1]) * VVDZ[k]) * denf_zi(P, PZ, k);
     }
-    SERIAL_END
-    // FSM_CHOI (1d.f:3365-3400)
-    PFOR(k, 2, n + 1) { W[k] = W[k] + dtconst * ((PP[k] - PP[k - 1]) * VVDZ[k]) * denf_zi(P, PZ, k); }
     PFOR_END
+    if (n < 2) {
+        SERIAL { bc(W[n]); }
+        SERIAL_END
+    }
     // DIVGS (1d.f:3180-3303) and the POISSON matrix (3529-3543)
     PFOR(k, 1, n + 1) {
         double d = (k <= n - 1) ? (W[k + 1] - W[k]) * SSDZ[k] : 0.0;
@@ -448,43 +454,65 @@ DEV void body_ucalc(const BatchDev &bd, int inst, BlkSh &sh)
     double *__restrict__ W = cp.A(NS_W), *__restrict__ PP = cp.A(NS_P), *__restrict__ PZ = cp.A(NS_PSI_Z), *__restrict__ WO = cp.A(NS_WO), *__restrict__ WT = cp.A(NS_WT);
     const double *__restrict__ T = cp.A(NS_T);
     const double *__restrict__ VVDZ = cp.A(NS_VVDZ);
+    const double *__restrict__ Z = cp.A(NS_Z), *__restrict__ SSDZ = cp.A(NS_SSDZ), *__restrict__ ZF = cp.R(RF_ZF);
+    const double *__restrict__ SFGAS = cp.R(RF_SFGAS);
+    const int *__restrict__ KCOU1 = cp.AI(NSI_KCOU1), *__restrict__ KCOU2 = cp.AI(NSI_KCOU2);
     const double dtconst = S.dtconst;
     const long long m = S.m;
-    PFOR(k, 2, n + 1) { W[k] = W[k] - dtconst * (PP[k] - PP[k - 1]) * VVDZ[k] * denf_zi(P, PZ, k); }
-    PFOR_END
-    SERIAL {
-        W[1] = 0.0;
-        W[n3] = W[n];
-        sh.v[0] = PP[1];
-        S.pshift = PP[1];
+    const int sfl = S.sf_lo, sfh = S.sf_hi;
+    // p(1) before the shift (nothing below writes it before the shift pass)
+    const double phiref = PP[1];
+    stage_refwin(*sh.win, sfl - 2 * P.r - 4, sfh + 2 * P.r + 4, P.m3l, P.n3f + P.r, ZF, SFGAS, nullptr);
+    const RefWin &win = *sh.win;
+    auto zfa = [&](int kk) { return (kk >= win.lo && kk <= win.hi + 1) ? win.zf[kk - win.lo] : ZF[kk]; };
+    auto fsf = [&](int kk) { return (kk >= win.lo && kk <= win.hi) ? win.f0[kk - win.lo] : SFGAS[kk]; };
+    // UCALC with the end values W(1) = 0, W(n3) = W(n) (from the thread of
+    // cell n, which has the corrected W(n))
+    PFOR(k, 2, n + 1) {
+        const double wk = W[k] - dtconst * (PP[k] - PP[k - 1]) * VVDZ[k] * denf_zi(P, PZ, k);
+        W[k] = wk;
+        if (k == n) { W[1] = 0.0; W[n3] = wk; }
     }
-    SERIAL_END
-    const double phiref = sh.v[0];
-    PFOR(k, 1, n + 1) { PP[k] = PP[k] - phiref; }
     PFOR_END
-    SERIAL { PP[0] = PP[1]; }
-    SERIAL_END
-    double dvmax, c3max;
-    conv_maxima(cp, S, sh, &dvmax, &c3max);
-    // the end-of-step finiteness check of MAIN (W, T and P no longer change
-    // after the last substep's UCALC) takes effect after that substep's
-    // TRANSPORT, whose own STOPs come first
-    int bad = 0;
-    if (msub == 3)
-        bad = red_or(0, n3 + 1, [&](int k) { return !(isfinite(W[k]) && isfinite(T[k]) && isfinite(PP[k])); }, sh.red);
-    SERIAL {
-        S.dvmax = dvmax;
-        S.c3max = c3max;
-        S.nf_pending = bad;
+    if (n < 2) {
+        SERIAL { W[1] = 0.0; W[n3] = W[n]; }
+        SERIAL_END
     }
-    SERIAL_END
+    // one pass over k = 0..n3 (W no longer changes): p -= p(1) (1d.f:3433-3470,
+    // p(0) = p(1) after it), CONVRGE1's DVMAX and the CFL input (as
+    // conv_maxima), MAIN's end-of-step finiteness check (the last substep;
+    // W, T and P no longer change after it, and it takes effect after that
+    // substep's TRANSPORT, whose own STOPs come first) and VEL_INTP's
+    // UT = (U + UO)/2, UO = U (two_phase.f:5570-5616)
     const int first = (m == 1 && msub == 1);
-    PFOR(k, 0, n3 + 1) {
-        double wo = first ? W[k] : WO[k];
-        WT[k] = 0.5 * (W[k] + wo);
-        WO[k] = W[k];
+    double mx[3];
+    red_maxN<3>(0, n3 + 1, [&](int k, double *v) {
+        double pk = PP[k];
+        if (k >= 1 && k <= n) { pk = pk - phiref; PP[k] = pk; }
+        else if (k == 0) { pk = phiref - phiref; PP[0] = pk; }
+        v[0] = 0.0;
+        v[1] = 0.0;
+        const double wk = W[k];
+        if (k >= 1 && k <= n) {
+            double d = (W[k + 1] - wk) * SSDZ[k];
+            bool t = !(KCOU1[k + 1] < sfl || KCOU2[k] > sfh);
+            double ss = t ? box_avg_acc(zfa, P.ssdzf, fsf, Z[k], Z[k + 1], KCOU2[k], KCOU1[k + 1]) : 0.0;
+            d = d - ss * (P.den_diff / P.denp);
+            v[0] = fabs(d);
+            v[1] = fabs(wk + W[k + 1]) * SSDZ[k];
+        }
+        v[2] = (msub == 3 && !(isfinite(wk) && isfinite(T[k]) && isfinite(pk))) ? 1.0 : 0.0;
+        const double wo = first ? wk : WO[k];
+        WT[k] = 0.5 * (wk + wo);
+        WO[k] = wk;
+    }, sh.red, mx);
+    SERIAL {
+        S.pshift = phiref;
+        S.dvmax = mx[0];
+        S.c3max = mx[1];
+        S.nf_pending = (mx[2] > 0.0) ? 1 : 0;
     }
-    PFOR_END
+    SERIAL_END
 }
 
 // ---------------------------------------------------------------------
