@@ -400,6 +400,9 @@ struct lica_batch {
     InstState *d_st = nullptr;
     double *d_ns = nullptr, *d_rf = nullptr;
     int *d_nsi = nullptr, *d_rfi = nullptr, *d_order = nullptr;
+    // TEMP_GRAD's lead slots: 4 x LICA_TG_LEAD keys for the launches over
+    // every case, then the same for each lane
+    unsigned long long *d_lead = nullptr;
     double *d_mg = nullptr;      // multigrid scratch (allocated on first use of a multigrid mode)
     lica_net *net = nullptr;     // warm-start network of the learned mode (not owned)
     double *d_guess = nullptr, *d_scale = nullptr, *d_cap = nullptr, *d_hist = nullptr;
@@ -458,6 +461,8 @@ struct lica_batch {
 
 namespace {
 
+size_t lead_bytes() { return sizeof(unsigned long long) * 4 * LICA_TG_LEAD * (LICA_MAX_LANES + 1); }
+
 // launch shape of the set being enqueued
 const LaneCfg &cur_cfg(const lica_batch *b)
 {
@@ -504,6 +509,7 @@ BatchDev make_bd(const lica_batch *b, int msub)
     // kernel's own start test is then cheaper than the pass)
     bd.wscale = (b->poisson_mode == LICA_POISSON_LEARNED && b->net) ? b->d_scale : nullptr;
     bd.mg_lds = mg_lds(b);
+    bd.lead = nullptr;
     return bd;
 }
 
@@ -581,6 +587,9 @@ int launch(lica_batch *b, int which, int msub)
         bd.order = lane_order(b);
         bd.ninst = lane_cases(b, b->cur_lane);
     }
+    // the lead slots of the set being enqueued (a key holds a position in 24 bits)
+    if (which == KL_TEMP_GRAD && b->d_lead && bd.ninst <= (1 << 24))
+        bd.lead = b->d_lead + (size_t)(b->cur_lane + 1) * 4 * LICA_TG_LEAD;
     int rc = lica_launch(which, bd, block_threads(b, which), (void *)cur_stream(b));
     return rc == 0 ? 0 : LICA_HIP_ERROR;
 }
@@ -927,6 +936,7 @@ int lica_batch_create(int64_t ninst, const lica_case_desc *cases, lica_batch **o
     rc = rc ? rc : dmalloc((void **)&b->d_nsi, sizeof(int) * b->nsi_total);
     rc = rc ? rc : dmalloc((void **)&b->d_rfi, sizeof(int) * b->rfi_total);
     rc = rc ? rc : dmalloc((void **)&b->d_order, sizeof(int) * (size_t)ninst);
+    rc = rc ? rc : dmalloc((void **)&b->d_lead, lead_bytes());
     rc = rc ? rc : dmalloc((void **)&b->d_guess, sizeof(double) * (size_t)gs_total);
     rc = rc ? rc : dmalloc((void **)&b->d_cap, sizeof(double) * 3 * (size_t)gs_total);
     rc = rc ? rc : dmalloc((void **)&b->d_scale, sizeof(double) * LICA_WARM_INFO * (size_t)ninst);
@@ -1198,6 +1208,7 @@ int lica_batch_autotune(lica_batch *b, int64_t steps)
     rc = rc ? rc : sc.add(b->d_scale, sizeof(double) * LICA_WARM_INFO * (size_t)b->ninst);
     rc = rc ? rc : sc.add(b->d_cap, sizeof(double) * 3 * (size_t)guess_doubles(b));
     if (b->d_hist) rc = rc ? rc : sc.add(b->d_hist, sizeof(double) * (size_t)hist_doubles(b));
+    rc = rc ? rc : sc.add(b->d_lead, lead_bytes());
     rc = rc ? rc : sc.save(b->stream);
     if (rc) return rc;
     const bool cap_valid = b->cap_valid;
@@ -1481,7 +1492,7 @@ void lica_batch_destroy(lica_batch *b)
     }
 #endif
     dfree(b->d_par); dfree(b->d_st); dfree(b->d_ns); dfree(b->d_rf);
-    dfree(b->d_nsi); dfree(b->d_rfi); dfree(b->d_order); dfree(b->d_order_lanes); dfree(b->d_mg);
+    dfree(b->d_nsi); dfree(b->d_rfi); dfree(b->d_order); dfree(b->d_order_lanes); dfree(b->d_lead); dfree(b->d_mg);
     dfree(b->d_guess); dfree(b->d_scale); dfree(b->d_cap); dfree(b->d_netcases); dfree(b->d_hist);
 #ifndef LICA_EMUL
     if (b->stream) (void)hipStreamDestroy(b->stream);

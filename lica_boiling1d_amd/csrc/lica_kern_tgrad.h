@@ -101,6 +101,7 @@ DEV void xneighbors(const double *a, double *l, double *r)
     }
 }
 DEV bool xwave_any(bool f) { return f; }
+DEV void xwave_urgent(bool on) { (void)on; }
 #else
 template <int CPL>
 DEV int xcell(int s) { return CPL * (int)(threadIdx.x & 63) + s; }
@@ -118,6 +119,16 @@ DEV void xneighbors(const double *a, double *l, double *r)
     }
 }
 DEV bool xwave_any(bool f) { return __any((int)f) != 0; }
+// Most loops converge in their first 100 iterations or after one halving; a
+// loop that needs a second halving is several times longer than the batch's
+// typical one (ten times at the tail) and ends its case's stage last: its
+// wave takes the SIMD's issue priority over the other waves (which have the
+// parallel slack) until the extrapolation ends.
+DEV void xwave_urgent(bool on)
+{
+    if (on) __builtin_amdgcn_s_setprio(2);
+    else __builtin_amdgcn_s_setprio(0);
+}
 #endif
 
 // one forward-Euler stage on a (cells outside [ia,ib] stay as they are);
@@ -263,9 +274,10 @@ DEV int temp_extp_loops(const ExtpWin &X, const InstParam &P, const double *PCN,
             if (!xwave_any(fail)) { conv = 1; break; }
         }
         if (conv) break;
-        if (++halvings > 20) return ST_EXTRAP;
+        if (++halvings > 20) { xwave_urgent(false); return ST_EXTRAP; }
         *halv += 1.0;
         redtl = 0.5 * redtl;
+        xwave_urgent(halvings >= 2);
     }
     for (int s = 0; s < CPL; s++) { c.dtn[s] = a[s]; c.small[s] = 0; }
     // (b) T (ND=1) with the extrapolated dT/dn as source
@@ -292,10 +304,12 @@ DEV int temp_extp_loops(const ExtpWin &X, const InstParam &P, const double *PCN,
             if (!xwave_any(fail)) { conv = 1; break; }
         }
         if (conv) break;
-        if (++halvings > 20) return ST_EXTRAP;
+        if (++halvings > 20) { xwave_urgent(false); return ST_EXTRAP; }
         *halv += 1.0;
         redtl = 0.5 * redtl;
+        xwave_urgent(halvings >= 2);
     }
+    xwave_urgent(false);
     for (int s = 0; s < CPL; s++) {
         int i = xcell<CPL>(s);
         if (i < nw) tw[i] = t[s];
@@ -605,3 +619,52 @@ DEV void body_temp_grad(const BatchDev &bd, int inst, TgradSh &TS)
     }
     SERIAL_END
 }
+
+// ---------------------------------------------------------------------
+// Lead workgroups of TEMP_GRAD.  A case's extrapolation takes from about a
+// hundred to over a thousand iterations per substep (one dependent chain per
+// phase), and a case that is slow in one substep is slow in the next: if it
+// starts late it alone sets the end of the launch.  A launch with lead slots
+// runs LICA_TG_LEAD lead workgroups before its cases.  Lead j takes the case
+// whose key is in slot j of the previous substep's row -- the costliest case
+// among the positions p = j (mod LICA_TG_LEAD) -- and the workgroup of that
+// case's position returns at once.  Both read the same key, which no
+// workgroup of the launch writes (they record into the current substep's
+// row), and decide from it and the case's step counter alone, so every case
+// runs exactly once.  Key: substep index 3m + msub (28 bits), cost (the
+// slower phase's iterations, 12 bits), position (24 bits); atomicMax keeps
+// the latest substep's costliest case.  A key of another substep (the first
+// launch, a restored state) leads nothing.
+#define TG_SUB_MASK ((1ull << 28) - 1)
+DEV unsigned long long tg_sub(const InstState &S, int msub)
+{
+    return (unsigned long long)(3 * S.m + msub) & TG_SUB_MASK;
+}
+
+// position of the case that workgroup wg runs (-1: none)
+DEV int tg_lead_pos(const BatchDev &bd, int wg)
+{
+    const bool lead = wg < LICA_TG_LEAD;
+    const int pos = lead ? -1 : wg - LICA_TG_LEAD;
+    const int j = lead ? wg : pos % LICA_TG_LEAD;
+    const int prev = (bd.msub == 1) ? 3 : bd.msub - 1;
+    const unsigned long long key = bd.lead[prev * LICA_TG_LEAD + j];
+    const int kp = (int)(key & 0xFFFFFFull);
+    bool take = key != 0ull && kp < bd.ninst && kp % LICA_TG_LEAD == j;
+    if (take) take = (key >> 36) == ((tg_sub(bd.st[bd.order[kp]], bd.msub) + TG_SUB_MASK) & TG_SUB_MASK);
+    if (lead) return take ? kp : -1;
+    return (take && kp == pos) ? -1 : pos;
+}
+
+#ifndef LICA_EMUL
+// the case's cost into the current substep's row (TS.xit negative: no
+// extrapolation ran)
+DEV void tg_lead_record(const BatchDev &bd, int inst, int pos, const TgradSh &TS)
+{
+    if (!(TS.xit[0] >= 0.0 && TS.xit[1] >= 0.0)) return;
+    const double c = fmax(TS.xit[0], TS.xit[1]);
+    const unsigned long long cost = (c >= 4095.0) ? 4095ull : (unsigned long long)c;
+    const unsigned long long key = (tg_sub(bd.st[inst], bd.msub) << 36) | (cost << 24) | (unsigned long long)pos;
+    atomicMax(&bd.lead[bd.msub * LICA_TG_LEAD + pos % LICA_TG_LEAD], key);
+}
+#endif

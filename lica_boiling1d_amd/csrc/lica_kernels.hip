@@ -71,11 +71,22 @@ __global__ void __launch_bounds__(1024) k_cont_energy(BatchDev bd)
     body_cont_energy(bd, bd.order[blockIdx.x], sh);
 }
 // 20.1 KB of LDS and at most 128 VGPRs: eight two-wave workgroups per CU,
-// four waves per SIMD to hide the latency of the extrapolation chains
+// four waves per SIMD to hide the latency of the extrapolation chains.  With
+// lead slots the grid is LICA_TG_LEAD lead workgroups, then one per case.
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k_temp_grad(BatchDev bd)
 {
     __shared__ TgradSh sh;
-    body_temp_grad(bd, bd.order[blockIdx.x], sh);
+    if (!bd.lead) {
+        body_temp_grad(bd, bd.order[blockIdx.x], sh);
+        return;
+    }
+    const int pos = tg_lead_pos(bd, (int)blockIdx.x);
+    if (pos < 0) return;
+    const int inst = bd.order[pos];
+    if (threadIdx.x == 0) { sh.xit[0] = -1.0; sh.xit[1] = -1.0; }
+    body_temp_grad(bd, inst, sh);
+    __syncthreads();
+    if (threadIdx.x == 0) tg_lead_record(bd, inst, pos, sh);
 }
 // the momentum stage of the Thomas and multigrid modes (no refinement in
 // its solve): 79 VGPRs at six waves per SIMD (a few spills), so that three
@@ -258,7 +269,9 @@ int lica_launch(int which, const BatchDev &bd, int threads, void *stream)
     case KL_INIT: LAUNCH(k_init, threads); break;
     case KL_DT: LAUNCH_BLK(k_dt, threads); break;
     case KL_CONT_ENERGY: LAUNCH_TRI(k_cont_energy, threads); break;
-    case KL_TEMP_GRAD: LAUNCH(k_temp_grad, 128); break;
+    case KL_TEMP_GRAD:
+        hipLaunchKernelGGL(k_temp_grad, dim3(bd.ninst + (bd.lead ? LICA_TG_LEAD : 0)), dim3(128), 0, (hipStream_t)stream, bd);
+        break;
     case KL_MOMENTUM:
         if (bd.tri_mode == 1) LAUNCH_TRI(k_momentum_refine, threads);
         else LAUNCH_TRI(k_momentum, threads);

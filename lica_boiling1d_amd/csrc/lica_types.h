@@ -92,8 +92,15 @@ struct BatchDev {
                          // backward error of the warm-start base, its residual scale s_r, the row scale
                          // s_b, the network step alpha, the base's backward error as the multigrid start
                          // test computes it, 1 if the solve may take the guess as it is
+    unsigned long long *lead;   // TEMP_GRAD's lead slots of this launch's case set (4 x LICA_TG_LEAD
+                                // keys, by substep), or null: no lead workgroups
 };
 #define LICA_WARM_INFO 7
+// TEMP_GRAD's lead workgroups: a launch with lead slots runs LICA_TG_LEAD
+// workgroups ahead of its cases, each taking the costliest case (by the
+// previous substep's extrapolation iterations) of one residue class of the
+// positions
+#define LICA_TG_LEAD 32
 
 #define LICA_MG_FP64_MODE 2
 #define LICA_MG_MIXED_MODE 3
