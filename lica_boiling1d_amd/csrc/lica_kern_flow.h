@@ -119,10 +119,15 @@ DEV void body_momentum(const BatchDev &bd, int inst, BlkSh &sh)
     tri_solve<NREF>(TA, TB, TC, TG, 2, n, bd.tri_mode, sh.tri);
     // the solve's workspace overlays the window: DIVGS reads it again
     stage_refwin(*sh.win, sfl - 2 * P.r - 4, sfh + 2 * P.r + 4, P.m3l, P.n3f + P.r, ZF, SFGAS, SFGAS2);
-    // The solve's correction, BC + BC_FLUX (1d.f:1812-1923) and FSM_CHOI
-    // (1d.f:3365-3400) in one pass: FSM_CHOI reads W only at the cell it
-    // updates, and the boundary values need only the corrected W(n):
-    // W(1) = 0, W(n3) = W(n) + UBC with UBC = -(W(n3) - W(1)) SDX1 SDY1 / AA.
+    // The solve's correction, BC + BC_FLUX (1d.f:1812-1923), FSM_CHOI
+    // (1d.f:3365-3400), DIVGS (3180-3303) and the POISSON matrix (3529-3543)
+    // in one pass over chunks of the workgroup's size.  The thread of cell k
+    // also evaluates FSM_CHOI's W(k+1) (the same expression of the same
+    // values) for DIVGS's difference, and a chunk writes W after a barrier,
+    // when every thread of the chunk has read the old W(k), W(k+1) (a later
+    // chunk reads only its own cells and the one after it).  W(1) = 0 and
+    // W(n3) = W(n) + UBC from the corrected W(n) before FSM_CHOI, with
+    // UBC = -(W(n3) - W(1)) SDX1 SDY1 / AA.
     auto bc = [&](double wn) {
         const double w1 = 0.0, wn3 = wn;
         double qq3 = 0.0 + (wn3 - w1) * P.sdx1 * P.sdy1;
@@ -131,28 +136,35 @@ DEV void body_momentum(const BatchDev &bd, int inst, BlkSh &sh)
         W[1] = 0.0;
         W[n3] = wn3 + ubc;
     };
-    PFOR(k, 2, n + 1) {
+    auto fsm = [&](int k, double zi) {
         const double wk = TG[k] + W[k];
-        if (k == n) bc(wk);
-        W[k] = wk + dtconst * ((PP[k] - PP[k - 1]) * VVDZ[k]) * denf_zi(P, PZ, k);
-    }
-    PFOR_END
+        return wk + dtconst * ((PP[k] - PP[k - 1]) * VVDZ[k]) * zi;
+    };
     if (n < 2) {
         SERIAL { bc(W[n]); }
         SERIAL_END
     }
-    // DIVGS (1d.f:3180-3303) and the POISSON matrix (3529-3543)
-    PFOR(k, 1, n + 1) {
-        double d = (k <= n - 1) ? (W[k + 1] - W[k]) * SSDZ[k] : 0.0;
-        double ss = touches(KCOU2[k], KCOU1[k + 1]) ? box_avg_acc(zfa, P.ssdzf, fsf, Z[k], Z[k + 1], KCOU2[k], KCOU1[k + 1]) : 0.0;
-        d = d - ss * (P.den_diff / (P.denp * P.denm));
-        if (k >= 2) d = d * dtconsti;
-        double fixkl = (k == 1) ? 1.0 : 0.0;
-        double ak = VVDZ[k] * SSDZ[k] * denf_zi(P, PZ, k) * (1.0 - fixkl);
-        double ck = VVDZ[k + 1] * SSDZ[k] * denf_zi(P, PZ, k + 1);
-        PA[k] = ak; PCc[k] = ck; PB[k] = -(ak + ck); PG[k] = d;
+    for (int k0 = 1; k0 <= n; k0 += PSTRIDE) {
+        const int k = k0 + PTID;
+        double wk = 0.0, wn = 0.0;
+        if (k <= n) {
+            const double zk = denf_zi(P, PZ, k), zk1 = denf_zi(P, PZ, k + 1);
+            if (k >= 2) wk = fsm(k, zk);
+            if (k == n) wn = TG[n] + W[n];
+            double d = (k <= n - 1) ? (fsm(k + 1, zk1) - wk) * SSDZ[k] : 0.0;
+            double ss = touches(KCOU2[k], KCOU1[k + 1]) ? box_avg_acc(zfa, P.ssdzf, fsf, Z[k], Z[k + 1], KCOU2[k], KCOU1[k + 1]) : 0.0;
+            d = d - ss * (P.den_diff / (P.denp * P.denm));
+            if (k >= 2) d = d * dtconsti;
+            double fixkl = (k == 1) ? 1.0 : 0.0;
+            double ak = VVDZ[k] * SSDZ[k] * zk * (1.0 - fixkl);
+            double ck = VVDZ[k + 1] * SSDZ[k] * zk1;
+            PA[k] = ak; PCc[k] = ck; PB[k] = -(ak + ck); PG[k] = d;
+        }
+        BARRIER();
+        if (k >= 2 && k <= n) W[k] = wk;
+        if (k == n && n >= 2) bc(wn);
     }
-    PFOR_END
+    BARRIER();
 }
 
 // POISSON solve (1d.f:3544-3576) of the system written by body_momentum
