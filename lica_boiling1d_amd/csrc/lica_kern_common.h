@@ -626,13 +626,20 @@ DEV void tri_solve(double *a, double *b, double *c, double *g, int l1, int l2, i
 }
 
 // A tridiagonal solve whose rows come from a function instead of arrays:
-// row(k, a, b, c, g) gives row k of l1..l2 (called once per row, by the
-// thread whose chunk holds it), out(k, x) receives its solution (after every
-// row was read).  In mode 2 with the rows in the per-thread register chunks
-// nothing goes through global memory: the chunks are filled by row() and the
-// solve is tri_solve's register path, operation for operation.  Otherwise
-// the rows go to the arrays a, b, c, g and tri_solve runs on them.
-template <class RowF, class OutF>
+// row(k, a, b, c, g) gives row k of l1..l2 (called once per row), out(k, x)
+// receives its solution (after every row was read).  In mode 2 with one
+// chunk per thread nothing goes through global memory: the rows are
+// evaluated in the coalesced order, CR = NTHREADS / TRI_RPT chunks at a time
+// (at most NTHREADS rows, one per thread), staged in LDS as slot j of chunk
+// ch at j * CR + ch (so that a chunk's thread reads its rows without bank
+// conflicts), and moved into the chunks' registers; the solve is then
+// tri_solve's register path, operation for operation.  (Evaluating each
+// chunk's rows in its own thread reads the rows' inputs lane-strided, 64 B
+// apart, and the lines leave the caches before the next row of the chunk
+// comes back to them: three to four times the bytes.)  The staging uses the
+// LDS workspace, so row() must not read what the workspace overlays.
+// Otherwise the rows go to the arrays a, b, c, g and tri_solve runs on them.
+template <int NREF_MAX = TRI_NREF, class RowF, class OutF>
 DEV void tri_solve_rows(double *a, double *b, double *c, double *g, int l1, int l2, int mode, TriSh &ts, RowF row,
                         OutF out)
 {
@@ -647,7 +654,7 @@ DEV void tri_solve_rows(double *a, double *b, double *c, double *g, int l1, int 
     if (!regs) {
         PFOR(k, l1, l2 + 1) { row(k, a[k], b[k], c[k], g[k]); }
         PFOR_END
-        tri_solve(a, b, c, g, l1, l2, mode, ts);
+        tri_solve<NREF_MAX>(a, b, c, g, l1, l2, mode, ts);
         PFOR(k, l1, l2 + 1) { out(k, g[k]); }
         PFOR_END
         return;
@@ -658,16 +665,44 @@ DEV void tri_solve_rows(double *a, double *b, double *c, double *g, int l1, int 
     const bool act = ch < nc;
     int s = 0, L = 0;
     double xa[TRI_RPT], xb[TRI_RPT], xc[TRI_RPT], x[TRI_RPT];
-    if (act) {
-        chunk_bounds(nrow, nc, ch, &s, &L);
-        static_for<TRI_RPT>([&](auto jc) {
-            constexpr int j = decltype(jc)::value;
-            if (j < L) {
-                row(l1 + s + j, xa[j], xb[j], xc[j], x[j]);
-            } else {
-                xa[j] = 0.0; xb[j] = 1.0; xc[j] = 0.0; x[j] = 0.0;
+    if (act) chunk_bounds(nrow, nc, ch, &s, &L);
+    {
+        const int CR = NTHREADS / TRI_RPT;
+        double *SA = ts.ws, *SB = SA + TRI_RPT * CR, *SC = SB + TRI_RPT * CR, *SG = SC + TRI_RPT * CR;
+        // chunks 0..rem-1 hold base+1 rows, the others base (chunk_bounds)
+        const int base = nrow / nc, rem = nrow % nc, big = rem * (base + 1);
+        for (int c0 = 0; c0 < nc; c0 += CR) {
+            const int c1 = (c0 + CR < nc) ? c0 + CR : nc;
+            int r0, r1, ll;
+            chunk_bounds(nrow, nc, c0, &r0, &ll);
+            chunk_bounds(nrow, nc, c1 - 1, &r1, &ll);
+            r1 += ll;
+            const int i = r0 + (int)threadIdx.x;
+            if (i < r1) {
+                int cc, j;
+                if (i < big) { cc = i / (base + 1); j = i - cc * (base + 1); }
+                else { cc = rem + (i - big) / base; j = (i - big) - (cc - rem) * base; }
+                double ra, rb, rc, rg;
+                row(l1 + i, ra, rb, rc, rg);
+                const int q = j * CR + (cc - c0);
+                SA[q] = ra; SB[q] = rb; SC[q] = rc; SG[q] = rg;
             }
-        });
+            BARRIER();
+            if (ch >= c0 && ch < c1) {
+                static_for<TRI_RPT>([&](auto jc) {
+                    constexpr int j = decltype(jc)::value;
+                    const int q = j * CR + (ch - c0);
+                    if (j < L) {
+                        xa[j] = SA[q]; xb[j] = SB[q]; xc[j] = SC[q]; x[j] = SG[q];
+                    } else {
+                        xa[j] = 0.0; xb[j] = 1.0; xc[j] = 0.0; x[j] = 0.0;
+                    }
+                });
+            }
+            BARRIER();
+        }
+    }
+    if (act) {
         double r[8];
         chunk_reduce_reg(xa, xb, xc, x, L, r);
         put_end_rows(RA, RB, RC, RG, ch, nc, r);

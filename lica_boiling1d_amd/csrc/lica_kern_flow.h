@@ -110,13 +110,9 @@ DEV void body_momentum(const BatchDev &bd, int inst, BlkSh &sh)
         double dk2 = -dk1 - dk3;
         ta = dk1; tc = dk3; tb = 1.0 + dk2; tg = rhs1;
     };
-    // the rows go through the scratch arrays: assembled in the coalesced
-    // order, the solve reads them back by chunks (assembling straight into
-    // the chunks reads the fifteen input arrays chunk-strided, which moved
-    // 65% more HBM bytes for no gain in time)
-    PFOR(k, 2, n + 1) { lhsw_row(k, TA[k], TB[k], TC[k], TG[k]); }
-    PFOR_END
-    tri_solve<NREF>(TA, TB, TC, TG, 2, n, bd.tri_mode, sh.tri);
+    // in the multigrid modes the rows reach the solve's registers through
+    // LDS (no row arrays); otherwise through TA, TB, TC, TG
+    tri_solve_rows<NREF>(TA, TB, TC, TG, 2, n, bd.tri_mode, sh.tri, lhsw_row, [&](int k, double x) { TG[k] = x; });
     // the solve's workspace overlays the window: DIVGS reads it again
     stage_refwin(*sh.win, sfl - 2 * P.r - 4, sfh + 2 * P.r + 4, P.m3l, P.n3f + P.r, ZF, SFGAS, SFGAS2);
     // The solve's correction, BC + BC_FLUX (1d.f:1812-1923), FSM_CHOI

@@ -89,9 +89,8 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k
     if (threadIdx.x == 0) tg_lead_record(bd, inst, pos, sh);
 }
 // the momentum stage of the Thomas and multigrid modes (no refinement in
-// its solve): 79 VGPRs at six waves per SIMD (a few spills), so that three
-// 512-thread workgroups share a CU (44 KB of LDS each)
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) k_momentum(BatchDev bd)
+// its solve)
+__global__ void __launch_bounds__(1024) k_momentum(BatchDev bd)
 {
     __shared__ BlkSh sh;
     tri_attach(sh);
