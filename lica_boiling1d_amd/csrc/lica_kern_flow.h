@@ -192,11 +192,13 @@ DEV void body_poisson_mg(const BatchDev &bd, int inst, MgSh &sh)
     const InstParam &P = *cp.P;
     const int n = P.n, n3 = P.n3;
     double *PP = cp.A(NS_P);
-    const double *PA = cp.A(NS_PA), *PB = cp.A(NS_PB), *PCc = cp.A(NS_PC), *PG = cp.A(NS_PG);
+    const double *PA = cp.A(NS_PA), *PCc = cp.A(NS_PC), *PG = cp.A(NS_PG);
     const bool warm = LICA_MG_WARM(bd.pmode) && bd.guess && bd.hist;
     const double *x0 = warm ? bd.guess + P.gs_off : nullptr;
     double res = 0.0;
-    const int cyc = mg_solve(PA, PB, PCc, PG, PP, n, bd.mg + P.mg_off, bd.pmode, bd.mg_rtol, bd.mg_maxc, x0, sh,
+    // PB = -(PA + PC) row by row (the momentum stage's POISSON matrix): the
+    // solve evaluates it instead of reading it
+    const int cyc = mg_solve(PA, nullptr, PCc, PG, PP, n, bd.mg + P.mg_off, bd.pmode, bd.mg_rtol, bd.mg_maxc, x0, sh,
                              &res);
     if (warm) {
         // the solution joins this substep's history ring
@@ -315,7 +317,10 @@ DEV void body_warm_base(const BatchDev &bd, int inst, WarmSh &sh)
     CaseP cp = case_ptrs(bd, inst);
     const InstParam &P = *cp.P;
     const int n = P.n, s = bd.msub - 1;
-    const double *__restrict__ PA = cp.A(NS_PA), *__restrict__ PB = cp.A(NS_PB), *__restrict__ PCc = cp.A(NS_PC);
+    const double *__restrict__ PA = cp.A(NS_PA), *__restrict__ PCc = cp.A(NS_PC);
+    // PB = -(PA + PC) row by row (the momentum stage's POISSON matrix),
+    // evaluated instead of read
+    auto pbv = [&](int row) { return -(PA[row] + PCc[row]); };
     const double *__restrict__ PG = cp.A(NS_PG), *__restrict__ PP = cp.A(NS_P);
     double *B = bd.guess + P.gs_off;
     const double *Hs = bd.hist + P.hist_off + (size_t)s * MG_HIST * (size_t)n;
@@ -328,7 +333,7 @@ DEV void body_warm_base(const BatchDev &bd, int inst, WarmSh &sh)
         return PP[row] + pq;
     };
     auto aval = [&](int j, int row) -> double {   // (A v_j)(row), rows 1..n of the POISSON operator
-        double v = PB[row] * val(j, row);
+        double v = pbv(row) * val(j, row);
         if (row > 1) v = v + PA[row] * val(j, row - 1);
         if (row < n) v = v + PCc[row] * val(j, row + 1);
         return v;
@@ -381,16 +386,16 @@ DEV void body_warm_base(const BatchDev &bd, int inst, WarmSh &sh)
     red_maxN<6>(1, n + 1, [&](int row, double *v) {
         const double x = B[row];
         const double xl = (row > 1) ? B[row - 1] : 0.0, xr = (row < n) ? B[row + 1] : 0.0;
-        double ax = PB[row] * x;
+        double ax = pbv(row) * x;
         if (row > 1) ax = ax + PA[row] * xl;
         if (row < n) ax = ax + PCc[row] * xr;
-        double rm = PG[row] - PB[row] * x;
+        double rm = PG[row] - pbv(row) * x;
         if (row > 1) rm = rm - PA[row] * xl;
         if (row < n) rm = rm - PCc[row] * xr;
         const double ak = (row > 1) ? fabs(PA[row]) : 0.0, ck = (row < n) ? fabs(PCc[row]) : 0.0;
         v[0] = fabs(PG[row] - ax);
-        v[1] = fabs(PB[row]);
-        v[2] = ak + fabs(PB[row]) + ck;
+        v[1] = fabs(pbv(row));
+        v[2] = ak + fabs(pbv(row)) + ck;
         v[3] = fabs(x);
         v[4] = fabs(PG[row]);
         v[5] = fabs(rm);

@@ -166,9 +166,17 @@ struct alignas(4 * sizeof(T)) MgFw {
 // row i (0..n-1) of the finest level from the POISSON rows k = i+1 (a(1) and
 // c(n) are not part of the system): the row sum exactly, by error-free
 // additions, rounded once
+// b(k) of the rows: b null means the POISSON rows' b = -(a + c), evaluated
+// as the momentum stage evaluated it (the same value as the stored one,
+// without reading it)
+DEV double mg_b(const double *a, const double *b, const double *c, int k)
+{
+    return b ? b[k] : -(a[k] + c[k]);
+}
+
 DEV MgSet mg_row0(const double *a, const double *b, const double *c, int n, int i)
 {
-    const double ai = (i == 0) ? 0.0 : a[i + 1], bi = b[i + 1], ci = (i == n - 1) ? 0.0 : c[i + 1];
+    const double ai = (i == 0) ? 0.0 : a[i + 1], bi = mg_b(a, b, c, i + 1), ci = (i == n - 1) ? 0.0 : c[i + 1];
     double t, e1, s, e2;
     two_sum(ai, ci, &t, &e1);
     two_sum(t, bi, &s, &e2);
@@ -397,7 +405,7 @@ DEV void mg_correct(const double *a, const double *b, const double *c, const dou
     const T *e0 = mg_e0<T>(V, sh);
     auto xn = [&](int k) -> double { return first ? xa[k] : xa[k] + (double)e0[k]; };
     red_maxN<MG_NRED>(0, n, [&](int i, double *v) {
-        const double ai = (i == 0) ? 0.0 : a[i + 1], bi = b[i + 1], ci = (i == n - 1) ? 0.0 : c[i + 1];
+        const double ai = (i == 0) ? 0.0 : a[i + 1], bi = mg_b(a, b, c, i + 1), ci = (i == n - 1) ? 0.0 : c[i + 1];
         const double x = xn(i), xl = (i > 0) ? xn(i - 1) : 0.0, xr = (i < n - 1) ? xn(i + 1) : 0.0;
         const double r = dd_residual(g[i + 1], ai, xl, bi, x, ci, xr);
         xb[i] = x;
@@ -434,7 +442,7 @@ DEV void mg_first(const double *a, const double *b, const double *c, const doubl
         if (has_f) fw[J] = f;
         for (int q = 0; q < 5; q++) v[q] = 0.0;
         for (int k = 2 * J; k <= 2 * J + 1 && k < n; k++) {
-            const double ai = (k == 0) ? 0.0 : a[k + 1], bi = b[k + 1], ci = (k == n - 1) ? 0.0 : c[k + 1];
+            const double ai = (k == 0) ? 0.0 : a[k + 1], bi = mg_b(a, b, c, k + 1), ci = (k == n - 1) ? 0.0 : c[k + 1];
             const double x = xv(k), xl = (k > 0) ? xv(k - 1) : 0.0, xr = (k < n - 1) ? xv(k + 1) : 0.0;
             const double r = dd_residual(g[k + 1], ai, xl, bi, x, ci, xr);
             xb[k] = x;
@@ -496,7 +504,8 @@ DEV int mg_run(const double *a, const double *b, const double *c, const double *
     return conv ? cyc : -1;
 }
 
-// Solve rows 1..n of a, b, c, g (a(1), c(n) ignored) into p(1..n).
+// Solve rows 1..n of a, b, c, g (a(1), c(n) ignored) into p(1..n); b null:
+// b = -(a + c) (the POISSON rows).
 // mode 2: FP64 cycles; 3-5: FP32 cycles (x0, rows 1..n, holds the initial
 // guess for modes 4 and 5).  Returns the cycles, or -1 when maxc cycles did
 // not meet the stopping test; *res_out the final backward error.  Rows with
