@@ -1157,18 +1157,26 @@ struct StateCopy {
 // GPU time of `steps` steps of every lane, the lanes on their streams at
 // once as in a run (events on the batch's stream before the fork and after
 // the join)
-int time_steps(lica_batch *b, int64_t steps, double *ms)
+int time_steps(lica_batch *b, int64_t steps, double *ms, double *lane_ms = nullptr)
 {
-    hipEvent_t e[2];
+    // lane_ms: each lane's time from its first launch to the end of its last
+    // (the host enqueues the lanes' first steps one after the other)
+    hipEvent_t e[2], ls[LICA_MAX_LANES] = {}, le[LICA_MAX_LANES] = {};
+    const int nl = b->lanes;
     if (hipEventCreate(&e[0]) != hipSuccess) return LICA_HIP_ERROR;
     if (hipEventCreate(&e[1]) != hipSuccess) { (void)hipEventDestroy(e[0]); return LICA_HIP_ERROR; }
-    const int nl = b->lanes;
-    int rc = (hipEventRecord(e[0], b->stream) == hipSuccess) ? 0 : LICA_HIP_ERROR;
+    int rc = 0;
+    for (int l = 0; l < nl && lane_ms && !rc; l++)
+        if (hipEventCreate(&ls[l]) != hipSuccess || hipEventCreate(&le[l]) != hipSuccess) rc = LICA_HIP_ERROR;
+    rc = rc ? rc : (hipEventRecord(e[0], b->stream) == hipSuccess) ? 0 : LICA_HIP_ERROR;
     if (!rc && nl > 1 && fork_lanes(b)) rc = LICA_HIP_ERROR;
     for (int64_t s = 0; s < steps && !rc; s++)
         for (int l = 0; l < nl && !rc; l++) {
             b->cur_lane = (nl > 1) ? l : -1;
-            rc = enqueue_step_cases(b);
+            if (lane_ms && s == 0 && hipEventRecord(ls[l], cur_stream(b)) != hipSuccess) rc = LICA_HIP_ERROR;
+            rc = rc ? rc : enqueue_step_cases(b);
+            if (!rc && lane_ms && s == steps - 1 && hipEventRecord(le[l], cur_stream(b)) != hipSuccess)
+                rc = LICA_HIP_ERROR;
         }
     b->cur_lane = -1;
     if (nl > 1 && join_lanes(b)) rc = rc ? rc : LICA_HIP_ERROR;
@@ -1176,10 +1184,56 @@ int time_steps(lica_batch *b, int64_t steps, double *ms)
     if (!rc && (hipEventRecord(e[1], b->stream) != hipSuccess || hipEventSynchronize(e[1]) != hipSuccess ||
                 hipEventElapsedTime(&t, e[0], e[1]) != hipSuccess))
         rc = LICA_HIP_ERROR;
+    for (int l = 0; l < nl && lane_ms && !rc; l++) {
+        float tl = 0.0f;
+        if (hipEventElapsedTime(&tl, ls[l], le[l]) != hipSuccess) rc = LICA_HIP_ERROR;
+        lane_ms[l] = (double)tl;
+    }
+    for (int l = 0; l < nl; l++) {
+        if (ls[l]) (void)hipEventDestroy(ls[l]);
+        if (le[l]) (void)hipEventDestroy(le[l]);
+    }
     (void)hipEventDestroy(e[0]);
     (void)hipEventDestroy(e[1]);
     *ms = (double)t;
     return rc;
+}
+
+// New cuts of the binned split from the lanes' own times: a position's
+// modelled time is its work in the split's measure (n + 512) times its
+// lane's measured time per unit of work, and lane l's cut is where the
+// cumulative modelled time reaches l/L of the total (at least one case per
+// lane).  The lanes keep their tuned launch shapes; their largest cases are
+// updated.
+void rebalance_bins(lica_batch *b, const double *lane_ms)
+{
+    const int L = b->lanes;
+    const int64_t N = b->ninst;
+    auto wof = [&](int64_t pos) { return b->par[(size_t)b->order_h[(size_t)pos]].n + 512.0; };
+    std::vector<double> t((size_t)N, 0.0);
+    double tot = 0.0;
+    for (int l = 0; l < L; l++) {
+        double w = 0.0;
+        for (int64_t p = b->bin_lo[l]; p < b->bin_lo[l + 1]; p++) w += wof(p);
+        const double rate = (w > 0.0) ? lane_ms[l] / w : 0.0;
+        for (int64_t p = b->bin_lo[l]; p < b->bin_lo[l + 1]; p++) { t[(size_t)p] = wof(p) * rate; tot += t[(size_t)p]; }
+    }
+    int lo[LICA_MAX_LANES + 1];
+    lo[0] = 0;
+    double acc = 0.0;
+    int64_t pos = 0;
+    for (int l = 1; l < L; l++) {
+        const double goal = tot * l / L;
+        while (pos < N - (L - l) && (pos < lo[l - 1] + 1 || acc + t[(size_t)pos] * 0.5 < goal)) {
+            acc += t[(size_t)pos];
+            pos++;
+        }
+        lo[l] = (int)pos;
+    }
+    lo[L] = (int)N;
+    for (int l = 0; l <= L; l++) b->bin_lo[l] = lo[l];
+    for (int l = 0; l < L; l++) b->lcfg[l].nmax = (N > 0) ? b->par[(size_t)b->order_h[(size_t)lo[l]]].n : 0;
+    b->cfg_version++;
 }
 #endif
 }  // namespace
@@ -1213,9 +1267,9 @@ int lica_batch_autotune(lica_batch *b, int64_t steps)
     if (rc) return rc;
     const bool cap_valid = b->cap_valid;
     // a trial: `steps` steps of every lane from the saved state, back to it after
-    auto trial = [&](double *ms) -> int {
+    auto trial = [&](double *ms, double *lane_ms = nullptr, int64_t nst = 0) -> int {
         b->cfg_version++;
-        int e = time_steps(b, steps, ms);
+        int e = time_steps(b, nst ? nst : steps, ms, lane_ms);
         int e2 = sc.restore(b->stream);
         return e ? e : e2;
     };
@@ -1274,6 +1328,32 @@ int lica_batch_autotune(lica_batch *b, int64_t steps)
                 if (!rc && t < best * 0.99) { best = t; keep = c; }
             }
             cfg.mg_lds = keep;
+        }
+    }
+    if (b->lane_split && b->lanes > 1) {
+        // the binned split from the lanes' measured times (the lanes run
+        // independently to the end of a run, which ends with the slowest);
+        // trials of twice the steps, the imbalance being a drift
+        double bt = 0.0;
+        for (int it = 0; it < 3 && !rc; it++) {
+            double t0 = 0.0, lt[LICA_MAX_LANES] = {};
+            rc = trial(&t0, lt, 2 * steps);
+            if (rc) break;
+            if (it == 0) bt = t0;
+            int lo[LICA_MAX_LANES + 1], nm[LICA_MAX_LANES];
+            for (int l = 0; l <= b->lanes; l++) lo[l] = b->bin_lo[l];
+            for (int l = 0; l < b->lanes; l++) nm[l] = b->lcfg[l].nmax;
+            rebalance_bins(b, lt);
+            double t = 0.0;
+            rc = trial(&t, nullptr, 2 * steps);
+            if (!rc && t < std::min(bt, t0) * 0.99) {
+                bt = t;
+                continue;
+            }
+            for (int l = 0; l <= b->lanes; l++) b->bin_lo[l] = lo[l];
+            for (int l = 0; l < b->lanes; l++) b->lcfg[l].nmax = nm[l];
+            b->cfg_version++;
+            break;
         }
     }
     b->cfg_version++;
