@@ -80,15 +80,20 @@ def kernel_bytes(case, kernel, mode):
     not counted).  Band-sized work (a few dozen refined planes) is counted
     with its own arrays."""
     n = int(case.n)
+    # the direct and Thomas modes pass the energy and momentum rows through
+    # four tridiagonal work vectors (written, read back); the multigrid modes
+    # move them from LDS into the solve's registers
+    trid = 4 * 16 if mode < 2 else 0
     per_cell = {
         # PSI_CN read, RK3ZO written
         "dt": 16,
         # W,PSI_ZN,PSI_CN,VVDZ,AKUV,CKUV read; T r/w; DF_Z,VF_Z,DF_C,PSI_Z,PSI_C,RHSPS
-        # written; four tridiagonal work vectors written and read back
-        "cont_energy": 6 * 8 + 16 + 6 * 8 + 4 * 16,
+        # written
+        "cont_energy": 6 * 8 + 16 + 6 * 8 + trid,
         # W,PSI_Z,PSI_C,RK3ZO r/w; PSI_CN,DF_Z,VF_Z,SSDZ,VVDZ,P,PSI_ZN,AKW,CKW read;
-        # IMASK (int); KCOU1/KCOU2 (int); tridiagonal work r/w; Poisson matrix written
-        "momentum": 4 * 16 + 9 * 8 + 4 + 8 + 4 * 16 + 4 * 8,
+        # IMASK (int); KCOU1/KCOU2 (int); Poisson matrix written; in the
+        # multigrid modes the solution through TG (written, read back)
+        "momentum": 4 * 16 + 9 * 8 + 4 + 8 + trid + 4 * 8 + (16 if mode >= 2 else 0),
         # W,P,WO r/w; PSI_Z,VVDZ,SSDZ read; KCOU1/KCOU2; WT written
         "ucalc": 3 * 16 + 3 * 8 + 8 + 8,
     }
@@ -99,13 +104,14 @@ def kernel_bytes(case, kernel, mode):
             return (16 + 8 + 8 + 16 + 8) * n
         if mode == 1:   # partitioned: a,b,c,g r/w, P written
             return (4 * 16 + 8) * n
-        # V-cycles: a,b,c,g read, P written, the two iterate buffers r/w,
-        # the coarse levels' setup rows (4 doubles) r/w, the weights of every
-        # level's pairs of rows (2 for the C row, 3 for the F row, in the
-        # cycle's precision) r/w, the warm-start guess read
+        # V-cycles: a,c,g read (b = -(a + c) is evaluated), P written, the
+        # two iterate buffers r/w, the coarse levels' setup rows (4 doubles)
+        # r/w, the weights of every level's pairs of rows (2 for the C row, 3
+        # for the F row, in the cycle's precision) r/w, the warm-start guess
+        # read
         tot = mg_rows_total(n)
         w = 8 if mode == 2 else 4
-        return (4 * 8 + 8 + 2 * 16) * n + 2 * 32 * (tot - n) + 2 * 5 * w * (tot - n) + (8 * n if mode in (4, 5) else 0)
+        return (3 * 8 + 8 + 2 * 16) * n + 2 * 32 * (tot - n) + 2 * 5 * w * (tot - n) + (8 * n if mode in (4, 5) else 0)
     # band kernels: 11 layers of <=16 planes, refined window of ~200 planes of
     # PSIF/SFGAS/geometry, NS window of ~100 cells of T/PSI_CN/geometry
     r = int(case.r)
