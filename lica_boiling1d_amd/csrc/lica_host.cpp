@@ -1331,23 +1331,28 @@ int lica_batch_autotune(lica_batch *b, int64_t steps)
         }
     }
     if (b->lane_split && b->lanes > 1) {
-        // the binned split from the lanes' measured times (the lanes run
-        // independently to the end of a run, which ends with the slowest);
-        // trials of twice the steps, the imbalance being a drift
-        double bt = 0.0;
+        // The binned split from the lanes' measured times: the lanes run
+        // independently to the end of a run, which ends with the slowest.
+        // The new cuts follow from measurements, so they are kept unless the
+        // batch's time grows by more than 0.5 %; trials of four times the
+        // steps (the imbalance is a drift), until the lanes' times are
+        // within 1 % of each other.
+        const int64_t rs = 4 * steps;
+        double t0 = 0.0, lt[LICA_MAX_LANES] = {};
+        rc = trial(&t0, lt, rs);
         for (int it = 0; it < 3 && !rc; it++) {
-            double t0 = 0.0, lt[LICA_MAX_LANES] = {};
-            rc = trial(&t0, lt, 2 * steps);
-            if (rc) break;
-            if (it == 0) bt = t0;
+            double lmax = 0.0, lmin = 1e300;
+            for (int l = 0; l < b->lanes; l++) { lmax = std::max(lmax, lt[l]); lmin = std::min(lmin, lt[l]); }
+            if (!(lmax > 0.0) || lmax - lmin < 0.01 * lmax) break;
             int lo[LICA_MAX_LANES + 1], nm[LICA_MAX_LANES];
             for (int l = 0; l <= b->lanes; l++) lo[l] = b->bin_lo[l];
             for (int l = 0; l < b->lanes; l++) nm[l] = b->lcfg[l].nmax;
             rebalance_bins(b, lt);
-            double t = 0.0;
-            rc = trial(&t, nullptr, 2 * steps);
-            if (!rc && t < std::min(bt, t0) * 0.99) {
-                bt = t;
+            double t1 = 0.0, lt1[LICA_MAX_LANES] = {};
+            rc = trial(&t1, lt1, rs);
+            if (!rc && t1 <= t0 * 1.005) {
+                t0 = t1;
+                for (int l = 0; l < b->lanes; l++) lt[l] = lt1[l];
                 continue;
             }
             for (int l = 0; l <= b->lanes; l++) b->bin_lo[l] = lo[l];

@@ -76,17 +76,15 @@ __global__ void __launch_bounds__(1024) k_cont_energy(BatchDev bd)
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) k_temp_grad(BatchDev bd)
 {
     __shared__ TgradSh sh;
-    if (!bd.lead) {
-        body_temp_grad(bd, bd.order[blockIdx.x], sh);
-        return;
-    }
-    const int pos = tg_lead_pos(bd, (int)blockIdx.x);
+    const int pos = bd.lead ? tg_lead_pos(bd, (int)blockIdx.x) : (int)blockIdx.x;
     if (pos < 0) return;
     const int inst = bd.order[pos];
     if (threadIdx.x == 0) { sh.xit[0] = -1.0; sh.xit[1] = -1.0; }
     body_temp_grad(bd, inst, sh);
-    __syncthreads();
-    if (threadIdx.x == 0) tg_lead_record(bd, inst, pos, sh);
+    if (bd.lead) {
+        __syncthreads();
+        if (threadIdx.x == 0) tg_lead_record(bd, inst, pos, sh);
+    }
 }
 // the momentum stage of the Thomas and multigrid modes (no refinement in
 // its solve)
