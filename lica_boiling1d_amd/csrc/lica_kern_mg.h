@@ -86,7 +86,7 @@ HOSTDEV long long mg_rows_total(int n) { return mg_rows_before(n, mg_levels(n));
 // row J of the next level, tot - n pairs in all:
 //   4 (tot - n)  FP64 setup rows (a, c, s, 1/b) of levels 1..L-1
 //   2 (tot - n)  C weights (w0, w1) of every pair (FP64, or FP32 in its first half)
-//   4 (tot - n)  F weights (w0, w1, w2, pad) of every pair (likewise)
+//   4 (tot - n)  room for the F weights (w0, w1, w2) of every pair (likewise)
 //   2 n          the iterate and its double buffer
 //   n + tot      the cycle's vectors kept outside LDS (room for FP64)
 // (the C and F weights are apart so that the downward pass, which reads
@@ -147,7 +147,7 @@ struct MgCw {
 };
 template <class T>
 struct MgFw {
-    T w0, w1, w2, pad;
+    T w0, w1, w2;
 };
 #else
 struct alignas(32) MgSet {
@@ -157,9 +157,11 @@ template <class T>
 struct alignas(2 * sizeof(T)) MgCw {
     T w0, w1;
 };
+// three weights, no padding: the upward pass reads 12 (FP32) or 24 bytes
+// per pair instead of 16 or 32
 template <class T>
-struct alignas(4 * sizeof(T)) MgFw {
-    T w0, w1, w2, pad;
+struct MgFw {
+    T w0, w1, w2;
 };
 #endif
 
@@ -270,7 +272,7 @@ DEV MgSet mg_pair(RowF row, int i, int nl, MgCw<T> &cw, MgFw<T> &fw, bool &has_f
         cv = -q * Up.c;
         sv = sv - q * Up.s;
         w1 = -q;
-        fw.w0 = (T)Up.ib; fw.w1 = (T)(-Up.a * Up.ib); fw.w2 = (T)(-Up.c * Up.ib); fw.pad = (T)0;
+        fw.w0 = (T)Up.ib; fw.w1 = (T)(-Up.a * Up.ib); fw.w2 = (T)(-Up.c * Up.ib);
     }
     cw.w0 = (T)w0; cw.w1 = (T)w1;
     MgSet C;
@@ -392,10 +394,10 @@ DEV void mg_cycle(const MgView &V, MgSh &sh)
 
 DEV double mg_mag(double v) { return isfinite(v) ? fabs(v) : INFINITY; }
 
-// x_new = x + e (e absent on the first pass) into xb, r = g - A x_new in
-// double-double into level 0's r; mx = {max|r|, max|x_new|, max|e|,
-// max|g|, ||A||_inf} (non-finite values as infinity; the last two on the
-// first pass only)
+// x_new = x + e (e absent on the first pass; xa null: x = 0) into xb,
+// r = g - A x_new in double-double into level 0's r; mx = {max|r|,
+// max|x_new|, max|e|, max|g|, ||A||_inf} (non-finite values as infinity; the
+// last two on the first pass only)
 template <class T>
 DEV void mg_correct(const double *a, const double *b, const double *c, const double *g, const MgView &V,
                     const double *xa, double *xb, bool first, MgSh &sh, double *mx)
@@ -403,7 +405,8 @@ DEV void mg_correct(const double *a, const double *b, const double *c, const dou
     const int n = V.n;
     T *r0 = mg_r0<T>(V, sh);
     const T *e0 = mg_e0<T>(V, sh);
-    auto xn = [&](int k) -> double { return first ? xa[k] : xa[k] + (double)e0[k]; };
+    auto xv = [&](int k) -> double { return xa ? xa[k] : 0.0; };
+    auto xn = [&](int k) -> double { return first ? xv(k) : xv(k) + (double)e0[k]; };
     red_maxN<MG_NRED>(0, n, [&](int i, double *v) {
         const double ai = (i == 0) ? 0.0 : a[i + 1], bi = mg_b(a, b, c, i + 1), ci = (i == n - 1) ? 0.0 : c[i + 1];
         const double x = xn(i), xl = (i > 0) ? xn(i - 1) : 0.0, xr = (i < n - 1) ? xn(i + 1) : 0.0;
@@ -420,12 +423,12 @@ DEV void mg_correct(const double *a, const double *b, const double *c, const dou
 
 // The finest level's pass of the setup together with the first residual
 // (one read of a, b, c, g and the starting point; needs L >= 2): level 1's
-// rows and level 0's pair weights, x = x0 (rows 1..n; zero when x0 is null)
-// into xb, r = g - A x in double-double into level 0's r, and the maxima of
-// mg_correct's first pass
+// rows and level 0's pair weights, r = g - A x for x = x0 (rows 1..n; zero
+// when x0 is null) in double-double into level 0's r, and the maxima of
+// mg_correct's first pass (the first correction reads x0 where it is)
 template <class T>
 DEV void mg_first(const double *a, const double *b, const double *c, const double *g, const MgView &V,
-                  const double *x0, double *xb, MgSh &sh, double *mx)
+                  const double *x0, MgSh &sh, double *mx)
 {
     const int n = V.n, nc = sh.lv_n[1];
     MgSet *crs = V.set;
@@ -445,7 +448,6 @@ DEV void mg_first(const double *a, const double *b, const double *c, const doubl
             const double ai = (k == 0) ? 0.0 : a[k + 1], bi = mg_b(a, b, c, k + 1), ci = (k == n - 1) ? 0.0 : c[k + 1];
             const double x = xv(k), xl = (k > 0) ? xv(k - 1) : 0.0, xr = (k < n - 1) ? xv(k + 1) : 0.0;
             const double r = dd_residual(g[k + 1], ai, xl, bi, x, ci, xr);
-            xb[k] = x;
             r0[k] = (T)r;
             v[0] = fmax(v[0], mg_mag(r));
             v[1] = fmax(v[1], mg_mag(x));
@@ -465,41 +467,47 @@ template <class T>
 DEV int mg_run(const double *a, const double *b, const double *c, const double *g, const MgView &V, double rtol,
                int maxc, const double *x0, MgSh &sh, double **cur, double *res_out, bool *nonfinite_rows)
 {
-    const int n = V.n;
     double *xa = V.x0, *xb = V.x1;
     double mx[MG_NRED];
+    // the starting point, rows 0..n-1 (null: zero): the first correction
+    // reads it where it is
+    const double *xs = x0 ? x0 + 1 : nullptr;
     if (sh.nlev >= 2) {
-        mg_first<T>(a, b, c, g, V, x0, xb, sh, mx);
+        mg_first<T>(a, b, c, g, V, x0, sh, mx);
         mg_setup_rest<T>(a, b, c, V, sh);
     } else {
         mg_setup_rest<T>(a, b, c, V, sh);
-        PFOR(i, 0, n) { xa[i] = x0 ? x0[i + 1] : 0.0; }
-        PFOR_END
-        mg_correct<T>(a, b, c, g, V, xa, xb, true, sh, mx);
+        mg_correct<T>(a, b, c, g, V, xs, xb, true, sh, mx);
     }
     const double gmax = mx[3], anorm = mx[4];
     *nonfinite_rows = !(gmax < INFINITY) || !(anorm < INFINITY);
     if (*nonfinite_rows) { *cur = xb; *res_out = NAN; return 0; }
     if (x0 && !(mx[1] < INFINITY && mx[0] < INFINITY)) {
         // a non-finite guess is dropped: start from zero
-        PFOR(i, 0, n) { xa[i] = 0.0; }
-        PFOR_END
-        mg_correct<T>(a, b, c, g, V, xa, xb, true, sh, mx);
+        xs = nullptr;
+        mg_correct<T>(a, b, c, g, V, xs, xb, true, sh, mx);
     }
-    { double *t = xa; xa = xb; xb = t; }
+    // the corrections read the iterate xc and write the other buffer
+    const double *xc = xs;
     double res = mg_berr(mx[0], mx[1], gmax, anorm);
     int cyc = 0;
     bool conv = false;
     while (cyc < maxc) {
         mg_cycle<T>(V, sh);
-        mg_correct<T>(a, b, c, g, V, xa, xb, false, sh, mx);
+        mg_correct<T>(a, b, c, g, V, xc, xa, false, sh, mx);
+        xc = xa;
         { double *t = xa; xa = xb; xb = t; }
         cyc++;
         res = mg_berr(mx[0], mx[1], gmax, anorm);
         if (!(mx[1] < INFINITY)) break;
         if (res <= rtol && mx[2] <= rtol * mx[1]) { conv = true; break; }
     }
-    *cur = xa;
+    if (cyc == 0) {
+        // no cycle allowed: the starting point is the result
+        PFOR(i, 0, V.n) { xb[i] = xs ? xs[i] : 0.0; }
+        PFOR_END
+    }
+    *cur = xb;   // the last correction's buffer
     *res_out = res;
     return conv ? cyc : -1;
 }
