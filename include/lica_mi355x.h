@@ -216,11 +216,15 @@ int lica_batch_set_lane_split(lica_batch *b, int64_t split);
  * UCALC, DETERMINE_DT, in that order) and for each lane, tries 64..1024
  * threads per workgroup, then multigrid LDS sizes per lane, each trial
  * `steps` (1..16) steps of every lane at once as in a run, and keeps a
- * change that gains more than 1 %.  The batch's device state is copied
- * before and put back after every trial, so the run continues from where
- * it was; no result depends on the shapes chosen (maxima only, chunks of
- * the partitioned solves set by the row count).  Call after
- * lica_batch_set_lanes / set_lane_split / set_poisson, which drop it.      */
+ * change that gains more than 1 %.  With the binned split it then moves
+ * the split's cuts so that the lanes' measured times (each from its first
+ * launch to its last, trials of 4 x `steps` steps) come out equal, keeping
+ * a new split unless the batch gets more than 0.5 % slower.  The batch's
+ * device state is copied before and put back after every trial, so the run
+ * continues from where it was; no result depends on the shapes or the split
+ * chosen (maxima only, chunks of the partitioned solves set by the row
+ * count, cases independent).  Call after lica_batch_set_lanes /
+ * set_lane_split / set_poisson, which drop it.                             */
 int lica_batch_autotune(lica_batch *b, int64_t steps);
 /* per lane (3 + LICA_KERNEL_CLASSES values): cases, largest n, multigrid LDS
  * bytes, then threads per workgroup of every kernel class in the library's
