@@ -346,12 +346,130 @@ def calibrate_copy(mib):
     print(json.dumps({"calibrate_copy_bytes": n * 8, "dispatches": 5}), flush=True)
 
 
+# algorithmic bytes per row of a line: TRDIAG1 reads A, B, C, R and writes UU
+# and the multiplier on the way down, reads both and writes UU on the way up;
+# TRDIAG1P reads A, B, C, F and writes Q, S, F down, reads Q, S, F and writes
+# S, QE on the backward pass, reads S, QE and writes F at the end; PTDIAG1A
+# reads A..F and writes O, Q, R, then reads O, Q, R and writes UU
+LINE_BYTES = {"trdiag1": 72, "trdiag1p": 120, "ptdiag1a": 104}
+
+
+def bench_lines(args):
+    """The line solvers of the 3-D code on device arrays: one plane of
+    --line-count lines x --line-rows rows (column-major, lines contiguous),
+    each solver launched W + K times on the null stream and timed with HIP
+    events around each launch; the FP64 oracle on 4096 of the lines as the
+    CPU baseline.  Device memory through the HIP runtime the library uses."""
+    import ctypes
+    import numpy as np
+    from lica_boiling1d_amd import _abi as abi
+    lib = abi.load()
+    rt = ctypes.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"))
+    rt.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    rt.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    rt.hipFree.argtypes = [ctypes.c_void_p]
+    rt.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+    rt.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    rt.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+    rt.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+    rt.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def say(m):
+        print("bench: " + m, file=sys.stderr, flush=True)
+
+    m1, m2 = args.line_count, args.line_rows
+    steps = args.steps if args.steps is not None else 20
+    warm = args.warmup if args.warmup is not None else 3
+    rng = np.random.default_rng(20261203)
+    held = []
+
+    def put(x):
+        x = np.asfortranarray(x, dtype=np.float64)
+        p = ctypes.c_void_p()
+        if rt.hipMalloc(ctypes.byref(p), x.nbytes) != 0 or rt.hipMemcpy(p, x.ctypes.data, x.nbytes, 1) != 0:
+            raise RuntimeError("device memory")
+        held.append(p)
+        return p
+
+    def rnd(lo=-1.0, hi=1.0):
+        return rng.uniform(lo, hi, (m1, m2))
+
+    r_h = rnd()
+    a, c, r = put(rnd()), put(rnd()), put(r_h)
+    b, d, e, a5 = put(rnd(3.0, 4.0)), put(rnd()), put(rnd()), put(rnd(5.0, 6.0))
+    zero = np.zeros((m1, m2))
+    uu, o, q, rr, f = (put(zero) for _ in range(5))
+    work = put(np.zeros((3 * m1, m2)))
+    say(f"line solvers on {m1} lines x {m2} rows")
+    calls = {
+        "trdiag1": lambda: lib.lica_trdiag_lines_dev(a, b, c, r, uu, work, m1, m2, 1, m2, 1, m1, 0, None),
+        "trdiag1p": lambda: lib.lica_trdiag_periodic_lines_dev(a, b, c, f, work, m1, m2, 1, m2, 1, m1, None),
+        "ptdiag1a": lambda: lib.lica_ptdiag_lines_dev(a5, b, c, d, e, r, uu, m1, m2, 1, m1, 1, m2, o, q, rr, None),
+    }
+    ev = [ctypes.c_void_p(), ctypes.c_void_p()]
+    for x in ev:
+        rt.hipEventCreate(ctypes.byref(x))
+    res = {}
+    for name, call in calls.items():
+        ms = 0.0
+        for it in range(warm + steps):
+            if name == "trdiag1p":
+                # the periodic solve overwrites its right-hand side
+                rt.hipMemcpy(f, r, m1 * m2 * 8, 3)
+            rt.hipEventRecord(ev[0], None)
+            if call() != 0:
+                raise RuntimeError(name)
+            rt.hipEventRecord(ev[1], None)
+            rt.hipEventSynchronize(ev[1])
+            t = ctypes.c_float()
+            rt.hipEventElapsedTime(ctypes.byref(t), ev[0], ev[1])
+            if it >= warm:
+                ms += t.value
+        per = ms / steps
+        ach = LINE_BYTES[name] * m1 * m2 / (per * 1e-3) / 1e9
+        res[name] = {"ms_per_launch": per, "rows_per_s": m1 * m2 / (per * 1e-3),
+                     "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                                  "bytes_per_row": LINE_BYTES[name]}}
+        say(f"{name}: {per:.3f} ms per launch, {ach:.0f} GB/s")
+    for x in ev:
+        rt.hipEventDestroy(x)
+    for p in held:
+        rt.hipFree(p)
+    # the oracle on a sample of the plane's lines
+    from oracle import binding as O
+    ms_n = min(4096, m1)
+    A, B, C = (rng.uniform(-1.0, 1.0, (ms_n, m2)) for _ in range(3))
+    B += 4.0
+    R = r_h[:ms_n]
+    t0 = time.perf_counter()
+    reps = 0
+    while time.perf_counter() - t0 < 2.0:
+        O.trdiag_lines(A, B, C, R, 1, m2, 1, ms_n)
+        reps += 1
+    cpu = ms_n * m2 * reps / (time.perf_counter() - t0)
+    t1 = res["trdiag1"]
+    out = {"metric": "tridiagonal rows solved per second (TRDIAG1 lines, 1 GPU)", "value": t1["rows_per_s"],
+           "unit": "rows/s", "n_gpus": 1, "steps": steps, "warmup": warm, "ms_per_step": t1["ms_per_launch"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "line solvers", "lines": m1, "rows": m2},
+           "roofline": dict(t1["roofline"], kernel="k_trdiag_lines"),
+           "cpu_baseline": {"value": cpu, "unit": "rows/s", "cores": 1, "kind": "port",
+                            "sample": f"TRDIAG1 on {ms_n} lines x {m2} rows by the FP64 oracle, repeated for 2 s on "
+                                      f"one core (NumPy copies in and out included)"},
+           "periodic": res["trdiag1p"], "pentadiagonal": res["ptdiag1a"]}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", choices=["large", "batch", "deck"], default="batch")
+    ap.add_argument("--workload", choices=["large", "batch", "deck", "lines"], default="batch",
+                    help="lines: the 3-D code's line solvers (TRDIAG1, TRDIAG1P, PTDIAG1A) on one large plane")
+    ap.add_argument("--line-count", type=int, default=262144, help="lines of the plane (--workload lines)")
+    ap.add_argument("--line-rows", type=int, default=128, help="rows per line (--workload lines)")
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--mode", choices=["learned", "projected", "direct", "thomas", "mg64", "mixed"], default="learned")
     ap.add_argument("--rtol", type=float, default=1e-10)
@@ -383,6 +501,9 @@ def main():
         sys.exit(2)
     if args.steps is None:
         args.steps = 20 if args.workload == "batch" else 200
+    if args.workload == "lines":
+        bench_lines(args)
+        return
     if args.warmup is None:
         args.warmup = 5
 

@@ -305,6 +305,55 @@ int lica_f_temp_grad(lica_batch *ctx, const double *psi_cn);
 int lica_f_state(lica_batch *ctx, double *w, double *p, double *t, double *psi_zn, double *psi_cn,
                  double *vol_tot_ori, double *time, double *dt);
 
+/* Line solvers of the 3-D code (no live caller on the 1-D column).  Arrays
+ * are column-major m1 x m2 doubles (element (i, j), 1-based, at
+ * (i-1) + (j-1)*m1): the reference's A(M2,M1) dummies of TRDIAG1 have
+ * m1 = M2, m2 = M1, those of TRDIAG2 m1 = M1, m2 = M2, and so on.  The
+ * systems run along the second index, one per line of the first; results
+ * are the reference's serial loops' bit for bit.  Host arrays: the call
+ * copies them to the GPU and back and returns when done; elements outside
+ * the ranges are left as they were.
+ *
+ * TRDIAG1 (1d.f:3721-3749), TRDIAG2 (3752-3780), TRDIAG3 (3783-3812):
+ * A u(j-1) + B u(j) + C u(j+1) = R for j = l1..l2 on lines i = ll1..ll2
+ * into UU; TRDIAG3 also leaves UU in R(i, l1..l2-1), as the reference does. */
+int lica_trdiag1(const double *a, const double *b, const double *c, const double *r, double *uu, int64_t m1,
+                 int64_t m2, int64_t l1, int64_t l2, int64_t ll1, int64_t ll2);
+int lica_trdiag2(const double *a, const double *b, const double *c, const double *r, double *uu, int64_t m1,
+                 int64_t m2, int64_t l1, int64_t l2, int64_t ll1, int64_t ll2);
+int lica_trdiag3(const double *a, const double *b, const double *c, double *r, double *uu, int64_t m1,
+                 int64_t m2, int64_t l1, int64_t l2, int64_t ll1, int64_t ll2);
+/* TRDIAG1P (1d.f:3815-3869), TRDIAG2P (3872-3926), TRDIAG3P (3929-3985):
+ * periodic systems along j = j1..j2 (j2 > j1; A(j1) couples to u(j2), C(j2)
+ * to u(j1)) on lines k = l1..l2; F holds the right-hand side in and the
+ * solution out. */
+int lica_trdiag1p(const double *a, const double *b, const double *c, double *f, int64_t m1, int64_t m2, int64_t j1,
+                  int64_t j2, int64_t l1, int64_t l2);
+int lica_trdiag2p(const double *a, const double *b, const double *c, double *f, int64_t m1, int64_t m2, int64_t j1,
+                  int64_t j2, int64_t l1, int64_t l2);
+int lica_trdiag3p(const double *a, const double *b, const double *c, double *f, int64_t m1, int64_t m2, int64_t j1,
+                  int64_t j2, int64_t l1, int64_t l2);
+/* PTDIAG1A (1d.f:3988-4034): E u(j-2) + D u(j-1) + A u(j) + B u(j+1) +
+ * C u(j+2) = F for j = j1..jn (jn > j1) on lines i = i1..in into UU; O, Q,
+ * R receive the elimination's coefficients, as in the reference. */
+int lica_ptdiag1a(const double *a, const double *b, const double *c, const double *d, const double *e,
+                  const double *f, double *uu, int64_t m1, int64_t m2, int64_t i1, int64_t in, int64_t j1,
+                  int64_t jn, double *o, double *q, double *r);
+/* The same three solvers on DEVICE arrays, asynchronous on `stream` (a
+ * hipStream_t; NULL for the null stream), for data that stays on the GPU.
+ * work: m1 x m2 doubles (TRDIAG's multipliers) or 3 x m1 x m2 (the periodic
+ * solve's), owned by the caller; nothing is allocated.  overwrite_r = 1 is
+ * TRDIAG3.  No counterpart in the reference beyond the routines above. */
+int lica_trdiag_lines_dev(const double *a, const double *b, const double *c, double *r, double *uu, double *work,
+                          int64_t m1, int64_t m2, int64_t l1, int64_t l2, int64_t ll1, int64_t ll2,
+                          int64_t overwrite_r, void *stream);
+int lica_trdiag_periodic_lines_dev(const double *a, const double *b, const double *c, double *f, double *work,
+                                   int64_t m1, int64_t m2, int64_t j1, int64_t j2, int64_t l1, int64_t l2,
+                                   void *stream);
+int lica_ptdiag_lines_dev(const double *a, const double *b, const double *c, const double *d, const double *e,
+                          const double *f, double *uu, int64_t m1, int64_t m2, int64_t i1, int64_t in, int64_t j1,
+                          int64_t jn, double *o, double *q, double *r, void *stream);
+
 /* batched POISSON system of 1d.f:3535-3564: rows k=1..n of each of nsys
  * systems (a(1) is ignored as in the reference), contiguous per system.  */
 int lica_poisson(int64_t nsys, int64_t n, const double *a, const double *b, const double *c,

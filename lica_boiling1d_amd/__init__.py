@@ -7,11 +7,11 @@ CPU implementation: without the library or a GPU every entry point raises.
 """
 from ._abi import (CaseDesc, LicaError, FIELDS, SCALARS, POISSON_THOMAS, POISSON_DIRECT,
                    POISSON_MG64, POISSON_MIXED, POISSON_LEARNED, POISSON_PROJECTED, load, status_string)
-from .solver import Batch, Context, ReferenceSeams, ref_array, parse_deck, poisson
+from .solver import Batch, Context, ReferenceSeams, ref_array, parse_deck, poisson, trdiag, trdiag_periodic, ptdiag1a
 from .workloads import deck_case, single_large_case, batched_cases
 from .learned import Net, collect_pairs
 
 __all__ = ["CaseDesc", "LicaError", "FIELDS", "SCALARS", "POISSON_THOMAS", "POISSON_DIRECT",
            "POISSON_MG64", "POISSON_MIXED", "POISSON_LEARNED", "POISSON_PROJECTED", "load", "status_string",
-           "Batch", "Context", "ReferenceSeams", "ref_array", "parse_deck", "poisson", "deck_case", "single_large_case",
+           "Batch", "Context", "ReferenceSeams", "ref_array", "parse_deck", "poisson", "trdiag", "trdiag_periodic", "ptdiag1a", "deck_case", "single_large_case",
            "batched_cases", "Net", "collect_pairs"]

@@ -113,6 +113,16 @@ def _declare(lib):
         "lica_f_state": (ctypes.c_int, [V] + [P_dbl] * 8),
         # device addresses in and out, a hipStream_t (or None)
         "lica_poisson_noalloc": (ctypes.c_int, [c_i64, c_i64, V, V, V, V, V, c_i64, c_dbl, c_i64, P_i64, V]),
+        "lica_trdiag1": (ctypes.c_int, [P_dbl] * 5 + [c_i64] * 6),
+        "lica_trdiag2": (ctypes.c_int, [P_dbl] * 5 + [c_i64] * 6),
+        "lica_trdiag3": (ctypes.c_int, [P_dbl] * 5 + [c_i64] * 6),
+        "lica_trdiag1p": (ctypes.c_int, [P_dbl] * 4 + [c_i64] * 6),
+        "lica_trdiag2p": (ctypes.c_int, [P_dbl] * 4 + [c_i64] * 6),
+        "lica_trdiag3p": (ctypes.c_int, [P_dbl] * 4 + [c_i64] * 6),
+        "lica_ptdiag1a": (ctypes.c_int, [P_dbl] * 7 + [c_i64] * 6 + [P_dbl] * 3),
+        "lica_trdiag_lines_dev": (ctypes.c_int, [V] * 6 + [c_i64] * 7 + [V]),
+        "lica_trdiag_periodic_lines_dev": (ctypes.c_int, [V] * 5 + [c_i64] * 6 + [V]),
+        "lica_ptdiag_lines_dev": (ctypes.c_int, [V] * 7 + [c_i64] * 6 + [V] * 3 + [V]),
         "lica_status_string": (ctypes.c_char_p, [ctypes.c_int]),
         "lica_batch_set_net": (ctypes.c_int, [V, V]),
         "lica_batch_set_capture": (ctypes.c_int, [V, c_i64]),

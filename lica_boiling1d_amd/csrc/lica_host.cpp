@@ -1998,6 +1998,169 @@ int lica_f_temp_grad(lica_batch *ctx, const double *psi_cn)
 
 }  // extern "C"
 
+// ---------------------------------------------------------------------
+// Line solvers of the 3-D code (1d.f:3721-4034), column-major m1 x m2
+// arrays with 1-based ranges as in the reference
+namespace {
+bool line_shape_ok(int64_t m1, int64_t m2)
+{
+    return m1 >= 1 && m2 >= 1 && m1 <= INT32_MAX && m2 <= INT32_MAX && (double)m1 * (double)m2 * 8.0 < 1.0e12;
+}
+bool range_ok(int64_t lo, int64_t hi, int64_t m, int64_t min_len)
+{
+    return lo >= 1 && hi <= m && hi - lo + 1 >= min_len;
+}
+
+// device copies of host arrays (freed with the holder)
+struct LineBufs {
+    std::vector<double *> p;
+    size_t bytes = 0;
+    ~LineBufs()
+    {
+        for (double *x : p) dfree(x);
+    }
+    double *add(const double *h, hipStream_t st, int *rc)
+    {
+        double *d = nullptr;
+        if (*rc) return nullptr;
+        *rc = dmalloc((void **)&d, bytes);
+        if (*rc) return nullptr;
+        p.push_back(d);
+        if (h) *rc = h2d(d, h, bytes, st);
+        return d;
+    }
+};
+
+int trdiag_host(const double *a, const double *b, const double *c, double *r, double *uu, int64_t m1, int64_t m2,
+                int64_t l1, int64_t l2, int64_t ll1, int64_t ll2, int overwrite_r)
+{
+    if (!a || !b || !c || !r || !uu) return LICA_BAD_ARG;
+    if (!line_shape_ok(m1, m2) || !range_ok(l1, l2, m2, 1) || !range_ok(ll1, ll2, m1, 1)) return LICA_BAD_ARG;
+    if (!have_device()) return LICA_NO_DEVICE;
+    hipStream_t st = nullptr;
+    LineBufs B;
+    B.bytes = sizeof(double) * (size_t)m1 * (size_t)m2;
+    int rc = 0;
+    double *da = B.add(a, st, &rc), *db = B.add(b, st, &rc), *dc = B.add(c, st, &rc), *dr = B.add(r, st, &rc);
+    double *du = B.add(uu, st, &rc), *dg = B.add(nullptr, st, &rc);
+    if (!rc && lica_launch_trdiag_lines(da, db, dc, dr, du, dg, m1, (int)l1, (int)l2, (int)ll1, (int)ll2, overwrite_r,
+                                        (void *)st))
+        rc = LICA_HIP_ERROR;
+    rc = rc ? rc : dsync(st);
+    rc = rc ? rc : d2h(uu, du, B.bytes, st);
+    if (overwrite_r) rc = rc ? rc : d2h(r, dr, B.bytes, st);
+    return rc;
+}
+
+int trdiag_periodic_host(const double *a, const double *b, const double *c, double *f, int64_t m1, int64_t m2,
+                         int64_t j1, int64_t j2, int64_t l1, int64_t l2)
+{
+    if (!a || !b || !c || !f) return LICA_BAD_ARG;
+    if (!line_shape_ok(m1, m2) || !range_ok(j1, j2, m2, 2) || !range_ok(l1, l2, m1, 1)) return LICA_BAD_ARG;
+    if (!have_device()) return LICA_NO_DEVICE;
+    hipStream_t st = nullptr;
+    LineBufs B;
+    B.bytes = sizeof(double) * (size_t)m1 * (size_t)m2;
+    int rc = 0;
+    double *da = B.add(a, st, &rc), *db = B.add(b, st, &rc), *dc = B.add(c, st, &rc), *df = B.add(f, st, &rc);
+    double *dq = B.add(nullptr, st, &rc), *ds = B.add(nullptr, st, &rc), *de = B.add(nullptr, st, &rc);
+    if (!rc && lica_launch_trdiag_periodic_lines(da, db, dc, df, dq, ds, de, m1, (int)j1, (int)j2, (int)l1, (int)l2,
+                                                 (void *)st))
+        rc = LICA_HIP_ERROR;
+    rc = rc ? rc : dsync(st);
+    rc = rc ? rc : d2h(f, df, B.bytes, st);
+    return rc;
+}
+}  // namespace
+
+extern "C" int lica_trdiag1(const double *a, const double *b, const double *c, const double *r, double *uu, int64_t m1,
+                            int64_t m2, int64_t l1, int64_t l2, int64_t ll1, int64_t ll2)
+{
+    return trdiag_host(a, b, c, const_cast<double *>(r), uu, m1, m2, l1, l2, ll1, ll2, 0);
+}
+extern "C" int lica_trdiag2(const double *a, const double *b, const double *c, const double *r, double *uu, int64_t m1,
+                            int64_t m2, int64_t l1, int64_t l2, int64_t ll1, int64_t ll2)
+{
+    return trdiag_host(a, b, c, const_cast<double *>(r), uu, m1, m2, l1, l2, ll1, ll2, 0);
+}
+extern "C" int lica_trdiag3(const double *a, const double *b, const double *c, double *r, double *uu, int64_t m1,
+                            int64_t m2, int64_t l1, int64_t l2, int64_t ll1, int64_t ll2)
+{
+    return trdiag_host(a, b, c, r, uu, m1, m2, l1, l2, ll1, ll2, 1);
+}
+extern "C" int lica_trdiag1p(const double *a, const double *b, const double *c, double *f, int64_t m1, int64_t m2,
+                             int64_t j1, int64_t j2, int64_t l1, int64_t l2)
+{
+    return trdiag_periodic_host(a, b, c, f, m1, m2, j1, j2, l1, l2);
+}
+extern "C" int lica_trdiag2p(const double *a, const double *b, const double *c, double *f, int64_t m1, int64_t m2,
+                             int64_t j1, int64_t j2, int64_t l1, int64_t l2)
+{
+    return trdiag_periodic_host(a, b, c, f, m1, m2, j1, j2, l1, l2);
+}
+extern "C" int lica_trdiag3p(const double *a, const double *b, const double *c, double *f, int64_t m1, int64_t m2,
+                             int64_t j1, int64_t j2, int64_t l1, int64_t l2)
+{
+    return trdiag_periodic_host(a, b, c, f, m1, m2, j1, j2, l1, l2);
+}
+extern "C" int lica_ptdiag1a(const double *a, const double *b, const double *c, const double *d, const double *e,
+                             const double *f, double *uu, int64_t m1, int64_t m2, int64_t i1, int64_t in, int64_t j1,
+                             int64_t jn, double *o, double *q, double *r)
+{
+    if (!a || !b || !c || !d || !e || !f || !uu || !o || !q || !r) return LICA_BAD_ARG;
+    if (!line_shape_ok(m1, m2) || !range_ok(j1, jn, m2, 2) || !range_ok(i1, in, m1, 1)) return LICA_BAD_ARG;
+    if (!have_device()) return LICA_NO_DEVICE;
+    hipStream_t st = nullptr;
+    LineBufs B;
+    B.bytes = sizeof(double) * (size_t)m1 * (size_t)m2;
+    int rc = 0;
+    double *da = B.add(a, st, &rc), *db = B.add(b, st, &rc), *dc = B.add(c, st, &rc), *dd = B.add(d, st, &rc);
+    double *de = B.add(e, st, &rc), *df = B.add(f, st, &rc), *du = B.add(uu, st, &rc), *dO = B.add(o, st, &rc);
+    double *dq = B.add(q, st, &rc), *dr = B.add(r, st, &rc);
+    if (!rc && lica_launch_ptdiag_lines(da, db, dc, dd, de, df, du, dO, dq, dr, m1, (int)i1, (int)in, (int)j1, (int)jn,
+                                        (void *)st))
+        rc = LICA_HIP_ERROR;
+    rc = rc ? rc : dsync(st);
+    rc = rc ? rc : d2h(uu, du, B.bytes, st);
+    rc = rc ? rc : d2h(o, dO, B.bytes, st);
+    rc = rc ? rc : d2h(q, dq, B.bytes, st);
+    rc = rc ? rc : d2h(r, dr, B.bytes, st);
+    return rc;
+}
+
+extern "C" int lica_trdiag_lines_dev(const double *a, const double *b, const double *c, double *r, double *uu,
+                                     double *work, int64_t m1, int64_t m2, int64_t l1, int64_t l2, int64_t ll1,
+                                     int64_t ll2, int64_t overwrite_r, void *stream)
+{
+    if (!a || !b || !c || !r || !uu || !work || (overwrite_r != 0 && overwrite_r != 1)) return LICA_BAD_ARG;
+    if (!line_shape_ok(m1, m2) || !range_ok(l1, l2, m2, 1) || !range_ok(ll1, ll2, m1, 1)) return LICA_BAD_ARG;
+    if (!have_device()) return LICA_NO_DEVICE;
+    return lica_launch_trdiag_lines(a, b, c, r, uu, work, m1, (int)l1, (int)l2, (int)ll1, (int)ll2, (int)overwrite_r,
+                                    stream) ? LICA_HIP_ERROR : LICA_OK;
+}
+extern "C" int lica_trdiag_periodic_lines_dev(const double *a, const double *b, const double *c, double *f,
+                                              double *work, int64_t m1, int64_t m2, int64_t j1, int64_t j2,
+                                              int64_t l1, int64_t l2, void *stream)
+{
+    if (!a || !b || !c || !f || !work) return LICA_BAD_ARG;
+    if (!line_shape_ok(m1, m2) || !range_ok(j1, j2, m2, 2) || !range_ok(l1, l2, m1, 1)) return LICA_BAD_ARG;
+    if (!have_device()) return LICA_NO_DEVICE;
+    const size_t sz = (size_t)m1 * (size_t)m2;
+    return lica_launch_trdiag_periodic_lines(a, b, c, f, work, work + sz, work + 2 * sz, m1, (int)j1, (int)j2,
+                                             (int)l1, (int)l2, stream) ? LICA_HIP_ERROR : LICA_OK;
+}
+extern "C" int lica_ptdiag_lines_dev(const double *a, const double *b, const double *c, const double *d,
+                                     const double *e, const double *f, double *uu, int64_t m1, int64_t m2, int64_t i1,
+                                     int64_t in, int64_t j1, int64_t jn, double *o, double *q, double *r,
+                                     void *stream)
+{
+    if (!a || !b || !c || !d || !e || !f || !uu || !o || !q || !r) return LICA_BAD_ARG;
+    if (!line_shape_ok(m1, m2) || !range_ok(j1, jn, m2, 2) || !range_ok(i1, in, m1, 1)) return LICA_BAD_ARG;
+    if (!have_device()) return LICA_NO_DEVICE;
+    return lica_launch_ptdiag_lines(a, b, c, d, e, f, uu, o, q, r, m1, (int)i1, (int)in, (int)j1, (int)jn, stream)
+               ? LICA_HIP_ERROR : LICA_OK;
+}
+
 extern "C" int lica_poisson(int64_t nsys, int64_t n, const double *a, const double *b, const double *c,
                             const double *g, double *p, int64_t mode, double rtol, int64_t max_cycles,
                             int64_t *cycles_out)

@@ -1,5 +1,6 @@
 // Kernel entry points and launchers of the per-step path (gfx950).
 #include "lica_kern_flow.h"
+#include "lica_kern_lines.h"
 #include "lica_launch.h"
 #include <mutex>
 
@@ -236,6 +237,54 @@ int lica_launch_tri(int nsys, int n, double *a, double *b, double *c, double *g,
     return (int)hipGetLastError();
 }
 
+// the line solvers (lica_kern_lines.h): one thread per line i = i1..i2
+__global__ void __launch_bounds__(256) k_trdiag_lines(const double *a, const double *b, const double *c, double *r,
+                                                      double *uu, double *gam, long long m1, int l1, int l2, int i1,
+                                                      int i2, int overwrite_r)
+{
+    const int i = i1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i <= i2) line_trdiag(a, b, c, r, uu, gam, m1, i, l1, l2, overwrite_r);
+}
+__global__ void __launch_bounds__(256) k_trdiag_periodic_lines(const double *a, const double *b, const double *c,
+                                                               double *f, double *q, double *s, double *qe,
+                                                               long long m1, int j1, int j2, int k1, int k2)
+{
+    const int k = k1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k <= k2) line_trdiag_periodic(a, b, c, f, q, s, qe, m1, k, j1, j2);
+}
+__global__ void __launch_bounds__(256) k_ptdiag_lines(const double *a, const double *b, const double *c,
+                                                      const double *d, const double *e, const double *f, double *uu,
+                                                      double *o, double *q, double *r, long long m1, int i1, int i2,
+                                                      int j1, int jn)
+{
+    const int i = i1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i <= i2) line_ptdiag(a, b, c, d, e, f, uu, o, q, r, m1, i, j1, jn);
+}
+static dim3 line_grid(int i1, int i2) { return dim3((unsigned)((i2 - i1 + 1 + 255) / 256)); }
+int lica_launch_trdiag_lines(const double *a, const double *b, const double *c, double *r, double *uu, double *gam,
+                             long long m1, int l1, int l2, int i1, int i2, int overwrite_r, void *stream)
+{
+    hipLaunchKernelGGL(k_trdiag_lines, line_grid(i1, i2), dim3(256), 0, (hipStream_t)stream, a, b, c, r, uu, gam, m1,
+                       l1, l2, i1, i2, overwrite_r);
+    return (int)hipGetLastError();
+}
+int lica_launch_trdiag_periodic_lines(const double *a, const double *b, const double *c, double *f, double *q,
+                                      double *s, double *qe, long long m1, int j1, int j2, int k1, int k2,
+                                      void *stream)
+{
+    hipLaunchKernelGGL(k_trdiag_periodic_lines, line_grid(k1, k2), dim3(256), 0, (hipStream_t)stream, a, b, c, f, q,
+                       s, qe, m1, j1, j2, k1, k2);
+    return (int)hipGetLastError();
+}
+int lica_launch_ptdiag_lines(const double *a, const double *b, const double *c, const double *d, const double *e,
+                             const double *f, double *uu, double *o, double *q, double *r, long long m1, int i1,
+                             int i2, int j1, int jn, void *stream)
+{
+    hipLaunchKernelGGL(k_ptdiag_lines, line_grid(i1, i2), dim3(256), 0, (hipStream_t)stream, a, b, c, d, e, f, uu, o,
+                       q, r, m1, i1, i2, j1, jn);
+    return (int)hipGetLastError();
+}
+
 #define LAUNCH(K, threads) \
     hipLaunchKernelGGL(K, dim3(bd.ninst), dim3(threads), 0, (hipStream_t)stream, bd)
 #define LAUNCH_BLK(K, threads) \
@@ -324,6 +373,31 @@ int lica_launch_tri(int nsys, int n, double *a, double *b, double *c, double *g,
         tri_solve(a + off, b + off, c + off, g + off, 1, n, mode, sh->tri);
     }
     lica_emul_nthreads = saved;
+    return 0;
+}
+
+// the line solvers, one line after the other
+int lica_launch_trdiag_lines(const double *a, const double *b, const double *c, double *r, double *uu, double *gam,
+                             long long m1, int l1, int l2, int i1, int i2, int overwrite_r, void *stream)
+{
+    (void)stream;
+    for (int i = i1; i <= i2; i++) line_trdiag(a, b, c, r, uu, gam, m1, i, l1, l2, overwrite_r);
+    return 0;
+}
+int lica_launch_trdiag_periodic_lines(const double *a, const double *b, const double *c, double *f, double *q,
+                                      double *s, double *qe, long long m1, int j1, int j2, int k1, int k2,
+                                      void *stream)
+{
+    (void)stream;
+    for (int k = k1; k <= k2; k++) line_trdiag_periodic(a, b, c, f, q, s, qe, m1, k, j1, j2);
+    return 0;
+}
+int lica_launch_ptdiag_lines(const double *a, const double *b, const double *c, const double *d, const double *e,
+                             const double *f, double *uu, double *o, double *q, double *r, long long m1, int i1,
+                             int i2, int j1, int jn, void *stream)
+{
+    (void)stream;
+    for (int i = i1; i <= i2; i++) line_ptdiag(a, b, c, d, e, f, uu, o, q, r, m1, i, j1, jn);
     return 0;
 }
 

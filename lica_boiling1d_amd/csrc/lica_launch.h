@@ -22,6 +22,16 @@ int lica_launch_mg(int nsys, int n, const double *a, const double *b, const doub
                    double *scratch, int mode, double rtol, int maxc, int *cycles, double *resid, int threads,
                    void *stream);
 long long lica_mg_scratch_doubles(int n);
+// the line solvers of the 3-D code (lica_kern_lines.h) on device arrays,
+// column-major m1 x m2, 1-based ranges
+int lica_launch_trdiag_lines(const double *a, const double *b, const double *c, double *r, double *uu, double *gam,
+                             long long m1, int l1, int l2, int i1, int i2, int overwrite_r, void *stream);
+int lica_launch_trdiag_periodic_lines(const double *a, const double *b, const double *c, double *f, double *q,
+                                      double *s, double *qe, long long m1, int j1, int j2, int k1, int k2,
+                                      void *stream);
+int lica_launch_ptdiag_lines(const double *a, const double *b, const double *c, const double *d, const double *e,
+                             const double *f, double *uu, double *o, double *q, double *r, long long m1, int i1,
+                             int i2, int j1, int jn, void *stream);
 // the fixed workspace of lica_poisson_noalloc: this device's address of the
 // array (0, or -1 when HIP cannot give it)
 #define LICA_NOALLOC_SCRATCH_BYTES (64ull << 20)

@@ -363,6 +363,48 @@ class ReferenceSeams(Context):
         _check(self._lib.lica_f_temp_grad(self._h, self._a(psi_cn)), "lica_f_temp_grad")
 
 
+def _fcol(x):
+    """a column-major float64 copy: element (i, j) at (i-1) + (j-1)*m1, the
+    reference's array layout"""
+    return np.array(x, dtype=np.float64, order="F", copy=True)
+
+
+def trdiag(kind, a, b, c, r, l1, l2, ll1, ll2, uu=None, lib=None):
+    """TRDIAG1/2/3 (1d.f:3721-3812) on (m1, m2) arrays, 1-based ranges: the
+    tridiagonal systems along the second index for lines ll1..ll2.  Returns
+    (uu, r); r changes only for kind 3 (TRDIAG3 leaves UU in R(:, l1..l2-1))."""
+    lib = lib if lib is not None else _abi.load()
+    a, b, c, r = (_fcol(x) for x in (a, b, c, r))
+    uu = np.zeros_like(r, order="F") if uu is None else _fcol(uu)
+    m1, m2 = r.shape
+    fn = {1: lib.lica_trdiag1, 2: lib.lica_trdiag2, 3: lib.lica_trdiag3}[int(kind)]
+    _check(fn(_dptr(a), _dptr(b), _dptr(c), _dptr(r), _dptr(uu), m1, m2, l1, l2, ll1, ll2), "lica_trdiag")
+    return uu, r
+
+
+def trdiag_periodic(kind, a, b, c, f, j1, j2, l1, l2, lib=None):
+    """TRDIAG1P/2P/3P (1d.f:3815-3985): periodic systems along the second
+    index (rows j1..j2) for lines l1..l2; returns F after the solve."""
+    lib = lib if lib is not None else _abi.load()
+    a, b, c, f = (_fcol(x) for x in (a, b, c, f))
+    m1, m2 = f.shape
+    fn = {1: lib.lica_trdiag1p, 2: lib.lica_trdiag2p, 3: lib.lica_trdiag3p}[int(kind)]
+    _check(fn(_dptr(a), _dptr(b), _dptr(c), _dptr(f), m1, m2, j1, j2, l1, l2), "lica_trdiag_p")
+    return f
+
+
+def ptdiag1a(a, b, c, d, e, f, i1, i_n, j1, jn, uu=None, o=None, q=None, r=None, lib=None):
+    """PTDIAG1A (1d.f:3988-4034): pentadiagonal systems along the second
+    index (rows j1..jn) for lines i1..i_n; returns (uu, o, q, r)."""
+    lib = lib if lib is not None else _abi.load()
+    a, b, c, d, e, f = (_fcol(x) for x in (a, b, c, d, e, f))
+    uu, o, q, r = (np.zeros_like(f, order="F") if x is None else _fcol(x) for x in (uu, o, q, r))
+    m1, m2 = f.shape
+    _check(lib.lica_ptdiag1a(_dptr(a), _dptr(b), _dptr(c), _dptr(d), _dptr(e), _dptr(f), _dptr(uu), m1, m2, i1, i_n,
+                             j1, jn, _dptr(o), _dptr(q), _dptr(r)), "lica_ptdiag1a")
+    return uu, o, q, r
+
+
 def poisson(a, b, c, g, mode=_abi.POISSON_THOMAS, rtol=1.0e-10, max_cycles=100, lib=None):
     """Batched POISSON tridiagonal systems (1d.f:3535-3564); arrays (nsys, n)."""
     lib = lib if lib is not None else _abi.load()

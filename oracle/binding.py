@@ -95,6 +95,12 @@ def load():
         lib.orc_run_single.restype = ctypes.c_int
         lib.orc_init_fld.argtypes = [V, P_dbl, P_dbl, P_dbl, c_dbl, c_dbl, c_dbl, P_i64, P_i64, P_dbl]
         lib.orc_init_fld.restype = ctypes.c_int
+        lib.orc_trdiag_lines.argtypes = [P_dbl] * 5 + [c_i64] * 6 + [ctypes.c_int]
+        lib.orc_trdiag_lines.restype = None
+        lib.orc_trdiag_periodic_lines.argtypes = [P_dbl] * 4 + [c_i64] * 6
+        lib.orc_trdiag_periodic_lines.restype = None
+        lib.orc_ptdiag_lines.argtypes = [P_dbl] * 7 + [c_i64] * 6 + [P_dbl] * 3
+        lib.orc_ptdiag_lines.restype = None
         _lib = lib
     return _lib
 
@@ -326,6 +332,39 @@ def trdiag3(a, b, c, g):
     g = np.ascontiguousarray(g, dtype=np.float64).copy()
     load().orc_trdiag3(len(g), _p(a), _p(b), _p(c), _p(g))
     return g
+
+
+def _fa(x):
+    """a column-major float64 copy (the reference's array layout)"""
+    return np.array(x, dtype=np.float64, order="F", copy=True)
+
+
+def trdiag_lines(a, b, c, r, l1, l2, ll1, ll2, overwrite_r=False):
+    """TRDIAG1/2/3 (1d.f:3721-3812) on m1 x m2 arrays (1-based ranges):
+    returns (uu, r) -- r changes only with overwrite_r (TRDIAG3)."""
+    a, b, c, r = (_fa(x) for x in (a, b, c, r))
+    uu = np.zeros_like(r, order="F")
+    m1, m2 = r.shape
+    load().orc_trdiag_lines(_p(a), _p(b), _p(c), _p(r), _p(uu), m1, m2, l1, l2, ll1, ll2, int(overwrite_r))
+    return uu, r
+
+
+def trdiag_periodic_lines(a, b, c, f, j1, j2, l1, l2):
+    """TRDIAG1P/2P/3P (1d.f:3815-3985): returns F after the solve."""
+    a, b, c, f = (_fa(x) for x in (a, b, c, f))
+    m1, m2 = f.shape
+    load().orc_trdiag_periodic_lines(_p(a), _p(b), _p(c), _p(f), m1, m2, j1, j2, l1, l2)
+    return f
+
+
+def ptdiag_lines(a, b, c, d, e, f, i1, i_n, j1, jn):
+    """PTDIAG1A (1d.f:3988-4034): returns (uu, o, q, r)."""
+    a, b, c, d, e, f = (_fa(x) for x in (a, b, c, d, e, f))
+    m1, m2 = f.shape
+    uu, o, q, r = (np.zeros_like(f, order="F") for _ in range(4))
+    load().orc_ptdiag_lines(_p(a), _p(b), _p(c), _p(d), _p(e), _p(f), _p(uu), m1, m2, i1, i_n, j1, jn, _p(o), _p(q),
+                            _p(r))
+    return uu, o, q, r
 
 
 def grid_nodes(n, zl):

@@ -381,6 +381,117 @@ void orc_trdiag3(int64_t n, const double *a, const double *b, const double *c, d
     free(A.d); free(B.d); free(C.d); free(R.d); free(G.d);
 }
 
+/* ---- line solvers of the 3-D code (1d.f:3721-4034): serial restatements,
+ * loops in the reference's order (rows outer, lines inner) ---- */
+#define LX(i, j) ((size_t)((i) - 1) + (size_t)((j) - 1) * (size_t)m1)
+
+/* TRDIAG1/2/3 1d.f:3721-3812 */
+void orc_trdiag_lines(const double *a, const double *b, const double *c, double *r, double *uu, int64_t m1,
+                      int64_t m2, int64_t l1, int64_t l2, int64_t ll1, int64_t ll2, int overwrite_r)
+{
+    double *gam = calloc((size_t)m1 * (size_t)m2, sizeof(double)), *bet = calloc((size_t)m1, sizeof(double));
+    for (int64_t i = ll1; i <= ll2; i++) {
+        bet[i - 1] = 1.0 / b[LX(i, l1)];
+        uu[LX(i, l1)] = r[LX(i, l1)] * bet[i - 1];
+    }
+    for (int64_t j = l1 + 1; j <= l2; j++)
+        for (int64_t i = ll1; i <= ll2; i++) {
+            gam[LX(i, j)] = c[LX(i, j - 1)] * bet[i - 1];
+            bet[i - 1] = 1.0 / (b[LX(i, j)] - a[LX(i, j)] * gam[LX(i, j)]);
+            uu[LX(i, j)] = (r[LX(i, j)] - a[LX(i, j)] * uu[LX(i, j - 1)]) * bet[i - 1];
+        }
+    for (int64_t j = l2 - 1; j >= l1; j--)
+        for (int64_t i = ll1; i <= ll2; i++) {
+            uu[LX(i, j)] = uu[LX(i, j)] - gam[LX(i, j + 1)] * uu[LX(i, j + 1)];
+            if (overwrite_r) r[LX(i, j)] = uu[LX(i, j)];
+        }
+    free(gam);
+    free(bet);
+}
+
+/* TRDIAG1P/2P/3P 1d.f:3815-3985 (Q, S, QE, FN, PN local as in the reference) */
+void orc_trdiag_periodic_lines(const double *a, const double *b, const double *c, double *f, int64_t m1, int64_t m2,
+                               int64_t j1, int64_t j2, int64_t l1, int64_t l2)
+{
+    const size_t sz = (size_t)m1 * (size_t)m2;
+    double *q = calloc(sz, sizeof(double)), *s = calloc(sz, sizeof(double)), *qe = calloc(sz, sizeof(double));
+    double *fn = calloc((size_t)m1, sizeof(double)), *pn = calloc((size_t)m1, sizeof(double));
+    const int64_t ja = j1 + 1, jj = j1 + j2;
+    for (int64_t k = l1; k <= l2; k++) {
+        const double binv = 1.0 / b[LX(k, j1)];
+        q[LX(k, j1)] = -c[LX(k, j1)] * binv;
+        s[LX(k, j1)] = -a[LX(k, j1)] * binv;
+        fn[k - 1] = f[LX(k, j2)];
+        f[LX(k, j1)] = f[LX(k, j1)] * binv;
+    }
+    for (int64_t j = ja; j <= j2; j++)
+        for (int64_t k = l1; k <= l2; k++) {
+            pn[k - 1] = 1.0 / (b[LX(k, j)] + a[LX(k, j)] * q[LX(k, j - 1)]);
+            q[LX(k, j)] = -c[LX(k, j)] * pn[k - 1];
+            s[LX(k, j)] = -a[LX(k, j)] * s[LX(k, j - 1)] * pn[k - 1];
+            f[LX(k, j)] = (f[LX(k, j)] - a[LX(k, j)] * f[LX(k, j - 1)]) * pn[k - 1];
+        }
+    for (int64_t k = l1; k <= l2; k++) {
+        s[LX(k, j2)] = 1.0;
+        qe[LX(k, j2)] = 0.0;
+    }
+    for (int64_t i = ja; i <= j2; i++) {
+        const int64_t j = jj - i;
+        for (int64_t k = l1; k <= l2; k++) {
+            s[LX(k, j)] = s[LX(k, j)] + q[LX(k, j)] * s[LX(k, j + 1)];
+            qe[LX(k, j)] = f[LX(k, j)] + q[LX(k, j)] * qe[LX(k, j + 1)];
+        }
+    }
+    for (int64_t k = l1; k <= l2; k++)
+        f[LX(k, j2)] = (fn[k - 1] - c[LX(k, j2)] * qe[LX(k, j1)] - a[LX(k, j2)] * qe[LX(k, j2 - 1)]) /
+                       (c[LX(k, j2)] * s[LX(k, j1)] + a[LX(k, j2)] * s[LX(k, j2 - 1)] + b[LX(k, j2)]);
+    for (int64_t i = ja; i <= j2; i++) {
+        const int64_t j = jj - i;
+        for (int64_t k = l1; k <= l2; k++) f[LX(k, j)] = f[LX(k, j2)] * s[LX(k, j)] + qe[LX(k, j)];
+    }
+    free(q); free(s); free(qe); free(fn); free(pn);
+}
+
+/* PTDIAG1A 1d.f:3988-4034 */
+void orc_ptdiag_lines(const double *a, const double *b, const double *c, const double *d, const double *e,
+                      const double *f, double *uu, int64_t m1, int64_t m2, int64_t i1, int64_t in, int64_t j1,
+                      int64_t jn, double *o, double *q, double *r)
+{
+    (void)m2;
+    const int64_t j2 = j1 + 1, j3 = j2 + 1, jnm = jn - 1;
+    double *pd2 = calloc((size_t)m1, sizeof(double)), *pdj = calloc((size_t)m1, sizeof(double));
+    for (int64_t i = i1; i <= in; i++) {
+        o[LX(i, j1)] = -b[LX(i, j1)] / a[LX(i, j1)];
+        q[LX(i, j1)] = -c[LX(i, j1)] / a[LX(i, j1)];
+        r[LX(i, j1)] = f[LX(i, j1)] / a[LX(i, j1)];
+    }
+    for (int64_t i = i1; i <= in; i++) {
+        pd2[i - 1] = 1.0 / (a[LX(i, j2)] + d[LX(i, j2)] * o[LX(i, j1)]);
+        o[LX(i, j2)] = -(b[LX(i, j2)] + d[LX(i, j2)] * q[LX(i, j1)]) * pd2[i - 1];
+        q[LX(i, j2)] = -c[LX(i, j2)] * pd2[i - 1];
+        r[LX(i, j2)] = (f[LX(i, j2)] - d[LX(i, j2)] * r[LX(i, j1)]) * pd2[i - 1];
+    }
+    for (int64_t j = j3; j <= jn; j++) {
+        const int64_t jm = j - 1, jmm = jm - 1;
+        for (int64_t i = i1; i <= in; i++) {
+            pdj[i - 1] = 1.0 / (a[LX(i, j)] + e[LX(i, j)] * q[LX(i, jmm)] +
+                                (d[LX(i, j)] + e[LX(i, j)] * o[LX(i, jmm)]) * o[LX(i, jm)]);
+            o[LX(i, j)] = -(b[LX(i, j)] + (d[LX(i, j)] + e[LX(i, j)] * o[LX(i, jmm)]) * q[LX(i, jm)]) * pdj[i - 1];
+            q[LX(i, j)] = -c[LX(i, j)] * pdj[i - 1];
+            r[LX(i, j)] = (f[LX(i, j)] - e[LX(i, j)] * r[LX(i, jmm)] -
+                           (d[LX(i, j)] + e[LX(i, j)] * o[LX(i, jmm)]) * r[LX(i, jm)]) * pdj[i - 1];
+        }
+    }
+    for (int64_t i = i1; i <= in; i++) uu[LX(i, jn)] = r[LX(i, jn)];
+    for (int64_t i = i1; i <= in; i++) uu[LX(i, jnm)] = o[LX(i, jnm)] * uu[LX(i, jn)] + r[LX(i, jnm)];
+    for (int64_t j = jnm - 1; j >= j1; j--)
+        for (int64_t i = i1; i <= in; i++)
+            uu[LX(i, j)] = o[LX(i, j)] * uu[LX(i, j + 1)] + q[LX(i, j)] * uu[LX(i, j + 2)] + r[LX(i, j)];
+    free(pd2);
+    free(pdj);
+}
+#undef LX
+
 /* HJWENO two_phase.f:2477-2500 */
 double orc_hjweno(double a, double b, double c, double d)
 {

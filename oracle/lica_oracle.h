@@ -115,6 +115,24 @@ double orc_godunov(double asign, double p, double m);
 double orc_psif_cell(double phi, double d3);
 void orc_grid_nodes(int64_t n, double zl, double *z);
 
+/* The line solvers of the 3-D code (no live caller on the 1-D column).
+ * Arrays are column-major m1 x m2 (element (i, j), 1-based, at
+ * (i-1) + (j-1)*m1), as the reference's A(M2,M1) / A(M1,M2) dummies.
+ * TRDIAG1/2/3 (1d.f:3721-3812): systems along j = l1..l2 for lines
+ * i = ll1..ll2; overwrite_r (TRDIAG3) also stores UU in R(i, l1..l2-1). */
+void orc_trdiag_lines(const double *a, const double *b, const double *c, double *r, double *uu, int64_t m1,
+                      int64_t m2, int64_t l1, int64_t l2, int64_t ll1, int64_t ll2, int overwrite_r);
+/* TRDIAG1P/2P/3P (1d.f:3815-3985): periodic systems along j = j1..j2 for
+ * lines k = l1..l2, F in and out */
+void orc_trdiag_periodic_lines(const double *a, const double *b, const double *c, double *f, int64_t m1, int64_t m2,
+                               int64_t j1, int64_t j2, int64_t l1, int64_t l2);
+/* PTDIAG1A (1d.f:3988-4034): pentadiagonal systems along j = j1..jn
+ * (E u(j-2) + D u(j-1) + A u(j) + B u(j+1) + C u(j+2) = F) for lines
+ * i = i1..in; O, Q, R receive the elimination's coefficients */
+void orc_ptdiag_lines(const double *a, const double *b, const double *c, const double *d, const double *e,
+                      const double *f, double *uu, int64_t m1, int64_t m2, int64_t i1, int64_t in, int64_t j1,
+                      int64_t jn, double *o, double *q, double *r);
+
 #ifdef __cplusplus
 }
 #endif
