@@ -403,20 +403,49 @@ struct NetParams {
 // registers, then bias (+ ReLU) goes back into the same activation rows
 // (BF16; the halo and the rows beyond the window stay zero) or, for the
 // last layer, channel 0's FP32 pre-activation into ys.
+// The weights of the next tap group (the next layer's first group after a
+// layer's last, when next is given) are loaded into registers (pre) while
+// the current group's products run, and stored into ws after them.
 #define NET_TGRP 7
+#define NET_CASE_THREADS 256
 static_assert(NET_TAPS % NET_TGRP == 0, "tap groups");
+#define NET_PRE ((NET_TGRP * NET_C * NET_C / 8 + NET_CASE_THREADS - 1) / NET_CASE_THREADS)
+struct NetPre {
+    bf16x8 v[NET_PRE];
+};
 template <int COUT>
+__device__ __forceinline__ void net_group_load(NetPre &pre, const __bf16 *__restrict__ src)
+{
+    constexpr int UNITS = NET_TGRP * COUT * NET_C / 8;
+#pragma unroll
+    for (int j = 0; j < NET_PRE; j++) {
+        const int i = (int)threadIdx.x + j * NET_CASE_THREADS;
+        if (i < UNITS) pre.v[j] = *(const bf16x8 *)(src + i * 8);
+    }
+}
+template <int COUT>
+__device__ __forceinline__ void net_group_store(const NetPre &pre, __bf16 *ws)
+{
+    constexpr int UNITS = NET_TGRP * COUT * NET_C / 8;
+#pragma unroll
+    for (int j = 0; j < NET_PRE; j++) {
+        const int i = (int)threadIdx.x + j * NET_CASE_THREADS;
+        if (i < UNITS) *(bf16x8 *)(ws + i * 8) = pre.v[j];
+    }
+}
+template <int COUT, int NEXT_COUT>
 __device__ __forceinline__ void net_case_layer(__bf16 *xs, __bf16 *ws, float *ys, const __bf16 *__restrict__ wl,
-                                               const float *__restrict__ bl, int w, bool last)
+                                               const float *__restrict__ bl, int w, bool last, NetPre &pre,
+                                               const __bf16 *__restrict__ next)
 {
     const int tid = threadIdx.x;
     constexpr int NT = COUT / 16;
     f32x4 acc[4][NT];
     for (int t0 = 0; t0 < NET_TAPS; t0 += NET_TGRP) {
-        const __bf16 *src = wl + (size_t)t0 * COUT * NET_C;
-        for (int i = tid; i < NET_TGRP * COUT * NET_C / 8; i += blockDim.x)
-            *(bf16x8 *)(ws + i * 8) = *(const bf16x8 *)(src + i * 8);
+        net_group_store<COUT>(pre, ws);
         __syncthreads();
+        if (t0 + NET_TGRP < NET_TAPS) net_group_load<COUT>(pre, wl + (size_t)(t0 + NET_TGRP) * COUT * NET_C);
+        else if (next) net_group_load<NEXT_COUT>(pre, next);
         net_tile_mfma<COUT>(xs, ws, acc, t0, t0 + NET_TGRP);
         __syncthreads();   // ws (and, after the last group, xs) free
     }
@@ -451,7 +480,7 @@ __device__ __forceinline__ void net_case_layer(__bf16 *xs, __bf16 *ws, float *ys
 // weights, FP32 sums in the same order).  info[LICA_WARM_INFO i]: window
 // start (written), backward error, s_r, s_b (read), alpha (written), the
 // multigrid start verdict (cleared when the guess moves).
-__global__ void __launch_bounds__(256) k_net_case(const NetCase *__restrict__ cases, const int *__restrict__ order,
+__global__ void __launch_bounds__(NET_CASE_THREADS) __attribute__((amdgpu_waves_per_eu(4))) k_net_case(const NetCase *__restrict__ cases, const int *__restrict__ order,
                                                   NetParams prm, double *__restrict__ info, double rtol)
 {
     const int ci = order ? order[blockIdx.x] : (int)blockIdx.x;   // the case
@@ -467,6 +496,8 @@ __global__ void __launch_bounds__(256) k_net_case(const NetCase *__restrict__ ca
     if (tid == 0) { inf[0] = (double)k0; inf[4] = 0.0; }
     if (!(berr > rtol)) return;
     if (tid == 0) inf[6] = 0.0;   // the guess moves: the solve tests it afresh
+    NetPre pre;
+    net_group_load<NET_C>(pre, prm.w[0]);   // in flight while the features are built
     // layer 0 input (features, channels NET_NF.. zero), zero rows outside
     // the window
     for (int i = tid; i < ROWS; i += blockDim.x) {
@@ -480,8 +511,11 @@ __global__ void __launch_bounds__(256) k_net_case(const NetCase *__restrict__ ca
         for (int ch = ch0; ch < NET_C; ch++) row[ch] = to_bf16(0.0f);
     }
     __syncthreads();
-    for (int l = 0; l < NET_LAYERS - 1; l++) net_case_layer<NET_C>(xs, ws, ys, prm.w[l], prm.b[l], w, false);
-    net_case_layer<16>(xs, ws, ys, prm.w[NET_LAYERS - 1], prm.b[NET_LAYERS - 1], w, true);
+    static_assert(NET_LAYERS == 4, "the layer sequence below");
+    net_case_layer<NET_C, NET_C>(xs, ws, ys, prm.w[0], prm.b[0], w, false, pre, prm.w[1]);
+    net_case_layer<NET_C, NET_C>(xs, ws, ys, prm.w[1], prm.b[1], w, false, pre, prm.w[2]);
+    net_case_layer<NET_C, 16>(xs, ws, ys, prm.w[2], prm.b[2], w, false, pre, prm.w[3]);
+    net_case_layer<16, 16>(xs, ws, ys, prm.w[3], prm.b[3], w, true, pre, nullptr);
     auto d = [&](int k) -> double { return (k >= k0 && k < k0 + w) ? (double)ys[k - k0] : 0.0; };
     double v[2] = {0.0, 0.0};
     for (int k = max(k0 - 1, 1) + tid; k <= min(k0 + w, n); k += blockDim.x) {
@@ -920,7 +954,7 @@ int lica_net_guess(lica_net *t, const NetCase *dcases, const int *order, int nca
 {
     NetParams prm;
     for (int l = 0; l < NET_LAYERS; l++) { prm.w[l] = t->wb[l]; prm.b[l] = t->b[l]; }
-    hipLaunchKernelGGL(k_net_case, dim3(ncase), dim3(256), 0, (hipStream_t)stream, dcases, order, prm, dinfo, rtol);
+    hipLaunchKernelGGL(k_net_case, dim3(ncase), dim3(NET_CASE_THREADS), 0, (hipStream_t)stream, dcases, order, prm, dinfo, rtol);
     return hip_ok(hipGetLastError());
 }
 
