@@ -23,7 +23,12 @@ from a disjoint batch (seed 20261203); the training is not timed.
 Every run reports its stopped cases (the reference's STOP branches) and how
 they compare with the Thomas run's, case by case (status and step), and the
 interface position's largest relative difference from the Thomas run over
-every case alive there (a case alive there but stopped here counts apart).
+every case alive there (a case alive there but stopped here counts apart);
+over the cases alive in both, each field's (W, P, T, PHI, and the scalars
+QWALL and PLACE) relative difference from the Thomas run, case by case
+(max|f - f_Thomas| / max|f_Thomas|), its largest and median value.  A
+second Thomas run with every case's z0 one ulp up gives the same figures
+for rounding alone (the differences a one-ulp change of the input makes).
 
 The JSON line also carries
   roofline      the dominant kernel of the headline mode: its algorithmic
@@ -184,11 +189,16 @@ def lanes_of(cases, args):
     return args.lanes if args.lanes else (LANES_BATCH if len(cases) >= 256 else 1)
 
 
-def timed_run(L, cases, mode, args, net=None):
+# fields compared, case by case, with the Thomas run of the same batch
+FIELD_CMP = ("W", "P", "T", "PHI")
+
+
+def timed_run(L, cases, mode, args, net=None, keep_fields=False):
     """Warmup steps, then args.steps timed steps (host clock, synchronised
     on both sides); only the steps the cases completed are counted (a case
     that reaches one of the reference's STOP conditions keeps its status and
-    is left behind)."""
+    is left behind).  keep_fields: the FIELD_CMP fields of the running cases
+    after the timed steps (read after the clock stops)."""
     import numpy as np
     b = L.Batch(cases)
     if net is not None:
@@ -212,12 +222,16 @@ def timed_run(L, cases, mode, args, net=None):
     el = time.perf_counter() - t0
     st, steps1 = b.progress()
     s1 = np.array([b.scalar_array(i) for i in range(len(cases))])
+    ok = (st == 0)
+    fields = None
+    if keep_fields:
+        fields = {f: [b.field(f, i) if ok[i] else None for i in range(len(cases))] for f in FIELD_CMP}
     b.close()
     done = steps1 - steps0
-    ok = (st == 0)
     res = {"el": el, "done": int(done.sum()), "value": float(done.sum()) / el,
            "cells": float(sum(int(c.n) * int(d) for c, d in zip(cases, done))) / el,
-           "ok": ok, "place": s1[:, 3], "nbad": nbad, "status": st, "steps": steps1, "tuning": tuning}
+           "ok": ok, "place": s1[:, L.SCALARS.index("PLACE")], "qwall": s1[:, L.SCALARS.index("QWALL")],
+           "nbad": nbad, "status": st, "steps": steps1, "tuning": tuning, "fields": fields}
     if mode >= L.POISSON_MG64 and ok.any():
         ci, si, ri = (L.SCALARS.index(k) for k in ("POISSON_CYCLES", "POISSON_SOLVES", "POISSON_RESID"))
         res["vcycles_per_solve"] = float((s1[ok, ci] - s0[ok, ci]).sum() / max((s1[ok, si] - s0[ok, si]).sum(), 1.0))
@@ -299,10 +313,30 @@ def versus_thomas(res, ref):
     both = alive_ref & (res["status"] == 0)
     rel = np.abs(res["place"][both] - ref["place"][both]) / np.abs(ref["place"][both])
     diff = np.where(~same)[0]
-    return {"cases_with_other_stop_than_thomas": int(diff.size),
-            "those_cases": [int(i) for i in diff[:20]],
-            "alive_in_thomas_but_stopped_here": int((alive_ref & (res["status"] != 0)).sum()),
-            "interface_rel_diff_max_over_cases_alive_in_both": float(rel.max()) if rel.size else None}
+    out = {"cases_with_other_stop_than_thomas": int(diff.size),
+           "those_cases": [int(i) for i in diff[:20]],
+           "alive_in_thomas_but_stopped_here": int((alive_ref & (res["status"] != 0)).sum()),
+           "interface_rel_diff_max_over_cases_alive_in_both": float(rel.max()) if rel.size else None}
+    if res.get("fields") is not None and ref.get("fields") is not None and both.any():
+        # per case max|f - f_Thomas| / max|f_Thomas| over the field, then the
+        # largest and the median over the cases alive in both runs
+        errs = {}
+        for f in FIELD_CMP:
+            e = []
+            for i in np.where(both)[0]:
+                a, r = res["fields"][f][i], ref["fields"][f][i]
+                den = float(np.max(np.abs(r))) if r.size else 0.0
+                d = float(np.max(np.abs(a - r))) if r.size else 0.0
+                e.append(d / den if den > 0.0 else d)
+            errs[f] = e
+        qr = np.abs(ref["qwall"][both])
+        errs["QWALL"] = list(np.abs(res["qwall"][both] - ref["qwall"][both]) / np.where(qr > 0.0, qr, 1.0))
+        errs["PLACE"] = list(rel)
+        out["field_rel_diff_vs_thomas"] = {
+            "cases": int(both.sum()),
+            "max": {k: float(np.max(v)) for k, v in errs.items()},
+            "median": {k: float(np.median(v)) for k, v in errs.items()}}
+    return out
 
 
 def train_net(L, args):
@@ -544,7 +578,7 @@ def main():
         say(f"trained: {train['pairs']} windows, loss {train['final_loss']:.3e}, "
             f"{train['collect_s']:.1f} s + {train['train_s']:.1f} s")
     say(f"timed run, mode {args.mode}")
-    res = timed_run(L, cases, mode, args, net)
+    res = timed_run(L, cases, mode, args, net, keep_fields=not args.no_fp64)
     if res["done"] == 0:
         print("bench: no case advanced", file=sys.stderr)
         sys.exit(1)
@@ -572,13 +606,21 @@ def main():
         runs = {}
         if mode == L.POISSON_LEARNED:
             say("timed run, mixed+projected (no network)")
-            runs["mixed_projected"] = timed_run(L, cases, L.POISSON_PROJECTED, args)
+            runs["mixed_projected"] = timed_run(L, cases, L.POISSON_PROJECTED, args, keep_fields=True)
         if mode != L.POISSON_MG64:
             say("timed run, plain FP64 V-cycles from zero")
-            runs["plain_fp64"] = timed_run(L, cases, L.POISSON_MG64, args)
+            runs["plain_fp64"] = timed_run(L, cases, L.POISSON_MG64, args, keep_fields=True)
+        ulp = None
         if mode != L.POISSON_THOMAS:
             say("timed run, Thomas (the reference's direct solve)")
-            runs["thomas"] = timed_run(L, cases, L.POISSON_THOMAS, args)
+            runs["thomas"] = timed_run(L, cases, L.POISSON_THOMAS, args, keep_fields=True)
+            # the same with every case's film thickness z0 one ulp up: how far
+            # rounding alone moves each field (the yardstick of the differences
+            # of the multigrid runs from the Thomas run)
+            say("timed run, Thomas with every case's z0 one ulp up")
+            import numpy as np
+            ulp_cases = [c.copy(z0=float(np.nextafter(c.z0, 1.0))) for c in cases]
+            ulp = timed_run(L, ulp_cases, L.POISSON_THOMAS, args, keep_fields=True)
         ref = runs.get("thomas", res)
         for k, r in runs.items():
             out[k] = run_summary(L, r, args)
@@ -595,6 +637,8 @@ def main():
         for k, r in runs.items():
             if k != "thomas":
                 out["versus_thomas"][k] = versus_thomas(r, ref)
+        if ulp is not None:
+            out["versus_thomas"]["thomas_z0_one_ulp_up"] = versus_thomas(ulp, ref)
     if not args.no_cpu:
         say("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(cases, wl["workload"], args.cpu_steps)

@@ -98,3 +98,38 @@ def check_reference_main(seams, orc, steps, ref_array):
     if not out["untouched"]:
         bad.append("x/y arrays written")
     return bad + exact_mismatches(seams, 0, orc)
+
+
+def thomas_longdouble(a, b, c, g):
+    """Thomas elimination in x87 extended precision (64-bit significand)."""
+    a, b, c, g = (np.asarray(v, dtype=np.longdouble) for v in (a, b, c, g))
+    n = len(g)
+    cp, gp = np.zeros(n, np.longdouble), np.zeros(n, np.longdouble)
+    cp[0], gp[0] = c[0] / b[0], g[0] / b[0]
+    for k in range(1, n):
+        m = b[k] - a[k] * cp[k - 1]
+        cp[k] = c[k] / m if k < n - 1 else 0.0
+        gp[k] = (g[k] - a[k] * gp[k - 1]) / m
+    x = np.zeros(n, np.longdouble)
+    x[-1] = gp[-1]
+    for k in range(n - 2, -1, -1):
+        x[k] = gp[k] - cp[k] * x[k + 1]
+    return x.astype(np.float64)
+
+
+def nearly_singular_poisson(seed=5, n=3000, interface=900, ratio=1605.36):
+    """POISSON rows (1d.f:3535-3541) as the batch's hardest systems have
+    them: b = -(a + c) in floating point (interior rows sum to their
+    rounding error), 1/rho jumping by the liquid/vapour density ratio at the
+    interface row, Neumann at the wall, and the last row's c (the implicit
+    p = 0 above the top face) kept in its b only."""
+    rng = np.random.default_rng(seed)
+    rho = np.where(np.arange(n + 1) < interface, 1.0, ratio)
+    coef = (1.0 / rho) * rng.uniform(0.99, 1.01, n + 1)
+    a = coef[:-1].copy()
+    c = coef[1:].copy()
+    a[0] = 0.0
+    b = -(a + c)
+    g = rng.standard_normal(n) * 1e3
+    return a, b, c, g
+

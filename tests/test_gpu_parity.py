@@ -13,7 +13,7 @@ import pytest
 
 import lica_boiling1d_amd as L
 from oracle import Oracle
-from parity_util import exact_mismatches, max_rel, ulp_sensitivity
+from parity_util import exact_mismatches, max_rel, nearly_singular_poisson, thomas_longdouble, ulp_sensitivity
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -345,6 +345,25 @@ def test_batched_poisson_multigrid(mode):
         anorm = np.max(np.abs(a[s]) + np.abs(b[s]) + np.abs(c[s]))
         assert np.max(np.abs(r)) <= 1e-10 * (anorm * np.max(np.abs(p[s])) + np.max(np.abs(g[s])))
         assert np.max(np.abs(p[s] - x[s])) <= 1e-12 * np.max(np.abs(x[s]))
+
+
+@pytest.mark.parametrize("mode,tol", [(L.POISSON_MG64, 1e-14), (L.POISSON_MIXED, 1e-11)])
+def test_poisson_multigrid_nearly_singular_rows_gpu(mode, tol):
+    """The batch's hardest POISSON rows (b = -(a + c) rounded, a 1605x jump
+    of 1/rho, the Dirichlet row on top): the V-cycle solution within tol of
+    an extended-precision elimination, on the GPU.  The reference's own
+    elimination (the Thomas mode, TRDIAG3's order in FP64) is off by about
+    1e-8 on the same rows, its pivots losing the row sums; that error, not
+    the multigrid's, is what separates the multigrid runs' P from the Thomas
+    run's on such cases."""
+    _say(f"nearly singular POISSON rows n=3000, mode {mode}")
+    a, b, c, g = nearly_singular_poisson()
+    x = thomas_longdouble(a, b, c, g)
+    scale = np.max(np.abs(x))
+    p = L.poisson(a, b, c, g, mode=mode, rtol=1e-10, max_cycles=100)[0]
+    assert np.max(np.abs(p - x)) <= tol * scale
+    t = L.poisson(a, b, c, g, mode=L.POISSON_THOMAS)[0]
+    assert np.max(np.abs(t - x)) >= 1e-10 * scale
 
 
 def test_reference_seams_bit_exact():

@@ -10,7 +10,7 @@ import emul_build
 import lica_boiling1d_amd as L
 import lica_boiling1d_amd._abi as abi
 from oracle import Oracle
-from parity_util import exact_mismatches, max_rel
+from parity_util import exact_mismatches, max_rel, nearly_singular_poisson, thomas_longdouble
 
 
 @pytest.fixture(scope="module")
@@ -179,23 +179,6 @@ def test_batched_poisson_multigrid_cpu(emul, mode, n):
             assert np.max(np.abs(p[s] - x)) <= 1e-12 * np.max(np.abs(x))
 
 
-def _thomas_longdouble(a, b, c, g):
-    """Thomas elimination in x87 extended precision (64-bit significand)."""
-    a, b, c, g = (np.asarray(v, dtype=np.longdouble) for v in (a, b, c, g))
-    n = len(g)
-    cp, gp = np.zeros(n, np.longdouble), np.zeros(n, np.longdouble)
-    cp[0], gp[0] = c[0] / b[0], g[0] / b[0]
-    for k in range(1, n):
-        m = b[k] - a[k] * cp[k - 1]
-        cp[k] = c[k] / m if k < n - 1 else 0.0
-        gp[k] = (g[k] - a[k] * gp[k - 1]) / m
-    x = np.zeros(n, np.longdouble)
-    x[-1] = gp[-1]
-    for k in range(n - 2, -1, -1):
-        x[k] = gp[k] - cp[k] * x[k + 1]
-    return x.astype(np.float64)
-
-
 @pytest.mark.parametrize("mode,tol", [(L.POISSON_MG64, 1e-14), (L.POISSON_MIXED, 1e-11)])
 def test_poisson_multigrid_nearly_singular_rows(emul, mode, tol):
     """The POISSON sign pattern with rows that sum to their rounding error
@@ -205,17 +188,9 @@ def test_poisson_multigrid_nearly_singular_rows(emul, mode, tol):
     elimination.  (A normwise backward error of 1e-10 alone would allow an
     error of order 1e-2 here, and TRDIAG3 in FP64 is off by about 2e-8 on
     these rows because its pivots lose the row sums.)"""
-    rng = np.random.default_rng(5)
-    n = 3000
-    rho = np.where(np.arange(n + 1) < 900, 1.0, 1605.36)
-    coef = (1.0 / rho) * rng.uniform(0.99, 1.01, n + 1)
-    a = coef[:-1].copy()
-    c = coef[1:].copy()
-    a[0] = 0.0
-    b = -(a + c)
-    g = rng.standard_normal(n) * 1e3
+    a, b, c, g = nearly_singular_poisson()
     p = L.poisson(a, b, c, g, mode=mode, rtol=1e-10, max_cycles=100, lib=emul)[0]
-    x = _thomas_longdouble(a, b, c, g)
+    x = thomas_longdouble(a, b, c, g)
     assert np.max(np.abs(p - x)) <= tol * np.max(np.abs(x))
 
 
