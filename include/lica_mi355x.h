@@ -215,8 +215,10 @@ int lica_batch_set_lane_split(lica_batch *b, int64_t split);
  * (the multigrid or direct POISSON solve, the momentum and energy stages,
  * UCALC, DETERMINE_DT, in that order) and for each lane, tries 64..1024
  * threads per workgroup, then multigrid LDS sizes per lane, each trial
- * `steps` (1..16) steps of every lane at once as in a run, and keeps a
- * change that gains more than 1 %.  With the binned split it then moves
+ * `steps` (1..16) steps of every lane at once as in a run; a candidate
+ * within 0.5 % of the best or faster is timed again over 4 x `steps` steps
+ * beside the kept shape and taken when more than 0.5 % faster there.
+ * With the binned split it then moves
  * the split's cuts so that the lanes' measured times (each from its first
  * launch to its last, trials of 4 x `steps` steps) come out equal, keeping
  * a new split unless the batch gets more than 0.5 % slower.  The batch's

@@ -1275,12 +1275,26 @@ int lica_batch_autotune(lica_batch *b, int64_t steps)
     };
     // Coordinate descent on the whole batch's step time: one kernel class
     // (the longest first) of one lane at a time, each candidate timed with
-    // every lane running as in a run; a change is kept only when it gains
-    // more than 1 % (below that the difference is within the timing's own
-    // spread).  The first trial runs the default shapes once untimed.
+    // every lane running as in a run.  Trials of `steps` steps spread by 1-2 %
+    // from one to the next, so a candidate within 0.5 % of the best or faster
+    // is confirmed by two trials of four times the steps, the candidate's and
+    // then the kept shape's; it is taken when its long trial is more than
+    // 0.5 % faster.  The first trial runs the default shapes once untimed.
     double best = 0.0;
     rc = trial(&best);
     rc = rc ? rc : trial(&best);
+    const int64_t long_steps = 4 * steps;
+    auto confirm = [&](double t_short, auto &&set_cand, auto &&set_keep, bool *take) -> int {
+        *take = false;
+        if (!(t_short < best * 1.005)) return 0;
+        double tc = 0.0, tk = 0.0;
+        int e = trial(&tc, nullptr, long_steps);
+        set_keep();
+        e = e ? e : trial(&tk, nullptr, long_steps);
+        set_cand();
+        *take = !e && tc < tk * 0.995;
+        return e;
+    };
     const int classes[5] = {KL_POISSON_MG, KL_MOMENTUM, KL_CONT_ENERGY, KL_UCALC, KL_DT};
     const int cand[5] = {64, 128, 256, 512, 1024};
     for (int ci = 0; ci < 5 && !rc; ci++) {
@@ -1300,8 +1314,10 @@ int lica_batch_autotune(lica_batch *b, int64_t steps)
                 b->cur_lane = -1;
                 if (raised) continue;
                 double t = 0.0;
+                bool take = false;
                 rc = trial(&t);
-                if (!rc && t < best * 0.99) { best = t; keep = cand[j]; }
+                rc = rc ? rc : confirm(t, [&] { cfg.thr[k] = cand[j]; }, [&] { cfg.thr[k] = keep; }, &take);
+                if (take) { best = t; keep = cand[j]; }
             }
             cfg.thr[k] = keep;
         }
@@ -1324,8 +1340,10 @@ int lica_batch_autotune(lica_batch *b, int64_t steps)
                 if (c == cur || (j > 0 && c == need)) continue;
                 cfg.mg_lds = c;
                 double t = 0.0;
+                bool take = false;
                 rc = trial(&t);
-                if (!rc && t < best * 0.99) { best = t; keep = c; }
+                rc = rc ? rc : confirm(t, [&] { cfg.mg_lds = c; }, [&] { cfg.mg_lds = keep; }, &take);
+                if (take) { best = t; keep = c; }
             }
             cfg.mg_lds = keep;
         }
