@@ -40,6 +40,9 @@ The JSON line also carries
                 launch that rocprofv3's FETCH_SIZE/WRITE_SIZE counters gave
                 for that kernel on this workload, read from the committed
                 profile summary (null when there is none);
+  step_roofline the whole step: the algorithmic bytes of every kernel of
+                every step completed in the timed window (the same per-kernel
+                rule) over the window's time, against 8 TB/s;
   cpu_baseline  the FP64 oracle (the serial restatement, built -O3 -fopenmp)
                 on the GPU box's host cores: one case per OpenMP thread over
                 a bounded sample of the same batch, set-up excluded.
@@ -125,7 +128,31 @@ def kernel_bytes(case, kernel, mode):
         return band + 200 * 8 * 4 + 100 * 8 * 6 + 12 * r
     if kernel == "temp_grad":
         return band + 200 * 8 * 6 + 100 * 8 * 8
+    if kernel == "warm_start":
+        # the projection: a, c, g and the history's 4 solutions + P read in
+        # each of its 2 passes, the guess written (the network's window of
+        # 256 cells adds a few KB per case, not counted)
+        return (2 * 8 * (3 + 5) + 8) * n if mode in (4, 5) else 0
     return 0
+
+
+STEP_KERNELS = ("cont_energy", "temp_grad", "momentum", "warm_start", "poisson", "ucalc", "transport")
+
+
+def step_bytes(case, mode):
+    """Algorithmic HBM bytes of one full step of one case: DETERMINE_DT once,
+    the substep's kernels three times."""
+    return kernel_bytes(case, "dt", mode) + 3 * sum(kernel_bytes(case, k, mode) for k in STEP_KERNELS)
+
+
+def step_roofline(cases, mode, res):
+    """The whole step against the HBM roofline: the algorithmic bytes of
+    every step the cases completed in the timed window, over the window's
+    time (all lanes at once)."""
+    tot = sum(step_bytes(c, mode) * int(d) for c, d in zip(cases, res["done_per_case"]))
+    ach = tot / res["el"] / 1e9
+    return {"bound": "hbm", "algorithmic_bytes": tot, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS, "timing": "the timed steps, all lanes"}
 
 
 def host_info():
@@ -228,7 +255,7 @@ def timed_run(L, cases, mode, args, net=None, keep_fields=False):
         fields = {f: [b.field(f, i) if ok[i] else None for i in range(len(cases))] for f in FIELD_CMP}
     b.close()
     done = steps1 - steps0
-    res = {"el": el, "done": int(done.sum()), "value": float(done.sum()) / el,
+    res = {"el": el, "done": int(done.sum()), "done_per_case": done, "value": float(done.sum()) / el,
            "cells": float(sum(int(c.n) * int(d) for c, d in zip(cases, done))) / el,
            "ok": ok, "place": s1[:, L.SCALARS.index("PLACE")], "qwall": s1[:, L.SCALARS.index("QWALL")],
            "nbad": nbad, "status": st, "steps": steps1, "tuning": tuning, "fields": fields}
@@ -596,6 +623,7 @@ def main():
     if res["nbad"]:
         out["stopped_cases"] = stopped(L, res)
     out["launch_shapes"] = res["tuning"]
+    out["step_roofline"] = step_roofline(cases, mode, res)
     say("one-lane kernel profile")
     per, alive = kernel_profile(L, cases, mode, args, net)
     out["roofline"] = roofline(L, cases, mode, per, alive, wkey)

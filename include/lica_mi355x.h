@@ -217,7 +217,8 @@ int lica_batch_set_lane_split(lica_batch *b, int64_t split);
  * threads per workgroup, then multigrid LDS sizes per lane, each trial
  * `steps` (1..16) steps of every lane at once as in a run; a candidate
  * within 0.5 % of the best or faster is timed again over 4 x `steps` steps
- * beside the kept shape and taken when more than 0.5 % faster there.
+ * in turn with the kept shape, twice, and taken when more than 0.5 %
+ * faster there.
  * With the binned split it then moves
  * the split's cuts so that the lanes' measured times (each from its first
  * launch to its last, trials of 4 x `steps` steps) come out equal, keeping

@@ -1277,9 +1277,10 @@ int lica_batch_autotune(lica_batch *b, int64_t steps)
     // (the longest first) of one lane at a time, each candidate timed with
     // every lane running as in a run.  Trials of `steps` steps spread by 1-2 %
     // from one to the next, so a candidate within 0.5 % of the best or faster
-    // is confirmed by two trials of four times the steps, the candidate's and
-    // then the kept shape's; it is taken when its long trial is more than
-    // 0.5 % faster.  The first trial runs the default shapes once untimed.
+    // is confirmed by trials of four times the steps, the candidate's and the
+    // kept shape's in turn, twice (a drift of the clocks falls on both); it
+    // is taken when its two long trials together are more than 0.5 % faster.
+    // The first trial runs the default shapes once untimed.
     double best = 0.0;
     rc = trial(&best);
     rc = rc ? rc : trial(&best);
@@ -1288,9 +1289,16 @@ int lica_batch_autotune(lica_batch *b, int64_t steps)
         *take = false;
         if (!(t_short < best * 1.005)) return 0;
         double tc = 0.0, tk = 0.0;
-        int e = trial(&tc, nullptr, long_steps);
-        set_keep();
-        e = e ? e : trial(&tk, nullptr, long_steps);
+        int e = 0;
+        for (int r = 0; r < 2 && !e; r++) {
+            double t = 0.0;
+            set_cand();
+            e = trial(&t, nullptr, long_steps);
+            tc += t;
+            set_keep();
+            e = e ? e : trial(&t, nullptr, long_steps);
+            tk += t;
+        }
         set_cand();
         *take = !e && tc < tk * 0.995;
         return e;

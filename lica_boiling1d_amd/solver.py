@@ -173,8 +173,8 @@ class Batch(_Handle):
         shape is free and every lane, 64..1024 threads per workgroup, then
         multigrid LDS sizes, each trial `steps` steps of all lanes at once;
         a candidate within 0.5 % of the best or faster is confirmed by trials
-        of 4 x `steps` steps beside the kept shape and taken when more than
-        0.5 % faster there.  The batch returns to
+        of 4 x `steps` steps in turn with the kept shape, twice, and taken
+        when more than 0.5 % faster there.  The batch returns to
         the state it had; no result depends on the choice.  Returns
         tuning()."""
         _check(self._lib.lica_batch_autotune(self._h, int(steps)), "autotune")
