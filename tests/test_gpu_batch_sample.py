@@ -5,7 +5,10 @@ benchmark runs them (5 warmup + 20 timed).
 Sample (indices into batched_cases(4096), seed 20261202): the four largest
 cases (N = 4090..4094); cases that reach one of the reference's STOP branches
 in the oracle at steps 1 to 8 (CAL_PSIF_MTP's error branch,
-two_phase.f:2905, and non-finite fields); and ordinary cases of N = 222..3893.
+two_phase.f:2905, and non-finite fields); the two cases whose P the
+multigrid runs of the whole batch move farthest from the Thomas run's
+(2974 and 888: POISSON systems of condition number 4-5e10, where TRDIAG3's
+own error is 1.5e-8 to 5e-8); and ordinary cases of N = 222..3893.
 
 Bars:
 * Thomas mode: every case stops at the same step with the same status as the
@@ -32,7 +35,7 @@ from parity_util import exact_mismatches, max_rel
 pytestmark = pytest.mark.gpu
 
 SAMPLE = [2135, 680, 2967, 2176, 3398, 299, 2, 57, 8, 48, 80, 169, 732, 95, 144, 190, 103, 3363, 1948, 1480, 1535,
-          2039, 2276, 31, 1419, 3163, 1918, 782, 113, 3523, 3610, 2852]
+          2039, 2276, 31, 1419, 3163, 1918, 782, 113, 3523, 3610, 2852, 2974, 888]
 STEPS = 25
 FIELDS = ("W", "P", "T", "PHI")
 
