@@ -1,5 +1,5 @@
 ! MAIN's time loop (1d.f:196-295) on the reference's own arrays through the
-! lica_mi355x module (the binding INTEGRATION.md shows).  Arguments: lica.in,
+! lica_mi355x module (include/lica_mi355x.f90).  Arguments: lica.in,
 ! grid.in, N3 (the deck's NS node count), steps, output file.  Writes the
 ! columns (1,1,0:N3) of W, P, PSI_ZN, PSI_CN and T, then DVMAX, CFLM, DT,
 ! VOL_TOT_ORI, TIME, with 17 significant digits.

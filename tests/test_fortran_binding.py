@@ -1,10 +1,11 @@
-"""The Fortran binding INTEGRATION.md shows, compiled and run: its
-ISO_C_BINDING module (taken from INTEGRATION.md as it stands) and a driver
-that runs MAIN's loop (1d.f:196-295) on the reference's own arrays
-(tests/fortran/main_steps.f90), built with amdflang and linked to the CPU
-logic-check build of the kernel source (tests/_emul, the same C ABI as the
-GPU library).  Three steps of the shipped deck give the FP64 oracle's
-fields and scalars bit for bit.  CPU only."""
+"""The Fortran binding compiled and run: its ISO_C_BINDING module
+(include/lica_mi355x.f90) and a driver that runs MAIN's loop (1d.f:196-295)
+on the reference's own arrays (tests/fortran/main_steps.f90), built with
+amdflang and linked to the CPU logic-check build of the kernel source
+(tests/_emul, the same C ABI as the GPU library).  Three steps of the
+shipped deck give the FP64 oracle's fields and scalars bit for bit.  The
+module block the repository's markdown pages show is the file itself.  CPU
+only."""
 import os
 import re
 import shutil
@@ -26,12 +27,25 @@ def _flang():
     return None
 
 
+MODULE = os.path.join(ROOT, "include", "lica_mi355x.f90")
+
+
 def module_source():
-    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
-    blocks = re.findall(r"```fortran\n(.*?)```", text, flags=re.S)
-    mods = [b for b in blocks if b.lstrip().lower().startswith("module lica_mi355x")]
-    assert len(mods) == 1, "INTEGRATION.md must hold exactly one lica_mi355x module block"
-    return mods[0]
+    with open(MODULE) as f:
+        return f.read()
+
+
+def test_shown_module_is_the_shipped_file():
+    """Every lica_mi355x module block in the top-level markdown pages equals
+    include/lica_mi355x.f90, and at least one page shows it."""
+    shown = []
+    for name in sorted(os.listdir(ROOT)):
+        if name.endswith(".md"):
+            with open(os.path.join(ROOT, name)) as f:
+                blocks = re.findall(r"```fortran\n(.*?)```", f.read(), flags=re.S)
+            shown += [b for b in blocks if b.lstrip().lower().startswith("module lica_mi355x")]
+    assert shown
+    assert all(b == module_source() for b in shown)
 
 
 def test_fortran_driver_matches_oracle(tmp_path):
