@@ -14,6 +14,8 @@ window of steps after the same untimed warmup:
                    started from the history projection corrected by the
                    warm-start network (the value);
   mixed+projected  the same without the network (what the network adds);
+  mixed from zero  FP32 V-cycle corrections from a zero guess (what the
+                   history projection adds);
   plain FP64       FP64 V-cycles from a zero initial guess (IOLDV=0);
   thomas           the reference's own direct solve (TRDIAG3 order, bit for
                    bit the FP64 oracle): the yardstick for which cases stop.
@@ -635,6 +637,9 @@ def main():
         if mode == L.POISSON_LEARNED:
             say("timed run, mixed+projected (no network)")
             runs["mixed_projected"] = timed_run(L, cases, L.POISSON_PROJECTED, args, keep_fields=True)
+        if mode in (L.POISSON_LEARNED, L.POISSON_PROJECTED):
+            say("timed run, mixed from zero (no warm start)")
+            runs["mixed_from_zero"] = timed_run(L, cases, L.POISSON_MIXED, args, keep_fields=True)
         if mode != L.POISSON_MG64:
             say("timed run, plain FP64 V-cycles from zero")
             runs["plain_fp64"] = timed_run(L, cases, L.POISSON_MG64, args, keep_fields=True)
@@ -656,6 +661,12 @@ def main():
             pr = runs["mixed_projected"]
             out["network_gain_vs_projected"] = res["value"] / pr["value"] if pr["value"] > 0 else None
             out["vcycles_saved_by_network_per_solve"] = pr.get("vcycles_per_solve", 0.0) - res.get(
+                "vcycles_per_solve", 0.0)
+        if "mixed_from_zero" in runs:
+            mz = runs["mixed_from_zero"]
+            pr = runs.get("mixed_projected", res)
+            out["projection_gain_vs_mixed_from_zero"] = pr["value"] / mz["value"] if mz["value"] > 0 else None
+            out["vcycles_saved_by_projection_per_solve"] = mz.get("vcycles_per_solve", 0.0) - pr.get(
                 "vcycles_per_solve", 0.0)
         if "plain_fp64" in runs:
             pf = runs["plain_fp64"]
