@@ -44,6 +44,17 @@
  *   D9  the GLOBAL_MASS_CORRECTION call after the subcycles
  *       (two_phase.f:979-984) shifts ALPHI_EXT after ALPHI was stored and
  *       ALPHI_EXT is then deallocated, so it has no effect and is skipped.
+ *
+ * Parity unpinned (no closed form, invariant or golden vector of the
+ * reference checks these parts; they are a careful reading of the reference
+ * and the GPU path matches them bit for bit): TEMP_EXTP's halve-and-restart
+ * (chphase.f:653-678, 746-768; its fixed point and flux are pinned), the
+ * TRANSPORT control flow (subcycles, reinitialisation triggers, MOD(M,50),
+ * stale MFLVS slots; two_phase.f:792-984), the CONTI_Z/CONTI_C region logic
+ * and donor boxes (1d.f:2114-2169, 2258-2305), CAL_CONTINUITY2
+ * (1d.f:3006-3169), ALPHI_BC's overwrite by the periodic copy
+ * (two_phase.f:565-634), and the RHSZ/LHSW and CAL_RHSPS2 assembly
+ * (1d.f:1603-1809, 4175-4376; the projection after them is pinned).
  */
 #include "lica_oracle.h"
 #include <math.h>
